@@ -1,0 +1,261 @@
+"""AdaptiveMesh for MI355X: Python bindings of the gfx950 C ABI.
+
+The product is ``libadaptivemesh_amd.so`` (HIP kernels for gfx950 behind
+``include/adaptivemesh.h`` plus the native host runtime behind
+``include/adaptivemesh_host.h``).  This module loads it with ctypes and
+mirrors its structs; it raises if the library is missing -- there is no
+CPU fallback.  PyTorch is used only to own device memory: kernel entry
+points take ``tensor.data_ptr()`` and the current HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libadaptivemesh_amd.so"
+
+AM_MAX_VARS = 32
+AM_INTERP_LAGRANGE, AM_INTERP_TRILINEAR, AM_INTERP_TRICUBIC = 0, 1, 2
+
+ERRORS = {
+    -1: "AM_ERR_INVALID", -2: "AM_ERR_UNSUPPORTED_GEOMETRY", -3: "AM_ERR_SLOTS_EXHAUSTED",
+    -4: "AM_ERR_DEPTH_LIMIT", -5: "AM_ERR_OCCUPIED", -6: "AM_ERR_MISALIGNED",
+    -7: "AM_ERR_TOO_MANY_VARS", -8: "AM_ERR_ALLOC", -9: "AM_ERR_LAUNCH",
+}
+
+
+class AMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Geometry(C.Structure):
+    """am_geometry: S, P, FD order, outer_size, outer_dx, outer_dt."""
+
+    _fields_ = [("size_domain", C.c_int32), ("padding_domain", C.c_int32), ("fd_order", C.c_int32),
+                ("outer_size", C.c_float), ("outer_dx", C.c_float), ("outer_dt", C.c_float)]
+
+    @property
+    def S(self):
+        return self.size_domain
+
+    @property
+    def P(self):
+        return self.padding_domain
+
+    @property
+    def T(self):
+        return self.size_domain + 2 * self.padding_domain
+
+
+class Mesh(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("children", C.c_void_p), ("boundaries", C.c_void_p),
+                ("tiles", C.c_void_p), ("n_tiles", C.c_int32), ("substep", C.c_uint32),
+                ("max_depth", C.c_uint32)]
+
+
+class FdBatch(C.Structure):
+    _fields_ = [("field", C.c_void_p * AM_MAX_VARS), ("d1", C.c_void_p * AM_MAX_VARS),
+                ("d2diag", C.c_void_p * AM_MAX_VARS), ("d2off", C.c_void_p * AM_MAX_VARS),
+                ("seed_d", C.c_int32 * AM_MAX_VARS), ("seed_m", C.c_int32 * AM_MAX_VARS),
+                ("n_vars", C.c_int32), ("z_block", C.c_int32)]
+
+
+class FieldBatch(C.Structure):
+    _fields_ = [("old_", C.c_void_p * AM_MAX_VARS), ("new_", C.c_void_p * AM_MAX_VARS),
+                ("older", C.c_void_p * AM_MAX_VARS), ("n_vars", C.c_int32)]
+
+
+_lib = None
+
+_P = C.c_void_p
+_SIGS = {
+    "am_last_error_string": (C.c_char_p, []),
+    "am_version": (C.c_int, []),
+    "am_geometry_init": (C.c_int, [C.POINTER(Geometry), C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float]),
+    "am_geometry_supported": (C.c_int, [C.POINTER(Geometry)]),
+    "am_device_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "am_device_free": (C.c_int, [_P]),
+    "am_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "am_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "am_memset_async": (C.c_int, [_P, C.c_int, C.c_size_t, _P]),
+    "am_stream_sync": (C.c_int, [_P]),
+    "am_device_sync": (C.c_int, []),
+    "am_fd_d1": (C.c_int, [_P, _P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_int32, _P]),
+    "am_fd_d1_d2diag": (C.c_int, [_P, _P, _P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_int32, _P]),
+    "am_fd_mixed": (C.c_int, [_P, _P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_int32, _P]),
+    "am_fd_all": (C.c_int, [_P, _P, _P, _P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_int32, C.c_int32, _P]),
+    "am_fd_batched": (C.c_int, [C.POINTER(FdBatch), C.POINTER(Mesh), C.POINTER(Geometry), _P]),
+    "am_ghost_fill": (C.c_int, [_P, _P, _P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_int32, C.c_int32, _P]),
+    "am_ghost_fill_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), C.c_int32,
+                                        C.c_int32, _P]),
+    "am_copy_new_to_old": (C.c_int, [_P, _P, C.POINTER(Mesh), C.POINTER(Geometry), _P]),
+    "am_copy_new_to_old_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), _P]),
+    "am_copy_to_child": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int32, C.POINTER(Geometry), C.c_int32, _P]),
+    "am_wave_predictor": (C.c_int, [_P, _P, _P, _P, C.POINTER(Mesh), C.POINTER(Geometry), _P]),
+    "am_wave_corrector": (C.c_int, [_P, _P, _P, _P, C.POINTER(Mesh), C.POINTER(Geometry), _P]),
+    "am_deriv_trace": (C.c_int, [_P, _P, C.POINTER(C.c_void_p), _P, C.POINTER(Mesh), C.POINTER(Geometry), _P]),
+    "am_christoffel": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32,
+                                 C.POINTER(Mesh), C.POINTER(Geometry), C.c_int32, _P]),
+    # host runtime
+    "am_tree_create": (_P, [C.c_uint32, C.c_int32]),
+    "am_tree_destroy": (None, [_P]),
+    "am_tree_add_node": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "am_tree_remove_node": (C.c_int, [_P, C.c_int32]),
+    "am_tree_increment_timer": (C.c_int, [_P]),
+    "am_tree_find_node_idx": (C.c_uint32, [_P, C.c_uint32, C.c_uint32]),
+    "am_tree_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "am_tree_mesh": (C.c_int, [_P, C.c_int32, C.POINTER(Mesh)]),
+    "am_wave_sim_create": (_P, [C.POINTER(Geometry), C.c_uint32, C.c_int32]),
+    "am_wave_sim_destroy": (None, [_P]),
+    "am_wave_sim_add_node": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "am_wave_sim_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "am_wave_sim_set_field": (C.c_int, [_P, C.c_int32, _P]),
+    "am_wave_sim_get_field": (C.c_int, [_P, C.c_int32, _P]),
+    "am_wave_sim_copy_bounds": (C.c_int, [_P]),
+    "am_wave_sim_timestep": (C.c_int, [_P]),
+    "am_wave_sim_rng_state": (C.c_int32, [_P]),
+}
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def lib() -> C.CDLL:
+    """Load the gfx950 library (built by adaptivemesh_amd.build).  Raises if absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m adaptivemesh_amd.build` "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> int:
+    if rc != 0:
+        raise AMError(rc, f"{what}: {lib().am_last_error_string().decode()}")
+    return rc
+
+
+def geometry(S: int = 16, P: int = 3, fd_order: int = 0, outer_size: float = 100.0, cfl: float = 0.4) -> Geometry:
+    g = Geometry()
+    check(lib().am_geometry_init(C.byref(g), S, P, fd_order, outer_size, cfl), "am_geometry_init")
+    return g
+
+
+def ptr(x) -> int | None:
+    """Device address of a torch tensor (or None)."""
+    if x is None:
+        return None
+    return x.data_ptr()
+
+
+def stream_handle():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Tree:
+    """The native CPU-managed octree (AMR.h:118-294) through the C ABI."""
+
+    def __init__(self, max_slots: int, periodic: bool = False):
+        self.max_slots = max_slots
+        self.h = lib().am_tree_create(max_slots, 1 if periodic else 0)
+        if not self.h:
+            raise AMError(-8, lib().am_last_error_string().decode())
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.am_tree_destroy(h)
+            self.h = None
+
+    def add_node(self, parent: int, offset: int) -> int:
+        return lib().am_tree_add_node(self.h, parent, offset)
+
+    def remove_node(self, node: int) -> int:
+        return lib().am_tree_remove_node(self.h, node)
+
+    def increment_timer(self):
+        check(lib().am_tree_increment_timer(self.h), "increment_timer")
+
+    def find_node_idx(self, packed: int, max_depth: int = 0xFFFFFFFF) -> int:
+        return lib().am_tree_find_node_idx(self.h, packed, max_depth)
+
+    def export(self) -> dict:
+        n = self.max_slots
+        out = dict(parent=np.empty(n, np.int32), children=np.empty(8 * n, np.int32),
+                   positions=np.empty(n, np.uint32), boundaries=np.empty(6 * n, np.uint64),
+                   lifetime=np.empty(n, np.uint32), scalars=np.empty(4, np.int32))
+        check(lib().am_tree_export(self.h, *(a.ctypes.data for a in out.values())), "am_tree_export")
+        s = out["scalars"]
+        out.update(max_depth=int(s[0]), first_free=int(s[1]), final_used=int(s[2]), timer=int(s[3]))
+        return out
+
+    def mesh(self, depth: int = -1) -> Mesh:
+        m = Mesh()
+        check(lib().am_tree_mesh(self.h, depth, C.byref(m)), "am_tree_mesh")
+        return m
+
+
+class WaveSim:
+    """AMR<wave_AMR_data>: the reference SimpleWave driver (SimpleWave.h:191-279)."""
+
+    def __init__(self, geom: Geometry, max_slots: int, periodic: bool = False):
+        self.geom = geom
+        self.max_slots = max_slots
+        self.h = lib().am_wave_sim_create(C.byref(geom), max_slots, 1 if periodic else 0)
+        if not self.h:
+            raise AMError(-8, lib().am_last_error_string().decode())
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.am_wave_sim_destroy(h)
+            self.h = None
+
+    def add_node(self, parent: int, offset: int) -> int:
+        r = lib().am_wave_sim_add_node(self.h, parent, offset)
+        if r < 0:
+            raise AMError(r, lib().am_last_error_string().decode())
+        return r
+
+    def export(self) -> dict:
+        n = self.max_slots
+        out = dict(parent=np.empty(n, np.int32), children=np.empty(8 * n, np.int32),
+                   positions=np.empty(n, np.uint32), boundaries=np.empty(6 * n, np.uint64),
+                   lifetime=np.empty(n, np.uint32), scalars=np.empty(4, np.int32))
+        check(lib().am_wave_sim_export(self.h, *(a.ctypes.data for a in out.values())), "am_wave_sim_export")
+        return out
+
+    def set_field(self, which: int, arr: np.ndarray):
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        assert a.size == self.max_slots * self.geom.T ** 3
+        check(lib().am_wave_sim_set_field(self.h, which, a.ctypes.data), "set_field")
+
+    def get_field(self, which: int) -> np.ndarray:
+        a = np.empty(self.max_slots * self.geom.T ** 3, np.float32)
+        check(lib().am_wave_sim_get_field(self.h, which, a.ctypes.data), "get_field")
+        return a
+
+    def copy_bounds(self):
+        check(lib().am_wave_sim_copy_bounds(self.h), "copy_bounds")
+
+    def timestep(self):
+        check(lib().am_wave_sim_timestep(self.h), "timestep")
+
+    @property
+    def rng_state(self) -> int:
+        return lib().am_wave_sim_rng_state(self.h)

@@ -1,0 +1,110 @@
+// Device-side building blocks shared by every AdaptiveMesh kernel for gfx950:
+// the fast_prng step and per-thread seeding, the compressed_float3 codec, and
+// the octree / boundary decoders.  Every routine is written so that, compiled
+// with -ffp-contract=off and IEEE division, it produces the same bits as the
+// reference's routine on an IEEE host:
+//   fast_prng                  tensors.h:276-304
+//   compressed_float3          tensors.h:305-394
+//   octree_abs_pos::depth      AMR.h:27-29
+//   domain_boundary accessors  AMR.h:87-99
+//   active_depth               AMR.h:640-644
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "am_common.hpp"
+
+namespace am {
+
+// ---------------------------------------------------------------- PRNG
+__device__ __forceinline__ int32_t prng_next(int32_t &s)
+{
+    int32_t v = (int32_t)((uint32_t)s + 1124781675u);
+    v ^= v >> 7;
+    v = (int32_t)((uint32_t)v * 1246477263u);
+    s = v;
+    return v;
+}
+__device__ __forceinline__ float prng_float(int32_t &s)
+{
+    return __fmul_rn((float)prng_next(s), 2.3283064E-10f);
+}
+// seed = gs + x*2178614; seed ^= next()*y; seed ^= next()*z  (AMR.h:1160-1162)
+__device__ __forceinline__ int32_t thread_seed(int32_t gs, uint32_t x, uint32_t y, uint32_t z)
+{
+    int32_t s = (int32_t)((uint32_t)gs + x * 2178614u);
+    uint32_t t = (uint32_t)prng_next(s) * y;
+    s = (int32_t)((uint32_t)s ^ t);
+    t = (uint32_t)prng_next(s) * z;
+    s = (int32_t)((uint32_t)s ^ t);
+    return s;
+}
+
+// ------------------------------------------------- compressed_float3 codec
+__device__ __forceinline__ float cf3_clamp(float f) { return fmaxf(-255.f, fminf(f, 255.f)); }
+
+__device__ __forceinline__ uint32_t cf3_exponent(float x, float y, float z)
+{
+    const uint32_t ex = (__float_as_uint(x) >> 23) & 255u;
+    const uint32_t ey = (__float_as_uint(y) >> 23) & 255u;
+    const uint32_t ez = (__float_as_uint(z) >> 23) & 255u;
+    const uint32_t m = max(max(ex, max(ey, ez)), 112u) - 112u;
+    return min(m, 31u);
+}
+__device__ __forceinline__ uint32_t cf3_assemble(uint32_t e, float x, float y, float z)
+{
+    x = cf3_clamp(x); y = cf3_clamp(y); z = cf3_clamp(z);
+    uint32_t w = e << 27;
+    w |= (uint32_t)__fadd_rn(fabsf(x), .5f) | ((uint32_t)(x < 0.f) << 8);
+    w |= ((uint32_t)__fadd_rn(fabsf(y), .5f) << 9) | ((uint32_t)(y < 0.f) << 17);
+    w |= ((uint32_t)__fadd_rn(fabsf(z), .5f) << 18) | ((uint32_t)(z < 0.f) << 26);
+    return w;
+}
+// The scale 2^(22-e) is exact, so the product below is the reference's
+// result.x *= ((1<<15)/(1/128)) / float(1<<e) (tensors.h:361-363).
+__device__ __forceinline__ float cf3_scale(uint32_t e) { return __uint_as_float((127u + 22u - e) << 23); }
+
+// dithered pack (tensors.h:350-373): three draws, x then y then z
+__device__ __forceinline__ uint32_t cf3_pack(float x, float y, float z, int32_t &rng)
+{
+    const uint32_t e = cf3_exponent(x, y, z);
+    const float sc = cf3_scale(e);
+    x = __fmul_rn(x, sc); y = __fmul_rn(y, sc); z = __fmul_rn(z, sc);
+    x = __fadd_rn(x, __fmul_rn(prng_float(rng), 0.95f));
+    y = __fadd_rn(y, __fmul_rn(prng_float(rng), 0.95f));
+    z = __fadd_rn(z, __fmul_rn(prng_float(rng), 0.95f));
+    return cf3_assemble(e, x, y, z);
+}
+// undithered pack (tensors.h:374-393)
+__device__ __forceinline__ uint32_t cf3_pack_plain(float x, float y, float z)
+{
+    const uint32_t e = cf3_exponent(x, y, z);
+    const float sc = cf3_scale(e);
+    return cf3_assemble(e, __fmul_rn(x, sc), __fmul_rn(y, sc), __fmul_rn(z, sc));
+}
+// unpack (tensors.h:335-348): exact, v = +-m * 2^(e-22)
+__device__ __forceinline__ float3 cf3_unpack(uint32_t w)
+{
+    const float base = __uint_as_float((127u - 15u + (w >> 27)) << 23);
+    const float k = 1.f / 128.f;
+    float3 r;
+    r.x = base * (((float)(w & 255u) * (1.f - 2.f * (float)((w >> 8) & 1u))) * k);
+    r.y = base * (((float)((w >> 9) & 255u) * (1.f - 2.f * (float)((w >> 17) & 1u))) * k);
+    r.z = base * (((float)((w >> 18) & 255u) * (1.f - 2.f * (float)((w >> 26) & 1u))) * k);
+    return r;
+}
+
+// ------------------------------------------------------------ octree
+__device__ __forceinline__ int pos_depth(uint32_t d) { return (~d) ? (int)(d >> 27) : -1; }
+__device__ __forceinline__ bool active_depth(uint32_t sub, uint32_t depth, uint32_t maxd)
+{
+    return (sub & ((1u << (maxd - depth)) - 1u)) == 0u;
+}
+__device__ __forceinline__ int bnd_target(uint32_t nd)
+{
+    return (int)((nd & 16777215u) | ((nd & (1u << 23)) ? 4278190080u : 0u));
+}
+__device__ __forceinline__ uint32_t bnd_ddiff(uint32_t nd) { return (nd >> 24) & 15u; }
+__device__ __forceinline__ int bnd_depth(uint32_t nd) { return ((nd >> 28) == 15u) ? -1 : (int)(nd >> 28); }
+
+}  // namespace am

@@ -1,0 +1,244 @@
+// Finite-difference stencils for gfx950: gradient, diagonal Hessian and the
+// boundary-careful mixed partials of padded fp32 tiles, written as dithered
+// compressed_float3 words.
+//
+// One workgroup = one (tile, variable, z-block).  Its S*S threads own the
+// (x, y) columns of the tile and march up z.  The padded tile is streamed
+// through LDS one full T*T plane at a time (a plane is contiguous in HBM, so
+// each plane is a single coalesced 16-byte-vector read), into a ring of 2P+2
+// planes: the 2P+1 planes the stencil needs plus one being filled, which lets
+// every step use a single barrier.  Outputs for one plane are S*S contiguous
+// words per array.
+//
+// Arithmetic follows the reference kernels term by term (AMR.h:1152-1266)
+// with contraction disabled, so each word is bit-identical to the
+// reference's on an IEEE host.
+#include "am_device.hpp"
+
+#pragma clang fp contract(off)
+
+namespace am {
+
+struct FdBatchDev {
+    const float *field[AM_MAX_VARS];
+    uint32_t *d1[AM_MAX_VARS];
+    uint32_t *d2diag[AM_MAX_VARS];
+    uint32_t *d2off[AM_MAX_VARS];
+    int32_t seed_d[AM_MAX_VARS];
+    int32_t seed_m[AM_MAX_VARS];
+};
+
+struct MeshDev {
+    const uint32_t *positions;
+    const int32_t *tiles;
+    uint32_t substep;
+    uint32_t max_depth;
+};
+
+// __mixed_partial_c / _bd (AMR.h:1221-1230) on LDS rows: c = centre, the
+// other arguments are the neighbours named by their offsets.
+__device__ __forceinline__ float mixed_c(float pp, float mm, float pm, float mp)
+{
+    return .25f * (pp + mm - pm - mp);
+}
+__device__ __forceinline__ float mixed_bd(float c, float pp, float mm, float m0, float p0, float zm, float zp)
+{
+    return c + .5f * (pp + mm - m0 - p0 - zm - zp);
+}
+
+template <int S, int P, int Q, int ZB>
+__global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScales sc)
+{
+    constexpr int T = S + 2 * P, T2 = T * T, R = 2 * P + 2, NZB = S / ZB;
+    constexpr FdCoef C = fd_coef();
+    constexpr int BASE = Q * Q + Q - 1;
+    __shared__ float4 ring4[R][T2 / 4];
+
+    const int tile_i = blockIdx.x / NZB, zb = blockIdx.x - tile_i * NZB;
+    const int var = blockIdx.y;
+    const int slot = m.tiles[tile_i];
+    const int depth = pos_depth(m.positions[slot]);
+    if (depth == -1 || !active_depth(m.substep, (uint32_t)depth, m.max_depth)) return;
+
+    const float *src = b.field[var] + (size_t)slot * (T * T2);
+    uint32_t *o_d1 = b.d1[var], *o_d2 = b.d2diag[var], *o_off = b.d2off[var];
+    const bool want_d = o_d1 != nullptr, want_m = o_off != nullptr;
+    const int32_t gs_d = b.seed_d[var], gs_m = b.seed_m[var];
+    const float dx1 = sc.inv_dx[depth];
+    const float dx2 = dx1 * dx1;
+
+    const int tid = threadIdx.x;
+    const int x = tid % S, y = tid / S;
+    const int z0 = zb * ZB;
+    const int c = (y + P) * T + (x + P);  // centre offset within a plane
+
+    auto load_plane = [&](int zp) {  // padded plane index zp -> ring slot zp % R
+        const float4 *s4 = reinterpret_cast<const float4 *>(src + (size_t)zp * T2);
+        float4 *d4 = ring4[zp % R];
+        for (int i = tid; i < T2 / 4; i += S * S) d4[i] = s4[i];
+    };
+    // planes z0 .. z0+2P-1 (padded indices) precede the first output plane
+    for (int p = 0; p < 2 * P; p++) load_plane(z0 + p);
+
+    const int ex = x - min(max(x, 1), S - 2);
+    const int ey = y - min(max(y, 1), S - 2);
+    const int cxy = ex * ey;
+
+    for (int z = z0; z < z0 + ZB; z++) {
+        load_plane(z + 2 * P);
+        __syncthreads();
+        // padded plane of output z is z+P; plane z+P+i sits in slot (z+P+i) % R
+        const float *pl[2 * P + 1];
+#pragma unroll
+        for (int i = 0; i <= 2 * P; i++) pl[i] = reinterpret_cast<const float *>(ring4[(z + i) % R]);
+        const float *ctr = pl[P];
+        const uint32_t gz = (uint32_t)(slot * S + z);
+        const size_t o = ((size_t)slot * S + z) * (S * S) + (size_t)tid;
+
+        if (want_d) {
+            float vx = 0.f, vy = 0.f, vz = 0.f, xx = 0.f, yy = 0.f, zz = 0.f;
+#pragma unroll
+            for (int i = -Q; i <= Q; i++) {
+                float a = ctr[c + i];
+                vx = vx + a * C.d1[BASE + i];
+                xx = xx + a * C.d2[BASE + i];
+                a = ctr[c + i * T];
+                vy = vy + a * C.d1[BASE + i];
+                yy = yy + a * C.d2[BASE + i];
+                a = pl[P + i][c];
+                vz = vz + a * C.d1[BASE + i];
+                zz = zz + a * C.d2[BASE + i];
+            }
+            vx = vx * dx1; vy = vy * dx1; vz = vz * dx1;
+            int32_t rng = thread_seed(gs_d, (uint32_t)x, (uint32_t)y, gz);
+            if (o_d2) o_d2[o] = cf3_pack(xx * dx2, yy * dx2, zz * dx2, rng);
+            o_d1[o] = cf3_pack(vx, vy, vz, rng);
+        }
+        if (want_m) {
+            const int ez = z - min(max(z, 1), S - 2);
+            const int cxz = ex * ez, cyz = ey * ez;
+            const float *zm = pl[P - 1], *zp = pl[P + 1];
+            float vxy, vxz, vyz;
+            if (cxy == 0) {
+                vxy = mixed_c(ctr[c + 1 + T], ctr[c - 1 - T], ctr[c + 1 - T], ctr[c - 1 + T]);
+            } else {
+                const int dx = -cxy;  // +-1 along x, dy = +T
+                vxy = (float)(-cxy) * mixed_bd(ctr[c], ctr[c + dx + T], ctr[c - dx - T], ctr[c - dx], ctr[c + dx],
+                                               ctr[c - T], ctr[c + T]);
+            }
+            if (cxz == 0) {
+                vxz = mixed_c(zp[c + 1], zm[c - 1], zm[c + 1], zp[c - 1]);
+            } else {
+                const int dx = -cxz;  // +-1 along x, dy = one plane up
+                vxz = (float)(-cxz) * mixed_bd(ctr[c], zp[c + dx], zm[c - dx], ctr[c - dx], ctr[c + dx], zm[c], zp[c]);
+            }
+            if (cyz == 0) {
+                vyz = mixed_c(zp[c + T], zm[c - T], zm[c + T], zp[c - T]);
+            } else {
+                const int dy = -cyz * T;  // +-1 along y, dz = one plane up
+                vyz = (float)(-cyz) * mixed_bd(ctr[c], zp[c + dy], zm[c - dy], ctr[c - dy], ctr[c + dy], zm[c], zp[c]);
+            }
+            int32_t rng = thread_seed(gs_m, (uint32_t)x, (uint32_t)y, gz);
+            o_off[o] = cf3_pack(vxy * dx2, vxz * dx2, vyz * dx2, rng);
+        }
+    }
+}
+
+template <int S, int P, int Q>
+static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream)
+{
+    FdBatchDev d;
+    for (int v = 0; v < b->n_vars; v++) {
+        d.field[v] = b->field[v];
+        d.d1[v] = b->d1[v];
+        d.d2diag[v] = b->d2diag[v];
+        d.d2off[v] = b->d2off[v];
+        d.seed_d[v] = b->seed_d[v];
+        d.seed_m[v] = b->seed_m[v];
+        if (!b->field[v]) return set_error(AM_ERR_INVALID, "am_fd: null field for variable %d", v);
+        if (!aligned16(b->field[v])) return set_error(AM_ERR_MISALIGNED, "am_fd: field %d not 16-byte aligned", v);
+        if (!b->d1[v] && !b->d2off[v]) return set_error(AM_ERR_INVALID, "am_fd: variable %d has no output", v);
+        if (!b->d1[v] && b->d2diag[v])
+            return set_error(AM_ERR_INVALID, "am_fd: variable %d: diagonal Hessian needs the gradient word", v);
+    }
+    MeshDev m{mesh->positions, mesh->tiles, mesh->substep, mesh->max_depth};
+    const DepthScales sc = depth_scales(*g);
+    int zb = b->z_block;
+    if (zb == 0) zb = S;  // whole tile per workgroup: no plane is read twice
+    hipStream_t st = (hipStream_t)stream;
+    if (zb == S) {
+        dim3 grid(mesh->n_tiles * 1, b->n_vars);
+        hipLaunchKernelGGL((k_fd<S, P, Q, S>), grid, dim3(S * S), 0, st, d, m, sc);
+    } else if (zb == S / 2) {
+        dim3 grid(mesh->n_tiles * 2, b->n_vars);
+        hipLaunchKernelGGL((k_fd<S, P, Q, S / 2>), grid, dim3(S * S), 0, st, d, m, sc);
+    } else if (zb == S / 4 && (S % 4) == 0) {
+        dim3 grid(mesh->n_tiles * 4, b->n_vars);
+        hipLaunchKernelGGL((k_fd<S, P, Q, S / 4>), grid, dim3(S * S), 0, st, d, m, sc);
+    } else {
+        return set_error(AM_ERR_INVALID, "am_fd: z_block %d must be S, S/2 or S/4", zb);
+    }
+    return check_launch("k_fd");
+}
+
+static int fd_checks(const am_mesh *mesh, const am_geometry *g)
+{
+    if (!mesh || !g) return set_error(AM_ERR_INVALID, "am_fd: null mesh or geometry");
+    if (!mesh->positions || !mesh->tiles) return set_error(AM_ERR_INVALID, "am_fd: null positions or tiles");
+    if (mesh->n_tiles < 0) return set_error(AM_ERR_INVALID, "am_fd: negative tile count");
+    return AM_OK;
+}
+
+}  // namespace am
+
+extern "C" int am_fd_batched(const am_fd_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream)
+{
+    if (!b) return am::set_error(AM_ERR_INVALID, "am_fd_batched: null batch");
+    if (b->n_vars < 0 || b->n_vars > AM_MAX_VARS)
+        return am::set_error(AM_ERR_TOO_MANY_VARS, "am_fd_batched: n_vars %d", b->n_vars);
+    if (int rc = am::fd_checks(mesh, g)) return rc;
+    if (b->n_vars == 0 || mesh->n_tiles == 0) return AM_OK;
+    AM_DISPATCH_GEOMETRY(*g, am::launch_fd, b, mesh, g, stream);
+}
+
+static int fd_single(const float *field, uint32_t *d1, uint32_t *d2diag, uint32_t *d2off, const am_mesh *mesh,
+                     const am_geometry *g, int32_t seed_d, int32_t seed_m, void *stream)
+{
+    am_fd_batch b = {};
+    b.field[0] = field;
+    b.d1[0] = d1;
+    b.d2diag[0] = d2diag;
+    b.d2off[0] = d2off;
+    b.seed_d[0] = seed_d;
+    b.seed_m[0] = seed_m;
+    b.n_vars = 1;
+    return am_fd_batched(&b, mesh, g, stream);
+}
+
+extern "C" int am_fd_d1(const float *field, uint32_t *d1, const am_mesh *mesh, const am_geometry *g, int32_t seed,
+                        void *stream)
+{
+    if (!d1) return am::set_error(AM_ERR_INVALID, "am_fd_d1: null output");
+    return fd_single(field, d1, nullptr, nullptr, mesh, g, seed, 0, stream);
+}
+
+extern "C" int am_fd_d1_d2diag(const float *field, uint32_t *d1, uint32_t *d2diag, const am_mesh *mesh,
+                               const am_geometry *g, int32_t seed, void *stream)
+{
+    if (!d1 || !d2diag) return am::set_error(AM_ERR_INVALID, "am_fd_d1_d2diag: null output");
+    return fd_single(field, d1, d2diag, nullptr, mesh, g, seed, 0, stream);
+}
+
+extern "C" int am_fd_mixed(const float *field, uint32_t *d2off, const am_mesh *mesh, const am_geometry *g,
+                           int32_t seed, void *stream)
+{
+    if (!d2off) return am::set_error(AM_ERR_INVALID, "am_fd_mixed: null output");
+    return fd_single(field, nullptr, nullptr, d2off, mesh, g, 0, seed, stream);
+}
+
+extern "C" int am_fd_all(const float *field, uint32_t *d1, uint32_t *d2diag, uint32_t *d2off, const am_mesh *mesh,
+                         const am_geometry *g, int32_t seed_d, int32_t seed_m, void *stream)
+{
+    if (!d1 || !d2diag || !d2off) return am::set_error(AM_ERR_INVALID, "am_fd_all: null output");
+    return fd_single(field, d1, d2diag, d2off, mesh, g, seed_d, seed_m, stream);
+}

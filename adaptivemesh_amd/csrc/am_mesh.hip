@@ -1,0 +1,374 @@
+// Tile-to-tile kernels for gfx950: ghost-cell fill with space-time
+// interpolation, restriction on copy-back, and prolongation into a new child.
+//
+// Ghost fill: one workgroup per (tile, face, variable); its S*S threads cover
+// the face and loop over the P ghost layers.  The source cells are a small
+// neighbourhood of the boundary target tile, re-read by adjacent ghost cells,
+// so the gathers hit L1/L2; the writes are the face slab of this tile.
+// Restriction: one workgroup per (tile, z-plane, variable) streaming the
+// contiguous T*T plane.
+//
+// Index maps and arithmetic follow AMR.h:673-762 (quadratic), 767-848
+// (trilinear), 961-996 and 1053-1073 term by term with contraction off, so
+// results are bit-identical to the reference's on an IEEE host.  The
+// tricubic-space / quadratic-time mode is an extension.
+#include "am_device.hpp"
+
+#pragma clang fp contract(off)
+
+namespace am {
+
+struct FieldBatchDev {
+    float *old_[AM_MAX_VARS];
+    float *new_[AM_MAX_VARS];
+    const float *older[AM_MAX_VARS];
+};
+
+// __interpolate_dat, quadratic 10-point form (AMR.h:673-699)
+template <int T>
+__device__ __forceinline__ float interp_lagrange(const float *__restrict__ d, float rx, float ry, float rz, size_t n)
+{
+    constexpr int T2 = T * T;
+    float f = d[n], fx = d[n + 1], fy = d[n + T], fz = d[n + T2];
+    float fxx = d[n - 1], fyy = d[n - T], fzz = d[n - T2];
+    float fxy = d[n + T + 1], fxz = d[n + T2 + 1], fyz = d[n + T2 + T];
+    fxy = fxy + ((f - fx) - fy);
+    fxz = fxz + ((f - fx) - fz);
+    fyz = fyz + ((f - fy) - fz);
+    fx = (fx - fxx) * .5f;
+    fy = (fy - fyy) * .5f;
+    fz = (fz - fzz) * .5f;
+    fxx = (fx + fxx) - f;
+    fyy = (fy + fyy) - f;
+    fzz = (fz + fzz) - f;
+    return f + (fx + fxx * rx) * rx + (fy + fyy * ry) * ry + (fz + fzz * rz) * rz + fxy * rx * ry + fxz * rx * rz +
+           fyz * ry * rz;
+}
+// __interpolate_dat, trilinear form (AMR.h:767-785)
+template <int T>
+__device__ __forceinline__ float interp_trilinear(const float *__restrict__ d, float rx, float ry, float rz, size_t n)
+{
+    constexpr int T2 = T * T;
+    float r00 = d[n];
+    r00 = r00 * (1.f - rx) + d[n + 1] * rx;
+    float r01 = d[n + T];
+    r01 = r01 * (1.f - rx) + d[n + T + 1] * rx;
+    r00 = r00 * (1.f - ry) + r01 * ry;
+    float r10 = d[n + T2];
+    r10 = r10 * (1.f - rx) + d[n + T2 + 1] * rx;
+    float r11 = d[n + T2 + T];
+    r11 = r11 * (1.f - rx) + d[n + T2 + T + 1] * rx;
+    r10 = r10 * (1.f - ry) + r11 * ry;
+    return r00 * (1.f - rz) + r10 * rz;
+}
+// 4-point Lagrange weights on nodes -1, 0, 1, 2 (exact on cubics)
+__device__ __forceinline__ void lag4(float t, float *w)
+{
+    w[0] = -t * (t - 1.f) * (t - 2.f) / 6.f;
+    w[1] = (t + 1.f) * (t - 1.f) * (t - 2.f) / 2.f;
+    w[2] = -(t + 1.f) * t * (t - 2.f) / 2.f;
+    w[3] = (t + 1.f) * t * (t - 1.f) / 6.f;
+}
+template <int T>
+__device__ __forceinline__ float interp_tricubic(const float *__restrict__ d, const float *wx, const float *wy,
+                                                 const float *wz, size_t n)
+{
+    constexpr int T2 = T * T;
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float ay = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float *r = d + n + (ptrdiff_t)(k - 1) * T2 + (ptrdiff_t)(j - 1) * T - 1;
+            const float ax = r[0] * wx[0] + r[1] * wx[1] + r[2] * wx[2] + r[3] * wx[3];
+            ay = ay + ax * wy[j];
+        }
+        acc = acc + ay * wz[k];
+    }
+    return acc;
+}
+
+template <int S, int P, int MODE>
+__global__ void __launch_bounds__(S *S) k_ghost_fill(FieldBatchDev b, const uint64_t *__restrict__ bnd,
+                                                     const int32_t *__restrict__ tiles, uint32_t substep,
+                                                     uint32_t maxd, int copy_only_new)
+{
+    constexpr int T = S + 2 * P;
+    constexpr size_t CELLS = (size_t)T * T * T;
+    const int ti = blockIdx.x / 6, dir = blockIdx.x - ti * 6;
+    const int var = blockIdx.y;
+    const int node = tiles[ti];
+    const uint64_t bb = bnd[(size_t)node * 6 + dir];
+    const uint32_t nd = (uint32_t)bb;
+    const int tgt = bnd_target(nd), cd = bnd_depth(nd);
+    if (tgt == -1 || cd <= 0) return;
+    if (!active_depth(substep, (uint32_t)cd, maxd)) return;
+
+    const float3 relp = cf3_unpack((uint32_t)(bb >> 32));
+    const uint32_t ddiff = bnd_ddiff(nd);
+    const float rel_scale = 1.f / (float)(1u << ddiff);
+    // relative_time (AMR.h:634-639)
+    const float tmp = (float)((substep >> (maxd - (uint32_t)cd)) & ((1u << ddiff) - 1u));
+    const float tx = tmp / (float)(1u << ddiff), ty = (tmp + 1.f) / (float)(1u << ddiff);
+
+    float *__restrict__ dold = b.old_[var];
+    float *__restrict__ dnew = b.new_[var];
+    const float *__restrict__ dolder = b.older[var];
+    const uint32_t a = threadIdx.x % S, bq = threadIdx.x / S;
+    constexpr uint32_t uP = P, uS = S;
+    const float invS = 1.f / (float)S;
+
+    for (uint32_t l = 0; l < uP; l++) {
+        uint32_t w0, w1, w2;
+        switch (dir) {  // zxy / xzy swizzles of (a, b, l) (AMR.h:714-735)
+        case 0: w0 = l + uP + uS; w1 = a + uP; w2 = bq + uP; break;
+        case 1: w0 = a + uP; w1 = l + uP + uS; w2 = bq + uP; break;
+        case 2: w0 = a + uP; w1 = bq + uP; w2 = l + uP + uS; break;
+        case 3: w0 = l; w1 = a + uP; w2 = bq + uP; break;
+        case 4: w0 = a + uP; w1 = l; w2 = bq + uP; break;
+        default: w0 = a + uP; w1 = bq + uP; w2 = l; break;
+        }
+        float rp[3] = {(float)w0, (float)w1, (float)w2};
+        const float rl[3] = {relp.x, relp.y, relp.z};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            rp[k] = (rp[k] - (float)uP + .5f) * invS - .5f;
+            rp[k] = rp[k] * rel_scale + rl[k];
+            rp[k] = (rp[k] + .5f) * (float)uS + (float)uP - 0.5f;
+        }
+        uint32_t ri[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (MODE == AM_INTERP_TRILINEAR)
+                ri[k] = (tgt == 0) ? min((uint32_t)fmaxf(rp[k], .5f), (uint32_t)T - 2u)
+                                   : min((uint32_t)fmaxf(rp[k], .5f + (float)uP), (uint32_t)T - uP - 2u);
+            else if (MODE == AM_INTERP_TRICUBIC)  // 4 nodes ri-1..ri+2, interior-only on non-root targets
+                ri[k] = (tgt == 0) ? min((uint32_t)fmaxf(rp[k], 1.f), (uint32_t)T - 3u)
+                                   : min((uint32_t)fmaxf(rp[k], (float)(uP + 1u)), (uint32_t)T - uP - 3u);
+            else
+                ri[k] = (tgt == 0) ? min((uint32_t)(fmaxf(rp[k], 0.f) + .5f), (uint32_t)T - 2u)
+                                   : min((uint32_t)(fmaxf(rp[k], (float)uP) + .5f), (uint32_t)T - uP - 2u);
+        }
+        const size_t rn = (size_t)tgt * CELLS + ri[0] + ((size_t)ri[1] + (size_t)ri[2] * T) * T;
+#pragma unroll
+        for (int k = 0; k < 3; k++) rp[k] -= (float)ri[k];
+        const size_t wn = (size_t)node * CELLS + w0 + ((size_t)w1 + (size_t)w2 * T) * T;
+
+        float o, nw;
+        if (MODE == AM_INTERP_TRICUBIC) {
+            float wx[4], wy[4], wz[4];
+            lag4(rp[0], wx); lag4(rp[1], wy); lag4(rp[2], wz);
+            o = interp_tricubic<T>(dold, wx, wy, wz, rn);
+            nw = interp_tricubic<T>(dnew, wx, wy, wz, rn);
+            if (dolder && ddiff > 0) {
+                const float od = interp_tricubic<T>(dolder, wx, wy, wz, rn);
+                float s = ty;
+                float c0 = .5f * s * (s - 1.f), c1 = (1.f - s) * (1.f + s), c2 = .5f * s * (s + 1.f);
+                const float vnew = od * c0 + o * c1 + nw * c2;
+                if (!copy_only_new) {
+                    s = tx;
+                    c0 = .5f * s * (s - 1.f); c1 = (1.f - s) * (1.f + s); c2 = .5f * s * (s + 1.f);
+                    dold[wn] = od * c0 + o * c1 + nw * c2;
+                }
+                dnew[wn] = vnew;
+                continue;
+            }
+        } else if (MODE == AM_INTERP_TRILINEAR) {
+            o = interp_trilinear<T>(dold, rp[0], rp[1], rp[2], rn);
+            nw = interp_trilinear<T>(dnew, rp[0], rp[1], rp[2], rn);
+        } else {
+            o = interp_lagrange<T>(dold, rp[0], rp[1], rp[2], rn);
+            nw = interp_lagrange<T>(dnew, rp[0], rp[1], rp[2], rn);
+        }
+        if (!copy_only_new) dold[wn] = o * (1.f - tx) + nw * tx;
+        dnew[wn] = o * (1.f - ty) + nw * ty;
+    }
+}
+
+template <int S, int P>
+static int launch_ghost(const am_field_batch *b, const am_mesh *mesh, int copy_only_new, int interp, void *stream)
+{
+    FieldBatchDev d;
+    for (int v = 0; v < b->n_vars; v++) {
+        if (!b->old_[v] || !b->new_[v]) return set_error(AM_ERR_INVALID, "am_ghost_fill: null field %d", v);
+        d.old_[v] = b->old_[v];
+        d.new_[v] = b->new_[v];
+        d.older[v] = b->older[v];
+    }
+    dim3 grid(mesh->n_tiles * 6, b->n_vars);
+    hipStream_t st = (hipStream_t)stream;
+    if (interp == AM_INTERP_LAGRANGE)
+        hipLaunchKernelGGL((k_ghost_fill<S, P, AM_INTERP_LAGRANGE>), grid, dim3(S * S), 0, st, d, mesh->boundaries,
+                           mesh->tiles, mesh->substep, mesh->max_depth, copy_only_new);
+    else if (interp == AM_INTERP_TRILINEAR)
+        hipLaunchKernelGGL((k_ghost_fill<S, P, AM_INTERP_TRILINEAR>), grid, dim3(S * S), 0, st, d, mesh->boundaries,
+                           mesh->tiles, mesh->substep, mesh->max_depth, copy_only_new);
+    else if (interp == AM_INTERP_TRICUBIC)
+        hipLaunchKernelGGL((k_ghost_fill<S, P, AM_INTERP_TRICUBIC>), grid, dim3(S * S), 0, st, d, mesh->boundaries,
+                           mesh->tiles, mesh->substep, mesh->max_depth, copy_only_new);
+    else
+        return set_error(AM_ERR_INVALID, "am_ghost_fill: unknown interpolation mode %d", interp);
+    return check_launch("k_ghost_fill");
+}
+
+// __copy_new_to_old (AMR.h:961-996)
+template <int S, int P>
+__global__ void __launch_bounds__(256) k_copy_new_to_old(FieldBatchDev b, const uint32_t *__restrict__ positions,
+                                                         const int32_t *__restrict__ children,
+                                                         const int32_t *__restrict__ tiles, uint32_t substep,
+                                                         uint32_t maxd)
+{
+    constexpr int T = S + 2 * P, T2 = T * T;
+    constexpr size_t CELLS = (size_t)T * T2;
+    const int ti = blockIdx.x / T, z = blockIdx.x - ti * T;
+    const int node = tiles[ti];
+    const int depth = pos_depth(positions[node]);
+    if (depth == -1 || !active_depth(substep, (uint32_t)depth, maxd)) return;
+    float *__restrict__ dold = b.old_[blockIdx.y];
+    const float *__restrict__ dnew = b.new_[blockIdx.y];
+    constexpr uint32_t H = S / 2u;
+    const uint32_t tz = ((uint32_t)z - (uint32_t)P) / H;
+    for (int i = threadIdx.x; i < T2; i += 256) {
+        const int y = i / T, x = i - y * T;
+        const uint32_t tx = ((uint32_t)x - (uint32_t)P) / H, ty = ((uint32_t)y - (uint32_t)P) / H;
+        const int child = (tx < 2u && ty < 2u && tz < 2u) ? children[tx + ty * 2u + tz * 4u + (uint32_t)node * 8u] : -1;
+        const size_t o = (size_t)node * CELLS + (size_t)z * T2 + i;
+        if (child == -1) {
+            dold[o] = dnew[o];
+            continue;
+        }
+        const size_t cx = (((uint32_t)x - (uint32_t)P) * 2u) % (uint32_t)S + (uint32_t)P;
+        const size_t cy = (((uint32_t)y - (uint32_t)P) * 2u) % (uint32_t)S + (uint32_t)P;
+        const size_t cz = (((uint32_t)z - (uint32_t)P) * 2u) % (uint32_t)S + (uint32_t)P;
+        const size_t c = (size_t)child * CELLS + (cz * T + cy) * T + cx;
+        dold[o] = (dnew[c] + dnew[c + 1] + dnew[c + T] + dnew[c + T + 1] + dnew[c + T2] + dnew[c + T2 + 1] +
+                   dnew[c + T2 + T] + dnew[c + T2 + T + 1]) *
+                  .125f;
+    }
+}
+
+template <int S, int P>
+static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, void *stream)
+{
+    constexpr int T = S + 2 * P;
+    FieldBatchDev d;
+    for (int v = 0; v < b->n_vars; v++) {
+        if (!b->old_[v] || !b->new_[v]) return set_error(AM_ERR_INVALID, "am_copy_new_to_old: null field %d", v);
+        d.old_[v] = b->old_[v];
+        d.new_[v] = b->new_[v];
+        d.older[v] = nullptr;
+    }
+    dim3 grid(mesh->n_tiles * T, b->n_vars);
+    hipLaunchKernelGGL((k_copy_new_to_old<S, P>), grid, dim3(256), 0, (hipStream_t)stream, d, mesh->positions,
+                       mesh->children, mesh->tiles, mesh->substep, mesh->max_depth);
+    return check_launch("k_copy_new_to_old");
+}
+
+// __copy_to_child (AMR.h:1053-1073)
+template <int S, int P, int MODE>
+__global__ void __launch_bounds__(256) k_copy_to_child(float *buf, uint32_t offset, int child, int parent)
+{
+    constexpr int T = S + 2 * P, T2 = T * T;
+    constexpr size_t CELLS = (size_t)T * T2;
+    const int z = blockIdx.x;
+    const float invS = 1.f / (float)S;
+    const float off[3] = {(float)(offset & 1u), (float)((offset & 2u) >> 1), (float)((offset & 4u) >> 2)};
+    for (int i = threadIdx.x; i < T2; i += 256) {
+        const int y = i / T, x = i - y * T;
+        const uint32_t idx[3] = {(uint32_t)x, (uint32_t)y, (uint32_t)z};
+        float rp[3];
+        uint32_t ri[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            rp[k] = ((float)idx[k] - (float)(uint32_t)P + .5f) * invS - .5f;
+            rp[k] = rp[k] * .5f + (off[k] - .5f) * .5f;
+            rp[k] = (rp[k] + .5f) * (float)(uint32_t)S + (float)(uint32_t)P - .5f;
+            ri[k] = (MODE == AM_INTERP_TRILINEAR) ? (uint32_t)rp[k] : (uint32_t)(rp[k] + .5f);
+        }
+        const size_t rn = (size_t)parent * CELLS + ri[0] + ((size_t)ri[1] + (size_t)ri[2] * T) * T;
+#pragma unroll
+        for (int k = 0; k < 3; k++) rp[k] -= (float)ri[k];
+        const float v = (MODE == AM_INTERP_TRILINEAR) ? interp_trilinear<T>(buf, rp[0], rp[1], rp[2], rn)
+                                                      : interp_lagrange<T>(buf, rp[0], rp[1], rp[2], rn);
+        buf[(size_t)child * CELLS + x + ((size_t)y + (size_t)z * T) * T] = v;
+    }
+}
+
+template <int S, int P>
+static int launch_to_child(float *buf, uint32_t offset, int child, int parent, int interp, void *stream)
+{
+    constexpr int T = S + 2 * P;
+    if (interp == AM_INTERP_TRILINEAR)
+        hipLaunchKernelGGL((k_copy_to_child<S, P, AM_INTERP_TRILINEAR>), dim3(T), dim3(256), 0, (hipStream_t)stream,
+                           buf, offset, child, parent);
+    else
+        hipLaunchKernelGGL((k_copy_to_child<S, P, AM_INTERP_LAGRANGE>), dim3(T), dim3(256), 0, (hipStream_t)stream,
+                           buf, offset, child, parent);
+    return check_launch("k_copy_to_child");
+}
+
+static int mesh_checks(const am_mesh *mesh, const am_geometry *g, bool need_bnd, bool need_children)
+{
+    if (!mesh || !g) return set_error(AM_ERR_INVALID, "null mesh or geometry");
+    if (!mesh->tiles || !mesh->positions) return set_error(AM_ERR_INVALID, "mesh without tiles or positions");
+    if (need_bnd && !mesh->boundaries) return set_error(AM_ERR_INVALID, "mesh without boundaries");
+    if (need_children && !mesh->children) return set_error(AM_ERR_INVALID, "mesh without children");
+    if (mesh->n_tiles < 0) return set_error(AM_ERR_INVALID, "negative tile count");
+    return AM_OK;
+}
+
+}  // namespace am
+
+extern "C" int am_ghost_fill_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g,
+                                     int32_t copy_only_new, int32_t interp, void *stream)
+{
+    if (!b) return am::set_error(AM_ERR_INVALID, "am_ghost_fill_batched: null batch");
+    if (b->n_vars < 0 || b->n_vars > AM_MAX_VARS)
+        return am::set_error(AM_ERR_TOO_MANY_VARS, "am_ghost_fill_batched: n_vars %d", b->n_vars);
+    if (int rc = am::mesh_checks(mesh, g, true, false)) return rc;
+    if (b->n_vars == 0 || mesh->n_tiles == 0) return AM_OK;
+    AM_DISPATCH_TILE(*g, am::launch_ghost, b, mesh, copy_only_new, interp, stream);
+}
+
+extern "C" int am_ghost_fill(float *old_, float *new_, const float *older, const am_mesh *mesh, const am_geometry *g,
+                             int32_t copy_only_new, int32_t interp, void *stream)
+{
+    am_field_batch b = {};
+    b.old_[0] = old_;
+    b.new_[0] = new_;
+    b.older[0] = const_cast<float *>(older);
+    b.n_vars = 1;
+    return am_ghost_fill_batched(&b, mesh, g, copy_only_new, interp, stream);
+}
+
+extern "C" int am_copy_new_to_old_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g,
+                                          void *stream)
+{
+    if (!b) return am::set_error(AM_ERR_INVALID, "am_copy_new_to_old_batched: null batch");
+    if (b->n_vars < 0 || b->n_vars > AM_MAX_VARS)
+        return am::set_error(AM_ERR_TOO_MANY_VARS, "am_copy_new_to_old_batched: n_vars %d", b->n_vars);
+    if (int rc = am::mesh_checks(mesh, g, false, true)) return rc;
+    if (b->n_vars == 0 || mesh->n_tiles == 0) return AM_OK;
+    AM_DISPATCH_TILE(*g, am::launch_copy_back, b, mesh, stream);
+}
+
+extern "C" int am_copy_new_to_old(float *old_, const float *new_, const am_mesh *mesh, const am_geometry *g,
+                                  void *stream)
+{
+    am_field_batch b = {};
+    b.old_[0] = old_;
+    b.new_[0] = const_cast<float *>(new_);
+    b.n_vars = 1;
+    return am_copy_new_to_old_batched(&b, mesh, g, stream);
+}
+
+extern "C" int am_copy_to_child(float *buf, uint32_t offset, int32_t child, int32_t parent, const am_geometry *g,
+                                int32_t interp, void *stream)
+{
+    if (!buf || !g) return am::set_error(AM_ERR_INVALID, "am_copy_to_child: null buffer or geometry");
+    if (offset > 7u || child < 0 || parent < 0) return am::set_error(AM_ERR_INVALID, "am_copy_to_child: bad indices");
+    if (interp != AM_INTERP_LAGRANGE && interp != AM_INTERP_TRILINEAR)
+        return am::set_error(AM_ERR_INVALID, "am_copy_to_child: interpolation %d not supported", interp);
+    AM_DISPATCH_TILE(*g, am::launch_to_child, buf, offset, child, parent, interp, stream);
+}

@@ -1,0 +1,113 @@
+// Error reporting, geometry set-up and thin device-memory helpers of the C ABI.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <string>
+
+#include "am_common.hpp"
+
+namespace am {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int check_launch(const char *what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error((int)e, "%s: %s", what, hipGetErrorString(e));
+    return AM_OK;
+}
+
+DepthScales depth_scales(const am_geometry &g)
+{
+    DepthScales s;
+    for (int d = 0; d < 9; d++) {
+        s.inv_dx[d] = (float)(1u << d) / g.outer_dx;
+        s.dt[d] = g.outer_dt / (float)(1u << d);
+    }
+    return s;
+}
+
+bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+static int hip_status(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) return set_error((int)e, "%s: %s", what, hipGetErrorString(e));
+    return AM_OK;
+}
+
+}  // namespace am
+
+extern "C" {
+
+const char *am_last_error_string(void) { return am::g_last_error.c_str(); }
+int am_version(void) { return 100; }
+
+int am_geometry_init(am_geometry *g, int32_t S, int32_t P, int32_t fd_order, float outer_size, float cfl)
+{
+    if (!g) return am::set_error(AM_ERR_INVALID, "am_geometry_init: null geometry");
+    if (S <= 0 || P <= 0 || (S & 1)) return am::set_error(AM_ERR_INVALID, "am_geometry_init: bad S=%d P=%d", S, P);
+    if (fd_order <= 0) fd_order = 2 * P;
+    if ((fd_order & 1) || fd_order > 2 * P || fd_order > 6)
+        return am::set_error(AM_ERR_INVALID, "am_geometry_init: fd_order %d must be even and <= min(2P, 6)", fd_order);
+    g->size_domain = S;
+    g->padding_domain = P;
+    g->fd_order = fd_order;
+    g->outer_size = outer_size;
+    /* constexpr float outer_dx = outer_size / size_domain (AMR.h:16) */
+    g->outer_dx = outer_size / (float)(uint32_t)S;
+    /* constexpr float outer_dt = outer_dx * .4f (AMR.h:17), CFL as a parameter */
+    g->outer_dt = g->outer_dx * cfl;
+    return AM_OK;
+}
+
+int am_geometry_supported(const am_geometry *g)
+{
+    if (!g) return 0;
+    const int s = g->size_domain, p = g->padding_domain, q = g->fd_order / 2;
+    if (s == 16 && p == 3 && q >= 1 && q <= 3) return 1;
+    if ((s == 16 || s == 28) && p == 2 && q >= 1 && q <= 2) return 1;
+    return 0;
+}
+
+int am_device_alloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return am::set_error(AM_ERR_INVALID, "am_device_alloc: null ptr");
+    const hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return am::set_error(AM_ERR_ALLOC, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return AM_OK;
+}
+int am_device_free(void *ptr) { return am::hip_status(hipFree(ptr), "hipFree"); }
+int am_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream)
+{
+    return am::hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream),
+                          "hipMemcpyAsync H2D");
+}
+int am_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream)
+{
+    int rc = am::hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream),
+                            "hipMemcpyAsync D2H");
+    if (rc) return rc;
+    return am::hip_status(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+int am_memset_async(void *dst, int value, size_t bytes, void *stream)
+{
+    return am::hip_status(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream), "hipMemsetAsync");
+}
+int am_stream_sync(void *stream) { return am::hip_status(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize"); }
+int am_device_sync(void) { return am::hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+/*
+ * adaptivemesh.h -- C ABI of the MI355X (gfx950) AdaptiveMesh per-tile path.
+ *
+ * Plain C: pointers, sizes and small POD structs only.  Every device pointer
+ * is owned by the caller and is never freed here (the reference's
+ * smart_gpu_buffer is likewise a raw cudaMalloc pointer with shallow copies,
+ * CUDA_memory.h:89-134).  Every call enqueues on the HIP stream passed as
+ * `stream` (NULL = the default stream) and returns 0 on success, a positive
+ * hipError_t value, or a negative AM_ERR_* code; am_last_error_string()
+ * describes the last failure on the calling thread.  Nothing throws across
+ * this boundary.
+ *
+ * Data layout (identical to the reference):
+ *   field      float  [slot][z][y][x] over the padded tile, T = S + 2P per
+ *                     side (AMR.h:11-15, 752, 758)
+ *   words      uint32 [slot][z][y][x] over the S^3 interior, one
+ *                     compressed_float3 per cell (AMR.h:1117, 1177)
+ *   positions  uint32 octree_abs_pos per slot (AMR.h:22-55)
+ *   children   int32  8 per slot, -1 = none (AMR.h:123)
+ *   boundaries uint64 6 per slot, node_data | rel_pos << 32 (AMR.h:56-100)
+ *
+ * Each function comment names the reference kernel and host wrapper it
+ * replaces.  The reference launches over every used slot and lets inactive
+ * depths exit early; here the caller passes a device list of slots
+ * (am_mesh.tiles), and a list of every used slot reproduces the reference.
+ */
+#ifndef ADAPTIVEMESH_H
+#define ADAPTIVEMESH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AM_MAX_VARS 32
+
+enum am_status {
+    AM_OK = 0,
+    AM_ERR_INVALID = -1,              /* bad argument (null pointer, bad size) */
+    AM_ERR_UNSUPPORTED_GEOMETRY = -2, /* no kernel instantiated for (S, P, order) */
+    AM_ERR_SLOTS_EXHAUSTED = -3,      /* octree has no free slot (AMR.h:210) */
+    AM_ERR_DEPTH_LIMIT = -4,          /* parent already at depth_limit (AMR.h:211) */
+    AM_ERR_OCCUPIED = -5,             /* child offset already in use (AMR.h:210) */
+    AM_ERR_MISALIGNED = -6,           /* field pointer not 16-byte aligned */
+    AM_ERR_TOO_MANY_VARS = -7,        /* more than AM_MAX_VARS in one batch */
+    AM_ERR_ALLOC = -8,                /* device allocation failed */
+    AM_ERR_LAUNCH = -9                /* kernel launch failed */
+};
+
+enum am_interp {
+    AM_INTERP_LAGRANGE = 0,  /* 10-point quadratic, the reference default (AMR.h:4, 673-699) */
+    AM_INTERP_TRILINEAR = 1, /* the reference's alternative (AMR.h:767-785) */
+    AM_INTERP_TRICUBIC = 2   /* 4x4x4 Lagrange in space, quadratic in time (extension) */
+};
+
+const char *am_last_error_string(void);
+int am_version(void);
+
+/* Tile geometry.  The reference fixes these as constexpr globals
+ * (AMR.h:9-17); here they are a runtime value and the kernels dispatch to an
+ * instantiation for (S, P, fd_order).  Instantiated: (16,3,{2,4,6}),
+ * (16,2,{2,4}), (28,2,{2,4}). */
+typedef struct am_geometry {
+    int32_t size_domain;    /* S, interior cells per side (AMR.h:11) */
+    int32_t padding_domain; /* P, ghost layers (AMR.h:12) */
+    int32_t fd_order;       /* 2, 4 or 6 and <= 2P; the reference uses 2P (AMR.h:1171) */
+    float outer_size;       /* root cube side (AMR.h:10) */
+    float outer_dx;         /* outer_size / S (AMR.h:16) */
+    float outer_dt;         /* outer_dx * cfl (AMR.h:17) */
+} am_geometry;
+
+int am_geometry_init(am_geometry *g, int32_t size_domain, int32_t padding_domain, int32_t fd_order,
+                     float outer_size, float cfl);
+int am_geometry_supported(const am_geometry *g);
+
+/* What a kernel sees of the octree: device copies of AMR's arrays (the
+ * reference's AMR::copy_to_gpu, AMR.h:268-275), the slot list to visit, and
+ * the sub-cycling counter timer_helper with max_depth (AMR.h:178-188). */
+typedef struct am_mesh {
+    const uint32_t *positions;
+    const int32_t *children;
+    const uint64_t *boundaries;
+    const int32_t *tiles;
+    int32_t n_tiles;
+    uint32_t substep;
+    uint32_t max_depth;
+} am_mesh;
+
+/* ------------------------------------------------------- device memory */
+int am_device_alloc(void **ptr, size_t bytes);
+int am_device_free(void *ptr);
+int am_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int am_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int am_memset_async(void *dst, int value, size_t bytes, void *stream);
+int am_stream_sync(void *stream);
+int am_device_sync(void);
+
+/* ------------------------------------------------------- FD stencils */
+
+/* First derivatives -> one word per interior cell.
+ * Replaces __differentiate(field, dif, ...) AMR.h:1152-1179 and its wrapper
+ * differentiate(field, dif, amr, stream, rng) AMR.h:1280-1290; `seed` is the
+ * wrapper's rng.generate_int(). */
+int am_fd_d1(const float *field, uint32_t *d1, const am_mesh *mesh, const am_geometry *g, int32_t seed,
+             void *stream);
+
+/* First derivatives + diagonal second derivatives (d2 packed first).
+ * Replaces __differentiate(field, dif, d2if, ...) AMR.h:1181-1219 and
+ * differentiate(field, dif, d2if, amr, stream, rng) AMR.h:1292-1302. */
+int am_fd_d1_d2diag(const float *field, uint32_t *d1, uint32_t *d2diag, const am_mesh *mesh,
+                    const am_geometry *g, int32_t seed, void *stream);
+
+/* Off-diagonal second derivatives (xy, xz, yz), boundary-careful at tile
+ * edges.  Replaces __mixed_partials AMR.h:1232-1266 and mixed_partials
+ * AMR.h:1268-1278. */
+int am_fd_mixed(const float *field, uint32_t *d2off, const am_mesh *mesh, const am_geometry *g, int32_t seed,
+                void *stream);
+
+/* Fused: gradient + full Hessian in one pass over the tile, bit-identical to
+ * am_fd_d1_d2diag(seed_d) followed by am_fd_mixed(seed_m).  Replaces
+ * yield_first_second_derivs AMR.h:1304-1310. */
+int am_fd_all(const float *field, uint32_t *d1, uint32_t *d2diag, uint32_t *d2off, const am_mesh *mesh,
+              const am_geometry *g, int32_t seed_d, int32_t seed_m, void *stream);
+
+/* Several fields in one launch.  Per variable v: d1[v] is required unless
+ * only d2off[v] is set; d2diag[v] != NULL adds the diagonal Hessian;
+ * d2off[v] != NULL adds the mixed partials (seeded by seed_m[v]). */
+typedef struct am_fd_batch {
+    const float *field[AM_MAX_VARS];
+    uint32_t *d1[AM_MAX_VARS];
+    uint32_t *d2diag[AM_MAX_VARS];
+    uint32_t *d2off[AM_MAX_VARS];
+    int32_t seed_d[AM_MAX_VARS];
+    int32_t seed_m[AM_MAX_VARS];
+    int32_t n_vars;
+    int32_t z_block; /* planes per workgroup: 0 = autotuned choice, else S, S/2 or S/4 */
+} am_fd_batch;
+int am_fd_batched(const am_fd_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream);
+
+/* ------------------------------------------------- ghost cells and tree */
+
+/* Fill the six face ghost slabs of every listed active tile from its
+ * boundary target, interpolated in space (interp) and in time between old
+ * and new (linear; quadratic through older/old/new when interp is
+ * AM_INTERP_TRICUBIC and older != NULL).  Replaces __copy_boundaries
+ * AMR.h:702-762 / 789-848 and copy_boundaries AMR.h:880-900.  Tiles whose
+ * target is being filled in the same call race exactly as in the reference;
+ * the C++ layer therefore calls it once per depth, coarse first. */
+int am_ghost_fill(float *old_, float *new_, const float *older, const am_mesh *mesh, const am_geometry *g,
+                  int32_t copy_only_new, int32_t interp, void *stream);
+
+typedef struct am_field_batch {
+    float *old_[AM_MAX_VARS];
+    float *new_[AM_MAX_VARS];
+    float *older[AM_MAX_VARS];
+    int32_t n_vars;
+} am_field_batch;
+int am_ghost_fill_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g,
+                          int32_t copy_only_new, int32_t interp, void *stream);
+
+/* old <- new on every cell of each listed active tile; interior cells with
+ * a child take the mean of the child's 2x2x2 new cells (restriction).
+ * Replaces __copy_new_to_old AMR.h:961-996 and copy_new_to_old
+ * AMR.h:998-1051.  Uses mesh->children. */
+int am_copy_new_to_old(float *old_, const float *new_, const am_mesh *mesh, const am_geometry *g, void *stream);
+int am_copy_new_to_old_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream);
+
+/* Prolong parent `parent` into every cell of new child `child` at octant
+ * `offset`.  Replaces __copy_to_child AMR.h:1053-1073 / copy_to_child
+ * AMR.h:1075-1082.  interp: AM_INTERP_LAGRANGE or AM_INTERP_TRILINEAR. */
+int am_copy_to_child(float *buf, uint32_t offset, int32_t child, int32_t parent, const am_geometry *g,
+                     int32_t interp, void *stream);
+
+/* ------------------------------------------------------ SimpleWave */
+
+/* Heun stages of u_t = nu lap u - u u_x, nu = 0.01.  Replaces
+ * __predictor_step / __corrector_step SimpleWave.h:7-46 and
+ * wave_equation_predictor / _corrector SimpleWave.h:48-72. */
+int am_wave_predictor(const float *old_, const uint32_t *d1, const uint32_t *d2diag, float *new_,
+                      const am_mesh *mesh, const am_geometry *g, void *stream);
+int am_wave_corrector(const float *old_, const uint32_t *d1, const uint32_t *d2diag, float *new_,
+                      const am_mesh *mesh, const am_geometry *g, void *stream);
+
+/* ------------------------------------------------------------ BSSN.h */
+
+/* out = gamma^{ij} d_i d_j f from the (diag, off) Hessian words and the six
+ * metric fields xx,xy,xz,yy,yz,zz.  Replaces __deriv_trace BSSN.h:7-22. */
+int am_deriv_trace(const uint32_t *hdiag, const uint32_t *hoff, const float *const *metric6, float *out,
+                   const am_mesh *mesh, const am_geometry *g, void *stream);
+
+/* Christoffel symbols from the six metric-derivative words.  Word jk holds
+ * (Gamma^x_jk, Gamma^y_jk, Gamma^z_jk).  compat = 1 reproduces the
+ * reference, which contracts with gamma_ij instead of gamma^ij
+ * (BSSN.h:82-83); compat = 0 is the true raised symbol.  Replaces
+ * __compute_raised_christoffel_symbols BSSN.h:55-91 and its wrapper
+ * BSSN.h:93-102. */
+int am_christoffel(const float *const *metric6, const uint32_t *const *dmetric6, uint32_t *const *out6,
+                   int32_t compat, const am_mesh *mesh, const am_geometry *g, int32_t seed, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADAPTIVEMESH_H */

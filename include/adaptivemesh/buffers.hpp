@@ -1,0 +1,109 @@
+// Host/device buffers of the drop-in API, the HIP counterparts of the
+// reference's smart_cpu_buffer / smart_gpu_buffer / smart_gpu_cpu_buffer
+// (CUDA_memory.h:49-210).  Like the reference they are shallow handles with
+// an explicit destroy(); unlike it, a failed allocation throws
+// std::runtime_error instead of leaving a half-built object.
+#pragma once
+#include <stddef.h>
+#include <string.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "../adaptivemesh.h"
+
+namespace am {
+
+inline void check(int rc, const char *what)
+{
+    if (rc != AM_OK) throw std::runtime_error(std::string(what) + ": " + am_last_error_string());
+}
+
+template <class T>
+struct smart_cpu_buffer {
+    size_t dedicated_len = 0;
+    T *cpu_buffer_ptr = nullptr;
+    bool created = false;
+
+    smart_cpu_buffer() = default;
+    explicit smart_cpu_buffer(size_t n) : dedicated_len(n), cpu_buffer_ptr(new T[n]), created(true) {}
+    size_t total_size() const { return dedicated_len * sizeof(T); }
+    T &operator[](size_t i) { return cpu_buffer_ptr[i]; }
+    const T &operator[](size_t i) const { return cpu_buffer_ptr[i]; }
+    void destroy()
+    {
+        if (created) delete[] cpu_buffer_ptr;
+        cpu_buffer_ptr = nullptr;
+        created = false;
+    }
+};
+
+template <class T>
+struct smart_gpu_buffer {
+    size_t dedicated_len = 0;
+    T *gpu_buffer_ptr = nullptr;
+    bool created = false;
+
+    smart_gpu_buffer() = default;
+    explicit smart_gpu_buffer(size_t n) : dedicated_len(n)
+    {
+        void *p = nullptr;
+        check(am_device_alloc(&p, n * sizeof(T) + 16), "smart_gpu_buffer");
+        gpu_buffer_ptr = static_cast<T *>(p);
+        created = true;
+    }
+    size_t total_size() const { return dedicated_len * sizeof(T); }
+    void swap_pointers(smart_gpu_buffer<T> &o)
+    {
+        if (dedicated_len != o.dedicated_len) throw std::runtime_error("cannot swap buffers of differing lengths");
+        T *t = gpu_buffer_ptr;
+        gpu_buffer_ptr = o.gpu_buffer_ptr;
+        o.gpu_buffer_ptr = t;
+    }
+    void zero(void *stream = nullptr) { check(am_memset_async(gpu_buffer_ptr, 0, total_size(), stream), "zero"); }
+    void destroy()
+    {
+        if (created) am_device_free(gpu_buffer_ptr);
+        gpu_buffer_ptr = nullptr;
+        created = false;
+    }
+};
+
+// Host array plus a device mirror.  The device side is allocated on the
+// first upload, so the octree logic runs (and is tested) without a GPU.
+template <class T>
+struct smart_gpu_cpu_buffer {
+    size_t dedicated_len = 0;
+    T *cpu_buffer_ptr = nullptr;
+    T *gpu_buffer_ptr = nullptr;
+
+    smart_gpu_cpu_buffer() = default;
+    explicit smart_gpu_cpu_buffer(size_t n) : dedicated_len(n), cpu_buffer_ptr(new T[n]) {}
+    size_t total_size() const { return dedicated_len * sizeof(T); }
+    void ensure_gpu()
+    {
+        if (gpu_buffer_ptr) return;
+        void *p = nullptr;
+        check(am_device_alloc(&p, total_size() + 16), "smart_gpu_cpu_buffer");
+        gpu_buffer_ptr = static_cast<T *>(p);
+    }
+    void copy_to_gpu(void *stream = nullptr)
+    {
+        ensure_gpu();
+        check(am_memcpy_h2d(gpu_buffer_ptr, cpu_buffer_ptr, total_size(), stream), "copy_to_gpu");
+        check(am_stream_sync(stream), "copy_to_gpu");  // the host copy may change right after
+    }
+    void copy_to_cpu(void *stream = nullptr)
+    {
+        check(am_memcpy_d2h(cpu_buffer_ptr, gpu_buffer_ptr, total_size(), stream), "copy_to_cpu");
+    }
+    void destroy()
+    {
+        delete[] cpu_buffer_ptr;
+        cpu_buffer_ptr = nullptr;
+        if (gpu_buffer_ptr) am_device_free(gpu_buffer_ptr);
+        gpu_buffer_ptr = nullptr;
+    }
+};
+
+}  // namespace am

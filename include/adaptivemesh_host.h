@@ -1,0 +1,61 @@
+/*
+ * adaptivemesh_host.h -- C ABI of the native host runtime: the CPU-managed
+ * octree (the reference's AMR<> class, AMR.h:118-294) and the physics
+ * plugins driving the kernels of adaptivemesh.h (SimpleWave.h:191-279,
+ * BSSN.h:104-171).  Opaque handles; every function returns 0 or an AM_ERR_*
+ * / hipError_t code unless it says otherwise.
+ */
+#ifndef ADAPTIVEMESH_HOST_H
+#define ADAPTIVEMESH_HOST_H
+
+#include "adaptivemesh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- octree */
+typedef struct am_tree am_tree;
+
+/* AMR(node_slots) (AMR.h:189-200).  periodic != 0 wraps the outer faces
+ * instead of pointing them at the root tile.  Returns NULL on failure. */
+am_tree *am_tree_create(uint32_t max_slots, int32_t periodic);
+void am_tree_destroy(am_tree *t);
+/* AMR::add_node (AMR.h:208-232): the new slot (>= 1) or an AM_ERR_* code */
+int am_tree_add_node(am_tree *t, int32_t parent, int32_t offset);
+/* AMR::remove_node (AMR.h:233-254) */
+int am_tree_remove_node(am_tree *t, int32_t node);
+/* AMR::increment_timer (AMR.h:180-188) */
+int am_tree_increment_timer(am_tree *t);
+/* AMR::find_node_idx (AMR.h:255-263) */
+uint32_t am_tree_find_node_idx(const am_tree *t, uint32_t packed_pos, uint32_t max_depth);
+/* Copy out the host arrays (any pointer may be NULL).  scalars receives
+ * {max_depth, first_free_slot, final_used_slot, timer_helper}. */
+int am_tree_export(const am_tree *t, int32_t *parent, int32_t *children, uint32_t *positions, uint64_t *boundaries,
+                   uint32_t *lifetime, int32_t *scalars);
+/* Upload if changed and describe the slots at `depth` (-1 = every used slot)
+ * for direct kernel calls.  Needs a GPU. */
+int am_tree_mesh(am_tree *t, int32_t depth, am_mesh *out);
+
+/* ------------------------------------------------------ SimpleWave driver */
+typedef struct am_wave_sim am_wave_sim;
+
+/* AMR<wave_AMR_data>(node_slots) with the given geometry.  NULL on failure. */
+am_wave_sim *am_wave_sim_create(const am_geometry *g, uint32_t max_slots, int32_t periodic);
+void am_wave_sim_destroy(am_wave_sim *s);
+int am_wave_sim_add_node(am_wave_sim *s, int32_t parent, int32_t offset);
+int am_wave_sim_export(const am_wave_sim *s, int32_t *parent, int32_t *children, uint32_t *positions,
+                       uint64_t *boundaries, uint32_t *lifetime, int32_t *scalars);
+/* which: 0 = old_field, 1 = new_field; whole padded arrays, max_slots * T^3 */
+int am_wave_sim_set_field(am_wave_sim *s, int32_t which, const float *host);
+int am_wave_sim_get_field(am_wave_sim *s, int32_t which, float *host);
+/* wave_AMR_data phases (SimpleWave.h:214-278) */
+int am_wave_sim_copy_bounds(am_wave_sim *s);
+int am_wave_sim_timestep(am_wave_sim *s);
+int32_t am_wave_sim_rng_state(const am_wave_sim *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADAPTIVEMESH_HOST_H */

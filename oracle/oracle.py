@@ -1,0 +1,220 @@
+"""ctypes bindings of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke entry point and the
+CPU-baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_lib = None
+_P = C.c_void_p
+_i32, _u32, _f32 = C.c_int32, C.c_uint32, C.c_float
+
+_SIGS = {
+    "orc_prng_int": (_i32, [C.POINTER(_i32)]),
+    "orc_prng_float": (_f32, [C.POINTER(_i32)]),
+    "orc_thread_seed": (_i32, [_i32, _u32, _u32, _u32]),
+    "orc_cf3_pack": (_u32, [_f32, _f32, _f32]),
+    "orc_cf3_pack_dither": (_u32, [_f32, _f32, _f32, C.POINTER(_i32)]),
+    "orc_cf3_unpack": (None, [_u32, _P]),
+    "orc_cf3_pack_n": (None, [_P, _P, C.c_int64]),
+    "orc_cf3_unpack_n": (None, [_P, _P, C.c_int64]),
+    "orc_morton_sum": (_u32, [_u32, _u32, C.c_int]),
+    "orc_morton_diff": (_u32, [_u32, _u32, C.c_int]),
+    "orc_reverse_bits": (_u32, [_u32]),
+    "orc_tree_create": (_P, [C.c_int, C.c_int]),
+    "orc_tree_destroy": (None, [_P]),
+    "orc_find_node_idx": (_u32, [_P, _u32, _u32]),
+    "orc_yield_boundary": (C.c_uint64, [_P, C.c_int, _u32]),
+    "orc_add_node": (C.c_int, [_P, C.c_int, C.c_int]),
+    "orc_remove_node": (C.c_int, [_P, C.c_int]),
+    "orc_increment_timer": (None, [_P]),
+    "orc_tree_export": (None, [_P, _P, _P, _P, _P, _P, _P]),
+    "orc_tree_set_timer": (None, [_P, _u32]),
+    "orc_differentiate": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, _i32, C.c_int, C.c_int, C.c_int, _f32]),
+    "orc_mixed_partials": (None, [_P, _P, _P, _P, C.c_int, _u32, _u32, _i32, C.c_int, C.c_int, _f32]),
+    "orc_copy_boundaries": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_copy_new_to_old": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
+    "orc_copy_to_child": (None, [_P, _u32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_wave_stage": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, _f32]),
+    "orc_deriv_trace": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
+    "orc_christoffel": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, _u32, _u32, _i32, C.c_int, C.c_int]),
+}
+
+
+def build() -> Path:
+    """Compile liboracle.so with the Makefile next to this file."""
+    r = subprocess.run(["make", "-C", str(HERE), "-s"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _i32arr(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+# ------------------------------------------------------------- primitives
+def prng_ints(seed: int, n: int) -> list[int]:
+    s = C.c_int32(seed)
+    return [lib().orc_prng_int(C.byref(s)) for _ in range(n)]
+
+
+def prng_floats(seed: int, n: int) -> list[float]:
+    s = C.c_int32(seed)
+    return [lib().orc_prng_float(C.byref(s)) for _ in range(n)]
+
+
+def thread_seed(gs: int, x: int, y: int, z: int) -> int:
+    return lib().orc_thread_seed(gs, x, y, z)
+
+
+def cf3_pack(x: float, y: float, z: float) -> int:
+    return lib().orc_cf3_pack(x, y, z)
+
+
+def cf3_pack_dither(x: float, y: float, z: float, seed: int) -> tuple[int, int]:
+    s = C.c_int32(seed)
+    w = lib().orc_cf3_pack_dither(x, y, z, C.byref(s))
+    return w, s.value
+
+
+def cf3_unpack(w) -> np.ndarray:
+    w = np.ascontiguousarray(np.atleast_1d(np.asarray(w, dtype=np.uint32)))
+    out = np.empty((w.size, 3), np.float32)
+    lib().orc_cf3_unpack_n(w.ctypes.data, out.ctypes.data, w.size)
+    return out
+
+
+def cf3_pack_array(xyz: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float32).reshape(-1, 3))
+    out = np.empty(a.shape[0], np.uint32)
+    lib().orc_cf3_pack_n(a.ctypes.data, out.ctypes.data, a.shape[0])
+    return out
+
+
+# ----------------------------------------------------------------- octree
+class OracleTree:
+    def __init__(self, max_slots: int, periodic: bool = False):
+        self.max_slots = max_slots
+        self.h = lib().orc_tree_create(max_slots, 1 if periodic else 0)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_tree_destroy(self.h)
+            self.h = None
+
+    def add_node(self, parent: int, offset: int) -> int:
+        return lib().orc_add_node(self.h, parent, offset)
+
+    def remove_node(self, node: int) -> int:
+        return lib().orc_remove_node(self.h, node)
+
+    def increment_timer(self):
+        lib().orc_increment_timer(self.h)
+
+    def set_timer(self, t: int):
+        lib().orc_tree_set_timer(self.h, t)
+
+    def find_node_idx(self, packed: int, max_depth: int = 0xFFFFFFFF) -> int:
+        return lib().orc_find_node_idx(self.h, packed, max_depth)
+
+    def yield_boundary(self, node: int, face: int) -> int:
+        return lib().orc_yield_boundary(self.h, node, face)
+
+    def export(self) -> dict:
+        n = self.max_slots
+        out = dict(parent=np.empty(n, np.int32), children=np.empty(8 * n, np.int32),
+                   positions=np.empty(n, np.uint32), boundaries=np.empty(6 * n, np.uint64),
+                   lifetime=np.empty(n, np.uint32), scalars=np.empty(4, np.int32))
+        lib().orc_tree_export(self.h, *(a.ctypes.data for a in out.values()))
+        s = out["scalars"]
+        out.update(max_depth=int(s[0]), first_free=int(s[1]), final_used=int(s[2]), timer=int(s[3]))
+        return out
+
+
+# ---------------------------------------------------------------- kernels
+def differentiate(field, positions, tiles, substep, max_depth, seed, S, P, q, outer_dx, n_slots, with_d2=True):
+    d1 = np.zeros(n_slots * S ** 3, np.uint32)
+    d2 = np.zeros(n_slots * S ** 3, np.uint32) if with_d2 else None
+    t = _i32arr(tiles)
+    lib().orc_differentiate(_p(field), _p(d1), _p(d2), _p(positions), _p(t), t.size, substep, max_depth, seed,
+                            S, P, q, outer_dx)
+    return d1, d2
+
+
+def mixed_partials(field, positions, tiles, substep, max_depth, seed, S, P, outer_dx, n_slots):
+    off = np.zeros(n_slots * S ** 3, np.uint32)
+    t = _i32arr(tiles)
+    lib().orc_mixed_partials(_p(field), _p(off), _p(positions), _p(t), t.size, substep, max_depth, seed, S, P,
+                             outer_dx)
+    return off
+
+
+def copy_boundaries(old, new, older, boundaries, tiles, substep, max_depth, copy_only_new, mode, S, P):
+    """In place on old/new (float32 arrays)."""
+    t = _i32arr(tiles)
+    lib().orc_copy_boundaries(_p(old), _p(new), _p(older), _p(boundaries), _p(t), t.size, substep, max_depth,
+                              1 if copy_only_new else 0, mode, S, P)
+
+
+def copy_new_to_old(old, new, positions, children, tiles, substep, max_depth, S, P):
+    t = _i32arr(tiles)
+    lib().orc_copy_new_to_old(_p(old), _p(new), _p(positions), _p(children), _p(t), t.size, substep, max_depth,
+                              S, P)
+
+
+def copy_to_child(buf, offset, child, parent, mode, S, P):
+    lib().orc_copy_to_child(_p(buf), offset, child, parent, mode, S, P)
+
+
+def wave_stage(old, d1, d2, new, positions, tiles, substep, max_depth, corrector, S, P, outer_dt):
+    t = _i32arr(tiles)
+    lib().orc_wave_stage(_p(old), _p(d1), _p(d2), _p(new), _p(positions), _p(t), t.size, substep, max_depth,
+                         1 if corrector else 0, S, P, outer_dt)
+
+
+def _ptr_table(arrs):
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def deriv_trace(hd, ho, metric6, positions, tiles, substep, max_depth, S, P, n_slots):
+    out = np.zeros(n_slots * S ** 3, np.float32)
+    t = _i32arr(tiles)
+    lib().orc_deriv_trace(_p(hd), _p(ho), _ptr_table(metric6), _p(out), _p(positions), _p(t), t.size, substep,
+                          max_depth, S, P)
+    return out
+
+
+def christoffel(metric6, dg6, compat, positions, tiles, substep, max_depth, seed, S, P, n_slots):
+    outs = [np.zeros(n_slots * S ** 3, np.uint32) for _ in range(6)]
+    t = _i32arr(tiles)
+    lib().orc_christoffel(_ptr_table(metric6), _ptr_table(dg6), _ptr_table(outs), compat, _p(positions), _p(t),
+                          t.size, substep, max_depth, seed, S, P)
+    return outs

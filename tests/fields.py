@@ -1,0 +1,44 @@
+"""Field helpers for the tests: analytic fields sampled on padded tiles."""
+import numpy as np
+
+
+def cell_coords(positions, slot, S, P, outer_size):
+    """Physical coordinates (x, y, z) of every padded cell of `slot`, shape
+    (T, T, T) each, indexed [z, y, x] (kernel.cu:21-22)."""
+    T = S + 2 * P
+    pos = int(positions[slot])
+    d = pos >> 27
+    c = np.zeros(3)
+    for i in range(d):
+        o = (pos >> (3 * i)) & 7
+        c += np.array([1 if o & 1 else -1, 1 if o & 2 else -1, 1 if o & 4 else -1]) * 0.25 / 2 ** i
+    c *= outer_size
+    h = outer_size / S / 2 ** d
+    r = (np.arange(T) + 0.5 - T / 2) * h
+    z, y, x = np.meshgrid(c[2] + r, c[1] + r, c[0] + r, indexing="ij")
+    return x, y, z
+
+
+def sample(positions, n_slots, S, P, outer_size, fn, slots=None, dtype=np.float32):
+    """fn(x, y, z) on every padded cell of every used slot."""
+    T = S + 2 * P
+    out = np.zeros((n_slots, T, T, T), dtype)
+    for s in range(n_slots) if slots is None else slots:
+        if int(positions[s]) == 0xFFFFFFFF:
+            continue
+        x, y, z = cell_coords(positions, s, S, P, outer_size)
+        out[s] = fn(x, y, z)
+    return out.reshape(-1)
+
+
+def interior(a, n_slots, S, P):
+    T = S + 2 * P
+    return a.reshape(n_slots, T, T, T)[:, P:P + S, P:P + S, P:P + S]
+
+
+def used_slots(exp):
+    return [i for i, p in enumerate(exp["positions"]) if int(p) != 0xFFFFFFFF]
+
+
+def depth_of(exp, s):
+    return int(exp["positions"][s]) >> 27

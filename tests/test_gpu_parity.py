@@ -1,0 +1,324 @@
+"""HIP kernels (through the C ABI) against the oracle, bit for bit.
+
+Every kernel that restates a reference kernel must produce the oracle's
+exact bits on the same seeded inputs: FD words (all orders, every supported
+tile geometry, every launch shape), ghost fills (three interpolation modes,
+every counter value), restriction, prolongation, SimpleWave stages,
+deriv_trace and Christoffel words, and a whole SimpleWave run through the
+native driver.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests import fields, meshes
+
+pytestmark = pytest.mark.gpu
+
+GEOMS = [(16, 3, 6), (16, 3, 4), (16, 3, 2), (16, 2, 4), (16, 2, 2), (28, 2, 4)]
+
+
+def dev(torch, a):
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    if a.dtype == np.uint64:
+        a = a.view(np.int64)
+    return torch.from_numpy(a.copy()).cuda()
+
+
+def host(t, dtype):
+    a = t.cpu().numpy()
+    return a.view(dtype) if a.dtype != dtype else a
+
+
+def setup_mesh(am, orc, recipe, periodic=False):
+    ops, n = {"graded": meshes.graded(), "uniform1": meshes.uniform(1), "uniform2": meshes.uniform(2),
+              "nested3": meshes.nested(3)}[recipe]
+    tree = am.Tree(n, periodic)
+    otree = orc.OracleTree(n, periodic)
+    meshes.apply(tree, ops)
+    meshes.apply(otree, ops)
+    return tree, otree.export(), n
+
+
+def mesh_for(tree, depth, substep):
+    m = tree.mesh(depth)
+    m.substep = substep
+    return m
+
+
+def sync(torch):
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("S,P,order", GEOMS)
+@pytest.mark.parametrize("recipe", ["graded", "uniform2"])
+@pytest.mark.parametrize("substep", [0, 1, 2])
+def test_fd_bit_exact(am, orc, torch_cuda, S, P, order, recipe, substep):
+    torch = torch_cuda
+    print(f"fd S={S} P={P} order={order} {recipe} substep={substep}", flush=True)
+    g = am.geometry(S, P, order, 100.0, 0.4)
+    tree, e, n = setup_mesh(am, orc, recipe)
+    T = S + 2 * P
+    rng = np.random.default_rng(S * 100 + order + substep)
+    field = rng.standard_normal(n * T ** 3).astype(np.float32)
+    seed_d, seed_m = int(rng.integers(-2 ** 31, 2 ** 31)), int(rng.integers(-2 ** 31, 2 ** 31))
+    tiles = fields.used_slots(e)
+    r1, r2 = orc.differentiate(field, e["positions"], tiles, substep, e["max_depth"], seed_d, S, P, order // 2,
+                               g.outer_dx, n)
+    r1only, _ = orc.differentiate(field, e["positions"], tiles, substep, e["max_depth"], seed_d, S, P, order // 2,
+                                  g.outer_dx, n, with_d2=False)
+    rm = orc.mixed_partials(field, e["positions"], tiles, substep, e["max_depth"], seed_m, S, P, g.outer_dx, n)
+
+    f = dev(torch, field)
+    m = mesh_for(tree, -1, substep)
+    L = am.lib()
+    st = am.stream_handle()
+    W = n * S ** 3
+    for zb in (0, S // 2, S // 4):
+        d1, d2, off = (torch.zeros(W, dtype=torch.int32, device="cuda") for _ in range(3))
+        b = am.FdBatch()
+        b.n_vars = 1
+        b.z_block = zb
+        b.field[0], b.d1[0], b.d2diag[0], b.d2off[0] = f.data_ptr(), d1.data_ptr(), d2.data_ptr(), off.data_ptr()
+        b.seed_d[0], b.seed_m[0] = seed_d, seed_m
+        am.check(L.am_fd_batched(C.byref(b), C.byref(m), C.byref(g), st), "am_fd_batched")
+        sync(torch)
+        assert np.array_equal(host(d1, np.uint32), r1), zb
+        assert np.array_equal(host(d2, np.uint32), r2), zb
+        assert np.array_equal(host(off, np.uint32), rm), zb
+    # single-purpose entry points
+    d1 = torch.zeros(W, dtype=torch.int32, device="cuda")
+    am.check(L.am_fd_d1(f.data_ptr(), d1.data_ptr(), C.byref(m), C.byref(g), seed_d, st), "am_fd_d1")
+    off = torch.zeros(W, dtype=torch.int32, device="cuda")
+    am.check(L.am_fd_mixed(f.data_ptr(), off.data_ptr(), C.byref(m), C.byref(g), seed_m, st), "am_fd_mixed")
+    sync(torch)
+    assert np.array_equal(host(d1, np.uint32), r1only)
+    assert np.array_equal(host(off, np.uint32), rm)
+
+
+def test_fd_batched_many_variables(am, orc, torch_cuda):
+    """24 fields in one launch, mixed outputs per variable, equal the
+    one-at-a-time oracle results."""
+    torch = torch_cuda
+    S, P = 16, 3
+    g = am.geometry(S, P, 4, 64.0, 0.25)
+    tree, e, n = setup_mesh(am, orc, "uniform2")
+    T = S + 2 * P
+    rng = np.random.default_rng(7)
+    tiles = fields.used_slots(e)
+    b = am.FdBatch()
+    b.n_vars = 24
+    keep, want = [], []
+    for v in range(24):
+        fld = rng.standard_normal(n * T ** 3).astype(np.float32)
+        sd, sm = int(rng.integers(-2 ** 31, 2 ** 31)), int(rng.integers(-2 ** 31, 2 ** 31))
+        hess = v % 2 == 0
+        r1, r2 = orc.differentiate(fld, e["positions"], tiles, 0, e["max_depth"], sd, S, P, 2, g.outer_dx, n, hess)
+        rm = orc.mixed_partials(fld, e["positions"], tiles, 0, e["max_depth"], sm, S, P, g.outer_dx, n) if hess else None
+        f = dev(torch, fld)
+        outs = [torch.zeros(n * S ** 3, dtype=torch.int32, device="cuda") for _ in range(3 if hess else 1)]
+        b.field[v] = f.data_ptr()
+        b.d1[v] = outs[0].data_ptr()
+        b.d2diag[v] = outs[1].data_ptr() if hess else None
+        b.d2off[v] = outs[2].data_ptr() if hess else None
+        b.seed_d[v], b.seed_m[v] = sd, sm
+        keep += [f] + outs
+        want.append((outs, [r1, r2, rm] if hess else [r1]))
+    m = mesh_for(tree, -1, 0)
+    am.check(am.lib().am_fd_batched(C.byref(b), C.byref(m), C.byref(g), am.stream_handle()), "batched")
+    sync(torch)
+    for outs, refs in want:
+        for o, r in zip(outs, refs):
+            assert np.array_equal(host(o, np.uint32), r)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("recipe,periodic", [("graded", False), ("nested3", False), ("uniform2", True)])
+@pytest.mark.parametrize("copy_only_new", [True, False])
+def test_ghost_fill_bit_exact(am, orc, torch_cuda, mode, recipe, periodic, copy_only_new):
+    torch = torch_cuda
+    S, P = 16, 3
+    print(f"ghost mode={mode} {recipe} periodic={periodic} con={copy_only_new}", flush=True)
+    g = am.geometry(S, P, 6, 100.0, 0.4)
+    tree, e, n = setup_mesh(am, orc, recipe, periodic)
+    T = S + 2 * P
+    rng = np.random.default_rng(mode * 10 + len(recipe))
+    depths = {}
+    for s in fields.used_slots(e):
+        depths.setdefault(fields.depth_of(e, s), []).append(s)
+    for substep in range(1 << e["max_depth"]):
+        old = rng.standard_normal(n * T ** 3).astype(np.float32)
+        new = rng.standard_normal(n * T ** 3).astype(np.float32)
+        older = rng.standard_normal(n * T ** 3).astype(np.float32)
+        do, dn, dol = dev(torch, old), dev(torch, new), dev(torch, older)
+        for d in sorted(depths):
+            if d == 0:
+                continue
+            orc.copy_boundaries(old, new, older if mode == 2 else None, e["boundaries"], depths[d], substep,
+                                e["max_depth"], copy_only_new, mode, S, P)
+            m = mesh_for(tree, d, substep)
+            am.check(am.lib().am_ghost_fill(do.data_ptr(), dn.data_ptr(), dol.data_ptr() if mode == 2 else None,
+                                            C.byref(m), C.byref(g), 1 if copy_only_new else 0, mode,
+                                            am.stream_handle()), "am_ghost_fill")
+        sync(torch)
+        assert np.array_equal(host(dn, np.float32).view(np.uint32), new.view(np.uint32)), substep
+        assert np.array_equal(host(do, np.float32).view(np.uint32), old.view(np.uint32)), substep
+
+
+@pytest.mark.parametrize("S,P", [(16, 3), (16, 2), (28, 2)])
+def test_copy_new_to_old_bit_exact(am, orc, torch_cuda, S, P):
+    torch = torch_cuda
+    g = am.geometry(S, P, 2 * P, 100.0, 0.4)
+    tree, e, n = setup_mesh(am, orc, "graded")
+    T = S + 2 * P
+    rng = np.random.default_rng(S + P)
+    for substep in range(4):
+        new = rng.standard_normal(n * T ** 3).astype(np.float32)
+        old = rng.standard_normal(n * T ** 3).astype(np.float32)
+        do, dn = dev(torch, old), dev(torch, new)
+        orc.copy_new_to_old(old, new, e["positions"], e["children"], fields.used_slots(e), substep, e["max_depth"],
+                            S, P)
+        m = mesh_for(tree, -1, substep)
+        am.check(am.lib().am_copy_new_to_old(do.data_ptr(), dn.data_ptr(), C.byref(m), C.byref(g),
+                                             am.stream_handle()), "copy_new_to_old")
+        sync(torch)
+        assert np.array_equal(host(do, np.float32).view(np.uint32), old.view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_copy_to_child_bit_exact(am, orc, torch_cuda, mode):
+    torch = torch_cuda
+    S, P = 16, 3
+    g = am.geometry(S, P, 6, 100.0, 0.4)
+    T = S + 2 * P
+    n = 9
+    rng = np.random.default_rng(mode)
+    buf = rng.standard_normal(n * T ** 3).astype(np.float32)
+    d = dev(torch, buf)
+    for c in range(8):
+        orc.copy_to_child(buf, c, c + 1, 0, mode, S, P)
+        am.check(am.lib().am_copy_to_child(d.data_ptr(), c, c + 1, 0, C.byref(g), mode, am.stream_handle()), "ctc")
+    sync(torch)
+    assert np.array_equal(host(d, np.float32).view(np.uint32), buf.view(np.uint32))
+
+
+def test_pointwise_bit_exact(am, orc, torch_cuda):
+    """SimpleWave stages, deriv_trace and both Christoffel modes."""
+    torch = torch_cuda
+    S, P = 16, 3
+    g = am.geometry(S, P, 6, 100.0, 0.4)
+    tree, e, n = setup_mesh(am, orc, "graded")
+    T = S + 2 * P
+    rng = np.random.default_rng(3)
+    tiles = fields.used_slots(e)
+    L, st = am.lib(), am.stream_handle()
+    for substep in range(3):
+        m = mesh_for(tree, -1, substep)
+        u = rng.standard_normal(n * T ** 3).astype(np.float32)
+        d1 = orc.cf3_pack_array(rng.standard_normal((n * S ** 3, 3)))
+        d2 = orc.cf3_pack_array(rng.standard_normal((n * S ** 3, 3)))
+        for corr in (False, True):
+            new = rng.standard_normal(n * T ** 3).astype(np.float32)
+            dnew = dev(torch, new)
+            orc.wave_stage(u, d1, d2, new, e["positions"], tiles, substep, e["max_depth"], corr, S, P, g.outer_dt)
+            fn = L.am_wave_corrector if corr else L.am_wave_predictor
+            du, dd1, dd2 = dev(torch, u), dev(torch, d1), dev(torch, d2)
+            am.check(fn(du.data_ptr(), dd1.data_ptr(), dd2.data_ptr(), dnew.data_ptr(), C.byref(m), C.byref(g), st),
+                     "wave")
+            sync(torch)
+            assert np.array_equal(host(dnew, np.float32).view(np.uint32), new.view(np.uint32))
+        a = rng.standard_normal((n * T ** 3, 3, 3)) * 0.2
+        met = np.eye(3) + a @ np.swapaxes(a, -1, -2)
+        metric = [np.ascontiguousarray(met[:, i, j].astype(np.float32)) for i, j in
+                  [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]]
+        dmet = [dev(torch, x) for x in metric]
+        mt = (C.c_void_p * 6)(*[x.data_ptr() for x in dmet])
+        hd, ho = d1, d2
+        want = orc.deriv_trace(hd, ho, metric, e["positions"], tiles, substep, e["max_depth"], S, P, n)
+        out = torch.zeros(n * S ** 3, dtype=torch.float32, device="cuda")
+        dhd, dho = dev(torch, hd), dev(torch, ho)
+        am.check(L.am_deriv_trace(dhd.data_ptr(), dho.data_ptr(), mt, out.data_ptr(), C.byref(m), C.byref(g), st),
+                 "trace")
+        sync(torch)
+        assert np.array_equal(host(out, np.float32).view(np.uint32), want.view(np.uint32))
+        dg = [orc.cf3_pack_array(rng.standard_normal((n * S ** 3, 3)) * 0.3) for _ in range(6)]
+        ddg = [dev(torch, x) for x in dg]
+        dgt = (C.c_void_p * 6)(*[x.data_ptr() for x in ddg])
+        for compat in (0, 1):
+            seed = int(rng.integers(-2 ** 31, 2 ** 31))
+            want = orc.christoffel(metric, dg, compat, e["positions"], tiles, substep, e["max_depth"], seed, S, P, n)
+            outs = [torch.zeros(n * S ** 3, dtype=torch.int32, device="cuda") for _ in range(6)]
+            ot = (C.c_void_p * 6)(*[x.data_ptr() for x in outs])
+            am.check(L.am_christoffel(mt, dgt, ot, compat, C.byref(m), C.byref(g), seed, st), "christoffel")
+            sync(torch)
+            for o, w in zip(outs, want):
+                assert np.array_equal(host(o, np.uint32), w)
+
+
+def oracle_wave_timestep(orc, e, n, old, new, rng, S, P, g, tiles_by_depth):
+    """wave_AMR_data::timestep (SimpleWave.h:268-278) driven on the oracle,
+    with the reference's seed draws: one per differentiate call."""
+    tiles = sorted(s for v in tiles_by_depth.values() for s in v)
+    maxd = e["max_depth"]
+    timer = 0
+    while True:
+        orc.copy_new_to_old(old, new, e["positions"], e["children"], tiles, timer, maxd, S, P)
+        d1, d2 = orc.differentiate(old, e["positions"], tiles, timer, maxd, orc.prng_ints(rng[0], 1)[0], S, P,
+                                   g.fd_order // 2, g.outer_dx, n)
+        rng[0] = orc.prng_ints(rng[0], 1)[0]
+        orc.wave_stage(old, d1, d2, new, e["positions"], tiles, timer, maxd, False, S, P, g.outer_dt)
+        for d in sorted(tiles_by_depth):
+            if d > 0:
+                orc.copy_boundaries(old, new, None, e["boundaries"], tiles_by_depth[d], timer, maxd, True, 0, S, P)
+        d1, d2 = orc.differentiate(new, e["positions"], tiles, timer, maxd, orc.prng_ints(rng[0], 1)[0], S, P,
+                                   g.fd_order // 2, g.outer_dx, n)
+        rng[0] = orc.prng_ints(rng[0], 1)[0]
+        orc.wave_stage(old, d1, d2, new, e["positions"], tiles, timer, maxd, True, S, P, g.outer_dt)
+        timer = (timer + 1) & ((1 << maxd) - 1)
+        if timer == 0:
+            return
+
+
+def test_simplewave_driver_bit_exact(am, orc, torch_cuda):
+    """The native AMR<wave_AMR_data> (static graded mesh) over two full
+    timesteps equals the oracle driven through the same phases and seeds."""
+    S, P = 16, 3
+    g = am.geometry(S, P, 6, 100.0, 0.4)
+    ops, n = meshes.graded()
+    sim = am.WaveSim(g, n)
+    otree = orc.OracleTree(n)
+    T = S + 2 * P
+    # initial data on the root, prolonged into children as add_node does
+    root = orc.OracleTree(1).export()
+    blob = fields.sample(root["positions"], 1, S, P, 100.0,
+                         lambda x, y, z: 10 / ((x - 25) ** 2 + (y - .5) ** 2 + z * z + 10)
+                         - 10 / ((x + 25) ** 2 + (y + .5) ** 2 + z * z + 10))
+    init = np.zeros(n * T ** 3, np.float32)
+    init[:T ** 3] = blob
+    sim.set_field(0, init)
+    sim.set_field(1, init)
+    old, new = init.copy(), init.copy()
+    for p, c in ops:
+        s = sim.add_node(p, c)
+        assert otree.add_node(p, c) == s
+        orc.copy_to_child(old, c, s, p, 0, S, P)
+        orc.copy_to_child(new, c, s, p, 0, S, P)
+    e = otree.export()
+    by_depth = {}
+    for s in fields.used_slots(e):
+        by_depth.setdefault(fields.depth_of(e, s), []).append(s)
+    assert np.array_equal(sim.get_field(0).view(np.uint32), old.view(np.uint32))
+    sim.copy_bounds()
+    for d in sorted(by_depth):
+        if d > 0:
+            orc.copy_boundaries(old, new, None, e["boundaries"], by_depth[d], 0, e["max_depth"], True, 0, S, P)
+    rng = [12894712]
+    for step in range(2):
+        sim.timestep()
+        oracle_wave_timestep(orc, e, n, old, new, rng, S, P, g, by_depth)
+        assert sim.rng_state == rng[0]
+        assert np.array_equal(sim.get_field(1).view(np.uint32), new.view(np.uint32)), step
+        assert np.array_equal(sim.get_field(0).view(np.uint32), old.view(np.uint32)), step
