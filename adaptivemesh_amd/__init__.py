@@ -134,6 +134,14 @@ def lib() -> C.CDLL:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: run `python -m adaptivemesh_amd.build` "
                                "(there is no CPU fallback)")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7.
+        # Loading torch first makes this library bind to that copy (same
+        # SONAME) instead of starting a second runtime that would find no
+        # device once torch holds it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)

@@ -60,7 +60,8 @@ struct domain_boundary {
     compressed_float3 rel_pos;
 
     domain_boundary() = default;
-    domain_boundary(const octree_abs_pos &this_d, const octree_abs_pos &other_d, int other_idx, bool periodic)
+    domain_boundary(const octree_abs_pos &this_d, const octree_abs_pos &other_d, int other_idx, bool periodic = false,
+                    int face = -1)
         : node_data(((uint32_t)other_idx & 16777215u) | ((uint32_t)((this_d.depth() - other_d.depth()) & 15) << 24u) |
                     ((uint32_t)(this_d.depth() & 15) << 28u))
     {
@@ -78,9 +79,14 @@ struct domain_boundary {
             r[1] -= ((s & 2u) ? mult : -mult) / (float)(1 << i);
             r[2] -= ((s & 4u) ? mult : -mult) / (float)(1 << i);
         }
-        if (periodic) {  // nearest periodic image; the cube is 2^d_target target widths wide
-            const float w = (float)(1u << other_d.depth()), h = .5f * w;
-            for (float &v : r) v = v > h ? v - w : (v < -h ? v + w : v);
+        if (periodic && face >= 0) {
+            // the periodic image of the target across `face`: along the face axis
+            // this - target < 0 for a + face, > 0 for a - face; the cube is
+            // 2^d_target target widths wide
+            const float w = (float)(1u << other_d.depth());
+            const int a = face % 3;
+            if (face < 3 && r[a] > 0.f) r[a] -= w;
+            if (face >= 3 && r[a] < 0.f) r[a] += w;
         }
         rel_pos = compressed_float3(r[0], r[1], r[2]);
     }
@@ -213,7 +219,7 @@ struct octree {
         int new_idx = (int)find_node_idx(reverse_bits(p) >> 5u, (uint32_t)depth);
         if (p == ~0u) new_idx = 0;  // the root tile stands in for the outer boundary
         return domain_boundary(positions.cpu_buffer_ptr[node_idx], positions.cpu_buffer_ptr[new_idx], new_idx,
-                               periodic);
+                               periodic, (int)face);
     }
 
    protected:

@@ -184,7 +184,7 @@ EXPORT uint32_t orc_morton_diff(uint32_t a, uint32_t b, int ov) { return morton_
 EXPORT uint32_t orc_reverse_bits(uint32_t a) { return reverse_bits(a); }
 
 /* domain_boundary ctor (AMR.h:64-86).  Stored as node_data | rel_pos << 32. */
-static uint64_t make_boundary(uint32_t this_d, uint32_t other_d, int other_idx, int periodic)
+static uint64_t make_boundary(uint32_t this_d, uint32_t other_d, int other_idx, int periodic, int face)
 {
     const int dt = pos_depth(this_d), dother = pos_depth(other_d);
     uint32_t nd = ((uint32_t)other_idx & 16777215u) | ((uint32_t)((dt - dother) & 15) << 24) |
@@ -203,13 +203,14 @@ static uint64_t make_boundary(uint32_t this_d, uint32_t other_d, int other_idx, 
         r[1] -= ((s & 2u) ? mult : -mult) / (float)(1 << i);
         r[2] -= ((s & 4u) ? mult : -mult) / (float)(1 << i);
     }
-    if (periodic) {
-        /* minimum image: the unit cube is 2^dother target widths across */
-        const float w = (float)(1u << dother), h = .5f * w;
-        for (int k = 0; k < 3; k++) {
-            if (r[k] > h) r[k] -= w;
-            else if (r[k] < -h) r[k] += w;
-        }
+    if (periodic && face >= 0) {
+        /* the image of the target that lies across this face: along the face
+         * axis this - target is negative for a + face and positive for a -
+         * face; the unit cube is 2^dother target widths across */
+        const float w = (float)(1u << dother);
+        const int a = face % 3;
+        if (face < 3 && r[a] > 0.f) r[a] -= w;
+        if (face >= 3 && r[a] < 0.f) r[a] += w;
     }
     return (uint64_t)nd | ((uint64_t)cf3_pack(r[0], r[1], r[2], NULL) << 32);
 }
@@ -290,7 +291,7 @@ static uint64_t yield_boundary(const orc_tree *t, int node, uint32_t face)
         p = morton_diff(p, (dirs[face - 3] << ((9u - (uint32_t)depth) * 3u)) & ((1u << 27) - 1u), ov);
     int idx = (int)find_node_idx(t, reverse_bits(p) >> 5, (uint32_t)depth);
     if (p == ~0u) idx = 0;
-    return make_boundary(t->pos[node], t->pos[idx], idx, t->periodic);
+    return make_boundary(t->pos[node], t->pos[idx], idx, t->periodic, (int)face);
 }
 EXPORT uint64_t orc_yield_boundary(const orc_tree *t, int node, uint32_t face)
 {

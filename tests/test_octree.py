@@ -61,19 +61,24 @@ def test_position_encoding_closed_form(orc):
 
 
 def brute_neighbour(pos, n_used, face, periodic):
-    """Deepest existing tile no deeper than `node` that contains the point one
-    cell-width beyond the face centre; -1 -> outer boundary."""
+    """Deepest existing tile no deeper than `node` that contains a point just
+    beyond the face centre (0 = the root, standing in for the outer
+    boundary), and the periodic shift applied to that point."""
     axis, sgn = face % 3, (1 if face < 3 else -1)
-    out = []
+    out, shifts = [], []
     for node in range(n_used):
         c, w = centre(int(pos[node]))
         p = c.copy()
         p[axis] += sgn * w * 0.75
+        shift = np.zeros(3)
         if np.any(np.abs(p) > 0.5):
             if not periodic:
                 out.append(0)
+                shifts.append(shift)
                 continue
-            p = (p + 0.5) % 1.0 - 0.5
+            q = (p + 0.5) % 1.0 - 0.5
+            shift, p = q - p, q
+        shifts.append(shift)
         best, bd = 0, -1
         for j in range(n_used):
             pj = int(pos[j])
@@ -84,7 +89,7 @@ def brute_neighbour(pos, n_used, face, periodic):
             if dj <= (int(pos[node]) >> 27) and np.all(np.abs(p - cj) < wj / 2) and dj > bd:
                 best, bd = j, dj
         out.append(best)
-    return out
+    return out, shifts
 
 
 @pytest.mark.parametrize("recipe", ["uniform2", "nested4", "graded"])
@@ -96,19 +101,19 @@ def test_neighbour_search_brute_force(orc, recipe, periodic):
     e = t.export()
     bnd = e["boundaries"].reshape(-1, 6)
     for face in range(6):
-        want = brute_neighbour(e["positions"], n, face, periodic)
+        want, shifts = brute_neighbour(e["positions"], n, face, periodic)
         for node in range(1, n):
             tgt, dd, depth, rel = decode_boundary(orc, bnd[node, face])
             assert tgt == want[node], (node, face)
             pn, pt = int(e["positions"][node]), int(e["positions"][tgt])
             assert depth == pn >> 27 and dd == (pn >> 27) - (pt >> 27)
-            # rel_pos = 2^d_target (c_this - c_target) in units of outer_size
+            # rel_pos = 2^d_target (c_this - c_target) in units of outer_size,
+            # with the target image that lies across the face
             cn, _ = centre(pn)
             ct, _ = centre(pt)
-            r = (cn - ct) * 2 ** (pt >> 27)
-            if periodic:
-                w = 2 ** (pt >> 27)
-                r = np.where(r > w / 2, r - w, np.where(r < -w / 2, r + w, r))
+            r = (cn - (ct - shifts[node])) * 2 ** (pt >> 27)
+            if tgt != 0 or periodic:
+                assert np.sign(r[face % 3]) == (-1 if face < 3 else 1)
             assert np.allclose(rel, r, atol=0), (node, face, rel, r)
         # root has no boundary
         assert decode_boundary(orc, bnd[0, face])[0] == -1
@@ -175,20 +180,6 @@ def test_refusals_leave_tree_unchanged(orc, am):
     for d in range(8):
         p = t.add_node(p, 0)
     assert t.add_node(p, 0) == -4  # depth limit (AMR.h:211)
-
-
-def random_ops(rng, n_slots, n_ops):
-    ops, live = [], {0: 0}
-    for _ in range(n_ops):
-        if rng.random() < 0.75 or len(live) == 1:
-            p = int(rng.choice(list(live)))
-            if live[p] >= 5:
-                continue
-            ops.append(("a", p, int(rng.integers(0, 8))))
-            live[len(live) + 1000 * len(ops)] = live[p] + 1  # placeholder; real slots tracked below
-        else:
-            ops.append(("r", int(rng.choice(list(live)[1:]))))
-    return ops
 
 
 @pytest.mark.parametrize("seed", range(6))
