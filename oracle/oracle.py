@@ -218,3 +218,43 @@ def christoffel(metric6, dg6, compat, positions, tiles, substep, max_depth, seed
     lib().orc_christoffel(_ptr_table(metric6), _ptr_table(dg6), _ptr_table(outs), compat, _p(positions), _p(t),
                           t.size, substep, max_depth, seed, S, P)
     return outs
+
+
+# ------------------------------------------------------------------- BSSN
+_SIGS_BSSN = {
+    "orc_bssn_point": (None, [_P, _P, _P, _P, _P, _P, _P]),
+    "orc_bssn_stage_from_words": (None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int,
+                                         _P, _P, C.c_int, C.c_int]),
+    "orc_bssn_constraints_from_words": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _P, C.c_int, C.c_int, _P]),
+    "orc_bssn64_stage": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, _P, _P, _P, C.c_int, C.c_int,
+                                C.c_int]),
+    "orc_bssn64_constraints": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P]),
+    "orc_ghost64": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, _u32, _u32, _f32, C.c_int, C.c_int, C.c_int]),
+    "orc_copy_back64": (None, [_P, _P, _P, C.c_int, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
+}
+_SIGS.update(_SIGS_BSSN)
+
+# eta, shift_coeff, chi_floor, advect_shift, slicing (0 = 1+log, 1 = harmonic)
+BSSN_DEFAULT = (2.0, 0.75, 1e-4, 1, 0)
+HESS_VARS = (0, 1, 2, 3, 4, 5, 6, 17, 18, 19, 20)
+
+
+def _params(p):
+    return np.ascontiguousarray(np.asarray(p, dtype=np.float64))
+
+
+def bssn_point(u, d1, dd, G=None, params=BSSN_DEFAULT):
+    """RHS (24) and constraints (H, M_x, M_y, M_z) at one point in FP64."""
+    u = np.ascontiguousarray(u, np.float64)
+    d1 = np.ascontiguousarray(d1, np.float64)
+    dd = np.ascontiguousarray(dd, np.float64)
+    G = None if G is None else np.ascontiguousarray(G, np.float64)
+    p = _params(params)
+    rhs = np.zeros(24)
+    hm = np.zeros(4)
+    lib().orc_bssn_point(_p(u), _p(d1), _p(dd), _p(G), _p(p), _p(rhs), _p(hm))
+    return rhs, hm
+
+
+def ptrs(arrs):
+    return (C.c_void_p * len(arrs))(*[None if a is None else a.ctypes.data for a in arrs])
