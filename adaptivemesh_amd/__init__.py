@@ -71,6 +71,26 @@ class FieldBatch(C.Structure):
                 ("older", C.c_void_p * AM_MAX_VARS), ("n_vars", C.c_int32)]
 
 
+class BssnParams(C.Structure):
+    _fields_ = [("eta", C.c_float), ("shift_coeff", C.c_float), ("chi_floor", C.c_float),
+                ("advect_shift", C.c_int32), ("slicing", C.c_int32)]
+
+
+class BssnFields(C.Structure):
+    _fields_ = [("state", C.c_void_p * 24), ("old_", C.c_void_p * 24), ("new_", C.c_void_p * 24),
+                ("acc", C.c_void_p * 24), ("d1", C.c_void_p * 24), ("d2diag", C.c_void_p * 11),
+                ("d2off", C.c_void_p * 11), ("chr", C.c_void_p * 6)]
+
+
+class BssnSimOptions(C.Structure):
+    _fields_ = [("params", BssnParams), ("interp", C.c_int32), ("z_block", C.c_int32), ("periodic", C.c_int32),
+                ("seed", C.c_int32)]
+
+
+BSSN_VARS = ["chi", "gt_xx", "gt_xy", "gt_xz", "gt_yy", "gt_yz", "gt_zz", "K", "At_xx", "At_xy", "At_xz", "At_yy",
+             "At_yz", "At_zz", "Gt_x", "Gt_y", "Gt_z", "alpha", "beta_x", "beta_y", "beta_z", "B_x", "B_y", "B_z"]
+BSSN_HESS_VARS = (0, 1, 2, 3, 4, 5, 6, 17, 18, 19, 20)
+
 _lib = None
 
 _P = C.c_void_p
@@ -120,6 +140,38 @@ _SIGS = {
     "am_wave_sim_copy_bounds": (C.c_int, [_P]),
     "am_wave_sim_timestep": (C.c_int, [_P]),
     "am_wave_sim_rng_state": (C.c_int32, [_P]),
+    "am_ghost_fill_stage_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), C.c_float,
+                                              C.c_int32, _P]),
+    "am_bssn_rhs_stage": (C.c_int, [C.POINTER(BssnFields), C.c_int32, C.POINTER(BssnParams), C.POINTER(Mesh),
+                                    C.POINTER(Geometry), _P]),
+    "am_bssn_constraints": (C.c_int, [C.POINTER(BssnFields), C.POINTER(BssnParams), C.POINTER(Mesh),
+                                      C.POINTER(Geometry), _P, _P]),
+    "am_bssn_constraint_blocks": (C.c_int64, [C.POINTER(Geometry), C.c_int32]),
+    "am_event_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "am_event_record": (C.c_int, [_P, _P]),
+    "am_event_elapsed_ms": (C.c_int, [_P, _P, C.POINTER(C.c_float)]),
+    "am_event_destroy": (C.c_int, [_P]),
+    "am_bssn_sim_create": (_P, [C.POINTER(Geometry), C.c_uint32, C.POINTER(BssnSimOptions)]),
+    "am_bssn_sim_destroy": (None, [_P]),
+    "am_bssn_sim_add_node": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "am_bssn_sim_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "am_bssn_sim_set_field": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
+    "am_bssn_sim_get_field": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
+    "am_bssn_sim_device_ptr": (_P, [_P, C.c_int32]),
+    "am_bssn_sim_fill_ghosts": (C.c_int, [_P]),
+    "am_bssn_sim_copy_back": (C.c_int, [_P]),
+    "am_bssn_sim_step_depth": (C.c_int, [_P, C.c_int32]),
+    "am_bssn_sim_stage_depth": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "am_bssn_sim_substep": (C.c_int, [_P]),
+    "am_bssn_sim_timestep": (C.c_int, [_P]),
+    "am_bssn_sim_constraints": (C.c_int, [_P, _P]),
+    "am_bssn_sim_sync": (C.c_int, [_P]),
+    "am_bssn_sim_zone_cycles": (C.c_uint64, [_P]),
+    "am_bssn_sim_rng_state": (C.c_int32, [_P]),
+    "am_bssn_sim_set_rng_state": (C.c_int, [_P, C.c_int32]),
+    "am_bssn_sim_counter": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]),
+    "am_bssn_sim_profile": (C.c_int, [_P, C.c_int32]),
+    "am_bssn_sim_kernel_times": (C.c_int, [_P, _P, _P]),
 }
 
 
@@ -267,3 +319,114 @@ class WaveSim:
     @property
     def rng_state(self) -> int:
         return lib().am_wave_sim_rng_state(self.h)
+
+
+def bssn_options(eta=2.0, shift_coeff=0.75, chi_floor=1e-4, advect_shift=1, slicing=0,
+                 interp=AM_INTERP_LAGRANGE, z_block=0, periodic=False, seed=12894712) -> BssnSimOptions:
+    o = BssnSimOptions()
+    o.params.eta, o.params.shift_coeff, o.params.chi_floor = eta, shift_coeff, chi_floor
+    o.params.advect_shift, o.params.slicing = advect_shift, slicing
+    o.interp, o.z_block, o.periodic, o.seed = interp, z_block, 1 if periodic else 0, seed
+    return o
+
+
+KERNEL_CLASSES = ("fd", "christoffel", "rhs_stage", "ghost", "copy_back")
+
+
+class BssnSim:
+    """AMR<BSSN_rk4_data>: 24-field BSSN with RK4 and sub-cycling, native."""
+
+    def __init__(self, geom: Geometry, max_slots: int, options: BssnSimOptions | None = None):
+        self.geom = geom
+        self.max_slots = max_slots
+        self.options = options or bssn_options()
+        self.h = lib().am_bssn_sim_create(C.byref(geom), max_slots, C.byref(self.options))
+        if not self.h:
+            raise AMError(-8, lib().am_last_error_string().decode())
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.am_bssn_sim_destroy(h)
+            self.h = None
+
+    def add_node(self, parent: int, offset: int) -> int:
+        r = lib().am_bssn_sim_add_node(self.h, parent, offset)
+        if r < 0:
+            raise AMError(r, lib().am_last_error_string().decode())
+        return r
+
+    def export(self) -> dict:
+        n = self.max_slots
+        out = dict(parent=np.empty(n, np.int32), children=np.empty(8 * n, np.int32),
+                   positions=np.empty(n, np.uint32), boundaries=np.empty(6 * n, np.uint64),
+                   lifetime=np.empty(n, np.uint32), scalars=np.empty(4, np.int32))
+        check(lib().am_bssn_sim_export(self.h, *(a.ctypes.data for a in out.values())), "export")
+        s = out["scalars"]
+        out.update(max_depth=int(s[0]), first_free=int(s[1]), final_used=int(s[2]), timer=int(s[3]))
+        return out
+
+    def set_field(self, which: int, var: int, arr: np.ndarray):
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        assert a.size == self.max_slots * self.geom.T ** 3
+        check(lib().am_bssn_sim_set_field(self.h, which, var, a.ctypes.data), "set_field")
+
+    def get_field(self, which: int, var: int) -> np.ndarray:
+        a = np.empty(self.max_slots * self.geom.T ** 3, np.float32)
+        check(lib().am_bssn_sim_get_field(self.h, which, var, a.ctypes.data), "get_field")
+        return a
+
+    def device_ptr(self, which: int) -> int:
+        return lib().am_bssn_sim_device_ptr(self.h, which)
+
+    def fill_ghosts(self):
+        check(lib().am_bssn_sim_fill_ghosts(self.h), "fill_ghosts")
+
+    def copy_back(self):
+        check(lib().am_bssn_sim_copy_back(self.h), "copy_back")
+
+    def step_depth(self, d: int):
+        check(lib().am_bssn_sim_step_depth(self.h, d), "step_depth")
+
+    def stage_depth(self, d: int, stage: int):
+        check(lib().am_bssn_sim_stage_depth(self.h, d, stage), "stage_depth")
+
+    def substep(self):
+        check(lib().am_bssn_sim_substep(self.h), "substep")
+
+    def timestep(self):
+        check(lib().am_bssn_sim_timestep(self.h), "timestep")
+
+    def sync(self):
+        check(lib().am_bssn_sim_sync(self.h), "sync")
+
+    def constraints(self) -> dict:
+        out = np.zeros(4)
+        check(lib().am_bssn_sim_constraints(self.h, out.ctypes.data), "constraints")
+        return dict(H_l2=out[0], M_l2=out[1], H_max=out[2], cells=out[3])
+
+    @property
+    def zone_cycles(self) -> int:
+        return lib().am_bssn_sim_zone_cycles(self.h)
+
+    @property
+    def rng_state(self) -> int:
+        return lib().am_bssn_sim_rng_state(self.h)
+
+    @rng_state.setter
+    def rng_state(self, v: int):
+        check(lib().am_bssn_sim_set_rng_state(self.h, v), "rng")
+
+    def counter(self):
+        t, d = C.c_uint32(), C.c_int32()
+        check(lib().am_bssn_sim_counter(self.h, C.byref(t), C.byref(d)), "counter")
+        return t.value, d.value
+
+    def profile(self, on: bool):
+        check(lib().am_bssn_sim_profile(self.h, 1 if on else 0), "profile")
+
+    def kernel_times(self) -> dict:
+        ms = np.zeros(len(KERNEL_CLASSES))
+        cnt = np.zeros(len(KERNEL_CLASSES), np.int64)
+        check(lib().am_bssn_sim_kernel_times(self.h, ms.ctypes.data, cnt.ctypes.data), "kernel_times")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}

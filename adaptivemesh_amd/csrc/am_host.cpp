@@ -4,6 +4,7 @@
 #include <stdexcept>
 
 #include "../../include/adaptivemesh/AMR.hpp"
+#include "../../include/adaptivemesh/BSSN.hpp"
 #include "../../include/adaptivemesh/SimpleWave.hpp"
 #include "../../include/adaptivemesh_host.h"
 #include "am_common.hpp"
@@ -16,6 +17,10 @@ struct am_tree {
 
 struct am_wave_sim {
     AMR<wave_AMR_data> *amr;
+};
+
+struct am_bssn_sim {
+    AMR<BSSN_rk4_data> *amr;
 };
 
 namespace {
@@ -193,5 +198,145 @@ int am_wave_sim_timestep(am_wave_sim *s)
     });
 }
 int32_t am_wave_sim_rng_state(const am_wave_sim *s) { return s ? s->amr->sim_data.rng.seed : 0; }
+
+am_bssn_sim *am_bssn_sim_create(const am_geometry *g, uint32_t max_slots, const am_bssn_sim_options *o)
+{
+    if (!g || !o || !am_geometry_supported(g)) {
+        am::set_error(AM_ERR_UNSUPPORTED_GEOMETRY, "am_bssn_sim_create: unsupported geometry");
+        return nullptr;
+    }
+    try {
+        bssn_rk4_options opt;
+        opt.params = o->params;
+        opt.interp = o->interp;
+        opt.z_block = o->z_block;
+        auto *s = new am_bssn_sim;
+        s->amr = new AMR<BSSN_rk4_data>(max_slots, o->periodic != 0, *g, opt);
+        s->amr->sim_data.rng.seed = o->seed;
+        return s;
+    } catch (const std::exception &e) {
+        am::set_error(AM_ERR_ALLOC, "am_bssn_sim_create: %s", e.what());
+        return nullptr;
+    }
+}
+void am_bssn_sim_destroy(am_bssn_sim *s)
+{
+    if (!s) return;
+    s->amr->sim_data.destroy();
+    s->amr->destroy();
+    delete s->amr;
+    delete s;
+}
+int am_bssn_sim_add_node(am_bssn_sim *s, int32_t parent, int32_t offset)
+{
+    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
+    return guarded([&] {
+        const int r = s->amr->add_node(parent, offset);
+        if (r < 0) am::set_error(r, "add_node(%d, %d) refused", parent, offset);
+        return r;
+    });
+}
+int am_bssn_sim_export(const am_bssn_sim *s, int32_t *parent, int32_t *children, uint32_t *positions,
+                       uint64_t *boundaries, uint32_t *lifetime, int32_t *scalars)
+{
+    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
+    return export_tree(*s->amr, parent, children, positions, boundaries, lifetime, scalars);
+}
+static smart_gpu_buffer<float> *bssn_state_buffer(am_bssn_sim *s, int which)
+{
+    auto &d = s->amr->sim_data;
+    switch (which) {
+    case 0: return &d.old_;
+    case 1: return &d.new_;
+    case 2: return &d.acc;
+    case 3: return d.older.created ? &d.older : nullptr;
+    default: return nullptr;
+    }
+}
+int am_bssn_sim_set_field(am_bssn_sim *s, int32_t which, int32_t var, const float *host)
+{
+    if (!s || !host || var < 0 || var >= AM_BSSN_NVARS) return am::set_error(AM_ERR_INVALID, "am_bssn_sim_set_field");
+    auto *b = bssn_state_buffer(s, which);
+    if (!b) return am::set_error(AM_ERR_INVALID, "am_bssn_sim_set_field: no buffer %d", which);
+    auto &d = s->amr->sim_data;
+    if (int rc = am_memcpy_h2d(d.field(*b, var), host, d.slots * d.cells * sizeof(float), d.stream)) return rc;
+    return am_stream_sync(d.stream);
+}
+int am_bssn_sim_get_field(am_bssn_sim *s, int32_t which, int32_t var, float *host)
+{
+    if (!s || !host || var < 0 || var >= AM_BSSN_NVARS) return am::set_error(AM_ERR_INVALID, "am_bssn_sim_get_field");
+    auto *b = bssn_state_buffer(s, which);
+    if (!b) return am::set_error(AM_ERR_INVALID, "am_bssn_sim_get_field: no buffer %d", which);
+    auto &d = s->amr->sim_data;
+    return am_memcpy_d2h(host, d.field(*b, var), d.slots * d.cells * sizeof(float), d.stream);
+}
+void *am_bssn_sim_device_ptr(am_bssn_sim *s, int32_t which)
+{
+    if (!s) return nullptr;
+    auto &d = s->amr->sim_data;
+    if (which >= 0 && which <= 3) {
+        auto *b = bssn_state_buffer(s, which);
+        return b ? (void *)b->gpu_buffer_ptr : nullptr;
+    }
+    switch (which) {
+    case 4: return d.d1.gpu_buffer_ptr;
+    case 5: return d.d2d.gpu_buffer_ptr;
+    case 6: return d.d2o.gpu_buffer_ptr;
+    case 7: return d.chr.gpu_buffer_ptr;
+    default: return nullptr;
+    }
+}
+#define BSSN_CALL(expr)                                             \
+    do {                                                            \
+        if (!s) return am::set_error(AM_ERR_INVALID, "null sim");    \
+        return guarded([&] {                                        \
+            expr;                                                   \
+            return AM_OK;                                           \
+        });                                                         \
+    } while (0)
+int am_bssn_sim_fill_ghosts(am_bssn_sim *s) { BSSN_CALL(s->amr->sim_data.fill_ghosts()); }
+int am_bssn_sim_copy_back(am_bssn_sim *s) { BSSN_CALL(s->amr->sim_data.copy_back()); }
+int am_bssn_sim_step_depth(am_bssn_sim *s, int32_t depth) { BSSN_CALL(s->amr->sim_data.step_depth(depth)); }
+int am_bssn_sim_stage_depth(am_bssn_sim *s, int32_t depth, int32_t stage)
+{
+    if (stage < 0 || stage > 3) return am::set_error(AM_ERR_INVALID, "stage %d", stage);
+    BSSN_CALL(s->amr->sim_data.stage_depth(depth, stage));
+}
+int am_bssn_sim_substep(am_bssn_sim *s) { BSSN_CALL(s->amr->sim_data.substep()); }
+int am_bssn_sim_timestep(am_bssn_sim *s) { BSSN_CALL(s->amr->sim_data.timestep()); }
+int am_bssn_sim_constraints(am_bssn_sim *s, double *out4)
+{
+    if (!out4) return am::set_error(AM_ERR_INVALID, "null output");
+    BSSN_CALL(s->amr->sim_data.constraints(out4));
+}
+int am_bssn_sim_sync(am_bssn_sim *s)
+{
+    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
+    return am_stream_sync(s->amr->sim_data.stream);
+}
+uint64_t am_bssn_sim_zone_cycles(const am_bssn_sim *s) { return s ? s->amr->sim_data.zone_cycles : 0; }
+int32_t am_bssn_sim_rng_state(const am_bssn_sim *s) { return s ? s->amr->sim_data.rng.seed : 0; }
+int am_bssn_sim_set_rng_state(am_bssn_sim *s, int32_t seed)
+{
+    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
+    s->amr->sim_data.rng.seed = seed;
+    return AM_OK;
+}
+int am_bssn_sim_profile(am_bssn_sim *s, int32_t enable)
+{
+    BSSN_CALL(s->amr->sim_data.timer.enable(enable != 0, s->amr->sim_data.stream));
+}
+int am_bssn_sim_kernel_times(am_bssn_sim *s, double *ms5, int64_t *count5)
+{
+    if (!ms5 || !count5) return am::set_error(AM_ERR_INVALID, "null output");
+    BSSN_CALL(s->amr->sim_data.timer.totals(ms5, count5));
+}
+int am_bssn_sim_counter(const am_bssn_sim *s, uint32_t *timer, int32_t *max_depth)
+{
+    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
+    if (timer) *timer = s->amr->timer_helper;
+    if (max_depth) *max_depth = s->amr->read_max_depth();
+    return AM_OK;
+}
 
 }  // extern "C"

@@ -92,7 +92,7 @@ __device__ __forceinline__ float interp_tricubic(const float *__restrict__ d, co
 template <int S, int P, int MODE>
 __global__ void __launch_bounds__(S *S) k_ghost_fill(FieldBatchDev b, const uint64_t *__restrict__ bnd,
                                                      const int32_t *__restrict__ tiles, uint32_t substep,
-                                                     uint32_t maxd, int copy_only_new)
+                                                     uint32_t maxd, int copy_only_new, float frac)
 {
     constexpr int T = S + 2 * P;
     constexpr size_t CELLS = (size_t)T * T * T;
@@ -110,7 +110,8 @@ __global__ void __launch_bounds__(S *S) k_ghost_fill(FieldBatchDev b, const uint
     const float rel_scale = 1.f / (float)(1u << ddiff);
     // relative_time (AMR.h:634-639)
     const float tmp = (float)((substep >> (maxd - (uint32_t)cd)) & ((1u << ddiff) - 1u));
-    const float tx = tmp / (float)(1u << ddiff), ty = (tmp + 1.f) / (float)(1u << ddiff);
+    // frac = 1 is the reference's k + 1; an equal-depth target holds the same stage
+    const float tx = tmp / (float)(1u << ddiff), ty = ddiff == 0 ? 1.f : (tmp + frac) / (float)(1u << ddiff);
 
     float *__restrict__ dold = b.old_[var];
     float *__restrict__ dnew = b.new_[var];
@@ -187,7 +188,8 @@ __global__ void __launch_bounds__(S *S) k_ghost_fill(FieldBatchDev b, const uint
 }
 
 template <int S, int P>
-static int launch_ghost(const am_field_batch *b, const am_mesh *mesh, int copy_only_new, int interp, void *stream)
+static int launch_ghost(const am_field_batch *b, const am_mesh *mesh, int copy_only_new, int interp, float frac,
+                        void *stream)
 {
     FieldBatchDev d;
     for (int v = 0; v < b->n_vars; v++) {
@@ -200,13 +202,13 @@ static int launch_ghost(const am_field_batch *b, const am_mesh *mesh, int copy_o
     hipStream_t st = (hipStream_t)stream;
     if (interp == AM_INTERP_LAGRANGE)
         hipLaunchKernelGGL((k_ghost_fill<S, P, AM_INTERP_LAGRANGE>), grid, dim3(S * S), 0, st, d, mesh->boundaries,
-                           mesh->tiles, mesh->substep, mesh->max_depth, copy_only_new);
+                           mesh->tiles, mesh->substep, mesh->max_depth, copy_only_new, frac);
     else if (interp == AM_INTERP_TRILINEAR)
         hipLaunchKernelGGL((k_ghost_fill<S, P, AM_INTERP_TRILINEAR>), grid, dim3(S * S), 0, st, d, mesh->boundaries,
-                           mesh->tiles, mesh->substep, mesh->max_depth, copy_only_new);
+                           mesh->tiles, mesh->substep, mesh->max_depth, copy_only_new, frac);
     else if (interp == AM_INTERP_TRICUBIC)
         hipLaunchKernelGGL((k_ghost_fill<S, P, AM_INTERP_TRICUBIC>), grid, dim3(S * S), 0, st, d, mesh->boundaries,
-                           mesh->tiles, mesh->substep, mesh->max_depth, copy_only_new);
+                           mesh->tiles, mesh->substep, mesh->max_depth, copy_only_new, frac);
     else
         return set_error(AM_ERR_INVALID, "am_ghost_fill: unknown interpolation mode %d", interp);
     return check_launch("k_ghost_fill");
@@ -227,6 +229,7 @@ __global__ void __launch_bounds__(256) k_copy_new_to_old(FieldBatchDev b, const 
     if (depth == -1 || !active_depth(substep, (uint32_t)depth, maxd)) return;
     float *__restrict__ dold = b.old_[blockIdx.y];
     const float *__restrict__ dnew = b.new_[blockIdx.y];
+    float *__restrict__ dolder = const_cast<float *>(b.older[blockIdx.y]);
     constexpr uint32_t H = S / 2u;
     const uint32_t tz = ((uint32_t)z - (uint32_t)P) / H;
     for (int i = threadIdx.x; i < T2; i += 256) {
@@ -234,6 +237,7 @@ __global__ void __launch_bounds__(256) k_copy_new_to_old(FieldBatchDev b, const 
         const uint32_t tx = ((uint32_t)x - (uint32_t)P) / H, ty = ((uint32_t)y - (uint32_t)P) / H;
         const int child = (tx < 2u && ty < 2u && tz < 2u) ? children[tx + ty * 2u + tz * 4u + (uint32_t)node * 8u] : -1;
         const size_t o = (size_t)node * CELLS + (size_t)z * T2 + i;
+        if (dolder) dolder[o] = dold[o];
         if (child == -1) {
             dold[o] = dnew[o];
             continue;
@@ -257,7 +261,7 @@ static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, void *
         if (!b->old_[v] || !b->new_[v]) return set_error(AM_ERR_INVALID, "am_copy_new_to_old: null field %d", v);
         d.old_[v] = b->old_[v];
         d.new_[v] = b->new_[v];
-        d.older[v] = nullptr;
+        d.older[v] = b->older[v];
     }
     dim3 grid(mesh->n_tiles * T, b->n_vars);
     hipLaunchKernelGGL((k_copy_new_to_old<S, P>), grid, dim3(256), 0, (hipStream_t)stream, d, mesh->positions,
@@ -328,7 +332,20 @@ extern "C" int am_ghost_fill_batched(const am_field_batch *b, const am_mesh *mes
         return am::set_error(AM_ERR_TOO_MANY_VARS, "am_ghost_fill_batched: n_vars %d", b->n_vars);
     if (int rc = am::mesh_checks(mesh, g, true, false)) return rc;
     if (b->n_vars == 0 || mesh->n_tiles == 0) return AM_OK;
-    AM_DISPATCH_TILE(*g, am::launch_ghost, b, mesh, copy_only_new, interp, stream);
+    AM_DISPATCH_TILE(*g, am::launch_ghost, b, mesh, copy_only_new, interp, 1.f, stream);
+}
+
+extern "C" int am_ghost_fill_stage_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g,
+                                           float stage_frac, int32_t interp, void *stream)
+{
+    if (!b) return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_batched: null batch");
+    if (b->n_vars < 0 || b->n_vars > AM_MAX_VARS)
+        return am::set_error(AM_ERR_TOO_MANY_VARS, "am_ghost_fill_stage_batched: n_vars %d", b->n_vars);
+    if (!(stage_frac >= 0.f && stage_frac <= 1.f))
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_batched: stage_frac %g", (double)stage_frac);
+    if (int rc = am::mesh_checks(mesh, g, true, false)) return rc;
+    if (b->n_vars == 0 || mesh->n_tiles == 0) return AM_OK;
+    AM_DISPATCH_TILE(*g, am::launch_ghost, b, mesh, 1, interp, stage_frac, stream);
 }
 
 extern "C" int am_ghost_fill(float *old_, float *new_, const float *older, const am_mesh *mesh, const am_geometry *g,

@@ -110,4 +110,21 @@ int am_memset_async(void *dst, int value, size_t bytes, void *stream)
 int am_stream_sync(void *stream) { return am::hip_status(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize"); }
 int am_device_sync(void) { return am::hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
 
+int am_event_create(void **ev)
+{
+    if (!ev) return am::set_error(AM_ERR_INVALID, "am_event_create: null");
+    return am::hip_status(hipEventCreate((hipEvent_t *)ev), "hipEventCreate");
+}
+int am_event_record(void *ev, void *stream)
+{
+    return am::hip_status(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream), "hipEventRecord");
+}
+int am_event_elapsed_ms(void *start, void *stop, float *ms)
+{
+    if (!ms) return am::set_error(AM_ERR_INVALID, "am_event_elapsed_ms: null");
+    if (int rc = am::hip_status(hipEventSynchronize((hipEvent_t)stop), "hipEventSynchronize")) return rc;
+    return am::hip_status(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop), "hipEventElapsedTime");
+}
+int am_event_destroy(void *ev) { return am::hip_status(hipEventDestroy((hipEvent_t)ev), "hipEventDestroy"); }
+
 }  // extern "C"

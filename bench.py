@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""BSSN zone-cycles/s on one MI355X, with the roofline of the dominant kernel
+and a CPU baseline.
+
+Workload (one GPU): the BSSN single-level mesh -- 512 tiles of 16^3 interior
+cells (128^3) at depth 3 with 3-cell padding, their 73 ancestors, periodic
+outer faces, domain [-32, 32]^3, 24 fp32 fields, 1+log slicing and
+Gamma-driver shift, flat metric plus seeded uniform noise of +-1e-6 on every
+field, FD order 4, derivatives and Christoffel symbols as compressed_float3
+words, CFL 0.25.  One step = one timestep() of the octree driver = 2^3 fine
+RK4 sub-steps (depth d advances every 2^(3-d) sub-steps), so the default
+12 steps are ~100 fine RK4 steps.  A zone-cycle is one interior cell
+advanced by one full RK4 step at its own depth, ancestors included.
+
+Output: one JSON line (rank 0).  --gpus > 1 is refused: this build runs on
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "BSSN zone-cycles/s + %HBM peak; learned vs plain ghost fill at equal ‖H‖₂"
+HBM_PEAK_GBS = 8000.0
+S, P, FD_ORDER, OUTER, CFL, DEPTH = 16, 3, 4, 64.0, 0.25, 3
+NOISE = 1e-6
+
+# algorithmic bytes per interior cell per RK4 stage (each distinct element
+# read once and written once per launch)
+WORD = 4
+BYTES = {
+    # 24 fields: padded reads (S^3 + 6 P S^2) * 4 per field; 24 gradient + 11 diag + 11 off-diag words
+    "fd": 24 * (S ** 3 + 6 * P * S * S) * 4 / S ** 3 + 46 * WORD,
+    # gt_ij (6 fp32) + 6 gradient words in, 6 words out
+    "christoffel": 6 * 4 + 6 * WORD + 6 * WORD,
+    # per stage: state 96, 46 derivative words 184, 6 Christoffel 24; y_n 96 and acc 96 when
+    # distinct; writes new 96 and acc 96 (stage 3 writes no acc)
+    "rhs_stage": (96 + 184 + 24 + 192) + (96 + 184 + 24 + 96 + 96 + 192) * 2 + (96 + 184 + 24 + 96 + 96 + 96),
+    # 6 P S^2 face cells per tile, one source read and one write, 24 fields
+    "ghost": 24 * 6 * P * S * S * 8 / S ** 3,
+    # T^3 read + T^3 write per field, once per step
+    "copy_back": 24 * 2 * 4 * (S + 2 * P) ** 3 / S ** 3,
+}
+STAGES_PER_STEP = {"fd": 4, "christoffel": 4, "rhs_stage": 1, "ghost": 4, "copy_back": 1}
+
+
+def build_mesh(sim):
+    frontier, nxt = [0], 1
+    for _ in range(DEPTH):
+        nf = []
+        for p in frontier:
+            for c in range(8):
+                assert sim.add_node(p, c) == nxt
+                nf.append(nxt)
+                nxt += 1
+        frontier = nf
+    return nxt
+
+
+def initial_data(n_slots, T):
+    """Flat space plus uniform noise in [-1e-6, 1e-6) on every field and
+    cell: numpy PCG64 seeded with 20240 + field index."""
+    flat = np.zeros(24, np.float32)
+    flat[[0, 1, 4, 6, 17]] = 1.0  # chi, gt_xx, gt_yy, gt_zz, alpha
+    out = []
+    for v in range(24):
+        rng = np.random.default_rng(20240 + v)
+        out.append((flat[v] + rng.uniform(-NOISE, NOISE, n_slots * T ** 3)).astype(np.float32))
+    return out
+
+
+def cpu_baseline(n_slots, fields, positions, tiles_all, budget_s=12.0):
+    """The oracle (a port of the scheme: bit-exact FP32 stencils and
+    Christoffel words, FP64 RHS) running one RK4 step on a sample of depth-3
+    tiles on this host's cores."""
+    from oracle import oracle as orc
+    T = S + 2 * P
+    outer_dx = np.float32(OUTER) / np.float32(S)
+    inv_dx = float(np.float32(2 ** DEPTH) / outer_dx)
+    dtd = np.array([float(np.float32(outer_dx * np.float32(CFL)) / 2 ** d) for d in range(9)])
+    params = np.array(orc.BSSN_DEFAULT, np.float64)
+    tiles = np.array([t for t in tiles_all if (int(positions[t]) >> 27) == DEPTH], np.int32)
+    threads = int(orc.lib().orc_num_threads())
+    seed = 12345
+
+    def one_step(sample):
+        state = list(fields)
+        acc = [np.zeros(n_slots * T ** 3) for _ in range(24)]
+        new = [np.zeros(n_slots * T ** 3) for _ in range(24)]
+        for stage in range(4):
+            d1, d2d, d2o = [], [], []
+            for v in range(24):
+                a, b = orc.differentiate(state[v], positions, sample, 0, DEPTH, seed + v, S, P, FD_ORDER // 2,
+                                         float(outer_dx), n_slots, with_d2=v in orc.HESS_VARS)
+                d1.append(a)
+                if v in orc.HESS_VARS:
+                    d2d.append(b)
+                    d2o.append(orc.mixed_partials(state[v], positions, sample, 0, DEPTH, seed - v, S, P,
+                                                  float(outer_dx), n_slots))
+            chw = orc.christoffel(state[1:7], d1[1:7], 0, positions, sample, 0, DEPTH, seed, S, P, n_slots)
+            orc.lib().orc_bssn_stage_from_words(orc.ptrs(state), orc.ptrs(fields), orc.ptrs(acc), orc.ptrs(acc),
+                                                orc.ptrs(new), orc.ptrs(d1), orc.ptrs(d2d), orc.ptrs(d2o),
+                                                orc.ptrs(chw), positions.ctypes.data, sample.ctypes.data,
+                                                len(sample), 0, DEPTH, stage, dtd.ctypes.data, params.ctypes.data,
+                                                S, P)
+            state = [x.astype(np.float32) for x in new]
+        return len(sample) * S ** 3
+
+    n = 8
+    t0 = time.perf_counter()
+    cells = one_step(tiles[:n])
+    dt = time.perf_counter() - t0
+    # scale the sample so the timed run is about budget_s
+    n = int(min(len(tiles), max(8, n * budget_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    cells = one_step(tiles[:n])
+    dt = time.perf_counter() - t0
+    return {"value": cells / dt, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+            "sample": f"one RK4 step (4 stages: FD words, Christoffel words, FP64 RHS + update) of {n} of the "
+                      f"512 depth-3 tiles ({cells} zone-cycles) in {dt:.2f} s with {threads} OpenMP threads; "
+                      "ghost fill not included"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing pass")
+    ap.add_argument("--z-block", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU: the multi-GPU sharding of this path is a design only "
+              "(see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import adaptivemesh_amd as am
+    import torch
+
+    T = S + 2 * P
+    g = am.geometry(S, P, FD_ORDER, OUTER, CFL)
+    n_slots = 1 + 8 + 64 + 512
+    sim = am.BssnSim(g, n_slots, am.bssn_options(periodic=True, z_block=args.z_block))
+    assert build_mesh(sim) == n_slots
+    fields = initial_data(n_slots, T)
+    for v in range(24):
+        sim.set_field(0, v, fields[v])
+        sim.set_field(1, v, fields[v])
+    sim.fill_ghosts()
+    sim.sync()
+
+    for _ in range(args.warmup):
+        sim.timestep()
+    sim.sync()
+    torch.cuda.synchronize()
+    z0 = sim.zone_cycles
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.timestep()
+    sim.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    zones = sim.zone_cycles - z0
+    elapsed = t1 - t0
+    value = zones / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    bytes_per_zone = sum(BYTES[k] * STAGES_PER_STEP[k] for k in BYTES)
+    whole_gbs = zones * bytes_per_zone / elapsed / 1e9
+
+    roofline = None
+    kernels = {}
+    if not args.no_profile:
+        # per-launch HIP events on the driver's stream over one more timed step
+        sim.profile(True)
+        zp = sim.zone_cycles
+        sim.timestep()
+        sim.sync()
+        zp = sim.zone_cycles - zp
+        times = sim.kernel_times()
+        sim.profile(False)
+        for k, (ms, cnt) in times.items():
+            if cnt == 0:
+                continue
+            total_bytes = zp * BYTES[k] * STAGES_PER_STEP[k]
+            kernels[k] = {"ms_total": round(ms, 4), "launches": cnt, "avg_us": round(1e3 * ms / cnt, 2),
+                          "GBps": round(total_bytes / (ms * 1e-3) / 1e9, 1),
+                          "bytes_per_zone_cycle": round(BYTES[k] * STAGES_PER_STEP[k], 1)}
+        dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
+        d = kernels[dom]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(d["GBps"] / HBM_PEAK_GBS, 4), "traffic": None}
+
+    cons = sim.constraints()
+    result = {
+        "metric": METRIC, "value": round(value, 1), "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "BSSN single level: 512 tiles of 16^3 (128^3) + 73 ancestors, P=3, 24 fp32 fields, "
+                               "1+log + Gamma-driver, flat + 1e-6 noise, cf3 derivative/Christoffel words, RK4, "
+                               "FD order 4, periodic, [-32,32]^3, CFL 0.25",
+                   "tiles": n_slots, "zone_cycles_per_step": zones // max(args.steps, 1),
+                   "fine_rk4_steps_per_step": 2 ** DEPTH},
+        "hbm_whole_step": {"achieved": round(whole_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(whole_gbs / HBM_PEAK_GBS, 4), "bytes_per_zone_cycle": round(bytes_per_zone)},
+        "roofline": roofline,
+        "kernels": kernels,
+        "constraints_after": {k: float(f"{v:.6g}") for k, v in cons.items()},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        e = sim.export()
+        result["cpu_baseline"] = cpu_baseline(n_slots, fields, e["positions"], list(range(n_slots)))
+        result["cpu_baseline"]["value"] = round(result["cpu_baseline"]["value"], 1)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -96,6 +96,11 @@ int am_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
 int am_memset_async(void *dst, int value, size_t bytes, void *stream);
 int am_stream_sync(void *stream);
 int am_device_sync(void);
+/* HIP events, for timing kernels on the stream they run on */
+int am_event_create(void **ev);
+int am_event_record(void *ev, void *stream);
+int am_event_elapsed_ms(void *start, void *stop, float *ms); /* synchronises on stop */
+int am_event_destroy(void *ev);
 
 /* ------------------------------------------------------- FD stencils */
 
@@ -163,7 +168,9 @@ int am_ghost_fill_batched(const am_field_batch *b, const am_mesh *mesh, const am
 /* old <- new on every cell of each listed active tile; interior cells with
  * a child take the mean of the child's 2x2x2 new cells (restriction).
  * Replaces __copy_new_to_old AMR.h:961-996 and copy_new_to_old
- * AMR.h:998-1051.  Uses mesh->children. */
+ * AMR.h:998-1051.  Uses mesh->children.  In the batched form a non-NULL
+ * older[v] first receives the previous old (a third time level for
+ * quadratic time interpolation; an extension). */
 int am_copy_new_to_old(float *old_, const float *new_, const am_mesh *mesh, const am_geometry *g, void *stream);
 int am_copy_new_to_old_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream);
 
@@ -198,6 +205,58 @@ int am_deriv_trace(const uint32_t *hdiag, const uint32_t *hoff, const float *con
  * BSSN.h:93-102. */
 int am_christoffel(const float *const *metric6, const uint32_t *const *dmetric6, uint32_t *const *out6,
                    int32_t compat, const am_mesh *mesh, const am_geometry *g, int32_t seed, void *stream);
+
+/* Ghost fill at an intermediate Runge-Kutta time: faces whose target is
+ * coarser by dd levels interpolate the target's old/new at
+ * (k + stage_frac) / 2^dd of its step (relative_time, AMR.h:634-639, gives
+ * k and k + 1); equal-depth faces copy the target's new, which holds the same
+ * stage.  stage_frac = 1 is exactly am_ghost_fill_batched(copy_only_new=1).
+ * No counterpart in the reference, whose Heun scheme only fills at k + 1. */
+int am_ghost_fill_stage_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, float stage_frac,
+                                int32_t interp, void *stream);
+
+/* ----------------------------------------------------------- BSSN scheme
+ * No counterpart in the reference (BSSN.h has no right-hand side or time
+ * update, BSSN.h:143-159): the moving-puncture BSSN system with 24 fields
+ *   0 chi | 1-6 gt_ij | 7 K | 8-13 At_ij | 14-16 Gt^i | 17 alpha | 18-20 beta^i | 21-23 B^i
+ * (symmetric tensors in xx,xy,xz,yy,yz,zz order).  The Hessian words are for
+ * chi, gt_ij, alpha, beta^i in that order (11 fields); chr[jk] holds
+ * (Gt^x_jk, Gt^y_jk, Gt^z_jk) as written by am_christoffel(compat = 0). */
+#define AM_BSSN_NVARS 24
+#define AM_BSSN_NHESS 11
+
+typedef struct am_bssn_params {
+    float eta;            /* Gamma-driver damping */
+    float shift_coeff;    /* d_t beta^i = shift_coeff B^i (+ advection), usually 3/4 */
+    float chi_floor;      /* chi used in divisions is max(chi, chi_floor) */
+    int32_t advect_shift; /* 1: beta^k d_k terms in the beta and B equations */
+    int32_t slicing;      /* 0: 1+log, d_t alpha = -2 alpha K; 1: harmonic, -alpha^2 K */
+} am_bssn_params;
+
+typedef struct am_bssn_fields {
+    const float *state[AM_BSSN_NVARS]; /* stage input, padded tiles */
+    const float *old_[AM_BSSN_NVARS];  /* y_n at the start of the step */
+    float *new_[AM_BSSN_NVARS];        /* stage output (interior cells written) */
+    float *acc[AM_BSSN_NVARS];         /* RK4 accumulator (interior cells) */
+    const uint32_t *d1[AM_BSSN_NVARS];
+    const uint32_t *d2diag[AM_BSSN_NHESS];
+    const uint32_t *d2off[AM_BSSN_NHESS];
+    const uint32_t *chr[6];
+} am_bssn_fields;
+
+/* One classical RK4 stage on the interior of every listed active tile:
+ * k = f(state); stage 0: acc = k, new = old + dt/2 k; stage 1: acc += 2k,
+ * new = old + dt/2 k; stage 2: acc += 2k, new = old + dt k; stage 3:
+ * new = old + dt/6 (acc + k), with dt = outer_dt / 2^depth. */
+int am_bssn_rhs_stage(const am_bssn_fields *f, int32_t stage, const am_bssn_params *p, const am_mesh *mesh,
+                      const am_geometry *g, void *stream);
+
+/* Hamiltonian and momentum constraints of `state` on every listed tile,
+ * reduced in FP64 to one {sum H^2, sum M_i M_i, cells, max |H|} quadruple per
+ * workgroup in `partial` (device, 4 * am_bssn_constraint_blocks() doubles). */
+int am_bssn_constraints(const am_bssn_fields *f, const am_bssn_params *p, const am_mesh *mesh, const am_geometry *g,
+                        double *partial, void *stream);
+int64_t am_bssn_constraint_blocks(const am_geometry *g, int32_t n_tiles);
 
 #ifdef __cplusplus
 }

@@ -141,10 +141,11 @@ struct octree {
     bool periodic = false;
     void *stream = nullptr;  // stream used for octree uploads
 
-    // device slot lists: [0] every used slot, [1 + d] the slots at depth d
+    // device slot lists: [0] every used slot, [1 + d] the slots at depth d,
+    // [depth_limit + 2] the leaves (no child)
     smart_gpu_cpu_buffer<int32_t> tile_lists;
-    int32_t list_offset[depth_limit + 2] = {};
-    int32_t list_count[depth_limit + 2] = {};
+    int32_t list_offset[depth_limit + 3] = {};
+    int32_t list_count[depth_limit + 3] = {};
 
    protected:
     bool dirty = true;
@@ -156,7 +157,7 @@ struct octree {
     explicit octree(uint32_t node_slots, bool periodic_ = false)
         : parent_idx_b(node_slots), children_idx_b(node_slots * 8u), positions(node_slots),
           boundaries(node_slots * 6u), lifetime(node_slots), max_slots(node_slots), periodic(periodic_),
-          tile_lists(node_slots * 2u)
+          tile_lists(node_slots * 3u)
     {
         memset(parent_idx_b.cpu_buffer_ptr, -1, parent_idx_b.total_size());
         memset(children_idx_b.cpu_buffer_ptr, -1, children_idx_b.total_size());
@@ -307,6 +308,14 @@ struct octree {
                 if (positions.cpu_buffer_ptr[i].depth() == (int)d) L[n++] = (int32_t)i;
             list_count[d + 1] = n - list_offset[d + 1];
         }
+        list_offset[depth_limit + 2] = n;
+        for (uint32_t i = 0; i <= final_used_slot; i++) {
+            if (positions.cpu_buffer_ptr[i].depth() == -1) continue;
+            bool leaf = true;
+            for (int c = 0; c < 8; c++) leaf &= children_idx_b.cpu_buffer_ptr[i * 8u + c] == -1;
+            if (leaf) L[n++] = (int32_t)i;
+        }
+        list_count[depth_limit + 2] = n - list_offset[depth_limit + 2];
         parent_idx_b.copy_to_gpu(stream);
         children_idx_b.copy_to_gpu(stream);
         positions.copy_to_gpu(stream);
@@ -315,6 +324,7 @@ struct octree {
         dirty = false;
     }
     int tiles_at_depth(int d) const { return list_count[d + 1]; }
+    int leaf_count() const { return list_count[depth_limit + 2]; }
 
     // Kernel view over every used slot (the reference's launch domain) ...
     am_mesh mesh_all()
@@ -322,11 +332,17 @@ struct octree {
         copy_to_gpu();
         return make_mesh(tile_lists.gpu_buffer_ptr + list_offset[0], list_count[0]);
     }
-    // ... or over the slots of one depth.
+    // ... or over the slots of one depth ...
     am_mesh mesh_depth(int d)
     {
         copy_to_gpu();
         return make_mesh(tile_lists.gpu_buffer_ptr + list_offset[d + 1], list_count[d + 1]);
+    }
+    // ... or over the leaves.
+    am_mesh mesh_leaves()
+    {
+        copy_to_gpu();
+        return make_mesh(tile_lists.gpu_buffer_ptr + list_offset[depth_limit + 2], list_count[depth_limit + 2]);
     }
     am_mesh make_mesh(const int32_t *tiles, int32_t n) const
     {

@@ -54,6 +54,53 @@ int am_wave_sim_copy_bounds(am_wave_sim *s);
 int am_wave_sim_timestep(am_wave_sim *s);
 int32_t am_wave_sim_rng_state(const am_wave_sim *s);
 
+/* --------------------------------------------- BSSN RK4 evolution driver
+ * AMR<BSSN_rk4_data>: the 24-field moving-puncture BSSN system advanced by
+ * classical RK4 with sub-cycling (no counterpart in the reference, whose
+ * BSSN_AMR_data computes derivatives only, BSSN.h:104-171). */
+typedef struct am_bssn_sim am_bssn_sim;
+
+typedef struct am_bssn_sim_options {
+    am_bssn_params params;
+    int32_t interp;   /* AM_INTERP_LAGRANGE (reference) or AM_INTERP_TRICUBIC (+ quadratic time) */
+    int32_t z_block;  /* FD workgroup z extent (0 = whole tile) */
+    int32_t periodic; /* outer faces wrap instead of pointing at the root */
+    int32_t seed;     /* host fast_prng seed for the dither draws */
+} am_bssn_sim_options;
+
+am_bssn_sim *am_bssn_sim_create(const am_geometry *g, uint32_t max_slots, const am_bssn_sim_options *o);
+void am_bssn_sim_destroy(am_bssn_sim *s);
+int am_bssn_sim_add_node(am_bssn_sim *s, int32_t parent, int32_t offset);
+int am_bssn_sim_export(const am_bssn_sim *s, int32_t *parent, int32_t *children, uint32_t *positions,
+                       uint64_t *boundaries, uint32_t *lifetime, int32_t *scalars);
+/* which: 0 old, 1 new, 2 acc, 3 older; var 0..23; max_slots * T^3 floats */
+int am_bssn_sim_set_field(am_bssn_sim *s, int32_t which, int32_t var, const float *host);
+int am_bssn_sim_get_field(am_bssn_sim *s, int32_t which, int32_t var, float *host);
+/* device addresses: which 0-3 as above (var stride max_slots*T^3 floats),
+ * 4 d1, 5 d2diag, 6 d2off, 7 chr words (var stride max_slots*S^3) */
+void *am_bssn_sim_device_ptr(am_bssn_sim *s, int32_t which);
+int am_bssn_sim_fill_ghosts(am_bssn_sim *s);
+int am_bssn_sim_copy_back(am_bssn_sim *s);
+int am_bssn_sim_step_depth(am_bssn_sim *s, int32_t depth);
+/* one RK4 stage (0-3) of every tile at `depth`: FD and Christoffel words of
+ * the stage input, RHS + update, ghost fill at the stage time */
+int am_bssn_sim_stage_depth(am_bssn_sim *s, int32_t depth, int32_t stage);
+int am_bssn_sim_substep(am_bssn_sim *s);
+int am_bssn_sim_timestep(am_bssn_sim *s);
+/* {rms H, rms |M|, max |H|, cells} over the leaves (FP64) */
+int am_bssn_sim_constraints(am_bssn_sim *s, double *out4);
+int am_bssn_sim_sync(am_bssn_sim *s);
+uint64_t am_bssn_sim_zone_cycles(const am_bssn_sim *s);
+int32_t am_bssn_sim_rng_state(const am_bssn_sim *s);
+int am_bssn_sim_set_rng_state(am_bssn_sim *s, int32_t seed);
+/* the counter and max depth (timer_helper, read_max_depth) */
+int am_bssn_sim_counter(const am_bssn_sim *s, uint32_t *timer, int32_t *max_depth);
+/* per-launch HIP-event timing of the driver's kernels (fd, christoffel,
+ * rhs_stage, ghost, copy_back); kernel_times returns total ms and launch
+ * counts per class since profiling was (re)enabled */
+int am_bssn_sim_profile(am_bssn_sim *s, int32_t enable);
+int am_bssn_sim_kernel_times(am_bssn_sim *s, double *ms5, int64_t *count5);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,11 +43,14 @@
  * neighbour search on small trees.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
 #define EXPORT __attribute__((visibility("default")))
+
+EXPORT int orc_num_threads(void) { return omp_get_max_threads(); }
 
 /* ------------------------------------------------------------------ PRNG */
 
@@ -575,7 +578,7 @@ enum { INTERP_LAGRANGE = 0, INTERP_TRILINEAR = 1, INTERP_TRICUBIC = 2 };
  * of the coarse step; older may be NULL, which selects linear time. */
 EXPORT void orc_copy_boundaries(float *old_, float *new_, const float *older, const uint64_t *bnd,
                                 const int32_t *tiles, int n_tiles, uint32_t substep, uint32_t maxd,
-                                int copy_only_new, int mode, int S, int P)
+                                int copy_only_new, int mode, int S, int P, float frac)
 {
     const int T = S + 2 * P;
     const int64_t cells = (int64_t)T * T * T;
@@ -595,7 +598,10 @@ EXPORT void orc_copy_boundaries(float *old_, float *new_, const float *older, co
             const float rel_scale = 1.f / (float)(1u << ddiff);
             /* relative_time */
             const float tmp = (float)((substep >> (maxd - (uint32_t)cd)) & ((1u << ddiff) - 1u));
-            const float tx = tmp / (float)(1u << ddiff), ty = (tmp + 1.f) / (float)(1u << ddiff);
+            /* frac = 1 is the reference (k + 1); an intermediate Runge-Kutta time
+             * uses k + frac, and an equal-depth target holds the same stage */
+            const float tx = tmp / (float)(1u << ddiff),
+                        ty = ddiff == 0 ? 1.f : (tmp + frac) / (float)(1u << ddiff);
             for (int l = 0; l < P; l++)
                 for (int b = 0; b < S; b++)
                     for (int a = 0; a < S; a++) {

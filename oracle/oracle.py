@@ -19,6 +19,7 @@ _P = C.c_void_p
 _i32, _u32, _f32 = C.c_int32, C.c_uint32, C.c_float
 
 _SIGS = {
+    "orc_num_threads": (C.c_int, []),
     "orc_prng_int": (_i32, [C.POINTER(_i32)]),
     "orc_prng_float": (_f32, [C.POINTER(_i32)]),
     "orc_thread_seed": (_i32, [_i32, _u32, _u32, _u32]),
@@ -41,7 +42,8 @@ _SIGS = {
     "orc_tree_set_timer": (None, [_P, _u32]),
     "orc_differentiate": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, _i32, C.c_int, C.c_int, C.c_int, _f32]),
     "orc_mixed_partials": (None, [_P, _P, _P, _P, C.c_int, _u32, _u32, _i32, C.c_int, C.c_int, _f32]),
-    "orc_copy_boundaries": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_copy_boundaries": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   _f32]),
     "orc_copy_new_to_old": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
     "orc_copy_to_child": (None, [_P, _u32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_wave_stage": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, _f32]),
@@ -177,11 +179,11 @@ def mixed_partials(field, positions, tiles, substep, max_depth, seed, S, P, oute
     return off
 
 
-def copy_boundaries(old, new, older, boundaries, tiles, substep, max_depth, copy_only_new, mode, S, P):
+def copy_boundaries(old, new, older, boundaries, tiles, substep, max_depth, copy_only_new, mode, S, P, frac=1.0):
     """In place on old/new (float32 arrays)."""
     t = _i32arr(tiles)
     lib().orc_copy_boundaries(_p(old), _p(new), _p(older), _p(boundaries), _p(t), t.size, substep, max_depth,
-                              1 if copy_only_new else 0, mode, S, P)
+                              1 if copy_only_new else 0, mode, S, P, frac)
 
 
 def copy_new_to_old(old, new, positions, children, tiles, substep, max_depth, S, P):

@@ -1,0 +1,208 @@
+"""The BSSN RK4 driver on the GPU against the oracle, stage by stage.
+
+For every stage the oracle is given the GPU's stage input, and then:
+* the FD and Christoffel words must equal the oracle's bit for bit (same
+  seeds, drawn in the driver's order);
+* the fp32 RHS + RK4 update must match the FP64 oracle evaluated on the same
+  words within 5e-5 of the largest |k| of each field (fp32 round-off of a
+  ~1.5 kflop expression; the words themselves are identical);
+* the ghost cells written at the stage time must equal the oracle's fill of
+  the GPU's own new interior bit for bit.
+The constraint reduction (FP64) must match the oracle's sums over the same
+words to 1e-4.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests import fields as F
+from tests import meshes
+
+pytestmark = pytest.mark.gpu
+
+S, P, ORDER, OUTER, CFL = 16, 3, 4, 16.0, 0.25
+T = S + 2 * P
+HESS = (0, 1, 2, 3, 4, 5, 6, 17, 18, 19, 20)
+STAGE_FRAC = (0.5, 0.5, 1.0, 1.0)
+
+
+def smooth_state(e, n, seed=0, eps=0.04):
+    """Flat space plus smooth periodic perturbations on every field."""
+    rng = np.random.default_rng(seed)
+    flat = np.zeros(24)
+    flat[[0, 1, 4, 6, 17]] = 1.0
+    out = []
+    L = OUTER
+    for v in range(24):
+        k = rng.integers(1, 3, size=3)
+        ph = rng.uniform(0, 2 * np.pi, 3)
+        a = eps * rng.uniform(0.5, 1.0)
+        fn = (lambda x, y, z, v=v, k=k, ph=ph, a=a:
+              flat[v] + a * np.sin(2 * np.pi * k[0] * x / L + ph[0]) * np.cos(2 * np.pi * k[1] * y / L + ph[1])
+              + 0.5 * a * np.sin(2 * np.pi * k[2] * z / L + ph[2]))
+        out.append(F.sample(e["positions"], n, S, P, OUTER, fn))
+    return out
+
+
+def download(am, dev_ptr, count, dtype, offset_elems=0):
+    a = np.empty(count, dtype)
+    itemsize = np.dtype(dtype).itemsize
+    am.check(am.lib().am_memcpy_d2h(a.ctypes.data, dev_ptr + offset_elems * itemsize, count * itemsize, None), "d2h")
+    return a
+
+
+def sim_words(am, sim, n):
+    W = n * S ** 3
+    d1 = [download(am, sim.device_ptr(4), W, np.uint32, v * W) for v in range(24)]
+    d2d = [download(am, sim.device_ptr(5), W, np.uint32, h * W) for h in range(11)]
+    d2o = [download(am, sim.device_ptr(6), W, np.uint32, h * W) for h in range(11)]
+    chw = [download(am, sim.device_ptr(7), W, np.uint32, c * W) for c in range(6)]
+    return d1, d2d, d2o, chw
+
+
+def tile_mask(n, tiles, inner=True):
+    m = np.zeros((n, T, T, T) if not inner else (n, S, S, S), bool)
+    m[tiles] = True
+    return m.reshape(-1)
+
+
+def interior_mask(n, tiles):
+    m = np.zeros((n, T, T, T), bool)
+    for t in tiles:
+        m[t, P:P + S, P:P + S, P:P + S] = True
+    return m.reshape(-1)
+
+
+@pytest.mark.parametrize("recipe,periodic,depth", [("uniform1", True, 1), ("graded", False, 2),
+                                                   ("graded", False, 3)])
+def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth):
+    print(f"bssn stages {recipe} periodic={periodic} depth={depth}", flush=True)
+    ops, n = {"uniform1": meshes.uniform(1), "graded": meshes.graded()}[recipe]
+    g = am.geometry(S, P, ORDER, OUTER, CFL)
+    sim = am.BssnSim(g, n, am.bssn_options(periodic=periodic))
+    ot = orc.OracleTree(n, periodic)
+    for p_, c in ops:
+        assert sim.add_node(p_, c) == ot.add_node(p_, c)
+    e = ot.export()
+    state0 = smooth_state(e, n, seed=depth)
+    for v in range(24):
+        sim.set_field(0, v, state0[v])
+        sim.set_field(1, v, state0[v])
+    tiles = [s for s in F.used_slots(e) if F.depth_of(e, s) == depth]
+    assert tiles
+    outer_dx = float(np.float32(OUTER) / np.float32(S))
+    dtd = np.array([float(np.float32(np.float32(outer_dx) * np.float32(CFL)) / np.float32(2 ** d)) for d in range(9)])
+    params = np.array(orc.BSSN_DEFAULT, np.float64)
+    imask = interior_mask(n, tiles)
+    wmask = tile_mask(n, tiles)
+    old = [x.copy() for x in state0]
+    for stage in range(4):
+        state_in = old if stage == 0 else [sim.get_field(1, v) for v in range(24)]
+        acc_in = [sim.get_field(2, v).astype(np.float64) for v in range(24)]
+        seed = sim.rng_state
+        sim.stage_depth(depth, stage)
+        sim.sync()
+        d1, d2d, d2o, chw = sim_words(am, sim, n)
+        # words: bit-exact with the oracle on the same input and seeds
+        s_ = seed
+        h = 0
+        for v in range(24):
+            sd = orc.prng_ints(s_, 1)[0]
+            s_ = sd
+            hess = v in HESS
+            w1, w2 = orc.differentiate(state_in[v], e["positions"], tiles, 0, e["max_depth"], sd, S, P, ORDER // 2,
+                                       outer_dx, n, with_d2=hess)
+            assert np.array_equal(d1[v][wmask], w1[wmask]), (stage, v)
+            if hess:
+                sm = orc.prng_ints(s_, 1)[0]
+                s_ = sm
+                wo = orc.mixed_partials(state_in[v], e["positions"], tiles, 0, e["max_depth"], sm, S, P, outer_dx, n)
+                assert np.array_equal(d2d[h][wmask], w2[wmask]), (stage, v)
+                assert np.array_equal(d2o[h][wmask], wo[wmask]), (stage, v)
+                h += 1
+        sc = orc.prng_ints(s_, 1)[0]
+        ch = orc.christoffel(state_in[1:7], d1[1:7], 0, e["positions"], tiles, 0, e["max_depth"], sc, S, P, n)
+        for c in range(6):
+            assert np.array_equal(chw[c][wmask], ch[c][wmask]), (stage, c)
+        assert sim.rng_state == sc
+        # RHS + update in FP64 on the GPU's words
+        acc_o = [np.zeros(n * T ** 3) for _ in range(24)]
+        new_o = [np.zeros(n * T ** 3) for _ in range(24)]
+        sin32 = [np.ascontiguousarray(x, np.float32) for x in state_in]
+        orc.lib().orc_bssn_stage_from_words(orc.ptrs(sin32), orc.ptrs(old), orc.ptrs(acc_in), orc.ptrs(acc_o),
+                                            orc.ptrs(new_o), orc.ptrs(d1), orc.ptrs(d2d), orc.ptrs(d2o),
+                                            orc.ptrs(chw), e["positions"].ctypes.data,
+                                            np.array(tiles, np.int32).ctypes.data, len(tiles), 0, e["max_depth"],
+                                            stage, dtd.ctypes.data, params.ctypes.data, S, P)
+        new_g = [sim.get_field(1, v) for v in range(24)]
+        acc_g = [sim.get_field(2, v) for v in range(24)]
+        worst = 0.0
+        for v in range(24):
+            if stage < 3:
+                k_o = (acc_o[v] - acc_in[v])[imask] / (1 if stage == 0 else 2)
+                k_g = (acc_g[v].astype(np.float64) - acc_in[v])[imask] / (1 if stage == 0 else 2)
+                scale = max(np.max(np.abs(k_o)), 1e-3)
+                err = np.max(np.abs(k_g - k_o)) / scale
+                worst = max(worst, err)
+                assert err < 5e-5, (stage, v, err)
+            dn = np.max(np.abs(new_g[v][imask] - new_o[v][imask]))
+            assert dn < 1e-6 * max(1.0, np.max(np.abs(new_o[v][imask]))), (stage, v, dn)
+        print(f"  stage {stage}: worst relative k error {worst:.2e}", flush=True)
+        # ghost cells of the stage output
+        if depth > 0:
+            ref = [x.copy() for x in new_g]
+            for v in range(24):
+                orc.copy_boundaries(old[v].copy(), ref[v], None, e["boundaries"], tiles, 0, e["max_depth"], True, 0,
+                                    S, P, STAGE_FRAC[stage])
+                assert np.array_equal(ref[v].view(np.uint32), new_g[v].view(np.uint32)), (stage, v)
+
+
+def test_bssn_constraints_match_oracle(am, orc, torch_cuda):
+    ops, n = meshes.uniform(1)
+    g = am.geometry(S, P, ORDER, OUTER, CFL)
+    sim = am.BssnSim(g, n, am.bssn_options(periodic=True))
+    ot = orc.OracleTree(n, True)
+    for p_, c in ops:
+        assert sim.add_node(p_, c) == ot.add_node(p_, c)
+    e = ot.export()
+    st = smooth_state(e, n, seed=9)
+    for v in range(24):
+        sim.set_field(0, v, st[v])
+        sim.set_field(1, v, st[v])
+    cons = sim.constraints()
+    d1, d2d, d2o, chw = sim_words(am, sim, n)
+    leaves = np.array(list(range(1, 9)), np.int32)
+    out = np.zeros(4 * len(leaves))
+    params = np.array(orc.BSSN_DEFAULT, np.float64)
+    orc.lib().orc_bssn_constraints_from_words(orc.ptrs(st), orc.ptrs(d1), orc.ptrs(d2d), orc.ptrs(d2o), orc.ptrs(chw),
+                                              leaves.ctypes.data, len(leaves), params.ctypes.data, S, P,
+                                              out.ctypes.data)
+    q = out.reshape(-1, 4)
+    h_l2, m_l2 = np.sqrt(q[:, 0].sum() / q[:, 2].sum()), np.sqrt(q[:, 1].sum() / q[:, 2].sum())
+    assert cons["cells"] == q[:, 2].sum()
+    assert abs(cons["H_l2"] - h_l2) <= 1e-4 * h_l2, (cons, h_l2)
+    assert abs(cons["M_l2"] - m_l2) <= 1e-4 * m_l2, (cons, m_l2)
+    assert abs(cons["H_max"] - q[:, 3].max()) <= 1e-4 * q[:, 3].max()
+
+
+def test_bssn_flat_space_stays_flat(am, torch_cuda):
+    """Exactly flat data: every RHS term is zero, so a full timestep leaves
+    every field bit-identical."""
+    ops, n = meshes.uniform(1)
+    g = am.geometry(S, P, ORDER, OUTER, CFL)
+    sim = am.BssnSim(g, n, am.bssn_options(periodic=True))
+    meshes.apply(sim, ops)
+    flat = np.zeros(24, np.float32)
+    flat[[0, 1, 4, 6, 17]] = 1.0
+    for v in range(24):
+        a = np.full(n * T ** 3, flat[v], np.float32)
+        sim.set_field(0, v, a)
+        sim.set_field(1, v, a)
+    sim.timestep()
+    sim.sync()
+    for v in range(24):
+        got = sim.get_field(1, v).reshape(n, T, T, T)[:, P:P + S, P:P + S, P:P + S]
+        assert np.all(got == flat[v]), v
+    c = sim.constraints()
+    assert c["H_l2"] == 0 and c["M_l2"] == 0
