@@ -187,8 +187,10 @@ def test_bssn_constraints_match_oracle(am, orc, torch_cuda):
 
 
 def test_bssn_flat_space_stays_flat(am, torch_cuda):
-    """Exactly flat data: every RHS term is zero, so a full timestep leaves
-    every field bit-identical."""
+    """Flat data stays flat to round-off.  Not bit-identically: the
+    reference's float second-derivative coefficients (AMR.h:1148-1150) do
+    not sum to exactly zero, so d^2 of a constant metric component is a few
+    1e-7 inv_dx^2 rather than 0, and that feeds the Ricci tensor."""
     ops, n = meshes.uniform(1)
     g = am.geometry(S, P, ORDER, OUTER, CFL)
     sim = am.BssnSim(g, n, am.bssn_options(periodic=True))
@@ -199,10 +201,11 @@ def test_bssn_flat_space_stays_flat(am, torch_cuda):
         a = np.full(n * T ** 3, flat[v], np.float32)
         sim.set_field(0, v, a)
         sim.set_field(1, v, a)
-    sim.timestep()
+    for _ in range(2):
+        sim.timestep()
     sim.sync()
     for v in range(24):
         got = sim.get_field(1, v).reshape(n, T, T, T)[:, P:P + S, P:P + S, P:P + S]
-        assert np.all(got == flat[v]), v
+        assert np.max(np.abs(got - flat[v])) < 1e-5, v
     c = sim.constraints()
-    assert c["H_l2"] == 0 and c["M_l2"] == 0
+    assert c["H_l2"] < 1e-4 and c["M_l2"] < 1e-4, c
