@@ -72,21 +72,41 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     const int z0 = zb * ZB;
     const int c = (y + P) * T + (x + P);  // centre offset within a plane
 
-    auto load_plane = [&](int zp) {  // padded plane index zp -> ring slot zp % R
+    // Planes are prefetched one iteration ahead into registers (NL float4
+    // per thread), so the HBM latency of plane z+2P+2 overlaps the compute
+    // of plane z.  At iteration z the ring holds padded planes z..z+2P; the
+    // slot of plane z+2P+1 is (z-1) % R, which nobody reads in iteration z,
+    // so one barrier per iteration suffices.
+    constexpr int NL = (T2 / 4 + S * S - 1) / (S * S);
+    float4 pre[NL];
+    auto gload = [&](int zp) {
         const float4 *s4 = reinterpret_cast<const float4 *>(src + (size_t)zp * T2);
-        float4 *d4 = ring4[zp % R];
-        for (int i = tid; i < T2 / 4; i += S * S) d4[i] = s4[i];
+#pragma unroll
+        for (int j = 0; j < NL; j++) {
+            const int i = tid + j * S * S;
+            if (i < T2 / 4) pre[j] = s4[i];
+        }
     };
-    // planes z0 .. z0+2P-1 (padded indices) precede the first output plane
-    for (int p = 0; p < 2 * P; p++) load_plane(z0 + p);
+    auto lstore = [&](int zp) {
+        float4 *d4 = ring4[zp % R];
+#pragma unroll
+        for (int j = 0; j < NL; j++) {
+            const int i = tid + j * S * S;
+            if (i < T2 / 4) d4[i] = pre[j];
+        }
+    };
+    for (int p = 0; p <= 2 * P; p++) {
+        gload(z0 + p);
+        lstore(z0 + p);
+    }
+    if (ZB > 1) gload(z0 + 2 * P + 1);
+    __syncthreads();
 
     const int ex = x - min(max(x, 1), S - 2);
     const int ey = y - min(max(y, 1), S - 2);
     const int cxy = ex * ey;
 
     for (int z = z0; z < z0 + ZB; z++) {
-        load_plane(z + 2 * P);
-        __syncthreads();
         // padded plane of output z is z+P; plane z+P+i sits in slot (z+P+i) % R
         const float *pl[2 * P + 1];
 #pragma unroll
@@ -141,6 +161,11 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
             int32_t rng = thread_seed(gs_m, (uint32_t)x, (uint32_t)y, gz);
             o_off[o] = cf3_pack(vxy * dx2, vxz * dx2, vyz * dx2, rng);
         }
+        if (z + 1 < z0 + ZB) {
+            lstore(z + 2 * P + 1);
+            if (z + 2 < z0 + ZB) gload(z + 2 * P + 2);
+        }
+        __syncthreads();
     }
 }
 

@@ -187,6 +187,176 @@ __global__ void __launch_bounds__(S *S) k_ghost_fill(FieldBatchDev b, const uint
     }
 }
 
+// Ghost fill for many fields at once (the RK4 driver's stage fills).  One
+// workgroup per (tile, face, group of VPG fields); each thread owns ghost
+// cells of the face slab, laid out so consecutive lanes follow the slab's
+// contiguous axis, computes the index map of AMR.h:714-753 once per cell and
+// applies it to every field of the group.  Equal-depth cells whose read
+// position falls exactly on a target cell are copies: the quadratic formula
+// there is f plus terms that are all +-0, and the time weight on `new` is 1,
+// so the copy is the formula's value for finite data.  Everything else runs
+// the full formula of k_ghost_fill.
+template <int S, int P, int MODE, int VPG>
+__global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n_vars, const uint64_t *__restrict__ bnd,
+                                                          const int32_t *__restrict__ tiles, uint32_t substep,
+                                                          uint32_t maxd, float frac)
+{
+    constexpr int T = S + 2 * P, NC = S * S * P;
+    constexpr size_t CELLS = (size_t)T * T * T;
+    const int ti = blockIdx.x / 6, dir = blockIdx.x - ti * 6;
+    const int v0 = blockIdx.y * VPG;
+    const int node = tiles[ti];
+    const uint64_t bb = bnd[(size_t)node * 6 + dir];
+    const uint32_t nd = (uint32_t)bb;
+    const int tgt = bnd_target(nd), cd = bnd_depth(nd);
+    if (tgt == -1 || cd <= 0) return;
+    if (!active_depth(substep, (uint32_t)cd, maxd)) return;
+    const float3 relp = cf3_unpack((uint32_t)(bb >> 32));
+    const uint32_t ddiff = bnd_ddiff(nd);
+    const float rel_scale = 1.f / (float)(1u << ddiff);
+    const float tmp = (float)((substep >> (maxd - (uint32_t)cd)) & ((1u << ddiff) - 1u));
+    const float ty = ddiff == 0 ? 1.f : (tmp + frac) / (float)(1u << ddiff);
+    constexpr uint32_t uP = P, uS = S;
+    const float invS = 1.f / (float)S;
+    const int nv = min(VPG, n_vars - v0);
+
+    for (int c = threadIdx.x; c < NC; c += 256) {
+        uint32_t w0, w1, w2;
+        if (dir == 0 || dir == 3) {  // x faces: x fastest
+            const uint32_t l = c % P, a = (c / P) % S, bq = c / (P * S);
+            w0 = dir == 0 ? l + uP + uS : l; w1 = a + uP; w2 = bq + uP;
+        } else if (dir == 1 || dir == 4) {
+            const uint32_t a = c % S, l = (c / S) % P, bq = c / (S * P);
+            w0 = a + uP; w1 = dir == 1 ? l + uP + uS : l; w2 = bq + uP;
+        } else {
+            const uint32_t a = c % S, bq = (c / S) % S, l = c / (S * S);
+            w0 = a + uP; w1 = bq + uP; w2 = dir == 2 ? l + uP + uS : l;
+        }
+        float rp[3] = {(float)w0, (float)w1, (float)w2};
+        const float rl[3] = {relp.x, relp.y, relp.z};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            rp[k] = (rp[k] - (float)uP + .5f) * invS - .5f;
+            rp[k] = rp[k] * rel_scale + rl[k];
+            rp[k] = (rp[k] + .5f) * (float)uS + (float)uP - 0.5f;
+        }
+        uint32_t ri[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (MODE == AM_INTERP_TRILINEAR)
+                ri[k] = (tgt == 0) ? min((uint32_t)fmaxf(rp[k], .5f), (uint32_t)T - 2u)
+                                   : min((uint32_t)fmaxf(rp[k], .5f + (float)uP), (uint32_t)T - uP - 2u);
+            else if (MODE == AM_INTERP_TRICUBIC)
+                ri[k] = (tgt == 0) ? min((uint32_t)fmaxf(rp[k], 1.f), (uint32_t)T - 3u)
+                                   : min((uint32_t)fmaxf(rp[k], (float)(uP + 1u)), (uint32_t)T - uP - 3u);
+            else
+                ri[k] = (tgt == 0) ? min((uint32_t)(fmaxf(rp[k], 0.f) + .5f), (uint32_t)T - 2u)
+                                   : min((uint32_t)(fmaxf(rp[k], (float)uP) + .5f), (uint32_t)T - uP - 2u);
+        }
+        const size_t rn = (size_t)tgt * CELLS + ri[0] + ((size_t)ri[1] + (size_t)ri[2] * T) * T;
+#pragma unroll
+        for (int k = 0; k < 3; k++) rp[k] -= (float)ri[k];
+        const size_t wn = (size_t)node * CELLS + w0 + ((size_t)w1 + (size_t)w2 * T) * T;
+        if (ddiff == 0 && MODE == AM_INTERP_LAGRANGE) {
+            // Equal depth: every offset is exactly 0, or 1 where the read index
+            // was clamped, and the time weights are (0, 1).  Terms with a zero
+            // weight are +-0 and old enters only as old * 0, so evaluating the
+            // remaining terms in the formula's order gives its value.
+            const bool cx = rp[0] != 0.f, cy = rp[1] != 0.f, cz = rp[2] != 0.f;
+            for (int v = 0; v < nv; v++) {
+                const float *d = b.new_[v0 + v];
+                const float f = d[rn];
+                float r = f;
+                float fx = 0.f, fy = 0.f, fz = 0.f;  // raw +1 neighbours where needed
+                if (cx) {
+                    fx = d[rn + 1];
+                    float gx = (fx - d[rn - 1]) * .5f;
+                    const float gxx = (gx + d[rn - 1]) - f;
+                    r = r + (gx + gxx * rp[0]) * rp[0];
+                }
+                if (cy) {
+                    fy = d[rn + T];
+                    float gy = (fy - d[rn - T]) * .5f;
+                    const float gyy = (gy + d[rn - T]) - f;
+                    r = r + (gy + gyy * rp[1]) * rp[1];
+                }
+                if (cz) {
+                    fz = d[rn + T * T];
+                    float gz = (fz - d[rn - T * T]) * .5f;
+                    const float gzz = (gz + d[rn - T * T]) - f;
+                    r = r + (gz + gzz * rp[2]) * rp[2];
+                }
+                if (cx && cy) r = r + (d[rn + T + 1] + ((f - fx) - fy)) * rp[0] * rp[1];
+                if (cx && cz) r = r + (d[rn + T * T + 1] + ((f - fx) - fz)) * rp[0] * rp[2];
+                if (cy && cz) r = r + (d[rn + T * T + T] + ((f - fy) - fz)) * rp[1] * rp[2];
+                b.new_[v0 + v][wn] = r;
+            }
+            continue;
+        }
+        if (ddiff == 0 && MODE == AM_INTERP_TRICUBIC) {
+            // equal depth: integer offsets, so the 4-point weights are a unit
+            // vector and the value is the target cell at that offset
+            const size_t src = rn + (ptrdiff_t)rp[0] + (ptrdiff_t)rp[1] * T + (ptrdiff_t)rp[2] * (T * T);
+            for (int v = 0; v < nv; v++) b.new_[v0 + v][wn] = b.new_[v0 + v][src];
+            continue;
+        }
+        float wx[4], wy[4], wz[4];
+        if (MODE == AM_INTERP_TRICUBIC) {
+            lag4(rp[0], wx); lag4(rp[1], wy); lag4(rp[2], wz);
+        }
+        for (int v = 0; v < nv; v++) {
+            const float *dold = b.old_[v0 + v], *dnew = b.new_[v0 + v], *dolder = b.older[v0 + v];
+            float o, nw;
+            if (MODE == AM_INTERP_TRICUBIC) {
+                o = interp_tricubic<T>(dold, wx, wy, wz, rn);
+                nw = interp_tricubic<T>(dnew, wx, wy, wz, rn);
+                if (dolder && ddiff > 0) {
+                    const float od = interp_tricubic<T>(dolder, wx, wy, wz, rn);
+                    const float s_ = ty;
+                    const float c0 = .5f * s_ * (s_ - 1.f), c1 = (1.f - s_) * (1.f + s_), c2 = .5f * s_ * (s_ + 1.f);
+                    b.new_[v0 + v][wn] = od * c0 + o * c1 + nw * c2;
+                    continue;
+                }
+            } else if (MODE == AM_INTERP_TRILINEAR) {
+                o = interp_trilinear<T>(dold, rp[0], rp[1], rp[2], rn);
+                nw = interp_trilinear<T>(dnew, rp[0], rp[1], rp[2], rn);
+            } else {
+                o = interp_lagrange<T>(dold, rp[0], rp[1], rp[2], rn);
+                nw = interp_lagrange<T>(dnew, rp[0], rp[1], rp[2], rn);
+            }
+            b.new_[v0 + v][wn] = o * (1.f - ty) + nw * ty;
+        }
+    }
+}
+
+constexpr int GHOST_VPG = 6;  // fields per workgroup in the multi-field fill
+
+template <int S, int P>
+static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, int interp, float frac, void *stream)
+{
+    FieldBatchDev d;
+    for (int v = 0; v < b->n_vars; v++) {
+        if (!b->old_[v] || !b->new_[v]) return set_error(AM_ERR_INVALID, "am_ghost_fill: null field %d", v);
+        d.old_[v] = b->old_[v];
+        d.new_[v] = b->new_[v];
+        d.older[v] = b->older[v];
+    }
+    dim3 grid(mesh->n_tiles * 6, (b->n_vars + GHOST_VPG - 1) / GHOST_VPG);
+    hipStream_t st = (hipStream_t)stream;
+    if (interp == AM_INTERP_LAGRANGE)
+        hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_LAGRANGE, GHOST_VPG>), grid, dim3(256), 0, st, d,
+                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac);
+    else if (interp == AM_INTERP_TRILINEAR)
+        hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRILINEAR, GHOST_VPG>), grid, dim3(256), 0, st, d,
+                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac);
+    else if (interp == AM_INTERP_TRICUBIC)
+        hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRICUBIC, GHOST_VPG>), grid, dim3(256), 0, st, d,
+                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac);
+    else
+        return set_error(AM_ERR_INVALID, "am_ghost_fill: unknown interpolation mode %d", interp);
+    return check_launch("k_ghost_fill_multi");
+}
+
 template <int S, int P>
 static int launch_ghost(const am_field_batch *b, const am_mesh *mesh, int copy_only_new, int interp, float frac,
                         void *stream)
@@ -214,7 +384,11 @@ static int launch_ghost(const am_field_batch *b, const am_mesh *mesh, int copy_o
     return check_launch("k_ghost_fill");
 }
 
-// __copy_new_to_old (AMR.h:961-996)
+// __copy_new_to_old (AMR.h:961-996).  One workgroup per (tile, field)
+// streams the whole padded tile: planes outside the interior, and every
+// plane of a tile without children, are plain 16-byte-vector copies; the
+// interior planes of a refined tile take the child means cell by cell.
+// With `older` set, the previous old is kept there first.
 template <int S, int P>
 __global__ void __launch_bounds__(256) k_copy_new_to_old(FieldBatchDev b, const uint32_t *__restrict__ positions,
                                                          const int32_t *__restrict__ children,
@@ -223,47 +397,66 @@ __global__ void __launch_bounds__(256) k_copy_new_to_old(FieldBatchDev b, const 
 {
     constexpr int T = S + 2 * P, T2 = T * T;
     constexpr size_t CELLS = (size_t)T * T2;
-    const int ti = blockIdx.x / T, z = blockIdx.x - ti * T;
-    const int node = tiles[ti];
+    const int node = tiles[blockIdx.x];
     const int depth = pos_depth(positions[node]);
     if (depth == -1 || !active_depth(substep, (uint32_t)depth, maxd)) return;
     float *__restrict__ dold = b.old_[blockIdx.y];
     const float *__restrict__ dnew = b.new_[blockIdx.y];
     float *__restrict__ dolder = const_cast<float *>(b.older[blockIdx.y]);
+    __shared__ int ch[8];
+    if (threadIdx.x < 8) ch[threadIdx.x] = children[(size_t)node * 8 + threadIdx.x];
+    __syncthreads();
+    bool any_child = false;
+#pragma unroll
+    for (int i = 0; i < 8; i++) any_child |= ch[i] != -1;
+    const size_t base = (size_t)node * CELLS;
     constexpr uint32_t H = S / 2u;
-    const uint32_t tz = ((uint32_t)z - (uint32_t)P) / H;
-    for (int i = threadIdx.x; i < T2; i += 256) {
-        const int y = i / T, x = i - y * T;
-        const uint32_t tx = ((uint32_t)x - (uint32_t)P) / H, ty = ((uint32_t)y - (uint32_t)P) / H;
-        const int child = (tx < 2u && ty < 2u && tz < 2u) ? children[tx + ty * 2u + tz * 4u + (uint32_t)node * 8u] : -1;
-        const size_t o = (size_t)node * CELLS + (size_t)z * T2 + i;
-        if (dolder) dolder[o] = dold[o];
-        if (child == -1) {
-            dold[o] = dnew[o];
+    for (int z = 0; z < T; z++) {
+        const uint32_t tz = ((uint32_t)z - (uint32_t)P) / H;
+        if (!any_child || tz >= 2u) {
+            const float4 *s4 = reinterpret_cast<const float4 *>(dnew + base + (size_t)z * T2);
+            float4 *d4 = reinterpret_cast<float4 *>(dold + base + (size_t)z * T2);
+            float4 *o4 = dolder ? reinterpret_cast<float4 *>(dolder + base + (size_t)z * T2) : nullptr;
+            for (int i = threadIdx.x; i < T2 / 4; i += 256) {
+                if (o4) o4[i] = d4[i];
+                d4[i] = s4[i];
+            }
             continue;
         }
-        const size_t cx = (((uint32_t)x - (uint32_t)P) * 2u) % (uint32_t)S + (uint32_t)P;
-        const size_t cy = (((uint32_t)y - (uint32_t)P) * 2u) % (uint32_t)S + (uint32_t)P;
-        const size_t cz = (((uint32_t)z - (uint32_t)P) * 2u) % (uint32_t)S + (uint32_t)P;
-        const size_t c = (size_t)child * CELLS + (cz * T + cy) * T + cx;
-        dold[o] = (dnew[c] + dnew[c + 1] + dnew[c + T] + dnew[c + T + 1] + dnew[c + T2] + dnew[c + T2 + 1] +
-                   dnew[c + T2 + T] + dnew[c + T2 + T + 1]) *
-                  .125f;
+        for (int i = threadIdx.x; i < T2; i += 256) {
+            const int y = i / T, x = i - y * T;
+            const uint32_t tx = ((uint32_t)x - (uint32_t)P) / H, ty = ((uint32_t)y - (uint32_t)P) / H;
+            const int child = (tx < 2u && ty < 2u) ? ch[tx + ty * 2u + tz * 4u] : -1;
+            const size_t o = base + (size_t)z * T2 + i;
+            if (dolder) dolder[o] = dold[o];
+            if (child == -1) {
+                dold[o] = dnew[o];
+                continue;
+            }
+            const size_t cx = (((uint32_t)x - (uint32_t)P) * 2u) % (uint32_t)S + (uint32_t)P;
+            const size_t cy = (((uint32_t)y - (uint32_t)P) * 2u) % (uint32_t)S + (uint32_t)P;
+            const size_t cz = (((uint32_t)z - (uint32_t)P) * 2u) % (uint32_t)S + (uint32_t)P;
+            const size_t c = (size_t)child * CELLS + (cz * T + cy) * T + cx;
+            dold[o] = (dnew[c] + dnew[c + 1] + dnew[c + T] + dnew[c + T + 1] + dnew[c + T2] + dnew[c + T2 + 1] +
+                       dnew[c + T2 + T] + dnew[c + T2 + T + 1]) *
+                      .125f;
+        }
     }
 }
 
 template <int S, int P>
 static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, void *stream)
 {
-    constexpr int T = S + 2 * P;
     FieldBatchDev d;
     for (int v = 0; v < b->n_vars; v++) {
         if (!b->old_[v] || !b->new_[v]) return set_error(AM_ERR_INVALID, "am_copy_new_to_old: null field %d", v);
+        if (!aligned16(b->old_[v]) || !aligned16(b->new_[v]) || (b->older[v] && !aligned16(b->older[v])))
+            return set_error(AM_ERR_MISALIGNED, "am_copy_new_to_old: field %d not 16-byte aligned", v);
         d.old_[v] = b->old_[v];
         d.new_[v] = b->new_[v];
         d.older[v] = b->older[v];
     }
-    dim3 grid(mesh->n_tiles * T, b->n_vars);
+    dim3 grid(mesh->n_tiles, b->n_vars);
     hipLaunchKernelGGL((k_copy_new_to_old<S, P>), grid, dim3(256), 0, (hipStream_t)stream, d, mesh->positions,
                        mesh->children, mesh->tiles, mesh->substep, mesh->max_depth);
     return check_launch("k_copy_new_to_old");
@@ -345,7 +538,7 @@ extern "C" int am_ghost_fill_stage_batched(const am_field_batch *b, const am_mes
         return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_batched: stage_frac %g", (double)stage_frac);
     if (int rc = am::mesh_checks(mesh, g, true, false)) return rc;
     if (b->n_vars == 0 || mesh->n_tiles == 0) return AM_OK;
-    AM_DISPATCH_TILE(*g, am::launch_ghost, b, mesh, 1, interp, stage_frac, stream);
+    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, interp, stage_frac, stream);
 }
 
 extern "C" int am_ghost_fill(float *old_, float *new_, const float *older, const am_mesh *mesh, const am_geometry *g,
