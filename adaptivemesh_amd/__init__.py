@@ -20,6 +20,7 @@ LIB_PATH = HERE / "libadaptivemesh_amd.so"
 
 AM_MAX_VARS = 32
 AM_INTERP_LAGRANGE, AM_INTERP_TRILINEAR, AM_INTERP_TRICUBIC = 0, 1, 2
+AM_EQUAL_FORMULA, AM_EQUAL_COPY, AM_EQUAL_SKIP = 0, 1, 2
 
 ERRORS = {
     -1: "AM_ERR_INVALID", -2: "AM_ERR_UNSUPPORTED_GEOMETRY", -3: "AM_ERR_SLOTS_EXHAUSTED",
@@ -141,9 +142,9 @@ _SIGS = {
     "am_wave_sim_timestep": (C.c_int, [_P]),
     "am_wave_sim_rng_state": (C.c_int32, [_P]),
     "am_ghost_fill_stage_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), C.c_float,
-                                              C.c_int32, _P]),
+                                              C.c_int32, C.c_int32, _P]),
     "am_bssn_rhs_stage": (C.c_int, [C.POINTER(BssnFields), C.c_int32, C.POINTER(BssnParams), C.POINTER(Mesh),
-                                    C.POINTER(Geometry), _P]),
+                                    C.POINTER(Geometry), C.c_int32, _P]),
     "am_bssn_constraints": (C.c_int, [C.POINTER(BssnFields), C.POINTER(BssnParams), C.POINTER(Mesh),
                                       C.POINTER(Geometry), _P, _P]),
     "am_bssn_constraint_blocks": (C.c_int64, [C.POINTER(Geometry), C.c_int32]),

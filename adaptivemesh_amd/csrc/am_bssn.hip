@@ -286,10 +286,11 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, size_t r, size_t
 struct TileArgs {
     const uint32_t *positions;
     const int32_t *tiles;
+    const uint64_t *boundaries;
     uint32_t substep, max_depth;
 };
 
-template <int S, int P, int STAGE>
+template <int S, int P, int STAGE, bool PUSH>
 __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, BssnParDev par, DepthScales sc)
 {
 #pragma clang fp contract(fast)
@@ -309,14 +310,37 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
 #pragma unroll
     for (int v = 0; v < NV; v++) {
         const float yn = STAGE == 0 ? a.state[v][r] : a.old_[v][r];
+        float nv;
         if (STAGE == 0) {
             a.acc[v][r] = k[v];
-            a.new_[v][r] = yn + (.5f * dt) * k[v];
+            nv = yn + (.5f * dt) * k[v];
         } else if (STAGE == 1 || STAGE == 2) {
             a.acc[v][r] = a.acc[v][r] + 2.f * k[v];
-            a.new_[v][r] = yn + (STAGE == 1 ? .5f * dt : dt) * k[v];
+            nv = yn + (STAGE == 1 ? .5f * dt : dt) * k[v];
         } else {
-            a.new_[v][r] = yn + (dt * (1.f / 6.f)) * (a.acc[v][r] + k[v]);
+            nv = yn + (dt * (1.f / 6.f)) * (a.acc[v][r] + k[v]);
+        }
+        a.new_[v][r] = nv;
+        k[v] = nv;
+    }
+    if (PUSH) {
+        // copy into the face ghosts of equal-depth neighbours: a cell within P
+        // of face f lands in the opposite face slab of f's target, shifted by
+        // S cells along f's axis
+        const int c3[3] = {x, y, z};
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            const int ax = f % 3;
+            const bool near = f < 3 ? c3[ax] >= S - P : c3[ax] < P;
+            if (!near) continue;
+            const uint32_t nd = (uint32_t)m.boundaries[(size_t)node * 6 + f];
+            const int tgt = bnd_target(nd);
+            if (tgt < 0 || bnd_depth(nd) <= 0 || bnd_ddiff(nd) != 0) continue;
+            int p3[3] = {x + P, y + P, z + P};
+            p3[ax] += f < 3 ? -S : S;
+            const size_t dst = (size_t)tgt * (T * T * T) + ((size_t)p3[2] * T + p3[1]) * T + p3[0];
+#pragma unroll
+            for (int v = 0; v < NV; v++) a.new_[v][dst] = k[v];
         }
     }
 }
@@ -400,25 +424,35 @@ static BssnParDev dev_par(const am_bssn_params *p)
     return d;
 }
 
+template <int S, int P, bool PUSH>
+static void launch_stage_push(int stage, dim3 grid, hipStream_t st, const BssnArgsDev &a, const TileArgs &m,
+                              const BssnParDev &d, const DepthScales &sc)
+{
+    switch (stage) {
+    case 0: hipLaunchKernelGGL((k_bssn_stage<S, P, 0, PUSH>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    case 1: hipLaunchKernelGGL((k_bssn_stage<S, P, 1, PUSH>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    case 2: hipLaunchKernelGGL((k_bssn_stage<S, P, 2, PUSH>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    default: hipLaunchKernelGGL((k_bssn_stage<S, P, 3, PUSH>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    }
+}
+
 template <int S, int P>
 static int launch_stage(const am_bssn_fields *f, int stage, const am_bssn_params *p, const am_mesh *mesh,
-                        const am_geometry *g, void *stream)
+                        const am_geometry *g, int push, void *stream)
 {
     constexpr int BPT = (S * S * S + 255) / 256;
+    if (stage < 0 || stage > 3) return set_error(AM_ERR_INVALID, "am_bssn_rhs_stage: stage %d", stage);
+    if (push && !mesh->boundaries) return set_error(AM_ERR_INVALID, "am_bssn_rhs_stage: push needs boundaries");
     BssnArgsDev a;
     if (int rc = fill_args(f, a, true)) return rc;
-    TileArgs m{mesh->positions, mesh->tiles, mesh->substep, mesh->max_depth};
+    TileArgs m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth};
     const DepthScales sc = depth_scales(*g);
     const BssnParDev d = dev_par(p);
     dim3 grid(mesh->n_tiles * BPT);
-    hipStream_t st = (hipStream_t)stream;
-    switch (stage) {
-    case 0: hipLaunchKernelGGL((k_bssn_stage<S, P, 0>), grid, dim3(256), 0, st, a, m, d, sc); break;
-    case 1: hipLaunchKernelGGL((k_bssn_stage<S, P, 1>), grid, dim3(256), 0, st, a, m, d, sc); break;
-    case 2: hipLaunchKernelGGL((k_bssn_stage<S, P, 2>), grid, dim3(256), 0, st, a, m, d, sc); break;
-    case 3: hipLaunchKernelGGL((k_bssn_stage<S, P, 3>), grid, dim3(256), 0, st, a, m, d, sc); break;
-    default: return set_error(AM_ERR_INVALID, "am_bssn_rhs_stage: stage %d", stage);
-    }
+    if (push)
+        launch_stage_push<S, P, true>(stage, grid, (hipStream_t)stream, a, m, d, sc);
+    else
+        launch_stage_push<S, P, false>(stage, grid, (hipStream_t)stream, a, m, d, sc);
     return check_launch("k_bssn_stage");
 }
 
@@ -429,7 +463,7 @@ static int launch_cons(const am_bssn_fields *f, const am_bssn_params *p, const a
     constexpr int BPT = (S * S * S + 255) / 256;
     BssnArgsDev a;
     if (int rc = fill_args(f, a, false)) return rc;
-    TileArgs m{mesh->positions, mesh->tiles, mesh->substep, mesh->max_depth};
+    TileArgs m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth};
     hipLaunchKernelGGL((k_bssn_constraints<S, P>), dim3(mesh->n_tiles * BPT), dim3(256), 0, (hipStream_t)stream, a,
                        m, dev_par(p), partial);
     return check_launch("k_bssn_constraints");
@@ -438,12 +472,12 @@ static int launch_cons(const am_bssn_fields *f, const am_bssn_params *p, const a
 }  // namespace am
 
 extern "C" int am_bssn_rhs_stage(const am_bssn_fields *f, int32_t stage, const am_bssn_params *p, const am_mesh *mesh,
-                                 const am_geometry *g, void *stream)
+                                 const am_geometry *g, int32_t push_ghosts, void *stream)
 {
     if (!f || !p || !mesh || !g || !mesh->tiles || !mesh->positions)
         return am::set_error(AM_ERR_INVALID, "am_bssn_rhs_stage: null argument");
     if (mesh->n_tiles <= 0) return AM_OK;
-    AM_DISPATCH_TILE(*g, am::launch_stage, f, stage, p, mesh, g, stream);
+    AM_DISPATCH_TILE(*g, am::launch_stage, f, stage, p, mesh, g, push_ghosts, stream);
 }
 
 extern "C" int am_bssn_constraints(const am_bssn_fields *f, const am_bssn_params *p, const am_mesh *mesh,

@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(S *S) k_ghost_fill(FieldBatchDev b, const uint
 template <int S, int P, int MODE, int VPG>
 __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n_vars, const uint64_t *__restrict__ bnd,
                                                           const int32_t *__restrict__ tiles, uint32_t substep,
-                                                          uint32_t maxd, float frac)
+                                                          uint32_t maxd, float frac, int equal_mode)
 {
     constexpr int T = S + 2 * P, NC = S * S * P;
     constexpr size_t CELLS = (size_t)T * T * T;
@@ -213,6 +213,7 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
     if (!active_depth(substep, (uint32_t)cd, maxd)) return;
     const float3 relp = cf3_unpack((uint32_t)(bb >> 32));
     const uint32_t ddiff = bnd_ddiff(nd);
+    if (ddiff == 0 && equal_mode == AM_EQUAL_SKIP) return;
     const float rel_scale = 1.f / (float)(1u << ddiff);
     const float tmp = (float)((substep >> (maxd - (uint32_t)cd)) & ((1u << ddiff) - 1u));
     const float ty = ddiff == 0 ? 1.f : (tmp + frac) / (float)(1u << ddiff);
@@ -240,6 +241,13 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
             rp[k] = rp[k] * rel_scale + rl[k];
             rp[k] = (rp[k] + .5f) * (float)uS + (float)uP - 0.5f;
         }
+        const size_t wn = (size_t)node * CELLS + w0 + ((size_t)w1 + (size_t)w2 * T) * T;
+        if (ddiff == 0 && equal_mode == AM_EQUAL_COPY) {
+            // the neighbour cell the ghost maps onto (rp is an exact integer)
+            const size_t src = (size_t)tgt * CELLS + (size_t)rp[0] + ((size_t)rp[1] + (size_t)rp[2] * T) * T;
+            for (int v = 0; v < nv; v++) b.new_[v0 + v][wn] = b.new_[v0 + v][src];
+            continue;
+        }
         uint32_t ri[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -256,7 +264,6 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
         const size_t rn = (size_t)tgt * CELLS + ri[0] + ((size_t)ri[1] + (size_t)ri[2] * T) * T;
 #pragma unroll
         for (int k = 0; k < 3; k++) rp[k] -= (float)ri[k];
-        const size_t wn = (size_t)node * CELLS + w0 + ((size_t)w1 + (size_t)w2 * T) * T;
         if (ddiff == 0 && MODE == AM_INTERP_LAGRANGE) {
             // Equal depth: every offset is exactly 0, or 1 where the read index
             // was clamped, and the time weights are (0, 1).  Terms with a zero
@@ -332,7 +339,8 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
 constexpr int GHOST_VPG = 6;  // fields per workgroup in the multi-field fill
 
 template <int S, int P>
-static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, int interp, float frac, void *stream)
+static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, int interp, float frac, int equal_mode,
+                              void *stream)
 {
     FieldBatchDev d;
     for (int v = 0; v < b->n_vars; v++) {
@@ -345,13 +353,16 @@ static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, int 
     hipStream_t st = (hipStream_t)stream;
     if (interp == AM_INTERP_LAGRANGE)
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_LAGRANGE, GHOST_VPG>), grid, dim3(256), 0, st, d,
-                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac);
+                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac,
+                           equal_mode);
     else if (interp == AM_INTERP_TRILINEAR)
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRILINEAR, GHOST_VPG>), grid, dim3(256), 0, st, d,
-                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac);
+                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac,
+                           equal_mode);
     else if (interp == AM_INTERP_TRICUBIC)
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRICUBIC, GHOST_VPG>), grid, dim3(256), 0, st, d,
-                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac);
+                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac,
+                           equal_mode);
     else
         return set_error(AM_ERR_INVALID, "am_ghost_fill: unknown interpolation mode %d", interp);
     return check_launch("k_ghost_fill_multi");
@@ -529,8 +540,10 @@ extern "C" int am_ghost_fill_batched(const am_field_batch *b, const am_mesh *mes
 }
 
 extern "C" int am_ghost_fill_stage_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g,
-                                           float stage_frac, int32_t interp, void *stream)
+                                           float stage_frac, int32_t interp, int32_t equal_mode, void *stream)
 {
+    if (equal_mode < AM_EQUAL_FORMULA || equal_mode > AM_EQUAL_SKIP)
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_batched: equal_mode %d", equal_mode);
     if (!b) return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_batched: null batch");
     if (b->n_vars < 0 || b->n_vars > AM_MAX_VARS)
         return am::set_error(AM_ERR_TOO_MANY_VARS, "am_ghost_fill_stage_batched: n_vars %d", b->n_vars);
@@ -538,7 +551,7 @@ extern "C" int am_ghost_fill_stage_batched(const am_field_batch *b, const am_mes
         return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_batched: stage_frac %g", (double)stage_frac);
     if (int rc = am::mesh_checks(mesh, g, true, false)) return rc;
     if (b->n_vars == 0 || mesh->n_tiles == 0) return AM_OK;
-    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, interp, stage_frac, stream);
+    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, interp, stage_frac, equal_mode, stream);
 }
 
 extern "C" int am_ghost_fill(float *old_, float *new_, const float *older, const am_mesh *mesh, const am_geometry *g,

@@ -209,11 +209,17 @@ int am_christoffel(const float *const *metric6, const uint32_t *const *dmetric6,
 /* Ghost fill at an intermediate Runge-Kutta time: faces whose target is
  * coarser by dd levels interpolate the target's old/new at
  * (k + stage_frac) / 2^dd of its step (relative_time, AMR.h:634-639, gives
- * k and k + 1); equal-depth faces copy the target's new, which holds the same
- * stage.  stage_frac = 1 is exactly am_ghost_fill_batched(copy_only_new=1).
+ * k and k + 1); equal-depth faces take the target's new, which holds the same
+ * stage, by `equal_mode`:
+ *   AM_EQUAL_FORMULA  the reference's clamped interpolation; with
+ *                     stage_frac = 1 the values of am_ghost_fill_batched
+ *                     (copy_only_new = 1) for finite data
+ *   AM_EQUAL_COPY     an exact copy of the neighbour cell the ghost maps onto
+ *   AM_EQUAL_SKIP     left alone (written by am_bssn_rhs_stage's push)
  * No counterpart in the reference, whose Heun scheme only fills at k + 1. */
+enum am_equal_mode { AM_EQUAL_FORMULA = 0, AM_EQUAL_COPY = 1, AM_EQUAL_SKIP = 2 };
 int am_ghost_fill_stage_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, float stage_frac,
-                                int32_t interp, void *stream);
+                                int32_t interp, int32_t equal_mode, void *stream);
 
 /* ----------------------------------------------------------- BSSN scheme
  * No counterpart in the reference (BSSN.h has no right-hand side or time
@@ -247,9 +253,12 @@ typedef struct am_bssn_fields {
 /* One classical RK4 stage on the interior of every listed active tile:
  * k = f(state); stage 0: acc = k, new = old + dt/2 k; stage 1: acc += 2k,
  * new = old + dt/2 k; stage 2: acc += 2k, new = old + dt k; stage 3:
- * new = old + dt/6 (acc + k), with dt = outer_dt / 2^depth. */
+ * new = old + dt/6 (acc + k), with dt = outer_dt / 2^depth.  With
+ * push_ghosts != 0 (mesh->boundaries required) every new value within P
+ * cells of a face whose neighbour has the same depth is also written into
+ * that neighbour's face ghost cell: the AM_EQUAL_COPY fill, fused. */
 int am_bssn_rhs_stage(const am_bssn_fields *f, int32_t stage, const am_bssn_params *p, const am_mesh *mesh,
-                      const am_geometry *g, void *stream);
+                      const am_geometry *g, int32_t push_ghosts, void *stream);
 
 /* Hamiltonian and momentum constraints of `state` on every listed tile,
  * reduced in FP64 to one {sum H^2, sum M_i M_i, cells, max |H|} quadruple per

@@ -146,6 +146,7 @@ struct octree {
     smart_gpu_cpu_buffer<int32_t> tile_lists;
     int32_t list_offset[depth_limit + 3] = {};
     int32_t list_count[depth_limit + 3] = {};
+    int32_t coarse_faces[depth_limit + 1] = {};  // faces per depth whose target is coarser
 
    protected:
     bool dirty = true;
@@ -316,6 +317,15 @@ struct octree {
             if (leaf) L[n++] = (int32_t)i;
         }
         list_count[depth_limit + 2] = n - list_offset[depth_limit + 2];
+        for (uint32_t d = 0; d <= depth_limit; d++) coarse_faces[d] = 0;
+        for (uint32_t i = 0; i <= final_used_slot; i++) {
+            const int d = positions.cpu_buffer_ptr[i].depth();
+            if (d <= 0) continue;
+            for (uint32_t f = 0; f < 6; f++) {
+                const domain_boundary &b = boundaries.cpu_buffer_ptr[i * 6u + f];
+                if (b.target_idx() >= 0 && b.depth_diff() > 0) coarse_faces[d]++;
+            }
+        }
         parent_idx_b.copy_to_gpu(stream);
         children_idx_b.copy_to_gpu(stream);
         positions.copy_to_gpu(stream);
@@ -324,6 +334,7 @@ struct octree {
         dirty = false;
     }
     int tiles_at_depth(int d) const { return list_count[d + 1]; }
+    int coarse_faces_at_depth(int d) const { return coarse_faces[d]; }
     int leaf_count() const { return list_count[depth_limit + 2]; }
 
     // Kernel view over every used slot (the reference's launch domain) ...

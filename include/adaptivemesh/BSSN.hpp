@@ -318,13 +318,16 @@ struct BSSN_rk4_data {
         {
             derivatives(s == 0 ? old_ : new_, m);
             const am_bssn_fields f = fields(s == 0);
+            // equal-depth face ghosts are pushed by the RHS kernel; faces with a
+            // coarser target are interpolated afterwards
             int ev = timer.begin(kernel_timer::RHS);
-            check(am_bssn_rhs_stage(&f, s, &opt.params, &m, &geom, stream), "rhs");
+            check(am_bssn_rhs_stage(&f, s, &opt.params, &m, &geom, 1, stream), "rhs");
             timer.end(ev);
-            if (d > 0) {
+            if (d > 0 && parent.coarse_faces_at_depth(d) > 0) {
                 am_field_batch b = batch(true);
                 ev = timer.begin(kernel_timer::GHOST);
-                check(am_ghost_fill_stage_batched(&b, &m, &geom, frac[s], opt.interp, stream), "ghost");
+                check(am_ghost_fill_stage_batched(&b, &m, &geom, frac[s], opt.interp, AM_EQUAL_SKIP, stream),
+                      "ghost");
                 timer.end(ev);
             }
         }
@@ -357,7 +360,7 @@ struct BSSN_rk4_data {
             const am_mesh m = parent.mesh_depth(d);
             if (m.n_tiles == 0) continue;
             am_field_batch b = batch(false);
-            check(am_ghost_fill_stage_batched(&b, &m, &geom, 1.f, opt.interp, stream), "ghost");
+            check(am_ghost_fill_stage_batched(&b, &m, &geom, 1.f, opt.interp, AM_EQUAL_COPY, stream), "ghost");
         }
     }
     // FP64 L2 norms of the Hamiltonian and momentum constraints of `new`

@@ -578,7 +578,7 @@ enum { INTERP_LAGRANGE = 0, INTERP_TRILINEAR = 1, INTERP_TRICUBIC = 2 };
  * of the coarse step; older may be NULL, which selects linear time. */
 EXPORT void orc_copy_boundaries(float *old_, float *new_, const float *older, const uint64_t *bnd,
                                 const int32_t *tiles, int n_tiles, uint32_t substep, uint32_t maxd,
-                                int copy_only_new, int mode, int S, int P, float frac)
+                                int copy_only_new, int mode, int S, int P, float frac, int equal_copy)
 {
     const int T = S + 2 * P;
     const int64_t cells = (int64_t)T * T * T;
@@ -622,6 +622,17 @@ EXPORT void orc_copy_boundaries(float *old_, float *new_, const float *older, co
                             rp[k] = ((float)w[k] - (float)uP + .5f) * invS - .5f;
                             rp[k] = rp[k] * rel_scale + relp[k];
                             rp[k] = (rp[k] + .5f) * (float)uS + (float)uP - 0.5f;
+                        }
+                        if (equal_copy && ddiff == 0) {
+                            /* the RK4 driver's equal-depth faces: an exact copy of the
+                             * neighbour cell the ghost cell maps onto (an extension; the
+                             * reference clamps and extrapolates on the last row) */
+                            const int64_t src = (int64_t)tgt * cells + (int64_t)rp[0] +
+                                                ((int64_t)rp[1] + (int64_t)rp[2] * T) * T;
+                            const int64_t wn = (int64_t)node * cells + w[0] + ((int64_t)w[1] + (int64_t)w[2] * T) * T;
+                            if (!copy_only_new) old_[wn] = old_[src];
+                            new_[wn] = new_[src];
+                            continue;
                         }
                         if (mode == INTERP_TRILINEAR) {
                             for (int k = 0; k < 3; k++)

@@ -581,7 +581,7 @@ static inline double tric(const double *d, const double *rp, int64_t n, int T)
  * faces interpolate at (k + frac) / 2^dd. */
 EXPORT void orc_ghost64(double *const *old_, double *const *new_, const double *const *older, int nvars,
                         const uint64_t *bnd, const int32_t *tiles, int n_tiles, uint32_t substep, uint32_t maxd,
-                        float frac, int mode, int S, int P)
+                        float frac, int mode, int S, int P, int equal_copy)
 {
     const int T = S + 2 * P;
     const int64_t cells = (int64_t)T * T * T;
@@ -628,6 +628,13 @@ EXPORT void orc_ghost64(double *const *old_, double *const *new_, const double *
                             const uint32_t hi = mode == 2 ? (tgt == 0 ? (uint32_t)T - 3u : (uint32_t)T - uP - 3u)
                                                           : (tgt == 0 ? (uint32_t)T - 2u : (uint32_t)T - uP - 2u);
                             ri[k] = v < hi ? v : hi;
+                        }
+                        const int64_t wn0 = (int64_t)node * cells + w[0] + ((int64_t)w[1] + (int64_t)w[2] * T) * T;
+                        if (equal_copy && dd == 0) {
+                            const int64_t src = (int64_t)tgt * cells + (int64_t)rp[0] +
+                                                ((int64_t)rp[1] + (int64_t)rp[2] * T) * T;
+                            for (int v = 0; v < nvars; v++) new_[v][wn0] = new_[v][src];
+                            continue;
                         }
                         const int64_t rn = (int64_t)tgt * cells + ri[0] + ((int64_t)ri[1] + (int64_t)ri[2] * T) * T;
                         double fr[3];

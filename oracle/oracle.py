@@ -43,7 +43,7 @@ _SIGS = {
     "orc_differentiate": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, _i32, C.c_int, C.c_int, C.c_int, _f32]),
     "orc_mixed_partials": (None, [_P, _P, _P, _P, C.c_int, _u32, _u32, _i32, C.c_int, C.c_int, _f32]),
     "orc_copy_boundaries": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, C.c_int,
-                                   _f32]),
+                                   _f32, C.c_int]),
     "orc_copy_new_to_old": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
     "orc_copy_to_child": (None, [_P, _u32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_wave_stage": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, _f32]),
@@ -179,11 +179,13 @@ def mixed_partials(field, positions, tiles, substep, max_depth, seed, S, P, oute
     return off
 
 
-def copy_boundaries(old, new, older, boundaries, tiles, substep, max_depth, copy_only_new, mode, S, P, frac=1.0):
-    """In place on old/new (float32 arrays)."""
+def copy_boundaries(old, new, older, boundaries, tiles, substep, max_depth, copy_only_new, mode, S, P, frac=1.0,
+                    equal_copy=False):
+    """In place on old/new (float32 arrays).  equal_copy: equal-depth faces
+    copy the neighbour cell exactly (the RK4 driver's rule)."""
     t = _i32arr(tiles)
     lib().orc_copy_boundaries(_p(old), _p(new), _p(older), _p(boundaries), _p(t), t.size, substep, max_depth,
-                              1 if copy_only_new else 0, mode, S, P, frac)
+                              1 if copy_only_new else 0, mode, S, P, frac, 1 if equal_copy else 0)
 
 
 def copy_new_to_old(old, new, positions, children, tiles, substep, max_depth, S, P):
@@ -231,7 +233,8 @@ _SIGS_BSSN = {
     "orc_bssn64_stage": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, _P, _P, _P, C.c_int, C.c_int,
                                 C.c_int]),
     "orc_bssn64_constraints": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P]),
-    "orc_ghost64": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, _u32, _u32, _f32, C.c_int, C.c_int, C.c_int]),
+    "orc_ghost64": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, _u32, _u32, _f32, C.c_int, C.c_int, C.c_int,
+                           C.c_int]),
     "orc_copy_back64": (None, [_P, _P, _P, C.c_int, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
 }
 _SIGS.update(_SIGS_BSSN)
