@@ -206,12 +206,11 @@ am_bssn_sim *am_bssn_sim_create(const am_geometry *g, uint32_t max_slots, const 
         return nullptr;
     }
     try {
-        bssn_rk4_options opt;
-        opt.params = o->params;
+        am::rk4_options opt;
         opt.interp = o->interp;
         opt.z_block = o->z_block;
         auto *s = new am_bssn_sim;
-        s->amr = new AMR<BSSN_rk4_data>(max_slots, o->periodic != 0, *g, opt);
+        s->amr = new AMR<BSSN_rk4_data>(max_slots, o->periodic != 0, *g, opt, o->params);
         s->amr->sim_data.rng.seed = o->seed;
         return s;
     } catch (const std::exception &e) {
@@ -307,7 +306,7 @@ int am_bssn_sim_timestep(am_bssn_sim *s) { BSSN_CALL(s->amr->sim_data.timestep()
 int am_bssn_sim_constraints(am_bssn_sim *s, double *out4)
 {
     if (!out4) return am::set_error(AM_ERR_INVALID, "null output");
-    BSSN_CALL(s->amr->sim_data.constraints(out4));
+    BSSN_CALL(s->amr->sim_data.diagnostics(out4));
 }
 int am_bssn_sim_sync(am_bssn_sim *s)
 {

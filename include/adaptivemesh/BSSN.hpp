@@ -6,19 +6,16 @@
 // predictor and corrector with the reference's 26 seed draws per substep.
 // Like the reference it never advances the metric.
 //
-// BSSN_rk4_data is the evolution the reference leaves out: 24 fields of the
-// moving-puncture BSSN system, classical RK4 with Berger-Oliger sub-cycling
-// (each depth steps 2^(max_depth - d) times less often, AMR.h:640-644), run
-// depth by depth, coarse first, so that a fine tile's ghost cells always
-// interpolate a coarse step that is complete.  Per stage and depth:
-//   FD words for all 24 fields (one batched launch) -> Christoffel words ->
-//   RHS + RK4 update -> ghost fill at the stage time.
+// BSSN_rk4_data is the evolution the reference leaves out: the 24 fields of
+// the moving-puncture BSSN system advanced by the generic RK4 plugin
+// (RK4.hpp) with sub-cycling.
 #pragma once
 #include <math.h>
 
 #include <vector>
 
 #include "AMR.hpp"
+#include "RK4.hpp"
 
 namespace am {
 
@@ -116,264 +113,46 @@ struct BSSN_AMR_data {
 };
 
 // -------------------------------------------------------------- evolution
-struct bssn_rk4_options {
-    am_bssn_params params{2.0f, 0.75f, 1e-4f, 1, 0};
-    int32_t interp = AM_INTERP_LAGRANGE;  // ghost fill: reference quadratic, or tricubic + quadratic time
-    int32_t z_block = 0;                  // FD launch shape (0 = whole tile per workgroup)
-};
+// The 24-field moving-puncture system for rk4_AMR_data.
+struct bssn_system {
+    static constexpr int NV = AM_BSSN_NVARS;
+    // chi, gt_ij, alpha, beta^i need full Hessians; everything else a gradient
+    static constexpr int FD_KIND[NV] = {3, 3, 3, 3, 3, 3, 3, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3, 3, 3, 1, 1, 1};
+    static constexpr bool CHRISTOFFEL = true;
+    using options_t = am_bssn_params;
 
-// Optional per-launch timing with HIP events on the launch stream.
-struct kernel_timer {
-    enum { FD = 0, CHRISTOFFEL, RHS, GHOST, COPY_BACK, N_CLASSES };
-    bool on = false;
-    std::vector<void *> pool;
-    std::vector<std::pair<int, int>> marks;  // (class, index of the start event)
-    size_t next = 0;
-    void *stream = nullptr;
-
-    void enable(bool e, void *st)
-    {
-        stream = st;
-        on = e;
-        if (e && pool.empty()) {
-            pool.resize(16384);
-            for (auto &ev : pool) check(am_event_create(&ev), "am_event_create");
-        }
-        marks.clear();
-        next = 0;
-    }
-    int begin(int cls)
-    {
-        if (!on || next + 2 > pool.size()) return -1;
-        const int i = (int)next;
-        next += 2;
-        check(am_event_record(pool[i], stream), "am_event_record");
-        marks.emplace_back(cls, i);
-        return i;
-    }
-    void end(int i)
-    {
-        if (i >= 0) check(am_event_record(pool[i + 1], stream), "am_event_record");
-    }
-    // total ms and launch count per class since enable(); resets the marks
-    void totals(double *ms, int64_t *count)
-    {
-        for (int c = 0; c < N_CLASSES; c++) { ms[c] = 0; count[c] = 0; }
-        for (auto &m : marks) {
-            float t = 0.f;
-            check(am_event_elapsed_ms(pool[m.second], pool[m.second + 1], &t), "am_event_elapsed_ms");
-            ms[m.first] += t;
-            count[m.first]++;
-        }
-        marks.clear();
-        next = 0;
-    }
-    void destroy()
-    {
-        for (auto ev : pool) am_event_destroy(ev);
-        pool.clear();
-    }
-};
-
-struct BSSN_rk4_data {
-    static constexpr int NV = AM_BSSN_NVARS, NH = AM_BSSN_NHESS;
-    static constexpr int HESS_VAR[NH] = {0, 1, 2, 3, 4, 5, 6, 17, 18, 19, 20};
-
-    fast_prng rng;
-    AMR<BSSN_rk4_data> &parent;
-    am_geometry geom;
-    bssn_rk4_options opt;
-    void *stream = nullptr;
-    size_t cells = 0, inner = 0, slots = 0;
-
-    // one allocation per kind; variable v at base + v * slots * cells
-    smart_gpu_buffer<float> old_, new_, acc, older;
-    smart_gpu_buffer<uint32_t> d1, d2d, d2o, chr;
-    smart_gpu_buffer<double> partial;
-    uint64_t zone_cycles = 0;  // interior cells advanced by one full step, all depths
-    kernel_timer timer;
-
-    BSSN_rk4_data(uint32_t node_slots, AMR<BSSN_rk4_data> &p, const am_geometry &g,
-                  const bssn_rk4_options &o = bssn_rk4_options())
-        : rng(), parent(p), geom(g), opt(o)
-    {
-        const size_t T = (size_t)(g.size_domain + 2 * g.padding_domain), S = (size_t)g.size_domain;
-        cells = T * T * T;
-        inner = S * S * S;
-        slots = node_slots;
-        old_ = smart_gpu_buffer<float>(NV * slots * cells);
-        new_ = smart_gpu_buffer<float>(NV * slots * cells);
-        acc = smart_gpu_buffer<float>(NV * slots * cells);
-        if (opt.interp == AM_INTERP_TRICUBIC) older = smart_gpu_buffer<float>(NV * slots * cells);
-        d1 = smart_gpu_buffer<uint32_t>(NV * slots * inner);
-        d2d = smart_gpu_buffer<uint32_t>(NH * slots * inner);
-        d2o = smart_gpu_buffer<uint32_t>(NH * slots * inner);
-        chr = smart_gpu_buffer<uint32_t>(6 * slots * inner);
-        partial = smart_gpu_buffer<double>(4 * (size_t)am_bssn_constraint_blocks(&geom, (int32_t)slots) + 4);
-        old_.zero(); new_.zero(); acc.zero();
-        if (older.created) older.zero();
-    }
-    void destroy()
-    {
-        old_.destroy(); new_.destroy(); acc.destroy(); older.destroy();
-        d1.destroy(); d2d.destroy(); d2o.destroy(); chr.destroy(); partial.destroy();
-        timer.destroy();
-    }
-    float *field(smart_gpu_buffer<float> &b, int v) { return b.gpu_buffer_ptr + (size_t)v * slots * cells; }
-    uint32_t *word(smart_gpu_buffer<uint32_t> &b, int v) { return b.gpu_buffer_ptr + (size_t)v * slots * inner; }
-
-    am_field_batch batch(bool with_older)
-    {
-        am_field_batch b{};
-        b.n_vars = NV;
-        for (int v = 0; v < NV; v++) {
-            b.old_[v] = field(old_, v);
-            b.new_[v] = field(new_, v);
-            b.older[v] = with_older && older.created ? field(older, v) : nullptr;
-        }
-        return b;
-    }
-    am_bssn_fields fields(bool stage_input_is_old)
+    static am_bssn_fields fields(const rk4_views &v)
     {
         am_bssn_fields f{};
-        for (int v = 0; v < NV; v++) {
-            f.state[v] = stage_input_is_old ? field(old_, v) : field(new_, v);
-            f.old_[v] = field(old_, v);
-            f.new_[v] = field(new_, v);
-            f.acc[v] = field(acc, v);
-            f.d1[v] = word(d1, v);
+        for (int i = 0; i < NV; i++) {
+            f.state[i] = v.state[i];
+            f.old_[i] = v.old_[i];
+            f.new_[i] = v.new_[i];
+            f.acc[i] = v.acc[i];
+            f.d1[i] = v.d1[i];
         }
-        for (int h = 0; h < NH; h++) {
-            f.d2diag[h] = word(d2d, h);
-            f.d2off[h] = word(d2o, h);
+        for (int h = 0; h < AM_BSSN_NHESS; h++) {
+            f.d2diag[h] = v.d2diag[h];
+            f.d2off[h] = v.d2off[h];
         }
-        for (int c = 0; c < 6; c++) f.chr[c] = word(chr, c);
+        for (int c = 0; c < 6; c++) f.chr[c] = v.chr[c];
         return f;
     }
-
-    void node_ctor(int node_idx)
+    static int rhs_stage(const rk4_views &v, int stage, const options_t &p, const am_mesh &m, const am_geometry &g,
+                         int push, void *stream)
     {
-        const uint32_t off = (uint32_t)parent.positions.cpu_buffer_ptr[node_idx].final_offset();
-        const int par = parent.parent_idx_b.cpu_buffer_ptr[node_idx];
-        for (int v = 0; v < NV; v++) {
-            check(am_copy_to_child(field(old_, v), off, node_idx, par, &geom, AM_INTERP_LAGRANGE, stream), "node_ctor");
-            check(am_copy_to_child(field(new_, v), off, node_idx, par, &geom, AM_INTERP_LAGRANGE, stream), "node_ctor");
-        }
+        const am_bssn_fields f = fields(v);
+        return am_bssn_rhs_stage(&f, stage, &p, &m, &g, push, stream);
     }
-    void node_dtor(int) {}
-    void modify_nodes() {}
-
-    // derivative words of the state held in `src` for the tiles of `m`
-    void derivatives(smart_gpu_buffer<float> &src, const am_mesh &m)
+    // {rms H, rms |M|, max |H|, cells} over the leaves
+    static void diagnostics(const rk4_views &v, const options_t &p, const am_mesh &m, const am_geometry &g,
+                            double *partial, double *out4, void *stream)
     {
-        am_fd_batch b{};
-        b.n_vars = NV;
-        b.z_block = opt.z_block;
-        int h = 0;
-        for (int v = 0; v < NV; v++) {
-            b.field[v] = field(src, v);
-            b.d1[v] = word(d1, v);
-            b.seed_d[v] = rng.generate_int();
-            if (h < NH && HESS_VAR[h] == v) {
-                b.d2diag[v] = word(d2d, h);
-                b.d2off[v] = word(d2o, h);
-                b.seed_m[v] = rng.generate_int();
-                h++;
-            }
-        }
-        int ev = timer.begin(kernel_timer::FD);
-        check(am_fd_batched(&b, &m, &geom, stream), "fd");
-        timer.end(ev);
-        const float *met[6];
-        const uint32_t *dg[6];
-        uint32_t *out[6];
-        for (int c = 0; c < 6; c++) {
-            met[c] = field(src, 1 + c);
-            dg[c] = word(d1, 1 + c);
-            out[c] = word(chr, c);
-        }
-        ev = timer.begin(kernel_timer::CHRISTOFFEL);
-        check(am_christoffel(met, dg, out, 0, &m, &geom, rng.generate_int(), stream), "christoffel");
-        timer.end(ev);
-    }
-
-    // copy-back: old <- new (+ restriction), keeping the previous old when a
-    // third time level is in use (AMR.h:998-1051)
-    void copy_back()
-    {
-        am_field_batch b = batch(true);
-        const am_mesh m = parent.mesh_all();
-        const int ev = timer.begin(kernel_timer::COPY_BACK);
-        check(am_copy_new_to_old_batched(&b, &m, &geom, stream), "copy_back");
-        timer.end(ev);
-    }
-
-    // one RK4 stage of every tile at depth d: derivative words of the stage
-    // input, RHS + update, ghost fill of new at the stage time
-    void stage_depth(int d, int s)
-    {
-        const am_mesh m = parent.mesh_depth(d);
-        if (m.n_tiles == 0) return;
-        static const float frac[4] = {.5f, .5f, 1.f, 1.f};
-        {
-            derivatives(s == 0 ? old_ : new_, m);
-            const am_bssn_fields f = fields(s == 0);
-            // equal-depth face ghosts are pushed by the RHS kernel; faces with a
-            // coarser target are interpolated afterwards
-            int ev = timer.begin(kernel_timer::RHS);
-            check(am_bssn_rhs_stage(&f, s, &opt.params, &m, &geom, 1, stream), "rhs");
-            timer.end(ev);
-            if (d > 0 && parent.coarse_faces_at_depth(d) > 0) {
-                am_field_batch b = batch(true);
-                ev = timer.begin(kernel_timer::GHOST);
-                check(am_ghost_fill_stage_batched(&b, &m, &geom, frac[s], opt.interp, AM_EQUAL_SKIP, stream),
-                      "ghost");
-                timer.end(ev);
-            }
-        }
-    }
-    // one RK4 step of every tile at depth d
-    void step_depth(int d)
-    {
-        for (int s = 0; s < 4; s++) stage_depth(d, s);
-        zone_cycles += (uint64_t)parent.tiles_at_depth(d) * inner;
-    }
-
-    void substep()
-    {
-        copy_back();
-        for (int d = 0; d <= parent.read_max_depth(); d++)
-            if (parent.depth_active(d)) step_depth(d);
-        parent.increment_timer();
-    }
-    // the reference's timestep(): sub-steps until the counter wraps
-    void timestep()
-    {
-        do {
-            substep();
-        } while (parent.timer_helper != 0u);
-    }
-    // fill every face ghost of `new` from the current data (initial data)
-    void fill_ghosts()
-    {
-        for (int d = 1; d <= parent.read_max_depth(); d++) {
-            const am_mesh m = parent.mesh_depth(d);
-            if (m.n_tiles == 0) continue;
-            am_field_batch b = batch(false);
-            check(am_ghost_fill_stage_batched(&b, &m, &geom, 1.f, opt.interp, AM_EQUAL_COPY, stream), "ghost");
-        }
-    }
-    // FP64 L2 norms of the Hamiltonian and momentum constraints of `new`
-    // over the leaves: {rms H, rms |M|, max |H|, cells}
-    void constraints(double *out4)
-    {
-        const am_mesh m = parent.mesh_leaves();
-        derivatives(new_, m);
-        const am_bssn_fields f = fields(false);
-        check(am_bssn_constraints(&f, &opt.params, &m, &geom, partial.gpu_buffer_ptr, stream), "constraints");
-        const int64_t nb = am_bssn_constraint_blocks(&geom, m.n_tiles);
+        const am_bssn_fields f = fields(v);
+        check(am_bssn_constraints(&f, &p, &m, &g, partial, stream), "constraints");
+        const int64_t nb = am_bssn_constraint_blocks(&g, m.n_tiles);
         std::vector<double> h((size_t)(4 * nb));
-        check(am_memcpy_d2h(h.data(), partial.gpu_buffer_ptr, h.size() * sizeof(double), stream), "constraints");
+        check(am_memcpy_d2h(h.data(), partial, h.size() * sizeof(double), stream), "constraints");
         double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
         for (int64_t i = 0; i < nb; i++) {
             s0 += h[4 * i]; s1 += h[4 * i + 1]; s2 += h[4 * i + 2]; s3 = fmax(s3, h[4 * i + 3]);
@@ -384,5 +163,7 @@ struct BSSN_rk4_data {
         out4[3] = s2;
     }
 };
+
+using BSSN_rk4_data = rk4_AMR_data<bssn_system>;
 
 }  // namespace am

@@ -1,0 +1,300 @@
+// Generic classical-RK4 plugin for the octree driver, parameterised by a
+// physics System.  The reference integrates with Heun's method and a single
+// old/new pair (SimpleWave.h:249-278); this plugin keeps the reference's
+// octree, counter and phase order (copy_back at the start of each sub-step,
+// then the active depths, then increment_timer) and adds RK4 stages with
+// Berger-Oliger sub-cycling: depth d advances every 2^(max_depth - d)
+// sub-steps, depths run coarse first, so a fine tile's ghosts interpolate a
+// coarse step that is complete.  Per depth and stage:
+//   FD words of every field that needs derivatives (one batched launch)
+//   -> Christoffel words (if the System uses them)
+//   -> System RHS + RK4 update, which also pushes equal-depth face ghosts
+//   -> interpolated ghosts for faces whose target is coarser.
+//
+// A System provides: NV fields; FD_KIND[v] (0 none, 1 gradient, 2 gradient
+// + diagonal Hessian, 3 gradient + full Hessian); CHRISTOFFEL (metric in
+// fields 1..6); options_t; rhs_stage(...) and diagnostics(...).
+#pragma once
+#include <math.h>
+
+#include <vector>
+
+#include "AMR.hpp"
+
+namespace am {
+
+// Optional per-launch timing with HIP events on the launch stream.
+struct kernel_timer {
+    enum { FD = 0, CHRISTOFFEL, RHS, GHOST, COPY_BACK, N_CLASSES };
+    bool on = false;
+    std::vector<void *> pool;
+    std::vector<std::pair<int, int>> marks;  // (class, index of the start event)
+    size_t next = 0;
+    void *stream = nullptr;
+
+    void enable(bool e, void *st)
+    {
+        stream = st;
+        on = e;
+        if (e && pool.empty()) {
+            pool.resize(16384);
+            for (auto &ev : pool) check(am_event_create(&ev), "am_event_create");
+        }
+        marks.clear();
+        next = 0;
+    }
+    int begin(int cls)
+    {
+        if (!on || next + 2 > pool.size()) return -1;
+        const int i = (int)next;
+        next += 2;
+        check(am_event_record(pool[i], stream), "am_event_record");
+        marks.emplace_back(cls, i);
+        return i;
+    }
+    void end(int i)
+    {
+        if (i >= 0) check(am_event_record(pool[i + 1], stream), "am_event_record");
+    }
+    // total ms and launch count per class since enable(); resets the marks
+    void totals(double *ms, int64_t *count)
+    {
+        for (int c = 0; c < N_CLASSES; c++) { ms[c] = 0; count[c] = 0; }
+        for (auto &m : marks) {
+            float t = 0.f;
+            check(am_event_elapsed_ms(pool[m.second], pool[m.second + 1], &t), "am_event_elapsed_ms");
+            ms[m.first] += t;
+            count[m.first]++;
+        }
+        marks.clear();
+        next = 0;
+    }
+    void destroy()
+    {
+        for (auto ev : pool) am_event_destroy(ev);
+        pool.clear();
+    }
+};
+
+struct rk4_options {
+    int32_t interp = AM_INTERP_LAGRANGE;  // coarse-face ghosts: reference quadratic, or tricubic + quadratic time
+    int32_t z_block = 0;                  // FD launch shape (0 = whole tile per workgroup)
+};
+
+// Device views of the driver's buffers handed to a System's kernels.
+struct rk4_views {
+    const float *state[AM_MAX_VARS];
+    const float *old_[AM_MAX_VARS];
+    float *new_[AM_MAX_VARS];
+    float *acc[AM_MAX_VARS];
+    const uint32_t *d1[AM_MAX_VARS];      // gradient word of field v
+    const uint32_t *d2diag[AM_MAX_VARS];  // by Hessian index (fields with FD_KIND >= 2, in order)
+    const uint32_t *d2off[AM_MAX_VARS];   // by full-Hessian index (FD_KIND == 3, in order)
+    const uint32_t *chr[6];
+};
+
+template <class System>
+struct rk4_AMR_data {
+    static constexpr int NV = System::NV;
+    using options_t = typename System::options_t;
+
+    fast_prng rng;
+    AMR<rk4_AMR_data> &parent;
+    am_geometry geom;
+    rk4_options opt;
+    options_t sys;
+    void *stream = nullptr;
+    size_t cells = 0, inner = 0, slots = 0;
+    int n_d2 = 0, n_off = 0;
+
+    // one allocation per kind; field v at base + v * slots * cells
+    smart_gpu_buffer<float> old_, new_, acc, older;
+    smart_gpu_buffer<uint32_t> d1, d2d, d2o, chr;
+    smart_gpu_buffer<double> partial;
+    uint64_t zone_cycles = 0;  // interior cells advanced by one full step, all depths
+    kernel_timer timer;
+
+    rk4_AMR_data(uint32_t node_slots, AMR<rk4_AMR_data> &p, const am_geometry &g, const rk4_options &o,
+                 const options_t &so)
+        : rng(), parent(p), geom(g), opt(o), sys(so)
+    {
+        const size_t T = (size_t)(g.size_domain + 2 * g.padding_domain), S = (size_t)g.size_domain;
+        cells = T * T * T;
+        inner = S * S * S;
+        slots = node_slots;
+        for (int v = 0; v < NV; v++) {
+            n_d2 += System::FD_KIND[v] >= 2;
+            n_off += System::FD_KIND[v] == 3;
+        }
+        old_ = smart_gpu_buffer<float>(NV * slots * cells);
+        new_ = smart_gpu_buffer<float>(NV * slots * cells);
+        acc = smart_gpu_buffer<float>(NV * slots * cells);
+        if (opt.interp == AM_INTERP_TRICUBIC) older = smart_gpu_buffer<float>(NV * slots * cells);
+        d1 = smart_gpu_buffer<uint32_t>(NV * slots * inner);
+        if (n_d2) d2d = smart_gpu_buffer<uint32_t>(n_d2 * slots * inner);
+        if (n_off) d2o = smart_gpu_buffer<uint32_t>(n_off * slots * inner);
+        if (System::CHRISTOFFEL) chr = smart_gpu_buffer<uint32_t>(6 * slots * inner);
+        partial = smart_gpu_buffer<double>(4 * (size_t)am_bssn_constraint_blocks(&geom, (int32_t)slots) + 4);
+        old_.zero(); new_.zero(); acc.zero();
+        if (older.created) older.zero();
+    }
+    void destroy()
+    {
+        old_.destroy(); new_.destroy(); acc.destroy(); older.destroy();
+        d1.destroy(); d2d.destroy(); d2o.destroy(); chr.destroy(); partial.destroy();
+        timer.destroy();
+    }
+    float *field(smart_gpu_buffer<float> &b, int v) { return b.gpu_buffer_ptr + (size_t)v * slots * cells; }
+    uint32_t *word(smart_gpu_buffer<uint32_t> &b, int v) { return b.gpu_buffer_ptr + (size_t)v * slots * inner; }
+
+    am_field_batch batch(bool with_older)
+    {
+        am_field_batch b{};
+        b.n_vars = NV;
+        for (int v = 0; v < NV; v++) {
+            b.old_[v] = field(old_, v);
+            b.new_[v] = field(new_, v);
+            b.older[v] = with_older && older.created ? field(older, v) : nullptr;
+        }
+        return b;
+    }
+    rk4_views views(bool stage_input_is_old)
+    {
+        rk4_views f{};
+        for (int v = 0; v < NV; v++) {
+            f.state[v] = stage_input_is_old ? field(old_, v) : field(new_, v);
+            f.old_[v] = field(old_, v);
+            f.new_[v] = field(new_, v);
+            f.acc[v] = field(acc, v);
+            f.d1[v] = word(d1, v);
+        }
+        for (int h = 0; h < n_d2; h++) f.d2diag[h] = word(d2d, h);
+        for (int h = 0; h < n_off; h++) f.d2off[h] = word(d2o, h);
+        if (System::CHRISTOFFEL)
+            for (int c = 0; c < 6; c++) f.chr[c] = word(chr, c);
+        return f;
+    }
+
+    void node_ctor(int node_idx)
+    {
+        const uint32_t off = (uint32_t)parent.positions.cpu_buffer_ptr[node_idx].final_offset();
+        const int par = parent.parent_idx_b.cpu_buffer_ptr[node_idx];
+        for (int v = 0; v < NV; v++) {
+            check(am_copy_to_child(field(old_, v), off, node_idx, par, &geom, AM_INTERP_LAGRANGE, stream), "node_ctor");
+            check(am_copy_to_child(field(new_, v), off, node_idx, par, &geom, AM_INTERP_LAGRANGE, stream), "node_ctor");
+        }
+    }
+    void node_dtor(int) {}
+    void modify_nodes() {}
+
+    // derivative words of the state in `src` for the tiles of `m`; seeds are
+    // drawn per field in order (gradient+diagonal, then mixed), then one for
+    // the Christoffel words
+    void derivatives(smart_gpu_buffer<float> &src, const am_mesh &m)
+    {
+        am_fd_batch b{};
+        b.z_block = opt.z_block;
+        int h = 0, o = 0;
+        for (int v = 0; v < NV; v++) {
+            const int kind = System::FD_KIND[v];
+            if (kind == 0) continue;
+            const int i = b.n_vars++;
+            b.field[i] = field(src, v);
+            b.d1[i] = word(d1, v);
+            b.seed_d[i] = rng.generate_int();
+            if (kind >= 2) b.d2diag[i] = word(d2d, h++);
+            if (kind == 3) {
+                b.d2off[i] = word(d2o, o++);
+                b.seed_m[i] = rng.generate_int();
+            }
+        }
+        int ev = timer.begin(kernel_timer::FD);
+        check(am_fd_batched(&b, &m, &geom, stream), "fd");
+        timer.end(ev);
+        if (System::CHRISTOFFEL) {
+            const float *met[6];
+            const uint32_t *dg[6];
+            uint32_t *out[6];
+            for (int c = 0; c < 6; c++) {
+                met[c] = field(src, 1 + c);
+                dg[c] = word(d1, 1 + c);
+                out[c] = word(chr, c);
+            }
+            ev = timer.begin(kernel_timer::CHRISTOFFEL);
+            check(am_christoffel(met, dg, out, 0, &m, &geom, rng.generate_int(), stream), "christoffel");
+            timer.end(ev);
+        }
+    }
+
+    // copy-back: old <- new (+ restriction), keeping the previous old when a
+    // third time level is in use (AMR.h:998-1051)
+    void copy_back()
+    {
+        am_field_batch b = batch(true);
+        const am_mesh m = parent.mesh_all();
+        const int ev = timer.begin(kernel_timer::COPY_BACK);
+        check(am_copy_new_to_old_batched(&b, &m, &geom, stream), "copy_back");
+        timer.end(ev);
+    }
+
+    // one RK4 stage of every tile at depth d: derivative words of the stage
+    // input, RHS + update (+ equal-depth ghost push), coarse-face ghosts at
+    // the stage time
+    void stage_depth(int d, int s)
+    {
+        const am_mesh m = parent.mesh_depth(d);
+        if (m.n_tiles == 0) return;
+        static const float frac[4] = {.5f, .5f, 1.f, 1.f};
+        derivatives(s == 0 ? old_ : new_, m);
+        const rk4_views f = views(s == 0);
+        int ev = timer.begin(kernel_timer::RHS);
+        check(System::rhs_stage(f, s, sys, m, geom, 1, stream), "rhs");
+        timer.end(ev);
+        if (d > 0 && parent.coarse_faces_at_depth(d) > 0) {
+            am_field_batch b = batch(true);
+            ev = timer.begin(kernel_timer::GHOST);
+            check(am_ghost_fill_stage_batched(&b, &m, &geom, frac[s], opt.interp, AM_EQUAL_SKIP, stream), "ghost");
+            timer.end(ev);
+        }
+    }
+    void step_depth(int d)
+    {
+        for (int s = 0; s < 4; s++) stage_depth(d, s);
+        zone_cycles += (uint64_t)parent.tiles_at_depth(d) * inner;
+    }
+    void substep()
+    {
+        copy_back();
+        for (int d = 0; d <= parent.read_max_depth(); d++)
+            if (parent.depth_active(d)) step_depth(d);
+        parent.increment_timer();
+    }
+    // the reference's timestep(): sub-steps until the counter wraps
+    void timestep()
+    {
+        do {
+            substep();
+        } while (parent.timer_helper != 0u);
+    }
+    // fill every face ghost of `new` from the current data (initial data):
+    // equal-depth faces copy, coarser faces interpolate
+    void fill_ghosts()
+    {
+        for (int d = 1; d <= parent.read_max_depth(); d++) {
+            const am_mesh m = parent.mesh_depth(d);
+            if (m.n_tiles == 0) continue;
+            am_field_batch b = batch(false);
+            check(am_ghost_fill_stage_batched(&b, &m, &geom, 1.f, opt.interp, AM_EQUAL_COPY, stream), "ghost");
+        }
+    }
+    // the System's diagnostics of `new` over the leaves (FP64), four values
+    void diagnostics(double *out4)
+    {
+        const am_mesh m = parent.mesh_leaves();
+        derivatives(new_, m);
+        const rk4_views f = views(false);
+        System::diagnostics(f, sys, m, geom, partial.gpu_buffer_ptr, out4, stream);
+    }
+};
+
+}  // namespace am
