@@ -88,6 +88,15 @@ class BssnSimOptions(C.Structure):
                 ("seed", C.c_int32)]
 
 
+class SwaveFields(C.Structure):
+    _fields_ = [("state", C.c_void_p * 2), ("old_", C.c_void_p * 2), ("new_", C.c_void_p * 2),
+                ("acc", C.c_void_p * 2), ("d1_u", C.c_void_p), ("d2diag_u", C.c_void_p)]
+
+
+class SwaveSimOptions(C.Structure):
+    _fields_ = [("interp", C.c_int32), ("z_block", C.c_int32), ("periodic", C.c_int32), ("seed", C.c_int32)]
+
+
 BSSN_VARS = ["chi", "gt_xx", "gt_xy", "gt_xz", "gt_yy", "gt_yz", "gt_zz", "K", "At_xx", "At_xy", "At_xz", "At_yy",
              "At_yz", "At_zz", "Gt_x", "Gt_y", "Gt_z", "alpha", "beta_x", "beta_y", "beta_z", "B_x", "B_y", "B_z"]
 BSSN_HESS_VARS = (0, 1, 2, 3, 4, 5, 6, 17, 18, 19, 20)
@@ -118,6 +127,8 @@ _SIGS = {
     "am_copy_new_to_old": (C.c_int, [_P, _P, C.POINTER(Mesh), C.POINTER(Geometry), _P]),
     "am_copy_new_to_old_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), _P]),
     "am_copy_to_child": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int32, C.POINTER(Geometry), C.c_int32, _P]),
+    "am_copy_to_child_batched": (C.c_int, [C.POINTER(FieldBatch), C.c_uint32, C.c_int32, C.c_int32,
+                                           C.POINTER(Geometry), C.c_int32, _P]),
     "am_wave_predictor": (C.c_int, [_P, _P, _P, _P, C.POINTER(Mesh), C.POINTER(Geometry), _P]),
     "am_wave_corrector": (C.c_int, [_P, _P, _P, _P, C.POINTER(Mesh), C.POINTER(Geometry), _P]),
     "am_deriv_trace": (C.c_int, [_P, _P, C.POINTER(C.c_void_p), _P, C.POINTER(Mesh), C.POINTER(Geometry), _P]),
@@ -153,27 +164,39 @@ _SIGS = {
     "am_event_elapsed_ms": (C.c_int, [_P, _P, C.POINTER(C.c_float)]),
     "am_event_destroy": (C.c_int, [_P]),
     "am_bssn_sim_create": (_P, [C.POINTER(Geometry), C.c_uint32, C.POINTER(BssnSimOptions)]),
-    "am_bssn_sim_destroy": (None, [_P]),
-    "am_bssn_sim_add_node": (C.c_int, [_P, C.c_int32, C.c_int32]),
-    "am_bssn_sim_export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
-    "am_bssn_sim_set_field": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
-    "am_bssn_sim_get_field": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
-    "am_bssn_sim_device_ptr": (_P, [_P, C.c_int32]),
-    "am_bssn_sim_fill_ghosts": (C.c_int, [_P]),
-    "am_bssn_sim_copy_back": (C.c_int, [_P]),
-    "am_bssn_sim_step_depth": (C.c_int, [_P, C.c_int32]),
-    "am_bssn_sim_stage_depth": (C.c_int, [_P, C.c_int32, C.c_int32]),
-    "am_bssn_sim_substep": (C.c_int, [_P]),
-    "am_bssn_sim_timestep": (C.c_int, [_P]),
     "am_bssn_sim_constraints": (C.c_int, [_P, _P]),
-    "am_bssn_sim_sync": (C.c_int, [_P]),
-    "am_bssn_sim_zone_cycles": (C.c_uint64, [_P]),
-    "am_bssn_sim_rng_state": (C.c_int32, [_P]),
-    "am_bssn_sim_set_rng_state": (C.c_int, [_P, C.c_int32]),
-    "am_bssn_sim_counter": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]),
-    "am_bssn_sim_profile": (C.c_int, [_P, C.c_int32]),
-    "am_bssn_sim_kernel_times": (C.c_int, [_P, _P, _P]),
+    "am_swave_sim_create": (_P, [C.POINTER(Geometry), C.c_uint32, C.POINTER(SwaveSimOptions)]),
+    "am_swave_sim_energy": (C.c_int, [_P, _P]),
+    "am_swave_rhs_stage": (C.c_int, [C.POINTER(SwaveFields), C.c_int32, C.POINTER(Mesh), C.POINTER(Geometry),
+                                     C.c_int32, _P]),
+    "am_swave_energy": (C.c_int, [C.POINTER(SwaveFields), C.POINTER(Mesh), C.POINTER(Geometry), _P, _P]),
 }
+
+# the RK4 driver families share one call set (adaptivemesh_host.h)
+_RK4_SIM_SIGS = {
+    "destroy": (None, [_P]),
+    "add_node": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "export": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "set_field": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
+    "get_field": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
+    "device_ptr": (_P, [_P, C.c_int32]),
+    "fill_ghosts": (C.c_int, [_P]),
+    "copy_back": (C.c_int, [_P]),
+    "step_depth": (C.c_int, [_P, C.c_int32]),
+    "stage_depth": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "substep": (C.c_int, [_P]),
+    "timestep": (C.c_int, [_P]),
+    "sync": (C.c_int, [_P]),
+    "zone_cycles": (C.c_uint64, [_P]),
+    "rng_state": (C.c_int32, [_P]),
+    "set_rng_state": (C.c_int, [_P, C.c_int32]),
+    "counter": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]),
+    "profile": (C.c_int, [_P, C.c_int32]),
+    "kernel_times": (C.c_int, [_P, _P, _P]),
+}
+for _prefix in ("am_bssn_sim", "am_swave_sim"):
+    for _k, _v in _RK4_SIM_SIGS.items():
+        _SIGS[f"{_prefix}_{_k}"] = _v
 
 
 def exported_symbols() -> list[str]:
@@ -334,25 +357,31 @@ def bssn_options(eta=2.0, shift_coeff=0.75, chi_floor=1e-4, advect_shift=1, slic
 KERNEL_CLASSES = ("fd", "christoffel", "rhs_stage", "ghost", "copy_back")
 
 
-class BssnSim:
-    """AMR<BSSN_rk4_data>: 24-field BSSN with RK4 and sub-cycling, native."""
+class _Rk4Sim:
+    """Shared binding of the RK4 driver calls (adaptivemesh_host.h); PREFIX
+    names the exported family."""
+    PREFIX = ""
+    NV = 0
 
-    def __init__(self, geom: Geometry, max_slots: int, options: BssnSimOptions | None = None):
+    def _f(self, name):
+        return getattr(lib(), f"{self.PREFIX}_{name}")
+
+    def _create(self, geom: Geometry, max_slots: int, options):
         self.geom = geom
         self.max_slots = max_slots
-        self.options = options or bssn_options()
-        self.h = lib().am_bssn_sim_create(C.byref(geom), max_slots, C.byref(self.options))
+        self.options = options
+        self.h = self._f("create")(C.byref(geom), max_slots, C.byref(options))
         if not self.h:
             raise AMError(-8, lib().am_last_error_string().decode())
 
     def __del__(self):
         h = getattr(self, "h", None)
         if h and _lib is not None:
-            _lib.am_bssn_sim_destroy(h)
+            getattr(_lib, f"{self.PREFIX}_destroy")(h)
             self.h = None
 
     def add_node(self, parent: int, offset: int) -> int:
-        r = lib().am_bssn_sim_add_node(self.h, parent, offset)
+        r = self._f("add_node")(self.h, parent, offset)
         if r < 0:
             raise AMError(r, lib().am_last_error_string().decode())
         return r
@@ -362,7 +391,7 @@ class BssnSim:
         out = dict(parent=np.empty(n, np.int32), children=np.empty(8 * n, np.int32),
                    positions=np.empty(n, np.uint32), boundaries=np.empty(6 * n, np.uint64),
                    lifetime=np.empty(n, np.uint32), scalars=np.empty(4, np.int32))
-        check(lib().am_bssn_sim_export(self.h, *(a.ctypes.data for a in out.values())), "export")
+        check(self._f("export")(self.h, *(a.ctypes.data for a in out.values())), "export")
         s = out["scalars"]
         out.update(max_depth=int(s[0]), first_free=int(s[1]), final_used=int(s[2]), timer=int(s[3]))
         return out
@@ -370,64 +399,93 @@ class BssnSim:
     def set_field(self, which: int, var: int, arr: np.ndarray):
         a = np.ascontiguousarray(arr, dtype=np.float32)
         assert a.size == self.max_slots * self.geom.T ** 3
-        check(lib().am_bssn_sim_set_field(self.h, which, var, a.ctypes.data), "set_field")
+        check(self._f("set_field")(self.h, which, var, a.ctypes.data), "set_field")
 
     def get_field(self, which: int, var: int) -> np.ndarray:
         a = np.empty(self.max_slots * self.geom.T ** 3, np.float32)
-        check(lib().am_bssn_sim_get_field(self.h, which, var, a.ctypes.data), "get_field")
+        check(self._f("get_field")(self.h, which, var, a.ctypes.data), "get_field")
         return a
 
     def device_ptr(self, which: int) -> int:
-        return lib().am_bssn_sim_device_ptr(self.h, which)
+        return self._f("device_ptr")(self.h, which)
 
     def fill_ghosts(self):
-        check(lib().am_bssn_sim_fill_ghosts(self.h), "fill_ghosts")
+        check(self._f("fill_ghosts")(self.h), "fill_ghosts")
 
     def copy_back(self):
-        check(lib().am_bssn_sim_copy_back(self.h), "copy_back")
+        check(self._f("copy_back")(self.h), "copy_back")
 
     def step_depth(self, d: int):
-        check(lib().am_bssn_sim_step_depth(self.h, d), "step_depth")
+        check(self._f("step_depth")(self.h, d), "step_depth")
 
     def stage_depth(self, d: int, stage: int):
-        check(lib().am_bssn_sim_stage_depth(self.h, d, stage), "stage_depth")
+        check(self._f("stage_depth")(self.h, d, stage), "stage_depth")
 
     def substep(self):
-        check(lib().am_bssn_sim_substep(self.h), "substep")
+        check(self._f("substep")(self.h), "substep")
 
     def timestep(self):
-        check(lib().am_bssn_sim_timestep(self.h), "timestep")
+        check(self._f("timestep")(self.h), "timestep")
 
     def sync(self):
-        check(lib().am_bssn_sim_sync(self.h), "sync")
+        check(self._f("sync")(self.h), "sync")
+
+    @property
+    def zone_cycles(self) -> int:
+        return self._f("zone_cycles")(self.h)
+
+    @property
+    def rng_state(self) -> int:
+        return self._f("rng_state")(self.h)
+
+    @rng_state.setter
+    def rng_state(self, v: int):
+        check(self._f("set_rng_state")(self.h, v), "rng")
+
+    def counter(self):
+        t, d = C.c_uint32(), C.c_int32()
+        check(self._f("counter")(self.h, C.byref(t), C.byref(d)), "counter")
+        return t.value, d.value
+
+    def profile(self, on: bool):
+        check(self._f("profile")(self.h, 1 if on else 0), "profile")
+
+    def kernel_times(self) -> dict:
+        ms = np.zeros(len(KERNEL_CLASSES))
+        cnt = np.zeros(len(KERNEL_CLASSES), np.int64)
+        check(self._f("kernel_times")(self.h, ms.ctypes.data, cnt.ctypes.data), "kernel_times")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+
+class BssnSim(_Rk4Sim):
+    """AMR<BSSN_rk4_data>: 24-field BSSN with RK4 and sub-cycling, native."""
+    PREFIX = "am_bssn_sim"
+    NV = 24
+
+    def __init__(self, geom: Geometry, max_slots: int, options: BssnSimOptions | None = None):
+        self._create(geom, max_slots, options or bssn_options())
 
     def constraints(self) -> dict:
         out = np.zeros(4)
         check(lib().am_bssn_sim_constraints(self.h, out.ctypes.data), "constraints")
         return dict(H_l2=out[0], M_l2=out[1], H_max=out[2], cells=out[3])
 
-    @property
-    def zone_cycles(self) -> int:
-        return lib().am_bssn_sim_zone_cycles(self.h)
 
-    @property
-    def rng_state(self) -> int:
-        return lib().am_bssn_sim_rng_state(self.h)
+def swave_options(interp=AM_INTERP_LAGRANGE, z_block=0, periodic=True, seed=12894712) -> SwaveSimOptions:
+    o = SwaveSimOptions()
+    o.interp, o.z_block, o.periodic, o.seed = interp, z_block, 1 if periodic else 0, seed
+    return o
 
-    @rng_state.setter
-    def rng_state(self, v: int):
-        check(lib().am_bssn_sim_set_rng_state(self.h, v), "rng")
 
-    def counter(self):
-        t, d = C.c_uint32(), C.c_int32()
-        check(lib().am_bssn_sim_counter(self.h, C.byref(t), C.byref(d)), "counter")
-        return t.value, d.value
+class SwaveSim(_Rk4Sim):
+    """AMR<swave_rk4_data>: u_tt = lap u as (u, v) with RK4 and sub-cycling."""
+    PREFIX = "am_swave_sim"
+    NV = 2
 
-    def profile(self, on: bool):
-        check(lib().am_bssn_sim_profile(self.h, 1 if on else 0), "profile")
+    def __init__(self, geom: Geometry, max_slots: int, options: SwaveSimOptions | None = None):
+        self._create(geom, max_slots, options or swave_options())
 
-    def kernel_times(self) -> dict:
-        ms = np.zeros(len(KERNEL_CLASSES))
-        cnt = np.zeros(len(KERNEL_CLASSES), np.int64)
-        check(lib().am_bssn_sim_kernel_times(self.h, ms.ctypes.data, cnt.ctypes.data), "kernel_times")
-        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
+    def energy(self) -> dict:
+        out = np.zeros(4)
+        check(lib().am_swave_sim_energy(self.h, out.ctypes.data), "energy")
+        return dict(E=out[0], u2=out[1], u_max=out[2], cells=out[3])

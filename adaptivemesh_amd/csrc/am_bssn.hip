@@ -16,6 +16,7 @@
 // against ~1.5 kflop): bulk arithmetic is fp32 with FMA contraction, the
 // constraint sums are reduced in FP64.
 #include "am_device.hpp"
+#include "am_rk4.hpp"
 
 namespace am {
 
@@ -306,43 +307,8 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
     const size_t w = (size_t)node * S3 + cell;
     float k[NV];
     bssn_cell<true, false>(a, r, w, par, k, nullptr, nullptr);
-    const float dt = sc.dt[depth];
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-        const float yn = STAGE == 0 ? a.state[v][r] : a.old_[v][r];
-        float nv;
-        if (STAGE == 0) {
-            a.acc[v][r] = k[v];
-            nv = yn + (.5f * dt) * k[v];
-        } else if (STAGE == 1 || STAGE == 2) {
-            a.acc[v][r] = a.acc[v][r] + 2.f * k[v];
-            nv = yn + (STAGE == 1 ? .5f * dt : dt) * k[v];
-        } else {
-            nv = yn + (dt * (1.f / 6.f)) * (a.acc[v][r] + k[v]);
-        }
-        a.new_[v][r] = nv;
-        k[v] = nv;
-    }
-    if (PUSH) {
-        // copy into the face ghosts of equal-depth neighbours: a cell within P
-        // of face f lands in the opposite face slab of f's target, shifted by
-        // S cells along f's axis
-        const int c3[3] = {x, y, z};
-#pragma unroll
-        for (int f = 0; f < 6; f++) {
-            const int ax = f % 3;
-            const bool near = f < 3 ? c3[ax] >= S - P : c3[ax] < P;
-            if (!near) continue;
-            const uint32_t nd = (uint32_t)m.boundaries[(size_t)node * 6 + f];
-            const int tgt = bnd_target(nd);
-            if (tgt < 0 || bnd_depth(nd) <= 0 || bnd_ddiff(nd) != 0) continue;
-            int p3[3] = {x + P, y + P, z + P};
-            p3[ax] += f < 3 ? -S : S;
-            const size_t dst = (size_t)tgt * (T * T * T) + ((size_t)p3[2] * T + p3[1]) * T + p3[0];
-#pragma unroll
-            for (int v = 0; v < NV; v++) a.new_[v][dst] = k[v];
-        }
-    }
+    rk4_combine<NV, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, sc.dt[depth]);
+    if (PUSH) rk4_push<NV, S, P>(a.new_, k, m.boundaries, node, x, y, z);
 }
 
 // Per-workgroup FP64 partial sums {sum H^2, sum |M|^2, cells, max |H|}.

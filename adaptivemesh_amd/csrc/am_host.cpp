@@ -1,10 +1,12 @@
 // Native host runtime behind adaptivemesh_host.h: instantiates the drop-in
 // C++ API (include/adaptivemesh/*.hpp) and exports it as opaque handles.
 #include <new>
+#include <type_traits>
 #include <stdexcept>
 
 #include "../../include/adaptivemesh/AMR.hpp"
 #include "../../include/adaptivemesh/BSSN.hpp"
+#include "../../include/adaptivemesh/ScalarWave.hpp"
 #include "../../include/adaptivemesh/SimpleWave.hpp"
 #include "../../include/adaptivemesh_host.h"
 #include "am_common.hpp"
@@ -19,9 +21,13 @@ struct am_wave_sim {
     AMR<wave_AMR_data> *amr;
 };
 
-struct am_bssn_sim {
-    AMR<BSSN_rk4_data> *amr;
+// RK4 drivers: one handle type per System, one set of generic bodies
+template <class Data>
+struct rk4_sim {
+    AMR<Data> *amr;
 };
+struct am_bssn_sim : rk4_sim<BSSN_rk4_data> {};
+struct am_swave_sim : rk4_sim<swave_rk4_data> {};
 
 namespace {
 
@@ -59,6 +65,120 @@ int export_tree(const Tree &a, int32_t *parent, int32_t *children, uint32_t *pos
         scalars[2] = (int32_t)a.curr_used_slots() - 1;
         scalars[3] = (int32_t)a.timer_helper;
     }
+    return AM_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------- RK4 driver bodies
+// Shared by every System's handle (am_bssn_sim, am_swave_sim).
+namespace {
+
+template <class H, class Data, class SysOpt>
+H *rk4_create(const char *name, const am_geometry *g, uint32_t max_slots, int32_t interp, int32_t z_block,
+              int32_t periodic, int32_t seed, const SysOpt &so)
+{
+    if (!g || !am_geometry_supported(g)) {
+        am::set_error(AM_ERR_UNSUPPORTED_GEOMETRY, "%s: unsupported geometry", name);
+        return nullptr;
+    }
+    try {
+        am::rk4_options opt;
+        opt.interp = interp;
+        opt.z_block = z_block;
+        auto *s = new H;
+        s->amr = new AMR<Data>(max_slots, periodic != 0, *g, opt, so);
+        s->amr->sim_data.rng.seed = seed;
+        return s;
+    } catch (const std::exception &e) {
+        am::set_error(AM_ERR_ALLOC, "%s: %s", name, e.what());
+        return nullptr;
+    }
+}
+template <class H>
+void rk4_destroy(H *s)
+{
+    if (!s) return;
+    s->amr->sim_data.destroy();
+    s->amr->destroy();
+    delete s->amr;
+    delete s;
+}
+template <class H>
+int rk4_add_node(H *s, int32_t parent, int32_t offset)
+{
+    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
+    return guarded([&] {
+        const int r = s->amr->add_node(parent, offset);
+        if (r < 0) am::set_error(r, "add_node(%d, %d) refused", parent, offset);
+        return r;
+    });
+}
+template <class H>
+smart_gpu_buffer<float> *rk4_state_buffer(H *s, int which)
+{
+    auto &d = s->amr->sim_data;
+    switch (which) {
+    case 0: return &d.old_;
+    case 1: return &d.new_;
+    case 2: return &d.acc;
+    case 3: return d.older.created ? &d.older : nullptr;
+    default: return nullptr;
+    }
+}
+template <class H>
+int rk4_set_field(H *s, int32_t which, int32_t var, const float *host)
+{
+    using D = typename std::remove_reference<decltype(s->amr->sim_data)>::type;
+    if (!s || !host || var < 0 || var >= D::NV) return am::set_error(AM_ERR_INVALID, "set_field: bad argument");
+    auto *b = rk4_state_buffer(s, which);
+    if (!b) return am::set_error(AM_ERR_INVALID, "set_field: no buffer %d", which);
+    auto &d = s->amr->sim_data;
+    if (int rc = am_memcpy_h2d(d.field(*b, var), host, d.slots * d.cells * sizeof(float), d.stream)) return rc;
+    return am_stream_sync(d.stream);
+}
+template <class H>
+int rk4_get_field(H *s, int32_t which, int32_t var, float *host)
+{
+    using D = typename std::remove_reference<decltype(s->amr->sim_data)>::type;
+    if (!s || !host || var < 0 || var >= D::NV) return am::set_error(AM_ERR_INVALID, "get_field: bad argument");
+    auto *b = rk4_state_buffer(s, which);
+    if (!b) return am::set_error(AM_ERR_INVALID, "get_field: no buffer %d", which);
+    auto &d = s->amr->sim_data;
+    return am_memcpy_d2h(host, d.field(*b, var), d.slots * d.cells * sizeof(float), d.stream);
+}
+template <class H>
+void *rk4_device_ptr(H *s, int32_t which)
+{
+    if (!s) return nullptr;
+    auto &d = s->amr->sim_data;
+    if (which >= 0 && which <= 3) {
+        auto *b = rk4_state_buffer(s, which);
+        return b ? (void *)b->gpu_buffer_ptr : nullptr;
+    }
+    switch (which) {
+    case 4: return d.d1.gpu_buffer_ptr;
+    case 5: return d.d2d.gpu_buffer_ptr;
+    case 6: return d.d2o.gpu_buffer_ptr;
+    case 7: return d.chr.gpu_buffer_ptr;
+    default: return nullptr;
+    }
+}
+template <class H, class F>
+int rk4_call(H *s, F &&f)
+{
+    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
+    return guarded([&] {
+        f(s->amr->sim_data);
+        return AM_OK;
+    });
+}
+template <class H>
+int rk4_counter(const H *s, uint32_t *timer, int32_t *max_depth)
+{
+    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
+    if (timer) *timer = s->amr->timer_helper;
+    if (max_depth) *max_depth = s->amr->read_max_depth();
     return AM_OK;
 }
 
@@ -199,143 +319,100 @@ int am_wave_sim_timestep(am_wave_sim *s)
 }
 int32_t am_wave_sim_rng_state(const am_wave_sim *s) { return s ? s->amr->sim_data.rng.seed : 0; }
 
+// ----------------------------------------------------- RK4 driver exports
+#define AM_RK4_SIM_EXPORTS(P, H)                                                                                     \
+    void P##_destroy(H *s) { rk4_destroy(s); }                                                                     \
+    int P##_add_node(H *s, int32_t parent, int32_t offset) { return rk4_add_node(s, parent, offset); }             \
+    int P##_export(const H *s, int32_t *parent, int32_t *children, uint32_t *positions, uint64_t *boundaries,      \
+                   uint32_t *lifetime, int32_t *scalars)                                                           \
+    {                                                                                                              \
+        if (!s) return am::set_error(AM_ERR_INVALID, "null sim");                                                  \
+        return export_tree(*s->amr, parent, children, positions, boundaries, lifetime, scalars);                   \
+    }                                                                                                              \
+    int P##_set_field(H *s, int32_t which, int32_t var, const float *host)                                         \
+    {                                                                                                              \
+        return rk4_set_field(s, which, var, host);                                                                 \
+    }                                                                                                              \
+    int P##_get_field(H *s, int32_t which, int32_t var, float *host) { return rk4_get_field(s, which, var, host); } \
+    void *P##_device_ptr(H *s, int32_t which) { return rk4_device_ptr(s, which); }                                 \
+    int P##_fill_ghosts(H *s)                                                                                      \
+    {                                                                                                              \
+        return rk4_call(s, [](auto &d) { d.fill_ghosts(); });                                                      \
+    }                                                                                                              \
+    int P##_copy_back(H *s)                                                                                        \
+    {                                                                                                              \
+        return rk4_call(s, [](auto &d) { d.copy_back(); });                                                        \
+    }                                                                                                              \
+    int P##_step_depth(H *s, int32_t depth)                                                                        \
+    {                                                                                                              \
+        return rk4_call(s, [&](auto &d) { d.step_depth(depth); });                                                 \
+    }                                                                                                              \
+    int P##_stage_depth(H *s, int32_t depth, int32_t stage)                                                        \
+    {                                                                                                              \
+        if (stage < 0 || stage > 3) return am::set_error(AM_ERR_INVALID, "stage %d", stage);                       \
+        return rk4_call(s, [&](auto &d) { d.stage_depth(depth, stage); });                                         \
+    }                                                                                                              \
+    int P##_substep(H *s)                                                                                          \
+    {                                                                                                              \
+        return rk4_call(s, [](auto &d) { d.substep(); });                                                          \
+    }                                                                                                              \
+    int P##_timestep(H *s)                                                                                         \
+    {                                                                                                              \
+        return rk4_call(s, [](auto &d) { d.timestep(); });                                                         \
+    }                                                                                                              \
+    int P##_sync(H *s)                                                                                             \
+    {                                                                                                              \
+        if (!s) return am::set_error(AM_ERR_INVALID, "null sim");                                                  \
+        return am_stream_sync(s->amr->sim_data.stream);                                                            \
+    }                                                                                                              \
+    uint64_t P##_zone_cycles(const H *s) { return s ? s->amr->sim_data.zone_cycles : 0; }                          \
+    int32_t P##_rng_state(const H *s) { return s ? s->amr->sim_data.rng.seed : 0; }                                \
+    int P##_set_rng_state(H *s, int32_t seed)                                                                      \
+    {                                                                                                              \
+        return rk4_call(s, [&](auto &d) { d.rng.seed = seed; });                                                   \
+    }                                                                                                              \
+    int P##_counter(const H *s, uint32_t *timer, int32_t *max_depth) { return rk4_counter(s, timer, max_depth); }  \
+    int P##_profile(H *s, int32_t enable)                                                                          \
+    {                                                                                                              \
+        return rk4_call(s, [&](auto &d) { d.timer.enable(enable != 0, d.stream); });                              \
+    }                                                                                                              \
+    int P##_kernel_times(H *s, double *ms5, int64_t *count5)                                                       \
+    {                                                                                                              \
+        if (!ms5 || !count5) return am::set_error(AM_ERR_INVALID, "null output");                                  \
+        return rk4_call(s, [&](auto &d) { d.timer.totals(ms5, count5); });                                         \
+    }
+
+AM_RK4_SIM_EXPORTS(am_bssn_sim, am_bssn_sim)
+AM_RK4_SIM_EXPORTS(am_swave_sim, am_swave_sim)
+
 am_bssn_sim *am_bssn_sim_create(const am_geometry *g, uint32_t max_slots, const am_bssn_sim_options *o)
 {
-    if (!g || !o || !am_geometry_supported(g)) {
-        am::set_error(AM_ERR_UNSUPPORTED_GEOMETRY, "am_bssn_sim_create: unsupported geometry");
+    if (!o) {
+        am::set_error(AM_ERR_INVALID, "am_bssn_sim_create: null options");
         return nullptr;
     }
-    try {
-        am::rk4_options opt;
-        opt.interp = o->interp;
-        opt.z_block = o->z_block;
-        auto *s = new am_bssn_sim;
-        s->amr = new AMR<BSSN_rk4_data>(max_slots, o->periodic != 0, *g, opt, o->params);
-        s->amr->sim_data.rng.seed = o->seed;
-        return s;
-    } catch (const std::exception &e) {
-        am::set_error(AM_ERR_ALLOC, "am_bssn_sim_create: %s", e.what());
-        return nullptr;
-    }
+    return rk4_create<am_bssn_sim, BSSN_rk4_data>("am_bssn_sim_create", g, max_slots, o->interp, o->z_block,
+                                                  o->periodic, o->seed, o->params);
 }
-void am_bssn_sim_destroy(am_bssn_sim *s)
-{
-    if (!s) return;
-    s->amr->sim_data.destroy();
-    s->amr->destroy();
-    delete s->amr;
-    delete s;
-}
-int am_bssn_sim_add_node(am_bssn_sim *s, int32_t parent, int32_t offset)
-{
-    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
-    return guarded([&] {
-        const int r = s->amr->add_node(parent, offset);
-        if (r < 0) am::set_error(r, "add_node(%d, %d) refused", parent, offset);
-        return r;
-    });
-}
-int am_bssn_sim_export(const am_bssn_sim *s, int32_t *parent, int32_t *children, uint32_t *positions,
-                       uint64_t *boundaries, uint32_t *lifetime, int32_t *scalars)
-{
-    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
-    return export_tree(*s->amr, parent, children, positions, boundaries, lifetime, scalars);
-}
-static smart_gpu_buffer<float> *bssn_state_buffer(am_bssn_sim *s, int which)
-{
-    auto &d = s->amr->sim_data;
-    switch (which) {
-    case 0: return &d.old_;
-    case 1: return &d.new_;
-    case 2: return &d.acc;
-    case 3: return d.older.created ? &d.older : nullptr;
-    default: return nullptr;
-    }
-}
-int am_bssn_sim_set_field(am_bssn_sim *s, int32_t which, int32_t var, const float *host)
-{
-    if (!s || !host || var < 0 || var >= AM_BSSN_NVARS) return am::set_error(AM_ERR_INVALID, "am_bssn_sim_set_field");
-    auto *b = bssn_state_buffer(s, which);
-    if (!b) return am::set_error(AM_ERR_INVALID, "am_bssn_sim_set_field: no buffer %d", which);
-    auto &d = s->amr->sim_data;
-    if (int rc = am_memcpy_h2d(d.field(*b, var), host, d.slots * d.cells * sizeof(float), d.stream)) return rc;
-    return am_stream_sync(d.stream);
-}
-int am_bssn_sim_get_field(am_bssn_sim *s, int32_t which, int32_t var, float *host)
-{
-    if (!s || !host || var < 0 || var >= AM_BSSN_NVARS) return am::set_error(AM_ERR_INVALID, "am_bssn_sim_get_field");
-    auto *b = bssn_state_buffer(s, which);
-    if (!b) return am::set_error(AM_ERR_INVALID, "am_bssn_sim_get_field: no buffer %d", which);
-    auto &d = s->amr->sim_data;
-    return am_memcpy_d2h(host, d.field(*b, var), d.slots * d.cells * sizeof(float), d.stream);
-}
-void *am_bssn_sim_device_ptr(am_bssn_sim *s, int32_t which)
-{
-    if (!s) return nullptr;
-    auto &d = s->amr->sim_data;
-    if (which >= 0 && which <= 3) {
-        auto *b = bssn_state_buffer(s, which);
-        return b ? (void *)b->gpu_buffer_ptr : nullptr;
-    }
-    switch (which) {
-    case 4: return d.d1.gpu_buffer_ptr;
-    case 5: return d.d2d.gpu_buffer_ptr;
-    case 6: return d.d2o.gpu_buffer_ptr;
-    case 7: return d.chr.gpu_buffer_ptr;
-    default: return nullptr;
-    }
-}
-#define BSSN_CALL(expr)                                             \
-    do {                                                            \
-        if (!s) return am::set_error(AM_ERR_INVALID, "null sim");    \
-        return guarded([&] {                                        \
-            expr;                                                   \
-            return AM_OK;                                           \
-        });                                                         \
-    } while (0)
-int am_bssn_sim_fill_ghosts(am_bssn_sim *s) { BSSN_CALL(s->amr->sim_data.fill_ghosts()); }
-int am_bssn_sim_copy_back(am_bssn_sim *s) { BSSN_CALL(s->amr->sim_data.copy_back()); }
-int am_bssn_sim_step_depth(am_bssn_sim *s, int32_t depth) { BSSN_CALL(s->amr->sim_data.step_depth(depth)); }
-int am_bssn_sim_stage_depth(am_bssn_sim *s, int32_t depth, int32_t stage)
-{
-    if (stage < 0 || stage > 3) return am::set_error(AM_ERR_INVALID, "stage %d", stage);
-    BSSN_CALL(s->amr->sim_data.stage_depth(depth, stage));
-}
-int am_bssn_sim_substep(am_bssn_sim *s) { BSSN_CALL(s->amr->sim_data.substep()); }
-int am_bssn_sim_timestep(am_bssn_sim *s) { BSSN_CALL(s->amr->sim_data.timestep()); }
 int am_bssn_sim_constraints(am_bssn_sim *s, double *out4)
 {
     if (!out4) return am::set_error(AM_ERR_INVALID, "null output");
-    BSSN_CALL(s->amr->sim_data.diagnostics(out4));
+    return rk4_call(s, [&](auto &d) { d.diagnostics(out4); });
 }
-int am_bssn_sim_sync(am_bssn_sim *s)
+
+am_swave_sim *am_swave_sim_create(const am_geometry *g, uint32_t max_slots, const am_swave_sim_options *o)
 {
-    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
-    return am_stream_sync(s->amr->sim_data.stream);
+    if (!o) {
+        am::set_error(AM_ERR_INVALID, "am_swave_sim_create: null options");
+        return nullptr;
+    }
+    return rk4_create<am_swave_sim, swave_rk4_data>("am_swave_sim_create", g, max_slots, o->interp, o->z_block,
+                                                    o->periodic, o->seed, swave_options{});
 }
-uint64_t am_bssn_sim_zone_cycles(const am_bssn_sim *s) { return s ? s->amr->sim_data.zone_cycles : 0; }
-int32_t am_bssn_sim_rng_state(const am_bssn_sim *s) { return s ? s->amr->sim_data.rng.seed : 0; }
-int am_bssn_sim_set_rng_state(am_bssn_sim *s, int32_t seed)
+int am_swave_sim_energy(am_swave_sim *s, double *out4)
 {
-    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
-    s->amr->sim_data.rng.seed = seed;
-    return AM_OK;
-}
-int am_bssn_sim_profile(am_bssn_sim *s, int32_t enable)
-{
-    BSSN_CALL(s->amr->sim_data.timer.enable(enable != 0, s->amr->sim_data.stream));
-}
-int am_bssn_sim_kernel_times(am_bssn_sim *s, double *ms5, int64_t *count5)
-{
-    if (!ms5 || !count5) return am::set_error(AM_ERR_INVALID, "null output");
-    BSSN_CALL(s->amr->sim_data.timer.totals(ms5, count5));
-}
-int am_bssn_sim_counter(const am_bssn_sim *s, uint32_t *timer, int32_t *max_depth)
-{
-    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
-    if (timer) *timer = s->amr->timer_helper;
-    if (max_depth) *max_depth = s->amr->read_max_depth();
-    return AM_OK;
+    if (!out4) return am::set_error(AM_ERR_INVALID, "null output");
+    return rk4_call(s, [&](auto &d) { d.diagnostics(out4); });
 }
 
 }  // extern "C"

@@ -24,6 +24,11 @@ struct FieldBatchDev {
     const float *older[AM_MAX_VARS];
 };
 
+// padded-tile buffers prolonged by one k_copy_to_child launch
+struct ChildBufs {
+    float *buf[3 * AM_MAX_VARS];
+};
+
 // __interpolate_dat, quadratic 10-point form (AMR.h:673-699)
 template <int T>
 __device__ __forceinline__ float interp_lagrange(const float *__restrict__ d, float rx, float ry, float rz, size_t n)
@@ -475,10 +480,11 @@ static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, void *
 
 // __copy_to_child (AMR.h:1053-1073)
 template <int S, int P, int MODE>
-__global__ void __launch_bounds__(256) k_copy_to_child(float *buf, uint32_t offset, int child, int parent)
+__global__ void __launch_bounds__(256) k_copy_to_child(ChildBufs bufs, uint32_t offset, int child, int parent)
 {
     constexpr int T = S + 2 * P, T2 = T * T;
     constexpr size_t CELLS = (size_t)T * T2;
+    float *buf = bufs.buf[blockIdx.y];
     const int z = blockIdx.x;
     const float invS = 1.f / (float)S;
     const float off[3] = {(float)(offset & 1u), (float)((offset & 2u) >> 1), (float)((offset & 4u) >> 2)};
@@ -503,16 +509,19 @@ __global__ void __launch_bounds__(256) k_copy_to_child(float *buf, uint32_t offs
     }
 }
 
+// one launch for n buffers (grid.y = n): each buffer is prolonged from its
+// own parent tile
 template <int S, int P>
-static int launch_to_child(float *buf, uint32_t offset, int child, int parent, int interp, void *stream)
+static int launch_to_child(const ChildBufs &bufs, int n, uint32_t offset, int child, int parent, int interp,
+                           void *stream)
 {
     constexpr int T = S + 2 * P;
     if (interp == AM_INTERP_TRILINEAR)
-        hipLaunchKernelGGL((k_copy_to_child<S, P, AM_INTERP_TRILINEAR>), dim3(T), dim3(256), 0, (hipStream_t)stream,
-                           buf, offset, child, parent);
+        hipLaunchKernelGGL((k_copy_to_child<S, P, AM_INTERP_TRILINEAR>), dim3(T, n), dim3(256), 0,
+                           (hipStream_t)stream, bufs, offset, child, parent);
     else
-        hipLaunchKernelGGL((k_copy_to_child<S, P, AM_INTERP_LAGRANGE>), dim3(T), dim3(256), 0, (hipStream_t)stream,
-                           buf, offset, child, parent);
+        hipLaunchKernelGGL((k_copy_to_child<S, P, AM_INTERP_LAGRANGE>), dim3(T, n), dim3(256), 0,
+                           (hipStream_t)stream, bufs, offset, child, parent);
     return check_launch("k_copy_to_child");
 }
 
@@ -586,12 +595,39 @@ extern "C" int am_copy_new_to_old(float *old_, const float *new_, const am_mesh 
     return am_copy_new_to_old_batched(&b, mesh, g, stream);
 }
 
-extern "C" int am_copy_to_child(float *buf, uint32_t offset, int32_t child, int32_t parent, const am_geometry *g,
-                                int32_t interp, void *stream)
+static int to_child_checks(uint32_t offset, int32_t child, int32_t parent, const am_geometry *g, int32_t interp)
 {
-    if (!buf || !g) return am::set_error(AM_ERR_INVALID, "am_copy_to_child: null buffer or geometry");
+    if (!g) return am::set_error(AM_ERR_INVALID, "am_copy_to_child: null geometry");
     if (offset > 7u || child < 0 || parent < 0) return am::set_error(AM_ERR_INVALID, "am_copy_to_child: bad indices");
     if (interp != AM_INTERP_LAGRANGE && interp != AM_INTERP_TRILINEAR)
         return am::set_error(AM_ERR_INVALID, "am_copy_to_child: interpolation %d not supported", interp);
-    AM_DISPATCH_TILE(*g, am::launch_to_child, buf, offset, child, parent, interp, stream);
+    return AM_OK;
+}
+
+extern "C" int am_copy_to_child(float *buf, uint32_t offset, int32_t child, int32_t parent, const am_geometry *g,
+                                int32_t interp, void *stream)
+{
+    if (!buf) return am::set_error(AM_ERR_INVALID, "am_copy_to_child: null buffer");
+    if (int rc = to_child_checks(offset, child, parent, g, interp)) return rc;
+    am::ChildBufs b{};
+    b.buf[0] = buf;
+    AM_DISPATCH_TILE(*g, am::launch_to_child, b, 1, offset, child, parent, interp, stream);
+}
+
+extern "C" int am_copy_to_child_batched(const am_field_batch *fb, uint32_t offset, int32_t child, int32_t parent,
+                                        const am_geometry *g, int32_t interp, void *stream)
+{
+    if (!fb) return am::set_error(AM_ERR_INVALID, "am_copy_to_child_batched: null batch");
+    if (fb->n_vars < 0 || fb->n_vars > AM_MAX_VARS)
+        return am::set_error(AM_ERR_TOO_MANY_VARS, "am_copy_to_child_batched: n_vars %d", fb->n_vars);
+    if (int rc = to_child_checks(offset, child, parent, g, interp)) return rc;
+    am::ChildBufs b{};
+    int n = 0;
+    for (int v = 0; v < fb->n_vars; v++) {
+        if (fb->old_[v]) b.buf[n++] = fb->old_[v];
+        if (fb->new_[v]) b.buf[n++] = fb->new_[v];
+        if (fb->older[v]) b.buf[n++] = const_cast<float *>(fb->older[v]);
+    }
+    if (n == 0) return AM_OK;
+    AM_DISPATCH_TILE(*g, am::launch_to_child, b, n, offset, child, parent, interp, stream);
 }

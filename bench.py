@@ -130,6 +130,29 @@ def cpu_baseline(n_slots, fields, positions, tiles_all, budget_s=12.0):
                       "ghost fill not included"}
 
 
+def copy_probe(mib=1024, reps=10):
+    """A plain device-to-device copy of `mib` MiB: the reference point for
+    the HBM byte counters (it reads and writes exactly that many bytes) and
+    the achievable copy bandwidth."""
+    import torch
+    n = mib * 2 ** 20 // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_probe": {"bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+                                     "avg_us": round(1e3 * ms, 2),
+                                     "GBps": round(8 * n / (ms * 1e-3) / 1e9, 1)}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,11 +161,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing pass")
     ap.add_argument("--z-block", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="time only a plain 1 GiB device copy (the calibration for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU: the multi-GPU sharding of this path is a design only "
-              "(see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU: sharding this path over several GPUs is a design only; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import adaptivemesh_amd as am
     import torch

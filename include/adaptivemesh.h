@@ -179,6 +179,11 @@ int am_copy_new_to_old_batched(const am_field_batch *b, const am_mesh *mesh, con
  * AMR.h:1075-1082.  interp: AM_INTERP_LAGRANGE or AM_INTERP_TRILINEAR. */
 int am_copy_to_child(float *buf, uint32_t offset, int32_t child, int32_t parent, const am_geometry *g,
                      int32_t interp, void *stream);
+/* am_copy_to_child for every non-null old_, new_ and older buffer of the
+ * batch in one launch (the prolongation a new tile needs for all its fields,
+ * AMR.h:1075-1115 called once per field and buffer). */
+int am_copy_to_child_batched(const am_field_batch *b, uint32_t offset, int32_t child, int32_t parent,
+                             const am_geometry *g, int32_t interp, void *stream);
 
 /* ------------------------------------------------------ SimpleWave */
 
@@ -266,6 +271,32 @@ int am_bssn_rhs_stage(const am_bssn_fields *f, int32_t stage, const am_bssn_para
 int am_bssn_constraints(const am_bssn_fields *f, const am_bssn_params *p, const am_mesh *mesh, const am_geometry *g,
                         double *partial, void *stream);
 int64_t am_bssn_constraint_blocks(const am_geometry *g, int32_t n_tiles);
+
+/* ------------------------------------------------------ scalar wave, RK4
+ * No counterpart in the reference, whose SimpleWave.h plugin is a Heun step
+ * of a Burgers-type equation (SimpleWave.h:7-72): the wave equation
+ * u_tt = lap u as the first-order system u_t = v, v_t = lap u, advanced by
+ * the same RK4 stage update and equal-depth push as am_bssn_rhs_stage.  The
+ * Laplacian is the sum of the components of u's d2diag word (the FD kernel's
+ * second-derivative output). */
+typedef struct am_swave_fields {
+    const float *state[2]; /* u, v: stage input */
+    const float *old_[2];
+    float *new_[2];
+    float *acc[2];
+    const uint32_t *d1_u;     /* gradient word of the stage input u */
+    const uint32_t *d2diag_u; /* (u_xx, u_yy, u_zz) word of the stage input u */
+} am_swave_fields;
+
+int am_swave_rhs_stage(const am_swave_fields *f, int32_t stage, const am_mesh *mesh, const am_geometry *g,
+                       int32_t push_ghosts, void *stream);
+
+/* Energy of `state` on every listed tile, reduced in FP64 to one
+ * {sum (v^2 + |grad u|^2) h^3 / 2, sum u^2 h^3, cells, max |u|} quadruple per
+ * workgroup in `partial` (4 * am_bssn_constraint_blocks() doubles; h is the
+ * tile's cell size outer_dx / 2^depth, grad u comes from d1_u). */
+int am_swave_energy(const am_swave_fields *f, const am_mesh *mesh, const am_geometry *g, double *partial,
+                    void *stream);
 
 #ifdef __cplusplus
 }

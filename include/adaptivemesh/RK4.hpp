@@ -179,10 +179,8 @@ struct rk4_AMR_data {
     {
         const uint32_t off = (uint32_t)parent.positions.cpu_buffer_ptr[node_idx].final_offset();
         const int par = parent.parent_idx_b.cpu_buffer_ptr[node_idx];
-        for (int v = 0; v < NV; v++) {
-            check(am_copy_to_child(field(old_, v), off, node_idx, par, &geom, AM_INTERP_LAGRANGE, stream), "node_ctor");
-            check(am_copy_to_child(field(new_, v), off, node_idx, par, &geom, AM_INTERP_LAGRANGE, stream), "node_ctor");
-        }
+        const am_field_batch b = batch(true);
+        check(am_copy_to_child_batched(&b, off, node_idx, par, &geom, AM_INTERP_LAGRANGE, stream), "node_ctor");
     }
     void node_dtor(int) {}
     void modify_nodes() {}
@@ -287,13 +285,17 @@ struct rk4_AMR_data {
             check(am_ghost_fill_stage_batched(&b, &m, &geom, 1.f, opt.interp, AM_EQUAL_COPY, stream), "ghost");
         }
     }
-    // the System's diagnostics of `new` over the leaves (FP64), four values
+    // the System's diagnostics of `new` over the leaves (FP64), four values;
+    // the dither draws they use are given back, so measuring does not change
+    // the trajectory
     void diagnostics(double *out4)
     {
+        const int32_t seed = rng.seed;
         const am_mesh m = parent.mesh_leaves();
         derivatives(new_, m);
         const rk4_views f = views(false);
         System::diagnostics(f, sys, m, geom, partial.gpu_buffer_ptr, out4, stream);
+        rng.seed = seed;
     }
 };
 

@@ -101,6 +101,47 @@ int am_bssn_sim_counter(const am_bssn_sim *s, uint32_t *timer, int32_t *max_dept
 int am_bssn_sim_profile(am_bssn_sim *s, int32_t enable);
 int am_bssn_sim_kernel_times(am_bssn_sim *s, double *ms5, int64_t *count5);
 
+/* -------------------------------------------- scalar wave RK4 driver
+ * AMR<swave_rk4_data>: u_tt = lap u as fields (u, v = u_t) on the same RK4
+ * sub-cycling driver as the BSSN system (no counterpart in the reference:
+ * its SimpleWave.h plugin is the Heun Burgers step behind am_wave_sim).
+ * Every call means what the am_bssn_sim call of the same name means; var is
+ * 0 (u) or 1 (v); device_ptr(4) is u's gradient word, (5) its d2diag word. */
+typedef struct am_swave_sim am_swave_sim;
+
+typedef struct am_swave_sim_options {
+    int32_t interp;   /* AM_INTERP_LAGRANGE or AM_INTERP_TRICUBIC */
+    int32_t z_block;  /* FD workgroup z extent (0 = whole tile) */
+    int32_t periodic; /* outer faces wrap */
+    int32_t seed;     /* host fast_prng seed for the dither draws */
+} am_swave_sim_options;
+
+am_swave_sim *am_swave_sim_create(const am_geometry *g, uint32_t max_slots, const am_swave_sim_options *o);
+void am_swave_sim_destroy(am_swave_sim *s);
+int am_swave_sim_add_node(am_swave_sim *s, int32_t parent, int32_t offset);
+int am_swave_sim_export(const am_swave_sim *s, int32_t *parent, int32_t *children, uint32_t *positions,
+                        uint64_t *boundaries, uint32_t *lifetime, int32_t *scalars);
+int am_swave_sim_set_field(am_swave_sim *s, int32_t which, int32_t var, const float *host);
+int am_swave_sim_get_field(am_swave_sim *s, int32_t which, int32_t var, float *host);
+void *am_swave_sim_device_ptr(am_swave_sim *s, int32_t which);
+int am_swave_sim_fill_ghosts(am_swave_sim *s);
+int am_swave_sim_copy_back(am_swave_sim *s);
+int am_swave_sim_step_depth(am_swave_sim *s, int32_t depth);
+int am_swave_sim_stage_depth(am_swave_sim *s, int32_t depth, int32_t stage);
+int am_swave_sim_substep(am_swave_sim *s);
+int am_swave_sim_timestep(am_swave_sim *s);
+/* {energy = sum (v^2 + |grad u|^2) h^3 / 2, sum u^2 h^3, max |u|, cells}
+ * over the leaves (FP64; grad u from dithered gradient words whose draws are
+ * given back, so the trajectory does not depend on how often it is called) */
+int am_swave_sim_energy(am_swave_sim *s, double *out4);
+int am_swave_sim_sync(am_swave_sim *s);
+uint64_t am_swave_sim_zone_cycles(const am_swave_sim *s);
+int32_t am_swave_sim_rng_state(const am_swave_sim *s);
+int am_swave_sim_set_rng_state(am_swave_sim *s, int32_t seed);
+int am_swave_sim_counter(const am_swave_sim *s, uint32_t *timer, int32_t *max_depth);
+int am_swave_sim_profile(am_swave_sim *s, int32_t enable);
+int am_swave_sim_kernel_times(am_swave_sim *s, double *ms5, int64_t *count5);
+
 #ifdef __cplusplus
 }
 #endif

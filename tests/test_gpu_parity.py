@@ -205,6 +205,35 @@ def test_copy_to_child_bit_exact(am, orc, torch_cuda, mode):
     assert np.array_equal(host(d, np.float32).view(np.uint32), buf.view(np.uint32))
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_copy_to_child_batched_bit_exact(am, orc, torch_cuda, mode):
+    """One launch prolonging every old/new/older buffer of a 5-field batch
+    equals the oracle applied buffer by buffer."""
+    torch = torch_cuda
+    S, P = 16, 3
+    g = am.geometry(S, P, 4, 64.0, 0.25)
+    T = S + 2 * P
+    n, nv = 9, 5
+    rng = np.random.default_rng(10 + mode)
+    bufs = {(k, v): rng.standard_normal(n * T ** 3).astype(np.float32) for k in range(3) for v in range(nv)}
+    devs = {key: dev(torch, b) for key, b in bufs.items()}
+    fb = am.FieldBatch()
+    fb.n_vars = nv
+    for v in range(nv):
+        fb.old_[v] = devs[(0, v)].data_ptr()
+        fb.new_[v] = devs[(1, v)].data_ptr()
+        fb.older[v] = devs[(2, v)].data_ptr() if v % 2 == 0 else None
+    child, parent, off = 5, 0, 4
+    am.check(am.lib().am_copy_to_child_batched(C.byref(fb), off, child, parent, C.byref(g), mode,
+                                               am.stream_handle()), "ctc batched")
+    sync(torch)
+    for (k, v), b in bufs.items():
+        ref = b.copy()
+        if k < 2 or v % 2 == 0:
+            orc.copy_to_child(ref, off, child, parent, mode, S, P)
+        assert np.array_equal(host(devs[(k, v)], np.float32).view(np.uint32), ref.view(np.uint32)), (k, v)
+
+
 def test_pointwise_bit_exact(am, orc, torch_cuda):
     """SimpleWave stages, deriv_trace and both Christoffel modes."""
     torch = torch_cuda
