@@ -4,11 +4,12 @@
 // (chi form, 1+log or harmonic slicing, Gamma-driver shift with B^i), the
 // same system the FP64 oracle states term by term.
 //
-// Inputs per interior cell: 24 fp32 state values (stage input), the
+// Inputs per interior cell: 24 fp32 state values (stage input) and the
 // compressed_float3 words written by the FD kernel (a gradient for every
-// field, a full Hessian for chi, gt_ij, alpha, beta^i) and six Christoffel
-// words Gt^i_jk.  The lowered symbols Gt_ijk are rebuilt from the metric
-// gradient words, so they carry one quantisation, not two.
+// field, a full Hessian for chi, gt_ij, alpha, beta^i).  The lowered
+// symbols Gt_ijk come from the metric gradient words and are raised in
+// registers (Gt^i_jk = gt^il Gt_ljk): no Christoffel words are read, so the
+// symbols carry one quantisation (that of the gradient words), not two.
 //
 // One thread per interior cell, 256 cells per workgroup, every array read
 // as contiguous runs (words: 1 KB per workgroup per array; state: 16-float
@@ -30,7 +31,6 @@ struct BssnArgsDev {
     const uint32_t *d1[NV];
     const uint32_t *d2d[NH];
     const uint32_t *d2o[NH];
-    const uint32_t *chr[6];
 };
 
 struct BssnParDev {
@@ -106,11 +106,10 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, size_t r, size_t
 #pragma unroll
             for (int k = j; k < 3; k++) Gl[i][SI[j][k]] = .5f * (dg[SI[i][j]][k] + dg[SI[i][k]][j] - dg[SI[j][k]][i]);
 #pragma unroll
-    for (int c = 0; c < 6; c++) {
-        float t[3];
-        dec3(a.chr[c][w], t);
-        G[0][c] = t[0]; G[1][c] = t[1]; G[2][c] = t[2];
-    }
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+            G[i][c] = gu[SI[i][0]] * Gl[0][c] + gu[SI[i][1]] * Gl[1][c] + gu[SI[i][2]] * Gl[2][c];
     float Gd[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) Gd[i] = sym_dot(gu, G[i]);
@@ -307,7 +306,7 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
     const size_t w = (size_t)node * S3 + cell;
     float k[NV];
     bssn_cell<true, false>(a, r, w, par, k, nullptr, nullptr);
-    rk4_combine<NV, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, sc.dt[depth]);
+    rk4_combine<NV, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
     if (PUSH) rk4_push<NV, S, P>(a.new_, k, m.boundaries, node, x, y, z);
 }
 
@@ -371,10 +370,6 @@ static int fill_args(const am_bssn_fields *f, BssnArgsDev &a, bool need_stage)
         a.d2d[h] = f->d2diag[h];
         a.d2o[h] = f->d2off[h];
         if (!a.d2d[h] || !a.d2o[h]) return set_error(AM_ERR_INVALID, "am_bssn: null Hessian word %d", h);
-    }
-    for (int c = 0; c < 6; c++) {
-        a.chr[c] = f->chr[c];
-        if (!a.chr[c]) return set_error(AM_ERR_INVALID, "am_bssn: null Christoffel word %d", c);
     }
     return AM_OK;
 }

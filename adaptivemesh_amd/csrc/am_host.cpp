@@ -2,6 +2,8 @@
 // C++ API (include/adaptivemesh/*.hpp) and exports it as opaque handles.
 #include <new>
 #include <type_traits>
+#include <vector>
+#include <string.h>
 #include <stdexcept>
 
 #include "../../include/adaptivemesh/AMR.hpp"
@@ -126,6 +128,29 @@ smart_gpu_buffer<float> *rk4_state_buffer(H *s, int which)
     default: return nullptr;
     }
 }
+// interior cells of padded host tiles, and back (ghost cells zero)
+std::vector<float> interior_of(const float *padded, size_t slots, const am_geometry &g)
+{
+    const size_t S = (size_t)g.size_domain, P = (size_t)g.padding_domain, T = S + 2 * P;
+    std::vector<float> out(slots * S * S * S);
+    for (size_t s = 0; s < slots; s++)
+        for (size_t z = 0; z < S; z++)
+            for (size_t y = 0; y < S; y++)
+                memcpy(&out[((s * S + z) * S + y) * S], &padded[((s * T + z + P) * T + y + P) * T + P],
+                       S * sizeof(float));
+    return out;
+}
+void pad_from_interior(const std::vector<float> &in, float *padded, size_t slots, const am_geometry &g)
+{
+    const size_t S = (size_t)g.size_domain, P = (size_t)g.padding_domain, T = S + 2 * P;
+    memset(padded, 0, slots * T * T * T * sizeof(float));
+    for (size_t s = 0; s < slots; s++)
+        for (size_t z = 0; z < S; z++)
+            for (size_t y = 0; y < S; y++)
+                memcpy(&padded[((s * T + z + P) * T + y + P) * T + P], &in[((s * S + z) * S + y) * S],
+                       S * sizeof(float));
+}
+
 template <class H>
 int rk4_set_field(H *s, int32_t which, int32_t var, const float *host)
 {
@@ -134,6 +159,11 @@ int rk4_set_field(H *s, int32_t which, int32_t var, const float *host)
     auto *b = rk4_state_buffer(s, which);
     if (!b) return am::set_error(AM_ERR_INVALID, "set_field: no buffer %d", which);
     auto &d = s->amr->sim_data;
+    if (b == &d.acc) {  // interior-only on the device, padded on the host
+        const std::vector<float> in = interior_of(host, d.slots, d.geom);
+        if (int rc = am_memcpy_h2d(d.field(*b, var), in.data(), in.size() * sizeof(float), d.stream)) return rc;
+        return am_stream_sync(d.stream);
+    }
     if (int rc = am_memcpy_h2d(d.field(*b, var), host, d.slots * d.cells * sizeof(float), d.stream)) return rc;
     return am_stream_sync(d.stream);
 }
@@ -145,6 +175,12 @@ int rk4_get_field(H *s, int32_t which, int32_t var, float *host)
     auto *b = rk4_state_buffer(s, which);
     if (!b) return am::set_error(AM_ERR_INVALID, "get_field: no buffer %d", which);
     auto &d = s->amr->sim_data;
+    if (b == &d.acc) {
+        std::vector<float> in(d.slots * d.inner);
+        if (int rc = am_memcpy_d2h(in.data(), d.field(*b, var), in.size() * sizeof(float), d.stream)) return rc;
+        pad_from_interior(in, host, d.slots, d.geom);
+        return AM_OK;
+    }
     return am_memcpy_d2h(host, d.field(*b, var), d.slots * d.cells * sizeof(float), d.stream);
 }
 template <class H>

@@ -1,7 +1,9 @@
 // Classical RK4 combine for one interior cell of NV fields and the
 // equal-depth ghost push, shared by every stage kernel.
 //
-// Stage s of y' = k(y) with accumulator acc (interior cells only):
+// Stage s of y' = k(y) with accumulator acc (interior-only layout
+// slot * S^3 + (z * S + y) * S + x, index `w`; state, old_ and new_ are
+// padded tiles, index `r`):
 //   s = 0: acc  = k;   new = y_n + dt/2 k
 //   s = 1: acc += 2k;  new = y_n + dt/2 k
 //   s = 2: acc += 2k;  new = y_n + dt k
@@ -20,20 +22,20 @@ namespace am {
 
 template <int NV, int STAGE>
 __device__ __forceinline__ void rk4_combine(const float *const *state, const float *const *old_, float *const *acc,
-                                            float *const *new_, float (&k)[NV], size_t r, float dt)
+                                            float *const *new_, float (&k)[NV], size_t r, size_t w, float dt)
 {
 #pragma unroll
     for (int v = 0; v < NV; v++) {
         const float yn = STAGE == 0 ? state[v][r] : old_[v][r];
         float nv;
         if (STAGE == 0) {
-            acc[v][r] = k[v];
+            acc[v][w] = k[v];
             nv = yn + (.5f * dt) * k[v];
         } else if (STAGE == 1 || STAGE == 2) {
-            acc[v][r] = acc[v][r] + 2.f * k[v];
+            acc[v][w] = acc[v][w] + 2.f * k[v];
             nv = yn + (STAGE == 1 ? .5f * dt : dt) * k[v];
         } else {
-            nv = yn + (dt * (1.f / 6.f)) * (acc[v][r] + k[v]);
+            nv = yn + (dt * (1.f / 6.f)) * (acc[v][w] + k[v]);
         }
         new_[v][r] = nv;
         k[v] = nv;  // the stage output, for the push

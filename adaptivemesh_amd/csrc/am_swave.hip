@@ -37,9 +37,10 @@ __global__ void __launch_bounds__(256) k_swave_stage(SwaveArgsDev a, SwaveTiles 
     if (cell >= S3) return;
     const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
     const size_t r = (size_t)node * (T * T * T) + ((size_t)(z + P) * T + (y + P)) * T + (x + P);
-    const float3 h = cf3_unpack(a.d2[(size_t)node * S3 + cell]);
+    const size_t w = (size_t)node * S3 + cell;
+    const float3 h = cf3_unpack(a.d2[w]);
     float k[2] = {a.state[1][r], (h.x + h.y) + h.z};
-    rk4_combine<2, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, sc.dt[depth]);
+    rk4_combine<2, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
     if (PUSH) rk4_push<2, S, P>(a.new_, k, m.boundaries, node, x, y, z);
 }
 

@@ -34,23 +34,25 @@ HBM_PEAK_GBS = 8000.0
 S, P, FD_ORDER, OUTER, CFL, DEPTH = 16, 3, 4, 64.0, 0.25, 3
 NOISE = 1e-6
 
-# algorithmic bytes per interior cell per RK4 stage (each distinct element
-# read once and written once per launch)
+# algorithmic bytes per interior cell (each distinct element read once and
+# written once per launch)
 WORD = 4
+PUSH = 6 * P * S * S * 24 * 4 / S ** 3  # equal-depth face ghosts written by the stage kernel
 BYTES = {
-    # 24 fields: padded reads (S^3 + 6 P S^2) * 4 per field; 24 gradient + 11 diag + 11 off-diag words
+    # per stage, 24 fields: padded reads (S^3 + 6 P S^2) * 4 per field; 24 gradient + 11 diag + 11 off-diag words
     "fd": 24 * (S ** 3 + 6 * P * S * S) * 4 / S ** 3 + 46 * WORD,
-    # gt_ij (6 fp32) + 6 gradient words in, 6 words out
-    "christoffel": 6 * 4 + 6 * WORD + 6 * WORD,
-    # per stage: state 96, 46 derivative words 184, 6 Christoffel 24; y_n 96 and acc 96 when
-    # distinct; writes new 96 and acc 96 (stage 3 writes no acc)
-    "rhs_stage": (96 + 184 + 24 + 192) + (96 + 184 + 24 + 96 + 96 + 192) * 2 + (96 + 184 + 24 + 96 + 96 + 96),
-    # 6 P S^2 face cells per tile, one source read and one write, 24 fields
+    # per RK4 step, the four stages summed: state 96 and 46 words 184 read in every stage; y_n 96 and acc 96
+    # read in stages 1-3; new 96 written in every stage, acc 96 written in stages 0-2; the pushed face ghosts
+    "rhs_stage": (96 + 184 + 192 + PUSH) + (96 + 184 + 96 + 96 + 192 + PUSH) * 2 + (96 + 184 + 96 + 96 + 96 + PUSH),
+    # per stage: 6 P S^2 face cells per tile, one source read and one write, 24 fields (coarse faces only)
     "ghost": 24 * 6 * P * S * S * 8 / S ** 3,
-    # T^3 read + T^3 write per field, once per step
+    # per step: T^3 read + T^3 write per field
     "copy_back": 24 * 2 * 4 * (S + 2 * P) ** 3 / S ** 3,
 }
-STAGES_PER_STEP = {"fd": 4, "christoffel": 4, "rhs_stage": 1, "ghost": 4, "copy_back": 1}
+STAGES_PER_STEP = {"fd": 4, "rhs_stage": 1, "ghost": 4, "copy_back": 1}
+# kernel classes launched by this workload (every face of the uniform periodic mesh has an equal-depth target,
+# so no coarse-face ghost fill runs)
+WORKLOAD_CLASSES = ("fd", "rhs_stage", "copy_back")
 
 
 def build_mesh(sim):
@@ -79,9 +81,9 @@ def initial_data(n_slots, T):
 
 
 def cpu_baseline(n_slots, fields, positions, tiles_all, budget_s=12.0):
-    """The oracle (a port of the scheme: bit-exact FP32 stencils and
-    Christoffel words, FP64 RHS) running one RK4 step on a sample of depth-3
-    tiles on this host's cores."""
+    """The oracle (a port of the scheme: bit-exact FP32 stencil words, FP64
+    RHS with the Christoffel symbols raised from the metric gradient words)
+    running one RK4 step on a sample of depth-3 tiles on this host's cores."""
     from oracle import oracle as orc
     T = S + 2 * P
     outer_dx = np.float32(OUTER) / np.float32(S)
@@ -106,10 +108,9 @@ def cpu_baseline(n_slots, fields, positions, tiles_all, budget_s=12.0):
                     d2d.append(b)
                     d2o.append(orc.mixed_partials(state[v], positions, sample, 0, DEPTH, seed - v, S, P,
                                                   float(outer_dx), n_slots))
-            chw = orc.christoffel(state[1:7], d1[1:7], 0, positions, sample, 0, DEPTH, seed, S, P, n_slots)
             orc.lib().orc_bssn_stage_from_words(orc.ptrs(state), orc.ptrs(fields), orc.ptrs(acc), orc.ptrs(acc),
                                                 orc.ptrs(new), orc.ptrs(d1), orc.ptrs(d2d), orc.ptrs(d2o),
-                                                orc.ptrs(chw), positions.ctypes.data, sample.ctypes.data,
+                                                None, positions.ctypes.data, sample.ctypes.data,
                                                 len(sample), 0, DEPTH, stage, dtd.ctypes.data, params.ctypes.data,
                                                 S, P)
             state = [x.astype(np.float32) for x in new]
@@ -125,7 +126,7 @@ def cpu_baseline(n_slots, fields, positions, tiles_all, budget_s=12.0):
     cells = one_step(tiles[:n])
     dt = time.perf_counter() - t0
     return {"value": cells / dt, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
-            "sample": f"one RK4 step (4 stages: FD words, Christoffel words, FP64 RHS + update) of {n} of the "
+            "sample": f"one RK4 step (4 stages: FD words, FP64 RHS + update) of {n} of the "
                       f"512 depth-3 tiles ({cells} zone-cycles) in {dt:.2f} s with {threads} OpenMP threads; "
                       "ghost fill not included"}
 
@@ -203,7 +204,7 @@ def main():
     elapsed = t1 - t0
     value = zones / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    bytes_per_zone = sum(BYTES[k] * STAGES_PER_STEP[k] for k in BYTES)
+    bytes_per_zone = sum(BYTES[k] * STAGES_PER_STEP[k] for k in WORKLOAD_CLASSES)
     whole_gbs = zones * bytes_per_zone / elapsed / 1e9
 
     roofline = None
@@ -235,7 +236,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "BSSN single level: 512 tiles of 16^3 (128^3) + 73 ancestors, P=3, 24 fp32 fields, "
-                               "1+log + Gamma-driver, flat + 1e-6 noise, cf3 derivative/Christoffel words, RK4, "
+                               "1+log + Gamma-driver, flat + 1e-6 noise, cf3 derivative words (Christoffel symbols raised in registers), RK4, "
                                "FD order 4, periodic, [-32,32]^3, CFL 0.25",
                    "tiles": n_slots, "zone_cycles_per_step": zones // max(args.steps, 1),
                    "fine_rk4_steps_per_step": 2 ** DEPTH},

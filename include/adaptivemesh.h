@@ -231,8 +231,9 @@ int am_ghost_fill_stage_batched(const am_field_batch *b, const am_mesh *mesh, co
  * update, BSSN.h:143-159): the moving-puncture BSSN system with 24 fields
  *   0 chi | 1-6 gt_ij | 7 K | 8-13 At_ij | 14-16 Gt^i | 17 alpha | 18-20 beta^i | 21-23 B^i
  * (symmetric tensors in xx,xy,xz,yy,yz,zz order).  The Hessian words are for
- * chi, gt_ij, alpha, beta^i in that order (11 fields); chr[jk] holds
- * (Gt^x_jk, Gt^y_jk, Gt^z_jk) as written by am_christoffel(compat = 0). */
+ * chi, gt_ij, alpha, beta^i in that order (11 fields).  The conformal
+ * Christoffel symbols are formed in registers from gt_ij and its gradient
+ * words (am_christoffel remains for the reference's BSSN plugin). */
 #define AM_BSSN_NVARS 24
 #define AM_BSSN_NHESS 11
 
@@ -248,11 +249,10 @@ typedef struct am_bssn_fields {
     const float *state[AM_BSSN_NVARS]; /* stage input, padded tiles */
     const float *old_[AM_BSSN_NVARS];  /* y_n at the start of the step */
     float *new_[AM_BSSN_NVARS];        /* stage output (interior cells written) */
-    float *acc[AM_BSSN_NVARS];         /* RK4 accumulator (interior cells) */
+    float *acc[AM_BSSN_NVARS];         /* RK4 accumulator, interior-only layout (slot * S^3 + ...) */
     const uint32_t *d1[AM_BSSN_NVARS];
     const uint32_t *d2diag[AM_BSSN_NHESS];
     const uint32_t *d2off[AM_BSSN_NHESS];
-    const uint32_t *chr[6];
 } am_bssn_fields;
 
 /* One classical RK4 stage on the interior of every listed active tile:
@@ -283,7 +283,7 @@ typedef struct am_swave_fields {
     const float *state[2]; /* u, v: stage input */
     const float *old_[2];
     float *new_[2];
-    float *acc[2];
+    float *acc[2];            /* interior-only layout, as for am_bssn_fields */
     const uint32_t *d1_u;     /* gradient word of the stage input u */
     const uint32_t *d2diag_u; /* (u_xx, u_yy, u_zz) word of the stage input u */
 } am_swave_fields;

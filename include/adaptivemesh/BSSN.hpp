@@ -118,7 +118,7 @@ struct bssn_system {
     static constexpr int NV = AM_BSSN_NVARS;
     // chi, gt_ij, alpha, beta^i need full Hessians; everything else a gradient
     static constexpr int FD_KIND[NV] = {3, 3, 3, 3, 3, 3, 3, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3, 3, 3, 1, 1, 1};
-    static constexpr bool CHRISTOFFEL = true;
+    static constexpr bool CHRISTOFFEL = false;  // raised in registers by the RHS kernel
     using options_t = am_bssn_params;
 
     static am_bssn_fields fields(const rk4_views &v)
@@ -135,7 +135,6 @@ struct bssn_system {
             f.d2diag[h] = v.d2diag[h];
             f.d2off[h] = v.d2off[h];
         }
-        for (int c = 0; c < 6; c++) f.chr[c] = v.chr[c];
         return f;
     }
     static int rhs_stage(const rk4_views &v, int stage, const options_t &p, const am_mesh &m, const am_geometry &g,

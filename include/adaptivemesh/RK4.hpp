@@ -107,7 +107,9 @@ struct rk4_AMR_data {
     size_t cells = 0, inner = 0, slots = 0;
     int n_d2 = 0, n_off = 0;
 
-    // one allocation per kind; field v at base + v * slots * cells
+    // one allocation per kind; field v at base + v * slots * cells (padded
+    // tiles) or v * slots * inner (acc: interior-only, so every accumulator
+    // row is a whole aligned run)
     smart_gpu_buffer<float> old_, new_, acc, older;
     smart_gpu_buffer<uint32_t> d1, d2d, d2o, chr;
     smart_gpu_buffer<double> partial;
@@ -128,7 +130,7 @@ struct rk4_AMR_data {
         }
         old_ = smart_gpu_buffer<float>(NV * slots * cells);
         new_ = smart_gpu_buffer<float>(NV * slots * cells);
-        acc = smart_gpu_buffer<float>(NV * slots * cells);
+        acc = smart_gpu_buffer<float>(NV * slots * inner);
         if (opt.interp == AM_INTERP_TRICUBIC) older = smart_gpu_buffer<float>(NV * slots * cells);
         d1 = smart_gpu_buffer<uint32_t>(NV * slots * inner);
         if (n_d2) d2d = smart_gpu_buffer<uint32_t>(n_d2 * slots * inner);
@@ -144,7 +146,10 @@ struct rk4_AMR_data {
         d1.destroy(); d2d.destroy(); d2o.destroy(); chr.destroy(); partial.destroy();
         timer.destroy();
     }
-    float *field(smart_gpu_buffer<float> &b, int v) { return b.gpu_buffer_ptr + (size_t)v * slots * cells; }
+    float *field(smart_gpu_buffer<float> &b, int v)
+    {
+        return b.gpu_buffer_ptr + (size_t)v * slots * (&b == &acc ? inner : cells);
+    }
     uint32_t *word(smart_gpu_buffer<uint32_t> &b, int v) { return b.gpu_buffer_ptr + (size_t)v * slots * inner; }
 
     am_field_batch batch(bool with_older)

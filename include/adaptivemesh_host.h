@@ -73,11 +73,14 @@ void am_bssn_sim_destroy(am_bssn_sim *s);
 int am_bssn_sim_add_node(am_bssn_sim *s, int32_t parent, int32_t offset);
 int am_bssn_sim_export(const am_bssn_sim *s, int32_t *parent, int32_t *children, uint32_t *positions,
                        uint64_t *boundaries, uint32_t *lifetime, int32_t *scalars);
-/* which: 0 old, 1 new, 2 acc, 3 older; var 0..23; max_slots * T^3 floats */
+/* which: 0 old, 1 new, 2 acc, 3 older; var 0..23; max_slots * T^3 floats on
+ * the host (acc is interior-only on the device: its ghost cells read as 0) */
 int am_bssn_sim_set_field(am_bssn_sim *s, int32_t which, int32_t var, const float *host);
 int am_bssn_sim_get_field(am_bssn_sim *s, int32_t which, int32_t var, float *host);
-/* device addresses: which 0-3 as above (var stride max_slots*T^3 floats),
- * 4 d1, 5 d2diag, 6 d2off, 7 chr words (var stride max_slots*S^3) */
+/* device addresses: which 0, 1, 3 as above (var stride max_slots*T^3
+ * floats), 2 acc (var stride max_slots*S^3 floats), 4 d1, 5 d2diag, 6 d2off
+ * words (var stride max_slots*S^3); 7 Christoffel words where a driver keeps
+ * them (none: the BSSN RHS raises the symbols in registers) */
 void *am_bssn_sim_device_ptr(am_bssn_sim *s, int32_t which);
 int am_bssn_sim_fill_ghosts(am_bssn_sim *s);
 int am_bssn_sim_copy_back(am_bssn_sim *s);

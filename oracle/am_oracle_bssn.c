@@ -330,7 +330,9 @@ static inline void rk4_combine(int stage, double dt, double yn, double k, double
 /* The RK4 stage on the product's inputs: fp32 state (stage input), fp32
  * y_n and dithered words; FP64 arithmetic; acc and new are FP64 arrays in
  * the padded layout, one pointer per variable (interior cells written).
- * dtd[d] is the step of depth d. */
+ * dtd[d] is the step of depth d.  chw == NULL: the raised Christoffel
+ * symbols are computed from the metric and its gradient words (as the GPU
+ * RHS does) instead of read from Christoffel words. */
 EXPORT void orc_bssn_stage_from_words(const float *const *state, const float *const *old_, const double *const *acc_in,
                                       double *const *acc_out, double *const *new_out, const uint32_t *const *d1w,
                                       const uint32_t *const *d2dw, const uint32_t *const *d2ow,
@@ -364,9 +366,10 @@ EXPORT void orc_bssn_stage_from_words(const float *const *state, const float *co
                         dd[h][0] = a[0]; dd[h][1] = b[0]; dd[h][2] = b[1];
                         dd[h][3] = a[1]; dd[h][4] = b[2]; dd[h][5] = a[2];
                     }
-                    for (int c = 0; c < 6; c++) cf3_dec(chw[c][w], G[c]);
-                    bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd, (const double(*)[3])G, &par, k,
-                               NULL, NULL);
+                    if (chw)
+                        for (int c = 0; c < 6; c++) cf3_dec(chw[c][w], G[c]);
+                    bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd,
+                               chw ? (const double(*)[3])G : NULL, &par, k, NULL, NULL);
                     for (int v = 0; v < BSSN_NV; v++) {
                         const double ai = stage == 0 ? 0.0 : acc_in[v][r];
                         rk4_combine(stage, dt, (double)old_[v][r], k[v], ai, &acc_out[v][r], &new_out[v][r]);
@@ -406,9 +409,10 @@ EXPORT void orc_bssn_constraints_from_words(const float *const *state, const uin
                         dd[h][0] = a[0]; dd[h][1] = b[0]; dd[h][2] = b[1];
                         dd[h][3] = a[1]; dd[h][4] = b[2]; dd[h][5] = a[2];
                     }
-                    for (int c = 0; c < 6; c++) cf3_dec(chw[c][w], G[c]);
-                    bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd, (const double(*)[3])G, &par, k,
-                               &H, M);
+                    if (chw)
+                        for (int c = 0; c < 6; c++) cf3_dec(chw[c][w], G[c]);
+                    bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd,
+                               chw ? (const double(*)[3])G : NULL, &par, k, &H, M);
                     h2 += H * H;
                     m2 += M[0] * M[0] + M[1] * M[1] + M[2] * M[2];
                     hmax = fmax(hmax, fabs(H));
