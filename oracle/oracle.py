@@ -263,3 +263,110 @@ def bssn_point(u, d1, dd, G=None, params=BSSN_DEFAULT):
 
 def ptrs(arrs):
     return (C.c_void_p * len(arrs))(*[None if a is None else a.ctypes.data for a in arrs])
+
+
+# ------------------------------------------------------ FP64 scheme driver
+def bssn64_stage(state, old, acc, new, positions, tiles, substep, max_depth, stage, inv_dx, dtd,
+                 params=BSSN_DEFAULT, S=16, P=3, q=2):
+    """One FP64 RK4 stage with exact derivatives on the listed tiles (lists of
+    24 float64 padded arrays; acc/new written on interior cells)."""
+    t = _i32arr(tiles)
+    # the arrays must outlive the call: bind them before taking addresses
+    ix, dt, pa = np.asarray(inv_dx, np.float64), np.asarray(dtd, np.float64), _params(params)
+    lib().orc_bssn64_stage(ptrs(state), ptrs(old), ptrs(acc), ptrs(new), _p(positions), _p(t), t.size, substep,
+                           max_depth, stage, _p(ix), _p(dt), _p(pa), S, P, q)
+
+
+def ghost64(old, new, older, boundaries, tiles, substep, max_depth, frac, mode, S=16, P=3, equal_copy=True):
+    t = _i32arr(tiles)
+    lib().orc_ghost64(ptrs(old), ptrs(new), None if older is None else ptrs(older), len(new), _p(boundaries), _p(t),
+                      t.size, substep, max_depth, frac, mode, S, P, 1 if equal_copy else 0)
+
+
+def copy_back64(old, new, older, positions, children, tiles, substep, max_depth, S=16, P=3):
+    t = _i32arr(tiles)
+    lib().orc_copy_back64(ptrs(old), ptrs(new), None if older is None else ptrs(older), len(new), _p(positions),
+                          _p(children), _p(t), t.size, substep, max_depth, S, P)
+
+
+def bssn64_constraints(state, positions, tiles, inv_dx, params=BSSN_DEFAULT, S=16, P=3, q=2):
+    """{rms H, rms |M|, max |H|, cells} over the listed tiles, exact derivatives."""
+    t = _i32arr(tiles)
+    out = np.zeros(4 * t.size)
+    ix, pa = np.asarray(inv_dx, np.float64), _params(params)
+    lib().orc_bssn64_constraints(ptrs(state), _p(positions), _p(t), t.size, _p(ix), _p(pa), S, P, q, _p(out))
+    o = out.reshape(-1, 4)
+    c = o[:, 2].sum()
+    return np.sqrt(o[:, 0].sum() / c), np.sqrt(o[:, 1].sum() / c), o[:, 3].max(), c
+
+
+class Bssn64Sim:
+    """The RK4 octree driver's schedule in FP64 with exact derivatives: per
+    sub-step copy-back (keeping older), then every active depth coarse first,
+    four stages each with the stage-time ghost fill (equal-depth faces
+    copied, coarser faces interpolated), then the counter.  The product's
+    am_bssn_sim runs the same schedule in fp32 on dithered words."""
+
+    def __init__(self, tree_export, outer_dx, cfl, mode=0, params=BSSN_DEFAULT, S=16, P=3, q=2):
+        e = tree_export
+        self.e, self.S, self.P, self.q, self.mode, self.params = e, S, P, q, mode, params
+        self.n = len(e["positions"])
+        T = S + 2 * P
+        self.cells = self.n * T ** 3
+        self.old = [np.zeros(self.cells) for _ in range(24)]
+        self.new = [np.zeros(self.cells) for _ in range(24)]
+        self.acc = [np.zeros(self.cells) for _ in range(24)]
+        self.inp = [np.zeros(self.cells) for _ in range(24)]
+        self.older = [np.zeros(self.cells) for _ in range(24)] if mode == 2 else None
+        odx = np.float32(outer_dx)
+        self.inv_dx = [float(np.float32(2 ** d) / odx) for d in range(9)]
+        odt = np.float32(odx * np.float32(cfl))
+        self.dtd = [float(odt / np.float32(2 ** d)) for d in range(9)]
+        used = [i for i, p in enumerate(e["positions"]) if int(p) != 0xFFFFFFFF]
+        self.all = used
+        self.by_depth = {}
+        for i in used:
+            self.by_depth.setdefault(int(e["positions"][i]) >> 27, []).append(i)
+        self.max_depth = e["max_depth"]
+        self.timer = 0
+        self.leaves = [i for i in used if all(c == -1 for c in e["children"][8 * i:8 * i + 8])]
+
+    def set_state(self, fields):
+        for v in range(24):
+            self.old[v][:] = fields[v]
+            self.new[v][:] = fields[v]
+
+    def fill_ghosts(self):
+        for d in range(1, self.max_depth + 1):
+            if d in self.by_depth:
+                ghost64(self.old, self.new, None, self.e["boundaries"], self.by_depth[d], 0, self.max_depth, 1.0,
+                        self.mode, self.S, self.P, True)
+
+    def substep(self):
+        e, md, t = self.e, self.max_depth, self.timer
+        copy_back64(self.old, self.new, self.older, e["positions"], e["children"], self.all, t, md, self.S, self.P)
+        for d in range(md + 1):
+            if d not in self.by_depth or (t & ((1 << (md - d)) - 1)) != 0:
+                continue
+            tiles = self.by_depth[d]
+            for s in range(4):
+                st = self.old if s == 0 else self.new
+                if s == 0:
+                    bssn64_stage(st, self.old, self.acc, self.new, e["positions"], tiles, t, md, s, self.inv_dx,
+                                 self.dtd, self.params, self.S, self.P, self.q)
+                else:
+                    # the stage reads neighbouring cells of its input while
+                    # writing new: give it a copy of the listed tiles
+                    idx = np.asarray(tiles)
+                    for v in range(24):
+                        self.inp[v].reshape(self.n, -1)[idx] = self.new[v].reshape(self.n, -1)[idx]
+                    bssn64_stage(self.inp, self.old, self.acc, self.new, e["positions"], tiles, t, md, s,
+                                 self.inv_dx, self.dtd, self.params, self.S, self.P, self.q)
+                if d > 0:
+                    ghost64(self.old, self.new, self.older, e["boundaries"], tiles, t, md, (.5, .5, 1., 1.)[s],
+                            self.mode, self.S, self.P, True)
+        self.timer = (t + 1) % (1 << md)
+
+    def constraints(self):
+        return bssn64_constraints(self.new, self.e["positions"], self.leaves, self.inv_dx, self.params, self.S,
+                                  self.P, self.q)

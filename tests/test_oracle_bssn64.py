@@ -1,0 +1,125 @@
+"""Pins the FP64 BSSN scheme on tiles (the oracle the octree runs are judged
+against) by functions already pinned elsewhere:
+
+* orc_bssn64_stage: on fields that are quadratic polynomials the order-4
+  centred differences and the mixed formula are exact, so the stage's k must
+  equal orc_bssn_point (pinned by Kerr-Schild stationarity and the gauge
+  wave) given the polynomials' exact derivatives.
+* orc_ghost64 / orc_copy_back64: on float data they must agree with the
+  float restatement (orc_copy_boundaries / orc_copy_new_to_old, pinned by
+  polynomial exactness and closed forms) to float rounding.
+* Bssn64Sim: flat space stays flat; its schedule reproduces one hand-run
+  sub-step.
+"""
+import numpy as np
+import pytest
+
+from tests import fields as F
+from tests import meshes
+
+S, P, OUTER = 16, 3, 4.0
+T = S + 2 * P
+
+
+def tree(orc, recipe, periodic=False):
+    ops, n = {"uniform1": meshes.uniform(1), "graded": meshes.graded()}[recipe]
+    t = orc.OracleTree(n, periodic)
+    for p_, c in ops:
+        t.add_node(p_, c)
+    return t.export(), n
+
+
+def quadratic_fields(rng):
+    """24 quadratic polynomials a + b.x + x^T C x near flat space."""
+    flat = np.zeros(24)
+    flat[[0, 1, 4, 6, 17]] = 1.0
+    polys = []
+    for v in range(24):
+        a = flat[v] + 0.01 * rng.standard_normal()
+        b = 0.01 * rng.standard_normal(3)
+        C = 0.002 * rng.standard_normal((3, 3))
+        polys.append((a, b, (C + C.T) / 2))
+    return polys
+
+
+def test_bssn64_stage_exact_on_quadratics(orc):
+    e, n = tree(orc, "uniform1")
+    rng = np.random.default_rng(4)
+    polys = quadratic_fields(rng)
+    state = [F.sample(e["positions"], n, S, P, OUTER,
+                      lambda x, y, z, p=p: p[0] + p[1][0] * x + p[1][1] * y + p[1][2] * z
+                      + p[2][0, 0] * x * x + p[2][1, 1] * y * y + p[2][2, 2] * z * z
+                      + 2 * (p[2][0, 1] * x * y + p[2][0, 2] * x * z + p[2][1, 2] * y * z),
+                      dtype=np.float64) for p in polys]
+    acc = [np.zeros(n * T ** 3) for _ in range(24)]
+    new = [np.zeros(n * T ** 3) for _ in range(24)]
+    outer_dx = OUTER / S
+    inv_dx = [2 ** d / outer_dx for d in range(9)]
+    dtd = [0.1 / 2 ** d for d in range(9)]
+    tiles = [s for s in F.used_slots(e) if F.depth_of(e, s) == 1]
+    orc.bssn64_stage(state, state, acc, new, e["positions"], tiles, 0, 1, 0, inv_dx, dtd, S=S, P=P, q=2)
+    for slot, (z, y, x) in [(1, (3, 4, 5)), (4, (0, 0, 0)), (8, (15, 15, 15)), (6, (7, 0, 12)), (2, (15, 8, 0))]:
+        cx, cy, cz = F.cell_coords(e["positions"], slot, S, P, OUTER)
+        pt = np.array([cx[z + P, y + P, x + P], cy[z + P, y + P, x + P], cz[z + P, y + P, x + P]])
+        u = np.array([p[0] + p[1] @ pt + pt @ p[2] @ pt for p in polys])
+        d1 = np.array([p[1] + 2 * p[2] @ pt for p in polys])
+        dd = np.array([[2 * p[2][0, 0], 2 * p[2][0, 1], 2 * p[2][0, 2], 2 * p[2][1, 1], 2 * p[2][1, 2], 2 * p[2][2, 2]]
+                       for v, p in enumerate(polys) if v in orc.HESS_VARS])
+        k, _ = orc.bssn_point(u, d1, dd)
+        r = slot * T ** 3 + ((z + P) * T + y + P) * T + x + P
+        got = np.array([acc[v][r] for v in range(24)])
+        assert np.allclose(got, k, rtol=1e-9, atol=1e-11 * np.max(np.abs(k))), (slot, got - k)
+        # stage 0 update: new = y_n + dt/2 k
+        assert np.allclose([new[v][r] for v in range(24)], u + 0.5 * dtd[1] * k, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_ghost64_matches_float_restatement(orc, mode):
+    e, n = tree(orc, "graded")
+    rng = np.random.default_rng(mode)
+    for d in (1, 2, 3):
+        tiles = [s for s in F.used_slots(e) if F.depth_of(e, s) == d]
+        for substep, frac in ((0, 0.5), (1, 1.0), (3, 0.5)):
+            f32 = [(rng.random(n * T ** 3) + 0.5).astype(np.float32) for _ in range(3)]  # old, new, older
+            old64, new64, older64 = [[x.astype(np.float64)] for x in f32]
+            orc.ghost64(old64, new64, older64 if mode == 2 else None, e["boundaries"], tiles, substep, e["max_depth"],
+                        frac, mode, S, P, True)
+            new32 = f32[1].copy()
+            orc.copy_boundaries(f32[0].copy(), new32, f32[2] if mode == 2 else None, e["boundaries"], tiles, substep,
+                                e["max_depth"], True, mode, S, P, frac, equal_copy=True)
+            assert np.max(np.abs(new64[0] - new32)) < 2e-6, (d, substep)
+
+
+def test_copy_back64_matches_float_restatement(orc):
+    e, n = tree(orc, "graded")
+    rng = np.random.default_rng(9)
+    tiles = F.used_slots(e)
+    for substep in range(4):
+        old32 = rng.standard_normal(n * T ** 3).astype(np.float32)
+        new32 = rng.standard_normal(n * T ** 3).astype(np.float32)
+        old64, new64, older64 = [old32.astype(np.float64)], [new32.astype(np.float64)], [np.zeros(n * T ** 3)]
+        orc.copy_back64(old64, new64, older64, e["positions"], e["children"], tiles, substep, e["max_depth"], S, P)
+        ref_old = old32.copy()
+        orc.copy_new_to_old(ref_old, new32, e["positions"], e["children"], tiles, substep, e["max_depth"], S, P)
+        assert np.max(np.abs(old64[0] - ref_old)) < 1e-6
+        active = np.zeros((n, T ** 3), bool)
+        for s in tiles:
+            d = F.depth_of(e, s)
+            active[s] = (substep & ((1 << (e["max_depth"] - d)) - 1)) == 0
+        assert np.array_equal(older64[0][active.reshape(-1)], old32[active.reshape(-1)].astype(np.float64))
+
+
+def test_bssn64_sim_flat_space_and_schedule(orc):
+    e, n = tree(orc, "graded")
+    sim = orc.Bssn64Sim(e, OUTER / S, 0.25, mode=2)
+    flat = np.zeros(24)
+    flat[[0, 1, 4, 6, 17]] = 1.0
+    sim.set_state([np.full(n * T ** 3, flat[v]) for v in range(24)])
+    sim.fill_ghosts()
+    for _ in range(2 ** e["max_depth"]):
+        sim.substep()
+    assert sim.timer == 0
+    for v in range(24):
+        assert np.max(np.abs(sim.new[v] - flat[v])) < 1e-12, v
+    h, m, hmax, cells = sim.constraints()
+    assert h < 1e-12 and m < 1e-12 and cells == len(sim.leaves) * S ** 3
