@@ -57,14 +57,14 @@ class Geometry(C.Structure):
 class Mesh(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("children", C.c_void_p), ("boundaries", C.c_void_p),
                 ("tiles", C.c_void_p), ("n_tiles", C.c_int32), ("substep", C.c_uint32),
-                ("max_depth", C.c_uint32)]
+                ("max_depth", C.c_uint32), ("grid_limit", C.c_int32)]
 
 
 class FdBatch(C.Structure):
     _fields_ = [("field", C.c_void_p * AM_MAX_VARS), ("d1", C.c_void_p * AM_MAX_VARS),
                 ("d2diag", C.c_void_p * AM_MAX_VARS), ("d2off", C.c_void_p * AM_MAX_VARS),
                 ("seed_d", C.c_int32 * AM_MAX_VARS), ("seed_m", C.c_int32 * AM_MAX_VARS),
-                ("n_vars", C.c_int32), ("z_block", C.c_int32)]
+                ("n_vars", C.c_int32), ("z_block", C.c_int32), ("seed_table", C.c_void_p)]
 
 
 class FieldBatch(C.Structure):
@@ -80,12 +80,21 @@ class BssnParams(C.Structure):
 class BssnFields(C.Structure):
     _fields_ = [("state", C.c_void_p * 24), ("old_", C.c_void_p * 24), ("new_", C.c_void_p * 24),
                 ("acc", C.c_void_p * 24), ("d1", C.c_void_p * 24), ("d2diag", C.c_void_p * 11),
-                ("d2off", C.c_void_p * 11), ("chr", C.c_void_p * 6)]
+                ("d2off", C.c_void_p * 11)]
 
 
 class BssnSimOptions(C.Structure):
     _fields_ = [("params", BssnParams), ("interp", C.c_int32), ("z_block", C.c_int32), ("periodic", C.c_int32),
                 ("seed", C.c_int32)]
+
+
+class _Trial(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("value", C.c_int32), ("ms", C.c_float)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("z_block", C.c_int32), ("fd_grid_limit", C.c_int32), ("rhs_grid_limit", C.c_int32),
+                ("n_trials", C.c_int32), ("trial", _Trial * 32)]
 
 
 class SwaveFields(C.Structure):
@@ -114,6 +123,15 @@ _SIGS = {
     "am_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t, _P]),
     "am_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t, _P]),
     "am_memset_async": (C.c_int, [_P, C.c_int, C.c_size_t, _P]),
+    "am_memcpy_d2d": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "am_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "am_stream_destroy": (C.c_int, [_P]),
+    "am_device_cu_count": (C.c_int32, []),
+    "am_graph_begin": (C.c_int, [_P]),
+    "am_graph_end": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
+    "am_graph_launch": (C.c_int, [_P, _P]),
+    "am_graph_destroy": (C.c_int, [_P]),
+    "am_seed_table_fill": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
     "am_stream_sync": (C.c_int, [_P]),
     "am_device_sync": (C.c_int, []),
     "am_fd_d1": (C.c_int, [_P, _P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_int32, _P]),
@@ -193,6 +211,10 @@ _RK4_SIM_SIGS = {
     "counter": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]),
     "profile": (C.c_int, [_P, C.c_int32]),
     "kernel_times": (C.c_int, [_P, _P, _P]),
+    "set_launch": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
+    "get_launch": (C.c_int, [_P, _P]),
+    "use_graph": (C.c_int, [_P, C.c_int32]),
+    "autotune": (C.c_int, [_P, C.c_int32, _P]),
 }
 for _prefix in ("am_bssn_sim", "am_swave_sim"):
     for _k, _v in _RK4_SIM_SIGS.items():
@@ -449,6 +471,25 @@ class _Rk4Sim:
 
     def profile(self, on: bool):
         check(self._f("profile")(self.h, 1 if on else 0), "profile")
+
+    def set_launch(self, z_block: int = 0, fd_grid_limit: int = 0, rhs_grid_limit: int = 0):
+        check(self._f("set_launch")(self.h, z_block, fd_grid_limit, rhs_grid_limit), "set_launch")
+
+    def get_launch(self) -> tuple:
+        out = np.zeros(3, np.int32)
+        check(self._f("get_launch")(self.h, out.ctypes.data), "get_launch")
+        return tuple(int(x) for x in out)
+
+    def use_graph(self, on: bool = True):
+        check(self._f("use_graph")(self.h, 1 if on else 0), "use_graph")
+
+    def autotune(self, reps: int = 5) -> dict:
+        r = TuneReport()
+        check(self._f("autotune")(self.h, reps, C.byref(r)), "autotune")
+        kinds = ("fd_z_block", "fd_grid_limit", "rhs_grid_limit")
+        return {"z_block": r.z_block, "fd_grid_limit": r.fd_grid_limit, "rhs_grid_limit": r.rhs_grid_limit,
+                "trials": [(kinds[r.trial[i].kind], r.trial[i].value, round(r.trial[i].ms, 4))
+                           for i in range(r.n_trials)]}
 
     def kernel_times(self) -> dict:
         ms = np.zeros(len(KERNEL_CLASSES))

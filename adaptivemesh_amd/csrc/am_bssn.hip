@@ -288,6 +288,7 @@ struct TileArgs {
     const int32_t *tiles;
     const uint64_t *boundaries;
     uint32_t substep, max_depth;
+    int32_t n_items;  // tiles x 256-cell blocks; workgroups loop over them
 };
 
 template <int S, int P, int STAGE, bool PUSH>
@@ -295,19 +296,21 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
 {
 #pragma clang fp contract(fast)
     constexpr int T = S + 2 * P, S3 = S * S * S, BPT = (S3 + 255) / 256;
-    const int ti = blockIdx.x / BPT, cb = blockIdx.x - ti * BPT;
-    const int node = m.tiles[ti];
-    const int depth = pos_depth(m.positions[node]);
-    if (depth == -1 || !active_depth(m.substep, (uint32_t)depth, m.max_depth)) return;
-    const int cell = cb * 256 + (int)threadIdx.x;
-    if (cell >= S3) return;
-    const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
-    const size_t r = (size_t)node * (T * T * T) + ((size_t)(z + P) * T + (y + P)) * T + (x + P);
-    const size_t w = (size_t)node * S3 + cell;
-    float k[NV];
-    bssn_cell<true, false>(a, r, w, par, k, nullptr, nullptr);
-    rk4_combine<NV, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
-    if (PUSH) rk4_push<NV, S, P>(a.new_, k, m.boundaries, node, x, y, z);
+    for (int item = blockIdx.x; item < m.n_items; item += gridDim.x) {
+        const int ti = item / BPT, cb = item - ti * BPT;
+        const int node = m.tiles[ti];
+        const int depth = pos_depth(m.positions[node]);
+        if (depth == -1 || !active_depth(m.substep, (uint32_t)depth, m.max_depth)) continue;
+        const int cell = cb * 256 + (int)threadIdx.x;
+        if (cell >= S3) continue;
+        const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
+        const size_t r = (size_t)node * (T * T * T) + ((size_t)(z + P) * T + (y + P)) * T + (x + P);
+        const size_t w = (size_t)node * S3 + cell;
+        float k[NV];
+        bssn_cell<true, false>(a, r, w, par, k, nullptr, nullptr);
+        rk4_combine<NV, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
+        if (PUSH) rk4_push<NV, S, P>(a.new_, k, m.boundaries, node, x, y, z);
+    }
 }
 
 // Per-workgroup FP64 partial sums {sum H^2, sum |M|^2, cells, max |H|}.
@@ -406,10 +409,11 @@ static int launch_stage(const am_bssn_fields *f, int stage, const am_bssn_params
     if (push && !mesh->boundaries) return set_error(AM_ERR_INVALID, "am_bssn_rhs_stage: push needs boundaries");
     BssnArgsDev a;
     if (int rc = fill_args(f, a, true)) return rc;
-    TileArgs m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth};
+    const int n_items = mesh->n_tiles * BPT;
+    TileArgs m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth, n_items};
     const DepthScales sc = depth_scales(*g);
     const BssnParDev d = dev_par(p);
-    dim3 grid(mesh->n_tiles * BPT);
+    dim3 grid(mesh->grid_limit > 0 && mesh->grid_limit < n_items ? mesh->grid_limit : n_items);
     if (push)
         launch_stage_push<S, P, true>(stage, grid, (hipStream_t)stream, a, m, d, sc);
     else
@@ -424,7 +428,8 @@ static int launch_cons(const am_bssn_fields *f, const am_bssn_params *p, const a
     constexpr int BPT = (S * S * S + 255) / 256;
     BssnArgsDev a;
     if (int rc = fill_args(f, a, false)) return rc;
-    TileArgs m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth};
+    TileArgs m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth,
+               mesh->n_tiles * BPT};
     hipLaunchKernelGGL((k_bssn_constraints<S, P>), dim3(mesh->n_tiles * BPT), dim3(256), 0, (hipStream_t)stream, a,
                        m, dev_par(p), partial);
     return check_launch("k_bssn_constraints");

@@ -26,6 +26,7 @@ struct FdBatchDev {
     uint32_t *d2off[AM_MAX_VARS];
     int32_t seed_d[AM_MAX_VARS];
     int32_t seed_m[AM_MAX_VARS];
+    const int32_t *seed_table;  // non-null: seed_d/seed_m index it
 };
 
 struct MeshDev {
@@ -33,6 +34,7 @@ struct MeshDev {
     const int32_t *tiles;
     uint32_t substep;
     uint32_t max_depth;
+    int32_t n_items;  // tiles x z-blocks; workgroups loop over them
 };
 
 // __mixed_partial_c / _bd (AMR.h:1221-1230) on LDS rows: c = centre, the
@@ -53,17 +55,20 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     constexpr FdCoef C = fd_coef();
     constexpr int BASE = Q * Q + Q - 1;
     __shared__ float4 ring4[R][T2 / 4];
-
-    const int tile_i = blockIdx.x / NZB, zb = blockIdx.x - tile_i * NZB;
     const int var = blockIdx.y;
+    const int32_t gs_d = b.seed_table ? b.seed_table[b.seed_d[var]] : b.seed_d[var];
+    const int32_t gs_m = b.seed_table ? b.seed_table[b.seed_m[var]] : b.seed_m[var];
+
+    // one work item = one (tile, z-block); a persistent launch loops
+    for (int item = blockIdx.x; item < m.n_items; item += gridDim.x) {
+    const int tile_i = item / NZB, zb = item - tile_i * NZB;
     const int slot = m.tiles[tile_i];
     const int depth = pos_depth(m.positions[slot]);
-    if (depth == -1 || !active_depth(m.substep, (uint32_t)depth, m.max_depth)) return;
+    if (depth == -1 || !active_depth(m.substep, (uint32_t)depth, m.max_depth)) continue;  // uniform per workgroup
 
     const float *src = b.field[var] + (size_t)slot * (T * T2);
     uint32_t *o_d1 = b.d1[var], *o_d2 = b.d2diag[var], *o_off = b.d2off[var];
     const bool want_d = o_d1 != nullptr, want_m = o_off != nullptr;
-    const int32_t gs_d = b.seed_d[var], gs_m = b.seed_m[var];
     const float dx1 = sc.inv_dx[depth];
     const float dx2 = dx1 * dx1;
 
@@ -167,6 +172,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
         }
         __syncthreads();
     }
+    }  // work items
 }
 
 template <int S, int P, int Q>
@@ -180,30 +186,49 @@ static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometr
         d.d2off[v] = b->d2off[v];
         d.seed_d[v] = b->seed_d[v];
         d.seed_m[v] = b->seed_m[v];
+        if (b->seed_table && (b->seed_d[v] < 0 || b->seed_m[v] < 0))
+            return set_error(AM_ERR_INVALID, "am_fd: negative seed index for variable %d", v);
         if (!b->field[v]) return set_error(AM_ERR_INVALID, "am_fd: null field for variable %d", v);
         if (!aligned16(b->field[v])) return set_error(AM_ERR_MISALIGNED, "am_fd: field %d not 16-byte aligned", v);
         if (!b->d1[v] && !b->d2off[v]) return set_error(AM_ERR_INVALID, "am_fd: variable %d has no output", v);
         if (!b->d1[v] && b->d2diag[v])
             return set_error(AM_ERR_INVALID, "am_fd: variable %d: diagonal Hessian needs the gradient word", v);
     }
-    MeshDev m{mesh->positions, mesh->tiles, mesh->substep, mesh->max_depth};
+    d.seed_table = b->seed_table;
     const DepthScales sc = depth_scales(*g);
     int zb = b->z_block;
     if (zb == 0) zb = S;  // whole tile per workgroup: no plane is read twice
     hipStream_t st = (hipStream_t)stream;
+    // grid.x = work items, capped by the persistent limit (split over the
+    // variables on grid.y)
+    auto items = [&](int nzb) {
+        return MeshDev{mesh->positions, mesh->tiles, mesh->substep, mesh->max_depth, mesh->n_tiles * nzb};
+    };
+    auto gx = [&](int nzb) {
+        const int n = mesh->n_tiles * nzb;
+        if (mesh->grid_limit <= 0) return n;
+        const int lim = (mesh->grid_limit + b->n_vars - 1) / b->n_vars;
+        return lim < n ? (lim > 0 ? lim : 1) : n;
+    };
     if (zb == S) {
-        dim3 grid(mesh->n_tiles * 1, b->n_vars);
-        hipLaunchKernelGGL((k_fd<S, P, Q, S>), grid, dim3(S * S), 0, st, d, m, sc);
+        hipLaunchKernelGGL((k_fd<S, P, Q, S>), dim3(gx(1), b->n_vars), dim3(S * S), 0, st, d, items(1), sc);
     } else if (zb == S / 2) {
-        dim3 grid(mesh->n_tiles * 2, b->n_vars);
-        hipLaunchKernelGGL((k_fd<S, P, Q, S / 2>), grid, dim3(S * S), 0, st, d, m, sc);
+        hipLaunchKernelGGL((k_fd<S, P, Q, S / 2>), dim3(gx(2), b->n_vars), dim3(S * S), 0, st, d, items(2), sc);
     } else if (zb == S / 4 && (S % 4) == 0) {
-        dim3 grid(mesh->n_tiles * 4, b->n_vars);
-        hipLaunchKernelGGL((k_fd<S, P, Q, S / 4>), grid, dim3(S * S), 0, st, d, m, sc);
+        hipLaunchKernelGGL((k_fd<S, P, Q, S / 4>), dim3(gx(4), b->n_vars), dim3(S * S), 0, st, d, items(4), sc);
     } else {
         return set_error(AM_ERR_INVALID, "am_fd: z_block %d must be S, S/2 or S/4", zb);
     }
     return check_launch("k_fd");
+}
+
+// table[i] = draw i + 1 of fast_prng from `seed`: the draws the host would
+// make, in its order, for the launches of a recorded graph
+__global__ void __launch_bounds__(64) k_seed_table(int32_t *table, int32_t n, int32_t seed)
+{
+    if (threadIdx.x != 0) return;
+    int32_t s = seed;
+    for (int32_t i = 0; i < n; i++) table[i] = prng_next(s);
 }
 
 static int fd_checks(const am_mesh *mesh, const am_geometry *g)
@@ -266,4 +291,12 @@ extern "C" int am_fd_all(const float *field, uint32_t *d1, uint32_t *d2diag, uin
 {
     if (!d1 || !d2diag || !d2off) return am::set_error(AM_ERR_INVALID, "am_fd_all: null output");
     return fd_single(field, d1, d2diag, d2off, mesh, g, seed_d, seed_m, stream);
+}
+
+extern "C" int am_seed_table_fill(int32_t *table, int32_t n, int32_t seed, void *stream)
+{
+    if (!table || n < 0) return am::set_error(AM_ERR_INVALID, "am_seed_table_fill: bad table");
+    if (n == 0) return AM_OK;
+    hipLaunchKernelGGL(am::k_seed_table, dim3(1), dim3(64), 0, (hipStream_t)stream, table, n, seed);
+    return am::check_launch("k_seed_table");
 }

@@ -416,8 +416,37 @@ int32_t am_wave_sim_rng_state(const am_wave_sim *s) { return s ? s->amr->sim_dat
     {                                                                                                              \
         if (!ms5 || !count5) return am::set_error(AM_ERR_INVALID, "null output");                                  \
         return rk4_call(s, [&](auto &d) { d.timer.totals(ms5, count5); });                                         \
+    }                                                                                                              \
+    int P##_set_launch(H *s, int32_t z_block, int32_t fd_grid_limit, int32_t rhs_grid_limit)                       \
+    {                                                                                                              \
+        if (fd_grid_limit < 0 || rhs_grid_limit < 0) return am::set_error(AM_ERR_INVALID, "negative grid limit");  \
+        return rk4_call(s, [&](auto &d) {                                                                          \
+            d.opt.z_block = z_block;                                                                               \
+            d.opt.fd_grid_limit = fd_grid_limit;                                                                   \
+            d.opt.rhs_grid_limit = rhs_grid_limit;                                                                 \
+            d.drop_graph();                                                                                        \
+        });                                                                                                        \
+    }                                                                                                              \
+    int P##_get_launch(const H *s, int32_t *out3)                                                                  \
+    {                                                                                                              \
+        if (!s || !out3) return am::set_error(AM_ERR_INVALID, "null argument");                                    \
+        const auto &o = s->amr->sim_data.opt;                                                                      \
+        out3[0] = o.z_block;                                                                                       \
+        out3[1] = o.fd_grid_limit;                                                                                 \
+        out3[2] = o.rhs_grid_limit;                                                                                \
+        return AM_OK;                                                                                              \
+    }                                                                                                              \
+    int P##_use_graph(H *s, int32_t enable)                                                                        \
+    {                                                                                                              \
+        return rk4_call(s, [&](auto &d) { d.set_use_graph(enable != 0); });                                        \
+    }                                                                                                              \
+    int P##_autotune(H *s, int32_t reps, am_tune_report *out)                                                      \
+    {                                                                                                              \
+        if (reps < 1) return am::set_error(AM_ERR_INVALID, "autotune needs reps >= 1");                            \
+        return rk4_call(s, [&](auto &d) { d.autotune(reps, reinterpret_cast<am::rk4_tune_report *>(out)); });      \
     }
 
+static_assert(sizeof(am_tune_report) == sizeof(am::rk4_tune_report), "tune report layout");
 AM_RK4_SIM_EXPORTS(am_bssn_sim, am_bssn_sim)
 AM_RK4_SIM_EXPORTS(am_swave_sim, am_swave_sim)
 

@@ -108,7 +108,68 @@ int am_memset_async(void *dst, int value, size_t bytes, void *stream)
     return am::hip_status(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream), "hipMemsetAsync");
 }
 int am_stream_sync(void *stream) { return am::hip_status(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize"); }
+int am_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream)
+{
+    return am::hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream),
+                          "hipMemcpyAsync(d2d)");
+}
+
+int am_stream_create(void **stream)
+{
+    if (!stream) return am::set_error(AM_ERR_INVALID, "am_stream_create: null output");
+    hipStream_t st = nullptr;
+    if (int rc = am::hip_status(hipStreamCreate(&st), "hipStreamCreate")) return rc;
+    *stream = (void *)st;
+    return AM_OK;
+}
+
+int am_stream_destroy(void *stream)
+{
+    if (!stream) return AM_OK;
+    return am::hip_status(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy");
+}
+
+int am_graph_begin(void *stream)
+{
+    if (!stream) return am::set_error(AM_ERR_INVALID, "am_graph_begin: capture needs a non-default stream");
+    return am::hip_status(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal),
+                          "hipStreamBeginCapture");
+}
+
+int am_graph_end(void *stream, void **exec)
+{
+    if (!exec) return am::set_error(AM_ERR_INVALID, "am_graph_end: null output");
+    hipGraph_t g = nullptr;
+    if (int rc = am::hip_status(hipStreamEndCapture((hipStream_t)stream, &g), "hipStreamEndCapture")) return rc;
+    hipGraphExec_t e = nullptr;
+    const int rc = am::hip_status(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    hipGraphDestroy(g);
+    if (rc) return rc;
+    *exec = (void *)e;
+    return AM_OK;
+}
+
+int am_graph_launch(void *exec, void *stream)
+{
+    if (!exec) return am::set_error(AM_ERR_INVALID, "am_graph_launch: null graph");
+    return am::hip_status(hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)stream), "hipGraphLaunch");
+}
+
+int am_graph_destroy(void *exec)
+{
+    if (!exec) return AM_OK;
+    return am::hip_status(hipGraphExecDestroy((hipGraphExec_t)exec), "hipGraphExecDestroy");
+}
+
 int am_device_sync(void) { return am::hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+int32_t am_device_cu_count(void)
+{
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return n;
+}
 
 int am_event_create(void **ev)
 {

@@ -23,25 +23,28 @@ struct SwaveTiles {
     const int32_t *tiles;
     const uint64_t *boundaries;
     uint32_t substep, max_depth;
+    int32_t n_items;  // tiles x 256-cell blocks; workgroups loop over them
 };
 
 template <int S, int P, int STAGE, bool PUSH>
 __global__ void __launch_bounds__(256) k_swave_stage(SwaveArgsDev a, SwaveTiles m, DepthScales sc)
 {
     constexpr int T = S + 2 * P, S3 = S * S * S, BPT = (S3 + 255) / 256;
-    const int ti = blockIdx.x / BPT, cb = blockIdx.x - ti * BPT;
-    const int node = m.tiles[ti];
-    const int depth = pos_depth(m.positions[node]);
-    if (depth == -1 || !active_depth(m.substep, (uint32_t)depth, m.max_depth)) return;
-    const int cell = cb * 256 + (int)threadIdx.x;
-    if (cell >= S3) return;
-    const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
-    const size_t r = (size_t)node * (T * T * T) + ((size_t)(z + P) * T + (y + P)) * T + (x + P);
-    const size_t w = (size_t)node * S3 + cell;
-    const float3 h = cf3_unpack(a.d2[w]);
-    float k[2] = {a.state[1][r], (h.x + h.y) + h.z};
-    rk4_combine<2, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
-    if (PUSH) rk4_push<2, S, P>(a.new_, k, m.boundaries, node, x, y, z);
+    for (int item = blockIdx.x; item < m.n_items; item += gridDim.x) {
+        const int ti = item / BPT, cb = item - ti * BPT;
+        const int node = m.tiles[ti];
+        const int depth = pos_depth(m.positions[node]);
+        if (depth == -1 || !active_depth(m.substep, (uint32_t)depth, m.max_depth)) continue;
+        const int cell = cb * 256 + (int)threadIdx.x;
+        if (cell >= S3) continue;
+        const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
+        const size_t r = (size_t)node * (T * T * T) + ((size_t)(z + P) * T + (y + P)) * T + (x + P);
+        const size_t w = (size_t)node * S3 + cell;
+        const float3 h = cf3_unpack(a.d2[w]);
+        float k[2] = {a.state[1][r], (h.x + h.y) + h.z};
+        rk4_combine<2, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
+        if (PUSH) rk4_push<2, S, P>(a.new_, k, m.boundaries, node, x, y, z);
+    }
 }
 
 // Per-workgroup FP64 partials {energy, sum u^2 h^3, cells, max |u|}.
@@ -126,9 +129,10 @@ static int swave_stage(const am_swave_fields *f, int stage, const am_mesh *mesh,
     if (push && !mesh->boundaries) return set_error(AM_ERR_INVALID, "am_swave_rhs_stage: push needs boundaries");
     SwaveArgsDev a;
     if (int rc = swave_args(f, a, true)) return rc;
-    const SwaveTiles m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth};
+    const int n_items = mesh->n_tiles * BPT;
+    const SwaveTiles m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth, n_items};
     const DepthScales sc = depth_scales(*g);
-    const dim3 grid(mesh->n_tiles * BPT);
+    const dim3 grid(mesh->grid_limit > 0 && mesh->grid_limit < n_items ? mesh->grid_limit : n_items);
     if (push)
         swave_launch<S, P, true>(stage, grid, (hipStream_t)stream, a, m, sc);
     else
@@ -143,7 +147,8 @@ static int swave_energy(const am_swave_fields *f, const am_mesh *mesh, const am_
     constexpr int BPT = (S * S * S + 255) / 256;
     SwaveArgsDev a;
     if (int rc = swave_args(f, a, false)) return rc;
-    const SwaveTiles m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth};
+    const SwaveTiles m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth,
+                       mesh->n_tiles * BPT};
     hipLaunchKernelGGL((k_swave_energy<S, P>), dim3(mesh->n_tiles * BPT), dim3(256), 0, (hipStream_t)stream, a, m,
                        (double)g->outer_dx, partial);
     return check_launch("k_swave_energy");

@@ -86,6 +86,11 @@ typedef struct am_mesh {
     int32_t n_tiles;
     uint32_t substep;
     uint32_t max_depth;
+    /* launch shape: 0 = one workgroup per work item (tile, or tile x block);
+     * > 0 = at most this many workgroups, each looping over work items
+     * (a persistent launch over the level; results do not depend on it).
+     * Honoured by am_fd_batched and the RHS stage kernels. */
+    int32_t grid_limit;
 } am_mesh;
 
 /* ------------------------------------------------------- device memory */
@@ -94,13 +99,35 @@ int am_device_free(void *ptr);
 int am_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
 int am_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
 int am_memset_async(void *dst, int value, size_t bytes, void *stream);
+int am_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 int am_stream_sync(void *stream);
+/* a blocking stream at the default priority (it orders with the NULL stream) */
+int am_stream_create(void **stream);
+int am_stream_destroy(void *stream);
 int am_device_sync(void);
+/* compute units of the current device (for persistent launch shapes) */
+int32_t am_device_cu_count(void);
 /* HIP events, for timing kernels on the stream they run on */
 int am_event_create(void **ev);
 int am_event_record(void *ev, void *stream);
 int am_event_elapsed_ms(void *start, void *stop, float *ms); /* synchronises on stop */
 int am_event_destroy(void *ev);
+
+/* ------------------------------------------------------------- graphs
+ * No counterpart in the reference, which launches eagerly between device
+ * syncs.  am_graph_begin starts capturing `stream`; am_graph_end stops and
+ * instantiates what was captured into *exec; am_graph_launch replays it on
+ * `stream`.  Captured work must not allocate, copy synchronously or memset
+ * (kernels only). */
+int am_graph_begin(void *stream);
+int am_graph_end(void *stream, void **exec);
+int am_graph_launch(void *exec, void *stream);
+int am_graph_destroy(void *exec);
+
+/* table[i] = the (i+1)-th fast_prng draw after state `seed`, i < n: the
+ * dither seeds a recorded launch reads through am_fd_batch.seed_table
+ * (one thread; n is a few thousand per coarse step). */
+int am_seed_table_fill(int32_t *table, int32_t n, int32_t seed, void *stream);
 
 /* ------------------------------------------------------- FD stencils */
 
@@ -140,7 +167,11 @@ typedef struct am_fd_batch {
     int32_t seed_d[AM_MAX_VARS];
     int32_t seed_m[AM_MAX_VARS];
     int32_t n_vars;
-    int32_t z_block; /* planes per workgroup: 0 = autotuned choice, else S, S/2 or S/4 */
+    int32_t z_block; /* planes per workgroup: 0 = whole tile, else S, S/2 or S/4 */
+    /* NULL: seed_d/seed_m are the per-launch dither seeds.  Otherwise they
+     * are indices into this device array, which holds the seeds: a launch
+     * recorded once in a graph then reads fresh seeds on every replay. */
+    const int32_t *seed_table;
 } am_fd_batch;
 int am_fd_batched(const am_fd_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream);
 

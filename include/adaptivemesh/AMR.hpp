@@ -150,6 +150,7 @@ struct octree {
 
    protected:
     bool dirty = true;
+    uint64_t version = 0;  // bumped by every structural change (recorded graphs check it)
     int max_depth = 0;
     uint32_t first_free_slot = 1;
     uint32_t final_used_slot = 0;
@@ -181,6 +182,7 @@ struct octree {
     uint32_t curr_used_slots() const { return final_used_slot + 1u; }
     int read_max_depth() const { return max_depth; }
     bool is_dirty() const { return dirty; }
+    uint64_t tree_version() const { return version; }
 
     // AMR::increment_timer (AMR.h:180-188)
     void increment_timer()
@@ -255,6 +257,7 @@ struct octree {
         }
         refresh_boundaries(new_depth);
         dirty = true;
+        version++;
     }
     void refresh_boundaries(int min_depth)
     {
@@ -290,6 +293,7 @@ struct octree {
         }
         refresh_boundaries(curr_depth);
         dirty = true;
+        version++;
         return AM_OK;
     }
 
@@ -365,6 +369,7 @@ struct octree {
         m.n_tiles = n;
         m.substep = timer_helper;
         m.max_depth = (uint32_t)max_depth;
+        m.grid_limit = 0;
         return m;
     }
 

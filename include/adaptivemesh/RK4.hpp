@@ -14,9 +14,21 @@
 // A System provides: NV fields; FD_KIND[v] (0 none, 1 gradient, 2 gradient
 // + diagonal Hessian, 3 gradient + full Hessian); CHRISTOFFEL (metric in
 // fields 1..6); options_t; rhs_stage(...) and diagnostics(...).
+//
+// Launch machinery (no counterpart in the reference, which launches eagerly
+// between device-wide syncs): the driver owns one stream; every depth level
+// is one launch per kernel over its tile list, optionally persistent (a
+// capped grid whose workgroups loop over the level's work items); with
+// use_graph, a whole timestep() -- every sub-step of the coarse step -- is
+// recorded once as a hipGraph and replayed, the dither seeds coming from a
+// device table that a one-thread kernel refills before each replay with
+// exactly the draws the eager schedule would make; autotune() times the
+// launch shapes on the live mesh and keeps the fastest.
 #pragma once
 #include <math.h>
 
+#include <algorithm>
+#include <stdexcept>
 #include <vector>
 
 #include "AMR.hpp"
@@ -78,7 +90,19 @@ struct kernel_timer {
 
 struct rk4_options {
     int32_t interp = AM_INTERP_LAGRANGE;  // coarse-face ghosts: reference quadratic, or tricubic + quadratic time
-    int32_t z_block = 0;                  // FD launch shape (0 = whole tile per workgroup)
+    int32_t z_block = 0;                  // FD planes per workgroup (0 = whole tile)
+    int32_t fd_grid_limit = 0;            // persistent FD launch: workgroup cap (0 = one per item)
+    int32_t rhs_grid_limit = 0;           // persistent RHS launch: workgroup cap (0 = one per item)
+};
+
+// What autotune() measured: each trial's launch shape and mean ms per launch.
+struct rk4_tune_report {
+    int32_t z_block, fd_grid_limit, rhs_grid_limit, n_trials;
+    struct {
+        int32_t kind;  // 0 FD z_block, 1 FD grid limit, 2 RHS grid limit
+        int32_t value;
+        float ms;
+    } trial[32];
 };
 
 // Device views of the driver's buffers handed to a System's kernels.
@@ -116,6 +140,15 @@ struct rk4_AMR_data {
     uint64_t zone_cycles = 0;  // interior cells advanced by one full step, all depths
     kernel_timer timer;
 
+    // recorded timestep
+    static constexpr int32_t SEED_CAPACITY = 1 << 16;
+    smart_gpu_buffer<int32_t> seed_table;
+    bool use_graph = false, recording = false;
+    int32_t n_draws = 0;             // draws of one recorded timestep
+    void *graph = nullptr;           // instantiated hipGraphExec
+    uint64_t graph_version = ~0ull;  // octree version it was recorded against
+    uint64_t graph_zone_cycles = 0;  // zone-cycles one replay advances
+
     rk4_AMR_data(uint32_t node_slots, AMR<rk4_AMR_data> &p, const am_geometry &g, const rk4_options &o,
                  const options_t &so)
         : rng(), parent(p), geom(g), opt(o), sys(so)
@@ -137,14 +170,26 @@ struct rk4_AMR_data {
         if (n_off) d2o = smart_gpu_buffer<uint32_t>(n_off * slots * inner);
         if (System::CHRISTOFFEL) chr = smart_gpu_buffer<uint32_t>(6 * slots * inner);
         partial = smart_gpu_buffer<double>(4 * (size_t)am_bssn_constraint_blocks(&geom, (int32_t)slots) + 4);
+        seed_table = smart_gpu_buffer<int32_t>(SEED_CAPACITY);
         old_.zero(); new_.zero(); acc.zero();
         if (older.created) older.zero();
+        check(am_stream_create(&stream), "am_stream_create");
     }
     void destroy()
     {
+        drop_graph();
+        if (stream) am_stream_sync(stream);
         old_.destroy(); new_.destroy(); acc.destroy(); older.destroy();
-        d1.destroy(); d2d.destroy(); d2o.destroy(); chr.destroy(); partial.destroy();
+        d1.destroy(); d2d.destroy(); d2o.destroy(); chr.destroy(); partial.destroy(); seed_table.destroy();
         timer.destroy();
+        am_stream_destroy(stream);
+        stream = nullptr;
+    }
+    void drop_graph()
+    {
+        if (graph) am_graph_destroy(graph);
+        graph = nullptr;
+        graph_version = ~0ull;
     }
     float *field(smart_gpu_buffer<float> &b, int v)
     {
@@ -190,13 +235,24 @@ struct rk4_AMR_data {
     void node_dtor(int) {}
     void modify_nodes() {}
 
+    // the next dither seed: a host draw, or while recording the index of the
+    // draw in the seed table
+    int32_t draw()
+    {
+        if (!recording) return rng.generate_int();
+        if (n_draws >= SEED_CAPACITY) throw std::runtime_error("recorded timestep needs more than 65536 seeds");
+        return n_draws++;
+    }
+
     // derivative words of the state in `src` for the tiles of `m`; seeds are
     // drawn per field in order (gradient+diagonal, then mixed), then one for
     // the Christoffel words
-    void derivatives(smart_gpu_buffer<float> &src, const am_mesh &m)
+    void derivatives(smart_gpu_buffer<float> &src, am_mesh m)
     {
         am_fd_batch b{};
         b.z_block = opt.z_block;
+        b.seed_table = recording ? seed_table.gpu_buffer_ptr : nullptr;
+        m.grid_limit = opt.fd_grid_limit;
         int h = 0, o = 0;
         for (int v = 0; v < NV; v++) {
             const int kind = System::FD_KIND[v];
@@ -204,11 +260,11 @@ struct rk4_AMR_data {
             const int i = b.n_vars++;
             b.field[i] = field(src, v);
             b.d1[i] = word(d1, v);
-            b.seed_d[i] = rng.generate_int();
+            b.seed_d[i] = draw();
             if (kind >= 2) b.d2diag[i] = word(d2d, h++);
             if (kind == 3) {
                 b.d2off[i] = word(d2o, o++);
-                b.seed_m[i] = rng.generate_int();
+                b.seed_m[i] = draw();
             }
         }
         int ev = timer.begin(kernel_timer::FD);
@@ -224,6 +280,7 @@ struct rk4_AMR_data {
                 out[c] = word(chr, c);
             }
             ev = timer.begin(kernel_timer::CHRISTOFFEL);
+            if (recording) throw std::runtime_error("Christoffel words cannot be recorded (per-launch seed)");
             check(am_christoffel(met, dg, out, 0, &m, &geom, rng.generate_int(), stream), "christoffel");
             timer.end(ev);
         }
@@ -250,8 +307,10 @@ struct rk4_AMR_data {
         static const float frac[4] = {.5f, .5f, 1.f, 1.f};
         derivatives(s == 0 ? old_ : new_, m);
         const rk4_views f = views(s == 0);
+        am_mesh mr = m;
+        mr.grid_limit = opt.rhs_grid_limit;
         int ev = timer.begin(kernel_timer::RHS);
-        check(System::rhs_stage(f, s, sys, m, geom, 1, stream), "rhs");
+        check(System::rhs_stage(f, s, sys, mr, geom, 1, stream), "rhs");
         timer.end(ev);
         if (d > 0 && parent.coarse_faces_at_depth(d) > 0) {
             am_field_batch b = batch(true);
@@ -272,12 +331,141 @@ struct rk4_AMR_data {
             if (parent.depth_active(d)) step_depth(d);
         parent.increment_timer();
     }
-    // the reference's timestep(): sub-steps until the counter wraps
+    // the reference's timestep(): sub-steps until the counter wraps; with
+    // use_graph (and the counter at 0) one replay of the recorded coarse step
     void timestep()
     {
+        if (use_graph && parent.timer_helper == 0u) {
+            if (!graph || graph_version != parent.tree_version()) record();
+            check(am_seed_table_fill(seed_table.gpu_buffer_ptr, n_draws, rng.seed, stream), "seed table");
+            for (int32_t i = 0; i < n_draws; i++) rng.generate_int();  // the host stream of draws moves on alike
+            check(am_graph_launch(graph, stream), "graph launch");
+            zone_cycles += graph_zone_cycles;
+            return;
+        }
         do {
             substep();
         } while (parent.timer_helper != 0u);
+    }
+    // record one coarse step (counter 0 back to 0) into a graph
+    void record()
+    {
+        drop_graph();
+        parent.mesh_all();  // device octree arrays current before capture
+        const uint64_t z0 = zone_cycles;
+        const bool timing = timer.on;
+        timer.on = false;
+        n_draws = 0;
+        recording = true;
+        check(am_graph_begin(stream), "graph begin");
+        try {
+            do {
+                substep();
+            } while (parent.timer_helper != 0u);
+        } catch (...) {
+            void *g = nullptr;
+            am_graph_end(stream, &g);
+            if (g) am_graph_destroy(g);
+            recording = false;
+            timer.on = timing;
+            zone_cycles = z0;
+            throw;
+        }
+        recording = false;
+        timer.on = timing;
+        check(am_graph_end(stream, &graph), "graph end");
+        graph_zone_cycles = zone_cycles - z0;
+        zone_cycles = z0;
+        graph_version = parent.tree_version();
+    }
+    void set_use_graph(bool on)
+    {
+        use_graph = on;
+        if (!on) drop_graph();
+    }
+
+    // Time the launch shapes on the live mesh (the level with the most tiles,
+    // every tile active) and keep the fastest: FD z-block, FD and RHS
+    // workgroup caps.  new_, acc and the host seed are restored, so the run
+    // continues as if nothing happened.
+    void autotune(int reps, rk4_tune_report *rep)
+    {
+        drop_graph();
+        int dbest = 0;
+        for (int d = 0; d <= parent.read_max_depth(); d++)
+            if (parent.tiles_at_depth(d) > parent.tiles_at_depth(dbest)) dbest = d;
+        am_mesh m = parent.mesh_depth(dbest);
+        m.substep = 0;
+        const size_t fbytes = (size_t)NV * slots * cells * sizeof(float), abytes = (size_t)NV * slots * inner * 4;
+        smart_gpu_buffer<float> save_new((size_t)NV * slots * cells), save_acc((size_t)NV * slots * inner);
+        check(am_memcpy_d2d(save_new.gpu_buffer_ptr, new_.gpu_buffer_ptr, fbytes, stream), "autotune save");
+        check(am_memcpy_d2d(save_acc.gpu_buffer_ptr, acc.gpu_buffer_ptr, abytes, stream), "autotune save");
+        const int32_t seed = rng.seed;
+        const rk4_options keep = opt;
+        const bool timing = timer.on;
+        timer.on = false;
+        void *e0 = nullptr, *e1 = nullptr;
+        check(am_event_create(&e0), "event");
+        check(am_event_create(&e1), "event");
+        rk4_tune_report r{};
+        auto time_it = [&](auto &&fn) {
+            fn();  // warm-up
+            check(am_event_record(e0, stream), "event");
+            for (int i = 0; i < reps; i++) fn();
+            check(am_event_record(e1, stream), "event");
+            float ms = 0.f;
+            check(am_event_elapsed_ms(e0, e1, &ms), "event");
+            return ms / (float)reps;
+        };
+        auto note = [&](int kind, int value, float ms) {
+            if (r.n_trials < 32) r.trial[r.n_trials++] = {kind, value, ms};
+        };
+        const int S = geom.size_domain, cu = std::max(am_device_cu_count(), 1);
+        // FD: planes per workgroup, then the workgroup cap
+        float best = 1e30f;
+        int zb_best = keep.z_block;
+        for (int zb : {S, S / 2, S / 4}) {
+            opt.z_block = zb;
+            const float ms = time_it([&] { derivatives(new_, m); });
+            note(0, zb, ms);
+            if (ms < best) { best = ms; zb_best = zb; }
+        }
+        opt.z_block = zb_best;
+        best = 1e30f;
+        int fl_best = 0;
+        for (int lim : {0, 4 * cu, 8 * cu, 16 * cu}) {
+            opt.fd_grid_limit = lim;
+            const float ms = time_it([&] { derivatives(new_, m); });
+            note(1, lim, ms);
+            if (ms < best) { best = ms; fl_best = lim; }
+        }
+        opt.fd_grid_limit = fl_best;
+        // RHS: stage 1 (reads y_n, acc; writes new, acc, pushed ghosts)
+        derivatives(new_, m);
+        best = 1e30f;
+        int rl_best = 0;
+        for (int lim : {0, 2 * cu, 4 * cu, 8 * cu}) {
+            am_mesh mr = m;
+            mr.grid_limit = lim;
+            const rk4_views f = views(false);
+            const float ms = time_it([&] { check(System::rhs_stage(f, 1, sys, mr, geom, 1, stream), "rhs"); });
+            note(2, lim, ms);
+            if (ms < best) { best = ms; rl_best = lim; }
+        }
+        opt.rhs_grid_limit = rl_best;
+        check(am_memcpy_d2d(new_.gpu_buffer_ptr, save_new.gpu_buffer_ptr, fbytes, stream), "autotune restore");
+        check(am_memcpy_d2d(acc.gpu_buffer_ptr, save_acc.gpu_buffer_ptr, abytes, stream), "autotune restore");
+        check(am_stream_sync(stream), "autotune");
+        save_new.destroy();
+        save_acc.destroy();
+        am_event_destroy(e0);
+        am_event_destroy(e1);
+        rng.seed = seed;
+        timer.on = timing;
+        r.z_block = opt.z_block;
+        r.fd_grid_limit = opt.fd_grid_limit;
+        r.rhs_grid_limit = opt.rhs_grid_limit;
+        if (rep) *rep = r;
     }
     // fill every face ghost of `new` from the current data (initial data):
     // equal-depth faces copy, coarser faces interpolate

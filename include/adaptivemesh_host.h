@@ -54,6 +54,25 @@ int am_wave_sim_copy_bounds(am_wave_sim *s);
 int am_wave_sim_timestep(am_wave_sim *s);
 int32_t am_wave_sim_rng_state(const am_wave_sim *s);
 
+/* ------------------------------------------- RK4 driver launch machinery
+ * Shared by am_bssn_sim and am_swave_sim (no counterpart in the reference):
+ *   _set_launch  FD planes per workgroup (0 = whole tile, else S, S/2, S/4)
+ *                and workgroup caps of persistent FD / RHS launches (0 =
+ *                one workgroup per work item);
+ *   _use_graph   timestep() records the coarse step once as a hipGraph and
+ *                replays it (results identical to eager launches: the dither
+ *                seeds are refilled on the device before every replay);
+ *   _autotune    times the launch shapes on the live mesh, keeps the
+ *                fastest and restores the state it touched. */
+typedef struct am_tune_report {
+    int32_t z_block, fd_grid_limit, rhs_grid_limit, n_trials;
+    struct {
+        int32_t kind; /* 0 FD z_block, 1 FD workgroup cap, 2 RHS workgroup cap */
+        int32_t value;
+        float ms; /* mean per launch */
+    } trial[32];
+} am_tune_report;
+
 /* --------------------------------------------- BSSN RK4 evolution driver
  * AMR<BSSN_rk4_data>: the 24-field moving-puncture BSSN system advanced by
  * classical RK4 with sub-cycling (no counterpart in the reference, whose
@@ -103,6 +122,10 @@ int am_bssn_sim_counter(const am_bssn_sim *s, uint32_t *timer, int32_t *max_dept
  * counts per class since profiling was (re)enabled */
 int am_bssn_sim_profile(am_bssn_sim *s, int32_t enable);
 int am_bssn_sim_kernel_times(am_bssn_sim *s, double *ms5, int64_t *count5);
+int am_bssn_sim_set_launch(am_bssn_sim *s, int32_t z_block, int32_t fd_grid_limit, int32_t rhs_grid_limit);
+int am_bssn_sim_get_launch(const am_bssn_sim *s, int32_t *out3);
+int am_bssn_sim_use_graph(am_bssn_sim *s, int32_t enable);
+int am_bssn_sim_autotune(am_bssn_sim *s, int32_t reps, am_tune_report *out);
 
 /* -------------------------------------------- scalar wave RK4 driver
  * AMR<swave_rk4_data>: u_tt = lap u as fields (u, v = u_t) on the same RK4
@@ -144,6 +167,10 @@ int am_swave_sim_set_rng_state(am_swave_sim *s, int32_t seed);
 int am_swave_sim_counter(const am_swave_sim *s, uint32_t *timer, int32_t *max_depth);
 int am_swave_sim_profile(am_swave_sim *s, int32_t enable);
 int am_swave_sim_kernel_times(am_swave_sim *s, double *ms5, int64_t *count5);
+int am_swave_sim_set_launch(am_swave_sim *s, int32_t z_block, int32_t fd_grid_limit, int32_t rhs_grid_limit);
+int am_swave_sim_get_launch(const am_swave_sim *s, int32_t *out3);
+int am_swave_sim_use_graph(am_swave_sim *s, int32_t enable);
+int am_swave_sim_autotune(am_swave_sim *s, int32_t reps, am_tune_report *out);
 
 #ifdef __cplusplus
 }

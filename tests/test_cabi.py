@@ -26,11 +26,27 @@ def test_every_declared_symbol_is_exported(am):
     assert not sorted(names - set(am.exported_symbols()))
 
 
-def test_struct_sizes(am):
-    assert C.sizeof(am.Geometry) == 24
-    assert C.sizeof(am.Mesh) == 48
-    assert C.sizeof(am.FdBatch) == 8 * 32 * 4 + 4 * 32 * 2 + 8
-    assert C.sizeof(am.FieldBatch) == 8 * 32 * 3 + 8
+STRUCTS = {"Geometry": "am_geometry", "Mesh": "am_mesh", "FdBatch": "am_fd_batch", "FieldBatch": "am_field_batch",
+           "BssnParams": "am_bssn_params", "BssnFields": "am_bssn_fields", "BssnSimOptions": "am_bssn_sim_options",
+           "SwaveFields": "am_swave_fields", "SwaveSimOptions": "am_swave_sim_options",
+           "TuneReport": "am_tune_report"}
+
+
+def test_struct_sizes(am, tmp_path):
+    """Every ctypes mirror has the size the C compiler gives the header's
+    struct (a C program built from include/*.h prints them)."""
+    import subprocess
+    from pathlib import Path
+    inc = Path(__file__).resolve().parents[1] / "include"
+    src = tmp_path / "sizes.c"
+    body = "".join(f'    printf("%zu\\n", sizeof({c}));\n' for c in STRUCTS.values())
+    src.write_text('#include <stdio.h>\n#include "adaptivemesh.h"\n#include "adaptivemesh_host.h"\n'
+                   f"int main(void) {{\n{body}    return 0;\n}}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-std=c99", "-I", str(inc), str(src), "-o", str(exe)], check=True)
+    sizes = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    for (py, c), n in zip(STRUCTS.items(), sizes):
+        assert C.sizeof(getattr(am, py)) == n, (py, c, C.sizeof(getattr(am, py)), n)
 
 
 def test_geometry_init_matches_reference_constants(am):
