@@ -291,12 +291,15 @@ struct TileArgs {
     int32_t n_items;  // tiles x 256-cell blocks; workgroups loop over them
 };
 
-template <int S, int P, int STAGE, bool PUSH>
+template <int S, int P, int STAGE, bool PUSH, bool PERSIST>
 __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, BssnParDev par, DepthScales sc)
 {
 #pragma clang fp contract(fast)
     constexpr int T = S + 2 * P, S3 = S * S * S, BPT = (S3 + 255) / 256;
-    for (int item = blockIdx.x; item < m.n_items; item += gridDim.x) {
+    // persistent launches loop over work items (a separate instantiation:
+    // the loop keeps every kernel-argument pointer live across iterations)
+    const int item_end = PERSIST ? m.n_items : (int)blockIdx.x + 1;
+    for (int item = blockIdx.x; item < item_end; item += PERSIST ? gridDim.x : 1) {
         const int ti = item / BPT, cb = item - ti * BPT;
         const int node = m.tiles[ti];
         const int depth = pos_depth(m.positions[node]);
@@ -388,15 +391,15 @@ static BssnParDev dev_par(const am_bssn_params *p)
     return d;
 }
 
-template <int S, int P, bool PUSH>
+template <int S, int P, bool PUSH, bool PERSIST>
 static void launch_stage_push(int stage, dim3 grid, hipStream_t st, const BssnArgsDev &a, const TileArgs &m,
                               const BssnParDev &d, const DepthScales &sc)
 {
     switch (stage) {
-    case 0: hipLaunchKernelGGL((k_bssn_stage<S, P, 0, PUSH>), grid, dim3(256), 0, st, a, m, d, sc); break;
-    case 1: hipLaunchKernelGGL((k_bssn_stage<S, P, 1, PUSH>), grid, dim3(256), 0, st, a, m, d, sc); break;
-    case 2: hipLaunchKernelGGL((k_bssn_stage<S, P, 2, PUSH>), grid, dim3(256), 0, st, a, m, d, sc); break;
-    default: hipLaunchKernelGGL((k_bssn_stage<S, P, 3, PUSH>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    case 0: hipLaunchKernelGGL((k_bssn_stage<S, P, 0, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    case 1: hipLaunchKernelGGL((k_bssn_stage<S, P, 1, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    case 2: hipLaunchKernelGGL((k_bssn_stage<S, P, 2, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    default: hipLaunchKernelGGL((k_bssn_stage<S, P, 3, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, d, sc); break;
     }
 }
 
@@ -414,10 +417,16 @@ static int launch_stage(const am_bssn_fields *f, int stage, const am_bssn_params
     const DepthScales sc = depth_scales(*g);
     const BssnParDev d = dev_par(p);
     dim3 grid(mesh->grid_limit > 0 && mesh->grid_limit < n_items ? mesh->grid_limit : n_items);
-    if (push)
-        launch_stage_push<S, P, true>(stage, grid, (hipStream_t)stream, a, m, d, sc);
+    const bool persist = grid.x < (unsigned)n_items;
+    hipStream_t st = (hipStream_t)stream;
+    if (push && persist)
+        launch_stage_push<S, P, true, true>(stage, grid, st, a, m, d, sc);
+    else if (push)
+        launch_stage_push<S, P, true, false>(stage, grid, st, a, m, d, sc);
+    else if (persist)
+        launch_stage_push<S, P, false, true>(stage, grid, st, a, m, d, sc);
     else
-        launch_stage_push<S, P, false>(stage, grid, (hipStream_t)stream, a, m, d, sc);
+        launch_stage_push<S, P, false, false>(stage, grid, st, a, m, d, sc);
     return check_launch("k_bssn_stage");
 }
 

@@ -48,7 +48,7 @@ __device__ __forceinline__ float mixed_bd(float c, float pp, float mm, float m0,
     return c + .5f * (pp + mm - m0 - p0 - zm - zp);
 }
 
-template <int S, int P, int Q, int ZB>
+template <int S, int P, int Q, int ZB, bool PERSIST>
 __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScales sc)
 {
     constexpr int T = S + 2 * P, T2 = T * T, R = 2 * P + 2, NZB = S / ZB;
@@ -59,8 +59,10 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     const int32_t gs_d = b.seed_table ? b.seed_table[b.seed_d[var]] : b.seed_d[var];
     const int32_t gs_m = b.seed_table ? b.seed_table[b.seed_m[var]] : b.seed_m[var];
 
-    // one work item = one (tile, z-block); a persistent launch loops
-    for (int item = blockIdx.x; item < m.n_items; item += gridDim.x) {
+    // one work item = one (tile, z-block); a persistent launch loops over
+    // them (a separate instantiation: the loop keeps kernel arguments live)
+    const int item_end = PERSIST ? m.n_items : (int)blockIdx.x + 1;
+    for (int item = blockIdx.x; item < item_end; item += PERSIST ? gridDim.x : 1) {
     const int tile_i = item / NZB, zb = item - tile_i * NZB;
     const int slot = m.tiles[tile_i];
     const int depth = pos_depth(m.positions[slot]);
@@ -175,6 +177,16 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     }  // work items
 }
 
+template <int S, int P, int Q, int ZB>
+static void fd_launch(bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
+                      const DepthScales &sc)
+{
+    if (persist)
+        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true>), grid, dim3(S * S), 0, st, d, m, sc);
+    else
+        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false>), grid, dim3(S * S), 0, st, d, m, sc);
+}
+
 template <int S, int P, int Q>
 static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream)
 {
@@ -210,12 +222,13 @@ static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometr
         const int lim = (mesh->grid_limit + b->n_vars - 1) / b->n_vars;
         return lim < n ? (lim > 0 ? lim : 1) : n;
     };
+    const bool persist = mesh->grid_limit > 0;
     if (zb == S) {
-        hipLaunchKernelGGL((k_fd<S, P, Q, S>), dim3(gx(1), b->n_vars), dim3(S * S), 0, st, d, items(1), sc);
+        fd_launch<S, P, Q, S>(persist, dim3(gx(1), b->n_vars), st, d, items(1), sc);
     } else if (zb == S / 2) {
-        hipLaunchKernelGGL((k_fd<S, P, Q, S / 2>), dim3(gx(2), b->n_vars), dim3(S * S), 0, st, d, items(2), sc);
+        fd_launch<S, P, Q, S / 2>(persist, dim3(gx(2), b->n_vars), st, d, items(2), sc);
     } else if (zb == S / 4 && (S % 4) == 0) {
-        hipLaunchKernelGGL((k_fd<S, P, Q, S / 4>), dim3(gx(4), b->n_vars), dim3(S * S), 0, st, d, items(4), sc);
+        fd_launch<S, P, Q, S / 4>(persist, dim3(gx(4), b->n_vars), st, d, items(4), sc);
     } else {
         return set_error(AM_ERR_INVALID, "am_fd: z_block %d must be S, S/2 or S/4", zb);
     }

@@ -26,11 +26,12 @@ struct SwaveTiles {
     int32_t n_items;  // tiles x 256-cell blocks; workgroups loop over them
 };
 
-template <int S, int P, int STAGE, bool PUSH>
+template <int S, int P, int STAGE, bool PUSH, bool PERSIST>
 __global__ void __launch_bounds__(256) k_swave_stage(SwaveArgsDev a, SwaveTiles m, DepthScales sc)
 {
     constexpr int T = S + 2 * P, S3 = S * S * S, BPT = (S3 + 255) / 256;
-    for (int item = blockIdx.x; item < m.n_items; item += gridDim.x) {
+    const int item_end = PERSIST ? m.n_items : (int)blockIdx.x + 1;
+    for (int item = blockIdx.x; item < item_end; item += PERSIST ? gridDim.x : 1) {
         const int ti = item / BPT, cb = item - ti * BPT;
         const int node = m.tiles[ti];
         const int depth = pos_depth(m.positions[node]);
@@ -108,15 +109,15 @@ static int swave_args(const am_swave_fields *f, SwaveArgsDev &a, bool need_stage
     return AM_OK;
 }
 
-template <int S, int P, bool PUSH>
+template <int S, int P, bool PUSH, bool PERSIST>
 static void swave_launch(int stage, dim3 grid, hipStream_t st, const SwaveArgsDev &a, const SwaveTiles &m,
                          const DepthScales &sc)
 {
     switch (stage) {
-    case 0: hipLaunchKernelGGL((k_swave_stage<S, P, 0, PUSH>), grid, dim3(256), 0, st, a, m, sc); break;
-    case 1: hipLaunchKernelGGL((k_swave_stage<S, P, 1, PUSH>), grid, dim3(256), 0, st, a, m, sc); break;
-    case 2: hipLaunchKernelGGL((k_swave_stage<S, P, 2, PUSH>), grid, dim3(256), 0, st, a, m, sc); break;
-    default: hipLaunchKernelGGL((k_swave_stage<S, P, 3, PUSH>), grid, dim3(256), 0, st, a, m, sc); break;
+    case 0: hipLaunchKernelGGL((k_swave_stage<S, P, 0, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, sc); break;
+    case 1: hipLaunchKernelGGL((k_swave_stage<S, P, 1, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, sc); break;
+    case 2: hipLaunchKernelGGL((k_swave_stage<S, P, 2, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, sc); break;
+    default: hipLaunchKernelGGL((k_swave_stage<S, P, 3, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, sc); break;
     }
 }
 
@@ -133,10 +134,16 @@ static int swave_stage(const am_swave_fields *f, int stage, const am_mesh *mesh,
     const SwaveTiles m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth, n_items};
     const DepthScales sc = depth_scales(*g);
     const dim3 grid(mesh->grid_limit > 0 && mesh->grid_limit < n_items ? mesh->grid_limit : n_items);
-    if (push)
-        swave_launch<S, P, true>(stage, grid, (hipStream_t)stream, a, m, sc);
+    const bool persist = grid.x < (unsigned)n_items;
+    hipStream_t st = (hipStream_t)stream;
+    if (push && persist)
+        swave_launch<S, P, true, true>(stage, grid, st, a, m, sc);
+    else if (push)
+        swave_launch<S, P, true, false>(stage, grid, st, a, m, sc);
+    else if (persist)
+        swave_launch<S, P, false, true>(stage, grid, st, a, m, sc);
     else
-        swave_launch<S, P, false>(stage, grid, (hipStream_t)stream, a, m, sc);
+        swave_launch<S, P, false, false>(stage, grid, st, a, m, sc);
     return check_launch("k_swave_stage");
 }
 

@@ -2,15 +2,19 @@
 """BSSN zone-cycles/s on one MI355X, with the roofline of the dominant kernel
 and a CPU baseline.
 
-Workload (one GPU): the BSSN single-level mesh -- 512 tiles of 16^3 interior
-cells (128^3) at depth 3 with 3-cell padding, their 73 ancestors, periodic
-outer faces, domain [-32, 32]^3, 24 fp32 fields, 1+log slicing and
+Default workload (one GPU): the BSSN single-level mesh -- 512 tiles of 16^3
+interior cells (128^3) at depth 3 with 3-cell padding, their 73 ancestors,
+periodic outer faces, domain [-32, 32]^3, 24 fp32 fields, 1+log slicing and
 Gamma-driver shift, flat metric plus seeded uniform noise of +-1e-6 on every
-field, FD order 4, derivatives and Christoffel symbols as compressed_float3
-words, CFL 0.25.  One step = one timestep() of the octree driver = 2^3 fine
-RK4 sub-steps (depth d advances every 2^(3-d) sub-steps), so the default
-12 steps are ~100 fine RK4 steps.  A zone-cycle is one interior cell
-advanced by one full RK4 step at its own depth, ancestors included.
+field, FD order 4, derivatives as compressed_float3 words, CFL 0.25.  One
+step = one timestep() of the octree driver = 2^3 fine RK4 sub-steps (depth
+d advances every 2^(3-d) sub-steps), so the default 12 steps are ~100 fine
+RK4 steps.  A zone-cycle is one interior cell advanced by one full RK4 step
+at its own depth, ancestors included.  Before timing, the autotuner picks
+the launch shapes and the coarse step is recorded as a hipGraph (replays
+are bit-identical to eager launches); --no-autotune / --no-graph turn them
+off.  --workload puncture times the octree puncture run instead (its own
+JSON line, not the headline).
 
 Output: one JSON line (rank 0).  --gpus > 1 is refused: this build runs on
 one GPU.
@@ -48,6 +52,14 @@ BYTES = {
     "ghost": 24 * 6 * P * S * S * 8 / S ** 3,
     # per step: T^3 read + T^3 write per field
     "copy_back": 24 * 2 * 4 * (S + 2 * P) ** 3 / S ** 3,
+}
+WORKLOADS = {
+    "bssn512": "BSSN single level: 512 tiles of 16^3 (128^3) + 73 ancestors, P=3, 24 fp32 fields, 1+log + "
+               "Gamma-driver, flat + 1e-6 noise, cf3 derivative words (Christoffel symbols raised in registers), RK4, "
+               "FD order 4, periodic, [-32,32]^3, CFL 0.25",
+    "puncture": "BSSN octree: single puncture (M=1) on [-32,32]^3, depth-2 base of 64 tiles refined where "
+                "|centre| < 16/2^k to depth 5 (1/8/64/64/64/64 tiles), P=3, 24 fp32 fields, RK4 sub-cycled 2x per "
+                "level, tricubic + quadratic-time coarse-face ghosts, FD order 4, CFL 0.25",
 }
 STAGES_PER_STEP = {"fd": 4, "rhs_stage": 1, "ghost": 4, "copy_back": 1}
 # kernel classes launched by this workload (every face of the uniform periodic mesh has an equal-depth target,
@@ -161,7 +173,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing pass")
-    ap.add_argument("--z-block", type=int, default=0)
+    ap.add_argument("--z-block", type=int, default=0, help="fixed FD z-block (skips the autotuner)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="bssn512")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a recorded step")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time only a plain 1 GiB device copy (the calibration for HBM byte counters)")
     args = ap.parse_args()
@@ -176,18 +191,34 @@ def main():
 
     import adaptivemesh_amd as am
     import torch
+    from adaptivemesh_amd import workloads as WL
 
     T = S + 2 * P
-    g = am.geometry(S, P, FD_ORDER, OUTER, CFL)
-    n_slots = 1 + 8 + 64 + 512
-    sim = am.BssnSim(g, n_slots, am.bssn_options(periodic=True, z_block=args.z_block))
-    assert build_mesh(sim) == n_slots
-    fields = initial_data(n_slots, T)
+    if args.workload == "bssn512":
+        g = am.geometry(S, P, FD_ORDER, OUTER, CFL)
+        n_slots = 1 + 8 + 64 + 512
+        sim = am.BssnSim(g, n_slots, am.bssn_options(periodic=True))
+        assert build_mesh(sim) == n_slots
+        fields = initial_data(n_slots, T)
+    else:
+        ops, n_slots = WL.puncture_ops(OUTER)
+        g = am.geometry(S, P, FD_ORDER, OUTER, CFL)
+        sim = am.BssnSim(g, n_slots, am.bssn_options(periodic=False, interp=am.AM_INTERP_TRICUBIC))
+        for p_, c in ops:
+            sim.add_node(p_, c)
+        fields = WL.puncture_fields(sim.export()["positions"], S, P, OUTER)
     for v in range(24):
         sim.set_field(0, v, fields[v])
         sim.set_field(1, v, fields[v])
     sim.fill_ghosts()
     sim.sync()
+    tune = None
+    if args.z_block or args.no_autotune:
+        sim.set_launch(args.z_block, 0, 0)
+    else:
+        tune = sim.autotune(reps=3)
+    if not args.no_graph:
+        sim.use_graph(True)
 
     for _ in range(args.warmup):
         sim.timestep()
@@ -204,13 +235,16 @@ def main():
     elapsed = t1 - t0
     value = zones / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
+    classes = WORKLOAD_CLASSES if args.workload == "bssn512" else WORKLOAD_CLASSES + ("ghost",)
     bytes_per_zone = sum(BYTES[k] * STAGES_PER_STEP[k] for k in WORKLOAD_CLASSES)
     whole_gbs = zones * bytes_per_zone / elapsed / 1e9
 
     roofline = None
     kernels = {}
     if not args.no_profile:
-        # per-launch HIP events on the driver's stream over one more timed step
+        # per-launch HIP events on the driver's stream over one more step,
+        # launched eagerly (a replayed graph has no per-kernel events)
+        sim.use_graph(False)
         sim.profile(True)
         zp = sim.zone_cycles
         sim.timestep()
@@ -221,11 +255,12 @@ def main():
         for k, (ms, cnt) in times.items():
             if cnt == 0:
                 continue
-            total_bytes = zp * BYTES[k] * STAGES_PER_STEP[k]
-            kernels[k] = {"ms_total": round(ms, 4), "launches": cnt, "avg_us": round(1e3 * ms / cnt, 2),
-                          "GBps": round(total_bytes / (ms * 1e-3) / 1e9, 1),
-                          "bytes_per_zone_cycle": round(BYTES[k] * STAGES_PER_STEP[k], 1)}
-        dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
+            kernels[k] = {"ms_total": round(ms, 4), "launches": cnt, "avg_us": round(1e3 * ms / cnt, 2)}
+            if k in WORKLOAD_CLASSES:
+                total_bytes = zp * BYTES[k] * STAGES_PER_STEP[k]
+                kernels[k].update(GBps=round(total_bytes / (ms * 1e-3) / 1e9, 1),
+                                  bytes_per_zone_cycle=round(BYTES[k] * STAGES_PER_STEP[k], 1))
+        dom = max((k for k in kernels if "GBps" in kernels[k]), key=lambda k: kernels[k]["ms_total"])
         d = kernels[dom]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(d["GBps"] / HBM_PEAK_GBS, 4), "traffic": None}
@@ -235,11 +270,12 @@ def main():
         "metric": METRIC, "value": round(value, 1), "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "BSSN single level: 512 tiles of 16^3 (128^3) + 73 ancestors, P=3, 24 fp32 fields, "
-                               "1+log + Gamma-driver, flat + 1e-6 noise, cf3 derivative words (Christoffel symbols raised in registers), RK4, "
-                               "FD order 4, periodic, [-32,32]^3, CFL 0.25",
+        "config": {"workload": WORKLOADS[args.workload],
                    "tiles": n_slots, "zone_cycles_per_step": zones // max(args.steps, 1),
-                   "fine_rk4_steps_per_step": 2 ** DEPTH},
+                   "fine_rk4_steps_per_step": 2 ** sim.counter()[1],
+                   "launch": {"z_block": sim.get_launch()[0], "fd_grid_limit": sim.get_launch()[1],
+                              "rhs_grid_limit": sim.get_launch()[2], "autotuned": tune is not None,
+                              "graph_per_coarse_step": not args.no_graph}},
         "hbm_whole_step": {"achieved": round(whole_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(whole_gbs / HBM_PEAK_GBS, 4), "bytes_per_zone_cycle": round(bytes_per_zone)},
         "roofline": roofline,
@@ -247,7 +283,7 @@ def main():
         "constraints_after": {k: float(f"{v:.6g}") for k, v in cons.items()},
         "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == "bssn512":
         e = sim.export()
         result["cpu_baseline"] = cpu_baseline(n_slots, fields, e["positions"], list(range(n_slots)))
         result["cpu_baseline"]["value"] = round(result["cpu_baseline"]["value"], 1)

@@ -101,7 +101,7 @@ struct rk4_tune_report {
     struct {
         int32_t kind;  // 0 FD z_block, 1 FD grid limit, 2 RHS grid limit
         int32_t value;
-        float ms;
+        float ms;      // best of the repetitions, one whole coarse step
     } trial[32];
 };
 
@@ -384,84 +384,87 @@ struct rk4_AMR_data {
         if (!on) drop_graph();
     }
 
-    // Time the launch shapes on the live mesh (the level with the most tiles,
-    // every tile active) and keep the fastest: FD z-block, FD and RHS
-    // workgroup caps.  new_, acc and the host seed are restored, so the run
-    // continues as if nothing happened.
+    // Time the launch shapes on the live run and keep the fastest: whole
+    // coarse steps (timestep(), eager) from a snapshot of the state, one
+    // parameter at a time -- FD planes per workgroup, then the FD and RHS
+    // workgroup caps.  Each trial starts from the same snapshot; the
+    // snapshot, the host seed, the counter and the zone-cycle count are
+    // restored at the end, so the run continues as if nothing happened.
     void autotune(int reps, rk4_tune_report *rep)
     {
         drop_graph();
-        int dbest = 0;
-        for (int d = 0; d <= parent.read_max_depth(); d++)
-            if (parent.tiles_at_depth(d) > parent.tiles_at_depth(dbest)) dbest = d;
-        am_mesh m = parent.mesh_depth(dbest);
-        m.substep = 0;
-        const size_t fbytes = (size_t)NV * slots * cells * sizeof(float), abytes = (size_t)NV * slots * inner * 4;
-        smart_gpu_buffer<float> save_new((size_t)NV * slots * cells), save_acc((size_t)NV * slots * inner);
-        check(am_memcpy_d2d(save_new.gpu_buffer_ptr, new_.gpu_buffer_ptr, fbytes, stream), "autotune save");
-        check(am_memcpy_d2d(save_acc.gpu_buffer_ptr, acc.gpu_buffer_ptr, abytes, stream), "autotune save");
+        const bool graph_on = use_graph;
+        use_graph = false;
+        const size_t fn = (size_t)NV * slots * cells, an = (size_t)NV * slots * inner;
+        smart_gpu_buffer<float> s_old(fn), s_new(fn), s_acc(an), s_older;
+        if (older.created) s_older = smart_gpu_buffer<float>(fn);
+        auto copy_all = [&](bool save) {
+            auto cp = [&](smart_gpu_buffer<float> &live, smart_gpu_buffer<float> &snap, size_t n) {
+                check(save ? am_memcpy_d2d(snap.gpu_buffer_ptr, live.gpu_buffer_ptr, n * 4, stream)
+                           : am_memcpy_d2d(live.gpu_buffer_ptr, snap.gpu_buffer_ptr, n * 4, stream),
+                      "autotune snapshot");
+            };
+            cp(old_, s_old, fn);
+            cp(new_, s_new, fn);
+            cp(acc, s_acc, an);
+            if (older.created) cp(older, s_older, fn);
+        };
+        copy_all(true);
         const int32_t seed = rng.seed;
-        const rk4_options keep = opt;
+        const uint32_t counter = parent.timer_helper;
+        const uint64_t zc = zone_cycles;
+        const std::vector<uint32_t> life(parent.lifetime.cpu_buffer_ptr, parent.lifetime.cpu_buffer_ptr + slots);
         const bool timing = timer.on;
         timer.on = false;
         void *e0 = nullptr, *e1 = nullptr;
         check(am_event_create(&e0), "event");
         check(am_event_create(&e1), "event");
         rk4_tune_report r{};
-        auto time_it = [&](auto &&fn) {
-            fn();  // warm-up
-            check(am_event_record(e0, stream), "event");
-            for (int i = 0; i < reps; i++) fn();
-            check(am_event_record(e1, stream), "event");
-            float ms = 0.f;
-            check(am_event_elapsed_ms(e0, e1, &ms), "event");
-            return ms / (float)reps;
+        auto trial = [&]() {
+            float best_ms = 1e30f;
+            for (int i = 0; i < reps; i++) {
+                copy_all(false);
+                rng.seed = seed;
+                parent.timer_helper = counter;
+                check(am_event_record(e0, stream), "event");
+                timestep();
+                check(am_event_record(e1, stream), "event");
+                float ms = 0.f;
+                check(am_event_elapsed_ms(e0, e1, &ms), "event");
+                best_ms = std::min(best_ms, ms);
+            }
+            return best_ms;
         };
-        auto note = [&](int kind, int value, float ms) {
-            if (r.n_trials < 32) r.trial[r.n_trials++] = {kind, value, ms};
+        auto sweep = [&](int kind, int32_t &knob, std::initializer_list<int> values) {
+            float best = 1e30f;
+            int keep = knob;
+            for (int v : values) {
+                knob = v;
+                const float ms = trial();
+                if (r.n_trials < 32) r.trial[r.n_trials++] = {kind, v, ms};
+                if (ms < best) { best = ms; keep = v; }
+            }
+            knob = keep;
         };
         const int S = geom.size_domain, cu = std::max(am_device_cu_count(), 1);
-        // FD: planes per workgroup, then the workgroup cap
-        float best = 1e30f;
-        int zb_best = keep.z_block;
-        for (int zb : {S, S / 2, S / 4}) {
-            opt.z_block = zb;
-            const float ms = time_it([&] { derivatives(new_, m); });
-            note(0, zb, ms);
-            if (ms < best) { best = ms; zb_best = zb; }
-        }
-        opt.z_block = zb_best;
-        best = 1e30f;
-        int fl_best = 0;
-        for (int lim : {0, 4 * cu, 8 * cu, 16 * cu}) {
-            opt.fd_grid_limit = lim;
-            const float ms = time_it([&] { derivatives(new_, m); });
-            note(1, lim, ms);
-            if (ms < best) { best = ms; fl_best = lim; }
-        }
-        opt.fd_grid_limit = fl_best;
-        // RHS: stage 1 (reads y_n, acc; writes new, acc, pushed ghosts)
-        derivatives(new_, m);
-        best = 1e30f;
-        int rl_best = 0;
-        for (int lim : {0, 2 * cu, 4 * cu, 8 * cu}) {
-            am_mesh mr = m;
-            mr.grid_limit = lim;
-            const rk4_views f = views(false);
-            const float ms = time_it([&] { check(System::rhs_stage(f, 1, sys, mr, geom, 1, stream), "rhs"); });
-            note(2, lim, ms);
-            if (ms < best) { best = ms; rl_best = lim; }
-        }
-        opt.rhs_grid_limit = rl_best;
-        check(am_memcpy_d2d(new_.gpu_buffer_ptr, save_new.gpu_buffer_ptr, fbytes, stream), "autotune restore");
-        check(am_memcpy_d2d(acc.gpu_buffer_ptr, save_acc.gpu_buffer_ptr, abytes, stream), "autotune restore");
+        opt.z_block = S;
+        opt.fd_grid_limit = 0;
+        opt.rhs_grid_limit = 0;
+        trial();  // warm-up
+        sweep(0, opt.z_block, {S, S / 2, S / 4});
+        sweep(1, opt.fd_grid_limit, {0, 8 * cu});
+        sweep(2, opt.rhs_grid_limit, {0, 2 * cu, 4 * cu});
+        copy_all(false);
         check(am_stream_sync(stream), "autotune");
-        save_new.destroy();
-        save_acc.destroy();
+        s_old.destroy(); s_new.destroy(); s_acc.destroy(); s_older.destroy();
         am_event_destroy(e0);
         am_event_destroy(e1);
         rng.seed = seed;
+        parent.timer_helper = counter;
+        std::copy(life.begin(), life.end(), parent.lifetime.cpu_buffer_ptr);
+        zone_cycles = zc;
         timer.on = timing;
+        use_graph = graph_on;
         r.z_block = opt.z_block;
         r.fd_grid_limit = opt.fd_grid_limit;
         r.rhs_grid_limit = opt.rhs_grid_limit;

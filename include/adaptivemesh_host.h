@@ -62,14 +62,16 @@ int32_t am_wave_sim_rng_state(const am_wave_sim *s);
  *   _use_graph   timestep() records the coarse step once as a hipGraph and
  *                replays it (results identical to eager launches: the dither
  *                seeds are refilled on the device before every replay);
- *   _autotune    times the launch shapes on the live mesh, keeps the
- *                fastest and restores the state it touched. */
+ *   _autotune    times whole coarse steps under each launch shape (one
+ *                knob at a time, every trial from the same snapshot), keeps
+ *                the fastest and restores the state, counter, seed and
+ *                lifetimes, so the run continues unchanged. */
 typedef struct am_tune_report {
     int32_t z_block, fd_grid_limit, rhs_grid_limit, n_trials;
     struct {
         int32_t kind; /* 0 FD z_block, 1 FD workgroup cap, 2 RHS workgroup cap */
         int32_t value;
-        float ms; /* mean per launch */
+        float ms; /* best of the repetitions, one coarse step */
     } trial[32];
 } am_tune_report;
 
