@@ -97,6 +97,10 @@ class TuneReport(C.Structure):
                 ("n_trials", C.c_int32), ("trial", _Trial * 32)]
 
 
+class MlpWork(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("z1", "h1", "z2", "h2", "y", "dy", "dz2", "dz1")]
+
+
 class SwaveFields(C.Structure):
     _fields_ = [("state", C.c_void_p * 2), ("old_", C.c_void_p * 2), ("new_", C.c_void_p * 2),
                 ("acc", C.c_void_p * 2), ("d1_u", C.c_void_p), ("d2diag_u", C.c_void_p)]
@@ -132,6 +136,17 @@ _SIGS = {
     "am_graph_launch": (C.c_int, [_P, _P]),
     "am_graph_destroy": (C.c_int, [_P]),
     "am_seed_table_fill": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
+    "am_mlp_param_count": (C.c_int64, []),
+    "am_gemm_f32": (C.c_int, [_P, C.c_int64, C.c_int64, _P, C.c_int64, C.c_int64, _P, C.c_int64, C.c_int32,
+                              C.c_int32, C.c_int32, _P, C.c_int32, _P, _P, _P]),
+    "am_mlp_forward": (C.c_int, [_P, _P, C.c_int32, _P, _P]),
+    "am_mlp_backward": (C.c_int, [_P, _P, C.c_int32, _P, _P, _P]),
+    "am_mlp_mse": (C.c_int, [_P, _P, C.c_int32, _P, _P, _P]),
+    "am_mlp_mse_blocks": (C.c_int64, [C.c_int32]),
+    "am_mlp_adam": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
+                              _P]),
+    "am_mlp_pairs_per_tile": (C.c_int64, [C.POINTER(Geometry)]),
+    "am_mlp_pairs": (C.c_int, [C.POINTER(C.c_void_p), _P, C.c_int32, C.POINTER(Geometry), _P, _P, _P, _P, _P]),
     "am_stream_sync": (C.c_int, [_P]),
     "am_device_sync": (C.c_int, []),
     "am_fd_d1": (C.c_int, [_P, _P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_int32, _P]),

@@ -329,6 +329,52 @@ int am_swave_rhs_stage(const am_swave_fields *f, int32_t stage, const am_mesh *m
 int am_swave_energy(const am_swave_fields *f, const am_mesh *mesh, const am_geometry *g, double *partial,
                     void *stream);
 
+/* ------------------------------------------------- learned prolongation
+ * No counterpart in the reference.  A per-octant MLP predicts the 2^3 fine
+ * children of a coarse cell (24 fields) from the cell's 4^3 coarse stencil
+ * (offsets -1..2 per axis), as a correction on top of the tricubic
+ * baseline; each field's stencil is shifted by its centre value mu and
+ * scaled by sigma = max |stencil - mu| + 1e-7 (1 + |mu|), the output is the
+ * residual divided by sigma.  Layers 1536 -> 128 -> 128 -> 192 with exact
+ * GELU; parameters in one fp32 array: W1[1536][128] b1[128] W2[128][128]
+ * b2[128] W3[128][192] b3[192] (weights [in][out], row-major). */
+#define AM_MLP_IN 1536
+#define AM_MLP_HID 128
+#define AM_MLP_OUT 192
+int64_t am_mlp_param_count(void);
+
+/* C = op(A) op(B) on fp32 MFMA: A(m,k) = A[m*sam + k*sak], B(k,n) =
+ * B[k*sbk + n*sbn], C row-major (ldc).  epilogue 0: C = AB (+ bias[n]);
+ * 1: C = AB + bias, out2 = gelu(C); 2: C = AB * gelu'(aux) (aux like C).
+ * M and N multiples of 64, K of 16. */
+int am_gemm_f32(const float *A, int64_t sam, int64_t sak, const float *B, int64_t sbk, int64_t sbn, float *C,
+                int64_t ldc, int32_t M, int32_t N, int32_t K, const float *bias, int32_t epilogue, const float *aux,
+                float *out2, void *stream);
+
+/* Activations and gradients of one batch (batch rows each): z1 h1 z2 h2
+ * dz1 dz2 are batch x 128, y dy batch x 192. */
+typedef struct am_mlp_work {
+    float *z1, *h1, *z2, *h2, *y, *dy, *dz2, *dz1;
+} am_mlp_work;
+int am_mlp_forward(const float *params, const float *x, int32_t batch, am_mlp_work *w, void *stream);
+/* gradients of the loss whose output gradient is w->dy (all of grads written) */
+int am_mlp_backward(const float *params, const float *x, int32_t batch, am_mlp_work *w, float *grads, void *stream);
+/* dy = 2 (y - t) / (batch * 192); partial[i] = FP64 sum of (y - t)^2 over
+ * block i of am_mlp_mse_blocks(batch) */
+int am_mlp_mse(const float *y, const float *t, int32_t batch, float *dy, double *partial, void *stream);
+int64_t am_mlp_mse_blocks(int32_t batch);
+/* Adam with bias correction; step counts from 1 */
+int am_mlp_adam(float *params, const float *grads, float *m, float *v, int64_t n, int32_t step, float lr,
+                float beta1, float beta2, float eps, void *stream);
+/* Training pairs from fine padded tiles (24 field pointers): for each
+ * listed tile, every coarse cell (restriction = mean of 8, AMR.h:975-995)
+ * whose stencil lies in the padded tile, am_mlp_pairs_per_tile() of them
+ * per tile, tile-major then z, y, x: x (n x 1536) normalised stencils,
+ * t (n x 192) normalised residuals, mu and sigma (n x 24). */
+int64_t am_mlp_pairs_per_tile(const am_geometry *g);
+int am_mlp_pairs(const float *const *fine, const int32_t *tiles, int32_t n_tiles, const am_geometry *g, float *x,
+                 float *t, float *mu, float *sigma, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,167 @@
+"""FP64 restatement of the learned prolongation (the checker).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker.  The product
+never imports it.
+
+The network and its training pairs have no counterpart in the reference.
+They are written here from their definitions, to check the GPU kernels:
+
+* pairs: fine padded tiles (24 fields) are restricted to coarse cells (mean
+  of the 2x2x2 children, the restriction of AMR.h:975-995); for each coarse
+  cell c whose 4^3 stencil c-1..c+2 lies inside the tile's padded region,
+  the input is that stencil, per field shifted by the centre value mu and
+  scaled by sigma = max |stencil - mu| + 1e-7 (1 + |mu|); the target is the
+  8 fine children minus the tricubic baseline (4-point Lagrange through the
+  stencil at offsets -1/4, +1/4 per axis), divided by sigma;
+* MLP 1536 -> 128 -> 128 -> 192 with exact GELU (z Phi(z)), parameters in
+  one array W1[1536][128] b1 W2[128][128] b2 W3[128][192] b3;
+* MSE loss mean((y - t)^2) over batch x 192, its gradients by backprop;
+* Adam with bias correction.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+from scipy.special import erf
+
+N_IN, N_HID, N_OUT = 1536, 128, 192
+SHAPES = [(N_IN, N_HID), (N_HID,), (N_HID, N_HID), (N_HID,), (N_HID, N_OUT), (N_OUT,)]
+N_PARAMS = sum(int(np.prod(s)) for s in SHAPES)
+
+
+def unpack(params):
+    out, o = [], 0
+    for s in SHAPES:
+        n = int(np.prod(s))
+        out.append(np.asarray(params[o:o + n], np.float64).reshape(s))
+        o += n
+    return out
+
+
+def pack(parts):
+    return np.concatenate([np.asarray(p, np.float64).ravel() for p in parts])
+
+
+def init_params(seed=0):
+    """He-style uniform init (fan-in), zero biases; W3 scaled down so the
+    untrained network starts near the tricubic baseline."""
+    rng = np.random.default_rng(seed)
+    w1 = rng.uniform(-1, 1, (N_IN, N_HID)) * math.sqrt(6.0 / N_IN)
+    w2 = rng.uniform(-1, 1, (N_HID, N_HID)) * math.sqrt(6.0 / N_HID)
+    w3 = rng.uniform(-1, 1, (N_HID, N_OUT)) * math.sqrt(6.0 / N_HID) * 0.1
+    return pack([w1, np.zeros(N_HID), w2, np.zeros(N_HID), w3, np.zeros(N_OUT)])
+
+
+def gelu(z):
+    return 0.5 * z * (1.0 + erf(z / math.sqrt(2.0)))
+
+
+def gelu_prime(z):
+    return 0.5 * (1.0 + erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2.0 * math.pi)
+
+
+def forward(params, x):
+    w1, b1, w2, b2, w3, b3 = unpack(params)
+    x = np.asarray(x, np.float64)
+    z1 = x @ w1 + b1
+    h1 = gelu(z1)
+    z2 = h1 @ w2 + b2
+    h2 = gelu(z2)
+    y = h2 @ w3 + b3
+    return dict(z1=z1, h1=h1, z2=z2, h2=h2, y=y)
+
+
+def loss(params, x, t):
+    y = forward(params, x)["y"]
+    return float(np.mean((y - np.asarray(t, np.float64)) ** 2))
+
+
+def grads(params, x, t):
+    """Loss and its gradient with respect to the flat parameter array."""
+    w1, b1, w2, b2, w3, b3 = unpack(params)
+    x = np.asarray(x, np.float64)
+    f = forward(params, x)
+    B = x.shape[0]
+    dy = 2.0 * (f["y"] - np.asarray(t, np.float64)) / (B * N_OUT)
+    dw3 = f["h2"].T @ dy
+    db3 = dy.sum(0)
+    dz2 = (dy @ w3.T) * gelu_prime(f["z2"])
+    dw2 = f["h1"].T @ dz2
+    db2 = dz2.sum(0)
+    dz1 = (dz2 @ w2.T) * gelu_prime(f["z1"])
+    dw1 = x.T @ dz1
+    db1 = dz1.sum(0)
+    return float(np.mean((f["y"] - t) ** 2)), pack([dw1, db1, dw2, db2, dw3, db3])
+
+
+def adam(params, g, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step (step counts from 1); returns new params, m, v."""
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    mh = m / (1 - b1 ** step)
+    vh = v / (1 - b2 ** step)
+    return params - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+# ------------------------------------------------------------- pairs
+def lag4(t):
+    """4-point Lagrange weights through nodes -1, 0, 1, 2 at t."""
+    nodes = (-1.0, 0.0, 1.0, 2.0)
+    w = []
+    for i, xi in enumerate(nodes):
+        p = 1.0
+        for m, xm in enumerate(nodes):
+            if m != i:
+                p *= (t - xm) / (xi - xm)
+        w.append(p)
+    return np.array(w)
+
+
+W_CHILD = np.stack([lag4(-0.25), lag4(0.25)])  # [child offset j][stencil node]
+
+
+def coarse_range(S, P):
+    """Coarse cells c (per axis, in the tile's coarse index) whose stencil
+    c-1..c+2 restricts from fine cells inside the padded tile."""
+    return range(max(0, -((P - 2) // 2)), (S + P - 6) // 2 + 1)
+
+
+def tile_pairs(fine, slot, S, P):
+    """(X, T, mu, sigma) for every coarse cell of one tile: fine is a list of
+    24 padded arrays (slots * T^3), X (n, 1536), T (n, 192), mu/sigma (n, 24)."""
+    T_ = S + 2 * P
+    cr = list(coarse_range(S, P))
+    xs, ts, mus, sgs = [], [], [], []
+    f = [np.asarray(a, np.float64).reshape(-1, T_, T_, T_)[slot] for a in fine]
+    for cz in cr:
+        for cy in cr:
+            for cx in cr:
+                X = np.zeros((24, 4, 4, 4))  # [v][z][y][x]
+                Y = np.zeros((24, 2, 2, 2))
+                for v in range(24):
+                    for k in range(4):
+                        for j in range(4):
+                            for i in range(4):
+                                z0 = 2 * (cz - 1 + k) + P
+                                y0 = 2 * (cy - 1 + j) + P
+                                x0 = 2 * (cx - 1 + i) + P
+                                X[v, k, j, i] = f[v][z0:z0 + 2, y0:y0 + 2, x0:x0 + 2].mean()
+                    z0, y0, x0 = 2 * cz + P, 2 * cy + P, 2 * cx + P
+                    Y[v] = f[v][z0:z0 + 2, y0:y0 + 2, x0:x0 + 2]
+                mu = X[:, 1, 1, 1]
+                sg = np.abs(X - mu[:, None, None, None]).reshape(24, -1).max(1) + 1e-7 * (1 + np.abs(mu))
+                base = np.einsum("ai,bj,ck,vkji->vcba", W_CHILD, W_CHILD, W_CHILD, X)  # [v][jz][jy][jx]
+                xs.append(((X - mu[:, None, None, None]) / sg[:, None, None, None]).reshape(-1))
+                ts.append(((Y - base) / sg[:, None, None, None]).reshape(-1))
+                mus.append(mu)
+                sgs.append(sg)
+    return np.array(xs), np.array(ts), np.array(mus), np.array(sgs)
+
+
+def children_from(X, Yn, mu, sigma):
+    """Fine children (n, 24, 8) from normalized stencils and outputs."""
+    n = X.shape[0]
+    Xr = X.reshape(n, 24, 4, 4, 4) * sigma[:, :, None, None, None] + mu[:, :, None, None, None]
+    base = np.einsum("ai,bj,ck,nvkji->nvcba", W_CHILD, W_CHILD, W_CHILD, Xr).reshape(n, 24, 8)
+    return base + Yn.reshape(n, 24, 8) * sigma[:, :, None]

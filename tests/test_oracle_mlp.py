@@ -1,0 +1,88 @@
+"""Pins the FP64 oracle of the learned prolongation.
+
+* Backprop against FP64 central differences of the loss, on parameters of
+  every block (a dropped term or a transposed product fails).
+* Adam's first step from zero moments has the closed form
+  p - lr g / (|g| + eps); every later step moves each parameter by at most
+  lr (1 - beta1) / sqrt(1 - beta2) (Kingma & Ba 2015, section 2.1, the
+  case 1 - beta1 > sqrt(1 - beta2)).
+* Pairs: the 8 fine children of a coarse cell average to its restricted
+  value; linear fields give zero targets (restriction of a linear field is
+  its centre value and the 4-point Lagrange baseline is exact on it);
+  denormalising the targets gives back the fine cells.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+from tests import fields as F
+from tests import meshes
+
+S, P = 16, 3
+T = S + 2 * P
+
+
+def test_param_count():
+    assert M.N_PARAMS == 238016
+
+
+def test_backprop_matches_central_differences():
+    rng = np.random.default_rng(0)
+    p = M.init_params(1)
+    p[M.N_IN * M.N_HID:M.N_IN * M.N_HID + M.N_HID] = rng.normal(0, 0.1, M.N_HID)  # non-zero b1
+    x = rng.normal(0, 1, (6, M.N_IN))
+    t = rng.normal(0, 0.1, (6, M.N_OUT))
+    _, g = M.grads(p, x, t)
+    offsets = np.cumsum([0] + [int(np.prod(s)) for s in M.SHAPES])
+    idx = np.concatenate([rng.integers(offsets[i], offsets[i + 1], 12) for i in range(6)])
+    h = 1e-6
+    for i in idx:
+        q = p.copy()
+        q[i] += h
+        lp = M.loss(q, x, t)
+        q[i] -= 2 * h
+        lm = M.loss(q, x, t)
+        num = (lp - lm) / (2 * h)
+        assert abs(num - g[i]) <= 1e-6 * max(abs(num), 1e-4), (i, num, g[i])
+
+
+def test_adam_closed_form_and_bound():
+    rng = np.random.default_rng(2)
+    p0 = rng.normal(0, 1, 1000)
+    g = rng.normal(0, 1, 1000)
+    lr, eps = 1e-2, 1e-8
+    p1, m, v = M.adam(p0, g, np.zeros(1000), np.zeros(1000), 1, lr=lr, eps=eps)
+    assert np.allclose(p1, p0 - lr * g / (np.abs(g) + eps), rtol=0, atol=1e-15)
+    p = p1
+    for step in range(2, 50):
+        q, m, v = M.adam(p, rng.normal(0, 1, 1000), m, v, step, lr=lr, eps=eps)
+        assert np.max(np.abs(q - p)) <= lr * (1 - 0.9) / np.sqrt(1 - 0.999) + 1e-12
+        p = q
+
+
+def _tile(recipe_fn):
+    ops, n = meshes.uniform(1)
+    from oracle import oracle as orc
+    t = orc.OracleTree(n, True)
+    for a, c in ops:
+        t.add_node(a, c)
+    e = t.export()
+    return [F.sample(e["positions"], n, S, P, 8.0, recipe_fn(v), dtype=np.float64) for v in range(24)], e
+
+
+def test_pairs_restriction_and_linear_exactness():
+    fine, e = _tile(lambda v: (lambda x, y, z, v=v: 0.3 + 0.01 * v + 0.2 * x - 0.1 * y + 0.05 * (v % 3) * z))
+    X, Tn, mu, sg = M.tile_pairs(fine, 3, S, P)
+    assert X.shape == (343, 1536) and Tn.shape == (343, 192)
+    assert np.max(np.abs(Tn * sg.repeat(8, axis=1))) < 1e-12
+    fine, e = _tile(lambda v: (lambda x, y, z, v=v: np.sin(0.7 * x + 0.1 * v) * np.cos(0.4 * y) + 0.02 * z * z))
+    X, Tn, mu, sg = M.tile_pairs(fine, 5, S, P)
+    ch = M.children_from(X, Tn, mu, sg)
+    assert np.allclose(ch.mean(axis=2), mu, atol=1e-13)
+    # denormalised targets are the fine cells themselves
+    f5 = [a.reshape(-1, T, T, T)[5] for a in fine]
+    cz, cy, cx = 2, 4, 6
+    k = (cz * 7 + cy) * 7 + cx
+    for v in (0, 7, 23):
+        want = f5[v][2 * cz + P:2 * cz + P + 2, 2 * cy + P:2 * cy + P + 2, 2 * cx + P:2 * cx + P + 2].ravel()
+        assert np.allclose(ch[k, v], want, atol=1e-13)
