@@ -19,7 +19,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libadaptivemesh_amd.so"
 
 AM_MAX_VARS = 32
-AM_INTERP_LAGRANGE, AM_INTERP_TRILINEAR, AM_INTERP_TRICUBIC = 0, 1, 2
+AM_INTERP_LAGRANGE, AM_INTERP_TRILINEAR, AM_INTERP_TRICUBIC, AM_INTERP_LEARNED = 0, 1, 2, 3
 AM_EQUAL_FORMULA, AM_EQUAL_COPY, AM_EQUAL_SKIP = 0, 1, 2
 
 ERRORS = {
@@ -146,6 +146,8 @@ _SIGS = {
     "am_mlp_adam": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
                               _P]),
     "am_mlp_pairs_per_tile": (C.c_int64, [C.POINTER(Geometry)]),
+    "am_ghost_fill_learned": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), C.c_float, _P,
+                                        C.c_float, _P, C.c_int32, _P]),
     "am_mlp_pairs": (C.c_int, [C.POINTER(C.c_void_p), _P, C.c_int32, C.POINTER(Geometry), _P, _P, _P, _P, _P]),
     "am_stream_sync": (C.c_int, [_P]),
     "am_device_sync": (C.c_int, []),

@@ -1,0 +1,393 @@
+// Learned coarse-to-fine ghost fill for gfx950: the per-octant MLP of
+// am_mlp.hip evaluated inside the ghost fill.  No counterpart in the
+// reference.
+//
+// One workgroup (4 waves) per (fine tile, face) whose target is one level
+// coarser and not the root.  The face's ghost slab covers a box of coarse
+// cells of the target (2 x 8 x 8 for S = 16, P = 3); each coarse cell is one
+// sample, processed 64 at a time:
+//   1. per field: gather the cell's 4^3 stencil (offsets -1..2) of the
+//      target at the stage time (quadratic in time through older/old/new,
+//      as the tricubic fill), normalise by the centre value mu and the
+//      spread sigma, add the tricubic baseline of the 8 children, and
+//      accumulate layer 1 (64 samples x 64 inputs x 128) on MFMA;
+//   2. layers 2 and 3 on MFMA from LDS; output + baseline = normalised
+//      children;
+//   3. the restriction residual |mean(children) - mu| (field units) is the
+//      face's self-check; the children are then shifted so their mean is mu;
+//   4. every ghost cell of the slab takes its child's value.
+// A stencil that would read an edge or corner ghost of the target (never
+// filled) makes its sample take the tricubic value; a face whose largest
+// residual exceeds the threshold is refilled entirely by the tricubic fill,
+// with the same arithmetic (am_interp.hpp) as the plain fill.
+#include "am_common.hpp"
+#include "am_device.hpp"
+#include "am_interp.hpp"
+
+namespace am {
+
+namespace {
+constexpr int L_IN = 1536, L_HID = 128, L_OUT = 192;
+constexpr int64_t L_W1 = 0, L_B1 = (int64_t)L_IN * L_HID, L_W2 = L_B1 + L_HID, L_B2 = L_W2 + (int64_t)L_HID * L_HID,
+                  L_W3 = L_B2 + L_HID, L_B3 = L_W3 + (int64_t)L_HID * L_OUT;
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float gelu_l(float z) { return 0.5f * z * (1.f + erff(z * 0.70710678118654752f)); }
+
+// 4-point Lagrange weights through -1, 0, 1, 2 at the children's offsets -1/4, +1/4
+__device__ constexpr float LQ[2][4] = {{15.f / 128, 135.f / 128, -27.f / 128, 5.f / 128},
+                                       {-7.f / 128, 105.f / 128, 35.f / 128, -5.f / 128}};
+}  // namespace
+
+template <int S, int P>
+__device__ __forceinline__ void ghost_cell_of(int dir, int c, uint32_t &w0, uint32_t &w1, uint32_t &w2)
+{
+    constexpr uint32_t uP = P, uS = S;
+    if (dir == 0 || dir == 3) {
+        const uint32_t l = c % P, a = (c / P) % S, bq = c / (P * S);
+        w0 = dir == 0 ? l + uP + uS : l; w1 = a + uP; w2 = bq + uP;
+    } else if (dir == 1 || dir == 4) {
+        const uint32_t a = c % S, l = (c / S) % P, bq = c / (S * P);
+        w0 = a + uP; w1 = dir == 1 ? l + uP + uS : l; w2 = bq + uP;
+    } else {
+        const uint32_t a = c % S, bq = (c / S) % S, l = c / (S * S);
+        w0 = a + uP; w1 = bq + uP; w2 = dir == 2 ? l + uP + uS : l;
+    }
+}
+
+// the target coordinate of fine index w along one axis (AMR.h:736-745)
+template <int S, int P>
+__device__ __forceinline__ float read_pos(uint32_t w, float rel, float rel_scale)
+{
+    float r = ((float)w - (float)P + .5f) * (1.f / (float)S) - .5f;
+    r = r * rel_scale + rel;
+    return (r + .5f) * (float)S + (float)P - 0.5f;
+}
+
+template <int S, int P>
+__global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const uint64_t *__restrict__ bnd,
+                                                       const int32_t *__restrict__ tiles, uint32_t substep,
+                                                       uint32_t maxd, float frac, const float *__restrict__ prm,
+                                                       float threshold, unsigned long long *__restrict__ stats)
+{
+    constexpr int T = S + 2 * P, NC = S * S * P;
+    constexpr size_t CELLS = (size_t)T * T * T;
+    __shared__ float Xs[64][65];
+    __shared__ float Hs[64][129];
+    __shared__ float Bn[64][193];
+    __shared__ float musg[2][24][64];
+    __shared__ int valid[64];
+    __shared__ int face_resid_bits;
+    __shared__ int n_invalid;
+
+    const int ti = blockIdx.x / 6, dir = blockIdx.x - ti * 6;
+    const int node = tiles[ti];
+    const uint64_t bb = bnd[(size_t)node * 6 + dir];
+    const uint32_t nd = (uint32_t)bb;
+    const int tgt = bnd_target(nd), cd = bnd_depth(nd);
+    const uint32_t ddiff = bnd_ddiff(nd);
+    if (tgt <= 0 || cd <= 0 || ddiff != 1u) return;  // uniform per workgroup
+    if (!active_depth(substep, (uint32_t)cd, maxd)) return;
+    const float3 relp = cf3_unpack((uint32_t)(bb >> 32));
+    const float rl[3] = {relp.x, relp.y, relp.z};
+    const float rel_scale = .5f;
+    const float tmp = (float)((substep >> (maxd - (uint32_t)cd)) & 1u);
+    const float ty = (tmp + frac) * .5f;
+    const bool quad = b.older[0] != nullptr;
+    const float tc0 = quad ? .5f * ty * (ty - 1.f) : 0.f;
+    const float tc1 = quad ? (1.f - ty) * (1.f + ty) : 1.f - ty;
+    const float tc2 = quad ? .5f * ty * (ty + 1.f) : ty;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // coarse cells covered by the ghost slab, per axis
+    int clo[3], ncl[3];
+    {
+        const int n = dir % 3;
+        for (int k = 0; k < 3; k++) {
+            const uint32_t wlo = k != n ? (uint32_t)P : (dir < 3 ? (uint32_t)(P + S) : 0u);
+            const uint32_t whi = k != n ? (uint32_t)(P + S - 1) : (dir < 3 ? (uint32_t)(2 * P + S - 1) : (uint32_t)(P - 1));
+            const int a = (int)floorf(read_pos<S, P>(wlo, rl[k], rel_scale) + .5f);
+            const int z = (int)floorf(read_pos<S, P>(whi, rl[k], rel_scale) + .5f);
+            clo[k] = a;
+            ncl[k] = z - a + 1;
+        }
+    }
+    const int NS = ncl[0] * ncl[1] * ncl[2];
+    if (tid == 0) {
+        face_resid_bits = 0;
+        n_invalid = 0;
+    }
+    const size_t tbase = (size_t)tgt * CELLS;
+
+    for (int chunk = 0; chunk < NS; chunk += 64) {
+        const int ns = min(64, NS - chunk);
+        if (tid < 64) valid[tid] = 1;
+        __syncthreads();
+        floatx4 acc[4][2];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        const int s = tid >> 2, q = tid & 3;  // sample and stencil quarter (z = q) of this thread
+        int cc[3] = {0, 0, 0};
+        if (s < ns) {
+            const int sa = chunk + s;
+            cc[0] = clo[0] + sa % ncl[0];
+            cc[1] = clo[1] + (sa / ncl[0]) % ncl[1];
+            cc[2] = clo[2] + sa / (ncl[0] * ncl[1]);
+        }
+        for (int v = 0; v < 24; v++) {
+            const float *dold = b.old_[v], *dnew = b.new_[v], *dolder = b.older[v];
+            // 1. gather this thread's 16 stencil points (z = q) at the stage time
+            bool bad = false;
+            if (s < ns) {
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const int px = cc[0] - 1 + (i & 3), py = cc[1] - 1 + (i >> 2), pz = cc[2] - 1 + q;
+                    const int out = (px < P || px >= P + S) + (py < P || py >= P + S) + (pz < P || pz >= P + S);
+                    const bool inside = px >= 0 && px < T && py >= 0 && py < T && pz >= 0 && pz < T && out <= 1;
+                    float val = 0.f;
+                    if (inside) {
+                        const size_t r = tbase + ((size_t)pz * T + py) * T + px;
+                        val = dold[r] * tc1 + dnew[r] * tc2;
+                        if (quad) val = dolder[r] * tc0 + val;
+                    } else {
+                        bad = true;
+                    }
+                    Xs[s][q * 16 + i] = val;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 16; i++) Xs[s][q * 16 + i] = 0.f;
+            }
+            if (bad) valid[s] = 0;
+            __syncthreads();
+            // normalise: centre (1,1,1) is stencil index 21, spread over the 4 quarters
+            if (s < ns) {
+                const float mu = Xs[s][21];
+                float dev = 0.f;
+#pragma unroll
+                for (int i = 0; i < 16; i++) dev = fmaxf(dev, fabsf(Xs[s][q * 16 + i] - mu));
+                dev = fmaxf(dev, __shfl_xor(dev, 1, 64));
+                dev = fmaxf(dev, __shfl_xor(dev, 2, 64));
+                const float sg = dev + 1e-3f * fabsf(mu) + 1e-7f;
+                const float inv = 1.f / sg;
+                float xn[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) xn[i] = (Xs[s][q * 16 + i] - mu) * inv;
+                // all lanes of the sample have read mu before anyone overwrites index 21
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int i = 0; i < 16; i++) Xs[s][q * 16 + i] = xn[i];
+                if (q == 0) {
+                    musg[0][v][s] = mu;
+                    musg[1][v][s] = sg;
+                }
+            }
+            __syncthreads();
+            // baseline of the 8 children (normalised units): 2 children per thread
+            if (s < ns) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int ch = q * 2 + h;
+                    const int jx = ch & 1, jy = (ch >> 1) & 1, jz = ch >> 2;
+                    float bsum = 0.f;
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+#pragma unroll
+                            for (int i = 0; i < 4; i++)
+                                bsum += LQ[jz][k] * LQ[jy][j] * LQ[jx][i] * Xs[s][k * 16 + j * 4 + i];
+                    Bn[s][v * 8 + ch] = bsum;
+                }
+            }
+            // layer 1 contribution of this field: Z1 += Xs (64 x 64) W1[v*64.., :]
+#pragma unroll 4
+            for (int kk = 0; kk < 16; kk++) {
+                const int ka = kk * 4 + (lane >> 4);
+                const float *wrow = prm + L_W1 + (int64_t)(v * 64 + ka) * L_HID + wave * 32 + (lane & 15);
+                const float b0 = wrow[0], b1 = wrow[16];
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++) {
+                    const float a = Xs[rb * 16 + (lane & 15)][ka];
+                    acc[rb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[rb][0], 0, 0, 0);
+                    acc[rb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[rb][1], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+        }
+        // layer 1 epilogue
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = rb * 16 + 4 * (lane >> 4) + r, col = wave * 32 + cb * 16 + (lane & 15);
+                    Hs[row][col] = gelu_l(acc[rb][cb][r] + prm[L_B1 + col]);
+                }
+        __syncthreads();
+        // layer 2
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+        for (int kk = 0; kk < 32; kk++) {
+            const int ka = kk * 4 + (lane >> 4);
+            const float *wrow = prm + L_W2 + (int64_t)ka * L_HID + wave * 32 + (lane & 15);
+            const float b0 = wrow[0], b1 = wrow[16];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                const float a = Hs[rb * 16 + (lane & 15)][ka];
+                acc[rb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[rb][0], 0, 0, 0);
+                acc[rb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[rb][1], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = rb * 16 + 4 * (lane >> 4) + r, col = wave * 32 + cb * 16 + (lane & 15);
+                    Hs[row][col] = gelu_l(acc[rb][cb][r] + prm[L_B2 + col]);
+                }
+        __syncthreads();
+        // layer 3: 48 output columns per wave, added onto the baseline
+        {
+            floatx4 acc3[4][3];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) acc3[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+            for (int kk = 0; kk < 32; kk++) {
+                const int ka = kk * 4 + (lane >> 4);
+                const float *wrow = prm + L_W3 + (int64_t)ka * L_OUT + wave * 48 + (lane & 15);
+                const float b0 = wrow[0], b1 = wrow[16], b2 = wrow[32];
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++) {
+                    const float a = Hs[rb * 16 + (lane & 15)][ka];
+                    acc3[rb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc3[rb][0], 0, 0, 0);
+                    acc3[rb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc3[rb][1], 0, 0, 0);
+                    acc3[rb][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b2, acc3[rb][2], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                for (int cb = 0; cb < 3; cb++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int row = rb * 16 + 4 * (lane >> 4) + r, col = wave * 48 + cb * 16 + (lane & 15);
+                        Bn[row][col] += acc3[rb][cb][r] + prm[L_B3 + col];
+                    }
+        }
+        __syncthreads();
+        // residual of the restriction (field units) and the shift that removes it
+        if (s < ns) {
+            float rmax = 0.f;
+            for (int v = q * 6; v < q * 6 + 6; v++) {
+                float m = 0.f;
+#pragma unroll
+                for (int ch = 0; ch < 8; ch++) m += Bn[s][v * 8 + ch];
+                m *= .125f;
+                rmax = fmaxf(rmax, fabsf(m * musg[1][v][s]));
+#pragma unroll
+                for (int ch = 0; ch < 8; ch++) Bn[s][v * 8 + ch] -= m;
+            }
+            if (valid[s]) atomicMax(&face_resid_bits, __float_as_int(rmax));  // non-negative: int order = float order
+            if (q == 0 && !valid[s]) atomicAdd(&n_invalid, 1);
+        }
+        __syncthreads();
+        // 4. write the slab cells of this chunk's samples
+        for (int c = tid; c < NC; c += 256) {
+            uint32_t w[3];
+            ghost_cell_of<S, P>(dir, c, w[0], w[1], w[2]);
+            float rp[3];
+            int cl[3], child = 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                rp[k] = read_pos<S, P>(w[k], rl[k], rel_scale);
+                cl[k] = (int)floorf(rp[k] + .5f);
+                child |= (rp[k] > (float)cl[k] ? 1 : 0) << k;
+            }
+            const int sa = (cl[0] - clo[0]) + ncl[0] * ((cl[1] - clo[1]) + ncl[1] * (cl[2] - clo[2]));
+            if (sa < chunk || sa >= chunk + ns) continue;
+            const int sl = sa - chunk;
+            const size_t wn = (size_t)node * CELLS + w[0] + ((size_t)w[1] + (size_t)w[2] * T) * T;
+            if (valid[sl]) {
+                for (int v = 0; v < 24; v++) b.new_[v][wn] = musg[0][v][sl] + musg[1][v][sl] * Bn[sl][v * 8 + child];
+            } else {
+                float wx[4], wy[4], wz[4];
+                const size_t rn = tbase + tricubic_nodes<T, P>(tgt, rp, wx, wy, wz);
+                for (int v = 0; v < 24; v++)
+                    b.new_[v][wn] = tricubic_time<T>(b.old_[v], b.new_[v], b.older[v], ddiff, ty, wx, wy, wz, rn);
+            }
+        }
+        __syncthreads();
+    }
+    // a face whose residual exceeds the threshold takes the tricubic fill
+    const bool fallback = __int_as_float(face_resid_bits) > threshold;
+    if (fallback) {
+        for (int c = tid; c < NC; c += 256) {
+            uint32_t w[3];
+            ghost_cell_of<S, P>(dir, c, w[0], w[1], w[2]);
+            float rp[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) rp[k] = read_pos<S, P>(w[k], rl[k], rel_scale);
+            float wx[4], wy[4], wz[4];
+            const size_t rn = tbase + tricubic_nodes<T, P>(tgt, rp, wx, wy, wz);
+            const size_t wn = (size_t)node * CELLS + w[0] + ((size_t)w[1] + (size_t)w[2] * T) * T;
+            for (int v = 0; v < 24; v++)
+                b.new_[v][wn] = tricubic_time<T>(b.old_[v], b.new_[v], b.older[v], ddiff, ty, wx, wy, wz, rn);
+        }
+    }
+    if (tid == 0 && stats) {
+        atomicAdd(&stats[fallback ? 1 : 0], 1ull);
+        atomicAdd(&stats[2], (unsigned long long)n_invalid);
+        atomicAdd(&stats[3], (unsigned long long)NS);
+    }
+}
+
+template <int S, int P>
+static int launch_learned(const am_field_batch *fb, const am_mesh *mesh, const am_geometry *g, float frac,
+                          const float *params, float threshold, unsigned long long *stats, int equal_mode,
+                          void *stream)
+{
+    if (fb->n_vars != 24) return set_error(AM_ERR_INVALID, "am_ghost_fill_learned: needs the 24 BSSN fields");
+    FieldBatchDev d;
+    for (int v = 0; v < 24; v++) {
+        if (!fb->old_[v] || !fb->new_[v]) return set_error(AM_ERR_INVALID, "am_ghost_fill_learned: null field %d", v);
+        d.old_[v] = fb->old_[v];
+        d.new_[v] = fb->new_[v];
+        d.older[v] = fb->older[v];
+        if ((fb->older[v] == nullptr) != (fb->older[0] == nullptr))
+            return set_error(AM_ERR_INVALID, "am_ghost_fill_learned: older must be given for all fields or none");
+    }
+    // every other face: the tricubic stage fill
+    if (int rc = ghost_stage_fill_skip_learned(fb, mesh, g, frac, equal_mode, stream)) return rc;
+    hipLaunchKernelGGL((k_ghost_learned<S, P>), dim3(mesh->n_tiles * 6), dim3(256), 0, (hipStream_t)stream, d,
+                       mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, params, threshold, stats);
+    return check_launch("k_ghost_learned");
+}
+
+}  // namespace am
+
+extern "C" int am_ghost_fill_learned(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g,
+                                     float stage_frac, const float *params, float threshold, uint64_t *stats,
+                                     int32_t equal_mode, void *stream)
+{
+    if (!b || !mesh || !g || !params) return am::set_error(AM_ERR_INVALID, "am_ghost_fill_learned: null argument");
+    if (!mesh->boundaries || !mesh->tiles) return am::set_error(AM_ERR_INVALID, "am_ghost_fill_learned: no tiles");
+    if (equal_mode < AM_EQUAL_FORMULA || equal_mode > AM_EQUAL_SKIP)
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_learned: equal_mode %d", equal_mode);
+    if (!(stage_frac >= 0.f && stage_frac <= 1.f))
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_learned: stage_frac %g", (double)stage_frac);
+    if (mesh->n_tiles <= 0) return AM_OK;
+    AM_DISPATCH_TILE(*g, am::launch_learned, b, mesh, g, stage_frac, params, threshold,
+                     reinterpret_cast<unsigned long long *>(stats), equal_mode, stream);
+}

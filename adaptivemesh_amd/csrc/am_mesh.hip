@@ -13,86 +13,11 @@
 // results are bit-identical to the reference's on an IEEE host.  The
 // tricubic-space / quadratic-time mode is an extension.
 #include "am_device.hpp"
+#include "am_interp.hpp"
 
 #pragma clang fp contract(off)
 
 namespace am {
-
-struct FieldBatchDev {
-    float *old_[AM_MAX_VARS];
-    float *new_[AM_MAX_VARS];
-    const float *older[AM_MAX_VARS];
-};
-
-// padded-tile buffers prolonged by one k_copy_to_child launch
-struct ChildBufs {
-    float *buf[3 * AM_MAX_VARS];
-};
-
-// __interpolate_dat, quadratic 10-point form (AMR.h:673-699)
-template <int T>
-__device__ __forceinline__ float interp_lagrange(const float *__restrict__ d, float rx, float ry, float rz, size_t n)
-{
-    constexpr int T2 = T * T;
-    float f = d[n], fx = d[n + 1], fy = d[n + T], fz = d[n + T2];
-    float fxx = d[n - 1], fyy = d[n - T], fzz = d[n - T2];
-    float fxy = d[n + T + 1], fxz = d[n + T2 + 1], fyz = d[n + T2 + T];
-    fxy = fxy + ((f - fx) - fy);
-    fxz = fxz + ((f - fx) - fz);
-    fyz = fyz + ((f - fy) - fz);
-    fx = (fx - fxx) * .5f;
-    fy = (fy - fyy) * .5f;
-    fz = (fz - fzz) * .5f;
-    fxx = (fx + fxx) - f;
-    fyy = (fy + fyy) - f;
-    fzz = (fz + fzz) - f;
-    return f + (fx + fxx * rx) * rx + (fy + fyy * ry) * ry + (fz + fzz * rz) * rz + fxy * rx * ry + fxz * rx * rz +
-           fyz * ry * rz;
-}
-// __interpolate_dat, trilinear form (AMR.h:767-785)
-template <int T>
-__device__ __forceinline__ float interp_trilinear(const float *__restrict__ d, float rx, float ry, float rz, size_t n)
-{
-    constexpr int T2 = T * T;
-    float r00 = d[n];
-    r00 = r00 * (1.f - rx) + d[n + 1] * rx;
-    float r01 = d[n + T];
-    r01 = r01 * (1.f - rx) + d[n + T + 1] * rx;
-    r00 = r00 * (1.f - ry) + r01 * ry;
-    float r10 = d[n + T2];
-    r10 = r10 * (1.f - rx) + d[n + T2 + 1] * rx;
-    float r11 = d[n + T2 + T];
-    r11 = r11 * (1.f - rx) + d[n + T2 + T + 1] * rx;
-    r10 = r10 * (1.f - ry) + r11 * ry;
-    return r00 * (1.f - rz) + r10 * rz;
-}
-// 4-point Lagrange weights on nodes -1, 0, 1, 2 (exact on cubics)
-__device__ __forceinline__ void lag4(float t, float *w)
-{
-    w[0] = -t * (t - 1.f) * (t - 2.f) / 6.f;
-    w[1] = (t + 1.f) * (t - 1.f) * (t - 2.f) / 2.f;
-    w[2] = -(t + 1.f) * t * (t - 2.f) / 2.f;
-    w[3] = (t + 1.f) * t * (t - 1.f) / 6.f;
-}
-template <int T>
-__device__ __forceinline__ float interp_tricubic(const float *__restrict__ d, const float *wx, const float *wy,
-                                                 const float *wz, size_t n)
-{
-    constexpr int T2 = T * T;
-    float acc = 0.f;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        float ay = 0.f;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const float *r = d + n + (ptrdiff_t)(k - 1) * T2 + (ptrdiff_t)(j - 1) * T - 1;
-            const float ax = r[0] * wx[0] + r[1] * wx[1] + r[2] * wx[2] + r[3] * wx[3];
-            ay = ay + ax * wy[j];
-        }
-        acc = acc + ay * wz[k];
-    }
-    return acc;
-}
 
 template <int S, int P, int MODE>
 __global__ void __launch_bounds__(S *S) k_ghost_fill(FieldBatchDev b, const uint64_t *__restrict__ bnd,
@@ -204,7 +129,7 @@ __global__ void __launch_bounds__(S *S) k_ghost_fill(FieldBatchDev b, const uint
 template <int S, int P, int MODE, int VPG>
 __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n_vars, const uint64_t *__restrict__ bnd,
                                                           const int32_t *__restrict__ tiles, uint32_t substep,
-                                                          uint32_t maxd, float frac, int equal_mode)
+                                                          uint32_t maxd, float frac, int equal_mode, int skip_learned)
 {
     constexpr int T = S + 2 * P, NC = S * S * P;
     constexpr size_t CELLS = (size_t)T * T * T;
@@ -219,6 +144,7 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
     const float3 relp = cf3_unpack((uint32_t)(bb >> 32));
     const uint32_t ddiff = bnd_ddiff(nd);
     if (ddiff == 0 && equal_mode == AM_EQUAL_SKIP) return;
+    if (skip_learned && ddiff == 1 && tgt > 0) return;  // the learned fill's faces
     const float rel_scale = 1.f / (float)(1u << ddiff);
     const float tmp = (float)((substep >> (maxd - (uint32_t)cd)) & ((1u << ddiff) - 1u));
     const float ty = ddiff == 0 ? 1.f : (tmp + frac) / (float)(1u << ddiff);
@@ -315,21 +241,15 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
         float wx[4], wy[4], wz[4];
         if (MODE == AM_INTERP_TRICUBIC) {
             lag4(rp[0], wx); lag4(rp[1], wy); lag4(rp[2], wz);
+            for (int v = 0; v < nv; v++)
+                b.new_[v0 + v][wn] =
+                    tricubic_time<T>(b.old_[v0 + v], b.new_[v0 + v], b.older[v0 + v], ddiff, ty, wx, wy, wz, rn);
+            continue;
         }
         for (int v = 0; v < nv; v++) {
-            const float *dold = b.old_[v0 + v], *dnew = b.new_[v0 + v], *dolder = b.older[v0 + v];
+            const float *dold = b.old_[v0 + v], *dnew = b.new_[v0 + v];
             float o, nw;
-            if (MODE == AM_INTERP_TRICUBIC) {
-                o = interp_tricubic<T>(dold, wx, wy, wz, rn);
-                nw = interp_tricubic<T>(dnew, wx, wy, wz, rn);
-                if (dolder && ddiff > 0) {
-                    const float od = interp_tricubic<T>(dolder, wx, wy, wz, rn);
-                    const float s_ = ty;
-                    const float c0 = .5f * s_ * (s_ - 1.f), c1 = (1.f - s_) * (1.f + s_), c2 = .5f * s_ * (s_ + 1.f);
-                    b.new_[v0 + v][wn] = od * c0 + o * c1 + nw * c2;
-                    continue;
-                }
-            } else if (MODE == AM_INTERP_TRILINEAR) {
+            if (MODE == AM_INTERP_TRILINEAR) {
                 o = interp_trilinear<T>(dold, rp[0], rp[1], rp[2], rn);
                 nw = interp_trilinear<T>(dnew, rp[0], rp[1], rp[2], rn);
             } else {
@@ -345,7 +265,7 @@ constexpr int GHOST_VPG = 6;  // fields per workgroup in the multi-field fill
 
 template <int S, int P>
 static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, int interp, float frac, int equal_mode,
-                              void *stream)
+                              int skip_learned, void *stream)
 {
     FieldBatchDev d;
     for (int v = 0; v < b->n_vars; v++) {
@@ -359,15 +279,15 @@ static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, int 
     if (interp == AM_INTERP_LAGRANGE)
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_LAGRANGE, GHOST_VPG>), grid, dim3(256), 0, st, d,
                            b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac,
-                           equal_mode);
+                           equal_mode, skip_learned);
     else if (interp == AM_INTERP_TRILINEAR)
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRILINEAR, GHOST_VPG>), grid, dim3(256), 0, st, d,
                            b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac,
-                           equal_mode);
+                           equal_mode, skip_learned);
     else if (interp == AM_INTERP_TRICUBIC)
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRICUBIC, GHOST_VPG>), grid, dim3(256), 0, st, d,
                            b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac,
-                           equal_mode);
+                           equal_mode, skip_learned);
     else
         return set_error(AM_ERR_INVALID, "am_ghost_fill: unknown interpolation mode %d", interp);
     return check_launch("k_ghost_fill_multi");
@@ -560,7 +480,13 @@ extern "C" int am_ghost_fill_stage_batched(const am_field_batch *b, const am_mes
         return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_batched: stage_frac %g", (double)stage_frac);
     if (int rc = am::mesh_checks(mesh, g, true, false)) return rc;
     if (b->n_vars == 0 || mesh->n_tiles == 0) return AM_OK;
-    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, interp, stage_frac, equal_mode, stream);
+    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, interp, stage_frac, equal_mode, 0, stream);
+}
+
+int am::ghost_stage_fill_skip_learned(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g,
+                                      float stage_frac, int equal_mode, void *stream)
+{
+    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, AM_INTERP_TRICUBIC, stage_frac, equal_mode, 1, stream);
 }
 
 extern "C" int am_ghost_fill(float *old_, float *new_, const float *older, const am_mesh *mesh, const am_geometry *g,

@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) k_mlp_pairs(PairFields pf, const int32_t 
     float dev = 0.f;
 #pragma unroll
     for (int i = 0; i < 64; i++) dev = fmaxf(dev, fabsf(st[i] - mu));
-    const float sg = dev + 1e-7f * (1.f + fabsf(mu));
+    const float sg = dev + 1e-3f * fabsf(mu) + 1e-7f;
     const float inv = 1.f / sg;
     float *x = X + sample * 1536 + v * 64;
 #pragma unroll

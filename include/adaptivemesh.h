@@ -52,7 +52,8 @@ enum am_status {
 enum am_interp {
     AM_INTERP_LAGRANGE = 0,  /* 10-point quadratic, the reference default (AMR.h:4, 673-699) */
     AM_INTERP_TRILINEAR = 1, /* the reference's alternative (AMR.h:767-785) */
-    AM_INTERP_TRICUBIC = 2   /* 4x4x4 Lagrange in space, quadratic in time (extension) */
+    AM_INTERP_TRICUBIC = 2,  /* 4x4x4 Lagrange in space, quadratic in time (extension) */
+    AM_INTERP_LEARNED = 3    /* learned per-octant prolongation on one-level-coarser faces, tricubic elsewhere */
 };
 
 const char *am_last_error_string(void);
@@ -334,7 +335,9 @@ int am_swave_energy(const am_swave_fields *f, const am_mesh *mesh, const am_geom
  * children of a coarse cell (24 fields) from the cell's 4^3 coarse stencil
  * (offsets -1..2 per axis), as a correction on top of the tricubic
  * baseline; each field's stencil is shifted by its centre value mu and
- * scaled by sigma = max |stencil - mu| + 1e-7 (1 + |mu|), the output is the
+ * scaled by sigma = max |stencil - mu| + 1e-3 |mu| + 1e-7 (a field flat to
+ * within rounding must not turn its rounding noise into O(1) inputs), the
+ * output is the
  * residual divided by sigma.  Layers 1536 -> 128 -> 128 -> 192 with exact
  * GELU; parameters in one fp32 array: W1[1536][128] b1[128] W2[128][128]
  * b2[128] W3[128][192] b3[192] (weights [in][out], row-major). */
@@ -374,6 +377,20 @@ int am_mlp_adam(float *params, const float *grads, float *m, float *v, int64_t n
 int64_t am_mlp_pairs_per_tile(const am_geometry *g);
 int am_mlp_pairs(const float *const *fine, const int32_t *tiles, int32_t n_tiles, const am_geometry *g, float *x,
                  float *t, float *mu, float *sigma, void *stream);
+
+/* The stage-time ghost fill of am_ghost_fill_stage_batched with the learned
+ * prolongation on every face whose target is exactly one level coarser and
+ * not the root (the 24 BSSN fields; older for all fields or none): per
+ * coarse cell of the face's ghost slab, the MLP's children shifted so they
+ * average to the cell's value.  A cell whose 4^3 stencil would read an edge
+ * or corner ghost of the target takes the tricubic value; a face whose
+ * largest restriction residual |mean(children) - value| (field units,
+ * before the shift) exceeds `threshold` is filled by tricubic interpolation
+ * throughout.  Other faces: the tricubic fill (equal_mode as there).
+ * stats (device, 4 uint64, accumulated): learned faces, faces that fell
+ * back, cells that took the tricubic value, cells evaluated. */
+int am_ghost_fill_learned(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, float stage_frac,
+                          const float *params, float threshold, uint64_t *stats, int32_t equal_mode, void *stream);
 
 #ifdef __cplusplus
 }

@@ -10,7 +10,9 @@ They are written here from their definitions, to check the GPU kernels:
   of the 2x2x2 children, the restriction of AMR.h:975-995); for each coarse
   cell c whose 4^3 stencil c-1..c+2 lies inside the tile's padded region,
   the input is that stencil, per field shifted by the centre value mu and
-  scaled by sigma = max |stencil - mu| + 1e-7 (1 + |mu|); the target is the
+  scaled by sigma = max |stencil - mu| + 1e-3 |mu| + 1e-7 (the floor keeps a
+  field that is flat to rounding from turning its rounding noise into O(1)
+  inputs); the target is the
   8 fine children minus the tricubic baseline (4-point Lagrange through the
   stencil at offsets -1/4, +1/4 per axis), divided by sigma;
 * MLP 1536 -> 128 -> 128 -> 192 with exact GELU (z Phi(z)), parameters in
@@ -150,7 +152,7 @@ def tile_pairs(fine, slot, S, P):
                     z0, y0, x0 = 2 * cz + P, 2 * cy + P, 2 * cx + P
                     Y[v] = f[v][z0:z0 + 2, y0:y0 + 2, x0:x0 + 2]
                 mu = X[:, 1, 1, 1]
-                sg = np.abs(X - mu[:, None, None, None]).reshape(24, -1).max(1) + 1e-7 * (1 + np.abs(mu))
+                sg = np.abs(X - mu[:, None, None, None]).reshape(24, -1).max(1) + 1e-3 * np.abs(mu) + 1e-7
                 base = np.einsum("ai,bj,ck,vkji->vcba", W_CHILD, W_CHILD, W_CHILD, X)  # [v][jz][jy][jx]
                 xs.append(((X - mu[:, None, None, None]) / sg[:, None, None, None]).reshape(-1))
                 ts.append(((Y - base) / sg[:, None, None, None]).reshape(-1))
@@ -165,3 +167,122 @@ def children_from(X, Yn, mu, sigma):
     Xr = X.reshape(n, 24, 4, 4, 4) * sigma[:, :, None, None, None] + mu[:, :, None, None, None]
     base = np.einsum("ai,bj,ck,nvkji->nvcba", W_CHILD, W_CHILD, W_CHILD, Xr).reshape(n, 24, 8)
     return base + Yn.reshape(n, 24, 8) * sigma[:, :, None]
+
+
+# ------------------------------------------------------- learned ghost fill
+def _read_pos(w, rel, S, P):
+    """AMR.h:736-745 in float32, for one axis (rel_scale 1/2)."""
+    f = np.float32
+    r = (f(w) - f(P) + f(.5)) * (f(1) / f(S)) - f(.5)
+    r = r * f(.5) + f(rel)
+    return (r + f(.5)) * f(S) + f(P) - f(.5)
+
+
+def _slab_cells(d, S, P):
+    """(w0, w1, w2) arrays of the ghost slab cells of face d, in the kernel's
+    order."""
+    c = np.arange(S * S * P)
+    if d in (0, 3):
+        l, a, b = c % P, (c // P) % S, c // (P * S)
+        return (l + P + S if d == 0 else l), a + P, b + P
+    if d in (1, 4):
+        a, l, b = c % S, (c // S) % P, c // (S * P)
+        return a + P, (l + P + S if d == 1 else l), b + P
+    a, b, l = c % S, (c // S) % S, c // (S * S)
+    return a + P, b + P, (l + P + S if d == 2 else l)
+
+
+def learned_fill(old, new, older, boundaries, positions, tiles, substep, max_depth, frac, params, threshold, S, P,
+                 orc):
+    """The learned stage fill on float32 arrays (24 fields, in place on
+    `new`): the tricubic fill (the float restatement, equal-depth faces
+    copied) everywhere, then, on faces one level coarser than a non-root
+    target, the MLP's children in FP64 for every cell whose stencil avoids
+    the target's edge and corner ghosts, unless the face's largest
+    restriction residual exceeds `threshold`.  Returns ((faces learned,
+    faces fallen back, cells tricubic, cells evaluated), the tricubic-only
+    result, the flat indices of the cells given learned values)."""
+    T = S + 2 * P
+    C = T ** 3
+    for v in range(24):
+        orc.copy_boundaries(old[v].copy(), new[v], None if older is None else older[v], boundaries, tiles, substep,
+                            max_depth, True, 2, S, P, frac, equal_copy=True)
+    tric = [a.copy() for a in new]
+    stats = [0, 0, 0, 0]
+    written = []
+    bnd = np.asarray(boundaries, np.uint64)
+    O = np.stack([np.asarray(a, np.float64) for a in old])
+    N = np.stack([np.asarray(a, np.float64) for a in new])
+    Q = None if older is None else np.stack([np.asarray(a, np.float64) for a in older])
+    si = np.arange(64)
+    ox, oy, oz = si & 3, (si >> 2) & 3, si >> 4
+    for node in tiles:
+        for d in range(6):
+            bb = int(bnd[node * 6 + d])
+            nd = bb & 0xFFFFFFFF
+            tgt = nd & 0xFFFFFF
+            tgt = tgt - (1 << 24) if tgt & (1 << 23) else tgt
+            cd = -1 if (nd >> 28) == 15 else nd >> 28
+            dd = (nd >> 24) & 15
+            if tgt <= 0 or cd <= 0 or dd != 1 or substep & ((1 << (max_depth - cd)) - 1):
+                continue
+            rel = orc.cf3_unpack(bb >> 32)[0]
+            ty = (float((substep >> (max_depth - cd)) & 1) + frac) * 0.5
+            tc = (0.5 * ty * (ty - 1), (1 - ty) * (1 + ty), 0.5 * ty * (ty + 1)) if older is not None else \
+                (0.0, 1 - ty, ty)
+            n = d % 3
+            clo, ncl = [], []
+            for k in range(3):
+                wlo = P if k != n else (P + S if d < 3 else 0)
+                whi = P + S - 1 if k != n else (2 * P + S - 1 if d < 3 else P - 1)
+                a = int(np.floor(_read_pos(wlo, rel[k], S, P) + np.float32(.5)))
+                z = int(np.floor(_read_pos(whi, rel[k], S, P) + np.float32(.5)))
+                clo.append(a)
+                ncl.append(z - a + 1)
+            NS = ncl[0] * ncl[1] * ncl[2]
+            sa = np.arange(NS)
+            cx = clo[0] + sa % ncl[0]
+            cy = clo[1] + (sa // ncl[0]) % ncl[1]
+            cz = clo[2] + sa // (ncl[0] * ncl[1])
+            px = cx[:, None] - 1 + ox[None, :]
+            py = cy[:, None] - 1 + oy[None, :]
+            pz = cz[:, None] - 1 + oz[None, :]
+            outs = sum(((p_ < P) | (p_ >= P + S)).astype(int) for p_ in (px, py, pz))
+            inb = (px >= 0) & (px < T) & (py >= 0) & (py < T) & (pz >= 0) & (pz < T) & (outs <= 1)
+            valid = inb.all(1)
+            r = tgt * C + (np.clip(pz, 0, T - 1) * T + np.clip(py, 0, T - 1)) * T + np.clip(px, 0, T - 1)
+            X = O[:, r] * tc[1] + N[:, r] * tc[2]
+            if Q is not None:
+                X = X + Q[:, r] * tc[0]
+            X = np.where(inb[None], X, 0.0).transpose(1, 0, 2)  # (NS, 24, 64)
+            mu = X[:, :, 21]
+            sg = np.abs(X - mu[:, :, None]).max(2) + 1e-3 * np.abs(mu) + 1e-7
+            Xn = (X - mu[:, :, None]) / sg[:, :, None]
+            y = forward(params, Xn.reshape(NS, -1))["y"].reshape(NS, 24, 8)
+            base = np.einsum("ai,bj,ck,nvkji->nvcba", W_CHILD, W_CHILD, W_CHILD,
+                             Xn.reshape(NS, 24, 4, 4, 4)).reshape(NS, 24, 8)
+            ch = base + y
+            m = ch.mean(2)
+            resid = np.abs(m * sg)[valid].max() if valid.any() else 0.0
+            ch = ch - m[:, :, None]
+            stats[3] += NS
+            stats[2] += int((~valid).sum())
+            if resid > threshold:
+                stats[1] += 1
+                continue
+            stats[0] += 1
+            w = _slab_cells(d, S, P)
+            cl, child = [], np.zeros(S * S * P, int)
+            for k in range(3):
+                rp = np.array([_read_pos(int(x), rel[k], S, P) for x in range(T)], np.float32)[w[k]]
+                c_ = np.floor(rp + np.float32(.5)).astype(int)
+                cl.append(c_)
+                child |= (rp > c_).astype(int) << k
+            sidx = (cl[0] - clo[0]) + ncl[0] * ((cl[1] - clo[1]) + ncl[1] * (cl[2] - clo[2]))
+            keep = valid[sidx]
+            wn = node * C + (w[2] * T + w[1]) * T + w[0]
+            written.append(wn[keep])
+            for v in range(24):
+                val = mu[sidx, v] + sg[sidx, v] * ch[sidx, v, child]
+                new[v][wn[keep]] = val[keep].astype(np.float32)
+    return tuple(stats), tric, (np.concatenate(written) if written else np.zeros(0, np.int64))

@@ -1,0 +1,120 @@
+"""The learned ghost fill on the GPU against the FP64 oracle (oracle/mlp.py
+learned_fill), on the puncture octree one sub-step into the run: depth-5
+tiles at counter 1 fill their faces from depth-4 targets at stage time
+(1 + 1/2) / 2 with quadratic time interpolation.
+
+* With a threshold no residual reaches, the learned cells match the oracle's
+  FP64 network to 2e-5 (1 + |value|) (fp32 MFMA over 1536 inputs); every
+  other ghost cell -- tricubic cells whose stencil would read the target's
+  edge ghosts, tricubic faces, equal-depth copies -- is bit-identical; the
+  statistics agree.
+* With a negative threshold every learned face falls back, and the whole
+  fill equals the tricubic fill of the float restatement bit for bit.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from adaptivemesh_amd import workloads as WL
+from oracle import mlp as M
+
+pytestmark = pytest.mark.gpu
+
+S, P, ORDER, OUTER, CFL = 16, 3, 4, 64.0, 0.25
+T = S + 2 * P
+
+
+@pytest.fixture(scope="module")
+def setup(am, orc, torch_cuda):
+    torch = torch_cuda
+    ops, n = WL.puncture_ops(OUTER)
+    g = am.geometry(S, P, ORDER, OUTER, CFL)
+    sim = am.BssnSim(g, n, am.bssn_options(periodic=False, interp=am.AM_INTERP_TRICUBIC))
+    tree = am.Tree(n, periodic=False)
+    ot = orc.OracleTree(n, False)
+    for p_, c in ops:
+        sim.add_node(p_, c)
+        tree.add_node(p_, c)
+        ot.add_node(p_, c)
+    e = ot.export()
+    st = WL.puncture_fields(e["positions"], S, P, OUTER)
+    for v in range(24):
+        sim.set_field(0, v, st[v])
+        sim.set_field(1, v, st[v])
+    sim.fill_ghosts()
+    sim.substep()
+    sim.sync()
+    bufs = {w: [sim.get_field(w, v) for v in range(24)] for w in (0, 1, 3)}
+    rng = np.random.default_rng(7)
+    params = M.init_params(3)
+    params[-M.N_OUT:] = rng.normal(0, 0.05, M.N_OUT)  # visible corrections
+    params32 = params.astype(np.float32)
+    tiles = [s for s in range(n) if int(e["positions"][s]) != 0xFFFFFFFF and int(e["positions"][s]) >> 27 == 5]
+    return dict(sim=sim, tree=tree, e=e, n=n, g=g, bufs=bufs, params=params32, tiles=tiles)
+
+
+def run_gpu(am, torch, s, threshold):
+    sim, g = s["sim"], s["g"]
+    for v in range(24):
+        sim.set_field(1, v, s["bufs"][1][v])  # the same input every call
+    stride = sim.max_slots * T ** 3 * 4
+    fb = am.FieldBatch()
+    fb.n_vars = 24
+    for v in range(24):
+        fb.old_[v] = sim.device_ptr(0) + v * stride
+        fb.new_[v] = sim.device_ptr(1) + v * stride
+        fb.older[v] = sim.device_ptr(3) + v * stride
+    m = s["tree"].mesh(5)
+    m.substep = 1
+    prm = torch.tensor(s["params"], device="cuda")
+    stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+    am.check(am.lib().am_ghost_fill_learned(C.byref(fb), C.byref(m), C.byref(g), 0.5, prm.data_ptr(), threshold,
+                                            stats.data_ptr(), am.AM_EQUAL_COPY, am.stream_handle()), "learned fill")
+    torch.cuda.synchronize()
+    return [sim.get_field(1, v) for v in range(24)], tuple(int(x) for x in stats.cpu().numpy())
+
+
+def run_oracle(orc, s, threshold):
+    new = [a.copy() for a in s["bufs"][1]]
+    stats, tric, idx = M.learned_fill(s["bufs"][0], new, s["bufs"][3], s["e"]["boundaries"], s["e"]["positions"],
+                                      s["tiles"], 1, 5, 0.5, s["params"].astype(np.float64), threshold, S, P, orc)
+    return new, tric, stats, idx
+
+
+def test_learned_fill_matches_oracle(am, orc, torch_cuda, setup):
+    print("learned ghost fill, depth-5 puncture tiles, counter 1", flush=True)
+    got, gstats = run_gpu(am, torch_cuda, setup, 1e30)
+    want, tric, ostats, idx = run_oracle(orc, setup, 1e30)
+    print(f"  stats GPU {gstats}, oracle {ostats}", flush=True)
+    assert gstats == ostats and ostats[0] == 96 and ostats[1] == 0
+    worst = 0.0
+    n_learned = 0
+    learned = np.zeros(want[0].size, bool)
+    learned[idx] = True
+    for v in range(24):
+        n_learned += int(learned.sum())
+        err = np.abs(got[v][learned].astype(np.float64) - want[v][learned].astype(np.float64))
+        tol = 2e-5 * (1 + np.abs(want[v][learned].astype(np.float64)))
+        if err.size:
+            worst = max(worst, float(np.max(err / tol)))
+        assert np.all(err <= tol), v
+        bad = np.nonzero(~learned & (got[v].view(np.uint32) != want[v].view(np.uint32)))[0]
+        if bad.size:
+            slot, rem = np.divmod(bad[:6], T ** 3)
+            zz, rem = np.divmod(rem, T * T)
+            yy, xx = np.divmod(rem, T)
+            print(f"  field {v}: {bad.size} mismatches, e.g. slot/z/y/x {list(zip(slot, zz, yy, xx))}, "
+                  f"got {got[v][bad[:6]]}, want {want[v][bad[:6]]}, tricubic {tric[v][bad[:6]]}", flush=True)
+        assert bad.size == 0, v
+    print(f"  {n_learned} learned cell values, worst error / tolerance {worst:.3f}", flush=True)
+    assert n_learned > 0
+
+
+def test_learned_fill_falls_back_to_tricubic(am, orc, torch_cuda, setup):
+    got, gstats = run_gpu(am, torch_cuda, setup, -1.0)
+    want, tric, ostats, idx = run_oracle(orc, setup, -1.0)
+    assert idx.size == 0
+    assert gstats == ostats and gstats[0] == 0 and gstats[1] == 96
+    for v in range(24):
+        assert np.array_equal(got[v].view(np.uint32), tric[v].view(np.uint32)), v

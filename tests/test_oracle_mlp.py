@@ -86,3 +86,24 @@ def test_pairs_restriction_and_linear_exactness():
     for v in (0, 7, 23):
         want = f5[v][2 * cz + P:2 * cz + P + 2, 2 * cy + P:2 * cy + P + 2, 2 * cx + P:2 * cx + P + 2].ravel()
         assert np.allclose(ch[k, v], want, atol=1e-13)
+
+
+def test_learned_fill_with_zero_network_is_exact_on_linear_fields(orc):
+    """With every parameter zero the network adds nothing: each child is
+    the tricubic baseline shifted so the 8 children average to the coarse
+    value.  On a linear field the baseline is exact and averages to the
+    centre, so the learned fill equals the (exact) tricubic fill."""
+    ops, n = meshes.graded()
+    t = orc.OracleTree(n, False)
+    for a, c in ops:
+        t.add_node(a, c)
+    e = t.export()
+    f = [F.sample(e["positions"], n, S, P, 16.0, lambda x, y, z, v=v: 1.0 + 0.01 * v + 0.03 * x - 0.02 * y + 0.01 * z)
+         for v in range(24)]
+    tiles = [s for s in F.used_slots(e) if F.depth_of(e, s) == 3]
+    new = [a.copy() for a in f]
+    stats, tric, _ = M.learned_fill(f, new, f, e["boundaries"], e["positions"], tiles, 0, e["max_depth"], 1.0,
+                                 np.zeros(M.N_PARAMS), 1e30, S, P, orc)
+    assert stats[0] > 0 and stats[1] == 0
+    for v in range(24):
+        assert np.max(np.abs(new[v].astype(np.float64) - tric[v])) < 2e-6, v
