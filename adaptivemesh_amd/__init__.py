@@ -232,6 +232,8 @@ _RK4_SIM_SIGS = {
     "get_launch": (C.c_int, [_P, _P]),
     "use_graph": (C.c_int, [_P, C.c_int32]),
     "autotune": (C.c_int, [_P, C.c_int32, _P]),
+    "set_prolongation": (C.c_int, [_P, _P, C.c_float]),
+    "prolongation_stats": (C.c_int, [_P, _P]),
 }
 for _prefix in ("am_bssn_sim", "am_swave_sim"):
     for _k, _v in _RK4_SIM_SIGS.items():
@@ -507,6 +509,20 @@ class _Rk4Sim:
         return {"z_block": r.z_block, "fd_grid_limit": r.fd_grid_limit, "rhs_grid_limit": r.rhs_grid_limit,
                 "trials": [(kinds[r.trial[i].kind], r.trial[i].value, round(r.trial[i].ms, 4))
                            for i in range(r.n_trials)]}
+
+    def set_prolongation(self, params, threshold: float = 1e-5):
+        """params: a device tensor of am_mlp_param_count() floats (kept
+        referenced here while the driver uses it) or an object with a
+        `params` tensor (learned.Prolongation)."""
+        t = getattr(params, "params", params)
+        self._mlp_params = t
+        check(self._f("set_prolongation")(self.h, t.data_ptr(), threshold), "set_prolongation")
+
+    def prolongation_stats(self) -> dict:
+        out = np.zeros(4, np.uint64)
+        check(self._f("prolongation_stats")(self.h, out.ctypes.data), "prolongation_stats")
+        return dict(learned_blocks=int(out[0]), fallback_blocks=int(out[1]), tricubic_cells=int(out[2]),
+                    cells=int(out[3]))
 
     def kernel_times(self) -> dict:
         ms = np.zeros(len(KERNEL_CLASSES))

@@ -33,10 +33,9 @@ int set_error(int code, const char *fmt, ...);
 int check_launch(const char *what);
 DepthScales depth_scales(const am_geometry &g);
 bool aligned16(const void *p);
-// the tricubic stage fill of every face except those the learned fill
-// handles (one level coarser, non-root target)
-int ghost_stage_fill_skip_learned(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g,
-                                  float stage_frac, int equal_mode, void *stream);
+// the tricubic stage fill of every face (what the learned fill starts from)
+int ghost_stage_fill(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, float stage_frac,
+                     int equal_mode, void *stream);
 
 }  // namespace am
 

@@ -2,10 +2,11 @@
 // am_mlp.hip evaluated inside the ghost fill.  No counterpart in the
 // reference.
 //
-// One workgroup (4 waves) per (fine tile, face) whose target is one level
-// coarser and not the root.  The face's ghost slab covers a box of coarse
-// cells of the target (2 x 8 x 8 for S = 16, P = 3); each coarse cell is one
-// sample, processed 64 at a time:
+// The plain tricubic stage fill runs first over every face; this kernel then
+// overwrites the cells it predicts.  Faces whose target is one level coarser
+// and not the root are covered by a box of coarse cells of the target (2 x 8
+// x 8 for S = 16, P = 3); each coarse cell is one sample, and one workgroup
+// (4 waves) takes a block of 64 samples:
 //   1. per field: gather the cell's 4^3 stencil (offsets -1..2) of the
 //      target at the stage time (quadratic in time through older/old/new,
 //      as the tricubic fill), normalise by the centre value mu and the
@@ -14,12 +15,13 @@
 //   2. layers 2 and 3 on MFMA from LDS; output + baseline = normalised
 //      children;
 //   3. the restriction residual |mean(children) - mu| (field units) is the
-//      face's self-check; the children are then shifted so their mean is mu;
-//   4. every ghost cell of the slab takes its child's value.
-// A stencil that would read an edge or corner ghost of the target (never
-// filled) makes its sample take the tricubic value; a face whose largest
-// residual exceeds the threshold is refilled entirely by the tricubic fill,
-// with the same arithmetic (am_interp.hpp) as the plain fill.
+//      block's self-check; the children are then shifted so their mean is
+//      mu;
+//   4. if the block's largest residual is within the threshold, every ghost
+//      cell of its samples takes its child's value.
+// A sample whose stencil would read an edge or corner ghost of the target
+// (never filled) keeps the tricubic value, as does every cell of a block
+// that fails the check.
 #include "am_common.hpp"
 #include "am_device.hpp"
 #include "am_interp.hpp"
@@ -68,7 +70,8 @@ template <int S, int P>
 __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const uint64_t *__restrict__ bnd,
                                                        const int32_t *__restrict__ tiles, uint32_t substep,
                                                        uint32_t maxd, float frac, const float *__restrict__ prm,
-                                                       float threshold, unsigned long long *__restrict__ stats)
+                                                       float threshold, unsigned long long *__restrict__ stats,
+                                                       int n_blocks)
 {
     constexpr int T = S + 2 * P, NC = S * S * P;
     constexpr size_t CELLS = (size_t)T * T * T;
@@ -80,7 +83,8 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
     __shared__ int face_resid_bits;
     __shared__ int n_invalid;
 
-    const int ti = blockIdx.x / 6, dir = blockIdx.x - ti * 6;
+    const int face = blockIdx.x / n_blocks, blk = blockIdx.x - face * n_blocks;
+    const int ti = face / 6, dir = face - ti * 6;
     const int node = tiles[ti];
     const uint64_t bb = bnd[(size_t)node * 6 + dir];
     const uint32_t nd = (uint32_t)bb;
@@ -113,13 +117,14 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
         }
     }
     const int NS = ncl[0] * ncl[1] * ncl[2];
+    const int chunk = blk * 64;
+    if (chunk >= NS) return;  // uniform per workgroup
     if (tid == 0) {
         face_resid_bits = 0;
         n_invalid = 0;
     }
     const size_t tbase = (size_t)tgt * CELLS;
-
-    for (int chunk = 0; chunk < NS; chunk += 64) {
+    {
         const int ns = min(64, NS - chunk);
         if (tid < 64) valid[tid] = 1;
         __syncthreads();
@@ -303,53 +308,33 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
             if (q == 0 && !valid[s]) atomicAdd(&n_invalid, 1);
         }
         __syncthreads();
-        // 4. write the slab cells of this chunk's samples
-        for (int c = tid; c < NC; c += 256) {
-            uint32_t w[3];
-            ghost_cell_of<S, P>(dir, c, w[0], w[1], w[2]);
-            float rp[3];
-            int cl[3], child = 0;
+        // 4. a block within the threshold writes its samples' slab cells
+        const bool pass = __int_as_float(face_resid_bits) <= threshold;
+        if (pass) {
+            for (int c = tid; c < NC; c += 256) {
+                uint32_t w[3];
+                ghost_cell_of<S, P>(dir, c, w[0], w[1], w[2]);
+                int cl[3], child = 0;
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                rp[k] = read_pos<S, P>(w[k], rl[k], rel_scale);
-                cl[k] = (int)floorf(rp[k] + .5f);
-                child |= (rp[k] > (float)cl[k] ? 1 : 0) << k;
-            }
-            const int sa = (cl[0] - clo[0]) + ncl[0] * ((cl[1] - clo[1]) + ncl[1] * (cl[2] - clo[2]));
-            if (sa < chunk || sa >= chunk + ns) continue;
-            const int sl = sa - chunk;
-            const size_t wn = (size_t)node * CELLS + w[0] + ((size_t)w[1] + (size_t)w[2] * T) * T;
-            if (valid[sl]) {
-                for (int v = 0; v < 24; v++) b.new_[v][wn] = musg[0][v][sl] + musg[1][v][sl] * Bn[sl][v * 8 + child];
-            } else {
-                float wx[4], wy[4], wz[4];
-                const size_t rn = tbase + tricubic_nodes<T, P>(tgt, rp, wx, wy, wz);
+                for (int k = 0; k < 3; k++) {
+                    const float rp = read_pos<S, P>(w[k], rl[k], rel_scale);
+                    cl[k] = (int)floorf(rp + .5f);
+                    child |= (rp > (float)cl[k] ? 1 : 0) << k;
+                }
+                const int sa = (cl[0] - clo[0]) + ncl[0] * ((cl[1] - clo[1]) + ncl[1] * (cl[2] - clo[2]));
+                if (sa < chunk || sa >= chunk + ns) continue;
+                const int sl = sa - chunk;
+                if (!valid[sl]) continue;
+                const size_t wn = (size_t)node * CELLS + w[0] + ((size_t)w[1] + (size_t)w[2] * T) * T;
                 for (int v = 0; v < 24; v++)
-                    b.new_[v][wn] = tricubic_time<T>(b.old_[v], b.new_[v], b.older[v], ddiff, ty, wx, wy, wz, rn);
+                    b.new_[v][wn] = musg[0][v][sl] + musg[1][v][sl] * Bn[sl][v * 8 + child];
             }
         }
-        __syncthreads();
-    }
-    // a face whose residual exceeds the threshold takes the tricubic fill
-    const bool fallback = __int_as_float(face_resid_bits) > threshold;
-    if (fallback) {
-        for (int c = tid; c < NC; c += 256) {
-            uint32_t w[3];
-            ghost_cell_of<S, P>(dir, c, w[0], w[1], w[2]);
-            float rp[3];
-#pragma unroll
-            for (int k = 0; k < 3; k++) rp[k] = read_pos<S, P>(w[k], rl[k], rel_scale);
-            float wx[4], wy[4], wz[4];
-            const size_t rn = tbase + tricubic_nodes<T, P>(tgt, rp, wx, wy, wz);
-            const size_t wn = (size_t)node * CELLS + w[0] + ((size_t)w[1] + (size_t)w[2] * T) * T;
-            for (int v = 0; v < 24; v++)
-                b.new_[v][wn] = tricubic_time<T>(b.old_[v], b.new_[v], b.older[v], ddiff, ty, wx, wy, wz, rn);
+        if (tid == 0 && stats) {
+            atomicAdd(&stats[pass ? 0 : 1], 1ull);
+            atomicAdd(&stats[2], (unsigned long long)n_invalid);
+            atomicAdd(&stats[3], (unsigned long long)ns);
         }
-    }
-    if (tid == 0 && stats) {
-        atomicAdd(&stats[fallback ? 1 : 0], 1ull);
-        atomicAdd(&stats[2], (unsigned long long)n_invalid);
-        atomicAdd(&stats[3], (unsigned long long)NS);
     }
 }
 
@@ -368,10 +353,15 @@ static int launch_learned(const am_field_batch *fb, const am_mesh *mesh, const a
         if ((fb->older[v] == nullptr) != (fb->older[0] == nullptr))
             return set_error(AM_ERR_INVALID, "am_ghost_fill_learned: older must be given for all fields or none");
     }
-    // every other face: the tricubic stage fill
-    if (int rc = ghost_stage_fill_skip_learned(fb, mesh, g, frac, equal_mode, stream)) return rc;
-    hipLaunchKernelGGL((k_ghost_learned<S, P>), dim3(mesh->n_tiles * 6), dim3(256), 0, (hipStream_t)stream, d,
-                       mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, params, threshold, stats);
+    // the tricubic stage fill everywhere, then the learned cells on top
+    if (int rc = ghost_stage_fill(fb, mesh, g, frac, equal_mode, stream)) return rc;
+    // samples per face: at most S/2 + 1 coarse cells per tangential axis and
+    // P/2 + 1 along the normal (the slab's P fine layers)
+    constexpr int NS_MAX = (S / 2 + 1) * (S / 2 + 1) * (P / 2 + 1);
+    constexpr int NB = (NS_MAX + 63) / 64;
+    hipLaunchKernelGGL((k_ghost_learned<S, P>), dim3(mesh->n_tiles * 6 * NB), dim3(256), 0, (hipStream_t)stream, d,
+                       mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, params, threshold, stats,
+                       NB);
     return check_launch("k_ghost_learned");
 }
 

@@ -444,6 +444,17 @@ int32_t am_wave_sim_rng_state(const am_wave_sim *s) { return s ? s->amr->sim_dat
     {                                                                                                              \
         if (reps < 1) return am::set_error(AM_ERR_INVALID, "autotune needs reps >= 1");                            \
         return rk4_call(s, [&](auto &d) { d.autotune(reps, reinterpret_cast<am::rk4_tune_report *>(out)); });      \
+    }                                                                                                              \
+    int P##_set_prolongation(H *s, const float *params, float threshold)                                           \
+    {                                                                                                              \
+        return rk4_call(s, [&](auto &d) { d.set_prolongation(params, threshold); });                               \
+    }                                                                                                              \
+    int P##_prolongation_stats(H *s, uint64_t *out4)                                                               \
+    {                                                                                                              \
+        if (!out4) return am::set_error(AM_ERR_INVALID, "null output");                                            \
+        return rk4_call(s, [&](auto &d) {                                                                          \
+            check(am_memcpy_d2h(out4, d.mlp_stats.gpu_buffer_ptr, 4 * sizeof(uint64_t), d.stream), "stats");       \
+        });                                                                                                        \
     }
 
 static_assert(sizeof(am_tune_report) == sizeof(am::rk4_tune_report), "tune report layout");

@@ -483,10 +483,10 @@ extern "C" int am_ghost_fill_stage_batched(const am_field_batch *b, const am_mes
     AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, interp, stage_frac, equal_mode, 0, stream);
 }
 
-int am::ghost_stage_fill_skip_learned(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g,
-                                      float stage_frac, int equal_mode, void *stream)
+int am::ghost_stage_fill(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, float stage_frac,
+                         int equal_mode, void *stream)
 {
-    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, AM_INTERP_TRICUBIC, stage_frac, equal_mode, 1, stream);
+    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, AM_INTERP_TRICUBIC, stage_frac, equal_mode, 0, stream);
 }
 
 extern "C" int am_ghost_fill(float *old_, float *new_, const float *older, const am_mesh *mesh, const am_geometry *g,

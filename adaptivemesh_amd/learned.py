@@ -188,3 +188,142 @@ class Prolongation:
 
     def numpy_params(self) -> np.ndarray:
         return self.params.detach().cpu().numpy()
+
+
+# ---------------------------------------------------------------- workflow
+def _sim_pairs(am, sim, min_depth: int = 1):
+    e = sim.export()
+    tiles = [s for s, p in enumerate(e["positions"]) if int(p) != 0xFFFFFFFF and (int(p) >> 27) >= min_depth]
+    return PairSet.from_sim(sim, tiles)
+
+
+def harvest(am, outer: float = 32.0, masses=(0.5, 1.0, 2.0), snapshots: int = 3,
+            gauge=((0.02, 0.0, 0), (0.05, 8.0, 1), (0.1, 16.0, 2), (0.05, 24.0, 0)), S: int = 16, P: int = 3):
+    """Training pairs from the plain solver at twice the resolution of the
+    puncture run on [-32, 32]^3 (outer = 32 by default): for each puncture
+    mass, the tricubic octree run's fields after 0..snapshots-1 coarse steps,
+    every tile of depth >= 2; plus analytic gauge waves (amplitude, time,
+    axis) on a uniform depth-2 mesh of one wavelength, every tile of depth
+    >= 1."""
+    from . import workloads as WL
+    sets = []
+    g = am.geometry(S, P, 4, outer, 0.25)
+    for m in masses:
+        ops, n = WL.puncture_ops(outer, radius=outer / 4)
+        sim = am.BssnSim(g, n, am.bssn_options(periodic=False, interp=am.AM_INTERP_TRICUBIC))
+        for p_, c in ops:
+            sim.add_node(p_, c)
+        f = WL.puncture_fields(sim.export()["positions"], S, P, outer, mass=m)
+        for v in range(24):
+            sim.set_field(0, v, f[v])
+            sim.set_field(1, v, f[v])
+        sim.fill_ghosts()
+        for k in range(snapshots):
+            if k:
+                sim.timestep()
+            sets.append(_sim_pairs(am, sim, 2))
+        del sim
+    for (a, t, axis) in gauge:
+        ops, n = WL.uniform_ops(2)
+        gw = am.geometry(S, P, 4, outer, 0.25)
+        sim = am.BssnSim(gw, n, am.bssn_options(periodic=True))
+        for p_, c in ops:
+            sim.add_node(p_, c)
+        f = WL.gauge_wave_fields(sim.export()["positions"], S, P, outer, amplitude=a, t=t, axis=axis)
+        for v in range(24):
+            sim.set_field(1, v, f[v])
+        sets.append(_sim_pairs(am, sim, 1))
+        del sim
+    return PairSet.cat(sets)
+
+
+def puncture_run(am, net=None, threshold: float = 1e-5, coarse_steps: int = 2, mass: float = 1.0,
+                 outer: float = 64.0, S: int = 16, P: int = 3):
+    """The octree puncture run with tricubic ghosts (net None) or the learned
+    prolongation: constraint norms after `coarse_steps` coarse steps, the
+    throughput of the timed steps and the learned fill's statistics."""
+    import time
+
+    import torch
+
+    from . import workloads as WL
+    ops, n = WL.puncture_ops(outer)
+    g = am.geometry(S, P, 4, outer, 0.25)
+    interp = am.AM_INTERP_TRICUBIC if net is None else am.AM_INTERP_LEARNED
+    sim = am.BssnSim(g, n, am.bssn_options(periodic=False, interp=interp))
+    for p_, c in ops:
+        sim.add_node(p_, c)
+    if net is not None:
+        sim.set_prolongation(net, threshold)
+    f = WL.puncture_fields(sim.export()["positions"], S, P, outer, mass=mass)
+    for v in range(24):
+        sim.set_field(0, v, f[v])
+        sim.set_field(1, v, f[v])
+    sim.fill_ghosts()
+    c0 = sim.constraints()
+    sim.sync()
+    torch.cuda.synchronize()
+    z0, t0 = sim.zone_cycles, time.perf_counter()
+    for _ in range(coarse_steps):
+        sim.timestep()
+    sim.sync()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    c1 = sim.constraints()
+    out = dict(H_l2=c1["H_l2"], M_l2=c1["M_l2"], H_l2_initial=c0["H_l2"],
+               zone_cycles_per_s=(sim.zone_cycles - z0) / dt, ms_per_coarse_step=1e3 * dt / coarse_steps)
+    if net is not None:
+        out["learned"] = sim.prolongation_stats()
+    return out
+
+
+def train_and_compare(am, train_steps: int = 3000, batch: int = 4096, seed: int = 0, threshold: float = 1e-5,
+                      coarse_steps: int = 2):
+    """The learned-versus-plain comparison: harvest pairs at twice the
+    resolution, train on the GPU, evaluate on held-out pairs of an unseen
+    mass at the run's resolution, then run the puncture with both fills."""
+    import time
+
+    import torch
+    t0 = time.perf_counter()
+    pairs = harvest(am)
+    train, val = pairs.split(16384, seed=seed)
+    del pairs
+    net = Prolongation(seed=seed)
+    t1 = time.perf_counter()
+    hist = net.fit(train, steps=train_steps, batch=batch, lr=1e-3, lr_final=3e-5, seed=seed,
+                   report_every=max(1, train_steps // 4))
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    n_train = train.n
+    del train
+    e_net, e_tri = net.evaluate(val, batch=4096)
+    # unseen data at the run's own resolution: mass 1.5 on [-32, 32]^3
+    test = _unseen(am)
+    u_net, u_tri = net.evaluate(test, batch=4096)
+    plain = puncture_run(am, None, coarse_steps=coarse_steps)
+    learned = puncture_run(am, net, threshold=threshold, coarse_steps=coarse_steps)
+    # the same run with every block that can use the network using it
+    learned_all = puncture_run(am, net, threshold=float("inf"), coarse_steps=coarse_steps)
+    return net, dict(pairs=n_train, train_steps=train_steps, batch=batch, loss_start=hist[0][1],
+                     loss_end=hist[-1][1], harvest_s=round(t1 - t0, 2), train_s=round(t2 - t1, 2),
+                     heldout_mse_network=e_net, heldout_mse_tricubic=e_tri,
+                     unseen_mse_network=u_net, unseen_mse_tricubic=u_tri, plain=plain, learned=learned,
+                     learned_all=learned_all, threshold=threshold)
+
+
+def _unseen(am, S: int = 16, P: int = 3):
+    from . import workloads as WL
+    outer = 64.0
+    ops, n = WL.puncture_ops(outer)
+    g = am.geometry(S, P, 4, outer, 0.25)
+    sim = am.BssnSim(g, n, am.bssn_options(periodic=False, interp=am.AM_INTERP_TRICUBIC))
+    for p_, c in ops:
+        sim.add_node(p_, c)
+    f = WL.puncture_fields(sim.export()["positions"], S, P, outer, mass=1.5)
+    for v in range(24):
+        sim.set_field(0, v, f[v])
+        sim.set_field(1, v, f[v])
+    sim.fill_ghosts()
+    sim.timestep()
+    return _sim_pairs(am, sim, 2)

@@ -104,3 +104,40 @@ def flat_noise_fields(n_slots: int, S: int, P: int, amplitude: float = 1e-6, see
         rng = np.random.default_rng(seed + v)
         out.append((FLAT[v] + rng.uniform(-amplitude, amplitude, n_slots * T ** 3)).astype(np.float32))
     return out
+
+
+def gauge_wave_fields(positions, S: int, P: int, outer: float, amplitude: float = 0.1, t: float = 0.0,
+                      axis: int = 0):
+    """The harmonic gauge wave (Alcubierre et al. 2004 testbed) as BSSN data:
+    ds^2 = -H dt^2 + H dw^2 + (the other two axes)^2 with
+    H = 1 - A sin(2 pi (w - t) / outer), w the coordinate along `axis`, so one
+    wavelength spans the domain.  alpha = sqrt(H), beta = B = 0,
+    chi = H^(-1/3), gt_ww = H^(2/3), the other diagonal gt = H^(-1/3),
+    K = K_ww / H with K_ww = -d_t H / (2 sqrt H), At_ww = (2/3) chi K_ww, the
+    other diagonal At = -chi K / 3, Gt^w = (2/3) H^(-5/3) d_w H."""
+    n = len(positions)
+    T = S + 2 * P
+    out = [np.full((n, T, T, T), FLAT[v], np.float64) for v in range(24)]
+    k = 2 * np.pi / outer
+    diag = (1, 4, 6)  # gt_xx, gt_yy, gt_zz
+    adiag = (8, 11, 13)
+    for s, pos in enumerate(positions):
+        pos = int(pos)
+        if pos == 0xFFFFFFFF:
+            continue
+        w = cell_coords(pos, S, P, outer)[axis]
+        ph = k * (w - t)
+        H = 1 - amplitude * np.sin(ph)
+        dtH = amplitude * k * np.cos(ph)
+        dwH = -amplitude * k * np.cos(ph)
+        chi = H ** (-1 / 3)
+        Kww = -dtH / (2 * np.sqrt(H))
+        K = Kww / H
+        out[0][s] = chi
+        for a in range(3):
+            out[diag[a]][s] = H ** (2 / 3) if a == axis else chi
+            out[adiag[a]][s] = (2 / 3) * chi * Kww if a == axis else -chi * K / 3
+        out[7][s] = K
+        out[14 + axis][s] = (2 / 3) * H ** (-5 / 3) * dwH
+        out[17][s] = np.sqrt(H)
+    return [a.reshape(-1).astype(np.float32) for a in out]

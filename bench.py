@@ -177,6 +177,9 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="bssn512")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a recorded step")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-versus-plain ghost fill comparison (training + two puncture runs)")
+    ap.add_argument("--learned-steps", type=int, default=3000, help="Adam steps of the learned prolongation")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time only a plain 1 GiB device copy (the calibration for HBM byte counters)")
     args = ap.parse_args()
@@ -287,6 +290,32 @@ def main():
         e = sim.export()
         result["cpu_baseline"] = cpu_baseline(n_slots, fields, e["positions"], list(range(n_slots)))
         result["cpu_baseline"]["value"] = round(result["cpu_baseline"]["value"], 1)
+    if not args.no_learned and args.workload == "bssn512":
+        # the second half of the metric: learned versus plain (tricubic)
+        # coarse-face ghost fill on the octree puncture run, same resolution
+        from adaptivemesh_amd import learned as LN
+        _, rep = LN.train_and_compare(am, train_steps=args.learned_steps)
+        pl, le, la = rep["plain"], rep["learned"], rep["learned_all"]
+        g6 = lambda x: float(f"{x:.6g}")
+        result["learned_vs_plain"] = {
+            "workload": WORKLOADS["puncture"] + "; 2 coarse steps",
+            "H_l2_plain": g6(pl["H_l2"]), "H_l2_learned": g6(le["H_l2"]),
+            "H_l2_ratio": round(le["H_l2"] / pl["H_l2"], 5),
+            "M_l2_plain": g6(pl["M_l2"]), "M_l2_learned": g6(le["M_l2"]),
+            "zone_cycles_per_s_plain": round(pl["zone_cycles_per_s"], 1),
+            "zone_cycles_per_s_learned": round(le["zone_cycles_per_s"], 1),
+            "residual_threshold": rep["threshold"],
+            "learned_fill": le["learned"],
+            "without_threshold": {"H_l2": g6(la["H_l2"]), "H_l2_ratio": round(la["H_l2"] / pl["H_l2"], 5),
+                                  "M_l2": g6(la["M_l2"]), "zone_cycles_per_s": round(la["zone_cycles_per_s"], 1),
+                                  "learned_fill": la["learned"]},
+            "training": {k: rep[k] for k in ("pairs", "train_steps", "batch", "loss_start", "loss_end", "harvest_s",
+                                             "train_s")},
+            "prolongation_mse": {"heldout_network": rep["heldout_mse_network"],
+                                 "heldout_tricubic": rep["heldout_mse_tricubic"],
+                                 "unseen_network": rep["unseen_mse_network"],
+                                 "unseen_tricubic": rep["unseen_mse_tricubic"]},
+        }
     print(json.dumps(result), flush=True)
 
 

@@ -378,17 +378,18 @@ int64_t am_mlp_pairs_per_tile(const am_geometry *g);
 int am_mlp_pairs(const float *const *fine, const int32_t *tiles, int32_t n_tiles, const am_geometry *g, float *x,
                  float *t, float *mu, float *sigma, void *stream);
 
-/* The stage-time ghost fill of am_ghost_fill_stage_batched with the learned
- * prolongation on every face whose target is exactly one level coarser and
- * not the root (the 24 BSSN fields; older for all fields or none): per
- * coarse cell of the face's ghost slab, the MLP's children shifted so they
- * average to the cell's value.  A cell whose 4^3 stencil would read an edge
- * or corner ghost of the target takes the tricubic value; a face whose
- * largest restriction residual |mean(children) - value| (field units,
- * before the shift) exceeds `threshold` is filled by tricubic interpolation
- * throughout.  Other faces: the tricubic fill (equal_mode as there).
- * stats (device, 4 uint64, accumulated): learned faces, faces that fell
- * back, cells that took the tricubic value, cells evaluated. */
+/* The stage-time ghost fill of am_ghost_fill_stage_batched (tricubic,
+ * equal_mode as there) with the learned prolongation on top, on every face
+ * whose target is exactly one level coarser and not the root (the 24 BSSN
+ * fields; older for all fields or none): the coarse cells of such a face's
+ * ghost slab form blocks of 64; each cell's ghost children are the MLP's,
+ * shifted so they average to the cell's value.  A cell whose 4^3 stencil
+ * would read an edge or corner ghost of the target keeps the tricubic
+ * value, and so does every cell of a block whose largest restriction
+ * residual |mean(children) - value| (field units, before the shift)
+ * exceeds `threshold`.  stats (device, 4 uint64, accumulated): learned
+ * blocks, blocks kept tricubic, cells kept tricubic for their stencil,
+ * cells evaluated. */
 int am_ghost_fill_learned(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, float stage_frac,
                           const float *params, float threshold, uint64_t *stats, int32_t equal_mode, void *stream);
 

@@ -89,7 +89,8 @@ struct kernel_timer {
 };
 
 struct rk4_options {
-    int32_t interp = AM_INTERP_LAGRANGE;  // coarse-face ghosts: reference quadratic, or tricubic + quadratic time
+    int32_t interp = AM_INTERP_LAGRANGE;  // coarse-face ghosts: reference quadratic, tricubic + quadratic time,
+                                          // or the learned prolongation (tricubic where it does not apply)
     int32_t z_block = 0;                  // FD planes per workgroup (0 = whole tile)
     int32_t fd_grid_limit = 0;            // persistent FD launch: workgroup cap (0 = one per item)
     int32_t rhs_grid_limit = 0;           // persistent RHS launch: workgroup cap (0 = one per item)
@@ -149,6 +150,11 @@ struct rk4_AMR_data {
     uint64_t graph_version = ~0ull;  // octree version it was recorded against
     uint64_t graph_zone_cycles = 0;  // zone-cycles one replay advances
 
+    // learned prolongation (AM_INTERP_LEARNED): caller-owned parameters
+    const float *mlp_params = nullptr;
+    float mlp_threshold = 1e-5f;
+    smart_gpu_buffer<uint64_t> mlp_stats;
+
     rk4_AMR_data(uint32_t node_slots, AMR<rk4_AMR_data> &p, const am_geometry &g, const rk4_options &o,
                  const options_t &so)
         : rng(), parent(p), geom(g), opt(o), sys(so)
@@ -164,7 +170,10 @@ struct rk4_AMR_data {
         old_ = smart_gpu_buffer<float>(NV * slots * cells);
         new_ = smart_gpu_buffer<float>(NV * slots * cells);
         acc = smart_gpu_buffer<float>(NV * slots * inner);
-        if (opt.interp == AM_INTERP_TRICUBIC) older = smart_gpu_buffer<float>(NV * slots * cells);
+        if (opt.interp == AM_INTERP_TRICUBIC || opt.interp == AM_INTERP_LEARNED)
+            older = smart_gpu_buffer<float>(NV * slots * cells);
+        mlp_stats = smart_gpu_buffer<uint64_t>(4);
+        mlp_stats.zero();
         d1 = smart_gpu_buffer<uint32_t>(NV * slots * inner);
         if (n_d2) d2d = smart_gpu_buffer<uint32_t>(n_d2 * slots * inner);
         if (n_off) d2o = smart_gpu_buffer<uint32_t>(n_off * slots * inner);
@@ -181,6 +190,7 @@ struct rk4_AMR_data {
         if (stream) am_stream_sync(stream);
         old_.destroy(); new_.destroy(); acc.destroy(); older.destroy();
         d1.destroy(); d2d.destroy(); d2o.destroy(); chr.destroy(); partial.destroy(); seed_table.destroy();
+        mlp_stats.destroy();
         timer.destroy();
         am_stream_destroy(stream);
         stream = nullptr;
@@ -315,9 +325,28 @@ struct rk4_AMR_data {
         if (d > 0 && parent.coarse_faces_at_depth(d) > 0) {
             am_field_batch b = batch(true);
             ev = timer.begin(kernel_timer::GHOST);
-            check(am_ghost_fill_stage_batched(&b, &m, &geom, frac[s], opt.interp, AM_EQUAL_SKIP, stream), "ghost");
+            ghost_fill(b, m, frac[s], AM_EQUAL_SKIP);
             timer.end(ev);
         }
+    }
+    // the coarse-face fill of the chosen interpolation
+    void ghost_fill(const am_field_batch &b, const am_mesh &m, float frac, int equal_mode)
+    {
+        if (opt.interp == AM_INTERP_LEARNED) {
+            if (!mlp_params) throw std::runtime_error("learned prolongation selected but no parameters set");
+            check(am_ghost_fill_learned(&b, &m, &geom, frac, mlp_params, mlp_threshold, mlp_stats.gpu_buffer_ptr,
+                                        equal_mode, stream),
+                  "learned ghost fill");
+        } else {
+            check(am_ghost_fill_stage_batched(&b, &m, &geom, frac, opt.interp, equal_mode, stream), "ghost");
+        }
+    }
+    void set_prolongation(const float *params, float threshold)
+    {
+        mlp_params = params;
+        mlp_threshold = threshold;
+        drop_graph();
+        check(am_memset_async(mlp_stats.gpu_buffer_ptr, 0, 4 * sizeof(uint64_t), stream), "stats");
     }
     void step_depth(int d)
     {
@@ -478,7 +507,7 @@ struct rk4_AMR_data {
             const am_mesh m = parent.mesh_depth(d);
             if (m.n_tiles == 0) continue;
             am_field_batch b = batch(false);
-            check(am_ghost_fill_stage_batched(&b, &m, &geom, 1.f, opt.interp, AM_EQUAL_COPY, stream), "ghost");
+            ghost_fill(b, m, 1.f, AM_EQUAL_COPY);
         }
     }
     // the System's diagnostics of `new` over the leaves (FP64), four values;

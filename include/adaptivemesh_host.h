@@ -128,6 +128,13 @@ int am_bssn_sim_set_launch(am_bssn_sim *s, int32_t z_block, int32_t fd_grid_limi
 int am_bssn_sim_get_launch(const am_bssn_sim *s, int32_t *out3);
 int am_bssn_sim_use_graph(am_bssn_sim *s, int32_t enable);
 int am_bssn_sim_autotune(am_bssn_sim *s, int32_t reps, am_tune_report *out);
+/* AM_INTERP_LEARNED: the MLP parameters (device, am_mlp_param_count()
+ * floats, owned by the caller and kept alive while the driver uses them) and
+ * the restriction-residual threshold of am_ghost_fill_learned; resets the
+ * statistics, which _prolongation_stats copies out (4 counters, see
+ * am_ghost_fill_learned). */
+int am_bssn_sim_set_prolongation(am_bssn_sim *s, const float *params, float threshold);
+int am_bssn_sim_prolongation_stats(am_bssn_sim *s, uint64_t *out4);
 
 /* -------------------------------------------- scalar wave RK4 driver
  * AMR<swave_rk4_data>: u_tt = lap u as fields (u, v = u_t) on the same RK4
@@ -173,6 +180,13 @@ int am_swave_sim_set_launch(am_swave_sim *s, int32_t z_block, int32_t fd_grid_li
 int am_swave_sim_get_launch(const am_swave_sim *s, int32_t *out3);
 int am_swave_sim_use_graph(am_swave_sim *s, int32_t enable);
 int am_swave_sim_autotune(am_swave_sim *s, int32_t reps, am_tune_report *out);
+/* AM_INTERP_LEARNED: the MLP parameters (device, am_mlp_param_count()
+ * floats, owned by the caller and kept alive while the driver uses them) and
+ * the restriction-residual threshold of am_ghost_fill_learned; resets the
+ * statistics, which _prolongation_stats copies out (4 counters, see
+ * am_ghost_fill_learned). */
+int am_swave_sim_set_prolongation(am_swave_sim *s, const float *params, float threshold);
+int am_swave_sim_prolongation_stats(am_swave_sim *s, uint64_t *out4);
 
 #ifdef __cplusplus
 }

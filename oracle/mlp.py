@@ -197,11 +197,12 @@ def learned_fill(old, new, older, boundaries, positions, tiles, substep, max_dep
     """The learned stage fill on float32 arrays (24 fields, in place on
     `new`): the tricubic fill (the float restatement, equal-depth faces
     copied) everywhere, then, on faces one level coarser than a non-root
-    target, the MLP's children in FP64 for every cell whose stencil avoids
-    the target's edge and corner ghosts, unless the face's largest
-    restriction residual exceeds `threshold`.  Returns ((faces learned,
-    faces fallen back, cells tricubic, cells evaluated), the tricubic-only
-    result, the flat indices of the cells given learned values)."""
+    target, whose coarse cells form blocks of 64 samples (z, y, x order), the
+    MLP's children in FP64 for every cell whose stencil avoids the target's
+    edge and corner ghosts, unless the block's largest restriction residual
+    exceeds `threshold`.  Returns ((blocks learned, blocks kept tricubic,
+    cells tricubic, cells evaluated), the tricubic-only result, the flat
+    indices of the cells given learned values)."""
     T = S + 2 * P
     C = T ** 3
     for v in range(24):
@@ -263,14 +264,19 @@ def learned_fill(old, new, older, boundaries, positions, tiles, substep, max_dep
                              Xn.reshape(NS, 24, 4, 4, 4)).reshape(NS, 24, 8)
             ch = base + y
             m = ch.mean(2)
-            resid = np.abs(m * sg)[valid].max() if valid.any() else 0.0
+            rs = np.abs(m * sg).max(1)
             ch = ch - m[:, :, None]
             stats[3] += NS
             stats[2] += int((~valid).sum())
-            if resid > threshold:
-                stats[1] += 1
-                continue
-            stats[0] += 1
+            passed = np.zeros(NS, bool)
+            for b0 in range(0, NS, 64):
+                blk = slice(b0, min(NS, b0 + 64))
+                r = rs[blk][valid[blk]].max() if valid[blk].any() else 0.0
+                if r > threshold:
+                    stats[1] += 1
+                else:
+                    stats[0] += 1
+                    passed[blk] = True
             w = _slab_cells(d, S, P)
             cl, child = [], np.zeros(S * S * P, int)
             for k in range(3):
@@ -279,7 +285,7 @@ def learned_fill(old, new, older, boundaries, positions, tiles, substep, max_dep
                 cl.append(c_)
                 child |= (rp > c_).astype(int) << k
             sidx = (cl[0] - clo[0]) + ncl[0] * ((cl[1] - clo[1]) + ncl[1] * (cl[2] - clo[2]))
-            keep = valid[sidx]
+            keep = valid[sidx] & passed[sidx]
             wn = node * C + (w[2] * T + w[1]) * T + w[0]
             written.append(wn[keep])
             for v in range(24):

@@ -6,10 +6,10 @@ tiles at counter 1 fill their faces from depth-4 targets at stage time
 * With a threshold no residual reaches, the learned cells match the oracle's
   FP64 network to 2e-5 (1 + |value|) (fp32 MFMA over 1536 inputs); every
   other ghost cell -- tricubic cells whose stencil would read the target's
-  edge ghosts, tricubic faces, equal-depth copies -- is bit-identical; the
-  statistics agree.
-* With a negative threshold every learned face falls back, and the whole
-  fill equals the tricubic fill of the float restatement bit for bit.
+  edge ghosts, equal-depth copies -- is bit-identical; the statistics (64-
+  sample blocks, two per face) agree.
+* With a negative threshold every block keeps the tricubic values, and the
+  whole fill equals the tricubic fill of the float restatement bit for bit.
 """
 import ctypes as C
 
@@ -87,7 +87,7 @@ def test_learned_fill_matches_oracle(am, orc, torch_cuda, setup):
     got, gstats = run_gpu(am, torch_cuda, setup, 1e30)
     want, tric, ostats, idx = run_oracle(orc, setup, 1e30)
     print(f"  stats GPU {gstats}, oracle {ostats}", flush=True)
-    assert gstats == ostats and ostats[0] == 96 and ostats[1] == 0
+    assert gstats == ostats and ostats[0] == 192 and ostats[1] == 0
     worst = 0.0
     n_learned = 0
     learned = np.zeros(want[0].size, bool)
@@ -115,6 +115,36 @@ def test_learned_fill_falls_back_to_tricubic(am, orc, torch_cuda, setup):
     got, gstats = run_gpu(am, torch_cuda, setup, -1.0)
     want, tric, ostats, idx = run_oracle(orc, setup, -1.0)
     assert idx.size == 0
-    assert gstats == ostats and gstats[0] == 0 and gstats[1] == 96
+    assert gstats == ostats and gstats[0] == 0 and gstats[1] == 192
     for v in range(24):
         assert np.array_equal(got[v].view(np.uint32), tric[v].view(np.uint32)), v
+
+
+def test_driver_learned_fallback_equals_tricubic(am, orc, torch_cuda, setup):
+    """The RK4 driver with the learned prolongation and a threshold every
+    face exceeds runs bit for bit like the tricubic driver over a coarse step
+    (its statistics count every learned face as a fallback)."""
+    torch = torch_cuda
+    ops, n = WL.puncture_ops(OUTER)
+    g = am.geometry(S, P, ORDER, OUTER, CFL)
+    sims = [am.BssnSim(g, n, am.bssn_options(periodic=False, interp=i))
+            for i in (am.AM_INTERP_TRICUBIC, am.AM_INTERP_LEARNED)]
+    st = WL.puncture_fields(setup["e"]["positions"], S, P, OUTER)
+    prm = torch.tensor(setup["params"], device="cuda")
+    sims[1].set_prolongation(prm, -1.0)
+    for s in sims:
+        for p_, c in ops:
+            s.add_node(p_, c)
+        for v in range(24):
+            s.set_field(0, v, st[v])
+            s.set_field(1, v, st[v])
+        s.fill_ghosts()
+        s.timestep()
+        s.sync()
+    for w in (0, 1, 2, 3):
+        for v in range(24):
+            a, b = sims[0].get_field(w, v), sims[1].get_field(w, v)
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (w, v)
+    stats = sims[1].prolongation_stats()
+    print(f"  driver fallback statistics {stats}", flush=True)
+    assert stats["learned_blocks"] == 0 and stats["fallback_blocks"] > 0
