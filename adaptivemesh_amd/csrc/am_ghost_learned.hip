@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
 {
     constexpr int T = S + 2 * P, NC = S * S * P;
     constexpr size_t CELLS = (size_t)T * T * T;
-    __shared__ float Xs[64][65];
+    __shared__ float Xs[2][64][65];
     __shared__ float Hs[64][129];
     __shared__ float Bn[64][193];
     __shared__ float musg[2][24][64];
@@ -141,55 +141,53 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
             cc[1] = clo[1] + (sa / ncl[0]) % ncl[1];
             cc[2] = clo[2] + sa / (ncl[0] * ncl[1]);
         }
-        for (int v = 0; v < 24; v++) {
+        // 1. per field: the stencil (16 points of quarter z = q per thread),
+        // normalised in registers -- mu is stencil point 21 (quarter 1,
+        // point 5), sigma the largest deviation over the sample's 4 lanes,
+        // which are adjacent in one wave -- then into the double-buffered
+        // LDS tile; the next field's points are loaded while this field's
+        // baseline and layer-1 products run
+        float raw[16];
+        bool bad = false;
+        auto gather = [&](int v) {
             const float *dold = b.old_[v], *dnew = b.new_[v], *dolder = b.older[v];
-            // 1. gather this thread's 16 stencil points (z = q) at the stage time
-            bool bad = false;
-            if (s < ns) {
 #pragma unroll
-                for (int i = 0; i < 16; i++) {
-                    const int px = cc[0] - 1 + (i & 3), py = cc[1] - 1 + (i >> 2), pz = cc[2] - 1 + q;
-                    const int out = (px < P || px >= P + S) + (py < P || py >= P + S) + (pz < P || pz >= P + S);
-                    const bool inside = px >= 0 && px < T && py >= 0 && py < T && pz >= 0 && pz < T && out <= 1;
-                    float val = 0.f;
-                    if (inside) {
-                        const size_t r = tbase + ((size_t)pz * T + py) * T + px;
-                        val = dold[r] * tc1 + dnew[r] * tc2;
-                        if (quad) val = dolder[r] * tc0 + val;
-                    } else {
-                        bad = true;
-                    }
-                    Xs[s][q * 16 + i] = val;
+            for (int i = 0; i < 16; i++) {
+                const int px = cc[0] - 1 + (i & 3), py = cc[1] - 1 + (i >> 2), pz = cc[2] - 1 + q;
+                const int out = (px < P || px >= P + S) + (py < P || py >= P + S) + (pz < P || pz >= P + S);
+                const bool inside = s < ns && px >= 0 && px < T && py >= 0 && py < T && pz >= 0 && pz < T && out <= 1;
+                float val = 0.f;
+                if (inside) {
+                    const size_t r = tbase + ((size_t)pz * T + py) * T + px;
+                    val = dold[r] * tc1 + dnew[r] * tc2;
+                    if (quad) val = dolder[r] * tc0 + val;
+                } else if (s < ns) {
+                    bad = true;
                 }
-            } else {
+                raw[i] = val;
+            }
+        };
+        gather(0);
+        for (int v = 0; v < 24; v++) {
+            const int buf = v & 1;
+            const float mu = __shfl(raw[5], (lane & ~3) | 1, 64);
+            float dev = 0.f;
 #pragma unroll
-                for (int i = 0; i < 16; i++) Xs[s][q * 16 + i] = 0.f;
+            for (int i = 0; i < 16; i++) dev = fmaxf(dev, fabsf(raw[i] - mu));
+            dev = fmaxf(dev, __shfl_xor(dev, 1, 64));
+            dev = fmaxf(dev, __shfl_xor(dev, 2, 64));
+            const float sg = dev + 1e-3f * fabsf(mu) + 1e-7f;
+            const float inv = 1.f / sg;
+#pragma unroll
+            for (int i = 0; i < 16; i++) Xs[buf][s][q * 16 + i] = s < ns ? (raw[i] - mu) * inv : 0.f;
+            if (s < ns && q == 0) {
+                musg[0][v][s] = mu;
+                musg[1][v][s] = sg;
             }
             if (bad) valid[s] = 0;
+            // Xs[buf] is complete; every wave has finished reading Xs[buf ^ 1]
             __syncthreads();
-            // normalise: centre (1,1,1) is stencil index 21, spread over the 4 quarters
-            if (s < ns) {
-                const float mu = Xs[s][21];
-                float dev = 0.f;
-#pragma unroll
-                for (int i = 0; i < 16; i++) dev = fmaxf(dev, fabsf(Xs[s][q * 16 + i] - mu));
-                dev = fmaxf(dev, __shfl_xor(dev, 1, 64));
-                dev = fmaxf(dev, __shfl_xor(dev, 2, 64));
-                const float sg = dev + 1e-3f * fabsf(mu) + 1e-7f;
-                const float inv = 1.f / sg;
-                float xn[16];
-#pragma unroll
-                for (int i = 0; i < 16; i++) xn[i] = (Xs[s][q * 16 + i] - mu) * inv;
-                // all lanes of the sample have read mu before anyone overwrites index 21
-                __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                for (int i = 0; i < 16; i++) Xs[s][q * 16 + i] = xn[i];
-                if (q == 0) {
-                    musg[0][v][s] = mu;
-                    musg[1][v][s] = sg;
-                }
-            }
-            __syncthreads();
+            if (v + 1 < 24) gather(v + 1);
             // baseline of the 8 children (normalised units): 2 children per thread
             if (s < ns) {
 #pragma unroll
@@ -203,7 +201,7 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
                         for (int j = 0; j < 4; j++)
 #pragma unroll
                             for (int i = 0; i < 4; i++)
-                                bsum += LQ[jz][k] * LQ[jy][j] * LQ[jx][i] * Xs[s][k * 16 + j * 4 + i];
+                                bsum += LQ[jz][k] * LQ[jy][j] * LQ[jx][i] * Xs[buf][s][k * 16 + j * 4 + i];
                     Bn[s][v * 8 + ch] = bsum;
                 }
             }
@@ -215,13 +213,13 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
                 const float b0 = wrow[0], b1 = wrow[16];
 #pragma unroll
                 for (int rb = 0; rb < 4; rb++) {
-                    const float a = Xs[rb * 16 + (lane & 15)][ka];
+                    const float a = Xs[buf][rb * 16 + (lane & 15)][ka];
                     acc[rb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[rb][0], 0, 0, 0);
                     acc[rb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[rb][1], 0, 0, 0);
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();
         // layer 1 epilogue
 #pragma unroll
         for (int rb = 0; rb < 4; rb++)
