@@ -20,6 +20,7 @@ LIB_PATH = HERE / "libadaptivemesh_amd.so"
 
 AM_MAX_VARS = 32
 AM_INTERP_LAGRANGE, AM_INTERP_TRILINEAR, AM_INTERP_TRICUBIC, AM_INTERP_LEARNED = 0, 1, 2, 3
+AM_WORD_CF3, AM_WORD_F32, AM_WORD_F16 = 0, 1, 2
 AM_EQUAL_FORMULA, AM_EQUAL_COPY, AM_EQUAL_SKIP = 0, 1, 2
 
 ERRORS = {
@@ -64,7 +65,8 @@ class FdBatch(C.Structure):
     _fields_ = [("field", C.c_void_p * AM_MAX_VARS), ("d1", C.c_void_p * AM_MAX_VARS),
                 ("d2diag", C.c_void_p * AM_MAX_VARS), ("d2off", C.c_void_p * AM_MAX_VARS),
                 ("seed_d", C.c_int32 * AM_MAX_VARS), ("seed_m", C.c_int32 * AM_MAX_VARS),
-                ("n_vars", C.c_int32), ("z_block", C.c_int32), ("seed_table", C.c_void_p)]
+                ("n_vars", C.c_int32), ("z_block", C.c_int32), ("seed_table", C.c_void_p),
+                ("word_format", C.c_int32)]
 
 
 class FieldBatch(C.Structure):
@@ -80,12 +82,12 @@ class BssnParams(C.Structure):
 class BssnFields(C.Structure):
     _fields_ = [("state", C.c_void_p * 24), ("old_", C.c_void_p * 24), ("new_", C.c_void_p * 24),
                 ("acc", C.c_void_p * 24), ("d1", C.c_void_p * 24), ("d2diag", C.c_void_p * 11),
-                ("d2off", C.c_void_p * 11)]
+                ("d2off", C.c_void_p * 11), ("word_format", C.c_int32)]
 
 
 class BssnSimOptions(C.Structure):
     _fields_ = [("params", BssnParams), ("interp", C.c_int32), ("z_block", C.c_int32), ("periodic", C.c_int32),
-                ("seed", C.c_int32)]
+                ("seed", C.c_int32), ("word_format", C.c_int32)]
 
 
 class _Trial(C.Structure):
@@ -131,6 +133,7 @@ _SIGS = {
     "am_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "am_stream_destroy": (C.c_int, [_P]),
     "am_device_cu_count": (C.c_int32, []),
+    "am_word_bytes": (C.c_int32, [C.c_int32]),
     "am_graph_begin": (C.c_int, [_P]),
     "am_graph_end": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "am_graph_launch": (C.c_int, [_P, _P]),
@@ -387,11 +390,13 @@ class WaveSim:
 
 
 def bssn_options(eta=2.0, shift_coeff=0.75, chi_floor=1e-4, advect_shift=1, slicing=0,
-                 interp=AM_INTERP_LAGRANGE, z_block=0, periodic=False, seed=12894712) -> BssnSimOptions:
+                 interp=AM_INTERP_LAGRANGE, z_block=0, periodic=False, seed=12894712,
+                 word_format=AM_WORD_CF3) -> BssnSimOptions:
     o = BssnSimOptions()
     o.params.eta, o.params.shift_coeff, o.params.chi_floor = eta, shift_coeff, chi_floor
     o.params.advect_shift, o.params.slicing = advect_shift, slicing
     o.interp, o.z_block, o.periodic, o.seed = interp, z_block, 1 if periodic else 0, seed
+    o.word_format = word_format
     return o
 
 

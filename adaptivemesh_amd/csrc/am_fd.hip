@@ -14,6 +14,7 @@
 // with contraction disabled, so each word is bit-identical to the
 // reference's on an IEEE host.
 #include "am_device.hpp"
+#include "am_words.hpp"
 
 #pragma clang fp contract(off)
 
@@ -48,7 +49,7 @@ __device__ __forceinline__ float mixed_bd(float c, float pp, float mm, float m0,
     return c + .5f * (pp + mm - m0 - p0 - zm - zp);
 }
 
-template <int S, int P, int Q, int ZB, bool PERSIST>
+template <int S, int P, int Q, int ZB, bool PERSIST, int FMT>
 __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScales sc)
 {
     constexpr int T = S + 2 * P, T2 = T * T, R = 2 * P + 2, NZB = S / ZB;
@@ -138,8 +139,8 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
             }
             vx = vx * dx1; vy = vy * dx1; vz = vz * dx1;
             int32_t rng = thread_seed(gs_d, (uint32_t)x, (uint32_t)y, gz);
-            if (o_d2) o_d2[o] = cf3_pack(xx * dx2, yy * dx2, zz * dx2, rng);
-            o_d1[o] = cf3_pack(vx, vy, vz, rng);
+            if (o_d2) word_put<FMT>(o_d2, o, xx * dx2, yy * dx2, zz * dx2, rng);
+            word_put<FMT>(o_d1, o, vx, vy, vz, rng);
         }
         if (want_m) {
             const int ez = z - min(max(z, 1), S - 2);
@@ -166,7 +167,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
                 vyz = (float)(-cyz) * mixed_bd(ctr[c], zp[c + dy], zm[c - dy], ctr[c - dy], ctr[c + dy], zm[c], zp[c]);
             }
             int32_t rng = thread_seed(gs_m, (uint32_t)x, (uint32_t)y, gz);
-            o_off[o] = cf3_pack(vxy * dx2, vxz * dx2, vyz * dx2, rng);
+            word_put<FMT>(o_off, o, vxy * dx2, vxz * dx2, vyz * dx2, rng);
         }
         if (z + 1 < z0 + ZB) {
             lstore(z + 2 * P + 1);
@@ -177,14 +178,33 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     }  // work items
 }
 
-template <int S, int P, int Q, int ZB>
-static void fd_launch(bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
-                      const DepthScales &sc)
+template <int S, int P, int Q, int ZB, int FMT>
+static void fd_launch_fmt(bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
+                          const DepthScales &sc)
 {
     if (persist)
-        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true>), grid, dim3(S * S), 0, st, d, m, sc);
+        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, FMT>), grid, dim3(S * S), 0, st, d, m, sc);
     else
-        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false>), grid, dim3(S * S), 0, st, d, m, sc);
+        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false, FMT>), grid, dim3(S * S), 0, st, d, m, sc);
+}
+
+// the fp32 / fp16 word formats (precision study) exist for the S = 16, P = 3,
+// order-4 tiles only
+template <int S, int P, int Q>
+constexpr bool has_word_formats()
+{
+    return S == 16 && P == 3 && Q == 2;
+}
+
+template <int S, int P, int Q, int ZB>
+static void fd_launch(int fmt, bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
+                      const DepthScales &sc)
+{
+    if constexpr (has_word_formats<S, P, Q>()) {
+        if (fmt == WORD_F32) return fd_launch_fmt<S, P, Q, ZB, WORD_F32>(persist, grid, st, d, m, sc);
+        if (fmt == WORD_F16) return fd_launch_fmt<S, P, Q, ZB, WORD_F16>(persist, grid, st, d, m, sc);
+    }
+    fd_launch_fmt<S, P, Q, ZB, WORD_CF3>(persist, grid, st, d, m, sc);
 }
 
 template <int S, int P, int Q>
@@ -223,12 +243,15 @@ static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometr
         return lim < n ? (lim > 0 ? lim : 1) : n;
     };
     const bool persist = mesh->grid_limit > 0;
+    const int fmt = b->word_format;
+    if (fmt != WORD_CF3 && !(has_word_formats<S, P, Q>() && (fmt == WORD_F32 || fmt == WORD_F16)))
+        return set_error(AM_ERR_INVALID, "am_fd: word format %d needs S=16, P=3, fd_order 4", fmt);
     if (zb == S) {
-        fd_launch<S, P, Q, S>(persist, dim3(gx(1), b->n_vars), st, d, items(1), sc);
+        fd_launch<S, P, Q, S>(fmt, persist, dim3(gx(1), b->n_vars), st, d, items(1), sc);
     } else if (zb == S / 2) {
-        fd_launch<S, P, Q, S / 2>(persist, dim3(gx(2), b->n_vars), st, d, items(2), sc);
+        fd_launch<S, P, Q, S / 2>(fmt, persist, dim3(gx(2), b->n_vars), st, d, items(2), sc);
     } else if (zb == S / 4 && (S % 4) == 0) {
-        fd_launch<S, P, Q, S / 4>(persist, dim3(gx(4), b->n_vars), st, d, items(4), sc);
+        fd_launch<S, P, Q, S / 4>(fmt, persist, dim3(gx(4), b->n_vars), st, d, items(4), sc);
     } else {
         return set_error(AM_ERR_INVALID, "am_fd: z_block %d must be S, S/2 or S/4", zb);
     }

@@ -78,7 +78,7 @@ namespace {
 
 template <class H, class Data, class SysOpt>
 H *rk4_create(const char *name, const am_geometry *g, uint32_t max_slots, int32_t interp, int32_t z_block,
-              int32_t periodic, int32_t seed, const SysOpt &so)
+              int32_t periodic, int32_t seed, int32_t word_format, const SysOpt &so)
 {
     if (!g || !am_geometry_supported(g)) {
         am::set_error(AM_ERR_UNSUPPORTED_GEOMETRY, "%s: unsupported geometry", name);
@@ -88,6 +88,7 @@ H *rk4_create(const char *name, const am_geometry *g, uint32_t max_slots, int32_
         am::rk4_options opt;
         opt.interp = interp;
         opt.z_block = z_block;
+        opt.word_format = word_format;
         auto *s = new H;
         s->amr = new AMR<Data>(max_slots, periodic != 0, *g, opt, so);
         s->amr->sim_data.rng.seed = seed;
@@ -468,7 +469,7 @@ am_bssn_sim *am_bssn_sim_create(const am_geometry *g, uint32_t max_slots, const 
         return nullptr;
     }
     return rk4_create<am_bssn_sim, BSSN_rk4_data>("am_bssn_sim_create", g, max_slots, o->interp, o->z_block,
-                                                  o->periodic, o->seed, o->params);
+                                                  o->periodic, o->seed, o->word_format, o->params);
 }
 int am_bssn_sim_constraints(am_bssn_sim *s, double *out4)
 {
@@ -483,7 +484,7 @@ am_swave_sim *am_swave_sim_create(const am_geometry *g, uint32_t max_slots, cons
         return nullptr;
     }
     return rk4_create<am_swave_sim, swave_rk4_data>("am_swave_sim_create", g, max_slots, o->interp, o->z_block,
-                                                    o->periodic, o->seed, swave_options{});
+                                                    o->periodic, o->seed, AM_WORD_CF3, swave_options{});
 }
 int am_swave_sim_energy(am_swave_sim *s, double *out4)
 {

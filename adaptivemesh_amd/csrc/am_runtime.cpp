@@ -163,6 +163,16 @@ int am_graph_destroy(void *exec)
 
 int am_device_sync(void) { return am::hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
 
+int32_t am_word_bytes(int32_t format)
+{
+    switch (format) {
+    case AM_WORD_CF3: return 4;
+    case AM_WORD_F32: return 12;
+    case AM_WORD_F16: return 6;
+    default: return -1;
+    }
+}
+
 int32_t am_device_cu_count(void)
 {
     int dev = 0, n = 0;

@@ -107,14 +107,15 @@ def flat_noise_fields(n_slots: int, S: int, P: int, amplitude: float = 1e-6, see
 
 
 def gauge_wave_fields(positions, S: int, P: int, outer: float, amplitude: float = 0.1, t: float = 0.0,
-                      axis: int = 0):
+                      axis: int = 0, dtype=np.float32):
     """The harmonic gauge wave (Alcubierre et al. 2004 testbed) as BSSN data:
     ds^2 = -H dt^2 + H dw^2 + (the other two axes)^2 with
     H = 1 - A sin(2 pi (w - t) / outer), w the coordinate along `axis`, so one
     wavelength spans the domain.  alpha = sqrt(H), beta = B = 0,
     chi = H^(-1/3), gt_ww = H^(2/3), the other diagonal gt = H^(-1/3),
     K = K_ww / H with K_ww = -d_t H / (2 sqrt H), At_ww = (2/3) chi K_ww, the
-    other diagonal At = -chi K / 3, Gt^w = (2/3) H^(-5/3) d_w H."""
+    other diagonal At = -chi K / 3, Gt^w = (2/3) H^(-5/3) d_w H.  Computed in
+    float64, returned as `dtype`."""
     n = len(positions)
     T = S + 2 * P
     out = [np.full((n, T, T, T), FLAT[v], np.float64) for v in range(24)]
@@ -140,4 +141,4 @@ def gauge_wave_fields(positions, S: int, P: int, outer: float, amplitude: float 
         out[7][s] = K
         out[14 + axis][s] = (2 / 3) * H ** (-5 / 3) * dwH
         out[17][s] = np.sqrt(H)
-    return [a.reshape(-1).astype(np.float32) for a in out]
+    return [a.reshape(-1).astype(dtype) for a in out]

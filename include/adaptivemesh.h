@@ -173,7 +173,16 @@ typedef struct am_fd_batch {
      * are indices into this device array, which holds the seeds: a launch
      * recorded once in a graph then reads fresh seeds on every replay. */
     const int32_t *seed_table;
+    /* output format, enum am_word_format: the reference's dithered
+     * compressed_float3 (4 bytes per cell), or for the precision study three
+     * fp32 (12 bytes) or three fp16 (6 bytes) per cell (S=16, P=3, order 4
+     * tiles only); d1/d2diag/d2off then point to arrays of that many bytes
+     * per cell */
+    int32_t word_format;
 } am_fd_batch;
+enum am_word_format { AM_WORD_CF3 = 0, AM_WORD_F32 = 1, AM_WORD_F16 = 2 };
+/* bytes per cell of one word array in `format` (4, 12, 6), -1 if unknown */
+int32_t am_word_bytes(int32_t format);
 int am_fd_batched(const am_fd_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream);
 
 /* ------------------------------------------------- ghost cells and tree */
@@ -285,6 +294,7 @@ typedef struct am_bssn_fields {
     const uint32_t *d1[AM_BSSN_NVARS];
     const uint32_t *d2diag[AM_BSSN_NHESS];
     const uint32_t *d2off[AM_BSSN_NHESS];
+    int32_t word_format; /* enum am_word_format of d1/d2diag/d2off (non-cf3: S=16, P=3 only) */
 } am_bssn_fields;
 
 /* One classical RK4 stage on the interior of every listed active tile:

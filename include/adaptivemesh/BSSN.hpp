@@ -119,11 +119,13 @@ struct bssn_system {
     // chi, gt_ij, alpha, beta^i need full Hessians; everything else a gradient
     static constexpr int FD_KIND[NV] = {3, 3, 3, 3, 3, 3, 3, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3, 3, 3, 1, 1, 1};
     static constexpr bool CHRISTOFFEL = false;  // raised in registers by the RHS kernel
+    static constexpr bool WORD_FORMATS = true;  // fp32 / fp16 words for S=16, P=3 (precision study)
     using options_t = am_bssn_params;
 
     static am_bssn_fields fields(const rk4_views &v)
     {
         am_bssn_fields f{};
+        f.word_format = v.word_format;
         for (int i = 0; i < NV; i++) {
             f.state[i] = v.state[i];
             f.old_[i] = v.old_[i];

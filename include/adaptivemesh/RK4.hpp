@@ -94,6 +94,7 @@ struct rk4_options {
     int32_t z_block = 0;                  // FD planes per workgroup (0 = whole tile)
     int32_t fd_grid_limit = 0;            // persistent FD launch: workgroup cap (0 = one per item)
     int32_t rhs_grid_limit = 0;           // persistent RHS launch: workgroup cap (0 = one per item)
+    int32_t word_format = AM_WORD_CF3;    // derivative-word storage (am_word_format)
 };
 
 // What autotune() measured: each trial's launch shape and mean ms per launch.
@@ -116,6 +117,7 @@ struct rk4_views {
     const uint32_t *d2diag[AM_MAX_VARS];  // by Hessian index (fields with FD_KIND >= 2, in order)
     const uint32_t *d2off[AM_MAX_VARS];   // by full-Hessian index (FD_KIND == 3, in order)
     const uint32_t *chr[6];
+    int32_t word_format;  // am_word_format of d1/d2diag/d2off
 };
 
 template <class System>
@@ -130,6 +132,7 @@ struct rk4_AMR_data {
     options_t sys;
     void *stream = nullptr;
     size_t cells = 0, inner = 0, slots = 0;
+    size_t word_stride = 0;  // bytes of one field's words: slots * inner * am_word_bytes(word_format)
     int n_d2 = 0, n_off = 0;
 
     // one allocation per kind; field v at base + v * slots * cells (padded
@@ -167,6 +170,12 @@ struct rk4_AMR_data {
             n_d2 += System::FD_KIND[v] >= 2;
             n_off += System::FD_KIND[v] == 3;
         }
+        if (opt.word_format != AM_WORD_CF3 &&
+            (!System::WORD_FORMATS || g.size_domain != 16 || g.padding_domain != 3 || g.fd_order != 4 ||
+             am_word_bytes(opt.word_format) <= 0))
+            throw std::runtime_error("word format " + std::to_string(opt.word_format) +
+                                     " needs the BSSN system on S=16, P=3, fd_order 4 tiles");
+        word_stride = slots * inner * (size_t)am_word_bytes(opt.word_format);
         old_ = smart_gpu_buffer<float>(NV * slots * cells);
         new_ = smart_gpu_buffer<float>(NV * slots * cells);
         acc = smart_gpu_buffer<float>(NV * slots * inner);
@@ -174,9 +183,9 @@ struct rk4_AMR_data {
             older = smart_gpu_buffer<float>(NV * slots * cells);
         mlp_stats = smart_gpu_buffer<uint64_t>(4);
         mlp_stats.zero();
-        d1 = smart_gpu_buffer<uint32_t>(NV * slots * inner);
-        if (n_d2) d2d = smart_gpu_buffer<uint32_t>(n_d2 * slots * inner);
-        if (n_off) d2o = smart_gpu_buffer<uint32_t>(n_off * slots * inner);
+        d1 = smart_gpu_buffer<uint32_t>(NV * word_stride / 4);
+        if (n_d2) d2d = smart_gpu_buffer<uint32_t>(n_d2 * word_stride / 4);
+        if (n_off) d2o = smart_gpu_buffer<uint32_t>(n_off * word_stride / 4);
         if (System::CHRISTOFFEL) chr = smart_gpu_buffer<uint32_t>(6 * slots * inner);
         partial = smart_gpu_buffer<double>(4 * (size_t)am_bssn_constraint_blocks(&geom, (int32_t)slots) + 4);
         seed_table = smart_gpu_buffer<int32_t>(SEED_CAPACITY);
@@ -205,7 +214,12 @@ struct rk4_AMR_data {
     {
         return b.gpu_buffer_ptr + (size_t)v * slots * (&b == &acc ? inner : cells);
     }
-    uint32_t *word(smart_gpu_buffer<uint32_t> &b, int v) { return b.gpu_buffer_ptr + (size_t)v * slots * inner; }
+    uint32_t *word(smart_gpu_buffer<uint32_t> &b, int v)
+    {
+        // Christoffel words are always compressed_float3
+        const size_t stride = &b == &chr ? slots * inner * 4 : word_stride;
+        return reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(b.gpu_buffer_ptr) + (size_t)v * stride);
+    }
 
     am_field_batch batch(bool with_older)
     {
@@ -221,6 +235,7 @@ struct rk4_AMR_data {
     rk4_views views(bool stage_input_is_old)
     {
         rk4_views f{};
+        f.word_format = opt.word_format;
         for (int v = 0; v < NV; v++) {
             f.state[v] = stage_input_is_old ? field(old_, v) : field(new_, v);
             f.old_[v] = field(old_, v);
@@ -261,6 +276,7 @@ struct rk4_AMR_data {
     {
         am_fd_batch b{};
         b.z_block = opt.z_block;
+        b.word_format = opt.word_format;
         b.seed_table = recording ? seed_table.gpu_buffer_ptr : nullptr;
         m.grid_limit = opt.fd_grid_limit;
         int h = 0, o = 0;

@@ -21,6 +21,7 @@ struct swave_system {
     static constexpr int NV = 2;
     static constexpr int FD_KIND[NV] = {2, 0};
     static constexpr bool CHRISTOFFEL = false;
+    static constexpr bool WORD_FORMATS = false;  // compressed_float3 words only
     using options_t = swave_options;
 
     static am_swave_fields fields(const rk4_views &v)

@@ -87,6 +87,10 @@ typedef struct am_bssn_sim_options {
     int32_t z_block;  /* FD workgroup z extent (0 = whole tile) */
     int32_t periodic; /* outer faces wrap instead of pointing at the root */
     int32_t seed;     /* host fast_prng seed for the dither draws */
+    /* enum am_word_format of the derivative words: AM_WORD_CF3 (the
+     * reference's format), or AM_WORD_F32 / AM_WORD_F16 (precision study,
+     * S=16, P=3, fd_order 4 only) */
+    int32_t word_format;
 } am_bssn_sim_options;
 
 am_bssn_sim *am_bssn_sim_create(const am_geometry *g, uint32_t max_slots, const am_bssn_sim_options *o);

@@ -419,9 +419,23 @@ static const float CFD2[15] = {1.f, -2.f, 1.f, -1.f / 12, 4.f / 3, -5.f / 2, 4.f
 /* __differentiate, both overloads (AMR.h:1152-1219).  d2 == NULL selects the
  * first-derivative-only kernel.  Output words are interior-only,
  * slot*S^3 + (z*S+y)*S+x. */
-EXPORT void orc_differentiate(const float *field, uint32_t *d1, uint32_t *d2, const uint32_t *pos,
-                              const int32_t *tiles, int n_tiles, uint32_t substep, uint32_t maxd,
-                              int32_t gs, int S, int P, int q, float outer_dx)
+/* One derivative triple into a word array: fmt 0 the dithered
+ * compressed_float3 word, fmt 1 the three fp32 values themselves (3 floats
+ * per cell, the precision study's fp32 words; its fp16 words are these
+ * values rounded to nearest even). */
+static inline void store3(void *out, int64_t o, int fmt, float x, float y, float z, int32_t *rng)
+{
+    if (fmt == 0) {
+        ((uint32_t *)out)[o] = cf3_pack(x, y, z, rng);
+    } else {
+        float *p = (float *)out + 3 * o;
+        p[0] = x; p[1] = y; p[2] = z;
+    }
+}
+
+static void differentiate(const float *field, void *d1, void *d2, const uint32_t *pos, const int32_t *tiles,
+                          int n_tiles, uint32_t substep, uint32_t maxd, int32_t gs, int S, int P, int q,
+                          float outer_dx, int fmt)
 {
     const geom g = mkgeom(S, P, q, outer_dx, 0.f);
     const int T = g.T, base = q * q + q - 1;
@@ -452,11 +466,25 @@ EXPORT void orc_differentiate(const float *field, uint32_t *d1, uint32_t *d2, co
                     const int64_t o = (((int64_t)slot * S + z) * S + y) * S + x;
                     if (d2) {
                         xx *= dx1 * dx1; yy *= dx1 * dx1; zz *= dx1 * dx1;
-                        d2[o] = cf3_pack(xx, yy, zz, &rng);
+                        store3(d2, o, fmt, xx, yy, zz, &rng);
                     }
-                    d1[o] = cf3_pack(vx, vy, vz, &rng);
+                    store3(d1, o, fmt, vx, vy, vz, &rng);
                 }
         }
+}
+
+EXPORT void orc_differentiate(const float *field, uint32_t *d1, uint32_t *d2, const uint32_t *pos,
+                              const int32_t *tiles, int n_tiles, uint32_t substep, uint32_t maxd,
+                              int32_t gs, int S, int P, int q, float outer_dx)
+{
+    differentiate(field, d1, d2, pos, tiles, n_tiles, substep, maxd, gs, S, P, q, outer_dx, 0);
+}
+/* the same derivatives as fp32 triples (d1, d2: 3 floats per cell) */
+EXPORT void orc_differentiate_f32(const float *field, float *d1, float *d2, const uint32_t *pos,
+                                  const int32_t *tiles, int n_tiles, uint32_t substep, uint32_t maxd, int S, int P,
+                                  int q, float outer_dx)
+{
+    differentiate(field, d1, d2, pos, tiles, n_tiles, substep, maxd, 0, S, P, q, outer_dx, 1);
 }
 
 /* __mixed_partial_c / _bd (AMR.h:1221-1230) */
@@ -471,9 +499,8 @@ static inline float mixed_bd(const float *f, int64_t i, int64_t dx, int64_t dy)
 static inline int iclamp(int v, int a, int b) { return v < a ? a : (v > b ? b : v); }
 
 /* __mixed_partials (AMR.h:1232-1266) */
-EXPORT void orc_mixed_partials(const float *field, uint32_t *off, const uint32_t *pos,
-                               const int32_t *tiles, int n_tiles, uint32_t substep, uint32_t maxd,
-                               int32_t gs, int S, int P, float outer_dx)
+static void mixed_partials(const float *field, void *off, const uint32_t *pos, const int32_t *tiles, int n_tiles,
+                           uint32_t substep, uint32_t maxd, int32_t gs, int S, int P, float outer_dx, int fmt)
 {
     const int T = S + 2 * P;
 #pragma omp parallel for collapse(2) schedule(static)
@@ -500,9 +527,21 @@ EXPORT void orc_mixed_partials(const float *field, uint32_t *off, const uint32_t
                     const float vyz = cyz == 0 ? mixed_c(field, rw, T, T2)
                                                : ((float)(-cyz) * mixed_bd(field, rw, (int64_t)(-cyz) * T, T2));
                     const int64_t o = (((int64_t)slot * S + z) * S + y) * S + x;
-                    off[o] = cf3_pack(vxy * dx1, vxz * dx1, vyz * dx1, &rng);
+                    store3(off, o, fmt, vxy * dx1, vxz * dx1, vyz * dx1, &rng);
                 }
         }
+}
+
+EXPORT void orc_mixed_partials(const float *field, uint32_t *off, const uint32_t *pos,
+                               const int32_t *tiles, int n_tiles, uint32_t substep, uint32_t maxd,
+                               int32_t gs, int S, int P, float outer_dx)
+{
+    mixed_partials(field, off, pos, tiles, n_tiles, substep, maxd, gs, S, P, outer_dx, 0);
+}
+EXPORT void orc_mixed_partials_f32(const float *field, float *off, const uint32_t *pos, const int32_t *tiles,
+                                   int n_tiles, uint32_t substep, uint32_t maxd, int S, int P, float outer_dx)
+{
+    mixed_partials(field, off, pos, tiles, n_tiles, substep, maxd, 0, S, P, outer_dx, 1);
 }
 
 /* ------------------------------------------------ interpolation kernels */

@@ -311,6 +311,31 @@ static inline void cf3_dec(uint32_t w, double *o)
     o[2] = base * (double)((w >> 18) & 255u) * ((w >> 26) & 1u ? -1.0 : 1.0);
 }
 
+/* IEEE 754 binary16 bits to double (exact: normals, subnormals, zeros;
+ * the words never hold infinities or NaNs) */
+static inline double half_to_double(uint16_t h)
+{
+    const int e = (h >> 10) & 31, m = h & 1023;
+    const double v = e == 0 ? ldexp((double)m, -24) : ldexp((double)(1024 + m), e - 25);
+    return (h & 0x8000u) ? -v : v;
+}
+
+/* One derivative triple of cell w from a word array in format fmt: 0 the
+ * dithered compressed_float3 words, 1 three fp32, 2 three fp16 per cell
+ * (the precision study's formats). */
+static inline void word_dec(const void *base, int64_t w, int fmt, double *o)
+{
+    if (fmt == 0) {
+        cf3_dec(((const uint32_t *)base)[w], o);
+    } else if (fmt == 1) {
+        const float *p = (const float *)base + 3 * w;
+        o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
+    } else {
+        const uint16_t *p = (const uint16_t *)base + 3 * w;
+        o[0] = half_to_double(p[0]); o[1] = half_to_double(p[1]); o[2] = half_to_double(p[2]);
+    }
+}
+
 /* RK4 stage combination shared by every stage kernel here:
  *   stage 0: acc = k;      new = y_n + dt/2 k
  *   stage 1: acc += 2k;    new = y_n + dt/2 k
@@ -333,12 +358,11 @@ static inline void rk4_combine(int stage, double dt, double yn, double k, double
  * dtd[d] is the step of depth d.  chw == NULL: the raised Christoffel
  * symbols are computed from the metric and its gradient words (as the GPU
  * RHS does) instead of read from Christoffel words. */
-EXPORT void orc_bssn_stage_from_words(const float *const *state, const float *const *old_, const double *const *acc_in,
-                                      double *const *acc_out, double *const *new_out, const uint32_t *const *d1w,
-                                      const uint32_t *const *d2dw, const uint32_t *const *d2ow,
-                                      const uint32_t *const *chw, const uint32_t *pos, const int32_t *tiles,
-                                      int n_tiles, uint32_t substep, uint32_t maxd, int stage, const double *dtd,
-                                      const double *params, int S, int P)
+static void stage_from_words(const float *const *state, const float *const *old_, const double *const *acc_in,
+                             double *const *acc_out, double *const *new_out, const void *const *d1w,
+                             const void *const *d2dw, const void *const *d2ow, const uint32_t *const *chw,
+                             const uint32_t *pos, const int32_t *tiles, int n_tiles, uint32_t substep, uint32_t maxd,
+                             int stage, const double *dtd, const double *params, int S, int P, int fmt)
 {
     const orc_bssn_par par = par_from(params);
     const int T = S + 2 * P;
@@ -357,12 +381,12 @@ EXPORT void orc_bssn_stage_from_words(const float *const *state, const float *co
                     double u[BSSN_NV], d1[BSSN_NV][3], dd[BSSN_NH][6], G[6][3], k[BSSN_NV];
                     for (int v = 0; v < BSSN_NV; v++) {
                         u[v] = state[v][r];
-                        cf3_dec(d1w[v][w], d1[v]);
+                        word_dec(d1w[v], w, fmt, d1[v]);
                     }
                     for (int h = 0; h < BSSN_NH; h++) {
                         double a[3], b[3];
-                        cf3_dec(d2dw[h][w], a);
-                        cf3_dec(d2ow[h][w], b);
+                        word_dec(d2dw[h], w, fmt, a);
+                        word_dec(d2ow[h], w, fmt, b);
                         dd[h][0] = a[0]; dd[h][1] = b[0]; dd[h][2] = b[1];
                         dd[h][3] = a[1]; dd[h][4] = b[2]; dd[h][5] = a[2];
                     }
@@ -378,12 +402,33 @@ EXPORT void orc_bssn_stage_from_words(const float *const *state, const float *co
         }
 }
 
+EXPORT void orc_bssn_stage_from_words(const float *const *state, const float *const *old_, const double *const *acc_in,
+                                      double *const *acc_out, double *const *new_out, const uint32_t *const *d1w,
+                                      const uint32_t *const *d2dw, const uint32_t *const *d2ow,
+                                      const uint32_t *const *chw, const uint32_t *pos, const int32_t *tiles,
+                                      int n_tiles, uint32_t substep, uint32_t maxd, int stage, const double *dtd,
+                                      const double *params, int S, int P)
+{
+    stage_from_words(state, old_, acc_in, acc_out, new_out, (const void *const *)d1w, (const void *const *)d2dw,
+                     (const void *const *)d2ow, chw, pos, tiles, n_tiles, substep, maxd, stage, dtd, params, S, P, 0);
+}
+/* the same with the words in format fmt (0 cf3, 1 fp32, 2 fp16 triples) */
+EXPORT void orc_bssn_stage_from_words_fmt(const float *const *state, const float *const *old_,
+                                          const double *const *acc_in, double *const *acc_out,
+                                          double *const *new_out, const void *const *d1w, const void *const *d2dw,
+                                          const void *const *d2ow, const uint32_t *pos, const int32_t *tiles,
+                                          int n_tiles, uint32_t substep, uint32_t maxd, int stage, const double *dtd,
+                                          const double *params, int S, int P, int fmt)
+{
+    stage_from_words(state, old_, acc_in, acc_out, new_out, d1w, d2dw, d2ow, NULL, pos, tiles, n_tiles, substep,
+                     maxd, stage, dtd, params, S, P, fmt);
+}
+
 /* Constraint sums on the product's inputs: per listed tile,
  * out[4*ti + {0,1,2,3}] = {sum H^2, sum M_i M_i, cells, max |H|}. */
-EXPORT void orc_bssn_constraints_from_words(const float *const *state, const uint32_t *const *d1w,
-                                            const uint32_t *const *d2dw, const uint32_t *const *d2ow,
-                                            const uint32_t *const *chw, const int32_t *tiles, int n_tiles,
-                                            const double *params, int S, int P, double *out)
+static void constraints_from_words(const float *const *state, const void *const *d1w, const void *const *d2dw,
+                                   const void *const *d2ow, const uint32_t *const *chw, const int32_t *tiles,
+                                   int n_tiles, const double *params, int S, int P, double *out, int fmt)
 {
     const orc_bssn_par par = par_from(params);
     const int T = S + 2 * P;
@@ -400,12 +445,12 @@ EXPORT void orc_bssn_constraints_from_words(const float *const *state, const uin
                     double u[BSSN_NV], d1[BSSN_NV][3], dd[BSSN_NH][6], G[6][3], k[BSSN_NV], H, M[3];
                     for (int v = 0; v < BSSN_NV; v++) {
                         u[v] = state[v][r];
-                        cf3_dec(d1w[v][w], d1[v]);
+                        word_dec(d1w[v], w, fmt, d1[v]);
                     }
                     for (int h = 0; h < BSSN_NH; h++) {
                         double a[3], b[3];
-                        cf3_dec(d2dw[h][w], a);
-                        cf3_dec(d2ow[h][w], b);
+                        word_dec(d2dw[h], w, fmt, a);
+                        word_dec(d2ow[h], w, fmt, b);
                         dd[h][0] = a[0]; dd[h][1] = b[0]; dd[h][2] = b[1];
                         dd[h][3] = a[1]; dd[h][4] = b[2]; dd[h][5] = a[2];
                     }
@@ -422,6 +467,22 @@ EXPORT void orc_bssn_constraints_from_words(const float *const *state, const uin
         out[4 * ti + 2] = (double)S * S * S;
         out[4 * ti + 3] = hmax;
     }
+}
+
+EXPORT void orc_bssn_constraints_from_words(const float *const *state, const uint32_t *const *d1w,
+                                            const uint32_t *const *d2dw, const uint32_t *const *d2ow,
+                                            const uint32_t *const *chw, const int32_t *tiles, int n_tiles,
+                                            const double *params, int S, int P, double *out)
+{
+    constraints_from_words(state, (const void *const *)d1w, (const void *const *)d2dw, (const void *const *)d2ow,
+                           chw, tiles, n_tiles, params, S, P, out, 0);
+}
+EXPORT void orc_bssn_constraints_from_words_fmt(const float *const *state, const void *const *d1w,
+                                                const void *const *d2dw, const void *const *d2ow,
+                                                const int32_t *tiles, int n_tiles, const double *params, int S, int P,
+                                                double *out, int fmt)
+{
+    constraints_from_words(state, d1w, d2dw, d2ow, NULL, tiles, n_tiles, params, S, P, out, fmt);
 }
 
 /* ===================================================== FP64 scheme on tiles */

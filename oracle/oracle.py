@@ -42,6 +42,8 @@ _SIGS = {
     "orc_tree_set_timer": (None, [_P, _u32]),
     "orc_differentiate": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, _i32, C.c_int, C.c_int, C.c_int, _f32]),
     "orc_mixed_partials": (None, [_P, _P, _P, _P, C.c_int, _u32, _u32, _i32, C.c_int, C.c_int, _f32]),
+    "orc_differentiate_f32": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, _f32]),
+    "orc_mixed_partials_f32": (None, [_P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, _f32]),
     "orc_copy_boundaries": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, C.c_int,
                                    _f32, C.c_int]),
     "orc_copy_new_to_old": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
@@ -179,6 +181,26 @@ def mixed_partials(field, positions, tiles, substep, max_depth, seed, S, P, oute
     return off
 
 
+def differentiate_f32(field, positions, tiles, substep, max_depth, S, P, q, outer_dx, n_slots, with_d2=True):
+    """differentiate() without the codec: (n_slots * S^3, 3) float32 arrays
+    of the gradient and the Hessian diagonal."""
+    d1 = np.zeros((n_slots * S ** 3, 3), np.float32)
+    d2 = np.zeros((n_slots * S ** 3, 3), np.float32) if with_d2 else None
+    t = _i32arr(tiles)
+    lib().orc_differentiate_f32(_p(field), _p(d1), _p(d2), _p(positions), _p(t), t.size, substep, max_depth, S, P,
+                                q, outer_dx)
+    return d1, d2
+
+
+def mixed_partials_f32(field, positions, tiles, substep, max_depth, S, P, outer_dx, n_slots):
+    """mixed_partials() without the codec: (n_slots * S^3, 3) float32 (xy, xz, yz)."""
+    off = np.zeros((n_slots * S ** 3, 3), np.float32)
+    t = _i32arr(tiles)
+    lib().orc_mixed_partials_f32(_p(field), _p(off), _p(positions), _p(t), t.size, substep, max_depth, S, P,
+                                 outer_dx)
+    return off
+
+
 def copy_boundaries(old, new, older, boundaries, tiles, substep, max_depth, copy_only_new, mode, S, P, frac=1.0,
                     equal_copy=False):
     """In place on old/new (float32 arrays).  equal_copy: equal-depth faces
@@ -230,6 +252,10 @@ _SIGS_BSSN = {
     "orc_bssn_stage_from_words": (None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int,
                                          _P, _P, C.c_int, C.c_int]),
     "orc_bssn_constraints_from_words": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _P, C.c_int, C.c_int, _P]),
+    "orc_bssn_stage_from_words_fmt": (None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int,
+                                             _P, _P, C.c_int, C.c_int, C.c_int]),
+    "orc_bssn_constraints_from_words_fmt": (None, [_P, _P, _P, _P, _P, C.c_int, _P, C.c_int, C.c_int, _P,
+                                                   C.c_int]),
     "orc_bssn64_stage": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, _P, _P, _P, C.c_int, C.c_int,
                                 C.c_int]),
     "orc_bssn64_constraints": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P]),

@@ -206,3 +206,105 @@ def test_bssn_flat_space_stays_flat(am, torch_cuda):
         assert np.max(np.abs(got - flat[v])) < 1e-5, v
     c = sim.constraints()
     assert c["H_l2"] < 1e-4 and c["M_l2"] < 1e-4, c
+
+
+def sim_word_bytes(am, sim, n, fmt):
+    """The driver's d1 / d2diag / d2off arrays as raw bytes per field."""
+    wb = am.lib().am_word_bytes(fmt)
+    W = n * S ** 3 * wb
+    get = lambda which, k: [download(am, sim.device_ptr(which), W, np.uint8, i * W) for i in range(k)]
+    return get(4, 24), get(5, 11), get(6, 11)
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_bssn_word_formats_match_oracle(am, orc, torch_cuda, fmt):
+    """fp32 / fp16 derivative words (the precision study's formats) through
+    two RK4 stages of a depth-1 periodic mesh: the words equal the oracle's
+    undithered fp32 stencils bit for bit (fp16: those values rounded to
+    nearest even by numpy); the RHS + update match the FP64 oracle on the
+    same words to the cf3 path's bar; the constraint sums match to 1e-4."""
+    print(f"bssn word format {fmt}", flush=True)
+    ops, n = meshes.uniform(1)
+    g = am.geometry(S, P, ORDER, OUTER, CFL)
+    sim = am.BssnSim(g, n, am.bssn_options(periodic=True, word_format=fmt))
+    ot = orc.OracleTree(n, True)
+    for p_, c in ops:
+        assert sim.add_node(p_, c) == ot.add_node(p_, c)
+    e = ot.export()
+    state0 = smooth_state(e, n, seed=5)
+    for v in range(24):
+        sim.set_field(0, v, state0[v])
+        sim.set_field(1, v, state0[v])
+    tiles = [s for s in F.used_slots(e) if F.depth_of(e, s) == 1]
+    outer_dx = float(np.float32(OUTER) / np.float32(S))
+    dtd = np.array([float(np.float32(np.float32(outer_dx) * np.float32(CFL)) / np.float32(2 ** d)) for d in range(9)])
+    params = np.array(orc.BSSN_DEFAULT, np.float64)
+    imask = interior_mask(n, tiles)
+    wmask = tile_mask(n, tiles)
+    old = [x.copy() for x in state0]
+    as_words = (lambda a: a.view(np.float32).reshape(-1, 3)) if fmt == 1 else \
+        (lambda a: a.view(np.float16).reshape(-1, 3))
+    to_fmt = (lambda x: x) if fmt == 1 else (lambda x: x.astype(np.float16))
+    for stage in range(2):
+        state_in = old if stage == 0 else [sim.get_field(1, v) for v in range(24)]
+        acc_in = [sim.get_field(2, v).astype(np.float64) for v in range(24)]
+        sim.stage_depth(1, stage)
+        sim.sync()
+        d1, d2d, d2o = sim_word_bytes(am, sim, n, fmt)
+        h = 0
+        for v in range(24):
+            hess = v in HESS
+            r1, r2 = orc.differentiate_f32(state_in[v], e["positions"], tiles, 0, e["max_depth"], S, P, ORDER // 2,
+                                           outer_dx, n, with_d2=hess)
+            assert np.array_equal(as_words(d1[v])[wmask].view(np.uint8), to_fmt(r1[wmask]).view(np.uint8)), \
+                (stage, v)
+            if hess:
+                ro = orc.mixed_partials_f32(state_in[v], e["positions"], tiles, 0, e["max_depth"], S, P, outer_dx, n)
+                assert np.array_equal(as_words(d2d[h])[wmask].view(np.uint8), to_fmt(r2[wmask]).view(np.uint8))
+                assert np.array_equal(as_words(d2o[h])[wmask].view(np.uint8), to_fmt(ro[wmask]).view(np.uint8))
+                h += 1
+        acc_o = [np.zeros(n * T ** 3) for _ in range(24)]
+        new_o = [np.zeros(n * T ** 3) for _ in range(24)]
+        sin32 = [np.ascontiguousarray(x, np.float32) for x in state_in]
+        orc.lib().orc_bssn_stage_from_words_fmt(orc.ptrs(sin32), orc.ptrs(old), orc.ptrs(acc_in), orc.ptrs(acc_o),
+                                                orc.ptrs(new_o), orc.ptrs(d1), orc.ptrs(d2d), orc.ptrs(d2o),
+                                                e["positions"].ctypes.data, np.array(tiles, np.int32).ctypes.data,
+                                                len(tiles), 0, e["max_depth"], stage, dtd.ctypes.data,
+                                                params.ctypes.data, S, P, fmt)
+        new_g = [sim.get_field(1, v) for v in range(24)]
+        acc_g = [sim.get_field(2, v) for v in range(24)]
+        worst = 0.0
+        for v in range(24):
+            k_o = (acc_o[v] - acc_in[v])[imask] / (1 if stage == 0 else 2)
+            k_g = (acc_g[v].astype(np.float64) - acc_in[v])[imask] / (1 if stage == 0 else 2)
+            scale = max(np.max(np.abs(k_o)), 1e-3)
+            err = np.max(np.abs(k_g - k_o)) / scale
+            worst = max(worst, err)
+            assert err < 5e-5, (stage, v, err)
+            dn = np.max(np.abs(new_g[v][imask] - new_o[v][imask]))
+            assert dn < 1e-6 * max(1.0, np.max(np.abs(new_o[v][imask]))), (stage, v, dn)
+        print(f"  stage {stage}: worst relative k error {worst:.2e}", flush=True)
+    # constraints of the stage-1 output, on its own words
+    cons = sim.constraints()
+    st = [sim.get_field(1, v) for v in range(24)]
+    d1, d2d, d2o = sim_word_bytes(am, sim, n, fmt)
+    out = np.zeros(4 * len(tiles))
+    orc.lib().orc_bssn_constraints_from_words_fmt(orc.ptrs(st), orc.ptrs(d1), orc.ptrs(d2d), orc.ptrs(d2o),
+                                                  np.array(tiles, np.int32).ctypes.data, len(tiles),
+                                                  params.ctypes.data, S, P, out.ctypes.data, fmt)
+    q = out.reshape(-1, 4)
+    h_l2, m_l2 = np.sqrt(q[:, 0].sum() / q[:, 2].sum()), np.sqrt(q[:, 1].sum() / q[:, 2].sum())
+    assert cons["cells"] == q[:, 2].sum()
+    assert abs(cons["H_l2"] - h_l2) <= 1e-4 * h_l2, (cons, h_l2)
+    assert abs(cons["M_l2"] - m_l2) <= 1e-4 * m_l2, (cons, m_l2)
+
+
+def test_word_format_rejected_off_geometry(am, torch_cuda):
+    """fp32 / fp16 words exist for S=16, P=3, order-4 tiles only: other
+    geometries and unknown formats are refused at creation."""
+    g = am.geometry(16, 2, 4, OUTER, CFL)
+    with pytest.raises(am.AMError):
+        am.BssnSim(g, 9, am.bssn_options(periodic=True, word_format=1))
+    g = am.geometry(S, P, ORDER, OUTER, CFL)
+    with pytest.raises(am.AMError):
+        am.BssnSim(g, 9, am.bssn_options(periodic=True, word_format=7))
