@@ -66,7 +66,7 @@ class FdBatch(C.Structure):
                 ("d2diag", C.c_void_p * AM_MAX_VARS), ("d2off", C.c_void_p * AM_MAX_VARS),
                 ("seed_d", C.c_int32 * AM_MAX_VARS), ("seed_m", C.c_int32 * AM_MAX_VARS),
                 ("n_vars", C.c_int32), ("z_block", C.c_int32), ("seed_table", C.c_void_p),
-                ("word_format", C.c_int32)]
+                ("word_format", C.c_int32), ("prefetch_planes", C.c_int32)]
 
 
 class FieldBatch(C.Structure):
@@ -96,7 +96,7 @@ class _Trial(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("z_block", C.c_int32), ("fd_grid_limit", C.c_int32), ("rhs_grid_limit", C.c_int32),
-                ("n_trials", C.c_int32), ("trial", _Trial * 32)]
+                ("n_trials", C.c_int32), ("trial", _Trial * 32), ("fd_prefetch", C.c_int32)]
 
 
 class MlpWork(C.Structure):
@@ -164,6 +164,7 @@ _SIGS = {
                                         C.c_int32, _P]),
     "am_copy_new_to_old": (C.c_int, [_P, _P, C.POINTER(Mesh), C.POINTER(Geometry), _P]),
     "am_copy_new_to_old_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), _P]),
+    "am_copy_new_to_old_interior": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), _P, C.POINTER(Geometry), _P]),
     "am_copy_to_child": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int32, C.POINTER(Geometry), C.c_int32, _P]),
     "am_copy_to_child_batched": (C.c_int, [C.POINTER(FieldBatch), C.c_uint32, C.c_int32, C.c_int32,
                                            C.POINTER(Geometry), C.c_int32, _P]),
@@ -233,6 +234,7 @@ _RK4_SIM_SIGS = {
     "kernel_times": (C.c_int, [_P, _P, _P]),
     "set_launch": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
     "get_launch": (C.c_int, [_P, _P]),
+    "set_fd_prefetch": (C.c_int, [_P, C.c_int32]),
     "use_graph": (C.c_int, [_P, C.c_int32]),
     "autotune": (C.c_int, [_P, C.c_int32, _P]),
     "set_prolongation": (C.c_int, [_P, _P, C.c_float]),
@@ -499,8 +501,12 @@ class _Rk4Sim:
     def set_launch(self, z_block: int = 0, fd_grid_limit: int = 0, rhs_grid_limit: int = 0):
         check(self._f("set_launch")(self.h, z_block, fd_grid_limit, rhs_grid_limit), "set_launch")
 
+    def set_fd_prefetch(self, planes: int):
+        check(self._f("set_fd_prefetch")(self.h, planes), "set_fd_prefetch")
+
     def get_launch(self) -> tuple:
-        out = np.zeros(3, np.int32)
+        """(z_block, fd_grid_limit, rhs_grid_limit, fd_prefetch)"""
+        out = np.zeros(4, np.int32)
         check(self._f("get_launch")(self.h, out.ctypes.data), "get_launch")
         return tuple(int(x) for x in out)
 
@@ -510,8 +516,9 @@ class _Rk4Sim:
     def autotune(self, reps: int = 5) -> dict:
         r = TuneReport()
         check(self._f("autotune")(self.h, reps, C.byref(r)), "autotune")
-        kinds = ("fd_z_block", "fd_grid_limit", "rhs_grid_limit")
+        kinds = ("fd_z_block", "fd_grid_limit", "rhs_grid_limit", "fd_prefetch")
         return {"z_block": r.z_block, "fd_grid_limit": r.fd_grid_limit, "rhs_grid_limit": r.rhs_grid_limit,
+                "fd_prefetch": r.fd_prefetch,
                 "trials": [(kinds[r.trial[i].kind], r.trial[i].value, round(r.trial[i].ms, 4))
                            for i in range(r.n_trials)]}
 

@@ -49,7 +49,7 @@ __device__ __forceinline__ float mixed_bd(float c, float pp, float mm, float m0,
     return c + .5f * (pp + mm - m0 - p0 - zm - zp);
 }
 
-template <int S, int P, int Q, int ZB, bool PERSIST, int FMT>
+template <int S, int P, int Q, int ZB, bool PERSIST, int FMT, int PF>
 __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScales sc)
 {
     constexpr int T = S + 2 * P, T2 = T * T, R = 2 * P + 2, NZB = S / ZB;
@@ -80,34 +80,37 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     const int z0 = zb * ZB;
     const int c = (y + P) * T + (x + P);  // centre offset within a plane
 
-    // Planes are prefetched one iteration ahead into registers (NL float4
-    // per thread), so the HBM latency of plane z+2P+2 overlaps the compute
-    // of plane z.  At iteration z the ring holds padded planes z..z+2P; the
-    // slot of plane z+2P+1 is (z-1) % R, which nobody reads in iteration z,
-    // so one barrier per iteration suffices.
+    // Planes are prefetched PF iterations ahead into registers (NL float4
+    // per thread per plane), so the HBM latency of the next planes overlaps
+    // the compute of plane z.  At iteration z the ring holds padded planes
+    // z..z+2P; the slot of plane z+2P+1 is (z-1) % R, which nobody reads in
+    // iteration z, so one barrier per iteration suffices.  Only planes
+    // z0..z0+ZB-1+2P are ever read.
     constexpr int NL = (T2 / 4 + S * S - 1) / (S * S);
-    float4 pre[NL];
-    auto gload = [&](int zp) {
+    float4 pre[PF][NL];
+    auto gload = [&](float4 *dst, int zp) {
         const float4 *s4 = reinterpret_cast<const float4 *>(src + (size_t)zp * T2);
 #pragma unroll
         for (int j = 0; j < NL; j++) {
             const int i = tid + j * S * S;
-            if (i < T2 / 4) pre[j] = s4[i];
+            if (i < T2 / 4) dst[j] = s4[i];
         }
     };
-    auto lstore = [&](int zp) {
+    auto lstore = [&](const float4 *from, int zp) {
         float4 *d4 = ring4[zp % R];
 #pragma unroll
         for (int j = 0; j < NL; j++) {
             const int i = tid + j * S * S;
-            if (i < T2 / 4) d4[i] = pre[j];
+            if (i < T2 / 4) d4[i] = from[j];
         }
     };
     for (int p = 0; p <= 2 * P; p++) {
-        gload(z0 + p);
-        lstore(z0 + p);
+        gload(pre[0], z0 + p);
+        lstore(pre[0], z0 + p);
     }
-    if (ZB > 1) gload(z0 + 2 * P + 1);
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (j + 1 < ZB) gload(pre[j], z0 + 2 * P + 1 + j);
     __syncthreads();
 
     const int ex = x - min(max(x, 1), S - 2);
@@ -170,22 +173,34 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
             word_put<FMT>(o_off, o, vxy * dx2, vxz * dx2, vyz * dx2, rng);
         }
         if (z + 1 < z0 + ZB) {
-            lstore(z + 2 * P + 1);
-            if (z + 2 < z0 + ZB) gload(z + 2 * P + 2);
+            lstore(pre[0], z + 2 * P + 1);
+#pragma unroll
+            for (int j = 0; j + 1 < PF; j++)
+#pragma unroll
+                for (int k = 0; k < NL; k++) pre[j][k] = pre[j + 1][k];
+            if (z + 1 + PF < z0 + ZB) gload(pre[PF - 1], z + 2 * P + 1 + PF);
         }
         __syncthreads();
     }
     }  // work items
 }
 
-template <int S, int P, int Q, int ZB, int FMT>
-static void fd_launch_fmt(bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
-                          const DepthScales &sc)
+template <int S, int P, int Q, int ZB, int FMT, int PF>
+static void fd_launch_pf(bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
+                         const DepthScales &sc)
 {
     if (persist)
-        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, FMT>), grid, dim3(S * S), 0, st, d, m, sc);
+        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, FMT, PF>), grid, dim3(S * S), 0, st, d, m, sc);
     else
-        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false, FMT>), grid, dim3(S * S), 0, st, d, m, sc);
+        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false, FMT, PF>), grid, dim3(S * S), 0, st, d, m, sc);
+}
+template <int S, int P, int Q, int ZB, int FMT>
+static void fd_launch_fmt(int pf, bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
+                          const DepthScales &sc)
+{
+    if (pf == 2) return fd_launch_pf<S, P, Q, ZB, FMT, 2>(persist, grid, st, d, m, sc);
+    if (pf == 3) return fd_launch_pf<S, P, Q, ZB, FMT, 3>(persist, grid, st, d, m, sc);
+    fd_launch_pf<S, P, Q, ZB, FMT, 1>(persist, grid, st, d, m, sc);
 }
 
 // the fp32 / fp16 word formats (precision study) exist for the S = 16, P = 3,
@@ -197,14 +212,15 @@ constexpr bool has_word_formats()
 }
 
 template <int S, int P, int Q, int ZB>
-static void fd_launch(int fmt, bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
-                      const DepthScales &sc)
+static void fd_launch(int fmt, int pf, bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d,
+                      const MeshDev &m, const DepthScales &sc)
 {
     if constexpr (has_word_formats<S, P, Q>()) {
-        if (fmt == WORD_F32) return fd_launch_fmt<S, P, Q, ZB, WORD_F32>(persist, grid, st, d, m, sc);
-        if (fmt == WORD_F16) return fd_launch_fmt<S, P, Q, ZB, WORD_F16>(persist, grid, st, d, m, sc);
+        // the study formats keep the one-plane prefetch
+        if (fmt == WORD_F32) return fd_launch_pf<S, P, Q, ZB, WORD_F32, 1>(persist, grid, st, d, m, sc);
+        if (fmt == WORD_F16) return fd_launch_pf<S, P, Q, ZB, WORD_F16, 1>(persist, grid, st, d, m, sc);
     }
-    fd_launch_fmt<S, P, Q, ZB, WORD_CF3>(persist, grid, st, d, m, sc);
+    fd_launch_fmt<S, P, Q, ZB, WORD_CF3>(pf, persist, grid, st, d, m, sc);
 }
 
 template <int S, int P, int Q>
@@ -244,14 +260,16 @@ static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometr
     };
     const bool persist = mesh->grid_limit > 0;
     const int fmt = b->word_format;
+    const int pf = b->prefetch_planes <= 1 ? 1 : b->prefetch_planes;
+    if (pf > 3) return set_error(AM_ERR_INVALID, "am_fd: prefetch_planes %d (0-3)", b->prefetch_planes);
     if (fmt != WORD_CF3 && !(has_word_formats<S, P, Q>() && (fmt == WORD_F32 || fmt == WORD_F16)))
         return set_error(AM_ERR_INVALID, "am_fd: word format %d needs S=16, P=3, fd_order 4", fmt);
     if (zb == S) {
-        fd_launch<S, P, Q, S>(fmt, persist, dim3(gx(1), b->n_vars), st, d, items(1), sc);
+        fd_launch<S, P, Q, S>(fmt, pf, persist, dim3(gx(1), b->n_vars), st, d, items(1), sc);
     } else if (zb == S / 2) {
-        fd_launch<S, P, Q, S / 2>(fmt, persist, dim3(gx(2), b->n_vars), st, d, items(2), sc);
+        fd_launch<S, P, Q, S / 2>(fmt, pf, persist, dim3(gx(2), b->n_vars), st, d, items(2), sc);
     } else if (zb == S / 4 && (S % 4) == 0) {
-        fd_launch<S, P, Q, S / 4>(fmt, persist, dim3(gx(4), b->n_vars), st, d, items(4), sc);
+        fd_launch<S, P, Q, S / 4>(fmt, pf, persist, dim3(gx(4), b->n_vars), st, d, items(4), sc);
     } else {
         return set_error(AM_ERR_INVALID, "am_fd: z_block %d must be S, S/2 or S/4", zb);
     }

@@ -428,14 +428,23 @@ int32_t am_wave_sim_rng_state(const am_wave_sim *s) { return s ? s->amr->sim_dat
             d.drop_graph();                                                                                        \
         });                                                                                                        \
     }                                                                                                              \
-    int P##_get_launch(const H *s, int32_t *out3)                                                                  \
+    int P##_get_launch(const H *s, int32_t *out4)                                                                  \
     {                                                                                                              \
-        if (!s || !out3) return am::set_error(AM_ERR_INVALID, "null argument");                                    \
+        if (!s || !out4) return am::set_error(AM_ERR_INVALID, "null argument");                                    \
         const auto &o = s->amr->sim_data.opt;                                                                      \
-        out3[0] = o.z_block;                                                                                       \
-        out3[1] = o.fd_grid_limit;                                                                                 \
-        out3[2] = o.rhs_grid_limit;                                                                                \
+        out4[0] = o.z_block;                                                                                       \
+        out4[1] = o.fd_grid_limit;                                                                                 \
+        out4[2] = o.rhs_grid_limit;                                                                                \
+        out4[3] = o.fd_prefetch;                                                                                   \
         return AM_OK;                                                                                              \
+    }                                                                                                              \
+    int P##_set_fd_prefetch(H *s, int32_t planes)                                                                  \
+    {                                                                                                              \
+        if (planes < 1 || planes > 3) return am::set_error(AM_ERR_INVALID, "fd prefetch %d (1-3)", planes);        \
+        return rk4_call(s, [&](auto &d) {                                                                          \
+            d.opt.fd_prefetch = planes;                                                                            \
+            d.drop_graph();                                                                                        \
+        });                                                                                                        \
     }                                                                                                              \
     int P##_use_graph(H *s, int32_t enable)                                                                        \
     {                                                                                                              \

@@ -324,12 +324,14 @@ static int launch_ghost(const am_field_batch *b, const am_mesh *mesh, int copy_o
 // streams the whole padded tile: planes outside the interior, and every
 // plane of a tile without children, are plain 16-byte-vector copies; the
 // interior planes of a refined tile take the child means cell by cell.
-// With `older` set, the previous old is kept there first.
+// With `older` set, the previous old is kept there first.  With
+// `keep_ghosts` set, a tile whose flag is 0 copies its interior cells only
+// (its ghost cells of old and older are left as they are).
 template <int S, int P>
 __global__ void __launch_bounds__(256) k_copy_new_to_old(FieldBatchDev b, const uint32_t *__restrict__ positions,
                                                          const int32_t *__restrict__ children,
                                                          const int32_t *__restrict__ tiles, uint32_t substep,
-                                                         uint32_t maxd)
+                                                         uint32_t maxd, const uint8_t *__restrict__ keep_ghosts)
 {
     constexpr int T = S + 2 * P, T2 = T * T;
     constexpr size_t CELLS = (size_t)T * T2;
@@ -347,6 +349,46 @@ __global__ void __launch_bounds__(256) k_copy_new_to_old(FieldBatchDev b, const 
     for (int i = 0; i < 8; i++) any_child |= ch[i] != -1;
     const size_t base = (size_t)node * CELLS;
     constexpr uint32_t H = S / 2u;
+    if (keep_ghosts && !keep_ghosts[node] && !any_child) {  // uniform per workgroup
+        // each interior plane's rows P..P+S-1 are one contiguous run of S*T
+        // floats: copied as 16-byte vectors from the vector boundary at or
+        // below its start to the one at or above its end (the few extra
+        // cells are ghost cells, copied as the full form would)
+        for (int z = P; z < P + S; z++) {
+            const int a4 = ((z * T + P) * T) / 4, b4 = ((z * T + P + S) * T + 3) / 4;
+            const float4 *s4 = reinterpret_cast<const float4 *>(dnew + base);
+            float4 *d4 = reinterpret_cast<float4 *>(dold + base);
+            float4 *o4 = dolder ? reinterpret_cast<float4 *>(dolder + base) : nullptr;
+            for (int i = a4 + (int)threadIdx.x; i < b4; i += 256) {
+                if (o4) o4[i] = d4[i];
+                d4[i] = s4[i];
+            }
+        }
+        return;
+    }
+    if (keep_ghosts && !keep_ghosts[node]) {  // uniform per workgroup: restriction on the interior only
+        for (int z = P; z < P + S; z++) {
+            const uint32_t tz = ((uint32_t)z - (uint32_t)P) / H;
+            for (int i = threadIdx.x; i < S * S; i += 256) {
+                const int yi = i / S, xi = i - yi * S;
+                const size_t o = base + ((size_t)z * T + (yi + P)) * T + (xi + P);
+                if (dolder) dolder[o] = dold[o];
+                const int child = any_child ? ch[(uint32_t)xi / H + ((uint32_t)yi / H) * 2u + tz * 4u] : -1;
+                if (child == -1) {
+                    dold[o] = dnew[o];
+                    continue;
+                }
+                const size_t cx = ((uint32_t)xi * 2u) % (uint32_t)S + (uint32_t)P;
+                const size_t cy = ((uint32_t)yi * 2u) % (uint32_t)S + (uint32_t)P;
+                const size_t cz = (((uint32_t)z - (uint32_t)P) * 2u) % (uint32_t)S + (uint32_t)P;
+                const size_t c = (size_t)child * CELLS + (cz * T + cy) * T + cx;
+                dold[o] = (dnew[c] + dnew[c + 1] + dnew[c + T] + dnew[c + T + 1] + dnew[c + T2] + dnew[c + T2 + 1] +
+                           dnew[c + T2 + T] + dnew[c + T2 + T + 1]) *
+                          .125f;
+            }
+        }
+        return;
+    }
     for (int z = 0; z < T; z++) {
         const uint32_t tz = ((uint32_t)z - (uint32_t)P) / H;
         if (!any_child || tz >= 2u) {
@@ -381,7 +423,7 @@ __global__ void __launch_bounds__(256) k_copy_new_to_old(FieldBatchDev b, const 
 }
 
 template <int S, int P>
-static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, void *stream)
+static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, const uint8_t *keep_ghosts, void *stream)
 {
     FieldBatchDev d;
     for (int v = 0; v < b->n_vars; v++) {
@@ -394,7 +436,7 @@ static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, void *
     }
     dim3 grid(mesh->n_tiles, b->n_vars);
     hipLaunchKernelGGL((k_copy_new_to_old<S, P>), grid, dim3(256), 0, (hipStream_t)stream, d, mesh->positions,
-                       mesh->children, mesh->tiles, mesh->substep, mesh->max_depth);
+                       mesh->children, mesh->tiles, mesh->substep, mesh->max_depth, keep_ghosts);
     return check_launch("k_copy_new_to_old");
 }
 
@@ -508,7 +550,18 @@ extern "C" int am_copy_new_to_old_batched(const am_field_batch *b, const am_mesh
         return am::set_error(AM_ERR_TOO_MANY_VARS, "am_copy_new_to_old_batched: n_vars %d", b->n_vars);
     if (int rc = am::mesh_checks(mesh, g, false, true)) return rc;
     if (b->n_vars == 0 || mesh->n_tiles == 0) return AM_OK;
-    AM_DISPATCH_TILE(*g, am::launch_copy_back, b, mesh, stream);
+    AM_DISPATCH_TILE(*g, am::launch_copy_back, b, mesh, nullptr, stream);
+}
+
+extern "C" int am_copy_new_to_old_interior(const am_field_batch *b, const am_mesh *mesh, const uint8_t *keep_ghosts,
+                                           const am_geometry *g, void *stream)
+{
+    if (!b || !keep_ghosts) return am::set_error(AM_ERR_INVALID, "am_copy_new_to_old_interior: null argument");
+    if (b->n_vars < 0 || b->n_vars > AM_MAX_VARS)
+        return am::set_error(AM_ERR_TOO_MANY_VARS, "am_copy_new_to_old_interior: n_vars %d", b->n_vars);
+    if (int rc = am::mesh_checks(mesh, g, false, true)) return rc;
+    if (b->n_vars == 0 || mesh->n_tiles == 0) return AM_OK;
+    AM_DISPATCH_TILE(*g, am::launch_copy_back, b, mesh, keep_ghosts, stream);
 }
 
 extern "C" int am_copy_new_to_old(float *old_, const float *new_, const am_mesh *mesh, const am_geometry *g,

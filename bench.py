@@ -50,8 +50,9 @@ BYTES = {
     "rhs_stage": (96 + 184 + 192 + PUSH) + (96 + 184 + 96 + 96 + 192 + PUSH) * 2 + (96 + 184 + 96 + 96 + 96 + PUSH),
     # per stage: 6 P S^2 face cells per tile, one source read and one write, 24 fields (coarse faces only)
     "ghost": 24 * 6 * P * S * S * 8 / S ** 3,
-    # per step: T^3 read + T^3 write per field
-    "copy_back": 24 * 2 * 4 * (S + 2 * P) ** 3 / S ** 3,
+    # per step: interior read of new + interior write of old per field (no tile of this mesh is the coarse
+    # source of a finer face, so the copy-back leaves ghost cells alone)
+    "copy_back": 24 * 2 * 4,
 }
 WORKLOADS = {
     "bssn512": "BSSN single level: 512 tiles of 16^3 (128^3) + 73 ancestors, P=3, 24 fp32 fields, 1+log + "
@@ -277,7 +278,9 @@ def main():
                    "tiles": n_slots, "zone_cycles_per_step": zones // max(args.steps, 1),
                    "fine_rk4_steps_per_step": 2 ** sim.counter()[1],
                    "launch": {"z_block": sim.get_launch()[0], "fd_grid_limit": sim.get_launch()[1],
-                              "rhs_grid_limit": sim.get_launch()[2], "autotuned": tune is not None,
+                              "rhs_grid_limit": sim.get_launch()[2], "fd_prefetch_planes": sim.get_launch()[3],
+                              "autotuned": tune is not None,
+                              "autotune_trials": tune["trials"] if tune else None,
                               "graph_per_coarse_step": not args.no_graph}},
         "hbm_whole_step": {"achieved": round(whole_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(whole_gbs / HBM_PEAK_GBS, 4), "bytes_per_zone_cycle": round(bytes_per_zone)},

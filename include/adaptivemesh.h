@@ -179,6 +179,9 @@ typedef struct am_fd_batch {
      * tiles only); d1/d2diag/d2off then point to arrays of that many bytes
      * per cell */
     int32_t word_format;
+    /* padded planes each workgroup prefetches into registers ahead of the
+     * plane it differentiates: 0 or 1 (default), 2 or 3; cf3 words only */
+    int32_t prefetch_planes;
 } am_fd_batch;
 enum am_word_format { AM_WORD_CF3 = 0, AM_WORD_F32 = 1, AM_WORD_F16 = 2 };
 /* bytes per cell of one word array in `format` (4, 12, 6), -1 if unknown */
@@ -214,6 +217,16 @@ int am_ghost_fill_batched(const am_field_batch *b, const am_mesh *mesh, const am
  * quadratic time interpolation; an extension). */
 int am_copy_new_to_old(float *old_, const float *new_, const am_mesh *mesh, const am_geometry *g, void *stream);
 int am_copy_new_to_old_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream);
+/* The batched copy-back that leaves ghost cells alone where nothing reads
+ * them: on a tile whose keep_ghosts[slot] (device, one byte per slot) is 0,
+ * only interior cells are copied (and restricted), and the ghost cells of
+ * old and older keep whatever they held.  No counterpart in the reference.
+ * For a driver whose first RK4 stage reads new (equal to old after the
+ * copy) rather than old: then old's ghost cells are read only by
+ * interpolation into a finer neighbour's face, and keep_ghosts marks the
+ * slots that are such a coarse source. */
+int am_copy_new_to_old_interior(const am_field_batch *b, const am_mesh *mesh, const uint8_t *keep_ghosts,
+                                const am_geometry *g, void *stream);
 
 /* Prolong parent `parent` into every cell of new child `child` at octant
  * `offset`.  Replaces __copy_to_child AMR.h:1053-1073 / copy_to_child

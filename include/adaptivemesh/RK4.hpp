@@ -95,16 +95,18 @@ struct rk4_options {
     int32_t fd_grid_limit = 0;            // persistent FD launch: workgroup cap (0 = one per item)
     int32_t rhs_grid_limit = 0;           // persistent RHS launch: workgroup cap (0 = one per item)
     int32_t word_format = AM_WORD_CF3;    // derivative-word storage (am_word_format)
+    int32_t fd_prefetch = 1;              // FD planes prefetched ahead (1-3, cf3 words)
 };
 
 // What autotune() measured: each trial's launch shape and mean ms per launch.
 struct rk4_tune_report {
     int32_t z_block, fd_grid_limit, rhs_grid_limit, n_trials;
     struct {
-        int32_t kind;  // 0 FD z_block, 1 FD grid limit, 2 RHS grid limit
+        int32_t kind;  // 0 FD z_block, 1 FD grid limit, 2 RHS grid limit, 3 FD prefetch planes
         int32_t value;
         float ms;      // best of the repetitions, one whole coarse step
     } trial[32];
+    int32_t fd_prefetch;
 };
 
 // Device views of the driver's buffers handed to a System's kernels.
@@ -277,6 +279,7 @@ struct rk4_AMR_data {
         am_fd_batch b{};
         b.z_block = opt.z_block;
         b.word_format = opt.word_format;
+        b.prefetch_planes = opt.fd_prefetch;
         b.seed_table = recording ? seed_table.gpu_buffer_ptr : nullptr;
         m.grid_limit = opt.fd_grid_limit;
         int h = 0, o = 0;
@@ -319,7 +322,9 @@ struct rk4_AMR_data {
         am_field_batch b = batch(true);
         const am_mesh m = parent.mesh_all();
         const int ev = timer.begin(kernel_timer::COPY_BACK);
-        check(am_copy_new_to_old_batched(&b, &m, &geom, stream), "copy_back");
+        // stage 0 reads new, so old's ghost cells matter only on coarse
+        // sources of a finer face
+        check(am_copy_new_to_old_interior(&b, &m, parent.coarse_source.gpu_buffer_ptr, &geom, stream), "copy_back");
         timer.end(ev);
     }
 
@@ -331,8 +336,12 @@ struct rk4_AMR_data {
         const am_mesh m = parent.mesh_depth(d);
         if (m.n_tiles == 0) return;
         static const float frac[4] = {.5f, .5f, 1.f, 1.f};
-        derivatives(s == 0 ? old_ : new_, m);
-        const rk4_views f = views(s == 0);
+        // the stage input: new at every stage (at stage 0 new equals old on
+        // the tiles of this depth -- copy-back made it so, and nothing has
+        // written their new since -- ghost cells included, which old need
+        // not hold)
+        derivatives(new_, m);
+        const rk4_views f = views(false);
         am_mesh mr = m;
         mr.grid_limit = opt.rhs_grid_limit;
         int ev = timer.begin(kernel_timer::RHS);
@@ -431,8 +440,8 @@ struct rk4_AMR_data {
 
     // Time the launch shapes on the live run and keep the fastest: whole
     // coarse steps (timestep(), eager) from a snapshot of the state, one
-    // parameter at a time -- FD planes per workgroup, then the FD and RHS
-    // workgroup caps.  Each trial starts from the same snapshot; the
+    // parameter at a time -- FD planes per workgroup, FD prefetch depth,
+    // then the FD and RHS workgroup caps.  Each trial starts from the same snapshot; the
     // snapshot, the host seed, the counter and the zone-cycle count are
     // restored at the end, so the run continues as if nothing happened.
     void autotune(int reps, rk4_tune_report *rep)
@@ -495,8 +504,10 @@ struct rk4_AMR_data {
         opt.z_block = S;
         opt.fd_grid_limit = 0;
         opt.rhs_grid_limit = 0;
+        opt.fd_prefetch = 1;
         trial();  // warm-up
         sweep(0, opt.z_block, {S, S / 2, S / 4});
+        if (opt.word_format == AM_WORD_CF3) sweep(3, opt.fd_prefetch, {1, 2, 3});
         sweep(1, opt.fd_grid_limit, {0, 8 * cu});
         sweep(2, opt.rhs_grid_limit, {0, 2 * cu, 4 * cu});
         copy_all(false);
@@ -513,6 +524,7 @@ struct rk4_AMR_data {
         r.z_block = opt.z_block;
         r.fd_grid_limit = opt.fd_grid_limit;
         r.rhs_grid_limit = opt.rhs_grid_limit;
+        r.fd_prefetch = opt.fd_prefetch;
         if (rep) *rep = r;
     }
     // fill every face ghost of `new` from the current data (initial data):

@@ -69,10 +69,11 @@ int32_t am_wave_sim_rng_state(const am_wave_sim *s);
 typedef struct am_tune_report {
     int32_t z_block, fd_grid_limit, rhs_grid_limit, n_trials;
     struct {
-        int32_t kind; /* 0 FD z_block, 1 FD workgroup cap, 2 RHS workgroup cap */
+        int32_t kind; /* 0 FD z_block, 1 FD workgroup cap, 2 RHS workgroup cap, 3 FD prefetch planes */
         int32_t value;
         float ms; /* best of the repetitions, one coarse step */
     } trial[32];
+    int32_t fd_prefetch;
 } am_tune_report;
 
 /* --------------------------------------------- BSSN RK4 evolution driver
@@ -129,7 +130,10 @@ int am_bssn_sim_counter(const am_bssn_sim *s, uint32_t *timer, int32_t *max_dept
 int am_bssn_sim_profile(am_bssn_sim *s, int32_t enable);
 int am_bssn_sim_kernel_times(am_bssn_sim *s, double *ms5, int64_t *count5);
 int am_bssn_sim_set_launch(am_bssn_sim *s, int32_t z_block, int32_t fd_grid_limit, int32_t rhs_grid_limit);
-int am_bssn_sim_get_launch(const am_bssn_sim *s, int32_t *out3);
+/* out4: z_block, fd_grid_limit, rhs_grid_limit, fd_prefetch */
+int am_bssn_sim_get_launch(const am_bssn_sim *s, int32_t *out4);
+/* FD planes prefetched ahead of the differentiated plane (1-3) */
+int am_bssn_sim_set_fd_prefetch(am_bssn_sim *s, int32_t planes);
 int am_bssn_sim_use_graph(am_bssn_sim *s, int32_t enable);
 int am_bssn_sim_autotune(am_bssn_sim *s, int32_t reps, am_tune_report *out);
 /* AM_INTERP_LEARNED: the MLP parameters (device, am_mlp_param_count()
@@ -181,7 +185,8 @@ int am_swave_sim_counter(const am_swave_sim *s, uint32_t *timer, int32_t *max_de
 int am_swave_sim_profile(am_swave_sim *s, int32_t enable);
 int am_swave_sim_kernel_times(am_swave_sim *s, double *ms5, int64_t *count5);
 int am_swave_sim_set_launch(am_swave_sim *s, int32_t z_block, int32_t fd_grid_limit, int32_t rhs_grid_limit);
-int am_swave_sim_get_launch(const am_swave_sim *s, int32_t *out3);
+int am_swave_sim_get_launch(const am_swave_sim *s, int32_t *out4);
+int am_swave_sim_set_fd_prefetch(am_swave_sim *s, int32_t planes);
 int am_swave_sim_use_graph(am_swave_sim *s, int32_t enable);
 int am_swave_sim_autotune(am_swave_sim *s, int32_t reps, am_tune_report *out);
 /* AM_INTERP_LEARNED: the MLP parameters (device, am_mlp_param_count()

@@ -108,11 +108,13 @@ def test_graph_replay_equals_eager_swave(am, orc, torch_cuda):
         assert sims[0].rng_state == sims[1].rng_state
 
 
-@pytest.mark.parametrize("launch", [(8, 0, 0), (4, 64, 32), (16, 1, 1), (0, 512, 7)])
+@pytest.mark.parametrize("launch", [(8, 0, 0, 1), (4, 64, 32, 2), (16, 1, 1, 3), (0, 512, 7, 2), (16, 0, 0, 3),
+                                    (4, 0, 0, 3)])
 def test_launch_shapes_bit_identical(am, orc, torch_cuda, launch):
     print(f"launch shape {launch}", flush=True)
     (ref, alt), n = bssn_pair(am, orc, 0, True)
-    alt.set_launch(*launch)
+    alt.set_launch(*launch[:3])
+    alt.set_fd_prefetch(launch[3])
     assert alt.get_launch() == launch
     ref.timestep()
     alt.timestep()
@@ -127,8 +129,8 @@ def test_autotune_restores_state(am, orc, torch_cuda):
     tuned.timestep()
     rep = tuned.autotune(reps=2)
     print(f"autotune: {rep}", flush=True)
-    assert rep["z_block"] in (4, 8, 16) and len(rep["trials"]) == 8
-    assert tuned.get_launch() == (rep["z_block"], rep["fd_grid_limit"], rep["rhs_grid_limit"])
+    assert rep["z_block"] in (4, 8, 16) and rep["fd_prefetch"] in (1, 2, 3) and len(rep["trials"]) == 11
+    assert tuned.get_launch() == (rep["z_block"], rep["fd_grid_limit"], rep["rhs_grid_limit"], rep["fd_prefetch"])
     tuned.sync()
     assert same_state(ref, tuned, 24, (0, 1, 2, 3)) is None
     assert ref.rng_state == tuned.rng_state and ref.counter() == tuned.counter()

@@ -77,11 +77,12 @@ def test_fd_bit_exact(am, orc, torch_cuda, S, P, order, recipe, substep):
     L = am.lib()
     st = am.stream_handle()
     W = n * S ** 3
-    for zb in (0, S // 2, S // 4):
+    for zb, pf in ((0, 1), (S // 2, 1), (S // 4, 1), (0, 2), (0, 3), (S // 2, 3), (S // 4, 2)):
         d1, d2, off = (torch.zeros(W, dtype=torch.int32, device="cuda") for _ in range(3))
         b = am.FdBatch()
         b.n_vars = 1
         b.z_block = zb
+        b.prefetch_planes = pf
         b.field[0], b.d1[0], b.d2diag[0], b.d2off[0] = f.data_ptr(), d1.data_ptr(), d2.data_ptr(), off.data_ptr()
         b.seed_d[0], b.seed_m[0] = seed_d, seed_m
         am.check(L.am_fd_batched(C.byref(b), C.byref(m), C.byref(g), st), "am_fd_batched")
@@ -186,6 +187,52 @@ def test_copy_new_to_old_bit_exact(am, orc, torch_cuda, S, P):
                                              am.stream_handle()), "copy_new_to_old")
         sync(torch)
         assert np.array_equal(host(do, np.float32).view(np.uint32), old.view(np.uint32))
+
+
+@pytest.mark.parametrize("S,P", [(16, 3), (28, 2)])
+def test_copy_new_to_old_interior(am, orc, torch_cuda, S, P):
+    """The driver's copy-back: tiles flagged in keep_ghosts copy like the
+    reference (every cell, restriction, older <- old first); on the others
+    every interior cell is copied the same way, and each ghost cell either
+    keeps its value or receives the reference copy's (the interior planes
+    are copied as contiguous vector runs, which take the x-ghost cells
+    along)."""
+    torch = torch_cuda
+    g = am.geometry(S, P, 2 * P, 100.0, 0.4)
+    tree, e, n = setup_mesh(am, orc, "graded")
+    T = S + 2 * P
+    rng = np.random.default_rng(S * P)
+    keep = (rng.random(n) < 0.4).astype(np.uint8)
+    ghost = np.ones((n, T, T, T), bool)
+    ghost[:, P:P + S, P:P + S, P:P + S] = False
+    lean = np.repeat(keep == 0, T ** 3) & ghost.reshape(-1)
+    for substep in range(4):
+        new = rng.standard_normal(n * T ** 3).astype(np.float32)
+        old = rng.standard_normal(n * T ** 3).astype(np.float32)
+        older = rng.standard_normal(n * T ** 3).astype(np.float32)
+        do, dn, dd = dev(torch, old), dev(torch, new), dev(torch, older)
+        want_old, want_older = old.copy(), older.copy()
+        active = [s for s in fields.used_slots(e)
+                  if (substep & ((1 << (e["max_depth"] - fields.depth_of(e, s))) - 1)) == 0]
+        idx = np.concatenate([np.arange(s * T ** 3, (s + 1) * T ** 3) for s in active])
+        want_older[idx] = old[idx]
+        orc.copy_new_to_old(want_old, new, e["positions"], e["children"], fields.used_slots(e), substep,
+                            e["max_depth"], S, P)
+        fb = am.FieldBatch()
+        fb.n_vars = 1
+        fb.old_[0], fb.new_[0], fb.older[0] = do.data_ptr(), dn.data_ptr(), dd.data_ptr()
+        kg = torch.tensor(keep, device="cuda")
+        m = mesh_for(tree, -1, substep)
+        am.check(am.lib().am_copy_new_to_old_interior(C.byref(fb), C.byref(m), kg.data_ptr(), C.byref(g),
+                                                      am.stream_handle()), "copy_new_to_old_interior")
+        sync(torch)
+        got_old, got_older = host(do, np.float32).view(np.uint32), host(dd, np.float32).view(np.uint32)
+        full = ~lean
+        assert np.array_equal(got_old[full], want_old.view(np.uint32)[full]), substep
+        assert np.array_equal(got_older[full], want_older.view(np.uint32)[full]), substep
+        assert np.all((got_old[lean] == old.view(np.uint32)[lean]) | (got_old[lean] == want_old.view(np.uint32)[lean]))
+        assert np.all((got_older[lean] == older.view(np.uint32)[lean]) |
+                      (got_older[lean] == want_older.view(np.uint32)[lean]))
 
 
 @pytest.mark.parametrize("mode", [0, 1])
