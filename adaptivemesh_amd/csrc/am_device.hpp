@@ -40,6 +40,23 @@ __device__ __forceinline__ int32_t thread_seed(int32_t gs, uint32_t x, uint32_t 
     return s;
 }
 
+// thread_seed split for a thread that keeps x and y and walks z: the part
+// that depends on (gs, x, y) -- the state after the second draw -- and the
+// final mix with z, so thread_seed(gs, x, y, z) == thread_seed_z(
+// thread_seed_xy(gs, x, y), z) bit for bit
+__device__ __forceinline__ int32_t thread_seed_xy(int32_t gs, uint32_t x, uint32_t y)
+{
+    int32_t s = (int32_t)((uint32_t)gs + x * 2178614u);
+    const uint32_t t = (uint32_t)prng_next(s) * y;
+    s = (int32_t)((uint32_t)s ^ t);
+    prng_next(s);
+    return s;
+}
+__device__ __forceinline__ int32_t thread_seed_z(int32_t sxy, uint32_t z)
+{
+    return (int32_t)((uint32_t)sxy ^ ((uint32_t)sxy * z));
+}
+
 // ------------------------------------------------- compressed_float3 codec
 __device__ __forceinline__ float cf3_clamp(float f) { return fmaxf(-255.f, fminf(f, 255.f)); }
 

@@ -116,6 +116,9 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     const int ex = x - min(max(x, 1), S - 2);
     const int ey = y - min(max(y, 1), S - 2);
     const int cxy = ex * ey;
+    // dither seeds: the (x, y) part once, the z mix per plane
+    const int32_t sxy_d = thread_seed_xy(gs_d, (uint32_t)x, (uint32_t)y);
+    const int32_t sxy_m = thread_seed_xy(gs_m, (uint32_t)x, (uint32_t)y);
 
     for (int z = z0; z < z0 + ZB; z++) {
         // padded plane of output z is z+P; plane z+P+i sits in slot (z+P+i) % R
@@ -141,7 +144,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
                 zz = zz + a * C.d2[BASE + i];
             }
             vx = vx * dx1; vy = vy * dx1; vz = vz * dx1;
-            int32_t rng = thread_seed(gs_d, (uint32_t)x, (uint32_t)y, gz);
+            int32_t rng = thread_seed_z(sxy_d, gz);
             if (o_d2) word_put<FMT>(o_d2, o, xx * dx2, yy * dx2, zz * dx2, rng);
             word_put<FMT>(o_d1, o, vx, vy, vz, rng);
         }
@@ -169,7 +172,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
                 const int dy = -cyz * T;  // +-1 along y, dz = one plane up
                 vyz = (float)(-cyz) * mixed_bd(ctr[c], zp[c + dy], zm[c - dy], ctr[c - dy], ctr[c + dy], zm[c], zp[c]);
             }
-            int32_t rng = thread_seed(gs_m, (uint32_t)x, (uint32_t)y, gz);
+            int32_t rng = thread_seed_z(sxy_m, gz);
             word_put<FMT>(o_off, o, vxy * dx2, vxz * dx2, vyz * dx2, rng);
         }
         if (z + 1 < z0 + ZB) {
