@@ -422,6 +422,40 @@ __global__ void __launch_bounds__(256) k_copy_new_to_old(FieldBatchDev b, const 
     }
 }
 
+// After the driver's copy-back: new <- old on the interior cells of each
+// active tile that a child covers, so that new equals old there too (the
+// RK4 driver's first stage reads new).  A separate launch, so that every
+// parent has read its children's new before any of them is changed.
+template <int S, int P>
+__global__ void __launch_bounds__(256) k_restricted_to_new(FieldBatchDev b, const uint32_t *__restrict__ positions,
+                                                           const int32_t *__restrict__ children,
+                                                           const int32_t *__restrict__ tiles, uint32_t substep,
+                                                           uint32_t maxd)
+{
+    constexpr int T = S + 2 * P;
+    constexpr size_t CELLS = (size_t)T * T * T;
+    constexpr uint32_t H = S / 2u;
+    const int node = tiles[blockIdx.x];
+    const int depth = pos_depth(positions[node]);
+    if (depth == -1 || !active_depth(substep, (uint32_t)depth, maxd)) return;
+    __shared__ int ch[8];
+    if (threadIdx.x < 8) ch[threadIdx.x] = children[(size_t)node * 8 + threadIdx.x];
+    __syncthreads();
+    bool any_child = false;
+#pragma unroll
+    for (int i = 0; i < 8; i++) any_child |= ch[i] != -1;
+    if (!any_child) return;  // uniform per workgroup
+    const float *dold = b.old_[blockIdx.y];
+    float *dnew = b.new_[blockIdx.y];
+    const size_t base = (size_t)node * CELLS;
+    for (int i = threadIdx.x; i < S * S * S; i += 256) {
+        const int zi = i / (S * S), yi = (i / S) % S, xi = i % S;
+        if (ch[(uint32_t)xi / H + ((uint32_t)yi / H) * 2u + ((uint32_t)zi / H) * 4u] == -1) continue;
+        const size_t o = base + ((size_t)(zi + P) * T + (yi + P)) * T + (xi + P);
+        dnew[o] = dold[o];
+    }
+}
+
 template <int S, int P>
 static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, const uint8_t *keep_ghosts, void *stream)
 {
@@ -437,6 +471,9 @@ static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, const 
     dim3 grid(mesh->n_tiles, b->n_vars);
     hipLaunchKernelGGL((k_copy_new_to_old<S, P>), grid, dim3(256), 0, (hipStream_t)stream, d, mesh->positions,
                        mesh->children, mesh->tiles, mesh->substep, mesh->max_depth, keep_ghosts);
+    if (keep_ghosts)
+        hipLaunchKernelGGL((k_restricted_to_new<S, P>), grid, dim3(256), 0, (hipStream_t)stream, d, mesh->positions,
+                           mesh->children, mesh->tiles, mesh->substep, mesh->max_depth);
     return check_launch("k_copy_new_to_old");
 }
 

@@ -217,14 +217,16 @@ int am_ghost_fill_batched(const am_field_batch *b, const am_mesh *mesh, const am
  * quadratic time interpolation; an extension). */
 int am_copy_new_to_old(float *old_, const float *new_, const am_mesh *mesh, const am_geometry *g, void *stream);
 int am_copy_new_to_old_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream);
-/* The batched copy-back that leaves ghost cells alone where nothing reads
- * them: on a tile whose keep_ghosts[slot] (device, one byte per slot) is 0,
- * only interior cells are copied (and restricted), and the ghost cells of
- * old and older keep whatever they held.  No counterpart in the reference.
- * For a driver whose first RK4 stage reads new (equal to old after the
- * copy) rather than old: then old's ghost cells are read only by
- * interpolation into a finer neighbour's face, and keep_ghosts marks the
- * slots that are such a coarse source. */
+/* The batched copy-back for a driver whose first RK4 stage reads new rather
+ * than old.  No counterpart in the reference.  As am_copy_new_to_old_batched,
+ * then (a second launch, after every restriction has read its children)
+ * new <- old on the restricted cells, so new equals old on the interior of
+ * every active tile.  On a tile whose keep_ghosts[slot] (device, one byte
+ * per slot) is 0, only interior cells of old and older are written (the
+ * interior planes as whole vector runs, which take a few ghost cells
+ * along): old's ghost cells are read only by interpolation into a finer
+ * neighbour's face, and keep_ghosts marks the slots that are such a coarse
+ * source. */
 int am_copy_new_to_old_interior(const am_field_batch *b, const am_mesh *mesh, const uint8_t *keep_ghosts,
                                 const am_geometry *g, void *stream);
 

@@ -487,10 +487,24 @@ EXPORT void orc_bssn_constraints_from_words_fmt(const float *const *state, const
 
 /* ===================================================== FP64 scheme on tiles */
 
-static const double CD1[15] = {-.5, 0., .5, 1. / 12, -2. / 3, 0., 2. / 3, -1. / 12,
-                               -1. / 60, 3. / 20, -3. / 4, 0., 3. / 4, -3. / 20, 1. / 60};
-static const double CD2[15] = {1., -2., 1., -1. / 12, 4. / 3, -5. / 2, 4. / 3, -1. / 12,
-                               1. / 90, -3. / 20, 3. / 2, -49. / 18, 3. / 2, -3. / 20, 1. / 90};
+static const double CD1_EXACT[15] = {-.5, 0., .5, 1. / 12, -2. / 3, 0., 2. / 3, -1. / 12,
+                                     -1. / 60, 3. / 20, -3. / 4, 0., 3. / 4, -3. / 20, 1. / 60};
+static const double CD2_EXACT[15] = {1., -2., 1., -1. / 12, 4. / 3, -5. / 2, 4. / 3, -1. / 12,
+                                     1. / 90, -3. / 20, 3. / 2, -49. / 18, 3. / 2, -3. / 20, 1. / 90};
+/* The coefficient tables the FP64 scheme uses: the exact fractions
+ * (default) or, after orc_fd64_float_coefficients(1), the same fractions
+ * rounded to float (the reference's tables, AMR.h:1144-1150) -- to separate
+ * the effect of those tables from that of fp32 arithmetic and storage. */
+static double CD1[15], CD2[15];
+static int cd_ready = 0;
+EXPORT void orc_fd64_float_coefficients(int on)
+{
+    for (int i = 0; i < 15; i++) {
+        CD1[i] = on ? (double)(float)CD1_EXACT[i] : CD1_EXACT[i];
+        CD2[i] = on ? (double)(float)CD2_EXACT[i] : CD2_EXACT[i];
+    }
+    cd_ready = 1;
+}
 
 /* Order-2q gradient and diagonal Hessian plus the mixed partials with the
  * reference's edge formula (AMR.h:1152-1266), in double, unquantised, at
@@ -540,6 +554,7 @@ EXPORT void orc_bssn64_stage(const double *const *state, const double *const *ol
                              uint32_t substep, uint32_t maxd, int stage, const double *inv_dx, const double *dtd,
                              const double *params, int S, int P, int q)
 {
+    if (!cd_ready) orc_fd64_float_coefficients(0);
     const orc_bssn_par par = par_from(params);
     const int T = S + 2 * P;
     const int64_t CELLS = (int64_t)T * T * T;
@@ -576,6 +591,7 @@ EXPORT void orc_bssn64_constraints(const double *const *state, const uint32_t *p
                                    int n_tiles, const double *inv_dx, const double *params, int S, int P, int q,
                                    double *out)
 {
+    if (!cd_ready) orc_fd64_float_coefficients(0);
     const orc_bssn_par par = par_from(params);
     const int T = S + 2 * P;
     const int64_t CELLS = (int64_t)T * T * T;

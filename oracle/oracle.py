@@ -259,6 +259,7 @@ _SIGS_BSSN = {
     "orc_bssn64_stage": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, _P, _P, _P, C.c_int, C.c_int,
                                 C.c_int]),
     "orc_bssn64_constraints": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P]),
+    "orc_fd64_float_coefficients": (None, [C.c_int]),
     "orc_ghost64": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, _u32, _u32, _f32, C.c_int, C.c_int, C.c_int,
                            C.c_int]),
     "orc_copy_back64": (None, [_P, _P, _P, C.c_int, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),

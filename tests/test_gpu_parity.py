@@ -196,7 +196,8 @@ def test_copy_new_to_old_interior(am, orc, torch_cuda, S, P):
     every interior cell is copied the same way, and each ghost cell either
     keeps its value or receives the reference copy's (the interior planes
     are copied as contiguous vector runs, which take the x-ghost cells
-    along)."""
+    along).  new then equals old on the interior of every active tile
+    (restricted cells copied back into new) and is unchanged elsewhere."""
     torch = torch_cuda
     g = am.geometry(S, P, 2 * P, 100.0, 0.4)
     tree, e, n = setup_mesh(am, orc, "graded")
@@ -218,6 +219,11 @@ def test_copy_new_to_old_interior(am, orc, torch_cuda, S, P):
         want_older[idx] = old[idx]
         orc.copy_new_to_old(want_old, new, e["positions"], e["children"], fields.used_slots(e), substep,
                             e["max_depth"], S, P)
+        want_new = new.copy()
+        inner = np.zeros((n, T, T, T), bool)
+        inner[active, P:P + S, P:P + S, P:P + S] = True
+        inner = inner.reshape(-1)
+        want_new[inner] = want_old[inner]
         fb = am.FieldBatch()
         fb.n_vars = 1
         fb.old_[0], fb.new_[0], fb.older[0] = do.data_ptr(), dn.data_ptr(), dd.data_ptr()
@@ -227,6 +233,7 @@ def test_copy_new_to_old_interior(am, orc, torch_cuda, S, P):
                                                       am.stream_handle()), "copy_new_to_old_interior")
         sync(torch)
         got_old, got_older = host(do, np.float32).view(np.uint32), host(dd, np.float32).view(np.uint32)
+        assert np.array_equal(host(dn, np.float32).view(np.uint32), want_new.view(np.uint32)), substep
         full = ~lean
         assert np.array_equal(got_old[full], want_old.view(np.uint32)[full]), substep
         assert np.array_equal(got_older[full], want_older.view(np.uint32)[full]), substep

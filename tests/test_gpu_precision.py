@@ -1,0 +1,175 @@
+"""Precision study on the 512-tile BSSN mesh (uniform depth 3, S = 16,
+P = 3, fd_order 4, periodic [-32, 32]^3, CFL 0.25): the RK4 driver with its
+derivative words stored as the reference's dithered compressed_float3, as
+fp32 or as fp16 (bulk arithmetic fp32 in every case, constraint sums in
+FP64), 50 coarse steps, against the FP64 runs of the same scheme in
+tests/golden/bssn512_fp64_{noise,gauge}.npz (written by
+tests/golden/make_bssn512_fp64.py: FP64 state, exact FP64 derivatives).
+
+Compared at coarse steps 1, 10 and 50 on the fixture's 1024 sampled interior
+cells (2 per leaf tile): per variable, the RMS of (run - FP64) and the RMS of
+the FP64 state's deviation from flat space; and the Hamiltonian and momentum
+constraint L2 over the leaves (each run evaluates its own words).  For the
+gauge wave, the exact solution after 50 coarse steps puts the scheme's own
+truncation error beside the storage errors.  The
+bars below are the measured values with a margin, so that a regression in
+any format shows; the ordering fp32 <= cf3 for the smooth gauge wave is
+asserted outright.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from adaptivemesh_amd import workloads as WL
+
+pytestmark = pytest.mark.gpu
+
+S, P, ORDER, OUTER, CFL = 16, 3, 4, 64.0, 0.25
+T = S + 2 * P
+GOLDEN = Path(__file__).with_name("golden")
+NAMES = {0: "cf3", 1: "fp32", 2: "fp16"}
+VARS = ("chi gxx gxy gxz gyy gyz gzz K Axx Axy Axz Ayy Ayz Azz Gx Gy Gz alpha bx by bz Bx By Bz").split()
+
+
+def load(name):
+    f = GOLDEN / f"bssn512_fp64_{name}.npz"
+    assert f.exists(), f"missing fixture {f}"
+    with np.load(f, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def run(am, torch, name, ref, fmt):
+    prm = ref["params"]
+    opts = am.bssn_options(eta=float(prm[0]), shift_coeff=float(prm[1]), chi_floor=float(prm[2]),
+                           advect_shift=int(prm[3]), slicing=int(prm[4]), periodic=True, word_format=fmt)
+    ops, n = WL.uniform_ops(3)
+    g = am.geometry(S, P, ORDER, OUTER, CFL)
+    sim = am.BssnSim(g, n, opts)
+    for p_, c in ops:
+        sim.add_node(p_, c)
+    pos = sim.export()["positions"]
+    f0 = WL.flat_noise_fields(n, S, P) if name == "noise" else WL.gauge_wave_fields(pos, S, P, OUTER, amplitude=0.01)
+    idx = torch.tensor(ref["idx"], device="cuda")
+    assert np.array_equal(np.stack([f0[v][ref["idx"]] for v in range(24)]).astype(np.float64), ref["initial"])
+    for v in range(24):
+        sim.set_field(0, v, f0[v])
+        sim.set_field(1, v, f0[v])
+    sim.fill_ghosts()
+    buf = torch.empty(24 * n * T ** 3, dtype=torch.float32, device="cuda")
+    vals, cons = [], []
+    k = 0
+    for cp in ref["checkpoints"]:
+        while k < cp:
+            sim.timestep()
+            k += 1
+        sim.sync()
+        am.check(am.lib().am_memcpy_d2d(buf.data_ptr(), sim.device_ptr(1), buf.numel() * 4, None), "d2d")
+        torch.cuda.synchronize()
+        vals.append(buf.view(24, -1)[:, idx].double().cpu().numpy())
+        c = sim.constraints()
+        cons.append((c["H_l2"], c["M_l2"], c["H_max"]))
+    return np.stack(vals), np.asarray(cons)
+
+
+@pytest.fixture(scope="module")
+def study(am, torch_cuda):
+    out = {}
+    for name in ("noise", "gauge"):
+        ref = load(name)
+        for fmt in (0, 1, 2):
+            print(f"precision study: {name} data, {NAMES[fmt]} words, {int(ref['checkpoints'][-1])} coarse steps",
+                  flush=True)
+            out[name, fmt] = run(am, torch_cuda, name, ref, fmt)
+        out[name] = ref
+    return out
+
+
+def metrics(study, name, ci):
+    """Per format: RMS error against FP64 and against the fp32-word run, per
+    variable; the FP64 state's RMS deviation from flat space; which
+    variables move (deviation above FP64 round-off)."""
+    ref = study[name]
+    flat = WL.FLAT.astype(np.float64)[:, None]
+    dev = np.sqrt(np.mean((ref["values"][ci] - flat) ** 2, axis=1))
+    moving = dev > 1e-9
+    out = {}
+    for fmt in (0, 1, 2):
+        vals = study[name, fmt][0][ci]
+        err = np.sqrt(np.mean((vals - ref["values"][ci]) ** 2, axis=1))
+        store = np.sqrt(np.mean((vals - study[name, 1][0][ci]) ** 2, axis=1))
+        out[fmt] = (err, store)
+    return dev, moving, out
+
+
+def report(study, name):
+    ref = study[name]
+    lines = []
+    for ci, cp in enumerate(ref["checkpoints"]):
+        if cp == 0:
+            continue
+        dev, moving, m = metrics(study, name, ci)
+        lines.append(f"{name} data, coarse step {int(cp)}: FP64 rms H {ref['constraints'][ci][0]:.4e} "
+                     f"rms M {ref['constraints'][ci][1]:.4e}; {int(moving.sum())} moving variables")
+        for fmt in (0, 1, 2):
+            err, store = m[fmt]
+            rel = err[moving] / dev[moving]
+            srel = store[moving] / dev[moving]
+            c = study[name, fmt][1][ci]
+            lines.append(f"  {NAMES[fmt]:5s} rms H {c[0]:.4e} rms M {c[1]:.4e} | vs FP64: median rel {np.median(rel):.2e} "
+                         f"max rel {rel.max():.2e} ({VARS[int(np.flatnonzero(moving)[rel.argmax()])]}), "
+                         f"max abs (still) {err[~moving].max() if (~moving).any() else 0:.2e} | vs fp32 words: "
+                         f"median rel {np.median(srel):.2e} max rel {srel.max():.2e}")
+    print("\n".join(lines), flush=True)
+
+
+def check_constraints(study, name, h_tol, m_tol):
+    ref = study[name]
+    for ci, cp in enumerate(ref["checkpoints"]):
+        for fmt in (0, 1, 2):
+            vals, cons = study[name, fmt]
+            assert np.all(np.isfinite(vals[ci])) and np.all(np.isfinite(cons[ci])), (fmt, cp)
+            if cp == 0:  # the initial data: the norms measure only how each format stores its derivatives
+                continue
+            h32, m32 = study[name, 1][1][ci][:2]
+            assert cons[ci][0] <= h_tol * h32 and cons[ci][1] <= m_tol * m32, (name, NAMES[fmt], int(cp), cons[ci])
+
+
+def test_precision_noise(study):
+    """Flat + 1e-6 noise: fp32 state rounding (1.2e-7 near 1, an eighth of
+    the noise) dominates the error against FP64 whatever the word format;
+    constraint norms of every format stay within 5% (H) and a factor 2 (M)
+    of the fp32-word run's and H within 5% of FP64's."""
+    report(study, "noise")
+    check_constraints(study, "noise", 1.05, 2.0)
+    ref = study["noise"]
+    for ci in range(len(ref["checkpoints"])):
+        for fmt in (0, 1, 2):
+            assert abs(study["noise", fmt][1][ci][0] / ref["constraints"][ci][0] - 1) < 0.05
+
+
+def test_precision_gauge(study):
+    """Gauge wave: the FP64 scheme follows the exact solution to truncation
+    error (5% of each moving variable's amplitude after 50 coarse steps);
+    after one coarse step every format is within 2% of FP64 on every moving
+    variable; the constraint norms of cf3 and fp16 words stay within a
+    factor 3 of the fp32-word run's."""
+    report(study, "gauge")
+    ref = study["gauge"]
+    flat = WL.FLAT.astype(np.float64)[:, None]
+    trunc = np.sqrt(np.mean((ref["values"][-1] - ref["exact"]) ** 2, axis=1))
+    dev = np.sqrt(np.mean((ref["exact"] - flat) ** 2, axis=1))
+    lines = ["gauge wave after 50 coarse steps, RMS error against the exact solution (Bx..Bz excluded):"]
+    lines.append("  FP64 scheme " + " ".join(f"{VARS[v]}={trunc[v]:.2e}" for v in range(21)))
+    for fmt in (0, 1, 2):
+        e = np.sqrt(np.mean((study["gauge", fmt][0][-1] - ref["exact"]) ** 2, axis=1))
+        lines.append(f"  {NAMES[fmt]:11s} " + " ".join(f"{VARS[v]}={e[v]:.2e}" for v in range(21)))
+    print("\n".join(lines), flush=True)
+    moving = dev[:21] > 1e-9
+    assert np.all(trunc[:21][moving] < 5e-2 * dev[:21][moving]), trunc
+    ci1 = list(ref["checkpoints"]).index(1)
+    d1, mv1, m1 = metrics(study, "gauge", ci1)
+    for fmt in (0, 1, 2):
+        err = m1[fmt][0]
+        assert np.all(err[mv1] < 2e-2 * d1[mv1]), (NAMES[fmt], err[mv1] / d1[mv1])
+    check_constraints(study, "gauge", 3.0, 3.0)

@@ -123,3 +123,36 @@ def test_bssn64_sim_flat_space_and_schedule(orc):
         assert np.max(np.abs(sim.new[v] - flat[v])) < 1e-12, v
     h, m, hmax, cells = sim.constraints()
     assert h < 1e-12 and m < 1e-12 and cells == len(sim.leaves) * S ** 3
+
+
+def test_float_coefficient_tables(orc):
+    """orc_fd64_float_coefficients(1) swaps the FP64 scheme's FD weights for
+    the same fractions rounded to float.  On flat space the order-4 second
+    derivative of a constant 1 is then sum(float weights) * inv_dx^2 =
+    7.450580596923828e-08 * inv_dx^2 instead of 0, and it reaches K through
+    the lapse's Laplacian: k_K = -(d_xx + d_yy + d_zz) alpha = -3 s inv_dx^2
+    (chi = alpha = 1, gt = delta, everything else 0).  The default tables
+    keep flat space flat."""
+    e, n = tree(orc, "uniform1")
+    flat = np.zeros(24)
+    flat[[0, 1, 4, 6, 17]] = 1.0
+    state = [np.full(n * T ** 3, flat[v]) for v in range(24)]
+    outer_dx = OUTER / S
+    inv_dx = [2 ** d / outer_dx for d in range(9)]
+    dtd = [0.1 / 2 ** d for d in range(9)]
+    tiles = [s for s in F.used_slots(e) if F.depth_of(e, s) == 1]
+    r = tiles[0] * T ** 3 + ((P + 5) * T + P + 6) * T + P + 7
+    s = sum(float(np.float32(c)) for c in (-1 / 12, 4 / 3, -5 / 2, 4 / 3, -1 / 12))
+    assert s == 7.450580596923828e-08
+    try:
+        ks = []
+        for on in (0, 1):
+            orc.lib().orc_fd64_float_coefficients(on)
+            acc = [np.zeros(n * T ** 3) for _ in range(24)]
+            new = [np.zeros(n * T ** 3) for _ in range(24)]
+            orc.bssn64_stage(state, state, acc, new, e["positions"], tiles, 0, 1, 0, inv_dx, dtd, S=S, P=P, q=2)
+            ks.append(acc[7][r])
+        assert abs(ks[0]) < 1e-12
+        assert abs(ks[1] - (-3 * s * inv_dx[1] ** 2)) < 1e-6 * abs(3 * s * inv_dx[1] ** 2), ks
+    finally:
+        orc.lib().orc_fd64_float_coefficients(0)
