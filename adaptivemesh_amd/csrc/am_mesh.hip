@@ -261,6 +261,179 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
     }
 }
 
+// Tricubic coarse-face fill, stencils shared between fine cells.  Along
+// each axis the fine ghost cells of a face map onto the target's clamped
+// stencil bases ri (AMR.h:736-745 positions); fine cells with the same base
+// on every axis -- usually 2 x 2 x 2 children of one coarse cell -- read the
+// same 4^3 nodes.  One thread gathers those nodes once per time level and
+// contracts them along x, then y, then z for each child's weights, which is
+// the same sequence of operations interp_tricubic performs for each child
+// alone, so every value is bit-identical to the per-cell fill while the
+// gathers drop about five-fold.  Where clamping at tile edges gives more
+// than 2 fine cells along an axis the same base, they are split into groups
+// of at most 2, so every thread's work stays 2 x 2 x 2 children at most.
+template <int S, int P, int VPG>
+__global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_vars, const uint64_t *__restrict__ bnd,
+                                                        const int32_t *__restrict__ tiles, uint32_t substep,
+                                                        uint32_t maxd, float frac, int equal_mode)
+{
+    constexpr int T = S + 2 * P, NC = S * S * P;
+    constexpr size_t CELLS = (size_t)T * T * T;
+    constexpr uint32_t uP = P, uS = S;
+    const int ti = blockIdx.x / 6, dir = blockIdx.x - ti * 6;
+    const int v0 = blockIdx.y * VPG;
+    const int node = tiles[ti];
+    const uint64_t bb = bnd[(size_t)node * 6 + dir];
+    const uint32_t nd = (uint32_t)bb;
+    const int tgt = bnd_target(nd), cd = bnd_depth(nd);
+    if (tgt == -1 || cd <= 0) return;
+    if (!active_depth(substep, (uint32_t)cd, maxd)) return;
+    const float3 relp = cf3_unpack((uint32_t)(bb >> 32));
+    const float rl[3] = {relp.x, relp.y, relp.z};
+    const uint32_t ddiff = bnd_ddiff(nd);
+    if (ddiff == 0 && equal_mode == AM_EQUAL_SKIP) return;
+    const float rel_scale = 1.f / (float)(1u << ddiff);
+    const float tmp = (float)((substep >> (maxd - (uint32_t)cd)) & ((1u << ddiff) - 1u));
+    const float ty = ddiff == 0 ? 1.f : (tmp + frac) / (float)(1u << ddiff);
+    const float invS = 1.f / (float)S;
+    const int nv = min(VPG, n_vars - v0);
+    const int na = dir % 3;  // the face's normal axis
+    int cnt[3], wlo[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        cnt[k] = k == na ? P : S;
+        wlo[k] = k != na ? P : (dir < 3 ? P + S : 0);
+    }
+
+    if (ddiff == 0) {
+        // equal depth: the read position is an exact target cell
+        for (int c = threadIdx.x; c < NC; c += 256) {
+            int ix[3];
+            ix[0] = c % cnt[0];
+            ix[1] = (c / cnt[0]) % cnt[1];
+            ix[2] = c / (cnt[0] * cnt[1]);
+            float rp[3];
+            uint32_t w[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                w[k] = (uint32_t)(wlo[k] + ix[k]);
+                rp[k] = (float)w[k];
+                rp[k] = (rp[k] - (float)uP + .5f) * invS - .5f;
+                rp[k] = rp[k] * rel_scale + rl[k];
+                rp[k] = (rp[k] + .5f) * (float)uS + (float)uP - 0.5f;
+            }
+            const size_t wn = (size_t)node * CELLS + w[0] + ((size_t)w[1] + (size_t)w[2] * T) * T;
+            const size_t src = (size_t)tgt * CELLS + (size_t)rp[0] + ((size_t)rp[1] + (size_t)rp[2] * T) * T;
+            for (int v = 0; v < nv; v++) b.new_[v0 + v][wn] = b.new_[v0 + v][src];
+        }
+        return;
+    }
+
+    // per axis and fine index: the clamped base and the 4-point weights
+    __shared__ uint32_t s_ri[3][S];
+    __shared__ float s_w[3][S][4];
+    __shared__ int s_run0[3][S], s_runn[3][S], s_nruns[3];
+    if (threadIdx.x < 3 * S) {
+        const int k = threadIdx.x / S, i = threadIdx.x - k * S;
+        if (i < cnt[k]) {
+            float rp = (float)(uint32_t)(wlo[k] + i);
+            rp = (rp - (float)uP + .5f) * invS - .5f;
+            rp = rp * rel_scale + rl[k];
+            rp = (rp + .5f) * (float)uS + (float)uP - 0.5f;
+            const uint32_t ri = (tgt == 0) ? min((uint32_t)fmaxf(rp, 1.f), (uint32_t)T - 3u)
+                                           : min((uint32_t)fmaxf(rp, (float)(uP + 1u)), (uint32_t)T - uP - 3u);
+            rp -= (float)ri;
+            lag4(rp, s_w[k][i]);
+            s_ri[k][i] = ri;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {  // runs of at most 2 equal bases (the positions rise with the index)
+        const int k = threadIdx.x;
+        int nr = 0;
+        for (int i = 0; i < cnt[k]; i++) {
+            if (i == 0 || s_ri[k][i] != s_ri[k][i - 1] || s_runn[k][nr - 1] == 2) {
+                s_run0[k][nr] = i;
+                s_runn[k][nr] = 1;
+                nr++;
+            } else {
+                s_runn[k][nr - 1]++;
+            }
+        }
+        s_nruns[k] = nr;
+    }
+    __syncthreads();
+    const int nr0 = s_nruns[0], nr1 = s_nruns[1], nr2 = s_nruns[2];
+    const int items = nr0 * nr1 * nr2;
+    const bool quad = ddiff > 0;  // with older: quadratic in time
+    const float c0 = .5f * ty * (ty - 1.f), c1 = (1.f - ty) * (1.f + ty), c2 = .5f * ty * (ty + 1.f);
+    constexpr int T2 = T * T;
+    for (int job = threadIdx.x; job < items * nv; job += 256) {
+        const int v = job / items, it = job - v * items;
+        const int q0 = it % nr0, q1 = (it / nr0) % nr1, q2 = it / (nr0 * nr1);
+        const int a0 = s_run0[0][q0], a1 = s_run0[1][q1], a2 = s_run0[2][q2];
+        const int n0 = s_runn[0][q0], n1 = s_runn[1][q1], n2 = s_runn[2][q2];
+        const size_t rn = (size_t)tgt * CELLS + s_ri[0][a0] + ((size_t)s_ri[1][a1] + (size_t)s_ri[2][a2] * T) * T;
+        const float *dold = b.old_[v0 + v], *dnew = b.new_[v0 + v], *dolder = b.older[v0 + v];
+        const bool three = dolder && quad;
+        float wx[2][4], wy[2][4], wz[2][4];
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                wx[c][m] = s_w[0][a0 + min(c, n0 - 1)][m];
+                wy[c][m] = s_w[1][a1 + min(c, n1 - 1)][m];
+                wz[c][m] = s_w[2][a2 + min(c, n2 - 1)][m];
+            }
+        float res[3][2][2][2];  // time level (older, old, new) x child
+#pragma unroll
+        for (int lv = 0; lv < 3; lv++) {
+            if (lv == 0 && !three) continue;
+            const float *d = lv == 0 ? dolder : (lv == 1 ? dold : dnew);
+            float ax[2][4][4];  // child x, k, j
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float *r = d + rn + (ptrdiff_t)(k - 1) * T2 + (ptrdiff_t)(j - 1) * T - 1;
+                    const float r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+#pragma unroll
+                    for (int cx = 0; cx < 2; cx++)
+                        ax[cx][k][j] = r0 * wx[cx][0] + r1 * wx[cx][1] + r2 * wx[cx][2] + r3 * wx[cx][3];
+                }
+#pragma unroll
+            for (int cx = 0; cx < 2; cx++)
+#pragma unroll
+                for (int cy = 0; cy < 2; cy++) {
+                    float accz[2] = {0.f, 0.f};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        float ay = 0.f;
+#pragma unroll
+                        for (int j = 0; j < 4; j++) ay = ay + ax[cx][k][j] * wy[cy][j];
+#pragma unroll
+                        for (int cz = 0; cz < 2; cz++) accz[cz] = accz[cz] + ay * wz[cz][k];
+                    }
+                    res[lv][cx][cy][0] = accz[0];
+                    res[lv][cx][cy][1] = accz[1];
+                }
+        }
+#pragma unroll
+        for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+            for (int cy = 0; cy < 2; cy++)
+#pragma unroll
+                for (int cx = 0; cx < 2; cx++) {
+                    if (cx >= n0 || cy >= n1 || cz >= n2) continue;
+                    const float o = res[1][cx][cy][cz], nw = res[2][cx][cy][cz];
+                    const float val = three ? res[0][cx][cy][cz] * c0 + o * c1 + nw * c2 : o * (1.f - ty) + nw * ty;
+                    const size_t wn = (size_t)node * CELLS + (uint32_t)(wlo[0] + a0 + cx) +
+                                      ((size_t)(uint32_t)(wlo[1] + a1 + cy) + (size_t)(uint32_t)(wlo[2] + a2 + cz) * T) * T;
+                    b.new_[v0 + v][wn] = val;
+                }
+    }
+}
+
 constexpr int GHOST_VPG = 6;  // fields per workgroup in the multi-field fill
 
 template <int S, int P>
@@ -284,6 +457,9 @@ static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, int 
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRILINEAR, GHOST_VPG>), grid, dim3(256), 0, st, d,
                            b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac,
                            equal_mode, skip_learned);
+    else if (interp == AM_INTERP_TRICUBIC && !skip_learned)
+        hipLaunchKernelGGL((k_ghost_tricubic<S, P, GHOST_VPG>), grid, dim3(256), 0, st, d, b->n_vars,
+                           mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, equal_mode);
     else if (interp == AM_INTERP_TRICUBIC)
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRICUBIC, GHOST_VPG>), grid, dim3(256), 0, st, d,
                            b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac,
