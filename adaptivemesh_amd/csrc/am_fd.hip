@@ -130,22 +130,25 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
         const size_t o = ((size_t)slot * S + z) * (S * S) + (size_t)tid;
 
         if (want_d) {
-            float vx = 0.f, vy = 0.f, vz = 0.f, xx = 0.f, yy = 0.f, zz = 0.f;
+            float vx = 0.f, vy = 0.f, vz = 0.f;
 #pragma unroll
             for (int i = -Q; i <= Q; i++) {
-                float a = ctr[c + i];
-                vx = vx + a * C.d1[BASE + i];
-                xx = xx + a * C.d2[BASE + i];
-                a = ctr[c + i * T];
-                vy = vy + a * C.d1[BASE + i];
-                yy = yy + a * C.d2[BASE + i];
-                a = pl[P + i][c];
-                vz = vz + a * C.d1[BASE + i];
-                zz = zz + a * C.d2[BASE + i];
+                vx = vx + ctr[c + i] * C.d1[BASE + i];
+                vy = vy + ctr[c + i * T] * C.d1[BASE + i];
+                vz = vz + pl[P + i][c] * C.d1[BASE + i];
             }
             vx = vx * dx1; vy = vy * dx1; vz = vz * dx1;
             int32_t rng = thread_seed_z(sxy_d, gz);
-            if (o_d2) word_put<FMT>(o_d2, o, xx * dx2, yy * dx2, zz * dx2, rng);
+            if (o_d2) {  // uniform: the second-derivative sums only where they are stored
+                float xx = 0.f, yy = 0.f, zz = 0.f;
+#pragma unroll
+                for (int i = -Q; i <= Q; i++) {
+                    xx = xx + ctr[c + i] * C.d2[BASE + i];
+                    yy = yy + ctr[c + i * T] * C.d2[BASE + i];
+                    zz = zz + pl[P + i][c] * C.d2[BASE + i];
+                }
+                word_put<FMT>(o_d2, o, xx * dx2, yy * dx2, zz * dx2, rng);
+            }
             word_put<FMT>(o_d1, o, vx, vy, vz, rng);
         }
         if (want_m) {
