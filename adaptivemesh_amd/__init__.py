@@ -165,6 +165,7 @@ _SIGS = {
     "am_copy_new_to_old": (C.c_int, [_P, _P, C.POINTER(Mesh), C.POINTER(Geometry), _P]),
     "am_copy_new_to_old_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), _P]),
     "am_copy_new_to_old_interior": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), _P, C.POINTER(Geometry), _P]),
+    "am_amr_criterion": (C.c_int, [_P, _P, C.POINTER(Mesh), C.POINTER(Geometry), _P, _P]),
     "am_copy_to_child": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int32, C.POINTER(Geometry), C.c_int32, _P]),
     "am_copy_to_child_batched": (C.c_int, [C.POINTER(FieldBatch), C.c_uint32, C.c_int32, C.c_int32,
                                            C.POINTER(Geometry), C.c_int32, _P]),
@@ -191,6 +192,8 @@ _SIGS = {
     "am_wave_sim_copy_bounds": (C.c_int, [_P]),
     "am_wave_sim_timestep": (C.c_int, [_P]),
     "am_wave_sim_rng_state": (C.c_int32, [_P]),
+    "am_wave_sim_set_regrid": (C.c_int, [_P, C.c_int32]),
+    "am_wave_sim_pressure": (C.c_int, [_P, _P]),
     "am_ghost_fill_stage_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), C.c_float,
                                               C.c_int32, C.c_int32, _P]),
     "am_bssn_rhs_stage": (C.c_int, [C.POINTER(BssnFields), C.c_int32, C.POINTER(BssnParams), C.POINTER(Mesh),
@@ -385,6 +388,16 @@ class WaveSim:
 
     def timestep(self):
         check(lib().am_wave_sim_timestep(self.h), "timestep")
+
+    def set_regrid(self, on: bool):
+        """modify_nodes on (the default, as in the reference) or off."""
+        check(lib().am_wave_sim_set_regrid(self.h, 1 if on else 0), "set_regrid")
+
+    def pressure(self) -> tuple:
+        """The two average refinement pressures of the last modify_nodes."""
+        out = np.zeros(2, np.float32)
+        check(lib().am_wave_sim_pressure(self.h, out.ctypes.data), "pressure")
+        return float(out[0]), float(out[1])
 
     @property
     def rng_state(self) -> int:

@@ -355,6 +355,19 @@ int am_wave_sim_timestep(am_wave_sim *s)
     });
 }
 int32_t am_wave_sim_rng_state(const am_wave_sim *s) { return s ? s->amr->sim_data.rng.seed : 0; }
+int am_wave_sim_set_regrid(am_wave_sim *s, int32_t on)
+{
+    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
+    s->amr->sim_data.regrid = on != 0;
+    return AM_OK;
+}
+int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
+{
+    if (!s || !out2) return am::set_error(AM_ERR_INVALID, "null argument");
+    out2[0] = s->amr->sim_data.last_pressure[0];
+    out2[1] = s->amr->sim_data.last_pressure[1];
+    return AM_OK;
+}
 
 // ----------------------------------------------------- RK4 driver exports
 #define AM_RK4_SIM_EXPORTS(P, H)                                                                                     \

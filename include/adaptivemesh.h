@@ -243,6 +243,15 @@ int am_copy_to_child_batched(const am_field_batch *b, uint32_t offset, int32_t c
 
 /* ------------------------------------------------------ SimpleWave */
 
+/* Refinement criterion of the SimpleWave plugin (replaces __amr_criterion_1
+ * and __amr_criterion_2, SimpleWave.h:91-144): for each listed tile and
+ * octant o = x + 2y + 4z, out[slot * 8 + o] = the largest over the octant's
+ * cells of max(|d1| outer_dx / 2^d, |d2diag| outer_dx^2 / 4^d), from the
+ * decoded gradient and diagonal-Hessian words; NaN for a slot that is unused
+ * or not active at mesh->substep.  out: device, 8 floats per slot. */
+int am_amr_criterion(const uint32_t *d1, const uint32_t *d2diag, const am_mesh *mesh, const am_geometry *g,
+                     float *out, void *stream);
+
 /* Heun stages of u_t = nu lap u - u u_x, nu = 0.01.  Replaces
  * __predictor_step / __corrector_step SimpleWave.h:7-46 and
  * wave_equation_predictor / _corrector SimpleWave.h:48-72. */

@@ -53,6 +53,12 @@ int am_wave_sim_get_field(am_wave_sim *s, int32_t which, float *host);
 int am_wave_sim_copy_bounds(am_wave_sim *s);
 int am_wave_sim_timestep(am_wave_sim *s);
 int32_t am_wave_sim_rng_state(const am_wave_sim *s);
+/* modify_nodes on (1, the default, as in the reference) or off (the mesh
+ * stays as built) */
+int am_wave_sim_set_regrid(am_wave_sim *s, int32_t on);
+/* the two average refinement pressures of the last modify_nodes (per-slot
+ * maxima; octants / 8), as amr_criterion returns them (SimpleWave.h:183) */
+int am_wave_sim_pressure(const am_wave_sim *s, float *out2);
 
 /* ------------------------------------------- RK4 driver launch machinery
  * Shared by am_bssn_sim and am_swave_sim (no counterpart in the reference):
