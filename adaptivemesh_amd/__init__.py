@@ -26,7 +26,7 @@ AM_EQUAL_FORMULA, AM_EQUAL_COPY, AM_EQUAL_SKIP = 0, 1, 2
 ERRORS = {
     -1: "AM_ERR_INVALID", -2: "AM_ERR_UNSUPPORTED_GEOMETRY", -3: "AM_ERR_SLOTS_EXHAUSTED",
     -4: "AM_ERR_DEPTH_LIMIT", -5: "AM_ERR_OCCUPIED", -6: "AM_ERR_MISALIGNED",
-    -7: "AM_ERR_TOO_MANY_VARS", -8: "AM_ERR_ALLOC", -9: "AM_ERR_LAUNCH",
+    -7: "AM_ERR_TOO_MANY_VARS", -8: "AM_ERR_ALLOC", -9: "AM_ERR_LAUNCH", -10: "AM_ERR_IO", -11: "AM_ERR_FORMAT",
 }
 
 
@@ -194,6 +194,9 @@ _SIGS = {
     "am_wave_sim_rng_state": (C.c_int32, [_P]),
     "am_wave_sim_set_regrid": (C.c_int, [_P, C.c_int32]),
     "am_wave_sim_pressure": (C.c_int, [_P, _P]),
+    "am_wave_sim_save": (C.c_int, [_P, C.c_char_p]),
+    "am_checkpoint_inspect": (C.c_int, [C.c_char_p, _P]),
+    "am_wave_sim_load": (C.c_int, [_P, C.c_char_p]),
     "am_ghost_fill_stage_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), C.c_float,
                                               C.c_int32, C.c_int32, _P]),
     "am_bssn_rhs_stage": (C.c_int, [C.POINTER(BssnFields), C.c_int32, C.POINTER(BssnParams), C.POINTER(Mesh),
@@ -242,6 +245,8 @@ _RK4_SIM_SIGS = {
     "autotune": (C.c_int, [_P, C.c_int32, _P]),
     "set_prolongation": (C.c_int, [_P, _P, C.c_float]),
     "prolongation_stats": (C.c_int, [_P, _P]),
+    "save": (C.c_int, [_P, C.c_char_p]),
+    "load": (C.c_int, [_P, C.c_char_p]),
 }
 for _prefix in ("am_bssn_sim", "am_swave_sim"):
     for _k, _v in _RK4_SIM_SIGS.items():
@@ -280,6 +285,21 @@ def check(rc: int, what: str = "") -> int:
     if rc != 0:
         raise AMError(rc, f"{what}: {lib().am_last_error_string().decode()}")
     return rc
+
+
+class CheckpointHeader(C.Structure):
+    """am_checkpoint_header (adaptivemesh_host.h)."""
+    _fields_ = [("magic", C.c_char * 8), ("kind", C.c_uint32), ("geometry", Geometry), ("max_slots", C.c_uint32),
+                ("periodic", C.c_uint32), ("max_depth", C.c_int32), ("first_free_slot", C.c_uint32),
+                ("final_used_slot", C.c_uint32), ("timer", C.c_uint32), ("seed", C.c_int32), ("n_vars", C.c_uint32),
+                ("flags", C.c_uint32), ("n_sections", C.c_uint32)]
+
+
+def inspect_checkpoint(path) -> CheckpointHeader:
+    """Check a checkpoint file natively (no GPU) and return its header."""
+    h = CheckpointHeader()
+    check(lib().am_checkpoint_inspect(os.fsencode(path), C.byref(h)), "am_checkpoint_inspect")
+    return h
 
 
 def geometry(S: int = 16, P: int = 3, fd_order: int = 0, outer_size: float = 100.0, cfl: float = 0.4) -> Geometry:
@@ -399,6 +419,14 @@ class WaveSim:
         check(lib().am_wave_sim_pressure(self.h, out.ctypes.data), "pressure")
         return float(out[0]), float(out[1])
 
+    def save(self, path):
+        """Checkpoint: octree, counter, seed, fields and derivative words."""
+        check(lib().am_wave_sim_save(self.h, os.fsencode(path)), "save")
+
+    def load(self, path):
+        """Resume from a checkpoint of a driver built alike."""
+        check(lib().am_wave_sim_load(self.h, os.fsencode(path)), "load")
+
     @property
     def rng_state(self) -> int:
         return lib().am_wave_sim_rng_state(self.h)
@@ -466,6 +494,14 @@ class _Rk4Sim:
         a = np.empty(self.max_slots * self.geom.T ** 3, np.float32)
         check(self._f("get_field")(self.h, which, var, a.ctypes.data), "get_field")
         return a
+
+    def save(self, path):
+        """Checkpoint: octree, counter, seed and the old/new(/older) fields."""
+        check(self._f("save")(self.h, os.fsencode(path)), "save")
+
+    def load(self, path):
+        """Resume from a checkpoint of a driver built alike."""
+        check(self._f("load")(self.h, os.fsencode(path)), "load")
 
     def device_ptr(self, which: int) -> int:
         return self._f("device_ptr")(self.h, which)

@@ -38,6 +38,8 @@ int guarded(F &&f)
 {
     try {
         return f();
+    } catch (const am::checkpoint_error &e) {
+        return am::set_error(e.code, "%s", e.what());
     } catch (const std::exception &e) {
         return am::set_error(AM_ERR_INVALID, "%s", e.what());
     }
@@ -361,6 +363,31 @@ int am_wave_sim_set_regrid(am_wave_sim *s, int32_t on)
     s->amr->sim_data.regrid = on != 0;
     return AM_OK;
 }
+int am_checkpoint_inspect(const char *path, am_checkpoint_header *out)
+{
+    if (!path) return am::set_error(AM_ERR_INVALID, "null path");
+    return guarded([&] {
+        const am_checkpoint_header h = inspect_checkpoint(path);
+        if (out) *out = h;
+        return AM_OK;
+    });
+}
+int am_wave_sim_save(am_wave_sim *s, const char *path)
+{
+    if (!s || !path) return am::set_error(AM_ERR_INVALID, "null argument");
+    return guarded([&] {
+        save_checkpoint(*s->amr, path);
+        return AM_OK;
+    });
+}
+int am_wave_sim_load(am_wave_sim *s, const char *path)
+{
+    if (!s || !path) return am::set_error(AM_ERR_INVALID, "null argument");
+    return guarded([&] {
+        load_checkpoint(*s->amr, path);
+        return AM_OK;
+    });
+}
 int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
 {
     if (!s || !out2) return am::set_error(AM_ERR_INVALID, "null argument");
@@ -471,6 +498,16 @@ int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
     int P##_set_prolongation(H *s, const float *params, float threshold)                                           \
     {                                                                                                              \
         return rk4_call(s, [&](auto &d) { d.set_prolongation(params, threshold); });                               \
+    }                                                                                                              \
+    int P##_save(H *s, const char *path)                                                                           \
+    {                                                                                                              \
+        if (!path) return am::set_error(AM_ERR_INVALID, "null path");                                              \
+        return rk4_call(s, [&](auto &d) { save_checkpoint(d.parent, path); });                                     \
+    }                                                                                                              \
+    int P##_load(H *s, const char *path)                                                                           \
+    {                                                                                                              \
+        if (!path) return am::set_error(AM_ERR_INVALID, "null path");                                              \
+        return rk4_call(s, [&](auto &d) { load_checkpoint(d.parent, path); });                                     \
     }                                                                                                              \
     int P##_prolongation_stats(H *s, uint64_t *out4)                                                               \
     {                                                                                                              \

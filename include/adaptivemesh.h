@@ -46,7 +46,9 @@ enum am_status {
     AM_ERR_MISALIGNED = -6,           /* field pointer not 16-byte aligned */
     AM_ERR_TOO_MANY_VARS = -7,        /* more than AM_MAX_VARS in one batch */
     AM_ERR_ALLOC = -8,                /* device allocation failed */
-    AM_ERR_LAUNCH = -9                /* kernel launch failed */
+    AM_ERR_LAUNCH = -9,               /* kernel launch failed */
+    AM_ERR_IO = -10,                  /* a file could not be opened, read or written */
+    AM_ERR_FORMAT = -11               /* a checkpoint file is damaged or does not fit the driver */
 };
 
 enum am_interp {

@@ -188,6 +188,84 @@ struct octree {
     bool is_dirty() const { return dirty; }
     uint64_t tree_version() const { return version; }
 
+    // A whole tree state from outside (a checkpoint file): check_snapshot
+    // returns nullptr when it is a tree this class could have built, else
+    // what is wrong; restore installs a checked one.
+    struct snapshot {
+        const int32_t *parent;
+        const int32_t *children;
+        const uint32_t *positions;
+        const uint64_t *boundaries;
+        const uint32_t *lifetime;
+        int max_depth;
+        uint32_t first_free, final_used, timer;
+    };
+    const char *check_snapshot(const snapshot &s) const
+    {
+        const uint32_t n = max_slots;
+        if (s.final_used >= n || s.first_free > n || s.first_free == 0) return "slot counters out of range";
+        if (s.positions[0] != 0u) return "slot 0 is not the root";
+        int deepest = 0;
+        uint32_t first_empty = n;
+        for (uint32_t i = 0; i < n; i++) {
+            octree_abs_pos p;
+            p.data = s.positions[i];
+            const int d = p.depth();
+            if (d == -1) {
+                if (first_empty == n) first_empty = i;
+                if (s.parent[i] != -1) return "an empty slot has a parent";
+                for (int c = 0; c < 8; c++)
+                    if (s.children[i * 8u + c] != -1) return "an empty slot has children";
+                continue;
+            }
+            if (i > s.final_used) return "a used slot lies beyond final_used";
+            if (d > (int)depth_limit) return "depth beyond the limit";
+            deepest = std::max(deepest, d);
+            for (int c = 0; c < 8; c++) {
+                const int ch = s.children[i * 8u + c];
+                if (ch == -1) continue;
+                if (ch <= 0 || (uint32_t)ch >= n || s.parent[ch] != (int32_t)i) return "child and parent disagree";
+                octree_abs_pos q;
+                q.data = s.positions[ch];
+                if (q.depth() != d + 1 || q.final_offset() != c) return "child position disagrees with its octant";
+            }
+            if (i > 0) {
+                const int par = s.parent[i];
+                if (par < 0 || (uint32_t)par >= n) return "parent out of range";
+                octree_abs_pos q;
+                q.data = s.positions[par];
+                if (q.depth() != d - 1 || s.children[(uint32_t)par * 8u + (uint32_t)p.final_offset()] != (int32_t)i)
+                    return "parent does not list the slot as its child";
+                for (int f = 0; f < 6; f++) {
+                    domain_boundary b;
+                    memcpy(&b, &s.boundaries[i * 6u + f], sizeof b);
+                    if (b.target_idx() < 0 || (uint32_t)b.target_idx() >= n) return "boundary target out of range";
+                }
+            }
+        }
+        if (s.final_used > 0 && octree_abs_pos{s.positions[s.final_used]}.depth() == -1)
+            return "final_used is an empty slot";
+        if (s.first_free != first_empty) return "first_free is not the first empty slot";
+        if (s.max_depth != deepest) return "max_depth differs from the deepest slot";
+        if (s.timer >= (1u << (uint32_t)s.max_depth)) return "counter out of range";
+        return nullptr;
+    }
+    void restore(const snapshot &s)
+    {
+        const size_t n = max_slots;
+        memcpy(parent_idx_b.cpu_buffer_ptr, s.parent, n * sizeof(int32_t));
+        memcpy(children_idx_b.cpu_buffer_ptr, s.children, 8 * n * sizeof(int32_t));
+        memcpy(positions.cpu_buffer_ptr, s.positions, n * sizeof(uint32_t));
+        memcpy(boundaries.cpu_buffer_ptr, s.boundaries, 6 * n * sizeof(uint64_t));
+        memcpy(lifetime.cpu_buffer_ptr, s.lifetime, n * sizeof(uint32_t));
+        max_depth = s.max_depth;
+        first_free_slot = s.first_free;
+        final_used_slot = s.final_used;
+        timer_helper = s.timer;
+        dirty = true;
+        version++;
+    }
+
     // AMR::increment_timer (AMR.h:180-188)
     void increment_timer()
     {

@@ -120,6 +120,7 @@ struct bssn_system {
     static constexpr int FD_KIND[NV] = {3, 3, 3, 3, 3, 3, 3, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3, 3, 3, 1, 1, 1};
     static constexpr bool CHRISTOFFEL = false;  // raised in registers by the RHS kernel
     static constexpr bool WORD_FORMATS = true;  // fp32 / fp16 words for S=16, P=3 (precision study)
+    static constexpr uint32_t CKPT_KIND = AM_CKPT_BSSN;
     using options_t = am_bssn_params;
 
     static am_bssn_fields fields(const rk4_views &v)

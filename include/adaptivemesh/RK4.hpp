@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "AMR.hpp"
+#include "checkpoint.hpp"
 
 namespace am {
 
@@ -261,6 +262,36 @@ struct rk4_AMR_data {
     }
     void node_dtor(int) {}
     void modify_nodes() {}
+
+    // checkpoint sections (checkpoint.hpp): old, new and older of the used
+    // slots, variable-major
+    static constexpr uint32_t CKPT_KIND = System::CKPT_KIND;
+    uint32_t ckpt_vars() const { return (uint32_t)NV; }
+    uint32_t ckpt_flags() const { return (uint32_t)opt.interp | ((uint32_t)opt.word_format << 4); }
+    uint32_t ckpt_sections() const { return older.created ? 3u : 2u; }
+    void ckpt_save(checkpoint_out &o)
+    {
+        const size_t fb = parent.curr_used_slots() * cells * sizeof(float);
+        smart_gpu_buffer<float> *bufs[3] = {&old_, &new_, &older};
+        const char *names[3] = {"old", "new", "older"};
+        for (uint32_t k = 0; k < ckpt_sections(); k++) {
+            std::vector<std::pair<const void *, size_t>> parts;
+            for (int v = 0; v < NV; v++) parts.push_back({field(*bufs[k], v), fb});
+            o.device(names[k], parts, stream);
+        }
+    }
+    void ckpt_load(checkpoint_in &in, uint32_t used, bool apply)
+    {
+        const size_t fb = used * cells * sizeof(float);
+        smart_gpu_buffer<float> *bufs[3] = {&old_, &new_, &older};
+        const char *names[3] = {"old", "new", "older"};
+        for (uint32_t k = 0; k < ckpt_sections(); k++) {
+            const char *src = in.take(names[k], NV * fb);
+            if (apply)
+                for (int v = 0; v < NV; v++) upload(field(*bufs[k], v), src + v * fb, fb, stream);
+        }
+        if (apply) drop_graph();
+    }
 
     // the next dither seed: a host draw, or while recording the index of the
     // draw in the seed table

@@ -22,6 +22,7 @@ struct swave_system {
     static constexpr int FD_KIND[NV] = {2, 0};
     static constexpr bool CHRISTOFFEL = false;
     static constexpr bool WORD_FORMATS = false;  // compressed_float3 words only
+    static constexpr uint32_t CKPT_KIND = AM_CKPT_SWAVE;
     using options_t = swave_options;
 
     static am_swave_fields fields(const rk4_views &v)

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "AMR.hpp"
+#include "checkpoint.hpp"
 
 namespace am {
 
@@ -66,6 +67,34 @@ struct wave_AMR_data {
         check(copy_to_child(new_field, off, node_idx, par, geom, stream), "node_ctor");
     }
     void node_dtor(int) {}
+
+    // checkpoint sections (checkpoint.hpp): old and new of the used slots,
+    // the derivative words of every slot
+    static constexpr uint32_t CKPT_KIND = AM_CKPT_WAVE;
+    uint32_t ckpt_vars() const { return 1u; }
+    uint32_t ckpt_flags() const { return regrid ? 1u : 0u; }
+    uint32_t ckpt_sections() const { return 4u; }
+    void ckpt_save(checkpoint_out &o)
+    {
+        const size_t fb = parent.curr_used_slots() * cells(geom) * sizeof(float);
+        const size_t wb = parent.max_slots * inner(geom) * sizeof(uint32_t);
+        o.device("old", {{old_field.gpu_buffer_ptr, fb}}, stream);
+        o.device("new", {{new_field.gpu_buffer_ptr, fb}}, stream);
+        o.device("d1", {{first_derivs.gpu_buffer_ptr, wb}}, stream);
+        o.device("d2diag", {{hessian_diag.gpu_buffer_ptr, wb}}, stream);
+    }
+    void ckpt_load(checkpoint_in &in, uint32_t used, bool apply)
+    {
+        const size_t fb = used * cells(geom) * sizeof(float);
+        const size_t wb = parent.max_slots * inner(geom) * sizeof(uint32_t);
+        const char *o = in.take("old", fb), *n = in.take("new", fb);
+        const char *w1 = in.take("d1", wb), *w2 = in.take("d2diag", wb);
+        if (!apply) return;
+        upload(old_field.gpu_buffer_ptr, o, fb, stream);
+        upload(new_field.gpu_buffer_ptr, n, fb, stream);
+        upload(first_derivs.gpu_buffer_ptr, w1, wb, stream);
+        upload(hessian_diag.gpu_buffer_ptr, w2, wb, stream);
+    }
 
     static float clampf(float f, float a, float b) { return fmaxf(a, fminf(f, b)); }
     // amr_criterion (SimpleWave.h:146-187): octant values of every used slot

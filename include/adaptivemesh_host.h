@@ -37,6 +37,52 @@ int am_tree_export(const am_tree *t, int32_t *parent, int32_t *children, uint32_
  * for direct kernel calls.  Needs a GPU. */
 int am_tree_mesh(am_tree *t, int32_t depth, am_mesh *out);
 
+/* ------------------------------------------------------------ checkpoints
+ * Every driver saves and loads its whole state: the octree, its counter, the
+ * host seed and every array its next timestep reads, so that a resumed run
+ * continues bit for bit.  The reference only has raw buffer dumps
+ * (io_helper.h:9-40, write_to_file / read_from_file); these files add a
+ * header and a tag per array.  Layout (little-endian): am_checkpoint_header,
+ * then n_sections sections of { char name[16] (NUL-padded); uint64_t bytes;
+ * bytes }, in this order:
+ *   "parent" int32[max_slots], "children" int32[8 max_slots],
+ *   "positions" uint32[max_slots], "boundaries" uint64[6 max_slots],
+ *   "lifetime" uint32[max_slots]   -- the octree arrays (AMR.h:118-132);
+ *   SimpleWave: "old", "new" float[used T^3] (used = final_used_slot + 1),
+ *     "d1", "d2diag" cf3 words uint32[max_slots S^3] (a re-used slot's old
+ *     words feed the refinement criterion until it is first differentiated);
+ *   RK4 drivers: "old", "new" and, with tricubic or learned ghosts, "older":
+ *     float[n_vars][used T^3], variable-major.
+ * A load must go to a driver created with the same geometry, max_slots,
+ * periodic flag and options encoded in `flags`; it checks the whole file
+ * (header, section names and sizes, tree consistency) before changing
+ * anything, and returns AM_ERR_FORMAT or AM_ERR_IO otherwise.  Physics
+ * options that `flags` does not encode (BSSN gauge parameters, learned
+ * prolongation weights) are the caller's to set alike.  A save writes
+ * "<path>.tmp" and renames it over <path>. */
+#define AM_CHECKPOINT_MAGIC "AMCKPT\0\1"
+enum am_checkpoint_kind { AM_CKPT_WAVE = 1, AM_CKPT_BSSN = 2, AM_CKPT_SWAVE = 3 };
+typedef struct am_checkpoint_header {
+    char magic[8];            /* AM_CHECKPOINT_MAGIC, 8 bytes */
+    uint32_t kind;            /* am_checkpoint_kind */
+    am_geometry geometry;
+    uint32_t max_slots;
+    uint32_t periodic;
+    int32_t max_depth;        /* AMR::max_depth */
+    uint32_t first_free_slot; /* AMR::first_free_slot */
+    uint32_t final_used_slot; /* AMR::final_used_slot */
+    uint32_t timer;           /* AMR::timer_helper, the sub-cycling counter */
+    int32_t seed;             /* host fast_prng state */
+    uint32_t n_vars;          /* fields per slot */
+    uint32_t flags;           /* SimpleWave: 1 = regridding on; RK4: interp | word_format << 4 */
+    uint32_t n_sections;
+} am_checkpoint_header;
+
+/* Check a checkpoint file on its own (section table, octree consistency,
+ * state sections for its kind) and copy its header to *out (may be NULL).
+ * Host only: needs no GPU. */
+int am_checkpoint_inspect(const char *path, am_checkpoint_header *out);
+
 /* ------------------------------------------------------ SimpleWave driver */
 typedef struct am_wave_sim am_wave_sim;
 
@@ -59,6 +105,8 @@ int am_wave_sim_set_regrid(am_wave_sim *s, int32_t on);
 /* the two average refinement pressures of the last modify_nodes (per-slot
  * maxima; octants / 8), as amr_criterion returns them (SimpleWave.h:183) */
 int am_wave_sim_pressure(const am_wave_sim *s, float *out2);
+int am_wave_sim_save(am_wave_sim *s, const char *path);
+int am_wave_sim_load(am_wave_sim *s, const char *path);
 
 /* ------------------------------------------- RK4 driver launch machinery
  * Shared by am_bssn_sim and am_swave_sim (no counterpart in the reference):
@@ -149,6 +197,9 @@ int am_bssn_sim_autotune(am_bssn_sim *s, int32_t reps, am_tune_report *out);
  * am_ghost_fill_learned). */
 int am_bssn_sim_set_prolongation(am_bssn_sim *s, const float *params, float threshold);
 int am_bssn_sim_prolongation_stats(am_bssn_sim *s, uint64_t *out4);
+/* checkpoints (see am_checkpoint_header) */
+int am_bssn_sim_save(am_bssn_sim *s, const char *path);
+int am_bssn_sim_load(am_bssn_sim *s, const char *path);
 
 /* -------------------------------------------- scalar wave RK4 driver
  * AMR<swave_rk4_data>: u_tt = lap u as fields (u, v = u_t) on the same RK4
@@ -195,6 +246,8 @@ int am_swave_sim_get_launch(const am_swave_sim *s, int32_t *out4);
 int am_swave_sim_set_fd_prefetch(am_swave_sim *s, int32_t planes);
 int am_swave_sim_use_graph(am_swave_sim *s, int32_t enable);
 int am_swave_sim_autotune(am_swave_sim *s, int32_t reps, am_tune_report *out);
+int am_swave_sim_save(am_swave_sim *s, const char *path);
+int am_swave_sim_load(am_swave_sim *s, const char *path);
 /* AM_INTERP_LEARNED: the MLP parameters (device, am_mlp_param_count()
  * floats, owned by the caller and kept alive while the driver uses them) and
  * the restriction-residual threshold of am_ghost_fill_learned; resets the
