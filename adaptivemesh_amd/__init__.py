@@ -196,6 +196,11 @@ _SIGS = {
     "am_wave_sim_pressure": (C.c_int, [_P, _P]),
     "am_wave_sim_save": (C.c_int, [_P, C.c_char_p]),
     "am_checkpoint_inspect": (C.c_int, [C.c_char_p, _P]),
+    "am_wave_sim_raster": (C.c_int, [_P, C.c_int32, C.c_uint32, C.c_uint32, C.c_float, C.c_int32, _P]),
+    "am_rasterise_slice": (C.c_int, [_P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_uint32, C.c_uint32, C.c_float,
+                                     C.c_int32, _P, _P]),
+    "am_encode_rgba8": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "am_write_png": (C.c_int, [C.c_char_p, _P, C.c_uint32, C.c_uint32]),
     "am_wave_sim_load": (C.c_int, [_P, C.c_char_p]),
     "am_ghost_fill_stage_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), C.c_float,
                                               C.c_int32, C.c_int32, _P]),
@@ -247,6 +252,7 @@ _RK4_SIM_SIGS = {
     "prolongation_stats": (C.c_int, [_P, _P]),
     "save": (C.c_int, [_P, C.c_char_p]),
     "load": (C.c_int, [_P, C.c_char_p]),
+    "raster": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_float, C.c_int32, _P]),
 }
 for _prefix in ("am_bssn_sim", "am_swave_sim"):
     for _k, _v in _RK4_SIM_SIGS.items():
@@ -293,6 +299,14 @@ class CheckpointHeader(C.Structure):
                 ("periodic", C.c_uint32), ("max_depth", C.c_int32), ("first_free_slot", C.c_uint32),
                 ("final_used_slot", C.c_uint32), ("timer", C.c_uint32), ("seed", C.c_int32), ("n_vars", C.c_uint32),
                 ("flags", C.c_uint32), ("n_sections", C.c_uint32)]
+
+
+def write_png(path, pixels: np.ndarray):
+    """am_write_png: a (height, width) array of RGBA8 words (R in the low byte) to a PNG file."""
+    px = np.ascontiguousarray(pixels, dtype=np.uint32)
+    if px.ndim != 2:
+        raise ValueError("pixels must be (height, width)")
+    check(lib().am_write_png(os.fsencode(path), px.ctypes.data, px.shape[1], px.shape[0]), "am_write_png")
 
 
 def inspect_checkpoint(path) -> CheckpointHeader:
@@ -427,6 +441,19 @@ class WaveSim:
         """Resume from a checkpoint of a driver built alike."""
         check(lib().am_wave_sim_load(self.h, os.fsencode(path)), "load")
 
+    def raster(self, which: int = 1, width: int = 1024, height: int = 1024, z_slice: float = 1e-9,
+               boxes: bool = True) -> np.ndarray:
+        """rasterise_to_image of old (0) or new (1) as (height, width) RGBA8 words."""
+        px = np.empty((height, width), np.uint32)
+        check(lib().am_wave_sim_raster(self.h, which, width, height, z_slice, 1 if boxes else 0, px.ctypes.data),
+              "raster")
+        return px
+
+    def save_image(self, path, which: int = 1, width: int = 1024, height: int = 1024, z_slice: float = 1e-9,
+                   boxes: bool = True):
+        """rasterise_to_image + save_image (AMR.h:1355, image_process.h:55)."""
+        write_png(path, self.raster(which, width, height, z_slice, boxes))
+
     @property
     def rng_state(self) -> int:
         return lib().am_wave_sim_rng_state(self.h)
@@ -502,6 +529,18 @@ class _Rk4Sim:
     def load(self, path):
         """Resume from a checkpoint of a driver built alike."""
         check(self._f("load")(self.h, os.fsencode(path)), "load")
+
+    def raster(self, var: int, which: int = 1, width: int = 1024, height: int = 1024, z_slice: float = 1e-9,
+               boxes: bool = True) -> np.ndarray:
+        """A z-slice of variable `var` of old (0) or new (1) as (height, width) RGBA8 words."""
+        px = np.empty((height, width), np.uint32)
+        check(self._f("raster")(self.h, which, var, width, height, z_slice, 1 if boxes else 0, px.ctypes.data),
+              "raster")
+        return px
+
+    def save_image(self, path, var: int, which: int = 1, width: int = 1024, height: int = 1024,
+                   z_slice: float = 1e-9, boxes: bool = True):
+        write_png(path, self.raster(var, which, width, height, z_slice, boxes))
 
     def device_ptr(self, which: int) -> int:
         return self._f("device_ptr")(self.h, which)

@@ -10,10 +10,18 @@
 #include "../../include/adaptivemesh/BSSN.hpp"
 #include "../../include/adaptivemesh/ScalarWave.hpp"
 #include "../../include/adaptivemesh/SimpleWave.hpp"
+#include "../../include/adaptivemesh/image.hpp"
 #include "../../include/adaptivemesh_host.h"
 #include "am_common.hpp"
 
 using namespace am;
+
+// the drop-in image templates compile against the driver types
+template void am::rasterise_to_image<AMR<wave_AMR_data>, smart_gpu_buffer<float>, smart_gpu_cpu_buffer<rgba_f32>>(
+    const smart_gpu_buffer<float> &, smart_gpu_cpu_buffer<rgba_f32> &, AMR<wave_AMR_data> &, uint32_t, uint32_t, float,
+    bool, void *);
+template void am::save_image<smart_gpu_cpu_buffer<uint32_t>, smart_gpu_cpu_buffer<rgba_f32>>(
+    smart_gpu_cpu_buffer<uint32_t> &, const smart_gpu_cpu_buffer<rgba_f32> &, uint32_t, uint32_t, const char *, void *);
 
 struct am_tree {
     AMR<blank_AMR_data> *amr;
@@ -388,6 +396,17 @@ int am_wave_sim_load(am_wave_sim *s, const char *path)
         return AM_OK;
     });
 }
+int am_wave_sim_raster(am_wave_sim *s, int32_t which, uint32_t width, uint32_t height, float z_slice, int32_t boxes,
+                       uint32_t *pixels)
+{
+    if (!s || !pixels || which < 0 || which > 1) return am::set_error(AM_ERR_INVALID, "am_wave_sim_raster");
+    return guarded([&] {
+        auto &d = s->amr->sim_data;
+        raster_to_host(*s->amr, (which ? d.new_field : d.old_field).gpu_buffer_ptr, width, height, z_slice, boxes != 0,
+                       pixels, d.stream);
+        return AM_OK;
+    });
+}
 int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
 {
     if (!s || !out2) return am::set_error(AM_ERR_INVALID, "null argument");
@@ -508,6 +527,16 @@ int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
     {                                                                                                              \
         if (!path) return am::set_error(AM_ERR_INVALID, "null path");                                              \
         return rk4_call(s, [&](auto &d) { load_checkpoint(d.parent, path); });                                     \
+    }                                                                                                              \
+    int P##_raster(H *s, int32_t which, int32_t var, uint32_t width, uint32_t height, float z_slice, int32_t boxes, \
+                   uint32_t *pixels)                                                                               \
+    {                                                                                                              \
+        if (!pixels || which < 0 || which > 1) return am::set_error(AM_ERR_INVALID, "raster: bad argument");       \
+        return rk4_call(s, [&](auto &d) {                                                                          \
+            if (var < 0 || var >= d.NV) throw std::runtime_error("raster: variable out of range");                \
+            raster_to_host(d.parent, d.field(which ? d.new_ : d.old_, var), width, height, z_slice, boxes != 0,    \
+                           pixels, d.stream);                                                                      \
+        });                                                                                                        \
     }                                                                                                              \
     int P##_prolongation_stats(H *s, uint64_t *out4)                                                               \
     {                                                                                                              \

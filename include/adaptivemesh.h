@@ -245,6 +245,22 @@ int am_copy_to_child_batched(const am_field_batch *b, uint32_t offset, int32_t c
 
 /* ------------------------------------------------------ SimpleWave */
 
+/* __rasterise_to_image (AMR.h:1313-1353): a z-slice through the whole
+ * octree as a width x height image.  Pixel (x, y) samples the point
+ * (x/width - 1/2, y/height - 1/2, z_slice) of the root cube (units of
+ * outer_size, z_slice in [-1/2, 1/2)) in the leaf holding it, with the
+ * quadratic __interpolate_dat of that leaf (AMR.h:4, 673-699), as
+ * (v + 1/2, v + 1/2, v + 1/2, 1); with boxes != 0, pixels within
+ * 2 * 2^depth / width of a tile's x or y edge are (1, 0, 0, 1) (the
+ * reference's DISPLAY_BOUNDING_BOXES).  raster: width * height float4
+ * (16-byte aligned, device), row-major.  mesh: children only. */
+int am_rasterise_slice(const float *field, const am_mesh *mesh, const am_geometry *g, uint32_t width,
+                       uint32_t height, float z_slice, int32_t boxes, float *raster, void *stream);
+/* __encode_img with ___rgba(float4) (image_process.h:10-54): n float4
+ * pixels to RGBA8, R | G << 8 | B << 16 | A << 24, each channel
+ * (uint)(clamp(c, 0, 1) * 255); a NaN channel gives 0xFFFF00FF. */
+int am_encode_rgba8(const float *raster, uint32_t *pixels, int64_t n, void *stream);
+
 /* Refinement criterion of the SimpleWave plugin (replaces __amr_criterion_1
  * and __amr_criterion_2, SimpleWave.h:91-144): for each listed tile and
  * octant o = x + 2y + 4z, out[slot * 8 + o] = the largest over the octant's

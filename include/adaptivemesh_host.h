@@ -83,6 +83,11 @@ typedef struct am_checkpoint_header {
  * Host only: needs no GPU. */
 int am_checkpoint_inspect(const char *path, am_checkpoint_header *out);
 
+/* ------------------------------------------------------------------ images
+ * lodepng_encode32_file as save_image calls it (image_process.h:55-62):
+ * width * height RGBA8 pixels (host, R in the low byte) to a PNG file. */
+int am_write_png(const char *path, const uint32_t *pixels, uint32_t width, uint32_t height);
+
 /* ------------------------------------------------------ SimpleWave driver */
 typedef struct am_wave_sim am_wave_sim;
 
@@ -106,6 +111,11 @@ int am_wave_sim_set_regrid(am_wave_sim *s, int32_t on);
  * maxima; octants / 8), as amr_criterion returns them (SimpleWave.h:183) */
 int am_wave_sim_pressure(const am_wave_sim *s, float *out2);
 int am_wave_sim_save(am_wave_sim *s, const char *path);
+/* rasterise_to_image + the encoding half of save_image (AMR.h:1355-1364,
+ * image_process.h:55-62) on field `which` (0 old, 1 new): width * height
+ * RGBA8 pixels into host memory; am_write_png stores them. */
+int am_wave_sim_raster(am_wave_sim *s, int32_t which, uint32_t width, uint32_t height, float z_slice, int32_t boxes,
+                       uint32_t *pixels);
 int am_wave_sim_load(am_wave_sim *s, const char *path);
 
 /* ------------------------------------------- RK4 driver launch machinery
@@ -200,6 +210,9 @@ int am_bssn_sim_prolongation_stats(am_bssn_sim *s, uint64_t *out4);
 /* checkpoints (see am_checkpoint_header) */
 int am_bssn_sim_save(am_bssn_sim *s, const char *path);
 int am_bssn_sim_load(am_bssn_sim *s, const char *path);
+/* as am_wave_sim_raster, on variable `var` of field `which` (0 old, 1 new) */
+int am_bssn_sim_raster(am_bssn_sim *s, int32_t which, int32_t var, uint32_t width, uint32_t height, float z_slice,
+                       int32_t boxes, uint32_t *pixels);
 
 /* -------------------------------------------- scalar wave RK4 driver
  * AMR<swave_rk4_data>: u_tt = lap u as fields (u, v = u_t) on the same RK4
@@ -248,6 +261,8 @@ int am_swave_sim_use_graph(am_swave_sim *s, int32_t enable);
 int am_swave_sim_autotune(am_swave_sim *s, int32_t reps, am_tune_report *out);
 int am_swave_sim_save(am_swave_sim *s, const char *path);
 int am_swave_sim_load(am_swave_sim *s, const char *path);
+int am_swave_sim_raster(am_swave_sim *s, int32_t which, int32_t var, uint32_t width, uint32_t height, float z_slice,
+                        int32_t boxes, uint32_t *pixels);
 /* AM_INTERP_LEARNED: the MLP parameters (device, am_mlp_param_count()
  * floats, owned by the caller and kept alive while the driver uses them) and
  * the restriction-residual threshold of am_ghost_fill_learned; resets the

@@ -30,6 +30,8 @@
  *   __predictor/__corrector    AdaptiveMesh/SimpleWave.h:7-46
  *   __deriv_trace              AdaptiveMesh/BSSN.h:7-22
  *   __compute_raised_christoffel_symbols  AdaptiveMesh/BSSN.h:55-91
+ *   __rasterise_to_image       AdaptiveMesh/AMR.h:1313-1353
+ *   ___rgba(float4)            AdaptiveMesh/image_process.h:14-21
  *
  * Extensions that the reference does not have, restated here so the product's
  * versions can be checked: FD order decoupled from the padding (the
@@ -921,4 +923,63 @@ EXPORT void orc_christoffel(const float *const *metric, const uint32_t *const *d
                     }
                 }
         }
+}
+
+/* __rasterise_to_image (AMR.h:1313-1353) with the quadratic __interpolate_dat
+ * that AMR.h:4 selects and, when boxes, the tile-edge boxes of
+ * DISPLAY_BOUNDING_BOXES.  out: width*height float4, row-major.  Index
+ * arithmetic in uint32 as the reference writes it. */
+EXPORT void orc_rasterise(const float *data, const int32_t *children, uint32_t width, uint32_t height, int S, int P,
+                          float z_slice, int boxes, float *out)
+{
+    const uint32_t T = (uint32_t)(S + 2 * P);
+    for (uint32_t py = 0; py < height; py++)
+        for (uint32_t px = 0; px < width; px++) {
+            float t[3] = {(float)px / (float)width - .5f, (float)py / (float)height - .5f, z_slice};
+            uint32_t cur = 0, off = (t[0] > 0.f) | ((t[1] > 0.f) << 1) | ((t[2] > 0.f) << 2);
+            int child = children[cur * 8u + off];
+            uint32_t depth = 0;
+            while (child != -1) {
+                cur = (uint32_t)child;
+                for (int k = 0; k < 3; k++) {
+                    const float v = t[k] * 2.f;
+                    t[k] = (v - floorf(v)) - .5f; /* fracf (helper_math.h:1475) */
+                }
+                off = (t[0] > 0.f) | ((t[1] > 0.f) << 1) | ((t[2] > 0.f) << 2);
+                child = children[cur * 8u + off];
+                depth++;
+            }
+            float *o = out + 4 * ((size_t)py * width + px);
+            const float threshold = .5f - 2.f * (float)(1u << depth) / (float)width;
+            if (boxes && fmaxf(fabsf(t[0]), fabsf(t[1])) >= threshold) {
+                o[0] = 1.f, o[1] = 0.f, o[2] = 0.f, o[3] = 1.f;
+                continue;
+            }
+            uint32_t ri[3];
+            for (int k = 0; k < 3; k++) {
+                t[k] = (t[k] + .5f) * (float)S + (float)P;
+                ri[k] = (uint32_t)(t[k] + .5f);
+                t[k] -= (float)ri[k];
+            }
+            uint32_t n = cur * T + ri[2];
+            n = n * T + ri[1];
+            n = n * T + ri[0];
+            const float v = interp_lagrange(data, t, (int64_t)n, (int)T) + .5f;
+            o[0] = v, o[1] = v, o[2] = v, o[3] = 1.f;
+        }
+}
+
+/* ___rgba(float4) (image_process.h:14-21) per pixel, as __encode_img applies it */
+EXPORT void orc_encode_rgba8(const float *raster, uint32_t *pixels, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) {
+        const float *v = raster + 4 * i;
+        if (isnan(v[0]) || isnan(v[1]) || isnan(v[2]) || isnan(v[3])) {
+            pixels[i] = 4294902015u;
+            continue;
+        }
+        uint32_t c[4];
+        for (int k = 0; k < 4; k++) c[k] = (uint32_t)(fclampf(v[k], 0.f, 1.f) * 255.f);
+        pixels[i] = (c[3] << 24) | (c[2] << 16) | (c[1] << 8) | c[0];
+    }
 }

@@ -48,6 +48,8 @@ _SIGS = {
                                    _f32, C.c_int]),
     "orc_copy_new_to_old": (None, [_P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
     "orc_copy_to_child": (None, [_P, _u32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_rasterise": (None, [_P, _P, _u32, _u32, C.c_int, C.c_int, C.c_float, C.c_int, _P]),
+    "orc_encode_rgba8": (None, [_P, _P, C.c_int64]),
     "orc_wave_stage": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, _f32]),
     "orc_deriv_trace": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
     "orc_christoffel": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, _u32, _u32, _i32, C.c_int, C.c_int]),
@@ -218,6 +220,23 @@ def copy_new_to_old(old, new, positions, children, tiles, substep, max_depth, S,
 
 def copy_to_child(buf, offset, child, parent, mode, S, P):
     lib().orc_copy_to_child(_p(buf), offset, child, parent, mode, S, P)
+
+
+def rasterise(field, children, width, height, S, P, z_slice=1e-9, boxes=True):
+    """(height, width, 4) float32 raster of a z-slice (AMR.h:1313-1353)."""
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    ch = np.ascontiguousarray(children, dtype=np.int32)
+    out = np.empty((height, width, 4), np.float32)
+    lib().orc_rasterise(_p(f), _p(ch), width, height, S, P, z_slice, 1 if boxes else 0, _p(out))
+    return out
+
+
+def encode_rgba8(raster):
+    """RGBA8 words of a float4 raster (image_process.h:14-21)."""
+    r = np.ascontiguousarray(raster, dtype=np.float32)
+    out = np.empty(r.shape[:-1], np.uint32)
+    lib().orc_encode_rgba8(_p(r), _p(out), out.size)
+    return out
 
 
 def wave_stage(old, d1, d2, new, positions, tiles, substep, max_depth, corrector, S, P, outer_dt):
