@@ -166,7 +166,7 @@ struct octree {
     {
         memset(parent_idx_b.cpu_buffer_ptr, -1, parent_idx_b.total_size());
         memset(children_idx_b.cpu_buffer_ptr, -1, children_idx_b.total_size());
-        memset(positions.cpu_buffer_ptr, -1, positions.total_size());
+        memset(static_cast<void *>(positions.cpu_buffer_ptr), -1, positions.total_size());
         memset(lifetime.cpu_buffer_ptr, 0, lifetime.total_size());
         for (uint32_t i = 0; i < node_slots * 6u; i++) boundaries.cpu_buffer_ptr[i] = domain_boundary();
         positions.cpu_buffer_ptr[0].data = 0u;  // root
@@ -238,7 +238,7 @@ struct octree {
                     return "parent does not list the slot as its child";
                 for (int f = 0; f < 6; f++) {
                     domain_boundary b;
-                    memcpy(&b, &s.boundaries[i * 6u + f], sizeof b);
+                    memcpy(static_cast<void *>(&b), &s.boundaries[i * 6u + f], sizeof b);
                     if (b.target_idx() < 0 || (uint32_t)b.target_idx() >= n) return "boundary target out of range";
                 }
             }
@@ -255,8 +255,8 @@ struct octree {
         const size_t n = max_slots;
         memcpy(parent_idx_b.cpu_buffer_ptr, s.parent, n * sizeof(int32_t));
         memcpy(children_idx_b.cpu_buffer_ptr, s.children, 8 * n * sizeof(int32_t));
-        memcpy(positions.cpu_buffer_ptr, s.positions, n * sizeof(uint32_t));
-        memcpy(boundaries.cpu_buffer_ptr, s.boundaries, 6 * n * sizeof(uint64_t));
+        memcpy(static_cast<void *>(positions.cpu_buffer_ptr), s.positions, n * sizeof(uint32_t));
+        memcpy(static_cast<void *>(boundaries.cpu_buffer_ptr), s.boundaries, 6 * n * sizeof(uint64_t));
         memcpy(lifetime.cpu_buffer_ptr, s.lifetime, n * sizeof(uint32_t));
         max_depth = s.max_depth;
         first_free_slot = s.first_free;

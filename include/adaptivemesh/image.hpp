@@ -25,12 +25,24 @@ struct rgba_f32 {
     float r, g, b, a;
 };
 
+// the device side of a smart_gpu_cpu_buffer is made on first use
+template <class B>
+auto ensure_device(B &b, int) -> decltype(b.ensure_gpu(), void())
+{
+    b.ensure_gpu();
+}
+template <class B>
+void ensure_device(B &, long)
+{
+}
+
 template <class AMR_t, class FieldBuf, class RasterBuf>
 void rasterise_to_image(const FieldBuf &data, RasterBuf &raster_image, AMR_t &amr, uint32_t width, uint32_t height,
                         float z_slice = 1e-9f, bool boxes = true, void *stream = nullptr)
 {
     static_assert(sizeof(*raster_image.gpu_buffer_ptr) == 16, "raster pixels are 4 floats");
     if (raster_image.total_size() < (size_t)width * height * 16u) throw std::runtime_error("raster buffer too small");
+    ensure_device(raster_image, 0);
     const am_mesh m = amr.mesh_all();
     check(am_rasterise_slice(static_cast<const float *>(data.gpu_buffer_ptr), &m, &amr.sim_data.geom, width, height,
                              z_slice, boxes ? 1 : 0, reinterpret_cast<float *>(raster_image.gpu_buffer_ptr), stream),
