@@ -82,7 +82,7 @@ class BssnParams(C.Structure):
 class BssnFields(C.Structure):
     _fields_ = [("state", C.c_void_p * 24), ("old_", C.c_void_p * 24), ("new_", C.c_void_p * 24),
                 ("acc", C.c_void_p * 24), ("d1", C.c_void_p * 24), ("d2diag", C.c_void_p * 11),
-                ("d2off", C.c_void_p * 11), ("word_format", C.c_int32)]
+                ("d2off", C.c_void_p * 11), ("word_format", C.c_int32), ("push_layers", C.c_int32)]
 
 
 class BssnSimOptions(C.Structure):
@@ -105,7 +105,7 @@ class MlpWork(C.Structure):
 
 class SwaveFields(C.Structure):
     _fields_ = [("state", C.c_void_p * 2), ("old_", C.c_void_p * 2), ("new_", C.c_void_p * 2),
-                ("acc", C.c_void_p * 2), ("d1_u", C.c_void_p), ("d2diag_u", C.c_void_p)]
+                ("acc", C.c_void_p * 2), ("d1_u", C.c_void_p), ("d2diag_u", C.c_void_p), ("push_layers", C.c_int32)]
 
 
 class SwaveSimOptions(C.Structure):
@@ -246,6 +246,8 @@ _RK4_SIM_SIGS = {
     "set_launch": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
     "get_launch": (C.c_int, [_P, _P]),
     "set_fd_prefetch": (C.c_int, [_P, C.c_int32]),
+    "set_push_layers": (C.c_int, [_P, C.c_int32]),
+    "get_push_layers": (C.c_int, [_P]),
     "use_graph": (C.c_int, [_P, C.c_int32]),
     "autotune": (C.c_int, [_P, C.c_int32, _P]),
     "set_prolongation": (C.c_int, [_P, _P, C.c_float]),
@@ -591,6 +593,14 @@ class _Rk4Sim:
 
     def set_fd_prefetch(self, planes: int):
         check(self._f("set_fd_prefetch")(self.h, planes), "set_fd_prefetch")
+
+    def set_push_layers(self, layers: int):
+        """Equal-depth ghost layers refreshed per stage (0 = fd_order / 2, P = all)."""
+        check(self._f("set_push_layers")(self.h, layers), "set_push_layers")
+
+    @property
+    def push_layers(self) -> int:
+        return self._f("get_push_layers")(self.h)
 
     def get_launch(self) -> tuple:
         """(z_block, fd_grid_limit, rhs_grid_limit, fd_prefetch)"""

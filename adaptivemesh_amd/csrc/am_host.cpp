@@ -497,6 +497,16 @@ int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
         out4[3] = o.fd_prefetch;                                                                                   \
         return AM_OK;                                                                                              \
     }                                                                                                              \
+    int P##_set_push_layers(H *s, int32_t layers)                                                                  \
+    {                                                                                                              \
+        return rk4_call(s, [&](auto &d) {                                                                          \
+            if (layers < 0 || layers > d.geom.padding_domain)                                                      \
+                throw std::runtime_error("push layers " + std::to_string(layers) + " (0..P)");                   \
+            d.opt.push_layers = layers;                                                                            \
+            d.drop_graph();                                                                                        \
+        });                                                                                                        \
+    }                                                                                                              \
+    int P##_get_push_layers(const H *s) { return s ? s->amr->sim_data.push_layers() : -1; }                       \
     int P##_set_fd_prefetch(H *s, int32_t planes)                                                                  \
     {                                                                                                              \
         if (planes < 1 || planes > 3) return am::set_error(AM_ERR_INVALID, "fd prefetch %d (1-3)", planes);        \

@@ -14,7 +14,9 @@
 // Push: an interior cell within P of face f is also a ghost cell of f's
 // target when that target has the same depth; it lands in the opposite face
 // slab of the target, shifted by S cells along f's axis.  Writing it here
-// makes equal-depth ghosts exact copies without a separate fill pass.
+// makes equal-depth ghosts exact copies without a separate fill pass.  Only
+// cells within `layers` (<= P) of the face are pushed: the innermost ghost
+// layers, all that stencils of half-width `layers` read.
 #pragma once
 #include "am_device.hpp"
 
@@ -44,14 +46,14 @@ __device__ __forceinline__ void rk4_combine(const float *const *state, const flo
 
 template <int NV, int S, int P>
 __device__ __forceinline__ void rk4_push(float *const *new_, const float (&y)[NV], const uint64_t *boundaries, int node,
-                                         int x, int y_, int z)
+                                         int x, int y_, int z, int layers)
 {
     constexpr int T = S + 2 * P;
     const int c3[3] = {x, y_, z};
 #pragma unroll
     for (int f = 0; f < 6; f++) {
         const int ax = f % 3;
-        const bool near = f < 3 ? c3[ax] >= S - P : c3[ax] < P;
+        const bool near = f < 3 ? c3[ax] >= S - layers : c3[ax] < layers;
         if (!near) continue;
         const uint32_t nd = (uint32_t)boundaries[(size_t)node * 6 + f];
         const int tgt = bnd_target(nd);

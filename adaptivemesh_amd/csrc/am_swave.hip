@@ -16,6 +16,7 @@ struct SwaveArgsDev {
     float *acc[2];
     const uint32_t *d1;
     const uint32_t *d2;
+    int push_layers;
 };
 
 struct SwaveTiles {
@@ -44,7 +45,7 @@ __global__ void __launch_bounds__(256) k_swave_stage(SwaveArgsDev a, SwaveTiles 
         const float3 h = cf3_unpack(a.d2[w]);
         float k[2] = {a.state[1][r], (h.x + h.y) + h.z};
         rk4_combine<2, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
-        if (PUSH) rk4_push<2, S, P>(a.new_, k, m.boundaries, node, x, y, z);
+        if (PUSH) rk4_push<2, S, P>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers);
     }
 }
 
@@ -130,6 +131,9 @@ static int swave_stage(const am_swave_fields *f, int stage, const am_mesh *mesh,
     if (push && !mesh->boundaries) return set_error(AM_ERR_INVALID, "am_swave_rhs_stage: push needs boundaries");
     SwaveArgsDev a;
     if (int rc = swave_args(f, a, true)) return rc;
+    if (f->push_layers < 0 || f->push_layers > P)
+        return set_error(AM_ERR_INVALID, "am_swave_rhs_stage: push_layers %d (0..%d)", f->push_layers, P);
+    a.push_layers = f->push_layers ? f->push_layers : P;
     const int n_items = mesh->n_tiles * BPT;
     const SwaveTiles m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth, n_items};
     const DepthScales sc = depth_scales(*g);

@@ -337,6 +337,7 @@ typedef struct am_bssn_fields {
     const uint32_t *d2diag[AM_BSSN_NHESS];
     const uint32_t *d2off[AM_BSSN_NHESS];
     int32_t word_format; /* enum am_word_format of d1/d2diag/d2off (non-cf3: S=16, P=3 only) */
+    int32_t push_layers; /* ghost layers the push writes, innermost first: 1..P, 0 = all P */
 } am_bssn_fields;
 
 /* One classical RK4 stage on the interior of every listed active tile:
@@ -370,6 +371,7 @@ typedef struct am_swave_fields {
     float *acc[2];            /* interior-only layout, as for am_bssn_fields */
     const uint32_t *d1_u;     /* gradient word of the stage input u */
     const uint32_t *d2diag_u; /* (u_xx, u_yy, u_zz) word of the stage input u */
+    int32_t push_layers;      /* as am_bssn_fields.push_layers */
 } am_swave_fields;
 
 int am_swave_rhs_stage(const am_swave_fields *f, int32_t stage, const am_mesh *mesh, const am_geometry *g,

@@ -97,6 +97,7 @@ struct rk4_options {
     int32_t rhs_grid_limit = 0;           // persistent RHS launch: workgroup cap (0 = one per item)
     int32_t word_format = AM_WORD_CF3;    // derivative-word storage (am_word_format)
     int32_t fd_prefetch = 1;              // FD planes prefetched ahead (1-3, cf3 words)
+    int32_t push_layers = 0;              // equal-depth ghost layers pushed (0 = the FD half-width)
 };
 
 // What autotune() measured: each trial's launch shape and mean ms per launch.
@@ -121,6 +122,7 @@ struct rk4_views {
     const uint32_t *d2off[AM_MAX_VARS];   // by full-Hessian index (FD_KIND == 3, in order)
     const uint32_t *chr[6];
     int32_t word_format;  // am_word_format of d1/d2diag/d2off
+    int32_t push_layers;  // equal-depth ghost layers the stage kernel pushes
 };
 
 template <class System>
@@ -224,6 +226,16 @@ struct rk4_AMR_data {
         return reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(b.gpu_buffer_ptr) + (size_t)v * stride);
     }
 
+    // Equal-depth ghost layers the stage kernels refresh: the FD stencils
+    // read fd_order / 2 of the P layers, so by default the outer ones are
+    // left as they are (they only matter to a reader of raw ghost cells)
+    int32_t push_layers() const
+    {
+        const int32_t P = geom.padding_domain;
+        const int32_t l = opt.push_layers > 0 ? opt.push_layers : std::max(1, geom.fd_order / 2);
+        return std::min(l, P);
+    }
+
     am_field_batch batch(bool with_older)
     {
         am_field_batch b{};
@@ -239,6 +251,7 @@ struct rk4_AMR_data {
     {
         rk4_views f{};
         f.word_format = opt.word_format;
+        f.push_layers = push_layers();
         for (int v = 0; v < NV; v++) {
             f.state[v] = stage_input_is_old ? field(old_, v) : field(new_, v);
             f.old_[v] = field(old_, v);

@@ -28,6 +28,7 @@ struct swave_system {
     static am_swave_fields fields(const rk4_views &v)
     {
         am_swave_fields f{};
+        f.push_layers = v.push_layers;
         for (int i = 0; i < NV; i++) {
             f.state[i] = v.state[i];
             f.old_[i] = v.old_[i];

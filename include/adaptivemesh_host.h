@@ -198,6 +198,12 @@ int am_bssn_sim_set_launch(am_bssn_sim *s, int32_t z_block, int32_t fd_grid_limi
 int am_bssn_sim_get_launch(const am_bssn_sim *s, int32_t *out4);
 /* FD planes prefetched ahead of the differentiated plane (1-3) */
 int am_bssn_sim_set_fd_prefetch(am_bssn_sim *s, int32_t planes);
+/* Equal-depth ghost layers the stage kernels refresh, innermost first:
+ * 0 (the default) = fd_order / 2, the layers the FD stencils read; P =
+ * every layer, as the reference's copy_boundaries writes them.  Layers
+ * beyond keep what they held.  _get_push_layers gives the layers in use. */
+int am_bssn_sim_set_push_layers(am_bssn_sim *s, int32_t layers);
+int am_bssn_sim_get_push_layers(const am_bssn_sim *s);
 int am_bssn_sim_use_graph(am_bssn_sim *s, int32_t enable);
 int am_bssn_sim_autotune(am_bssn_sim *s, int32_t reps, am_tune_report *out);
 /* AM_INTERP_LEARNED: the MLP parameters (device, am_mlp_param_count()
@@ -257,6 +263,8 @@ int am_swave_sim_kernel_times(am_swave_sim *s, double *ms5, int64_t *count5);
 int am_swave_sim_set_launch(am_swave_sim *s, int32_t z_block, int32_t fd_grid_limit, int32_t rhs_grid_limit);
 int am_swave_sim_get_launch(const am_swave_sim *s, int32_t *out4);
 int am_swave_sim_set_fd_prefetch(am_swave_sim *s, int32_t planes);
+int am_swave_sim_set_push_layers(am_swave_sim *s, int32_t layers);
+int am_swave_sim_get_push_layers(const am_swave_sim *s);
 int am_swave_sim_use_graph(am_swave_sim *s, int32_t enable);
 int am_swave_sim_autotune(am_swave_sim *s, int32_t reps, am_tune_report *out);
 int am_swave_sim_save(am_swave_sim *s, const char *path);

@@ -42,3 +42,26 @@ def used_slots(exp):
 
 def depth_of(exp, s):
     return int(exp["positions"][s]) >> 27
+
+
+def outer_push_layers(exp, n_slots, tiles, S, P, layers):
+    """Mask (n_slots * T^3) of the ghost cells that a push of the innermost
+    `layers` ghost layers leaves alone: the outer P - layers layers of each
+    face slab of `tiles` whose neighbour has the same depth."""
+    T = S + 2 * P
+    m = np.zeros((n_slots, T, T, T), bool)  # [slot, z, y, x]
+    inner = slice(P, P + S)
+    for s in tiles:
+        for g in range(6):
+            nd = int(exp["boundaries"][s * 6 + g]) & 0xFFFFFFFF
+            tgt = nd & 0xFFFFFF
+            if tgt & 0x800000:
+                tgt -= 1 << 24
+            own = nd >> 28
+            if tgt < 0 or own == 15 or own == 0 or (nd >> 24) & 15:
+                continue
+            band = slice(P + S + layers, T) if g < 3 else slice(0, P - layers)
+            idx = [inner, inner, inner]  # x, y, z
+            idx[g % 3] = band
+            m[s, idx[2], idx[1], idx[0]] = True
+    return m.reshape(-1)

@@ -106,10 +106,13 @@ def test_swave_stages_match_oracle(am, orc, torch_cuda):
                 assert np.max(np.abs(acc_g[v][imask] - acc_o[v])) < 2e-6 * ascale, (stage, v)
         # every face of a depth-1 tile in the periodic 2x2x2 block has an
         # equal-depth target: its ghosts are the pushed exact copies
+        # (the innermost ORDER / 2 layers; the outer layers keep what they held)
+        keep = F.outer_push_layers(e, n, tiles, S, P, sim.push_layers)
         for v in range(2):
             ref = new_g[v].copy()
             orc.copy_boundaries(old[v].copy(), ref, None, e["boundaries"], tiles, 0, e["max_depth"], True, 0, S, P,
                                 STAGE_FRAC[stage], equal_copy=True)
+            ref[keep] = state_in[v][keep]
             assert np.array_equal(ref.view(np.uint32), new_g[v].view(np.uint32)), (stage, v)
 
 
