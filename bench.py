@@ -41,13 +41,14 @@ NOISE = 1e-6
 # algorithmic bytes per interior cell (each distinct element read once and
 # written once per launch)
 WORD = 4
-PUSH = 6 * P * S * S * 24 * 4 / S ** 3  # equal-depth face ghosts written by the stage kernel
+PUSH_LAYERS = FD_ORDER // 2  # the driver pushes the ghost layers the FD stencils read
+PUSH = 6 * PUSH_LAYERS * S * S * 24 * 4 / S ** 3  # equal-depth face ghosts written by the stage kernel
 # HBM bytes per depth-3 k_bssn_stage launch (512 tiles), mean of the four
 # stages, from rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0, both
 # scaled by a 1 GiB copy): profiles/bssn_rk4_512tiles_hbm_counters.md.  The
-# algorithmic bytes of the same launches average 1.468e9.
-RHS_TRAFFIC_PER_LAUNCH = 2.013e9
-RHS_ALGORITHMIC_PER_LAUNCH = 1.468e9
+# algorithmic bytes of the same launches average 1.392e9.
+RHS_TRAFFIC_PER_LAUNCH = 1.940e9
+RHS_ALGORITHMIC_PER_LAUNCH = 1.392e9
 BYTES = {
     # per stage, 24 fields: padded reads (S^3 + 6 P S^2) * 4 per field; 24 gradient + 11 diag + 11 off-diag words
     "fd": 24 * (S ** 3 + 6 * P * S * S) * 4 / S ** 3 + 46 * WORD,
@@ -211,6 +212,7 @@ def main():
         g = am.geometry(S, P, FD_ORDER, OUTER, CFL)
         n_slots = 1 + 8 + 64 + 512
         sim = am.BssnSim(g, n_slots, am.bssn_options(periodic=True))
+        assert sim.push_layers == PUSH_LAYERS, "the byte model assumes the driver's default push"
         assert build_mesh(sim) == n_slots
         fields = initial_data(n_slots, T)
     else:
@@ -293,6 +295,7 @@ def main():
                    "fine_rk4_steps_per_step": 2 ** sim.counter()[1],
                    "launch": {"z_block": sim.get_launch()[0], "fd_grid_limit": sim.get_launch()[1],
                               "rhs_grid_limit": sim.get_launch()[2], "fd_prefetch_planes": sim.get_launch()[3],
+                              "push_layers": sim.push_layers,
                               "autotuned": tune is not None,
                               "autotune_trials": tune["trials"] if tune else None,
                               "graph_per_coarse_step": not args.no_graph}},
