@@ -200,6 +200,8 @@ _SIGS = {
     "am_rasterise_slice": (C.c_int, [_P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_uint32, C.c_uint32, C.c_float,
                                      C.c_int32, _P, _P]),
     "am_encode_rgba8": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "am_init_temp": (C.c_int, [_P, _P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_float, _P]),
+    "am_wave_sim_init_temp": (C.c_int, [_P, C.c_float]),
     "am_write_png": (C.c_int, [C.c_char_p, _P, C.c_uint32, C.c_uint32]),
     "am_wave_sim_load": (C.c_int, [_P, C.c_char_p]),
     "am_ghost_fill_stage_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), C.c_float,
@@ -442,6 +444,10 @@ class WaveSim:
     def load(self, path):
         """Resume from a checkpoint of a driver built alike."""
         check(lib().am_wave_sim_load(self.h, os.fsencode(path)), "load")
+
+    def init_temp(self, mul: float = 1.0):
+        """The reference wave test's initial data (kernel.cu:9-43) in both fields."""
+        check(lib().am_wave_sim_init_temp(self.h, mul), "init_temp")
 
     def raster(self, which: int = 1, width: int = 1024, height: int = 1024, z_slice: float = 1e-9,
                boxes: bool = True) -> np.ndarray:

@@ -396,6 +396,15 @@ int am_wave_sim_load(am_wave_sim *s, const char *path)
         return AM_OK;
     });
 }
+int am_wave_sim_init_temp(am_wave_sim *s, float mul)
+{
+    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
+    return guarded([&] {
+        auto &d = s->amr->sim_data;
+        init_temp(d.old_field, d.new_field, *s->amr, mul);
+        return am_stream_sync(d.stream);
+    });
+}
 int am_wave_sim_raster(am_wave_sim *s, int32_t which, uint32_t width, uint32_t height, float z_slice, int32_t boxes,
                        uint32_t *pixels)
 {

@@ -245,6 +245,15 @@ int am_copy_to_child_batched(const am_field_batch *b, uint32_t offset, int32_t c
 
 /* ------------------------------------------------------ SimpleWave */
 
+/* __init_temp (kernel.cu:9-43), the reference's wave-test initial data:
+ * f = mul (10 / (|x - a|^2 + 10) - 10 / (|x + a|^2 + 10)), a = (-25, 1/2, 0)
+ * in physical units, into both fields at every cell of a listed depth-0
+ * tile and at the interior cells of deeper ones.  Cell x = tile centre
+ * (AMR.h:33-43) + (i + 1/2 - T/2) outer_dx / 2^depth per axis.  The
+ * reference launches it for T = 32 tiles only; here any supported geometry. */
+int am_init_temp(float *old_field, float *new_field, const am_mesh *mesh, const am_geometry *g, float mul,
+                 void *stream);
+
 /* __rasterise_to_image (AMR.h:1313-1353): a z-slice through the whole
  * octree as a width x height image.  Pixel (x, y) samples the point
  * (x/width - 1/2, y/height - 1/2, z_slice) of the root cube (units of

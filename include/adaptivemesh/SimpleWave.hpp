@@ -182,4 +182,14 @@ struct wave_AMR_data {
     }
 };
 
+// init_temp (kernel.cu:34-43): the wave test's two-bump initial data in
+// both fields of the used tiles (all cells at depth 0, interior deeper)
+template <class T>
+void init_temp(smart_gpu_buffer<float> &old_d, smart_gpu_buffer<float> &new_d, AMR<T> &amr, float mul = 1.f)
+{
+    const am_mesh m = amr.mesh_all();
+    check(am_init_temp(old_d.gpu_buffer_ptr, new_d.gpu_buffer_ptr, &m, &amr.sim_data.geom, mul, amr.sim_data.stream),
+          "init_temp");
+}
+
 }  // namespace am

@@ -111,6 +111,8 @@ int am_wave_sim_set_regrid(am_wave_sim *s, int32_t on);
  * maxima; octants / 8), as amr_criterion returns them (SimpleWave.h:183) */
 int am_wave_sim_pressure(const am_wave_sim *s, float *out2);
 int am_wave_sim_save(am_wave_sim *s, const char *path);
+/* init_temp (kernel.cu:34-43) on both fields: am_init_temp over every used slot */
+int am_wave_sim_init_temp(am_wave_sim *s, float mul);
 /* rasterise_to_image + the encoding half of save_image (AMR.h:1355-1364,
  * image_process.h:55-62) on field `which` (0 old, 1 new): width * height
  * RGBA8 pixels into host memory; am_write_png stores them. */

@@ -31,6 +31,7 @@
  *   __deriv_trace              AdaptiveMesh/BSSN.h:7-22
  *   __compute_raised_christoffel_symbols  AdaptiveMesh/BSSN.h:55-91
  *   __rasterise_to_image       AdaptiveMesh/AMR.h:1313-1353
+ *   __init_temp                AdaptiveMesh/kernel.cu:9-43
  *   ___rgba(float4)            AdaptiveMesh/image_process.h:14-21
  *
  * Extensions that the reference does not have, restated here so the product's
@@ -981,5 +982,43 @@ EXPORT void orc_encode_rgba8(const float *raster, uint32_t *pixels, int64_t n)
         uint32_t c[4];
         for (int k = 0; k < 4; k++) c[k] = (uint32_t)(fclampf(v[k], 0.f, 1.f) * 255.f);
         pixels[i] = (c[3] << 24) | (c[2] << 16) | (c[1] << 8) | c[0];
+    }
+}
+
+/* __init_temp (kernel.cu:9-43) with absolute_central_position (AMR.h:33-43):
+ * the two-bump field in float with the reference's operation order, into
+ * both arrays; all cells of depth-0 slots, interior cells of deeper ones. */
+EXPORT void orc_init_temp(float *old_d, float *new_d, const uint32_t *pos, const int32_t *tiles, int n_tiles, int S,
+                          int P, float outer_size, float outer_dx, float mul)
+{
+    const int T = S + 2 * P;
+    for (int t = 0; t < n_tiles; t++) {
+        const int node = tiles[t];
+        const int depth = pos_depth(pos[node]);
+        if (depth == -1) continue;
+        float c[3] = {0.f, 0.f, 0.f};
+        uint32_t s = pos[node] & 134217727u;
+        for (int i = 0; i < depth; i++, s >>= 3) {
+            c[0] += ((s & 1u) ? .25f : -.25f) / (float)(1 << i);
+            c[1] += ((s & 2u) ? .25f : -.25f) / (float)(1 << i);
+            c[2] += ((s & 4u) ? .25f : -.25f) / (float)(1 << i);
+        }
+        for (int z = 0; z < T; z++)
+            for (int y = 0; y < T; y++)
+                for (int x = 0; x < T; x++) {
+                    if (depth > 0 && (x < P || x >= S + P || y < P || y >= S + P || z < P || z >= S + P)) continue;
+                    const int idx[3] = {x, y, z};
+                    float p[3];
+                    for (int k = 0; k < 3; k++)
+                        p[k] = c[k] * outer_size + ((((float)idx[k] + .5f) - (float)T * .5f) * outer_dx) /
+                                                       (float)(1u << depth);
+                    const float m0 = p[0] + 25.f, m1 = p[1] - .5f, m2 = p[2];
+                    const float q0 = p[0] - 25.f, q1 = p[1] + .5f, q2 = p[2];
+                    const float v = mul * (10.f / ((m0 * m0 + m1 * m1) + m2 * m2 + 10.f) -
+                                           10.f / ((q0 * q0 + q1 * q1) + q2 * q2 + 10.f));
+                    const int64_t o = (int64_t)node * T * T * T + ((int64_t)z * T + y) * T + x;
+                    old_d[o] = v;
+                    new_d[o] = v;
+                }
     }
 }

@@ -50,6 +50,7 @@ _SIGS = {
     "orc_copy_to_child": (None, [_P, _u32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_rasterise": (None, [_P, _P, _u32, _u32, C.c_int, C.c_int, C.c_float, C.c_int, _P]),
     "orc_encode_rgba8": (None, [_P, _P, C.c_int64]),
+    "orc_init_temp": (None, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]),
     "orc_wave_stage": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, _f32]),
     "orc_deriv_trace": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
     "orc_christoffel": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, _u32, _u32, _i32, C.c_int, C.c_int]),
@@ -237,6 +238,17 @@ def encode_rgba8(raster):
     out = np.empty(r.shape[:-1], np.uint32)
     lib().orc_encode_rgba8(_p(r), _p(out), out.size)
     return out
+
+
+def init_temp(positions, tiles, n_slots, S, P, outer_size, outer_dx, mul=1.0):
+    """(old, new) after __init_temp (kernel.cu:9-43); untouched cells are 0."""
+    T = S + 2 * P
+    old = np.zeros(n_slots * T ** 3, np.float32)
+    new = np.zeros(n_slots * T ** 3, np.float32)
+    pos = np.ascontiguousarray(positions, dtype=np.uint32)
+    t = _i32arr(tiles)
+    lib().orc_init_temp(_p(old), _p(new), _p(pos), _p(t), len(t), S, P, outer_size, outer_dx, mul)
+    return old, new
 
 
 def wave_stage(old, d1, d2, new, positions, tiles, substep, max_depth, corrector, S, P, outer_dt):

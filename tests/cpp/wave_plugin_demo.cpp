@@ -1,12 +1,11 @@
-// A user program written against the drop-in C++ headers, shaped like the
-// reference's wave_test (kernel.cu:45-64): an AMR<wave_AMR_data>, children
-// added under the root, copy_bounds, then timestep() with a slice rasterised
-// and saved as a PNG after every step.  Only the include lines and the
-// geometry argument differ from a program written for the reference.
+// A user program written against the drop-in C++ headers: the reference's
+// wave_test (kernel.cu:45-64) -- AMR<wave_AMR_data> with 128 slots,
+// init_temp, the root's 8 children, copy_bounds, then timestep() with a
+// slice rasterised and saved as a PNG after every step.  Only the include
+// lines and the geometry argument differ from the reference's program.
 //
-//   wave_plugin_demo <init.bin> <out_dir> <steps> <image_width> <image_height>
+//   wave_plugin_demo <out_dir> <steps> <image_width> <image_height>
 //
-// init.bin holds old_field = new_field for all slots (max_slots * T^3 fp32).
 // Writes <out_dir>/wave_<i>.png for i = 0..steps, <out_dir>/old.bin and
 // new.bin, and prints "seed <s> used <n> timer <t>".
 #include <stdio.h>
@@ -33,33 +32,23 @@ static bool dump(const char *path, const float *dev, size_t n)
 
 int main(int argc, char **argv)
 {
-    if (argc != 6) {
-        fprintf(stderr, "usage: %s init.bin out_dir steps width height\n", argv[0]);
+    if (argc != 5) {
+        fprintf(stderr, "usage: %s out_dir steps width height\n", argv[0]);
         return 2;
     }
-    const std::string out = argv[2];
-    const int steps = atoi(argv[3]);
-    const uint32_t W = (uint32_t)atoi(argv[4]), H = (uint32_t)atoi(argv[5]);
-    const uint32_t slots = 40u;
+    const std::string out = argv[1];
+    const int steps = atoi(argv[2]);
+    const uint32_t W = (uint32_t)atoi(argv[3]), H = (uint32_t)atoi(argv[4]);
+    const uint32_t slots = 128u;
     am_geometry g;
     if (am_geometry_init(&g, 16, 3, 6, 100.f, .4f)) return 3;
     try {
         AMR<wave_AMR_data> amr(slots, g);
         const size_t n = wave_AMR_data::cells(g) * slots;
-        std::vector<float> init(n);
-        FILE *f = fopen(argv[1], "rb");
-        if (!f || fread(init.data(), sizeof(float), n, f) != n) {
-            fprintf(stderr, "cannot read %s\n", argv[1]);
-            return 4;
-        }
-        fclose(f);
-        check(am_memcpy_h2d(amr.sim_data.old_field.gpu_buffer_ptr, init.data(), n * sizeof(float), nullptr), "init");
-        check(am_memcpy_h2d(amr.sim_data.new_field.gpu_buffer_ptr, init.data(), n * sizeof(float), nullptr), "init");
-        check(am_stream_sync(nullptr), "init");
-
-        for (int c = 0; c < 8; c++) amr.add_node(0, c);  // the mesh of the tests' graded recipe
-        for (int c = 0; c < 8; c++) amr.add_node(1, c);
-        for (int c = 0; c < 8; c++) amr.add_node(16, c);
+        amr.sim_data.old_field.zero();
+        amr.sim_data.new_field.zero();
+        init_temp<wave_AMR_data>(amr.sim_data.old_field, amr.sim_data.new_field, amr);
+        for (uint32_t c = 0; c < 8; c++) amr.add_node(0, c);
         amr.sim_data.copy_bounds();
 
         smart_gpu_cpu_buffer<rgba_f32> temp((size_t)W * H);

@@ -11,7 +11,6 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from tests import fields, meshes
 from tests.test_png import chunks
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -42,36 +41,26 @@ def png_pixels(path):
 
 @pytest.mark.gpu
 def test_demo_matches_python_driver(am, orc, torch_cuda, tmp_path):
-    S, P, n, steps, W, H = 16, 3, 40, 2, 96, 64
-    T = S + 2 * P
-    root = orc.OracleTree(1).export()
-    blob = fields.sample(root["positions"], 1, S, P, 100.0,
-                         lambda x, y, z: 10 / ((x - 25) ** 2 + (y - .5) ** 2 + z * z + 10)
-                         - 10 / ((x + 25) ** 2 + (y + .5) ** 2 + z * z + 10))
-    init = np.zeros(n * T ** 3, np.float32)
-    init[:T ** 3] = blob
-    (tmp_path / "init.bin").write_bytes(init.tobytes())
+    S, P, n, steps, W, H = 16, 3, 128, 3, 96, 64
     out = tmp_path / "out"
     out.mkdir()
     exe = build(tmp_path)
     print("running the C++ demo", flush=True)
-    r = subprocess.run([str(exe), str(tmp_path / "init.bin"), str(out), str(steps), str(W), str(H)],
-                       capture_output=True, text=True, timeout=120)
+    r = subprocess.run([str(exe), str(out), str(steps), str(W), str(H)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    seed = int(r.stdout.split()[1])
+    seed, used = int(r.stdout.split()[1]), int(r.stdout.split()[3])
 
     g = am.geometry(S, P, 6, 100.0, 0.4)
     sim = am.WaveSim(g, n)
-    sim.set_field(0, init)
-    sim.set_field(1, init)
-    ops, _ = meshes.graded()
-    meshes.apply(sim, ops)
+    sim.init_temp(1.0)
+    for c in range(8):
+        sim.add_node(0, c)
     sim.copy_bounds()
     assert np.array_equal(png_pixels(out / "wave_0.png"), sim.raster(1, W, H))
     for i in range(1, steps + 1):
         sim.timestep()
         assert np.array_equal(png_pixels(out / f"wave_{i}.png"), sim.raster(1, W, H)), i
-    assert sim.rng_state == seed
+    assert sim.rng_state == seed and sim.export()["scalars"][2] + 1 == used
     for which, name in ((0, "old.bin"), (1, "new.bin")):
         got = np.frombuffer((out / name).read_bytes(), np.float32)
         assert np.array_equal(got.view(np.uint32), sim.get_field(which).view(np.uint32)), name
