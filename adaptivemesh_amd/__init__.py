@@ -200,6 +200,8 @@ _SIGS = {
     "am_rasterise_slice": (C.c_int, [_P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_uint32, C.c_uint32, C.c_float,
                                      C.c_int32, _P, _P]),
     "am_encode_rgba8": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "am_encode_rgba8_f32": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "am_encode_rgba8_cf3": (C.c_int, [_P, _P, C.c_int64, _P]),
     "am_init_temp": (C.c_int, [_P, _P, C.POINTER(Mesh), C.POINTER(Geometry), C.c_float, _P]),
     "am_wave_sim_init_temp": (C.c_int, [_P, C.c_float]),
     "am_write_png": (C.c_int, [C.c_char_p, _P, C.c_uint32, C.c_uint32]),

@@ -71,6 +71,33 @@ __global__ void __launch_bounds__(256) k_encode_rgba8(const float4 *__restrict__
     pixels[i] = p;
 }
 
+// ___rgba(float) and ___rgba(float3) (image_process.h:22-38): a grey pixel
+// per float, an RGB pixel per decoded compressed_float3 word
+__global__ void __launch_bounds__(256) k_encode_rgba8_f32(const float *__restrict__ img, uint32_t *__restrict__ pixels,
+                                                          int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float v = img[i];
+    uint32_t p = 4294902015u;
+    if (!isnan(v)) {
+        const uint32_t c = channel(v);
+        p = (255u << 24) | (c << 16) | (c << 8) | c;
+    }
+    pixels[i] = p;
+}
+__global__ void __launch_bounds__(256) k_encode_rgba8_cf3(const uint32_t *__restrict__ words,
+                                                          uint32_t *__restrict__ pixels, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float3 v = cf3_unpack(words[i]);
+    uint32_t p = 4294902015u;
+    if (!(isnan(v.x) || isnan(v.y) || isnan(v.z)))
+        p = (255u << 24) | (channel(v.z) << 16) | (channel(v.y) << 8) | channel(v.x);
+    pixels[i] = p;
+}
+
 template <int S, int P>
 static int launch_rasterise(const float *field, const am_mesh *mesh, uint32_t width, uint32_t height, float z_slice,
                             int32_t boxes, float *raster, void *stream)
@@ -104,4 +131,22 @@ extern "C" int am_encode_rgba8(const float *raster, uint32_t *pixels, int64_t n,
     hipLaunchKernelGGL(am::k_encode_rgba8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        reinterpret_cast<const float4 *>(raster), pixels, n);
     return am::check_launch("k_encode_rgba8");
+}
+
+extern "C" int am_encode_rgba8_f32(const float *image, uint32_t *pixels, int64_t n, void *stream)
+{
+    if (!image || !pixels || n < 0) return am::set_error(AM_ERR_INVALID, "am_encode_rgba8_f32: bad argument");
+    if (n == 0) return AM_OK;
+    hipLaunchKernelGGL(am::k_encode_rgba8_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       image, pixels, n);
+    return am::check_launch("k_encode_rgba8_f32");
+}
+
+extern "C" int am_encode_rgba8_cf3(const uint32_t *words, uint32_t *pixels, int64_t n, void *stream)
+{
+    if (!words || !pixels || n < 0) return am::set_error(AM_ERR_INVALID, "am_encode_rgba8_cf3: bad argument");
+    if (n == 0) return AM_OK;
+    hipLaunchKernelGGL(am::k_encode_rgba8_cf3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       words, pixels, n);
+    return am::check_launch("k_encode_rgba8_cf3");
 }

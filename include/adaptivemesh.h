@@ -269,6 +269,12 @@ int am_rasterise_slice(const float *field, const am_mesh *mesh, const am_geometr
  * pixels to RGBA8, R | G << 8 | B << 16 | A << 24, each channel
  * (uint)(clamp(c, 0, 1) * 255); a NaN channel gives 0xFFFF00FF. */
 int am_encode_rgba8(const float *raster, uint32_t *pixels, int64_t n, void *stream);
+/* ___rgba(float) and ___rgba(compressed_float3) (image_process.h:22-38), as
+ * save_image applies them to a field or a word array: a float v gives grey
+ * (c, c, c, 255), c = (uint)(clamp(v, 0, 1) * 255); a word gives
+ * (r, g, b, 255) from its decoded components; NaN gives 0xFFFF00FF. */
+int am_encode_rgba8_f32(const float *image, uint32_t *pixels, int64_t n, void *stream);
+int am_encode_rgba8_cf3(const uint32_t *words, uint32_t *pixels, int64_t n, void *stream);
 
 /* Refinement criterion of the SimpleWave plugin (replaces __amr_criterion_1
  * and __amr_criterion_2, SimpleWave.h:91-144): for each listed tile and

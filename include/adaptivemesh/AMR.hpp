@@ -28,6 +28,65 @@ namespace am {
 
 constexpr uint32_t depth_limit = 8u;  // AMR.h:9
 
+// The reference's component bundles (AMR.h:428-633), named as there: one
+// buffer per component of a symmetric tensor field (xx, xy, xz, yy, yz, zz),
+// of symmetric compressed_float3 words, and of a Hessian's diagonal and
+// off-diagonal words.  operator[] walks the components in that order.
+struct tensor2_sym_field {
+    smart_gpu_buffer<float> xx, xy, xz, yy, yz, zz;
+    tensor2_sym_field() = default;
+    tensor2_sym_field(size_t node_slots, size_t domain_cells)
+        : xx(node_slots * domain_cells), xy(node_slots * domain_cells), xz(node_slots * domain_cells),
+          yy(node_slots * domain_cells), yz(node_slots * domain_cells), zz(node_slots * domain_cells)
+    {
+    }
+    smart_gpu_buffer<float> &operator[](int c)
+    {
+        smart_gpu_buffer<float> tensor2_sym_field::*const m[6] = {
+            &tensor2_sym_field::xx, &tensor2_sym_field::xy, &tensor2_sym_field::xz,
+            &tensor2_sym_field::yy, &tensor2_sym_field::yz, &tensor2_sym_field::zz};
+        return this->*m[c];
+    }
+    void destroy()
+    {
+        for (int c = 0; c < 6; c++) (*this)[c].destroy();
+    }
+};
+struct comp_tensor3_sym_field {
+    smart_gpu_buffer<compressed_float3> xx, xy, xz, yy, yz, zz;
+    comp_tensor3_sym_field() = default;
+    comp_tensor3_sym_field(size_t node_slots, size_t domain_cells)
+        : xx(node_slots * domain_cells), xy(node_slots * domain_cells), xz(node_slots * domain_cells),
+          yy(node_slots * domain_cells), yz(node_slots * domain_cells), zz(node_slots * domain_cells)
+    {
+    }
+    smart_gpu_buffer<compressed_float3> &operator[](int c)
+    {
+        smart_gpu_buffer<compressed_float3> comp_tensor3_sym_field::*const m[6] = {
+            &comp_tensor3_sym_field::xx, &comp_tensor3_sym_field::xy, &comp_tensor3_sym_field::xz,
+            &comp_tensor3_sym_field::yy, &comp_tensor3_sym_field::yz, &comp_tensor3_sym_field::zz};
+        return this->*m[c];
+    }
+    void destroy()
+    {
+        for (int c = 0; c < 6; c++) (*this)[c].destroy();
+    }
+};
+struct comp_tensor2_sym_field {
+    smart_gpu_buffer<compressed_float3> diag, off_diag;
+    comp_tensor2_sym_field() = default;
+    comp_tensor2_sym_field(size_t node_slots, size_t domain_cells)
+        : diag(node_slots * domain_cells), off_diag(node_slots * domain_cells)
+    {
+    }
+    smart_gpu_buffer<compressed_float3> &operator[](int c) { return c ? off_diag : diag; }
+    void destroy()
+    {
+        diag.destroy();
+        off_diag.destroy();
+    }
+};
+
 // LSB ddddd|zyxzyx...: depth in bits 27-31, a 3-bit octant per level with
 // level 0 lowest (AMR.h:22-55).
 struct octree_abs_pos {

@@ -27,33 +27,40 @@ struct BSSN_AMR_data {
     void *stream = nullptr;
     bool christoffel_compat = true;  // reproduce BSSN.h:82-83 (metric instead of inverse)
 
-    smart_gpu_buffer<float> cyij_old[6], cyij_new[6];     // xx,xy,xz,yy,yz,zz (BSSN.h:111-112)
-    smart_gpu_buffer<compressed_float3> cyij_k[6];        // gradient words (BSSN.h:115)
-    smart_gpu_buffer<compressed_float3> cGijk[6];         // Christoffel words (BSSN.h:116)
-    smart_gpu_buffer<float> cRij[6];                      // gamma^ij d_i d_j gamma_kl (BSSN.h:117)
-    smart_gpu_buffer<compressed_float3> didjA[2], didjW[2];  // Hessian scratch (diag, off) (BSSN.h:118-119)
+    // the reference's members and component names (BSSN.h:110-119)
+    tensor2_sym_field cyij_old, cyij_new;  // conformal metric, padded tiles
+    comp_tensor3_sym_field cyij_k;         // its gradient words
+    comp_tensor3_sym_field cGijk;          // Christoffel words
+    tensor2_sym_field cRij;                // gamma^ij d_i d_j gamma_kl, interior-only
+    comp_tensor2_sym_field didjA, didjW;   // Hessian scratch (diagonal, off-diagonal words)
 
-    BSSN_AMR_data(uint32_t node_slots, AMR<BSSN_AMR_data> &p, const am_geometry &g) : rng(), parent(p), geom(g)
+    static size_t tile_cells(const am_geometry &g)
     {
-        const size_t T = (size_t)(g.size_domain + 2 * g.padding_domain), S = (size_t)g.size_domain;
+        const size_t T = (size_t)(g.size_domain + 2 * g.padding_domain);
+        return T * T * T;
+    }
+    static size_t inner_cells(const am_geometry &g)
+    {
+        const size_t S = (size_t)g.size_domain;
+        return S * S * S;
+    }
+    BSSN_AMR_data(uint32_t node_slots, AMR<BSSN_AMR_data> &p, const am_geometry &g)
+        : rng(), parent(p), geom(g), cyij_old(node_slots, tile_cells(g)), cyij_new(node_slots, tile_cells(g)),
+          cyij_k(node_slots, inner_cells(g)), cGijk(node_slots, inner_cells(g)), cRij(node_slots, inner_cells(g)),
+          didjA(node_slots, inner_cells(g)), didjW(node_slots, inner_cells(g))
+    {
+        // zeroed, so a run does not depend on what the allocation held (the
+        // reference leaves its buffers uninitialised)
         for (int c = 0; c < 6; c++) {
-            cyij_old[c] = smart_gpu_buffer<float>(node_slots * T * T * T);
-            cyij_new[c] = smart_gpu_buffer<float>(node_slots * T * T * T);
-            cyij_k[c] = smart_gpu_buffer<compressed_float3>(node_slots * S * S * S);
-            cGijk[c] = smart_gpu_buffer<compressed_float3>(node_slots * S * S * S);
-            cRij[c] = smart_gpu_buffer<float>(node_slots * S * S * S);
+            cyij_old[c].zero(); cyij_new[c].zero(); cyij_k[c].zero(); cGijk[c].zero(); cRij[c].zero();
         }
-        for (int c = 0; c < 2; c++) {
-            didjA[c] = smart_gpu_buffer<compressed_float3>(node_slots * S * S * S);
-            didjW[c] = smart_gpu_buffer<compressed_float3>(node_slots * S * S * S);
-        }
+        for (int c = 0; c < 2; c++) { didjA[c].zero(); didjW[c].zero(); }
+        check(am_stream_sync(stream), "BSSN_AMR_data");
     }
     void destroy()
     {
-        for (int c = 0; c < 6; c++) {
-            cyij_old[c].destroy(); cyij_new[c].destroy(); cyij_k[c].destroy(); cGijk[c].destroy(); cRij[c].destroy();
-        }
-        for (int c = 0; c < 2; c++) { didjA[c].destroy(); didjW[c].destroy(); }
+        cyij_old.destroy(); cyij_new.destroy(); cyij_k.destroy(); cGijk.destroy(); cRij.destroy();
+        didjA.destroy(); didjW.destroy();
     }
     void node_ctor(int node_idx)  // BSSN.h:131-137
     {
@@ -71,7 +78,7 @@ struct BSSN_AMR_data {
     // metric_derivatives_noncovariant + compute_raised_christoffel_symbols
     // (BSSN.h:38-53, 93-102): per component differentiate (1 draw), mixed
     // partials (1 draw), metric trace; then the Christoffel words (1 draw).
-    void derivatives(smart_gpu_buffer<float> (&metric)[6])
+    void derivatives(tensor2_sym_field &metric)
     {
         const am_mesh m = parent.mesh_all();
         const float *met[6];

@@ -106,4 +106,33 @@ struct smart_gpu_cpu_buffer {
     }
 };
 
+// cuda_sync (CUDA_memory.h:213-220): wait for every launch on the device
+inline int cuda_sync() { return am_device_sync(); }
+
+// round_robin_threads (AMR.h:852-878): eight streams handed out in turn, so
+// code written for the reference's plugins passes them through unchanged.
+// The wrappers here take a stream as void*.
+struct round_robin_threads {
+    void *streams[8] = {};
+    uint32_t current_stream_idx = 0;
+    round_robin_threads()
+    {
+        for (int i = 0; i < 8; i++) check(am_stream_create(&streams[i]), "round_robin_threads");
+    }
+    round_robin_threads(const round_robin_threads &) = delete;
+    round_robin_threads &operator=(const round_robin_threads &) = delete;
+    ~round_robin_threads()
+    {
+        for (int i = 0; i < 8; i++) am_stream_destroy(streams[i]);
+    }
+    void set_stream_idx(uint32_t idx) { current_stream_idx = idx & 7u; }
+    void *current_stream() const { return streams[current_stream_idx & 7u]; }
+    void *next_stream() const { return streams[(current_stream_idx + 1u) & 7u]; }
+    void *yield_stream()
+    {
+        current_stream_idx &= 7u;
+        return streams[current_stream_idx++];
+    }
+};
+
 }  // namespace am

@@ -49,7 +49,26 @@ void rasterise_to_image(const FieldBuf &data, RasterBuf &raster_image, AMR_t &am
           "rasterise_to_image");
 }
 
-// temp: RGBA8 staging with a device side (smart_gpu_cpu_buffer<uint32_t>)
+// ___rgba per pixel type (image_process.h:9-38): 4-float raster pixels,
+// scalar fields (grey) and compressed_float3 words (RGB)
+inline int encode_rgba8(const float *img, uint32_t *px, int64_t n, void *stream)
+{
+    return am_encode_rgba8_f32(img, px, n, stream);
+}
+inline int encode_rgba8(const compressed_float3 *img, uint32_t *px, int64_t n, void *stream)
+{
+    return am_encode_rgba8_cf3(reinterpret_cast<const uint32_t *>(img), px, n, stream);
+}
+template <class Pixel16>
+int encode_rgba8(const Pixel16 *img, uint32_t *px, int64_t n, void *stream)
+{
+    static_assert(sizeof(Pixel16) == 16, "save_image encodes float, compressed_float3 or 4-float pixels");
+    return am_encode_rgba8(reinterpret_cast<const float *>(img), px, n, stream);
+}
+
+// temp: RGBA8 staging with a device side (smart_gpu_cpu_buffer<uint32_t>);
+// image: a raster, a float field or a word array (any buffer with
+// gpu_buffer_ptr), its first width * height elements as rows of `width`
 template <class PixelBuf, class RasterBuf>
 void save_image(PixelBuf &temp, const RasterBuf &image, uint32_t width, uint32_t height, const char *filename,
                 void *stream = nullptr)
@@ -57,9 +76,7 @@ void save_image(PixelBuf &temp, const RasterBuf &image, uint32_t width, uint32_t
     const size_t n = (size_t)width * height;
     if (temp.dedicated_len < n) throw std::runtime_error("save_image: pixel buffer too small");
     temp.ensure_gpu();
-    check(am_encode_rgba8(reinterpret_cast<const float *>(image.gpu_buffer_ptr), temp.gpu_buffer_ptr, (int64_t)n,
-                          stream),
-          "save_image");
+    check(encode_rgba8(image.gpu_buffer_ptr, temp.gpu_buffer_ptr, (int64_t)n, stream), "save_image");
     temp.copy_to_cpu(stream);
     check(am_write_png(filename, temp.cpu_buffer_ptr, width, height), "save_image");
 }

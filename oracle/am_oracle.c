@@ -32,7 +32,7 @@
  *   __compute_raised_christoffel_symbols  AdaptiveMesh/BSSN.h:55-91
  *   __rasterise_to_image       AdaptiveMesh/AMR.h:1313-1353
  *   __init_temp                AdaptiveMesh/kernel.cu:9-43
- *   ___rgba(float4)            AdaptiveMesh/image_process.h:14-21
+ *   ___rgba(float4/float/float3) AdaptiveMesh/image_process.h:14-38
  *
  * Extensions that the reference does not have, restated here so the product's
  * versions can be checked: FD order decoupled from the padding (the
@@ -1020,5 +1020,32 @@ EXPORT void orc_init_temp(float *old_d, float *new_d, const uint32_t *pos, const
                     old_d[o] = v;
                     new_d[o] = v;
                 }
+    }
+}
+
+/* ___rgba(float) and ___rgba(float3) of a decoded word (image_process.h:22-38) */
+EXPORT void orc_encode_rgba8_f32(const float *img, uint32_t *pixels, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) {
+        if (isnan(img[i])) {
+            pixels[i] = 4294902015u;
+            continue;
+        }
+        const uint32_t c = (uint32_t)(fclampf(img[i], 0.f, 1.f) * 255.f);
+        pixels[i] = (255u << 24) | (c << 16) | (c << 8) | c;
+    }
+}
+EXPORT void orc_encode_rgba8_cf3(const uint32_t *words, uint32_t *pixels, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) {
+        float v[3];
+        cf3_unpack(words[i], v);
+        if (isnan(v[0]) || isnan(v[1]) || isnan(v[2])) {
+            pixels[i] = 4294902015u;
+            continue;
+        }
+        uint32_t c[3];
+        for (int k = 0; k < 3; k++) c[k] = (uint32_t)(fclampf(v[k], 0.f, 1.f) * 255.f);
+        pixels[i] = (255u << 24) | (c[2] << 16) | (c[1] << 8) | c[0];
     }
 }

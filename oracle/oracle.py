@@ -50,6 +50,8 @@ _SIGS = {
     "orc_copy_to_child": (None, [_P, _u32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_rasterise": (None, [_P, _P, _u32, _u32, C.c_int, C.c_int, C.c_float, C.c_int, _P]),
     "orc_encode_rgba8": (None, [_P, _P, C.c_int64]),
+    "orc_encode_rgba8_f32": (None, [_P, _P, C.c_int64]),
+    "orc_encode_rgba8_cf3": (None, [_P, _P, C.c_int64]),
     "orc_init_temp": (None, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]),
     "orc_wave_stage": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, C.c_int, _f32]),
     "orc_deriv_trace": (None, [_P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
@@ -237,6 +239,22 @@ def encode_rgba8(raster):
     r = np.ascontiguousarray(raster, dtype=np.float32)
     out = np.empty(r.shape[:-1], np.uint32)
     lib().orc_encode_rgba8(_p(r), _p(out), out.size)
+    return out
+
+
+def encode_rgba8_f32(img):
+    """Grey RGBA8 words of a float image (image_process.h:31-38)."""
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    out = np.empty(a.shape, np.uint32)
+    lib().orc_encode_rgba8_f32(_p(a), _p(out), out.size)
+    return out
+
+
+def encode_rgba8_cf3(words):
+    """RGB RGBA8 words of compressed_float3 words (image_process.h:22-29)."""
+    a = np.ascontiguousarray(words, dtype=np.uint32)
+    out = np.empty(a.shape, np.uint32)
+    lib().orc_encode_rgba8_cf3(_p(a), _p(out), out.size)
     return out
 
 

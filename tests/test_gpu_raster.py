@@ -103,3 +103,18 @@ def test_driver_raster_and_png(am, orc, torch_cuda, tmp_path):
     got = bs.raster(0, which=0, width=64, height=48, z_slice=-0.2, boxes=False)
     assert np.array_equal(got, orc.encode_rgba8(orc.rasterise(chi, bs.export()["children"], 64, 48, S, P, -0.2,
                                                               False)))
+
+
+def test_encode_float_and_word_images(am, orc, torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(11)
+    img = rng.uniform(-0.5, 1.5, 4099).astype(np.float32)
+    img[::97] = np.nan
+    words = orc.cf3_pack_array(rng.uniform(-0.5, 1.5, (4099, 3)))
+    for arr, fn, want in ((img, "am_encode_rgba8_f32", orc.encode_rgba8_f32(img)),
+                          (words, "am_encode_rgba8_cf3", orc.encode_rgba8_cf3(words))):
+        d = dev(torch, arr)
+        px = torch.empty(arr.size, dtype=torch.int32, device="cuda")
+        am.check(getattr(am.lib(), fn)(d.data_ptr(), px.data_ptr(), arr.size, am.stream_handle()), fn)
+        torch.cuda.synchronize()
+        assert np.array_equal(px.cpu().numpy().view(np.uint32), want), fn

@@ -14,7 +14,7 @@ image_process.h:14-21), pinned by cases worked out by hand.
   when x mod 32 or y mod 32 is in {0, 1, 2, 30, 31} (|t| >= 1/2 - 4/64 in
   tile units), 1180 pixels.
 * Encoding: known words (grey 1/2 -> 0xFF7F7F7F, clamping, truncation,
-  magenta for NaN).
+  magenta for NaN) for 4-float pixels, floats and compressed_float3 words.
 """
 import numpy as np
 import pytest
@@ -103,3 +103,11 @@ def test_encode_known_words(orc):
     w = orc.encode_rgba8(r)
     assert [int(x) for x in w] == [0xFF7F7F7F, 0xFFFF00FF, 0xFFFE00FF, 0, 0xFFFFFFFF,
                                    (127 << 24) | (254 << 16) | (1 << 8), 0xFFFF00FF]
+
+
+def test_encode_grey_and_word_known_answers(orc):
+    g = orc.encode_rgba8_f32(np.array([.5, np.nan, -1, 2, .2], np.float32))
+    assert [int(x) for x in g] == [0xFF7F7F7F, 0xFFFF00FF, 0xFF000000, 0xFFFFFFFF, 0xFF333333]
+    w = [orc.cf3_pack(1., .5, 0.), orc.cf3_pack(-1., 2., .25), orc.cf3_pack(0., 0., 0.)]
+    c = orc.encode_rgba8_cf3(np.array(w, np.uint32))
+    assert [int(x) for x in c] == [0xFF007FFF, 0xFF3FFF00, 0xFF000000]
