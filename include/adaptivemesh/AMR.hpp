@@ -18,6 +18,8 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../adaptivemesh.h"
@@ -28,11 +30,59 @@ namespace am {
 
 constexpr uint32_t depth_limit = 8u;  // AMR.h:9
 
-// The reference's component bundles (AMR.h:428-633), named as there: one
-// buffer per component of a symmetric tensor field (xx, xy, xz, yy, yz, zz),
-// of symmetric compressed_float3 words, and of a Hessian's diagonal and
-// off-diagonal words.  operator[] walks the components in that order.
-struct tensor2_sym_field {
+// The reference's component bundles (AMR.h:350-633), named as there: one
+// buffer per component of a vector field (x, y, z), of a tensor field (xx ..
+// zz, row-major), of a symmetric tensor field (xx, xy, xz, yy, yz, zz), of
+// symmetric compressed_float3 words, and of a Hessian's diagonal and
+// off-diagonal words.  operator[] walks the components in that order.  The
+// reference's default for domain_cells is its compile-time tile size; the
+// tile size is a runtime value here, so the argument is required.
+template <class Bundle, class E, int N>
+struct field_bundle {
+    using component_type = E;
+    static constexpr int components = N;
+    smart_gpu_buffer<E> &operator[](int c) { return static_cast<Bundle *>(this)->*Bundle::member(c); }
+    const smart_gpu_buffer<E> &operator[](int c) const
+    {
+        return static_cast<const Bundle *>(this)->*Bundle::member(c);
+    }
+    void destroy()
+    {
+        for (int c = 0; c < N; c++) (*this)[c].destroy();
+    }
+};
+struct vector_field : field_bundle<vector_field, float, 3> {
+    smart_gpu_buffer<float> x, y, z;
+    vector_field() = default;
+    vector_field(size_t node_slots, size_t domain_cells)
+        : x(node_slots * domain_cells), y(node_slots * domain_cells), z(node_slots * domain_cells)
+    {
+    }
+    static smart_gpu_buffer<float> vector_field::*member(int c)
+    {
+        static smart_gpu_buffer<float> vector_field::*const m[3] = {&vector_field::x, &vector_field::y,
+                                                                    &vector_field::z};
+        return m[c];
+    }
+};
+struct tensor2_field : field_bundle<tensor2_field, float, 9> {
+    smart_gpu_buffer<float> xx, xy, xz, yx, yy, yz, zx, zy, zz;
+    tensor2_field() = default;
+    tensor2_field(size_t node_slots, size_t domain_cells)
+        : xx(node_slots * domain_cells), xy(node_slots * domain_cells), xz(node_slots * domain_cells),
+          yx(node_slots * domain_cells), yy(node_slots * domain_cells), yz(node_slots * domain_cells),
+          zx(node_slots * domain_cells), zy(node_slots * domain_cells), zz(node_slots * domain_cells)
+    {
+    }
+    static smart_gpu_buffer<float> tensor2_field::*member(int c)
+    {
+        static smart_gpu_buffer<float> tensor2_field::*const m[9] = {
+            &tensor2_field::xx, &tensor2_field::xy, &tensor2_field::xz, &tensor2_field::yx, &tensor2_field::yy,
+            &tensor2_field::yz, &tensor2_field::zx, &tensor2_field::zy, &tensor2_field::zz};
+        return m[c];
+    }
+};
+struct tensor2_sym_field : field_bundle<tensor2_sym_field, float, 6> {
     smart_gpu_buffer<float> xx, xy, xz, yy, yz, zz;
     tensor2_sym_field() = default;
     tensor2_sym_field(size_t node_slots, size_t domain_cells)
@@ -40,19 +90,15 @@ struct tensor2_sym_field {
           yy(node_slots * domain_cells), yz(node_slots * domain_cells), zz(node_slots * domain_cells)
     {
     }
-    smart_gpu_buffer<float> &operator[](int c)
+    static smart_gpu_buffer<float> tensor2_sym_field::*member(int c)
     {
-        smart_gpu_buffer<float> tensor2_sym_field::*const m[6] = {
+        static smart_gpu_buffer<float> tensor2_sym_field::*const m[6] = {
             &tensor2_sym_field::xx, &tensor2_sym_field::xy, &tensor2_sym_field::xz,
             &tensor2_sym_field::yy, &tensor2_sym_field::yz, &tensor2_sym_field::zz};
-        return this->*m[c];
-    }
-    void destroy()
-    {
-        for (int c = 0; c < 6; c++) (*this)[c].destroy();
+        return m[c];
     }
 };
-struct comp_tensor3_sym_field {
+struct comp_tensor3_sym_field : field_bundle<comp_tensor3_sym_field, compressed_float3, 6> {
     smart_gpu_buffer<compressed_float3> xx, xy, xz, yy, yz, zz;
     comp_tensor3_sym_field() = default;
     comp_tensor3_sym_field(size_t node_slots, size_t domain_cells)
@@ -60,30 +106,24 @@ struct comp_tensor3_sym_field {
           yy(node_slots * domain_cells), yz(node_slots * domain_cells), zz(node_slots * domain_cells)
     {
     }
-    smart_gpu_buffer<compressed_float3> &operator[](int c)
+    static smart_gpu_buffer<compressed_float3> comp_tensor3_sym_field::*member(int c)
     {
-        smart_gpu_buffer<compressed_float3> comp_tensor3_sym_field::*const m[6] = {
+        static smart_gpu_buffer<compressed_float3> comp_tensor3_sym_field::*const m[6] = {
             &comp_tensor3_sym_field::xx, &comp_tensor3_sym_field::xy, &comp_tensor3_sym_field::xz,
             &comp_tensor3_sym_field::yy, &comp_tensor3_sym_field::yz, &comp_tensor3_sym_field::zz};
-        return this->*m[c];
-    }
-    void destroy()
-    {
-        for (int c = 0; c < 6; c++) (*this)[c].destroy();
+        return m[c];
     }
 };
-struct comp_tensor2_sym_field {
+struct comp_tensor2_sym_field : field_bundle<comp_tensor2_sym_field, compressed_float3, 2> {
     smart_gpu_buffer<compressed_float3> diag, off_diag;
     comp_tensor2_sym_field() = default;
     comp_tensor2_sym_field(size_t node_slots, size_t domain_cells)
         : diag(node_slots * domain_cells), off_diag(node_slots * domain_cells)
     {
     }
-    smart_gpu_buffer<compressed_float3> &operator[](int c) { return c ? off_diag : diag; }
-    void destroy()
+    static smart_gpu_buffer<compressed_float3> comp_tensor2_sym_field::*member(int c)
     {
-        diag.destroy();
-        off_diag.destroy();
+        return c ? &comp_tensor2_sym_field::off_diag : &comp_tensor2_sym_field::diag;
     }
 };
 
@@ -674,6 +714,94 @@ inline int copy_to_child(smart_gpu_buffer<float> &buffer, uint32_t offset, int c
                          const am_geometry &g, void *stream, int interp = AM_INTERP_LAGRANGE)
 {
     return am_copy_to_child(buffer.gpu_buffer_ptr, offset, child_idx, parent_idx, &g, interp, stream);
+}
+
+// ------------------------------------------------- reference-shaped forms
+// The wrappers as the reference's plugins call them: with the plugin's AMR<>
+// and a stream (or none: the default stream), AMR.h:881-899, 998-1018,
+// 1268-1302, the tile geometry taken from the plugin's `geom`.
+template <class D>
+int differentiate(const smart_gpu_buffer<float> &field, smart_gpu_buffer<compressed_float3> &dif, AMR<D> &amr,
+                  void *stream, fast_prng &rng)
+{
+    return differentiate(field, dif, amr, amr.sim_data.geom, stream, rng);
+}
+template <class D>
+int differentiate(const smart_gpu_buffer<float> &field, smart_gpu_buffer<compressed_float3> &dif,
+                  smart_gpu_buffer<compressed_float3> &d2if, AMR<D> &amr, void *stream, fast_prng &rng)
+{
+    return differentiate(field, dif, d2if, amr, amr.sim_data.geom, stream, rng);
+}
+template <class D>
+int mixed_partials(const smart_gpu_buffer<float> &field, smart_gpu_buffer<compressed_float3> &off, AMR<D> &amr,
+                   void *stream, fast_prng &rng)
+{
+    return mixed_partials(field, off, amr, amr.sim_data.geom, stream, rng);
+}
+template <class D>
+int copy_boundaries(smart_gpu_buffer<float> &old_bfr, smart_gpu_buffer<float> &new_bfr, AMR<D> &amr, void *stream,
+                    bool copy_only_new = true)
+{
+    return copy_boundaries(old_bfr, new_bfr, amr, amr.sim_data.geom, stream, copy_only_new);
+}
+template <class D>
+int copy_boundaries(smart_gpu_buffer<float> &old_bfr, smart_gpu_buffer<float> &new_bfr, AMR<D> &amr,
+                    bool copy_only_new = true)
+{
+    return copy_boundaries(old_bfr, new_bfr, amr, amr.sim_data.geom, nullptr, copy_only_new);
+}
+template <class D>
+int copy_new_to_old(smart_gpu_buffer<float> &old_bfr, const smart_gpu_buffer<float> &new_bfr, AMR<D> &amr,
+                    void *stream = nullptr)
+{
+    return copy_new_to_old(old_bfr, new_bfr, amr, amr.sim_data.geom, stream);
+}
+
+// The component-bundle forms over a stream pool (AMR.h:902-933, 1020-1051,
+// 1083-1115, 1304-1310): one launch per component, each on the pool's next
+// stream.  As in the reference, launches on different streams are ordered
+// neither with each other nor with later work: cuda_sync() before the
+// results are read.  copy_to_child takes the tile geometry, which the
+// reference's form reads from its compile-time constants.
+template <class B>
+using float_bundle = std::enable_if_t<std::is_same<typename B::component_type, float>::value, int>;
+
+template <class B, class D, float_bundle<B> = 0>
+int copy_boundaries(B &old_field, B &new_field, AMR<D> &amr, round_robin_threads &threads)
+{
+    for (int c = 0; c < B::components; c++)
+        if (int rc = copy_boundaries(old_field[c], new_field[c], amr, amr.sim_data.geom, threads.yield_stream()))
+            return rc;
+    return AM_OK;
+}
+template <class B, class D, float_bundle<B> = 0>
+int copy_new_to_old(B &old_field, const B &new_field, AMR<D> &amr, round_robin_threads &threads)
+{
+    for (int c = 0; c < B::components; c++)
+        if (int rc = copy_new_to_old(old_field[c], new_field[c], amr, amr.sim_data.geom, threads.yield_stream()))
+            return rc;
+    return AM_OK;
+}
+template <class B, float_bundle<B> = 0>
+int copy_to_child(B &field, uint32_t offset, int child_idx, int parent_idx, const am_geometry &g,
+                  round_robin_threads &threads)
+{
+    for (int c = 0; c < B::components; c++)
+        if (int rc = copy_to_child(field[c], offset, child_idx, parent_idx, g, threads.yield_stream())) return rc;
+    return AM_OK;
+}
+// AMR.h:1304-1310 launches differentiate and mixed_partials on two streams of
+// the pool; here they are one fused launch on the first of the two, with the
+// seeds drawn in the same order, so the words are the same.
+template <class D>
+int yield_first_second_derivs(const smart_gpu_buffer<float> &field, smart_gpu_buffer<compressed_float3> &gradient,
+                              comp_tensor2_sym_field &hessian, AMR<D> &amr, round_robin_threads &threads,
+                              fast_prng &rng)
+{
+    void *stream = threads.yield_stream();
+    threads.yield_stream();
+    return yield_first_second_derivs(field, gradient, hessian.diag, hessian.off_diag, amr, amr.sim_data.geom, stream,
+                                     rng);
 }
 
 }  // namespace am

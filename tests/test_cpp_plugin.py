@@ -7,7 +7,10 @@ saved PNG must equal the same run driven through the Python bindings, and
 the BSSN program (the reference-shaped BSSN_AMR_data plugin: metric
 derivatives, traces and Christoffel words over a 6-level nested mesh, two
 timesteps) must equal the oracle driven through the same phases and seeds,
-bit for bit."""
+bit for bit.  tests/cpp/stream_pool_demo.cpp drives component bundles through
+the wrappers' stream-pool forms (AMR.h:902-933, 1020-1051, 1083-1115,
+1304-1310) and through their one-buffer forms on the default stream; the two
+runs must leave every buffer and the seed the same."""
 import subprocess
 import zlib
 from pathlib import Path
@@ -20,7 +23,8 @@ from tests.test_png import chunks
 
 ROOT = Path(__file__).resolve().parents[1]
 DEMOS = {"wave": ROOT / "tests" / "cpp" / "wave_plugin_demo.cpp",
-         "bssn": ROOT / "tests" / "cpp" / "bssn_plugin_demo.cpp"}
+         "bssn": ROOT / "tests" / "cpp" / "bssn_plugin_demo.cpp",
+         "pool": ROOT / "tests" / "cpp" / "stream_pool_demo.cpp"}
 
 
 def build(tmp_path, werror=False, demo="wave"):
@@ -173,3 +177,20 @@ def test_bssn_demo_matches_oracle(am, orc, torch_cuda, tmp_path):
         full = np.zeros(128 * S ** 3, np.uint32)
         full[:used * S ** 3] = want["cG"][c][:used * S ** 3]
         assert np.array_equal(px.reshape(-1)[:used * S ** 3], orc.encode_rgba8_cf3(full)[:used * S ** 3]), nm
+
+
+@pytest.mark.gpu
+def test_stream_pool_forms_match_serial_forms(am, torch_cuda, tmp_path):
+    exe = build(tmp_path, demo="pool")
+    lines = {}
+    for mode in ("serial", "pool"):
+        print(f"running the stream-pool demo ({mode})", flush=True)
+        r = subprocess.run([str(exe), mode], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        lines[mode] = r.stdout.strip().splitlines()
+    assert lines["pool"] == lines["serial"]
+    rows = {ln.split()[0]: ln.split()[1:] for ln in lines["pool"]}
+    assert len(rows) == 3 * 2 + 9 * 2 + 6 * 2 + 3 + 1
+    for name in ("gradient", "hessian.diag", "hessian.off_diag", "metric.xx.new", "k.zz.old", "shift.x.new"):
+        assert int(rows[name][1]) > 0, name  # nonzero words: the phases ran
+    assert int(rows["seed"][0]) != 12894712 and int(rows["seed"][2]) == 15
