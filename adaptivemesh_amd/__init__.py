@@ -66,7 +66,8 @@ class FdBatch(C.Structure):
                 ("d2diag", C.c_void_p * AM_MAX_VARS), ("d2off", C.c_void_p * AM_MAX_VARS),
                 ("seed_d", C.c_int32 * AM_MAX_VARS), ("seed_m", C.c_int32 * AM_MAX_VARS),
                 ("n_vars", C.c_int32), ("z_block", C.c_int32), ("seed_table", C.c_void_p),
-                ("word_format", C.c_int32), ("prefetch_planes", C.c_int32)]
+                ("word_format", C.c_int32), ("prefetch_planes", C.c_int32),
+                ("direct_faces", C.c_int32)]
 
 
 class FieldBatch(C.Structure):
@@ -82,7 +83,8 @@ class BssnParams(C.Structure):
 class BssnFields(C.Structure):
     _fields_ = [("state", C.c_void_p * 24), ("old_", C.c_void_p * 24), ("new_", C.c_void_p * 24),
                 ("acc", C.c_void_p * 24), ("d1", C.c_void_p * 24), ("d2diag", C.c_void_p * 11),
-                ("d2off", C.c_void_p * 11), ("word_format", C.c_int32), ("push_layers", C.c_int32)]
+                ("d2off", C.c_void_p * 11), ("word_format", C.c_int32), ("push_layers", C.c_int32),
+                ("push_axes", C.c_int32)]
 
 
 class BssnSimOptions(C.Structure):
@@ -105,7 +107,8 @@ class MlpWork(C.Structure):
 
 class SwaveFields(C.Structure):
     _fields_ = [("state", C.c_void_p * 2), ("old_", C.c_void_p * 2), ("new_", C.c_void_p * 2),
-                ("acc", C.c_void_p * 2), ("d1_u", C.c_void_p), ("d2diag_u", C.c_void_p), ("push_layers", C.c_int32)]
+                ("acc", C.c_void_p * 2), ("d1_u", C.c_void_p), ("d2diag_u", C.c_void_p), ("push_layers", C.c_int32),
+                ("push_axes", C.c_int32)]
 
 
 class SwaveSimOptions(C.Structure):
@@ -252,6 +255,8 @@ _RK4_SIM_SIGS = {
     "set_fd_prefetch": (C.c_int, [_P, C.c_int32]),
     "set_push_layers": (C.c_int, [_P, C.c_int32]),
     "get_push_layers": (C.c_int, [_P]),
+    "set_direct_faces": (C.c_int, [_P, C.c_int32]),
+    "get_direct_faces": (C.c_int, [_P]),
     "use_graph": (C.c_int, [_P, C.c_int32]),
     "autotune": (C.c_int, [_P, C.c_int32, _P]),
     "set_prolongation": (C.c_int, [_P, _P, C.c_float]),
@@ -609,6 +614,16 @@ class _Rk4Sim:
     @property
     def push_layers(self) -> int:
         return self._f("get_push_layers")(self.h)
+
+    def set_direct_faces(self, axes: int):
+        """Axes (bits 1 x, 2 y, 4 z; default 4) whose equal-depth faces
+        stages 0-2 do not push: the next stage's FD reads them from the
+        neighbours.  Stage 3 pushes every face."""
+        check(self._f("set_direct_faces")(self.h, int(axes)), "set_direct_faces")
+
+    @property
+    def direct_faces(self) -> int:
+        return self._f("get_direct_faces")(self.h)
 
     def get_launch(self) -> tuple:
         """(z_block, fd_grid_limit, rhs_grid_limit, fd_prefetch)"""

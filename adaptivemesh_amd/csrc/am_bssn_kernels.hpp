@@ -22,6 +22,7 @@ struct BssnArgsDev {
     const void *d2d[NH];
     const void *d2o[NH];
     int push_layers;
+    int push_axes;  // bits 1 x, 2 y, 4 z
 };
 
 struct BssnParDev {
@@ -305,7 +306,7 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
         float k[NV];
         bssn_cell<FMT, true, false>(a, r, w, par, k, nullptr, nullptr);
         rk4_combine<NV, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
-        if (PUSH) rk4_push<NV, S, P>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers);
+        if (PUSH) rk4_push<NV, S, P>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes);
     }
 }
 
@@ -409,6 +410,9 @@ int launch_stage_fmt(const am_bssn_fields *f, int stage, const am_bssn_params *p
     if (f->push_layers < 0 || f->push_layers > P)
         return set_error(AM_ERR_INVALID, "am_bssn_rhs_stage: push_layers %d (0..%d)", f->push_layers, P);
     a.push_layers = f->push_layers ? f->push_layers : P;
+    if (f->push_axes < 0 || f->push_axes > 7)
+        return set_error(AM_ERR_INVALID, "%s: push_axes %d (0..7)", "am_bssn_rhs_stage", f->push_axes);
+    a.push_axes = f->push_axes ? f->push_axes : 7;
     const int n_items = mesh->n_tiles * BPT;
     TileArgs m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth, n_items};
     const DepthScales sc = depth_scales(*g);

@@ -36,7 +36,42 @@ struct MeshDev {
     uint32_t substep;
     uint32_t max_depth;
     int32_t n_items;  // tiles x z-blocks; workgroups loop over them
+    // non-null: ghost cells across a face of the axes in direct_axes (bits
+    // 1 x, 2 y, 4 z) with an equal-depth neighbour are read from that
+    // neighbour's interior instead of the tile's own ghost layers (which
+    // then need not be current)
+    const uint64_t *direct;
+    uint32_t direct_axes;
 };
+
+// Which of the four floats of float4 i of a padded plane are ghost cells the
+// stencils read (the Q layers next to the interior) across the -x, +x, -y
+// and +y faces, one 4-bit mask each in bits 0-3, 4-7, 8-11, 12-15.  Edge
+// cells (ghost along both axes) are in none: no stencil reads them.
+template <int S, int P, int Q>
+__device__ __forceinline__ uint32_t face_ghost_masks(int i)
+{
+    constexpr int T = S + 2 * P;
+    uint32_t m = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int f = 4 * i + k, r = f / T, c = f - r * T;
+        const bool rin = r >= P && r < P + S, cin = c >= P && c < P + S;
+        if (rin && c >= P - Q && c < P) m |= 1u << k;
+        if (rin && c >= P + S && c < P + S + Q) m |= 1u << (4 + k);
+        if (cin && r >= P - Q && r < P) m |= 1u << (8 + k);
+        if (cin && r >= P + S && r < P + S + Q) m |= 1u << (12 + k);
+    }
+    return m;
+}
+__device__ __forceinline__ float4 take(float4 v, float4 t, uint32_t mask)
+{
+    if (mask & 1u) v.x = t.x;
+    if (mask & 2u) v.y = t.y;
+    if (mask & 4u) v.z = t.z;
+    if (mask & 8u) v.w = t.w;
+    return v;
+}
 
 // __mixed_partial_c / _bd (AMR.h:1221-1230) on LDS rows: c = centre, the
 // other arguments are the neighbours named by their offsets.
@@ -60,6 +95,12 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     const int32_t gs_d = b.seed_table ? b.seed_table[b.seed_d[var]] : b.seed_d[var];
     const int32_t gs_m = b.seed_table ? b.seed_table[b.seed_m[var]] : b.seed_m[var];
 
+    static_assert(S % 4 == 0, "direct reads shift float4 indices by S/4");
+    uint32_t gmask[(T2 / 4 + S * S - 1) / (S * S)];
+#pragma unroll
+    for (int j = 0; j < (T2 / 4 + S * S - 1) / (S * S); j++)
+        gmask[j] = (int)threadIdx.x + j * S * S < T2 / 4 ? face_ghost_masks<S, P, Q>((int)threadIdx.x + j * S * S) : 0u;
+
     // one work item = one (tile, z-block); a persistent launch loops over
     // them (a separate instantiation: the loop keeps kernel arguments live)
     const int item_end = PERSIST ? m.n_items : (int)blockIdx.x + 1;
@@ -70,6 +111,18 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     if (depth == -1 || !active_depth(m.substep, (uint32_t)depth, m.max_depth)) continue;  // uniform per workgroup
 
     const float *src = b.field[var] + (size_t)slot * (T * T2);
+    // face f (0-2: +x +y +z, 3-5: -x -y -z) -> the equal-depth neighbour's
+    // tile, or null: its ghost layers are read where they lie
+    const float *nb[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (m.direct) {
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            const uint32_t nd = (uint32_t)m.direct[(size_t)slot * 6 + f];
+            const int tgt = bnd_target(nd);
+            if (((m.direct_axes >> (f % 3)) & 1u) && tgt >= 0 && bnd_depth(nd) > 0 && bnd_ddiff(nd) == 0u)
+                nb[f] = b.field[var] + (size_t)tgt * (T * T2);
+        }
+    }
     uint32_t *o_d1 = b.d1[var], *o_d2 = b.d2diag[var], *o_off = b.d2off[var];
     const bool want_d = o_d1 != nullptr, want_m = o_off != nullptr;
     const float dx1 = sc.inv_dx[depth];
@@ -86,14 +139,43 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     // z..z+2P; the slot of plane z+2P+1 is (z-1) % R, which nobody reads in
     // iteration z, so one barrier per iteration suffices.  Only planes
     // z0..z0+ZB-1+2P are ever read.
+    //
+    // With direct reads, a ghost plane across a z face is the neighbour's
+    // interior plane S away, and in an interior plane the ghost columns
+    // (rows) across an x (y) face are the neighbour's interior columns (rows)
+    // S away: the same float4 index shifted by S/4 (S*T/4), as S % 4 == 0.
     constexpr int NL = (T2 / 4 + S * S - 1) / (S * S);
     float4 pre[PF][NL];
     auto gload = [&](float4 *dst, int zp) {
-        const float4 *s4 = reinterpret_cast<const float4 *>(src + (size_t)zp * T2);
+        const float *base = src;
+        int zs = zp;
+        if (zp < P && nb[5]) {
+            base = nb[5];
+            zs = zp + S;
+        } else if (zp >= P + S && nb[2]) {
+            base = nb[2];
+            zs = zp - S;
+        }
+        const float4 *s4 = reinterpret_cast<const float4 *>(base + (size_t)zs * T2);
+        const bool inner = zp >= P && zp < P + S;
 #pragma unroll
         for (int j = 0; j < NL; j++) {
             const int i = tid + j * S * S;
-            if (i < T2 / 4) dst[j] = s4[i];
+            if (i < T2 / 4) {
+                float4 v = s4[i];
+                const uint32_t gm = gmask[j];
+                if (inner && gm) {
+                    const size_t pz = (size_t)zp * (T2 / 4);
+                    if ((gm & 15u) && nb[3]) v = take(v, reinterpret_cast<const float4 *>(nb[3])[pz + i + S / 4], gm);
+                    if ((gm & 240u) && nb[0])
+                        v = take(v, reinterpret_cast<const float4 *>(nb[0])[pz + i - S / 4], gm >> 4);
+                    if ((gm & 3840u) && nb[4])
+                        v = take(v, reinterpret_cast<const float4 *>(nb[4])[pz + i + S * T / 4], gm >> 8);
+                    if ((gm & 61440u) && nb[1])
+                        v = take(v, reinterpret_cast<const float4 *>(nb[1])[pz + i - S * T / 4], gm >> 12);
+                }
+                dst[j] = v;
+            }
         }
     };
     auto lstore = [&](const float4 *from, int zp) {
@@ -256,8 +338,13 @@ static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometr
     // grid.x = work items, capped by the persistent limit (split over the
     // variables on grid.y)
     auto items = [&](int nzb) {
-        return MeshDev{mesh->positions, mesh->tiles, mesh->substep, mesh->max_depth, mesh->n_tiles * nzb};
+        return MeshDev{mesh->positions, mesh->tiles, mesh->substep, mesh->max_depth, mesh->n_tiles * nzb,
+                       b->direct_faces ? mesh->boundaries : nullptr, (uint32_t)b->direct_faces};
     };
+    if (b->direct_faces < 0 || b->direct_faces > 7)
+        return set_error(AM_ERR_INVALID, "am_fd: direct_faces %d (axis mask 0..7)", b->direct_faces);
+    if (b->direct_faces && !mesh->boundaries)
+        return set_error(AM_ERR_INVALID, "am_fd: direct_faces needs mesh->boundaries");
     auto gx = [&](int nzb) {
         const int n = mesh->n_tiles * nzb;
         if (mesh->grid_limit <= 0) return n;

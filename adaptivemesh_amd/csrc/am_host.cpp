@@ -516,6 +516,16 @@ int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
         });                                                                                                        \
     }                                                                                                              \
     int P##_get_push_layers(const H *s) { return s ? s->amr->sim_data.push_layers() : -1; }                       \
+    int P##_set_direct_faces(H *s, int32_t enable)                                                                 \
+    {                                                                                                              \
+        return rk4_call(s, [&](auto &d) {                                                                          \
+            if (enable < 0 || enable > 7)                                                                          \
+                throw std::runtime_error("direct faces axis mask " + std::to_string(enable) + " (0..7)");          \
+            d.opt.direct_faces = enable;                                                                               \
+            d.drop_graph();                                                                                        \
+        });                                                                                                        \
+    }                                                                                                              \
+    int P##_get_direct_faces(const H *s) { return s ? s->amr->sim_data.opt.direct_faces : -1; }                   \
     int P##_set_fd_prefetch(H *s, int32_t planes)                                                                  \
     {                                                                                                              \
         if (planes < 1 || planes > 3) return am::set_error(AM_ERR_INVALID, "fd prefetch %d (1-3)", planes);        \

@@ -174,8 +174,11 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
         }
         const size_t wn = (size_t)node * CELLS + w0 + ((size_t)w1 + (size_t)w2 * T) * T;
         if (ddiff == 0 && equal_mode == AM_EQUAL_COPY) {
-            // the neighbour cell the ghost maps onto (rp is an exact integer)
-            const size_t src = (size_t)tgt * CELLS + (size_t)rp[0] + ((size_t)rp[1] + (size_t)rp[2] * T) * T;
+            // the neighbour cell the ghost maps onto: rp is a whole cell index,
+            // exactly for power-of-two S and within rounding otherwise (1/S
+            // inexact), so it is rounded, not truncated
+            const size_t src = (size_t)tgt * CELLS + (size_t)(rp[0] + .5f) +
+                               ((size_t)(rp[1] + .5f) + (size_t)(rp[2] + .5f) * T) * T;
             for (int v = 0; v < nv; v++) b.new_[v0 + v][wn] = b.new_[v0 + v][src];
             continue;
         }

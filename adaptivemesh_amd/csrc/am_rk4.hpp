@@ -46,14 +46,14 @@ __device__ __forceinline__ void rk4_combine(const float *const *state, const flo
 
 template <int NV, int S, int P>
 __device__ __forceinline__ void rk4_push(float *const *new_, const float (&y)[NV], const uint64_t *boundaries, int node,
-                                         int x, int y_, int z, int layers)
+                                         int x, int y_, int z, int layers, int axes)
 {
     constexpr int T = S + 2 * P;
     const int c3[3] = {x, y_, z};
 #pragma unroll
     for (int f = 0; f < 6; f++) {
         const int ax = f % 3;
-        const bool near = f < 3 ? c3[ax] >= S - layers : c3[ax] < layers;
+        const bool near = (f < 3 ? c3[ax] >= S - layers : c3[ax] < layers) && ((axes >> ax) & 1);
         if (!near) continue;
         const uint32_t nd = (uint32_t)boundaries[(size_t)node * 6 + f];
         const int tgt = bnd_target(nd);

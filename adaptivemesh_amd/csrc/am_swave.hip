@@ -17,6 +17,7 @@ struct SwaveArgsDev {
     const uint32_t *d1;
     const uint32_t *d2;
     int push_layers;
+    int push_axes;  // bits 1 x, 2 y, 4 z
 };
 
 struct SwaveTiles {
@@ -45,7 +46,7 @@ __global__ void __launch_bounds__(256) k_swave_stage(SwaveArgsDev a, SwaveTiles 
         const float3 h = cf3_unpack(a.d2[w]);
         float k[2] = {a.state[1][r], (h.x + h.y) + h.z};
         rk4_combine<2, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
-        if (PUSH) rk4_push<2, S, P>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers);
+        if (PUSH) rk4_push<2, S, P>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes);
     }
 }
 
@@ -134,6 +135,9 @@ static int swave_stage(const am_swave_fields *f, int stage, const am_mesh *mesh,
     if (f->push_layers < 0 || f->push_layers > P)
         return set_error(AM_ERR_INVALID, "am_swave_rhs_stage: push_layers %d (0..%d)", f->push_layers, P);
     a.push_layers = f->push_layers ? f->push_layers : P;
+    if (f->push_axes < 0 || f->push_axes > 7)
+        return set_error(AM_ERR_INVALID, "%s: push_axes %d (0..7)", "am_swave_rhs_stage", f->push_axes);
+    a.push_axes = f->push_axes ? f->push_axes : 7;
     const int n_items = mesh->n_tiles * BPT;
     const SwaveTiles m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth, n_items};
     const DepthScales sc = depth_scales(*g);

@@ -49,18 +49,31 @@ PUSH = 6 * PUSH_LAYERS * S * S * 24 * 4 / S ** 3  # equal-depth face ghosts writ
 # algorithmic bytes of the same launches average 1.392e9.
 RHS_TRAFFIC_PER_LAUNCH = 1.940e9
 RHS_ALGORITHMIC_PER_LAUNCH = 1.392e9
-BYTES = {
-    # per stage, 24 fields: padded reads (S^3 + 6 P S^2) * 4 per field; 24 gradient + 11 diag + 11 off-diag words
-    "fd": 24 * (S ** 3 + 6 * P * S * S) * 4 / S ** 3 + 46 * WORD,
-    # per RK4 step, the four stages summed: state 96 and 46 words 184 read in every stage; y_n 96 and acc 96
-    # read in stages 1-3; new 96 written in every stage, acc 96 written in stages 0-2; the pushed face ghosts
-    "rhs_stage": (96 + 184 + 192 + PUSH) + (96 + 184 + 96 + 96 + 192 + PUSH) * 2 + (96 + 184 + 96 + 96 + 96 + PUSH),
-    # per stage: 6 P S^2 face cells per tile, one source read and one write, 24 fields (coarse faces only)
-    "ghost": 24 * 6 * P * S * S * 8 / S ** 3,
-    # per step: interior read of new + interior write of old per field (no tile of this mesh is the coarse
-    # source of a finer face, so the copy-back leaves ghost cells alone)
-    "copy_back": 24 * 2 * 4,
-}
+
+
+def bytes_model(direct_faces):
+    """Algorithmic bytes per interior cell and launch class.  direct_faces is
+    the driver's axis mask (bits 1 x, 2 y, 4 z) of the equal-depth faces that
+    stages 0-2 do not push (the next stage's FD reads them from the
+    neighbours: the same number of distinct elements as the pushed copies)."""
+    push_early = PUSH * (3 - bin(direct_faces & 7).count("1")) / 3
+    return {
+        # per stage, 24 fields: padded reads (S^3 + 6 P S^2) * 4 per field; 24 gradient + 11 diag + 11 off-diag
+        # words
+        "fd": 24 * (S ** 3 + 6 * P * S * S) * 4 / S ** 3 + 46 * WORD,
+        # per RK4 step, the four stages summed: state 96 and 46 words 184 read in every stage; y_n 96 and acc 96
+        # read in stages 1-3; new 96 written in every stage, acc 96 written in stages 0-2; the pushed face ghosts
+        # (stage 3 pushes every face)
+        "rhs_stage": (96 + 184 + 192 + push_early) + (96 + 184 + 96 + 96 + 192 + push_early) * 2
+                     + (96 + 184 + 96 + 96 + 96 + PUSH),
+        # per stage: 6 P S^2 face cells per tile, one source read and one write, 24 fields (coarse faces only)
+        "ghost": 24 * 6 * P * S * S * 8 / S ** 3,
+        # per step: interior read of new + interior write of old per field (no tile of this mesh is the coarse
+        # source of a finer face, so the copy-back leaves ghost cells alone)
+        "copy_back": 24 * 2 * 4,
+    }
+
+
 WORKLOADS = {
     "bssn512": "BSSN single level: 512 tiles of 16^3 (128^3) + 73 ancestors, P=3, 24 fp32 fields, 1+log + "
                "Gamma-driver, flat + 1e-6 noise, cf3 derivative words (Christoffel symbols raised in registers), RK4, "
@@ -187,6 +200,9 @@ def main():
     ap.add_argument("--z-block", type=int, default=0, help="fixed FD z-block (skips the autotuner)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="bssn512")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--direct-faces", type=int, default=None, choices=range(8),
+                    help="axis mask (1 x, 2 y, 4 z) of the equal-depth faces RK4 stages 0-2 leave to the next FD "
+                         "(default: the driver's)")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a recorded step")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-versus-plain ghost fill comparison (training + two puncture runs)")
@@ -222,6 +238,8 @@ def main():
         for p_, c in ops:
             sim.add_node(p_, c)
         fields = WL.puncture_fields(sim.export()["positions"], S, P, OUTER)
+    if args.direct_faces is not None:
+        sim.set_direct_faces(args.direct_faces)
     for v in range(24):
         sim.set_field(0, v, fields[v])
         sim.set_field(1, v, fields[v])
@@ -251,7 +269,8 @@ def main():
     value = zones / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     classes = WORKLOAD_CLASSES if args.workload == "bssn512" else WORKLOAD_CLASSES + ("ghost",)
-    bytes_per_zone = sum(BYTES[k] * STAGES_PER_STEP[k] for k in WORKLOAD_CLASSES)
+    BYTES_ = bytes_model(sim.direct_faces)
+    bytes_per_zone = sum(BYTES_[k] * STAGES_PER_STEP[k] for k in WORKLOAD_CLASSES)
     whole_gbs = zones * bytes_per_zone / elapsed / 1e9
 
     roofline = None
@@ -272,9 +291,9 @@ def main():
                 continue
             kernels[k] = {"ms_total": round(ms, 4), "launches": cnt, "avg_us": round(1e3 * ms / cnt, 2)}
             if k in WORKLOAD_CLASSES:
-                total_bytes = zp * BYTES[k] * STAGES_PER_STEP[k]
+                total_bytes = zp * BYTES_[k] * STAGES_PER_STEP[k]
                 kernels[k].update(GBps=round(total_bytes / (ms * 1e-3) / 1e9, 1),
-                                  bytes_per_zone_cycle=round(BYTES[k] * STAGES_PER_STEP[k], 1))
+                                  bytes_per_zone_cycle=round(BYTES_[k] * STAGES_PER_STEP[k], 1))
         dom = max((k for k in kernels if "GBps" in kernels[k]), key=lambda k: kernels[k]["ms_total"])
         d = kernels[dom]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -296,6 +315,7 @@ def main():
                    "launch": {"z_block": sim.get_launch()[0], "fd_grid_limit": sim.get_launch()[1],
                               "rhs_grid_limit": sim.get_launch()[2], "fd_prefetch_planes": sim.get_launch()[3],
                               "push_layers": sim.push_layers,
+                              "direct_faces": sim.direct_faces,
                               "autotuned": tune is not None,
                               "autotune_trials": tune["trials"] if tune else None,
                               "graph_per_coarse_step": not args.no_graph}},

@@ -184,6 +184,13 @@ typedef struct am_fd_batch {
     /* padded planes each workgroup prefetches into registers ahead of the
      * plane it differentiates: 0 or 1 (default), 2 or 3; cf3 words only */
     int32_t prefetch_planes;
+    /* axes (bits 1 x, 2 y, 4 z; mesh->boundaries required when nonzero)
+     * whose ghost cells across a face with an equal-depth neighbour (the
+     * faces the RK4 stage kernels push) are read from that neighbour's
+     * interior, so the tile's own ghost layers there need not be current.
+     * The words equal those of a field whose ghost layers hold the pushed
+     * copies. */
+    int32_t direct_faces;
 } am_fd_batch;
 enum am_word_format { AM_WORD_CF3 = 0, AM_WORD_F32 = 1, AM_WORD_F16 = 2 };
 /* bytes per cell of one word array in `format` (4, 12, 6), -1 if unknown */
@@ -353,6 +360,7 @@ typedef struct am_bssn_fields {
     const uint32_t *d2off[AM_BSSN_NHESS];
     int32_t word_format; /* enum am_word_format of d1/d2diag/d2off (non-cf3: S=16, P=3 only) */
     int32_t push_layers; /* ghost layers the push writes, innermost first: 1..P, 0 = all P */
+    int32_t push_axes;   /* faces the push writes, by axis: bits 1 x, 2 y, 4 z; 0 = all three */
 } am_bssn_fields;
 
 /* One classical RK4 stage on the interior of every listed active tile:
@@ -387,6 +395,7 @@ typedef struct am_swave_fields {
     const uint32_t *d1_u;     /* gradient word of the stage input u */
     const uint32_t *d2diag_u; /* (u_xx, u_yy, u_zz) word of the stage input u */
     int32_t push_layers;      /* as am_bssn_fields.push_layers */
+    int32_t push_axes;        /* as am_bssn_fields.push_axes */
 } am_swave_fields;
 
 int am_swave_rhs_stage(const am_swave_fields *f, int32_t stage, const am_mesh *mesh, const am_geometry *g,

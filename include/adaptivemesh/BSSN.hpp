@@ -135,6 +135,7 @@ struct bssn_system {
         am_bssn_fields f{};
         f.word_format = v.word_format;
         f.push_layers = v.push_layers;
+        f.push_axes = v.push_axes;
         for (int i = 0; i < NV; i++) {
             f.state[i] = v.state[i];
             f.old_[i] = v.old_[i];

@@ -98,6 +98,11 @@ struct rk4_options {
     int32_t word_format = AM_WORD_CF3;    // derivative-word storage (am_word_format)
     int32_t fd_prefetch = 1;              // FD planes prefetched ahead (1-3, cf3 words)
     int32_t push_layers = 0;              // equal-depth ghost layers pushed (0 = the FD half-width)
+    // axes (bits 1 x, 2 y, 4 z) whose equal-depth faces stages 0-2 do not
+    // push; the next stage's FD reads them from the neighbours' interiors.
+    // Stage 3 pushes every face, so the ghost layers are current after
+    // every fine step whatever the mask.
+    int32_t direct_faces = 4;
 };
 
 // What autotune() measured: each trial's launch shape and mean ms per launch.
@@ -123,6 +128,7 @@ struct rk4_views {
     const uint32_t *chr[6];
     int32_t word_format;  // am_word_format of d1/d2diag/d2off
     int32_t push_layers;  // equal-depth ghost layers the stage kernel pushes
+    int32_t push_axes;    // and the axes whose faces it pushes (bits 1 x, 2 y, 4 z)
 };
 
 template <class System>
@@ -252,6 +258,7 @@ struct rk4_AMR_data {
         rk4_views f{};
         f.word_format = opt.word_format;
         f.push_layers = push_layers();
+        f.push_axes = 7;
         for (int v = 0; v < NV; v++) {
             f.state[v] = stage_input_is_old ? field(old_, v) : field(new_, v);
             f.old_[v] = field(old_, v);
@@ -318,9 +325,10 @@ struct rk4_AMR_data {
     // derivative words of the state in `src` for the tiles of `m`; seeds are
     // drawn per field in order (gradient+diagonal, then mixed), then one for
     // the Christoffel words
-    void derivatives(smart_gpu_buffer<float> &src, am_mesh m)
+    void derivatives(smart_gpu_buffer<float> &src, am_mesh m, int direct_axes = 0)
     {
         am_fd_batch b{};
+        b.direct_faces = direct_axes;
         b.z_block = opt.z_block;
         b.word_format = opt.word_format;
         b.prefetch_planes = opt.fd_prefetch;
@@ -384,12 +392,17 @@ struct rk4_AMR_data {
         // the tiles of this depth -- copy-back made it so, and nothing has
         // written their new since -- ghost cells included, which old need
         // not hold)
-        derivatives(new_, m);
-        const rk4_views f = views(false);
+        // With direct faces the stage input's equal-depth ghost layers on
+        // those axes are current only at stage 0 (pushed by the previous
+        // step's stage 3).
+        const int direct = opt.direct_faces & 7;
+        derivatives(new_, m, s > 0 ? direct : 0);
+        rk4_views f = views(false);
+        if (s < 3) f.push_axes = 7 & ~direct;
         am_mesh mr = m;
         mr.grid_limit = opt.rhs_grid_limit;
         int ev = timer.begin(kernel_timer::RHS);
-        check(System::rhs_stage(f, s, sys, mr, geom, 1, stream), "rhs");
+        check(System::rhs_stage(f, s, sys, mr, geom, f.push_axes ? 1 : 0, stream), "rhs");
         timer.end(ev);
         if (d > 0 && parent.coarse_faces_at_depth(d) > 0) {
             am_field_batch b = batch(true);

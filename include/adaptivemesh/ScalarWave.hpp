@@ -29,6 +29,7 @@ struct swave_system {
     {
         am_swave_fields f{};
         f.push_layers = v.push_layers;
+        f.push_axes = v.push_axes;
         for (int i = 0; i < NV; i++) {
             f.state[i] = v.state[i];
             f.old_[i] = v.old_[i];

@@ -206,6 +206,14 @@ int am_bssn_sim_set_fd_prefetch(am_bssn_sim *s, int32_t planes);
  * beyond keep what they held.  _get_push_layers gives the layers in use. */
 int am_bssn_sim_set_push_layers(am_bssn_sim *s, int32_t layers);
 int am_bssn_sim_get_push_layers(const am_bssn_sim *s);
+/* Direct faces: the axes (bits 1 x, 2 y, 4 z; default 4) whose equal-depth
+ * faces stages 0-2 do not push; the next stage's FD reads those ghost cells
+ * from the neighbours' interiors.  Stage 3 pushes every face.  The fields
+ * after each fine step are the same for every mask; between stages 0-2 the
+ * equal-depth ghost layers on the masked axes are not current.
+ * _get_direct_faces gives the mask. */
+int am_bssn_sim_set_direct_faces(am_bssn_sim *s, int32_t axes);
+int am_bssn_sim_get_direct_faces(const am_bssn_sim *s);
 int am_bssn_sim_use_graph(am_bssn_sim *s, int32_t enable);
 int am_bssn_sim_autotune(am_bssn_sim *s, int32_t reps, am_tune_report *out);
 /* AM_INTERP_LEARNED: the MLP parameters (device, am_mlp_param_count()
@@ -267,6 +275,8 @@ int am_swave_sim_get_launch(const am_swave_sim *s, int32_t *out4);
 int am_swave_sim_set_fd_prefetch(am_swave_sim *s, int32_t planes);
 int am_swave_sim_set_push_layers(am_swave_sim *s, int32_t layers);
 int am_swave_sim_get_push_layers(const am_swave_sim *s);
+int am_swave_sim_set_direct_faces(am_swave_sim *s, int32_t axes);
+int am_swave_sim_get_direct_faces(const am_swave_sim *s);
 int am_swave_sim_use_graph(am_swave_sim *s, int32_t enable);
 int am_swave_sim_autotune(am_swave_sim *s, int32_t reps, am_tune_report *out);
 int am_swave_sim_save(am_swave_sim *s, const char *path);

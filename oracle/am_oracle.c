@@ -668,9 +668,10 @@ EXPORT void orc_copy_boundaries(float *old_, float *new_, const float *older, co
                         if (equal_copy && ddiff == 0) {
                             /* the RK4 driver's equal-depth faces: an exact copy of the
                              * neighbour cell the ghost cell maps onto (an extension; the
-                             * reference clamps and extrapolates on the last row) */
-                            const int64_t src = (int64_t)tgt * cells + (int64_t)rp[0] +
-                                                ((int64_t)rp[1] + (int64_t)rp[2] * T) * T;
+                             * reference clamps and extrapolates on the last row).  rp is
+                             * a whole index within the rounding of 1/S, so it is rounded */
+                            const int64_t src = (int64_t)tgt * cells + (int64_t)(rp[0] + .5f) +
+                                                ((int64_t)(rp[1] + .5f) + (int64_t)(rp[2] + .5f) * T) * T;
                             const int64_t wn = (int64_t)node * cells + w[0] + ((int64_t)w[1] + (int64_t)w[2] * T) * T;
                             if (!copy_only_new) old_[wn] = old_[src];
                             new_[wn] = new_[src];

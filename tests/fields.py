@@ -44,10 +44,11 @@ def depth_of(exp, s):
     return int(exp["positions"][s]) >> 27
 
 
-def outer_push_layers(exp, n_slots, tiles, S, P, layers):
+def outer_push_layers(exp, n_slots, tiles, S, P, layers, axes=7):
     """Mask (n_slots * T^3) of the ghost cells that a push of the innermost
     `layers` ghost layers leaves alone: the outer P - layers layers of each
-    face slab of `tiles` whose neighbour has the same depth."""
+    face slab of `tiles` whose neighbour has the same depth, on the axes in
+    `axes` (bits 1 x, 2 y, 4 z)."""
     T = S + 2 * P
     m = np.zeros((n_slots, T, T, T), bool)  # [slot, z, y, x]
     inner = slice(P, P + S)
@@ -58,10 +59,38 @@ def outer_push_layers(exp, n_slots, tiles, S, P, layers):
             if tgt & 0x800000:
                 tgt -= 1 << 24
             own = nd >> 28
-            if tgt < 0 or own == 15 or own == 0 or (nd >> 24) & 15:
+            if tgt < 0 or own == 15 or own == 0 or (nd >> 24) & 15 or not (axes >> (g % 3)) & 1:
                 continue
             band = slice(P + S + layers, T) if g < 3 else slice(0, P - layers)
             idx = [inner, inner, inner]  # x, y, z
             idx[g % 3] = band
             m[s, idx[2], idx[1], idx[0]] = True
     return m.reshape(-1)
+
+
+def fd_input(orc, e, n, tiles, state_in, old, stage, direct, S, P, frac):
+    """The fields an RK4 stage's FD reads: with direct faces (an axis mask),
+    stages 1-3 take the ghost cells across equal-depth faces of those axes
+    from the neighbours' interiors (the equal-depth copies of the oracle's
+    fill at time `frac`); otherwise the stage input as it lies."""
+    if not direct or stage == 0:
+        return state_in
+    eq = outer_push_layers(e, n, tiles, S, P, 0, direct)
+    out = []
+    for v in range(len(state_in)):
+        filled = state_in[v].copy()
+        orc.copy_boundaries(old[v].copy(), filled, None, e["boundaries"], tiles, 0, e["max_depth"], True, 0, S, P,
+                            frac, equal_copy=True)
+        x = state_in[v].copy()
+        x[eq] = filled[eq]
+        out.append(x)
+    return out
+
+
+def unpushed_layers(e, n, tiles, S, P, layers, direct, stage):
+    """Mask of the equal-depth ghost cells an RK4 stage leaves alone: the
+    outer P - layers layers of every face, and with direct faces in stages
+    0-2 every layer of the faces of those axes."""
+    if stage == 3 or not direct:
+        return outer_push_layers(e, n, tiles, S, P, layers)
+    return outer_push_layers(e, n, tiles, S, P, 0, direct) | outer_push_layers(e, n, tiles, S, P, layers, 7 & ~direct)

@@ -75,14 +75,23 @@ def interior_mask(n, tiles):
     return m.reshape(-1)
 
 
-@pytest.mark.parametrize("recipe,periodic,depth,layers", [("uniform1", True, 1, 0), ("graded", False, 2, 0),
-                                                          ("graded", False, 3, 0), ("uniform1", True, 1, P)])
-def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, layers):
-    print(f"bssn stages {recipe} periodic={periodic} depth={depth} push layers={layers}", flush=True)
+def fd_input(orc, e, n, tiles, state_in, old, stage, direct):
+    return F.fd_input(orc, e, n, tiles, state_in, old, stage, direct, S, P, STAGE_FRAC[stage])
+
+
+@pytest.mark.parametrize("recipe,periodic,depth,layers,direct", [
+    ("uniform1", True, 1, 0, 4), ("graded", False, 2, 0, 4), ("graded", False, 3, 0, 4),
+    ("uniform1", True, 1, P, 4), ("graded", False, 2, 0, 0), ("uniform1", True, 1, P, 0),
+    ("uniform1", True, 1, 0, 7), ("graded", False, 2, 0, 3), ("graded", False, 3, 0, 6)])
+def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, layers, direct):
+    print(f"bssn stages {recipe} periodic={periodic} depth={depth} push layers={layers} direct={direct}", flush=True)
     ops, n = {"uniform1": meshes.uniform(1), "graded": meshes.graded()}[recipe]
     g = am.geometry(S, P, ORDER, OUTER, CFL)
     sim = am.BssnSim(g, n, am.bssn_options(periodic=periodic))
     sim.set_push_layers(layers)
+    assert sim.direct_faces == 4
+    sim.set_direct_faces(direct)
+    assert sim.direct_faces == direct
     ot = orc.OracleTree(n, periodic)
     for p_, c in ops:
         assert sim.add_node(p_, c) == ot.add_node(p_, c)
@@ -101,6 +110,7 @@ def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, 
     old = [x.copy() for x in state0]
     for stage in range(4):
         state_in = old if stage == 0 else [sim.get_field(1, v) for v in range(24)]
+        state_fd = fd_input(orc, e, n, tiles, state_in, old, stage, direct)
         acc_in = [sim.get_field(2, v).astype(np.float64) for v in range(24)]
         seed = sim.rng_state
         sim.stage_depth(depth, stage)
@@ -113,13 +123,13 @@ def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, 
             sd = orc.prng_ints(s_, 1)[0]
             s_ = sd
             hess = v in HESS
-            w1, w2 = orc.differentiate(state_in[v], e["positions"], tiles, 0, e["max_depth"], sd, S, P, ORDER // 2,
+            w1, w2 = orc.differentiate(state_fd[v], e["positions"], tiles, 0, e["max_depth"], sd, S, P, ORDER // 2,
                                        outer_dx, n, with_d2=hess)
             assert np.array_equal(d1[v][wmask], w1[wmask]), (stage, v)
             if hess:
                 sm = orc.prng_ints(s_, 1)[0]
                 s_ = sm
-                wo = orc.mixed_partials(state_in[v], e["positions"], tiles, 0, e["max_depth"], sm, S, P, outer_dx, n)
+                wo = orc.mixed_partials(state_fd[v], e["positions"], tiles, 0, e["max_depth"], sm, S, P, outer_dx, n)
                 assert np.array_equal(d2d[h][wmask], w2[wmask]), (stage, v)
                 assert np.array_equal(d2o[h][wmask], wo[wmask]), (stage, v)
                 h += 1
@@ -149,11 +159,11 @@ def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, 
         print(f"  stage {stage}: worst relative k error {worst:.2e}", flush=True)
         # ghost cells of the stage output: the pushed layers (the FD
         # half-width) are exact copies, the outer equal-depth layers keep what
-        # they held
+        # they held; with direct faces stages 0-2 push nothing
         if depth > 0:
             ref = [x.copy() for x in new_g]
-            keep = F.outer_push_layers(e, n, tiles, S, P, sim.push_layers)
-            assert sim.push_layers == (layers or ORDER // 2) and keep.any() == (sim.push_layers < P)
+            keep = F.unpushed_layers(e, n, tiles, S, P, sim.push_layers, direct, stage)
+            assert sim.push_layers == (layers or ORDER // 2)
             for v in range(24):
                 orc.copy_boundaries(old[v].copy(), ref[v], None, e["boundaries"], tiles, 0, e["max_depth"], True, 0,
                                     S, P, STAGE_FRAC[stage], equal_copy=True)
@@ -253,6 +263,7 @@ def test_bssn_word_formats_match_oracle(am, orc, torch_cuda, fmt):
     to_fmt = (lambda x: x) if fmt == 1 else (lambda x: x.astype(np.float16))
     for stage in range(2):
         state_in = old if stage == 0 else [sim.get_field(1, v) for v in range(24)]
+        state_fd = fd_input(orc, e, n, tiles, state_in, old, stage, sim.direct_faces)
         acc_in = [sim.get_field(2, v).astype(np.float64) for v in range(24)]
         sim.stage_depth(1, stage)
         sim.sync()
@@ -260,12 +271,12 @@ def test_bssn_word_formats_match_oracle(am, orc, torch_cuda, fmt):
         h = 0
         for v in range(24):
             hess = v in HESS
-            r1, r2 = orc.differentiate_f32(state_in[v], e["positions"], tiles, 0, e["max_depth"], S, P, ORDER // 2,
+            r1, r2 = orc.differentiate_f32(state_fd[v], e["positions"], tiles, 0, e["max_depth"], S, P, ORDER // 2,
                                            outer_dx, n, with_d2=hess)
             assert np.array_equal(as_words(d1[v])[wmask].view(np.uint8), to_fmt(r1[wmask]).view(np.uint8)), \
                 (stage, v)
             if hess:
-                ro = orc.mixed_partials_f32(state_in[v], e["positions"], tiles, 0, e["max_depth"], S, P, outer_dx, n)
+                ro = orc.mixed_partials_f32(state_fd[v], e["positions"], tiles, 0, e["max_depth"], S, P, outer_dx, n)
                 assert np.array_equal(as_words(d2d[h])[wmask].view(np.uint8), to_fmt(r2[wmask]).view(np.uint8))
                 assert np.array_equal(as_words(d2o[h])[wmask].view(np.uint8), to_fmt(ro[wmask]).view(np.uint8))
                 h += 1

@@ -100,6 +100,71 @@ def test_fd_bit_exact(am, orc, torch_cuda, S, P, order, recipe, substep):
     assert np.array_equal(host(off, np.uint32), rm)
 
 
+@pytest.mark.parametrize("S,P,order", GEOMS)
+@pytest.mark.parametrize("recipe,periodic", [("graded", False), ("uniform2", True), ("nested3", False)])
+def test_fd_direct_faces(am, orc, torch_cuda, S, P, order, recipe, periodic):
+    """direct_faces: ghost cells across equal-depth faces of the chosen axes
+    come from the neighbours' interiors.  With every equal-depth ghost layer
+    of those axes poisoned (NaN), the words equal the oracle's on the field
+    whose ghost layers hold the neighbours' copies, for every axis mask
+    tried, launch shape and two counter values."""
+    torch = torch_cuda
+    print(f"fd direct S={S} P={P} order={order} {recipe} periodic={periodic}", flush=True)
+    g = am.geometry(S, P, order, 100.0, 0.4)
+    tree, e, n = setup_mesh(am, orc, recipe, periodic)
+    T = S + 2 * P
+    rng = np.random.default_rng(S * 10 + P + order)
+    field = rng.standard_normal(n * T ** 3).astype(np.float32)
+    tiles = [s for s in fields.used_slots(e) if fields.depth_of(e, s) > 0]
+    filled = field.copy()
+    orc.copy_boundaries(field.copy(), filled, None, e["boundaries"], tiles, 0, e["max_depth"], True, 0, S, P,
+                        equal_copy=True)
+    eq_all = fields.outer_push_layers(e, n, tiles, S, P, 0)
+    assert eq_all.any()
+    good = field.copy()
+    good[eq_all] = filled[eq_all]
+    L = am.lib()
+    st = am.stream_handle()
+    W = n * S ** 3
+    for axes in (7, 4, 3, 1):
+        bad = good.copy()
+        bad[fields.outer_push_layers(e, n, tiles, S, P, 0, axes)] = np.nan
+        fb = dev(torch, bad)
+        for substep in (0, 1):
+            seed_d, seed_m = int(rng.integers(-2 ** 31, 2 ** 31)), int(rng.integers(-2 ** 31, 2 ** 31))
+            r1, r2 = orc.differentiate(good, e["positions"], fields.used_slots(e), substep, e["max_depth"], seed_d,
+                                       S, P, order // 2, g.outer_dx, n)
+            rm = orc.mixed_partials(good, e["positions"], fields.used_slots(e), substep, e["max_depth"], seed_m, S,
+                                    P, g.outer_dx, n)
+            m = mesh_for(tree, -1, substep)
+            for zb, pf in ((0, 1), (S // 2, 2), (S // 4, 3)):
+                d1, d2, off = (torch.zeros(W, dtype=torch.int32, device="cuda") for _ in range(3))
+                b = am.FdBatch()
+                b.n_vars = 1
+                b.z_block = zb
+                b.prefetch_planes = pf
+                b.direct_faces = axes
+                b.field[0], b.d1[0], b.d2diag[0], b.d2off[0] = (fb.data_ptr(), d1.data_ptr(), d2.data_ptr(),
+                                                                off.data_ptr())
+                b.seed_d[0], b.seed_m[0] = seed_d, seed_m
+                am.check(L.am_fd_batched(C.byref(b), C.byref(m), C.byref(g), st), "am_fd_batched")
+                sync(torch)
+                assert np.array_equal(host(d1, np.uint32), r1), (axes, substep, zb)
+                assert np.array_equal(host(d2, np.uint32), r2), (axes, substep, zb)
+                assert np.array_equal(host(off, np.uint32), rm), (axes, substep, zb)
+        # without direct reads the poisoned layers reach the words
+        b.direct_faces = 0
+        am.check(L.am_fd_batched(C.byref(b), C.byref(m), C.byref(g), st), "am_fd_batched")
+        sync(torch)
+        assert not np.array_equal(host(d1, np.uint32), r1), axes
+    # an axis mask outside 0..7, or direct reads without the boundary table
+    b.direct_faces = 8
+    assert L.am_fd_batched(C.byref(b), C.byref(m), C.byref(g), st) == -1
+    m.boundaries = None
+    b.direct_faces = 1
+    assert L.am_fd_batched(C.byref(b), C.byref(m), C.byref(g), st) == -1
+
+
 def test_fd_batched_many_variables(am, orc, torch_cuda):
     """24 fields in one launch, mixed outputs per variable, equal the
     one-at-a-time oracle results."""

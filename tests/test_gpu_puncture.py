@@ -91,6 +91,7 @@ def test_puncture_stages_tricubic_match_oracle(am, orc, torch_cuda):
         assert np.array_equal(new0[v][imask].view(np.uint32), old[v][imask].view(np.uint32)), v
     for stage in range(4):
         state_in = new0 if stage == 0 else [sim.get_field(1, v) for v in range(24)]
+        state_fd = F.fd_input(orc, e, n, tiles, state_in, old, stage, sim.direct_faces, S, P, STAGE_FRAC[stage])
         acc_in = [sim.get_field(2, v).astype(np.float64) for v in range(24)]
         s_ = sim.rng_state
         sim.stage_depth(depth, stage)
@@ -103,13 +104,13 @@ def test_puncture_stages_tricubic_match_oracle(am, orc, torch_cuda):
             sd = orc.prng_ints(s_, 1)[0]
             s_ = sd
             hess = v in orc.HESS_VARS
-            w1, w2 = orc.differentiate(state_in[v], e["positions"], tiles, 1, 5, sd, S, P, ORDER // 2, outer_dx, n,
+            w1, w2 = orc.differentiate(state_fd[v], e["positions"], tiles, 1, 5, sd, S, P, ORDER // 2, outer_dx, n,
                                        with_d2=hess)
             assert np.array_equal(d1[v][wmask], w1[wmask]), (stage, v)
             if hess:
                 sm = orc.prng_ints(s_, 1)[0]
                 s_ = sm
-                wo = orc.mixed_partials(state_in[v], e["positions"], tiles, 1, 5, sm, S, P, outer_dx, n)
+                wo = orc.mixed_partials(state_fd[v], e["positions"], tiles, 1, 5, sm, S, P, outer_dx, n)
                 assert np.array_equal(d2d[h][wmask], w2[wmask]), (stage, v)
                 assert np.array_equal(d2o[h][wmask], wo[wmask]), (stage, v)
                 h += 1
@@ -129,7 +130,8 @@ def test_puncture_stages_tricubic_match_oracle(am, orc, torch_cuda):
             worst = max(worst, dn / scale)
             assert dn < 2e-6 * scale, (stage, v, dn)
         print(f"  stage {stage}: worst relative update difference {worst:.2e}", flush=True)
-        keep = F.outer_push_layers(e, n, tiles, S, P, sim.push_layers)  # equal-depth layers not pushed
+        # equal-depth layers not pushed (with direct faces, all of them in stages 0-2)
+        keep = F.unpushed_layers(e, n, tiles, S, P, sim.push_layers, sim.direct_faces, stage)
         for v in range(24):
             ref = new_g[v].copy()
             orc.copy_boundaries(old[v].copy(), ref, older[v], e["boundaries"], tiles, 1, 5, True, TRICUBIC, S, P,

@@ -59,9 +59,11 @@ def words(am, sim, n):
     return out
 
 
-def test_swave_stages_match_oracle(am, orc, torch_cuda):
-    print("swave stages uniform1 periodic", flush=True)
+@pytest.mark.parametrize("direct", [4, 0, 7])
+def test_swave_stages_match_oracle(am, orc, torch_cuda, direct):
+    print(f"swave stages uniform1 periodic direct={direct}", flush=True)
     sim, e, n, g = make_sim(am, orc)
+    sim.set_direct_faces(direct)
     u0 = F.sample(e["positions"], n, S, P, OUTER, gaussian)
     v0 = 0.3 * F.sample(e["positions"], n, S, P, OUTER, lambda x, y, z: np.sin(2 * np.pi * x) * np.cos(2 * np.pi * z))
     state0 = [u0, v0]
@@ -79,13 +81,14 @@ def test_swave_stages_match_oracle(am, orc, torch_cuda):
     old = [x.copy() for x in state0]
     for stage in range(4):
         state_in = old if stage == 0 else [sim.get_field(1, v) for v in range(2)]
+        state_fd = F.fd_input(orc, e, n, tiles, state_in, old, stage, direct, S, P, STAGE_FRAC[stage])
         acc_in = [sim.get_field(2, v).astype(np.float64) for v in range(2)]
         seed = sim.rng_state
         sim.stage_depth(depth, stage)
         sim.sync()
         d1, d2 = words(am, sim, n)
         sd = orc.prng_ints(seed, 1)[0]
-        w1, w2 = orc.differentiate(state_in[0], e["positions"], tiles, 0, e["max_depth"], sd, S, P, ORDER // 2,
+        w1, w2 = orc.differentiate(state_fd[0], e["positions"], tiles, 0, e["max_depth"], sd, S, P, ORDER // 2,
                                    outer_dx, n, with_d2=True)
         assert np.array_equal(d1[wmask], w1[wmask]), stage
         assert np.array_equal(d2[wmask], w2[wmask]), stage
@@ -106,8 +109,9 @@ def test_swave_stages_match_oracle(am, orc, torch_cuda):
                 assert np.max(np.abs(acc_g[v][imask] - acc_o[v])) < 2e-6 * ascale, (stage, v)
         # every face of a depth-1 tile in the periodic 2x2x2 block has an
         # equal-depth target: its ghosts are the pushed exact copies
-        # (the innermost ORDER / 2 layers; the outer layers keep what they held)
-        keep = F.outer_push_layers(e, n, tiles, S, P, sim.push_layers)
+        # (the innermost ORDER / 2 layers; the outer layers keep what they
+        # held); with direct faces stages 0-2 push nothing
+        keep = F.unpushed_layers(e, n, tiles, S, P, sim.push_layers, direct, stage)
         for v in range(2):
             ref = new_g[v].copy()
             orc.copy_boundaries(old[v].copy(), ref, None, e["boundaries"], tiles, 0, e["max_depth"], True, 0, S, P,

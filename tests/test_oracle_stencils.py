@@ -182,6 +182,41 @@ def test_ghost_equal_depth_is_exact_copy(orc):
         assert np.array_equal(got[:P - 1, :S - 1, :S - 1], ref[:P - 1, :S - 1, :S - 1])
 
 
+@pytest.mark.parametrize("S_,P_", [(16, 3), (16, 2), (28, 2)])
+def test_ghost_equal_copy_is_the_neighbour_cell(orc, S_, P_):
+    """The RK4 driver's equal-depth rule (equal_copy): every cell of every
+    P-layer face slab across an equal-depth face is the neighbour's interior
+    cell S away, bit for bit -- also at S = 28, where 1/S is inexact in fp32
+    and the mapped index is a whole number only within rounding."""
+    ops, n = meshes.uniform(2)
+    t = orc.OracleTree(n, True)
+    meshes.apply(t, ops)
+    e = t.export()
+    T = S_ + 2 * P_
+    new = np.random.default_rng(S_ + P_).random(n * T ** 3).astype(np.float32)
+    src = new.copy().reshape(n, T, T, T)
+    tiles = [s for s in fields.used_slots(e) if fields.depth_of(e, s) > 0]
+    orc.copy_boundaries(new.copy(), new, None, e["boundaries"], tiles, 0, e["max_depth"], True, 0, S_, P_,
+                        equal_copy=True)
+    got_all = new.reshape(n, T, T, T)
+    inner = slice(P_, P_ + S_)
+    checked = 0
+    for s in tiles:
+        for f in range(6):
+            nd = int(e["boundaries"][s * 6 + f]) & 0xFFFFFFFF
+            if (nd >> 24) & 15:
+                continue  # a coarse face
+            tgt = nd & 0xFFFFFF
+            gi, si = [inner] * 3, [inner] * 3  # x, y, z
+            gi[f % 3] = slice(P_ + S_, T) if f < 3 else slice(0, P_)
+            si[f % 3] = slice(P_, 2 * P_) if f < 3 else slice(S_, S_ + P_)
+            got = got_all[s, gi[2], gi[1], gi[0]]
+            ref = src[tgt, si[2], si[1], si[0]]
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (S_, s, f)
+            checked += 1
+    assert checked == 6 * len(tiles)
+
+
 @pytest.mark.parametrize("mode,fn", [
     (0, lambda x, y, z: 1 + x - 2 * y + 0.5 * z + 0.1 * x * x - 0.2 * y * z + 0.05 * z * z + 0.3 * x * y),
     (1, lambda x, y, z: 1 + x - 2 * y + 0.5 * z + 0.1 * x * y * z),
