@@ -84,7 +84,10 @@ __device__ __forceinline__ float mixed_bd(float c, float pp, float mm, float m0,
     return c + .5f * (pp + mm - m0 - p0 - zm - zp);
 }
 
-template <int S, int P, int Q, int ZB, bool PERSIST, int FMT, int PF>
+// DIR: 0 the tile's own ghost layers only; 1 direct reads across z faces
+// (a base-pointer swap per ghost plane); 2 direct reads on any axis mask.
+// Separate instantiations, so the plain kernel carries none of the code.
+template <int S, int P, int Q, int ZB, bool PERSIST, int FMT, int PF, int DIR>
 __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScales sc)
 {
     constexpr int T = S + 2 * P, T2 = T * T, R = 2 * P + 2, NZB = S / ZB;
@@ -96,10 +99,13 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     const int32_t gs_m = b.seed_table ? b.seed_table[b.seed_m[var]] : b.seed_m[var];
 
     static_assert(S % 4 == 0, "direct reads shift float4 indices by S/4");
-    uint32_t gmask[(T2 / 4 + S * S - 1) / (S * S)];
+    uint32_t gmask[(T2 / 4 + S * S - 1) / (S * S)] = {};
+    if constexpr (DIR == 2) {
 #pragma unroll
-    for (int j = 0; j < (T2 / 4 + S * S - 1) / (S * S); j++)
-        gmask[j] = (int)threadIdx.x + j * S * S < T2 / 4 ? face_ghost_masks<S, P, Q>((int)threadIdx.x + j * S * S) : 0u;
+        for (int j = 0; j < (T2 / 4 + S * S - 1) / (S * S); j++)
+            gmask[j] = (int)threadIdx.x + j * S * S < T2 / 4 ? face_ghost_masks<S, P, Q>((int)threadIdx.x + j * S * S)
+                                                             : 0u;
+    }
 
     // one work item = one (tile, z-block); a persistent launch loops over
     // them (a separate instantiation: the loop keeps kernel arguments live)
@@ -114,7 +120,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     // face f (0-2: +x +y +z, 3-5: -x -y -z) -> the equal-depth neighbour's
     // tile, or null: its ghost layers are read where they lie
     const float *nb[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    if (m.direct) {
+    if (DIR != 0) {
 #pragma unroll
         for (int f = 0; f < 6; f++) {
             const uint32_t nd = (uint32_t)m.direct[(size_t)slot * 6 + f];
@@ -149,15 +155,17 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     auto gload = [&](float4 *dst, int zp) {
         const float *base = src;
         int zs = zp;
-        if (zp < P && nb[5]) {
-            base = nb[5];
-            zs = zp + S;
-        } else if (zp >= P + S && nb[2]) {
-            base = nb[2];
-            zs = zp - S;
+        if constexpr (DIR != 0) {
+            if (zp < P && nb[5]) {
+                base = nb[5];
+                zs = zp + S;
+            } else if (zp >= P + S && nb[2]) {
+                base = nb[2];
+                zs = zp - S;
+            }
         }
         const float4 *s4 = reinterpret_cast<const float4 *>(base + (size_t)zs * T2);
-        const bool inner = zp >= P && zp < P + S;
+        const bool inner = DIR == 2 && zp >= P && zp < P + S;
 #pragma unroll
         for (int j = 0; j < NL; j++) {
             const int i = tid + j * S * S;
@@ -273,22 +281,22 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     }  // work items
 }
 
-template <int S, int P, int Q, int ZB, int FMT, int PF>
+template <int S, int P, int Q, int ZB, int DIR, int FMT, int PF>
 static void fd_launch_pf(bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
                          const DepthScales &sc)
 {
     if (persist)
-        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, FMT, PF>), grid, dim3(S * S), 0, st, d, m, sc);
+        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, FMT, PF, DIR>), grid, dim3(S * S), 0, st, d, m, sc);
     else
-        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false, FMT, PF>), grid, dim3(S * S), 0, st, d, m, sc);
+        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false, FMT, PF, DIR>), grid, dim3(S * S), 0, st, d, m, sc);
 }
-template <int S, int P, int Q, int ZB, int FMT>
+template <int S, int P, int Q, int ZB, int DIR, int FMT>
 static void fd_launch_fmt(int pf, bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
                           const DepthScales &sc)
 {
-    if (pf == 2) return fd_launch_pf<S, P, Q, ZB, FMT, 2>(persist, grid, st, d, m, sc);
-    if (pf == 3) return fd_launch_pf<S, P, Q, ZB, FMT, 3>(persist, grid, st, d, m, sc);
-    fd_launch_pf<S, P, Q, ZB, FMT, 1>(persist, grid, st, d, m, sc);
+    if (pf == 2) return fd_launch_pf<S, P, Q, ZB, DIR, FMT, 2>(persist, grid, st, d, m, sc);
+    if (pf == 3) return fd_launch_pf<S, P, Q, ZB, DIR, FMT, 3>(persist, grid, st, d, m, sc);
+    fd_launch_pf<S, P, Q, ZB, DIR, FMT, 1>(persist, grid, st, d, m, sc);
 }
 
 // the fp32 / fp16 word formats (precision study) exist for the S = 16, P = 3,
@@ -299,16 +307,24 @@ constexpr bool has_word_formats()
     return S == 16 && P == 3 && Q == 2;
 }
 
+template <int S, int P, int Q, int ZB, int DIR>
+static void fd_launch_dir(int fmt, int pf, bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d,
+                          const MeshDev &m, const DepthScales &sc)
+{
+    if constexpr (has_word_formats<S, P, Q>()) {
+        // the study formats keep the one-plane prefetch
+        if (fmt == WORD_F32) return fd_launch_pf<S, P, Q, ZB, DIR, WORD_F32, 1>(persist, grid, st, d, m, sc);
+        if (fmt == WORD_F16) return fd_launch_pf<S, P, Q, ZB, DIR, WORD_F16, 1>(persist, grid, st, d, m, sc);
+    }
+    fd_launch_fmt<S, P, Q, ZB, DIR, WORD_CF3>(pf, persist, grid, st, d, m, sc);
+}
 template <int S, int P, int Q, int ZB>
 static void fd_launch(int fmt, int pf, bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d,
                       const MeshDev &m, const DepthScales &sc)
 {
-    if constexpr (has_word_formats<S, P, Q>()) {
-        // the study formats keep the one-plane prefetch
-        if (fmt == WORD_F32) return fd_launch_pf<S, P, Q, ZB, WORD_F32, 1>(persist, grid, st, d, m, sc);
-        if (fmt == WORD_F16) return fd_launch_pf<S, P, Q, ZB, WORD_F16, 1>(persist, grid, st, d, m, sc);
-    }
-    fd_launch_fmt<S, P, Q, ZB, WORD_CF3>(pf, persist, grid, st, d, m, sc);
+    if (m.direct_axes == 0u) return fd_launch_dir<S, P, Q, ZB, 0>(fmt, pf, persist, grid, st, d, m, sc);
+    if (m.direct_axes == 4u) return fd_launch_dir<S, P, Q, ZB, 1>(fmt, pf, persist, grid, st, d, m, sc);
+    fd_launch_dir<S, P, Q, ZB, 2>(fmt, pf, persist, grid, st, d, m, sc);
 }
 
 template <int S, int P, int Q>

@@ -44,11 +44,13 @@ WORD = 4
 PUSH_LAYERS = FD_ORDER // 2  # the driver pushes the ghost layers the FD stencils read
 PUSH = 6 * PUSH_LAYERS * S * S * 24 * 4 / S ** 3  # equal-depth face ghosts written by the stage kernel
 # HBM bytes per depth-3 k_bssn_stage launch (512 tiles), mean of the four
-# stages, from rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0, both
-# scaled by a 1 GiB copy): profiles/bssn_rk4_512tiles_hbm_counters.md.  The
-# algorithmic bytes of the same launches average 1.392e9.
-RHS_TRAFFIC_PER_LAUNCH = 1.940e9
-RHS_ALGORITHMIC_PER_LAUNCH = 1.392e9
+# stages with the driver's default direct z faces, from rocprofv3 --pmc
+# FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0, both scaled by a 1 GiB copy):
+# profiles/bssn_rk4_512tiles_hbm_counters.md.  The algorithmic bytes of the
+# same launches average 1.355e9.
+RHS_TRAFFIC_PER_LAUNCH = 1.889e9
+RHS_ALGORITHMIC_PER_LAUNCH = 1.355e9
+TRAFFIC_DIRECT_FACES = 4  # the mask the counters were read with
 
 
 def bytes_model(direct_faces):
@@ -298,7 +300,8 @@ def main():
         d = kernels[dom]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
-                    "traffic": RHS_TRAFFIC_PER_LAUNCH if dom == "rhs_stage" and args.workload == "bssn512" else None}
+                    "traffic": RHS_TRAFFIC_PER_LAUNCH if dom == "rhs_stage" and args.workload == "bssn512"
+                    and sim.direct_faces == TRAFFIC_DIRECT_FACES else None}
         if roofline["traffic"]:
             roofline["traffic_note"] = ("HBM bytes per depth-3 launch from rocprofv3 FETCH_SIZE/WRITE_SIZE; "
                                         f"{RHS_TRAFFIC_PER_LAUNCH / RHS_ALGORITHMIC_PER_LAUNCH:.2f}x the algorithmic "
