@@ -224,6 +224,40 @@ def test_bssn_flat_space_stays_flat(am, torch_cuda):
     assert c["H_l2"] < 1e-4 and c["M_l2"] < 1e-4, c
 
 
+@pytest.mark.parametrize("recipe,periodic", [("graded", False), ("uniform1", True)])
+def test_direct_faces_leave_every_step_unchanged(am, torch_cuda, recipe, periodic):
+    """Whole coarse steps (graph replays) with the direct-face masks 0, 4
+    and 7: old, new and the accumulator, ghost cells included, and the
+    seed state are the same bit for bit after every step."""
+    print(f"direct faces over whole steps: {recipe} periodic={periodic}", flush=True)
+    ops, n = {"uniform1": meshes.uniform(1), "graded": meshes.graded()}[recipe]
+    g = am.geometry(S, P, ORDER, OUTER, CFL)
+    results = {}
+    for mask in (0, 4, 7):
+        sim = am.BssnSim(g, n, am.bssn_options(periodic=periodic))
+        meshes.apply(sim, ops)
+        sim.set_direct_faces(mask)
+        e = sim.export()
+        st = smooth_state(e, n, seed=11)
+        for v in range(24):
+            sim.set_field(0, v, st[v])
+            sim.set_field(1, v, st[v])
+        sim.fill_ghosts()
+        sim.use_graph(True)
+        snaps = []
+        for _ in range(2):
+            sim.timestep()
+            sim.sync()
+            snaps.append(([sim.get_field(w, v).view(np.uint32) for w in (0, 1, 2) for v in range(24)],
+                          sim.rng_state))
+        results[mask] = snaps
+    for mask in (4, 7):
+        for step, ((fa, sa), (fb, sb)) in enumerate(zip(results[0], results[mask])):
+            assert sa == sb, (mask, step)
+            for i, (a, b) in enumerate(zip(fa, fb)):
+                assert np.array_equal(a, b), (mask, step, i // 24, i % 24)
+
+
 def sim_word_bytes(am, sim, n, fmt):
     """The driver's d1 / d2diag / d2off arrays as raw bytes per field."""
     wb = am.lib().am_word_bytes(fmt)
