@@ -98,7 +98,8 @@ class _Trial(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("z_block", C.c_int32), ("fd_grid_limit", C.c_int32), ("rhs_grid_limit", C.c_int32),
-                ("n_trials", C.c_int32), ("trial", _Trial * 32), ("fd_prefetch", C.c_int32)]
+                ("n_trials", C.c_int32), ("trial", _Trial * 32), ("fd_prefetch", C.c_int32),
+                ("direct_faces", C.c_int32)]
 
 
 class MlpWork(C.Structure):
@@ -637,9 +638,9 @@ class _Rk4Sim:
     def autotune(self, reps: int = 5) -> dict:
         r = TuneReport()
         check(self._f("autotune")(self.h, reps, C.byref(r)), "autotune")
-        kinds = ("fd_z_block", "fd_grid_limit", "rhs_grid_limit", "fd_prefetch")
+        kinds = ("fd_z_block", "fd_grid_limit", "rhs_grid_limit", "fd_prefetch", "direct_faces")
         return {"z_block": r.z_block, "fd_grid_limit": r.fd_grid_limit, "rhs_grid_limit": r.rhs_grid_limit,
-                "fd_prefetch": r.fd_prefetch,
+                "fd_prefetch": r.fd_prefetch, "direct_faces": r.direct_faces,
                 "trials": [(kinds[r.trial[i].kind], r.trial[i].value, round(r.trial[i].ms, 4))
                            for i in range(r.n_trials)]}
 

@@ -204,7 +204,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--direct-faces", type=int, default=None, choices=range(8),
                     help="axis mask (1 x, 2 y, 4 z) of the equal-depth faces RK4 stages 0-2 leave to the next FD "
-                         "(default: the driver's)")
+                         "(default: the autotuner's choice)")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a recorded step")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-versus-plain ghost fill comparison (training + two puncture runs)")
@@ -240,8 +240,6 @@ def main():
         for p_, c in ops:
             sim.add_node(p_, c)
         fields = WL.puncture_fields(sim.export()["positions"], S, P, OUTER)
-    if args.direct_faces is not None:
-        sim.set_direct_faces(args.direct_faces)
     for v in range(24):
         sim.set_field(0, v, fields[v])
         sim.set_field(1, v, fields[v])
@@ -252,6 +250,8 @@ def main():
         sim.set_launch(args.z_block, 0, 0)
     else:
         tune = sim.autotune(reps=3)
+    if args.direct_faces is not None:  # after the autotuner, which also picks a mask
+        sim.set_direct_faces(args.direct_faces)
     if not args.no_graph:
         sim.use_graph(True)
 

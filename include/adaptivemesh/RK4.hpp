@@ -109,11 +109,12 @@ struct rk4_options {
 struct rk4_tune_report {
     int32_t z_block, fd_grid_limit, rhs_grid_limit, n_trials;
     struct {
-        int32_t kind;  // 0 FD z_block, 1 FD grid limit, 2 RHS grid limit, 3 FD prefetch planes
+        int32_t kind;  // 0 FD z_block, 1 FD grid limit, 2 RHS grid limit, 3 FD prefetch planes, 4 direct faces
         int32_t value;
         float ms;      // best of the repetitions, one whole coarse step
     } trial[32];
     int32_t fd_prefetch;
+    int32_t direct_faces;
 };
 
 // Device views of the driver's buffers handed to a System's kernels.
@@ -497,8 +498,9 @@ struct rk4_AMR_data {
 
     // Time the launch shapes on the live run and keep the fastest: whole
     // coarse steps (timestep(), eager) from a snapshot of the state, one
-    // parameter at a time -- FD planes per workgroup, FD prefetch depth,
-    // then the FD and RHS workgroup caps.  Each trial starts from the same snapshot; the
+    // parameter at a time -- FD planes per workgroup, the direct-face mask
+    // (every mask gives the same fields), FD prefetch depth, then the FD and
+    // RHS workgroup caps.  Each trial starts from the same snapshot; the
     // snapshot, the host seed, the counter and the zone-cycle count are
     // restored at the end, so the run continues as if nothing happened.
     void autotune(int reps, rk4_tune_report *rep)
@@ -564,6 +566,7 @@ struct rk4_AMR_data {
         opt.fd_prefetch = 1;
         trial();  // warm-up
         sweep(0, opt.z_block, {S, S / 2, S / 4});
+        sweep(4, opt.direct_faces, {4, 0, 7});
         if (opt.word_format == AM_WORD_CF3) sweep(3, opt.fd_prefetch, {1, 2, 3});
         sweep(1, opt.fd_grid_limit, {0, 8 * cu});
         sweep(2, opt.rhs_grid_limit, {0, 2 * cu, 4 * cu});
@@ -582,6 +585,7 @@ struct rk4_AMR_data {
         r.fd_grid_limit = opt.fd_grid_limit;
         r.rhs_grid_limit = opt.rhs_grid_limit;
         r.fd_prefetch = opt.fd_prefetch;
+        r.direct_faces = opt.direct_faces;
         if (rep) *rep = r;
     }
     // fill every face ghost of `new` from the current data (initial data):

@@ -135,11 +135,13 @@ int am_wave_sim_load(am_wave_sim *s, const char *path);
 typedef struct am_tune_report {
     int32_t z_block, fd_grid_limit, rhs_grid_limit, n_trials;
     struct {
-        int32_t kind; /* 0 FD z_block, 1 FD workgroup cap, 2 RHS workgroup cap, 3 FD prefetch planes */
+        int32_t kind; /* 0 FD z_block, 1 FD workgroup cap, 2 RHS workgroup cap, 3 FD prefetch planes,
+                       * 4 direct-face axis mask */
         int32_t value;
         float ms; /* best of the repetitions, one coarse step */
     } trial[32];
     int32_t fd_prefetch;
+    int32_t direct_faces;
 } am_tune_report;
 
 /* --------------------------------------------- BSSN RK4 evolution driver
