@@ -509,8 +509,9 @@ int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
     int P##_set_push_layers(H *s, int32_t layers)                                                                  \
     {                                                                                                              \
         return rk4_call(s, [&](auto &d) {                                                                          \
-            if (layers < 0 || layers > d.geom.padding_domain)                                                      \
-                throw std::runtime_error("push layers " + std::to_string(layers) + " (0..P)");                   \
+            if (layers < 0 || layers > d.geom.padding_domain || (layers > 0 && layers < d.geom.fd_order / 2))    \
+                throw std::runtime_error("push layers " + std::to_string(layers) +                               \
+                                         " (0, or fd_order/2..P: the FD stencils read fd_order/2 layers)");      \
             d.opt.push_layers = layers;                                                                            \
             d.drop_graph();                                                                                        \
         });                                                                                                        \

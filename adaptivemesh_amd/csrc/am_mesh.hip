@@ -234,9 +234,12 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
             }
             continue;
         }
-        if (ddiff == 0 && MODE == AM_INTERP_TRICUBIC) {
-            // equal depth: integer offsets, so the 4-point weights are a unit
-            // vector and the value is the target cell at that offset
+        if (ddiff == 0 && MODE == AM_INTERP_TRICUBIC && rp[0] == truncf(rp[0]) && rp[1] == truncf(rp[1]) &&
+            rp[2] == truncf(rp[2])) {
+            // equal depth with whole offsets (every cell for power-of-two S):
+            // the 4-point weights are a unit vector and the value is the
+            // target cell at that offset.  Otherwise (1/S inexact) the offsets
+            // are a rounding away from whole and the formula below applies.
             const size_t src = rn + (ptrdiff_t)rp[0] + (ptrdiff_t)rp[1] * T + (ptrdiff_t)rp[2] * (T * T);
             for (int v = 0; v < nv; v++) b.new_[v0 + v][wn] = b.new_[v0 + v][src];
             continue;
@@ -308,8 +311,12 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
         wlo[k] = k != na ? P : (dir < 3 ? P + S : 0);
     }
 
-    if (ddiff == 0) {
-        // equal depth: the read position is an exact target cell
+    if (ddiff == 0 && equal_mode == AM_EQUAL_COPY) {
+        // equal depth, copy rule: the neighbour cell the ghost maps onto.
+        // The read position is a whole cell index exactly for power-of-two
+        // S and within rounding otherwise (1/S inexact), so it is rounded.
+        // The formula rule (equal_mode AM_EQUAL_FORMULA) takes the general
+        // path below, where inexact offsets weigh in as they do per cell.
         for (int c = threadIdx.x; c < NC; c += 256) {
             int ix[3];
             ix[0] = c % cnt[0];
@@ -326,7 +333,8 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
                 rp[k] = (rp[k] + .5f) * (float)uS + (float)uP - 0.5f;
             }
             const size_t wn = (size_t)node * CELLS + w[0] + ((size_t)w[1] + (size_t)w[2] * T) * T;
-            const size_t src = (size_t)tgt * CELLS + (size_t)rp[0] + ((size_t)rp[1] + (size_t)rp[2] * T) * T;
+            const size_t src = (size_t)tgt * CELLS + (size_t)(rp[0] + .5f) +
+                               ((size_t)(rp[1] + .5f) + (size_t)(rp[2] + .5f) * T) * T;
             for (int v = 0; v < nv; v++) b.new_[v0 + v][wn] = b.new_[v0 + v][src];
         }
         return;

@@ -205,7 +205,9 @@ int am_bssn_sim_set_fd_prefetch(am_bssn_sim *s, int32_t planes);
 /* Equal-depth ghost layers the stage kernels refresh, innermost first:
  * 0 (the default) = fd_order / 2, the layers the FD stencils read; P =
  * every layer, as the reference's copy_boundaries writes them.  Layers
- * beyond keep what they held.  _get_push_layers gives the layers in use. */
+ * beyond keep what they held.  A value from 1 to fd_order / 2 - 1 is
+ * refused (AM_ERR_INVALID): the stencils would read stale ghost layers.
+ * _get_push_layers gives the layers in use. */
 int am_bssn_sim_set_push_layers(am_bssn_sim *s, int32_t layers);
 int am_bssn_sim_get_push_layers(const am_bssn_sim *s);
 /* Direct faces: the axes (bits 1 x, 2 y, 4 z; default 4) whose equal-depth

@@ -165,3 +165,108 @@ def gauge_wave(point, A=0.1, d=1.0):
     K = [[Hf.d(0) / (2 * alpha) if (i, j) == (0, 0) else Jet(0.0) for j in range(3)] for i in range(3)]
     u, d1, dd, Gw = _bssn_from_adm(gamma, K, alpha, [zero, zero, zero])
     return u, d1, dd, Gw, -d1[:, 0]
+
+
+def adm_constraints(gamma, K):
+    """Hamiltonian and momentum constraints of ADM data, computed from the
+    ADM definitions without any BSSN variable:
+      Gamma^k_ij = gamma^kl (d_i gamma_lj + d_j gamma_li - d_l gamma_ij) / 2
+      R_ij = d_k Gamma^k_ij - d_j Gamma^k_ik + Gamma^k_kl Gamma^l_ij
+             - Gamma^k_jl Gamma^l_ik
+      H = R + K^2 - K_ij K^ij,   M_i = D_j K^j_i - D_i K
+    gamma: second-order jets; K: first-order (or better) jets.  Returns
+    (H, M[3]) as floats."""
+    gi, _ = mat_inv(gamma)
+    G = [[[0.5 * sum((gi[k][l] * (gamma[l][j].d(i) + gamma[l][i].d(j) - gamma[i][j].d(l)) for l in range(3)),
+                     Jet(0.0))
+           for j in range(3)] for i in range(3)] for k in range(3)]
+    R = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            r = 0.0
+            for k in range(3):
+                r += G[k][i][j].g[k] - G[k][i][k].g[j]
+                for l in range(3):
+                    r += G[k][k][l].v * G[l][i][j].v - G[k][j][l].v * G[l][i][k].v
+            R[i, j] = r
+    giv = np.array([[gi[i][j].v for j in range(3)] for i in range(3)])
+    Kv = np.array([[K[i][j].v for j in range(3)] for i in range(3)])
+    Gv = np.array([[[G[k][i][j].v for j in range(3)] for i in range(3)] for k in range(3)])
+    dK = np.array([[K[i][j].g for j in range(3)] for i in range(3)])  # dK[i, j, k] = d_k K_ij
+    Rs = np.sum(giv * R)
+    Ktr = np.sum(giv * Kv)
+    Ku = giv @ Kv @ giv
+    H = Rs + Ktr ** 2 - np.sum(Kv * Ku)
+    # D_k K_ij = d_k K_ij - Gamma^l_ki K_lj - Gamma^l_kj K_il
+    DK = dK.copy()
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                DK[i, j, k] -= np.sum(Gv[:, k, i] * Kv[:, j]) + np.sum(Gv[:, k, j] * Kv[i, :])
+    dKtr = np.array([np.sum(giv * DK[:, :, k]) for k in range(3)])  # D_k K (gamma is covariantly constant)
+    M = np.array([np.sum(giv * DK[:, i, :].T) - dKtr[i] for i in range(3)])
+    return H, M
+
+
+def generic_adm(point, seed=0):
+    """A non-symmetric smooth spatial metric and extrinsic curvature near
+    flat space: gamma_ij = delta_ij + quadratic polynomials, K_ij = linear
+    polynomials, with seeded coefficients.  Returns (gamma, K) jets."""
+    rng = np.random.default_rng(seed)
+    x = [Jet.coord(i, point[i]) for i in range(3)]
+    mono = [Jet.const(1.0)] + x + [x[a] * x[b] for a in range(3) for b in range(a, 3)]
+    gamma = [[None] * 3 for _ in range(3)]
+    K = [[None] * 3 for _ in range(3)]
+    for a, b in SYM:
+        cg = rng.uniform(-0.08, 0.08, len(mono))
+        ck = rng.uniform(-0.3, 0.3, 4)
+        gval = sum((c * m for c, m in zip(cg, mono)), Jet.const(1.0 if a == b else 0.0))
+        kval = sum((c * m for c, m in zip(ck, mono[:4])), Jet.const(0.0))
+        gamma[a][b] = gamma[b][a] = gval
+        K[a][b] = K[b][a] = kval
+    return gamma, K
+
+
+def conformally_flat(point, a=0.05):
+    """gamma_ij = psi^4 delta_ij, psi = 1 + a r^2, K_ij = 0: the Hamiltonian
+    constraint is the Ricci scalar R = -8 psi^-5 lap(psi) = -48 a psi^-5."""
+    x = [Jet.coord(i, point[i]) for i in range(3)]
+    psi = 1 + a * (x[0] * x[0] + x[1] * x[1] + x[2] * x[2])
+    p4 = psi ** 4
+    zero = Jet.const(0.0)
+    gamma = [[p4 if i == j else zero for j in range(3)] for i in range(3)]
+    K = [[Jet(0.0) for _ in range(3)] for _ in range(3)]
+    return gamma, K, -48.0 * a * psi.v ** -5
+
+
+def bssn_of(gamma, K, alpha=None, beta_up=None):
+    """BSSN point data (u, d1, dd, G) of ADM data (unit lapse, zero shift by
+    default)."""
+    zero = Jet.const(0.0)
+    alpha = Jet.const(1.0) if alpha is None else alpha
+    beta_up = [zero, zero, zero] if beta_up is None else beta_up
+    return _bssn_from_adm(gamma, K, alpha, beta_up)
+
+
+def gauge_wave_Bx(w, t, A, wavelength, eta, nodes=400):
+    """The Gamma-driver's B^x on the harmonic gauge wave with the shift held
+    at zero (no advection): dB/dt = d_t Gt^x - eta B, B(0) = 0, so
+      B(t) = int_0^t e^{-eta (t - s)} d_s Gt^x(w, s) ds
+           = g(w - t) - e^{-eta t} g(w) - eta int_0^t e^{-eta (t - s)} g(w - s) ds
+    with Gt^x = g(w - t) = (2/3) H^(-5/3) d_w H, H = 1 - A sin(2 pi (w - t) /
+    wavelength).  The last integral by Gauss-Legendre quadrature (the
+    integrand is smooth; 400 nodes resolve it to round-off for t <= 64).
+    w: array of positions along the wave axis."""
+    k = 2 * np.pi / wavelength
+
+    def g(u):
+        H = 1 - A * np.sin(k * u)
+        return (2 / 3) * H ** (-5 / 3) * (-A * k * np.cos(k * u))
+
+    w = np.asarray(w, np.float64)
+    if t == 0:
+        return np.zeros_like(w)
+    x, wq = np.polynomial.legendre.leggauss(nodes)
+    s = (x + 1) / 2 * t
+    integral = (np.exp(-eta * (t - s))[None, :] * g(w[:, None] - s[None, :])) @ (wq * t / 2)
+    return g(w - t) - np.exp(-eta * t) * g(w) - eta * integral

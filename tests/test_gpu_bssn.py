@@ -359,3 +359,23 @@ def test_word_format_rejected_off_geometry(am, torch_cuda):
     g = am.geometry(S, P, ORDER, OUTER, CFL)
     with pytest.raises(am.AMError):
         am.BssnSim(g, 9, am.bssn_options(periodic=True, word_format=7))
+
+
+def test_push_layers_lower_bound(am, torch_cuda):
+    """The push must refresh every ghost layer the FD stencils read
+    (fd_order / 2): fewer layers are refused and leave the setting as it
+    was; 0 (the default, fd_order / 2) and fd_order / 2 .. P are taken."""
+    g = am.geometry(S, P, ORDER, OUTER, CFL)
+    sim = am.BssnSim(g, 9, am.bssn_options(periodic=True))
+    assert sim.push_layers == ORDER // 2
+    for bad in range(1, ORDER // 2):
+        with pytest.raises(am.AMError):
+            sim.set_push_layers(bad)
+        assert sim.push_layers == ORDER // 2
+    for ok in range(ORDER // 2, P + 1):
+        sim.set_push_layers(ok)
+        assert sim.push_layers == ok
+    with pytest.raises(am.AMError):
+        sim.set_push_layers(P + 1)
+    sim.set_push_layers(0)
+    assert sim.push_layers == ORDER // 2

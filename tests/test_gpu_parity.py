@@ -234,6 +234,55 @@ def test_ghost_fill_bit_exact(am, orc, torch_cuda, mode, recipe, periodic, copy_
         assert np.array_equal(host(do, np.float32).view(np.uint32), old.view(np.uint32)), substep
 
 
+@pytest.mark.parametrize("S,P", [(28, 2), (16, 3)])
+@pytest.mark.parametrize("mode,equal", [(0, 1), (1, 1), (2, 1), (1, 0), (2, 0)])
+@pytest.mark.parametrize("recipe,periodic", [("graded", False), ("uniform2", True)])
+def test_ghost_fill_stage_equal_modes(am, orc, torch_cuda, S, P, mode, equal, recipe, periodic):
+    """The RK4 driver's stage fill (am_ghost_fill_stage_batched, 8 fields per
+    launch) with equal-depth faces copied (equal=1, AM_EQUAL_COPY) or
+    interpolated by the formula (equal=0, AM_EQUAL_FORMULA) equals the
+    oracle's copy_boundaries bit for bit at stage fractions 1/2 and 1, on
+    the reference's default tile (S=28, P=2, where 1/S is inexact and the
+    mapped read positions are a rounding away from whole cells) and on
+    S=16.  The quadratic formula (mode 0) is tested with the copy rule only:
+    at S=28 its equal-depth reads reach the target's ghost cells, which the
+    same pass writes."""
+    torch = torch_cuda
+    print(f"stage fill S={S} P={P} mode={mode} equal={equal} {recipe}", flush=True)
+    g = am.geometry(S, P, 2 * P, 100.0, 0.4)
+    tree, e, n = setup_mesh(am, orc, recipe, periodic)
+    T = S + 2 * P
+    nv = 8
+    rng = np.random.default_rng(S * 100 + mode * 10 + equal)
+    depths = {}
+    for s in fields.used_slots(e):
+        depths.setdefault(fields.depth_of(e, s), []).append(s)
+    for substep in range(min(1 << e["max_depth"], 4)):
+        for frac in (0.5, 1.0):
+            old = rng.standard_normal((nv, n * T ** 3)).astype(np.float32)
+            new = rng.standard_normal((nv, n * T ** 3)).astype(np.float32)
+            older = rng.standard_normal((nv, n * T ** 3)).astype(np.float32)
+            do, dn, dl = dev(torch, old), dev(torch, new), dev(torch, older)
+            fb = am.FieldBatch()
+            fb.n_vars = nv
+            for v in range(nv):
+                fb.old_[v] = do[v].data_ptr()
+                fb.new_[v] = dn[v].data_ptr()
+                fb.older[v] = dl[v].data_ptr() if mode == 2 else None
+            for d in sorted(depths):
+                if d == 0:
+                    continue
+                for v in range(nv):
+                    orc.copy_boundaries(old[v], new[v], older[v] if mode == 2 else None, e["boundaries"], depths[d],
+                                        substep, e["max_depth"], True, mode, S, P, frac, equal_copy=equal == 1)
+                m = mesh_for(tree, d, substep)
+                am.check(am.lib().am_ghost_fill_stage_batched(C.byref(fb), C.byref(m), C.byref(g), C.c_float(frac),
+                                                              mode, equal, am.stream_handle()), "stage fill")
+            sync(torch)
+            got = host(dn, np.float32).view(np.uint32)
+            assert np.array_equal(got, new.view(np.uint32)), (substep, frac)
+
+
 @pytest.mark.parametrize("S,P", [(16, 3), (16, 2), (28, 2)])
 def test_copy_new_to_old_bit_exact(am, orc, torch_cuda, S, P):
     torch = torch_cuda
