@@ -217,6 +217,8 @@ _SIGS = {
     "am_bssn_constraints": (C.c_int, [C.POINTER(BssnFields), C.POINTER(BssnParams), C.POINTER(Mesh),
                                       C.POINTER(Geometry), _P, _P]),
     "am_bssn_constraint_blocks": (C.c_int64, [C.POINTER(Geometry), C.c_int32]),
+    "am_bssn_constraints_noalloc": (C.c_int, [C.POINTER(BssnFields), C.POINTER(BssnParams), C.POINTER(Mesh),
+                                              C.POINTER(Geometry), _P, _P]),
     "am_event_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "am_event_record": (C.c_int, [_P, _P]),
     "am_event_elapsed_ms": (C.c_int, [_P, _P, C.POINTER(C.c_float)]),

@@ -380,6 +380,18 @@ int am_bssn_constraints(const am_bssn_fields *f, const am_bssn_params *p, const 
                         double *partial, void *stream);
 int64_t am_bssn_constraint_blocks(const am_geometry *g, int32_t n_tiles);
 
+/* am_bssn_constraints that allocates nothing and reduces on the device:
+ * out4 (device, 4 doubles) = {sum H^2, sum M_i M_i, cells, max |H|} over
+ * every listed tile.  No counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc): the per-workgroup partial sums
+ * live in a fixed 2 MiB device array of the library, one per GPU, held
+ * until the process ends, so at most 65536 partial sums (4096 tiles of 16^3
+ * cells) per call; more is refused with AM_ERR_INVALID.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU race. */
+int am_bssn_constraints_noalloc(const am_bssn_fields *f, const am_bssn_params *p, const am_mesh *mesh,
+                                const am_geometry *g, double *out4, void *stream);
+
 /* ------------------------------------------------------ scalar wave, RK4
  * No counterpart in the reference, whose SimpleWave.h plugin is a Heun step
  * of a Burgers-type equation (SimpleWave.h:7-72): the wave equation
