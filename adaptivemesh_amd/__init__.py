@@ -22,6 +22,7 @@ AM_MAX_VARS = 32
 AM_INTERP_LAGRANGE, AM_INTERP_TRILINEAR, AM_INTERP_TRICUBIC, AM_INTERP_LEARNED = 0, 1, 2, 3
 AM_WORD_CF3, AM_WORD_F32, AM_WORD_F16 = 0, 1, 2
 AM_EQUAL_FORMULA, AM_EQUAL_COPY, AM_EQUAL_SKIP = 0, 1, 2
+AM_LAYOUT_PADDED, AM_LAYOUT_INTERIOR_FIRST = 0, 1
 
 ERRORS = {
     -1: "AM_ERR_INVALID", -2: "AM_ERR_UNSUPPORTED_GEOMETRY", -3: "AM_ERR_SLOTS_EXHAUSTED",
@@ -67,12 +68,12 @@ class FdBatch(C.Structure):
                 ("seed_d", C.c_int32 * AM_MAX_VARS), ("seed_m", C.c_int32 * AM_MAX_VARS),
                 ("n_vars", C.c_int32), ("z_block", C.c_int32), ("seed_table", C.c_void_p),
                 ("word_format", C.c_int32), ("prefetch_planes", C.c_int32),
-                ("direct_faces", C.c_int32)]
+                ("direct_faces", C.c_int32), ("tile_layout", C.c_int32)]
 
 
 class FieldBatch(C.Structure):
     _fields_ = [("old_", C.c_void_p * AM_MAX_VARS), ("new_", C.c_void_p * AM_MAX_VARS),
-                ("older", C.c_void_p * AM_MAX_VARS), ("n_vars", C.c_int32)]
+                ("older", C.c_void_p * AM_MAX_VARS), ("n_vars", C.c_int32), ("tile_layout", C.c_int32)]
 
 
 class BssnParams(C.Structure):
@@ -84,7 +85,7 @@ class BssnFields(C.Structure):
     _fields_ = [("state", C.c_void_p * 24), ("old_", C.c_void_p * 24), ("new_", C.c_void_p * 24),
                 ("acc", C.c_void_p * 24), ("d1", C.c_void_p * 24), ("d2diag", C.c_void_p * 11),
                 ("d2off", C.c_void_p * 11), ("word_format", C.c_int32), ("push_layers", C.c_int32),
-                ("push_axes", C.c_int32)]
+                ("push_axes", C.c_int32), ("tile_layout", C.c_int32)]
 
 
 class BssnSimOptions(C.Structure):
@@ -109,7 +110,7 @@ class MlpWork(C.Structure):
 class SwaveFields(C.Structure):
     _fields_ = [("state", C.c_void_p * 2), ("old_", C.c_void_p * 2), ("new_", C.c_void_p * 2),
                 ("acc", C.c_void_p * 2), ("d1_u", C.c_void_p), ("d2diag_u", C.c_void_p), ("push_layers", C.c_int32),
-                ("push_axes", C.c_int32)]
+                ("push_axes", C.c_int32), ("tile_layout", C.c_int32)]
 
 
 class SwaveSimOptions(C.Structure):
@@ -217,6 +218,9 @@ _SIGS = {
     "am_bssn_constraints": (C.c_int, [C.POINTER(BssnFields), C.POINTER(BssnParams), C.POINTER(Mesh),
                                       C.POINTER(Geometry), _P, _P]),
     "am_bssn_constraint_blocks": (C.c_int64, [C.POINTER(Geometry), C.c_int32]),
+    "am_tile_stride": (C.c_int64, [C.POINTER(Geometry), C.c_int32]),
+    "am_tile_cell_offsets": (C.c_int, [C.POINTER(Geometry), C.c_int32, _P]),
+    "am_tile_layout_convert": (C.c_int, [_P, _P, C.c_int32, C.POINTER(Geometry), C.c_int32, C.c_int32, _P]),
     "am_bssn_constraints_noalloc": (C.c_int, [C.POINTER(BssnFields), C.POINTER(BssnParams), C.POINTER(Mesh),
                                               C.POINTER(Geometry), _P, _P]),
     "am_event_create": (C.c_int, [C.POINTER(C.c_void_p)]),
@@ -334,6 +338,40 @@ def geometry(S: int = 16, P: int = 3, fd_order: int = 0, outer_size: float = 100
     g = Geometry()
     check(lib().am_geometry_init(C.byref(g), S, P, fd_order, outer_size, cfl), "am_geometry_init")
     return g
+
+
+def interior_first_index(S: int, P: int) -> np.ndarray:
+    """Offset within its slot, in the interior-first tile layout
+    (am_tile_layout), of every cell of a padded tile, in padded order
+    (index (z * T + y) * T + x): the interior block, the six face slabs
+    (face f = +x +y +z -x -y -z, layer counted outward, in-face cell (a, b)
+    = the other two axes in x, y, z order), the edges, the corners."""
+    T = S + 2 * P
+    z, y, x = np.meshgrid(np.arange(T), np.arange(T), np.arange(T), indexing="ij")
+    c = np.stack([x.ravel(), y.ravel(), z.ravel()])
+    inner = c - P
+    inside = (inner >= 0) & (inner < S)
+    code = np.where(c < P, c, c - S)
+    n_out = 3 - inside.sum(0)
+    out = np.empty(T ** 3, np.int64)
+    S2, G = S * S, 2 * P
+    F0, E0 = S ** 3, S ** 3 + 6 * P * S2
+    C0 = E0 + 3 * S * G * G
+    m = n_out == 0
+    out[m] = (inner[2, m] * S + inner[1, m]) * S + inner[0, m]
+    for k in range(3):
+        a_ax, b_ax = (1, 2) if k == 0 else ((0, 2) if k == 1 else (0, 1))
+        m = (n_out == 1) & ~inside[k]
+        high = c[k, m] >= P + S
+        f = np.where(high, k, k + 3)
+        l = np.where(high, c[k, m] - P - S, P - 1 - c[k, m])
+        out[m] = F0 + ((f * P + l) * S + inner[b_ax, m]) * S + inner[a_ax, m]
+        o0, o1 = (1, 2) if k == 0 else ((0, 2) if k == 1 else (0, 1))
+        m = (n_out == 2) & inside[k]
+        out[m] = E0 + ((k * S + inner[k, m]) * G + code[o0, m]) * G + code[o1, m]
+    m = n_out == 3
+    out[m] = C0 + (code[0, m] * G + code[1, m]) * G + code[2, m]
+    return out
 
 
 def ptr(x) -> int | None:
@@ -561,7 +599,35 @@ class _Rk4Sim:
         write_png(path, self.raster(var, which, width, height, z_slice, boxes))
 
     def device_ptr(self, which: int) -> int:
+        """Raw device pointer of a driver buffer: 0 old, 1 new, 2 acc, 3
+        older (fields in the interior-first layout, field v at v *
+        max_slots * tile_stride floats; acc interior-only), 4-7 the words."""
         return self._f("device_ptr")(self.h, which)
+
+    @property
+    def tile_stride(self) -> int:
+        """Floats per slot of the driver's state fields (interior-first layout)."""
+        return int(lib().am_tile_stride(C.byref(self.geom), AM_LAYOUT_INTERIOR_FIRST))
+
+    def field_ptr(self, which: int, var: int) -> int:
+        """Device pointer of field `var` of buffer `which` (0 old, 1 new, 3 older)."""
+        return self.device_ptr(which) + var * self.max_slots * self.tile_stride * 4
+
+    def padded_fields(self, which: int, out=None):
+        """Every field of buffer `which` (0 old, 1 new, 3 older) converted on the
+        device to the reference's padded layout: a (NV, max_slots * T^3)
+        float32 torch tensor on the GPU (`out` if given)."""
+        import torch
+        T = self.geom.T
+        if out is None:
+            out = torch.empty((self.NV, self.max_slots * T ** 3), dtype=torch.float32, device="cuda")
+        self.sync()
+        for v in range(self.NV):
+            check(lib().am_tile_layout_convert(self.field_ptr(which, v), out[v].data_ptr(), self.max_slots,
+                                               C.byref(self.geom), AM_LAYOUT_INTERIOR_FIRST, AM_LAYOUT_PADDED,
+                                               stream_handle()), "am_tile_layout_convert")
+        torch.cuda.synchronize()
+        return out
 
     def fill_ghosts(self):
         check(self._f("fill_ghosts")(self.h), "fill_ghosts")

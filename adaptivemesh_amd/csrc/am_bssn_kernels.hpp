@@ -285,11 +285,12 @@ struct TileArgs {
     int32_t n_items;  // tiles x 256-cell blocks; workgroups loop over them
 };
 
-template <int S, int P, int STAGE, bool PUSH, bool PERSIST, int FMT>
+template <int S, int P, int STAGE, bool PUSH, bool PERSIST, int FMT, int L>
 __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, BssnParDev par, DepthScales sc)
 {
 #pragma clang fp contract(fast)
-    constexpr int T = S + 2 * P, S3 = S * S * S, BPT = (S3 + 255) / 256;
+    constexpr int S3 = S * S * S, BPT = (S3 + 255) / 256;
+    constexpr size_t TS = tile_stride<S, P, L>();
     // persistent launches loop over work items (a separate instantiation:
     // the loop keeps every kernel-argument pointer live across iterations)
     const int item_end = PERSIST ? m.n_items : (int)blockIdx.x + 1;
@@ -301,27 +302,27 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
         const int cell = cb * 256 + (int)threadIdx.x;
         if (cell >= S3) continue;
         const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
-        const size_t r = (size_t)node * (T * T * T) + ((size_t)(z + P) * T + (y + P)) * T + (x + P);
+        const size_t r = (size_t)node * TS + tile_interior<S, P, L>(x, y, z);
         const size_t w = (size_t)node * S3 + cell;
         float k[NV];
         bssn_cell<FMT, true, false>(a, r, w, par, k, nullptr, nullptr);
         rk4_combine<NV, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
-        if (PUSH) rk4_push<NV, S, P>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes);
+        if (PUSH) rk4_push<NV, S, P, L>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes);
     }
 }
 
 // Per-workgroup FP64 partial sums {sum H^2, sum |M|^2, cells, max |H|}.
-template <int S, int P, int FMT>
+template <int S, int P, int FMT, int L>
 __global__ void __launch_bounds__(256) k_bssn_constraints(BssnArgsDev a, TileArgs m, BssnParDev par, double *partial)
 {
-    constexpr int T = S + 2 * P, S3 = S * S * S, BPT = (S3 + 255) / 256;
+    constexpr int S3 = S * S * S, BPT = (S3 + 255) / 256;
     const int ti = blockIdx.x / BPT, cb = blockIdx.x - ti * BPT;
     const int node = m.tiles[ti];
     const int cell = cb * 256 + (int)threadIdx.x;
     double h2 = 0.0, m2 = 0.0, cnt = 0.0, hmax = 0.0;
     if (cell < S3) {
         const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
-        const size_t r = (size_t)node * (T * T * T) + ((size_t)(z + P) * T + (y + P)) * T + (x + P);
+        const size_t r = (size_t)node * tile_stride<S, P, L>() + tile_interior<S, P, L>(x, y, z);
         const size_t w = (size_t)node * S3 + cell;
         float H, M[3];
         bssn_cell<FMT, false, true>(a, r, w, par, nullptr, &H, M);
@@ -385,16 +386,24 @@ inline BssnParDev dev_par(const am_bssn_params *p)
     return d;
 }
 
-template <int S, int P, bool PUSH, bool PERSIST, int FMT>
+template <int S, int P, bool PUSH, bool PERSIST, int FMT, int L>
 static void launch_stage_push(int stage, dim3 grid, hipStream_t st, const BssnArgsDev &a, const TileArgs &m,
                               const BssnParDev &d, const DepthScales &sc)
 {
     switch (stage) {
-    case 0: hipLaunchKernelGGL((k_bssn_stage<S, P, 0, PUSH, PERSIST, FMT>), grid, dim3(256), 0, st, a, m, d, sc); break;
-    case 1: hipLaunchKernelGGL((k_bssn_stage<S, P, 1, PUSH, PERSIST, FMT>), grid, dim3(256), 0, st, a, m, d, sc); break;
-    case 2: hipLaunchKernelGGL((k_bssn_stage<S, P, 2, PUSH, PERSIST, FMT>), grid, dim3(256), 0, st, a, m, d, sc); break;
-    default: hipLaunchKernelGGL((k_bssn_stage<S, P, 3, PUSH, PERSIST, FMT>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    case 0: hipLaunchKernelGGL((k_bssn_stage<S, P, 0, PUSH, PERSIST, FMT, L>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    case 1: hipLaunchKernelGGL((k_bssn_stage<S, P, 1, PUSH, PERSIST, FMT, L>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    case 2: hipLaunchKernelGGL((k_bssn_stage<S, P, 2, PUSH, PERSIST, FMT, L>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    default: hipLaunchKernelGGL((k_bssn_stage<S, P, 3, PUSH, PERSIST, FMT, L>), grid, dim3(256), 0, st, a, m, d, sc); break;
     }
+}
+template <int S, int P, bool PUSH, bool PERSIST, int FMT>
+static void launch_stage_layout(int layout, int stage, dim3 grid, hipStream_t st, const BssnArgsDev &a,
+                                const TileArgs &m, const BssnParDev &d, const DepthScales &sc)
+{
+    if (layout == AM_LAYOUT_INTERIOR_FIRST)
+        return launch_stage_push<S, P, PUSH, PERSIST, FMT, 1>(stage, grid, st, a, m, d, sc);
+    launch_stage_push<S, P, PUSH, PERSIST, FMT, 0>(stage, grid, st, a, m, d, sc);
 }
 
 // one stage launch with the words in format FMT (argument checks done)
@@ -420,14 +429,17 @@ int launch_stage_fmt(const am_bssn_fields *f, int stage, const am_bssn_params *p
     dim3 grid(mesh->grid_limit > 0 && mesh->grid_limit < n_items ? mesh->grid_limit : n_items);
     const bool persist = grid.x < (unsigned)n_items;
     hipStream_t st = (hipStream_t)stream;
+    const int lay = f->tile_layout;
+    if (lay != AM_LAYOUT_PADDED && lay != AM_LAYOUT_INTERIOR_FIRST)
+        return set_error(AM_ERR_INVALID, "am_bssn_rhs_stage: tile layout %d", lay);
     if (push && persist)
-        launch_stage_push<S, P, true, true, FMT>(stage, grid, st, a, m, d, sc);
+        launch_stage_layout<S, P, true, true, FMT>(lay, stage, grid, st, a, m, d, sc);
     else if (push)
-        launch_stage_push<S, P, true, false, FMT>(stage, grid, st, a, m, d, sc);
+        launch_stage_layout<S, P, true, false, FMT>(lay, stage, grid, st, a, m, d, sc);
     else if (persist)
-        launch_stage_push<S, P, false, true, FMT>(stage, grid, st, a, m, d, sc);
+        launch_stage_layout<S, P, false, true, FMT>(lay, stage, grid, st, a, m, d, sc);
     else
-        launch_stage_push<S, P, false, false, FMT>(stage, grid, st, a, m, d, sc);
+        launch_stage_layout<S, P, false, false, FMT>(lay, stage, grid, st, a, m, d, sc);
     return check_launch("k_bssn_stage");
 }
 
@@ -440,8 +452,15 @@ int launch_cons_fmt(const am_bssn_fields *f, const am_bssn_params *p, const am_m
     if (int rc = fill_args(f, a, false)) return rc;
     TileArgs m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth,
                mesh->n_tiles * BPT};
-    hipLaunchKernelGGL((k_bssn_constraints<S, P, FMT>), dim3(mesh->n_tiles * BPT), dim3(256), 0, (hipStream_t)stream, a,
-                       m, dev_par(p), partial);
+    const dim3 grid(mesh->n_tiles * BPT);
+    if (f->tile_layout == AM_LAYOUT_INTERIOR_FIRST)
+        hipLaunchKernelGGL((k_bssn_constraints<S, P, FMT, 1>), grid, dim3(256), 0, (hipStream_t)stream, a, m,
+                           dev_par(p), partial);
+    else if (f->tile_layout == AM_LAYOUT_PADDED)
+        hipLaunchKernelGGL((k_bssn_constraints<S, P, FMT, 0>), grid, dim3(256), 0, (hipStream_t)stream, a, m,
+                           dev_par(p), partial);
+    else
+        return set_error(AM_ERR_INVALID, "am_bssn_constraints: tile layout %d", f->tile_layout);
     return check_launch("k_bssn_constraints");
 }
 

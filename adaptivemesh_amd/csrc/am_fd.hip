@@ -87,10 +87,18 @@ __device__ __forceinline__ float mixed_bd(float c, float pp, float mm, float m0,
 // DIR: 0 the tile's own ghost layers only; 1 direct reads across z faces
 // (a base-pointer swap per ghost plane); 2 direct reads on any axis mask.
 // Separate instantiations, so the plain kernel carries none of the code.
-template <int S, int P, int Q, int ZB, bool PERSIST, int FMT, int PF, int DIR>
+// L: tile layout of the fields (0 padded, 1 interior-first).  The LDS ring
+// holds padded planes either way; with L = 1 a plane is assembled from the
+// interior plane (S*S contiguous floats, one per thread) and the Q slab rows
+// of each x and y face (S contiguous floats each), a z ghost plane from the
+// slab layer (or the z neighbour's interior plane), and ghost layers beyond
+// the stencil's half-width Q are not loaded at all.
+template <int S, int P, int Q, int ZB, bool PERSIST, int FMT, int PF, int DIR, int L>
 __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScales sc)
 {
+    static_assert(L == 0 || DIR != 2, "the interior-first layout takes direct z faces only");
     constexpr int T = S + 2 * P, T2 = T * T, R = 2 * P + 2, NZB = S / ZB;
+    constexpr size_t TS = tile_stride<S, P, L>();
     constexpr FdCoef C = fd_coef();
     constexpr int BASE = Q * Q + Q - 1;
     __shared__ float4 ring4[R][T2 / 4];
@@ -116,7 +124,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     const int depth = pos_depth(m.positions[slot]);
     if (depth == -1 || !active_depth(m.substep, (uint32_t)depth, m.max_depth)) continue;  // uniform per workgroup
 
-    const float *src = b.field[var] + (size_t)slot * (T * T2);
+    const float *src = b.field[var] + (size_t)slot * TS;
     // face f (0-2: +x +y +z, 3-5: -x -y -z) -> the equal-depth neighbour's
     // tile, or null: its ghost layers are read where they lie
     const float *nb[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -126,7 +134,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
             const uint32_t nd = (uint32_t)m.direct[(size_t)slot * 6 + f];
             const int tgt = bnd_target(nd);
             if (((m.direct_axes >> (f % 3)) & 1u) && tgt >= 0 && bnd_depth(nd) > 0 && bnd_ddiff(nd) == 0u)
-                nb[f] = b.field[var] + (size_t)tgt * (T * T2);
+                nb[f] = b.field[var] + (size_t)tgt * TS;
         }
     }
     uint32_t *o_d1 = b.d1[var], *o_d2 = b.d2diag[var], *o_off = b.d2off[var];
@@ -152,7 +160,40 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     // S away: the same float4 index shifted by S/4 (S*T/4), as S % 4 == 0.
     constexpr int NL = (T2 / 4 + S * S - 1) / (S * S);
     float4 pre[PF][NL];
+    // interior-first layout: this thread's element of the x/y face slab rows
+    // of an interior plane (4 faces x Q layers x S cells), if any: its offset
+    // from the slab region (plus z * S per plane) and its place in the plane
+    constexpr int NF = 4 * Q * S;
+    int f_goff = -1, f_loff = 0;
+    if constexpr (L == 1) {
+        if (tid < NF) {
+            const int k = tid / (Q * S), l = (tid / S) % Q, a = tid % S;
+            const int f = k == 0 ? 3 : (k == 1 ? 0 : (k == 2 ? 4 : 1));  // -x +x -y +y
+            f_goff = (f * P + l) * S * S + a;
+            const int g = (k & 1) ? P + S + l : P - 1 - l;  // the ghost coordinate of the face axis
+            f_loff = k < 2 ? (a + P) * T + g : g * T + a + P;
+        }
+    }
     auto gload = [&](float4 *dst, int zp) {
+        if constexpr (L == 1) {
+            if (zp >= P && zp < P + S) {
+                const float *pz = src + (size_t)(zp - P) * (S * S);
+                dst[0].x = pz[tid];
+                if (f_goff >= 0) dst[0].y = src[(size_t)(S * S * S) + f_goff + (zp - P) * S];
+            } else {
+                const bool low = zp < P;
+                const int l = low ? P - 1 - zp : zp - P - S;
+                if (l < Q) {
+                    const float *pl_ = src + (size_t)(S * S * S) + (size_t)(((low ? 5 : 2) * P + l) * S * S);
+                    if constexpr (DIR == 1) {
+                        if (low && nb[5]) pl_ = nb[5] + (size_t)(S - 1 - l) * (S * S);
+                        if (!low && nb[2]) pl_ = nb[2] + (size_t)l * (S * S);
+                    }
+                    dst[0].x = pl_[tid];
+                }
+            }
+            return;
+        }
         const float *base = src;
         int zs = zp;
         if constexpr (DIR != 0) {
@@ -187,6 +228,16 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
         }
     };
     auto lstore = [&](const float4 *from, int zp) {
+        if constexpr (L == 1) {
+            float *d = reinterpret_cast<float *>(ring4[zp % R]);
+            if (zp >= P && zp < P + S) {
+                d[(tid / S + P) * T + tid % S + P] = from[0].x;
+                if (f_goff >= 0) d[f_loff] = from[0].y;
+            } else if ((zp < P ? P - 1 - zp : zp - P - S) < Q) {
+                d[(tid / S + P) * T + tid % S + P] = from[0].x;
+            }
+            return;
+        }
         float4 *d4 = ring4[zp % R];
 #pragma unroll
         for (int j = 0; j < NL; j++) {
@@ -281,22 +332,24 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     }  // work items
 }
 
-template <int S, int P, int Q, int ZB, int DIR, int FMT, int PF>
+template <int S, int P, int Q, int ZB, int DIR, int FMT, int PF, int L>
 static void fd_launch_pf(bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
                          const DepthScales &sc)
 {
     if (persist)
-        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, FMT, PF, DIR>), grid, dim3(S * S), 0, st, d, m, sc);
+        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, FMT, PF, DIR, L>), grid, dim3(S * S), 0, st, d, m, sc);
     else
-        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false, FMT, PF, DIR>), grid, dim3(S * S), 0, st, d, m, sc);
+        hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false, FMT, PF, DIR, L>), grid, dim3(S * S), 0, st, d, m, sc);
 }
-template <int S, int P, int Q, int ZB, int DIR, int FMT>
+template <int S, int P, int Q, int ZB, int DIR, int FMT, int L>
 static void fd_launch_fmt(int pf, bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
                           const DepthScales &sc)
 {
-    if (pf == 2) return fd_launch_pf<S, P, Q, ZB, DIR, FMT, 2>(persist, grid, st, d, m, sc);
-    if (pf == 3) return fd_launch_pf<S, P, Q, ZB, DIR, FMT, 3>(persist, grid, st, d, m, sc);
-    fd_launch_pf<S, P, Q, ZB, DIR, FMT, 1>(persist, grid, st, d, m, sc);
+    if constexpr (L == 0) {  // the interior-first layout keeps the one-plane prefetch
+        if (pf == 2) return fd_launch_pf<S, P, Q, ZB, DIR, FMT, 2, L>(persist, grid, st, d, m, sc);
+        if (pf == 3) return fd_launch_pf<S, P, Q, ZB, DIR, FMT, 3, L>(persist, grid, st, d, m, sc);
+    }
+    fd_launch_pf<S, P, Q, ZB, DIR, FMT, 1, L>(persist, grid, st, d, m, sc);
 }
 
 // the fp32 / fp16 word formats (precision study) exist for the S = 16, P = 3,
@@ -307,24 +360,31 @@ constexpr bool has_word_formats()
     return S == 16 && P == 3 && Q == 2;
 }
 
-template <int S, int P, int Q, int ZB, int DIR>
+template <int S, int P, int Q, int ZB, int DIR, int L>
 static void fd_launch_dir(int fmt, int pf, bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d,
                           const MeshDev &m, const DepthScales &sc)
 {
     if constexpr (has_word_formats<S, P, Q>()) {
         // the study formats keep the one-plane prefetch
-        if (fmt == WORD_F32) return fd_launch_pf<S, P, Q, ZB, DIR, WORD_F32, 1>(persist, grid, st, d, m, sc);
-        if (fmt == WORD_F16) return fd_launch_pf<S, P, Q, ZB, DIR, WORD_F16, 1>(persist, grid, st, d, m, sc);
+        if (fmt == WORD_F32) return fd_launch_pf<S, P, Q, ZB, DIR, WORD_F32, 1, L>(persist, grid, st, d, m, sc);
+        if (fmt == WORD_F16) return fd_launch_pf<S, P, Q, ZB, DIR, WORD_F16, 1, L>(persist, grid, st, d, m, sc);
     }
-    fd_launch_fmt<S, P, Q, ZB, DIR, WORD_CF3>(pf, persist, grid, st, d, m, sc);
+    fd_launch_fmt<S, P, Q, ZB, DIR, WORD_CF3, L>(pf, persist, grid, st, d, m, sc);
 }
-template <int S, int P, int Q, int ZB>
+template <int S, int P, int Q, int ZB, int L>
 static void fd_launch(int fmt, int pf, bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d,
                       const MeshDev &m, const DepthScales &sc)
 {
-    if (m.direct_axes == 0u) return fd_launch_dir<S, P, Q, ZB, 0>(fmt, pf, persist, grid, st, d, m, sc);
-    if (m.direct_axes == 4u) return fd_launch_dir<S, P, Q, ZB, 1>(fmt, pf, persist, grid, st, d, m, sc);
-    fd_launch_dir<S, P, Q, ZB, 2>(fmt, pf, persist, grid, st, d, m, sc);
+    if (m.direct_axes == 0u) return fd_launch_dir<S, P, Q, ZB, 0, L>(fmt, pf, persist, grid, st, d, m, sc);
+    if (m.direct_axes == 4u) return fd_launch_dir<S, P, Q, ZB, 1, L>(fmt, pf, persist, grid, st, d, m, sc);
+    if constexpr (L == 0) fd_launch_dir<S, P, Q, ZB, 2, L>(fmt, pf, persist, grid, st, d, m, sc);
+}
+template <int S, int P, int Q, int ZB>
+static void fd_launch_layout(int layout, int fmt, int pf, bool persist, dim3 grid, hipStream_t st,
+                             const FdBatchDev &d, const MeshDev &m, const DepthScales &sc)
+{
+    if (layout == AM_LAYOUT_INTERIOR_FIRST) return fd_launch<S, P, Q, ZB, 1>(fmt, pf, persist, grid, st, d, m, sc);
+    fd_launch<S, P, Q, ZB, 0>(fmt, pf, persist, grid, st, d, m, sc);
 }
 
 template <int S, int P, int Q>
@@ -373,12 +433,19 @@ static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometr
     if (pf > 3) return set_error(AM_ERR_INVALID, "am_fd: prefetch_planes %d (0-3)", b->prefetch_planes);
     if (fmt != WORD_CF3 && !(has_word_formats<S, P, Q>() && (fmt == WORD_F32 || fmt == WORD_F16)))
         return set_error(AM_ERR_INVALID, "am_fd: word format %d needs S=16, P=3, fd_order 4", fmt);
+    const int lay = b->tile_layout;
+    if (lay != AM_LAYOUT_PADDED && lay != AM_LAYOUT_INTERIOR_FIRST)
+        return set_error(AM_ERR_INVALID, "am_fd: tile layout %d", lay);
+    if (lay == AM_LAYOUT_INTERIOR_FIRST && (pf > 1 || (b->direct_faces != 0 && b->direct_faces != 4)))
+        return set_error(AM_ERR_INVALID,
+                         "am_fd: the interior-first layout takes prefetch_planes 0-1 and direct_faces 0 or 4 "
+                         "(%d, %d)", b->prefetch_planes, b->direct_faces);
     if (zb == S) {
-        fd_launch<S, P, Q, S>(fmt, pf, persist, dim3(gx(1), b->n_vars), st, d, items(1), sc);
+        fd_launch_layout<S, P, Q, S>(lay, fmt, pf, persist, dim3(gx(1), b->n_vars), st, d, items(1), sc);
     } else if (zb == S / 2) {
-        fd_launch<S, P, Q, S / 2>(fmt, pf, persist, dim3(gx(2), b->n_vars), st, d, items(2), sc);
+        fd_launch_layout<S, P, Q, S / 2>(lay, fmt, pf, persist, dim3(gx(2), b->n_vars), st, d, items(2), sc);
     } else if (zb == S / 4 && (S % 4) == 0) {
-        fd_launch<S, P, Q, S / 4>(fmt, pf, persist, dim3(gx(4), b->n_vars), st, d, items(4), sc);
+        fd_launch_layout<S, P, Q, S / 4>(lay, fmt, pf, persist, dim3(gx(4), b->n_vars), st, d, items(4), sc);
     } else {
         return set_error(AM_ERR_INVALID, "am_fd: z_block %d must be S, S/2 or S/4", zb);
     }

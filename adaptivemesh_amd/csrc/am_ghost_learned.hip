@@ -66,7 +66,7 @@ __device__ __forceinline__ float read_pos(uint32_t w, float rel, float rel_scale
     return (r + .5f) * (float)S + (float)P - 0.5f;
 }
 
-template <int S, int P>
+template <int S, int P, int L>
 __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const uint64_t *__restrict__ bnd,
                                                        const int32_t *__restrict__ tiles, uint32_t substep,
                                                        uint32_t maxd, float frac, const float *__restrict__ prm,
@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
                                                        int n_blocks)
 {
     constexpr int T = S + 2 * P, NC = S * S * P;
-    constexpr size_t CELLS = (size_t)T * T * T;
+    constexpr size_t CELLS = tile_stride<S, P, L>();
     __shared__ float Xs[2][64][65];
     __shared__ float Hs[64][129];
     __shared__ float Bn[64][193];
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
                 const bool inside = s < ns && px >= 0 && px < T && py >= 0 && py < T && pz >= 0 && pz < T && out <= 1;
                 float val = 0.f;
                 if (inside) {
-                    const size_t r = tbase + ((size_t)pz * T + py) * T + px;
+                    const size_t r = tbase + tile_cell<S, P, L>((uint32_t)px, (uint32_t)py, (uint32_t)pz);
                     val = dold[r] * tc1 + dnew[r] * tc2;
                     if (quad) val = dolder[r] * tc0 + val;
                 } else if (s < ns) {
@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
                 if (sa < chunk || sa >= chunk + ns) continue;
                 const int sl = sa - chunk;
                 if (!valid[sl]) continue;
-                const size_t wn = (size_t)node * CELLS + w[0] + ((size_t)w[1] + (size_t)w[2] * T) * T;
+                const size_t wn = (size_t)node * CELLS + tile_cell<S, P, L>(w[0], w[1], w[2]);
                 for (int v = 0; v < 24; v++)
                     b.new_[v][wn] = musg[0][v][sl] + musg[1][v][sl] * Bn[sl][v * 8 + child];
             }
@@ -342,6 +342,8 @@ static int launch_learned(const am_field_batch *fb, const am_mesh *mesh, const a
                           void *stream)
 {
     if (fb->n_vars != 24) return set_error(AM_ERR_INVALID, "am_ghost_fill_learned: needs the 24 BSSN fields");
+    if (fb->tile_layout != AM_LAYOUT_PADDED && fb->tile_layout != AM_LAYOUT_INTERIOR_FIRST)
+        return set_error(AM_ERR_INVALID, "am_ghost_fill_learned: tile layout %d", fb->tile_layout);
     FieldBatchDev d;
     for (int v = 0; v < 24; v++) {
         if (!fb->old_[v] || !fb->new_[v]) return set_error(AM_ERR_INVALID, "am_ghost_fill_learned: null field %d", v);
@@ -357,9 +359,14 @@ static int launch_learned(const am_field_batch *fb, const am_mesh *mesh, const a
     // P/2 + 1 along the normal (the slab's P fine layers)
     constexpr int NS_MAX = (S / 2 + 1) * (S / 2 + 1) * (P / 2 + 1);
     constexpr int NB = (NS_MAX + 63) / 64;
-    hipLaunchKernelGGL((k_ghost_learned<S, P>), dim3(mesh->n_tiles * 6 * NB), dim3(256), 0, (hipStream_t)stream, d,
-                       mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, params, threshold, stats,
-                       NB);
+    if (fb->tile_layout == AM_LAYOUT_INTERIOR_FIRST)
+        hipLaunchKernelGGL((k_ghost_learned<S, P, 1>), dim3(mesh->n_tiles * 6 * NB), dim3(256), 0, (hipStream_t)stream,
+                           d, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, params, threshold,
+                           stats, NB);
+    else
+        hipLaunchKernelGGL((k_ghost_learned<S, P, 0>), dim3(mesh->n_tiles * 6 * NB), dim3(256), 0, (hipStream_t)stream,
+                           d, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, params, threshold,
+                           stats, NB);
     return check_launch("k_ghost_learned");
 }
 

@@ -175,8 +175,10 @@ int rk4_set_field(H *s, int32_t which, int32_t var, const float *host)
         if (int rc = am_memcpy_h2d(d.field(*b, var), in.data(), in.size() * sizeof(float), d.stream)) return rc;
         return am_stream_sync(d.stream);
     }
-    if (int rc = am_memcpy_h2d(d.field(*b, var), host, d.slots * d.cells * sizeof(float), d.stream)) return rc;
-    return am_stream_sync(d.stream);
+    return guarded([&] {  // padded on the host, the driver's layout on the device
+        d.from_padded_host(d.field(*b, var), host, d.slots);
+        return am_stream_sync(d.stream);
+    });
 }
 template <class H>
 int rk4_get_field(H *s, int32_t which, int32_t var, float *host)
@@ -192,7 +194,8 @@ int rk4_get_field(H *s, int32_t which, int32_t var, float *host)
         pad_from_interior(in, host, d.slots, d.geom);
         return AM_OK;
     }
-    return am_memcpy_d2h(host, d.field(*b, var), d.slots * d.cells * sizeof(float), d.stream);
+    return guarded([&] { return am_memcpy_d2h(host, d.to_padded(d.field(*b, var)), d.slots * d.padded * sizeof(float),
+                                              d.stream); });
 }
 template <class H>
 void *rk4_device_ptr(H *s, int32_t which)
@@ -520,8 +523,8 @@ int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
     int P##_set_direct_faces(H *s, int32_t enable)                                                                 \
     {                                                                                                              \
         return rk4_call(s, [&](auto &d) {                                                                          \
-            if (enable < 0 || enable > 7)                                                                          \
-                throw std::runtime_error("direct faces axis mask " + std::to_string(enable) + " (0..7)");          \
+            if (enable != 0 && enable != 4)                                                                        \
+                throw std::runtime_error("direct faces axis mask " + std::to_string(enable) + " (0 or 4)");        \
             d.opt.direct_faces = enable;                                                                               \
             d.drop_graph();                                                                                        \
         });                                                                                                        \
@@ -529,7 +532,8 @@ int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
     int P##_get_direct_faces(const H *s) { return s ? s->amr->sim_data.opt.direct_faces : -1; }                   \
     int P##_set_fd_prefetch(H *s, int32_t planes)                                                                  \
     {                                                                                                              \
-        if (planes < 1 || planes > 3) return am::set_error(AM_ERR_INVALID, "fd prefetch %d (1-3)", planes);        \
+        if (planes != 1)                                                                                           \
+            return am::set_error(AM_ERR_INVALID, "fd prefetch %d (the interior-first FD prefetches 1)", planes);  \
         return rk4_call(s, [&](auto &d) {                                                                          \
             d.opt.fd_prefetch = planes;                                                                            \
             d.drop_graph();                                                                                        \
@@ -564,8 +568,8 @@ int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
         if (!pixels || which < 0 || which > 1) return am::set_error(AM_ERR_INVALID, "raster: bad argument");       \
         return rk4_call(s, [&](auto &d) {                                                                          \
             if (var < 0 || var >= d.NV) throw std::runtime_error("raster: variable out of range");                \
-            raster_to_host(d.parent, d.field(which ? d.new_ : d.old_, var), width, height, z_slice, boxes != 0,    \
-                           pixels, d.stream);                                                                      \
+            raster_to_host(d.parent, d.to_padded(d.field(which ? d.new_ : d.old_, var)), width, height, z_slice,   \
+                           boxes != 0, pixels, d.stream);                                                          \
         });                                                                                                        \
     }                                                                                                              \
     int P##_prolongation_stats(H *s, uint64_t *out4)                                                               \

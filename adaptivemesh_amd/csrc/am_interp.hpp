@@ -122,4 +122,98 @@ __device__ __forceinline__ float tricubic_time(const float *dold, const float *d
     return o * (1.f - ty) + nw * ty;
 }
 
+// ------------------------------------------------------------------
+// The same forms on a tile of either layout: a node (x, y, z) in padded
+// coordinates of the tile starting at `base`, and its neighbours by offset.
+// Each form reads the same cells in the same order and runs the same
+// arithmetic as its padded-only counterpart above, so values are identical.
+template <int S, int P, int L>
+struct Cells {
+    size_t base;
+    uint32_t x, y, z;
+    __device__ __forceinline__ size_t at(int dx, int dy, int dz) const
+    {
+        if constexpr (L == 0) {
+            return base + tile_cell<S, P, 0>((uint32_t)((int)x + dx), (uint32_t)((int)y + dy), (uint32_t)((int)z + dz));
+        } else {
+            // every read of the fills lies in the tile; the clamp keeps a read
+            // position outside it (which the padded layout would take from
+            // the neighbouring row) inside the slot
+            constexpr int T1 = S + 2 * P - 1;
+            return base + tile_cell<S, P, 1>((uint32_t)min(max((int)x + dx, 0), T1),
+                                             (uint32_t)min(max((int)y + dy, 0), T1),
+                                             (uint32_t)min(max((int)z + dz, 0), T1));
+        }
+    }
+};
+
+template <int S, int P, int L>
+__device__ __forceinline__ float interp_lagrange_c(const float *__restrict__ d, const Cells<S, P, L> &c, float rx,
+                                                   float ry, float rz)
+{
+    float f = d[c.at(0, 0, 0)], fx = d[c.at(1, 0, 0)], fy = d[c.at(0, 1, 0)], fz = d[c.at(0, 0, 1)];
+    float fxx = d[c.at(-1, 0, 0)], fyy = d[c.at(0, -1, 0)], fzz = d[c.at(0, 0, -1)];
+    float fxy = d[c.at(1, 1, 0)], fxz = d[c.at(1, 0, 1)], fyz = d[c.at(0, 1, 1)];
+    fxy = fxy + ((f - fx) - fy);
+    fxz = fxz + ((f - fx) - fz);
+    fyz = fyz + ((f - fy) - fz);
+    fx = (fx - fxx) * .5f;
+    fy = (fy - fyy) * .5f;
+    fz = (fz - fzz) * .5f;
+    fxx = (fx + fxx) - f;
+    fyy = (fy + fyy) - f;
+    fzz = (fz + fzz) - f;
+    return f + (fx + fxx * rx) * rx + (fy + fyy * ry) * ry + (fz + fzz * rz) * rz + fxy * rx * ry + fxz * rx * rz +
+           fyz * ry * rz;
+}
+template <int S, int P, int L>
+__device__ __forceinline__ float interp_trilinear_c(const float *__restrict__ d, const Cells<S, P, L> &c, float rx,
+                                                    float ry, float rz)
+{
+    float r00 = d[c.at(0, 0, 0)];
+    r00 = r00 * (1.f - rx) + d[c.at(1, 0, 0)] * rx;
+    float r01 = d[c.at(0, 1, 0)];
+    r01 = r01 * (1.f - rx) + d[c.at(1, 1, 0)] * rx;
+    r00 = r00 * (1.f - ry) + r01 * ry;
+    float r10 = d[c.at(0, 0, 1)];
+    r10 = r10 * (1.f - rx) + d[c.at(1, 0, 1)] * rx;
+    float r11 = d[c.at(0, 1, 1)];
+    r11 = r11 * (1.f - rx) + d[c.at(1, 1, 1)] * rx;
+    r10 = r10 * (1.f - ry) + r11 * ry;
+    return r00 * (1.f - rz) + r10 * rz;
+}
+template <int S, int P, int L>
+__device__ __forceinline__ float interp_tricubic_c(const float *__restrict__ d, const Cells<S, P, L> &c,
+                                                   const float *wx, const float *wy, const float *wz)
+{
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float ay = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float ax = d[c.at(-1, j - 1, k - 1)] * wx[0] + d[c.at(0, j - 1, k - 1)] * wx[1] +
+                             d[c.at(1, j - 1, k - 1)] * wx[2] + d[c.at(2, j - 1, k - 1)] * wx[3];
+            ay = ay + ax * wy[j];
+        }
+        acc = acc + ay * wz[k];
+    }
+    return acc;
+}
+template <int S, int P, int L>
+__device__ __forceinline__ float tricubic_time_c(const float *dold, const float *dnew, const float *dolder,
+                                                 uint32_t ddiff, float ty, const float *wx, const float *wy,
+                                                 const float *wz, const Cells<S, P, L> &c)
+{
+    const float o = interp_tricubic_c<S, P, L>(dold, c, wx, wy, wz);
+    const float nw = interp_tricubic_c<S, P, L>(dnew, c, wx, wy, wz);
+    if (dolder && ddiff > 0) {
+        const float od = interp_tricubic_c<S, P, L>(dolder, c, wx, wy, wz);
+        const float s_ = ty;
+        const float c0 = .5f * s_ * (s_ - 1.f), c1 = (1.f - s_) * (1.f + s_), c2 = .5f * s_ * (s_ + 1.f);
+        return od * c0 + o * c1 + nw * c2;
+    }
+    return o * (1.f - ty) + nw * ty;
+}
+
 }  // namespace am

@@ -126,13 +126,13 @@ __global__ void __launch_bounds__(S *S) k_ghost_fill(FieldBatchDev b, const uint
 // there is f plus terms that are all +-0, and the time weight on `new` is 1,
 // so the copy is the formula's value for finite data.  Everything else runs
 // the full formula of k_ghost_fill.
-template <int S, int P, int MODE, int VPG>
+template <int S, int P, int MODE, int VPG, int L>
 __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n_vars, const uint64_t *__restrict__ bnd,
                                                           const int32_t *__restrict__ tiles, uint32_t substep,
                                                           uint32_t maxd, float frac, int equal_mode, int skip_learned)
 {
     constexpr int T = S + 2 * P, NC = S * S * P;
-    constexpr size_t CELLS = (size_t)T * T * T;
+    constexpr size_t CELLS = tile_stride<S, P, L>();
     const int ti = blockIdx.x / 6, dir = blockIdx.x - ti * 6;
     const int v0 = blockIdx.y * VPG;
     const int node = tiles[ti];
@@ -153,8 +153,20 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
     const int nv = min(VPG, n_vars - v0);
 
     for (int c = threadIdx.x; c < NC; c += 256) {
+        // consecutive lanes follow the slab's contiguous axis: x for the
+        // padded layout, the in-face axis a for the interior-first slabs
         uint32_t w0, w1, w2;
-        if (dir == 0 || dir == 3) {  // x faces: x fastest
+        if constexpr (L == 1) {
+            const uint32_t a = c % S, bq = (c / S) % S, l = c / (S * S);
+            const uint32_t g = dir < 3 ? l + uP + uS : uP - 1u - l;
+            if (dir % 3 == 0) {
+                w0 = g; w1 = a + uP; w2 = bq + uP;
+            } else if (dir % 3 == 1) {
+                w0 = a + uP; w1 = g; w2 = bq + uP;
+            } else {
+                w0 = a + uP; w1 = bq + uP; w2 = g;
+            }
+        } else if (dir == 0 || dir == 3) {  // x faces: x fastest
             const uint32_t l = c % P, a = (c / P) % S, bq = c / (P * S);
             w0 = dir == 0 ? l + uP + uS : l; w1 = a + uP; w2 = bq + uP;
         } else if (dir == 1 || dir == 4) {
@@ -172,13 +184,13 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
             rp[k] = rp[k] * rel_scale + rl[k];
             rp[k] = (rp[k] + .5f) * (float)uS + (float)uP - 0.5f;
         }
-        const size_t wn = (size_t)node * CELLS + w0 + ((size_t)w1 + (size_t)w2 * T) * T;
+        const size_t wn = (size_t)node * CELLS + tile_cell<S, P, L>(w0, w1, w2);
         if (ddiff == 0 && equal_mode == AM_EQUAL_COPY) {
             // the neighbour cell the ghost maps onto: rp is a whole cell index,
             // exactly for power-of-two S and within rounding otherwise (1/S
             // inexact), so it is rounded, not truncated
-            const size_t src = (size_t)tgt * CELLS + (size_t)(rp[0] + .5f) +
-                               ((size_t)(rp[1] + .5f) + (size_t)(rp[2] + .5f) * T) * T;
+            const size_t src = (size_t)tgt * CELLS + tile_cell<S, P, L>((uint32_t)(rp[0] + .5f), (uint32_t)(rp[1] + .5f),
+                                                                         (uint32_t)(rp[2] + .5f));
             for (int v = 0; v < nv; v++) b.new_[v0 + v][wn] = b.new_[v0 + v][src];
             continue;
         }
@@ -195,7 +207,7 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
                 ri[k] = (tgt == 0) ? min((uint32_t)(fmaxf(rp[k], 0.f) + .5f), (uint32_t)T - 2u)
                                    : min((uint32_t)(fmaxf(rp[k], (float)uP) + .5f), (uint32_t)T - uP - 2u);
         }
-        const size_t rn = (size_t)tgt * CELLS + ri[0] + ((size_t)ri[1] + (size_t)ri[2] * T) * T;
+        const Cells<S, P, L> cn{(size_t)tgt * CELLS, ri[0], ri[1], ri[2]};
 #pragma unroll
         for (int k = 0; k < 3; k++) rp[k] -= (float)ri[k];
         if (ddiff == 0 && MODE == AM_INTERP_LAGRANGE) {
@@ -206,30 +218,33 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
             const bool cx = rp[0] != 0.f, cy = rp[1] != 0.f, cz = rp[2] != 0.f;
             for (int v = 0; v < nv; v++) {
                 const float *d = b.new_[v0 + v];
-                const float f = d[rn];
+                const float f = d[cn.at(0, 0, 0)];
                 float r = f;
                 float fx = 0.f, fy = 0.f, fz = 0.f;  // raw +1 neighbours where needed
                 if (cx) {
-                    fx = d[rn + 1];
-                    float gx = (fx - d[rn - 1]) * .5f;
-                    const float gxx = (gx + d[rn - 1]) - f;
+                    fx = d[cn.at(1, 0, 0)];
+                    const float fm = d[cn.at(-1, 0, 0)];
+                    float gx = (fx - fm) * .5f;
+                    const float gxx = (gx + fm) - f;
                     r = r + (gx + gxx * rp[0]) * rp[0];
                 }
                 if (cy) {
-                    fy = d[rn + T];
-                    float gy = (fy - d[rn - T]) * .5f;
-                    const float gyy = (gy + d[rn - T]) - f;
+                    fy = d[cn.at(0, 1, 0)];
+                    const float fm = d[cn.at(0, -1, 0)];
+                    float gy = (fy - fm) * .5f;
+                    const float gyy = (gy + fm) - f;
                     r = r + (gy + gyy * rp[1]) * rp[1];
                 }
                 if (cz) {
-                    fz = d[rn + T * T];
-                    float gz = (fz - d[rn - T * T]) * .5f;
-                    const float gzz = (gz + d[rn - T * T]) - f;
+                    fz = d[cn.at(0, 0, 1)];
+                    const float fm = d[cn.at(0, 0, -1)];
+                    float gz = (fz - fm) * .5f;
+                    const float gzz = (gz + fm) - f;
                     r = r + (gz + gzz * rp[2]) * rp[2];
                 }
-                if (cx && cy) r = r + (d[rn + T + 1] + ((f - fx) - fy)) * rp[0] * rp[1];
-                if (cx && cz) r = r + (d[rn + T * T + 1] + ((f - fx) - fz)) * rp[0] * rp[2];
-                if (cy && cz) r = r + (d[rn + T * T + T] + ((f - fy) - fz)) * rp[1] * rp[2];
+                if (cx && cy) r = r + (d[cn.at(1, 1, 0)] + ((f - fx) - fy)) * rp[0] * rp[1];
+                if (cx && cz) r = r + (d[cn.at(1, 0, 1)] + ((f - fx) - fz)) * rp[0] * rp[2];
+                if (cy && cz) r = r + (d[cn.at(0, 1, 1)] + ((f - fy) - fz)) * rp[1] * rp[2];
                 b.new_[v0 + v][wn] = r;
             }
             continue;
@@ -240,7 +255,7 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
             // the 4-point weights are a unit vector and the value is the
             // target cell at that offset.  Otherwise (1/S inexact) the offsets
             // are a rounding away from whole and the formula below applies.
-            const size_t src = rn + (ptrdiff_t)rp[0] + (ptrdiff_t)rp[1] * T + (ptrdiff_t)rp[2] * (T * T);
+            const size_t src = cn.at((int)rp[0], (int)rp[1], (int)rp[2]);
             for (int v = 0; v < nv; v++) b.new_[v0 + v][wn] = b.new_[v0 + v][src];
             continue;
         }
@@ -249,18 +264,18 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
             lag4(rp[0], wx); lag4(rp[1], wy); lag4(rp[2], wz);
             for (int v = 0; v < nv; v++)
                 b.new_[v0 + v][wn] =
-                    tricubic_time<T>(b.old_[v0 + v], b.new_[v0 + v], b.older[v0 + v], ddiff, ty, wx, wy, wz, rn);
+                    tricubic_time_c<S, P, L>(b.old_[v0 + v], b.new_[v0 + v], b.older[v0 + v], ddiff, ty, wx, wy, wz, cn);
             continue;
         }
         for (int v = 0; v < nv; v++) {
             const float *dold = b.old_[v0 + v], *dnew = b.new_[v0 + v];
             float o, nw;
             if (MODE == AM_INTERP_TRILINEAR) {
-                o = interp_trilinear<T>(dold, rp[0], rp[1], rp[2], rn);
-                nw = interp_trilinear<T>(dnew, rp[0], rp[1], rp[2], rn);
+                o = interp_trilinear_c<S, P, L>(dold, cn, rp[0], rp[1], rp[2]);
+                nw = interp_trilinear_c<S, P, L>(dnew, cn, rp[0], rp[1], rp[2]);
             } else {
-                o = interp_lagrange<T>(dold, rp[0], rp[1], rp[2], rn);
-                nw = interp_lagrange<T>(dnew, rp[0], rp[1], rp[2], rn);
+                o = interp_lagrange_c<S, P, L>(dold, cn, rp[0], rp[1], rp[2]);
+                nw = interp_lagrange_c<S, P, L>(dnew, cn, rp[0], rp[1], rp[2]);
             }
             b.new_[v0 + v][wn] = o * (1.f - ty) + nw * ty;
         }
@@ -278,13 +293,16 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
 // gathers drop about five-fold.  Where clamping at tile edges gives more
 // than 2 fine cells along an axis the same base, they are split into groups
 // of at most 2, so every thread's work stays 2 x 2 x 2 children at most.
-template <int S, int P, int VPG>
+// L: tile layout.  In the interior-first layout the 4 x 4 x 4 nodes of a
+// non-root target are interior cells, rows of S with planes of S^2; the
+// root (whose ghost cells a fill may read) is addressed cell by cell.
+template <int S, int P, int VPG, int L>
 __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_vars, const uint64_t *__restrict__ bnd,
                                                         const int32_t *__restrict__ tiles, uint32_t substep,
                                                         uint32_t maxd, float frac, int equal_mode)
 {
     constexpr int T = S + 2 * P, NC = S * S * P;
-    constexpr size_t CELLS = (size_t)T * T * T;
+    constexpr size_t CELLS = tile_stride<S, P, L>();
     constexpr uint32_t uP = P, uS = S;
     const int ti = blockIdx.x / 6, dir = blockIdx.x - ti * 6;
     const int v0 = blockIdx.y * VPG;
@@ -332,9 +350,9 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
                 rp[k] = rp[k] * rel_scale + rl[k];
                 rp[k] = (rp[k] + .5f) * (float)uS + (float)uP - 0.5f;
             }
-            const size_t wn = (size_t)node * CELLS + w[0] + ((size_t)w[1] + (size_t)w[2] * T) * T;
-            const size_t src = (size_t)tgt * CELLS + (size_t)(rp[0] + .5f) +
-                               ((size_t)(rp[1] + .5f) + (size_t)(rp[2] + .5f) * T) * T;
+            const size_t wn = (size_t)node * CELLS + tile_cell<S, P, L>(w[0], w[1], w[2]);
+            const size_t src = (size_t)tgt * CELLS + tile_cell<S, P, L>((uint32_t)(rp[0] + .5f), (uint32_t)(rp[1] + .5f),
+                                                                         (uint32_t)(rp[2] + .5f));
             for (int v = 0; v < nv; v++) b.new_[v0 + v][wn] = b.new_[v0 + v][src];
         }
         return;
@@ -378,13 +396,20 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
     const int items = nr0 * nr1 * nr2;
     const bool quad = ddiff > 0;  // with older: quadratic in time
     const float c0 = .5f * ty * (ty - 1.f), c1 = (1.f - ty) * (1.f + ty), c2 = .5f * ty * (ty + 1.f);
-    constexpr int T2 = T * T;
+    // row and plane pitch of the stencil rows: the padded tile, or the
+    // interior block of the interior-first layout (non-root targets)
+    constexpr int PY = L == 0 ? T : S, PZ = L == 0 ? T * T : S * S;
+    const bool by_cell = L == 1 && tgt == 0;  // uniform per workgroup
     for (int job = threadIdx.x; job < items * nv; job += 256) {
         const int v = job / items, it = job - v * items;
         const int q0 = it % nr0, q1 = (it / nr0) % nr1, q2 = it / (nr0 * nr1);
         const int a0 = s_run0[0][q0], a1 = s_run0[1][q1], a2 = s_run0[2][q2];
         const int n0 = s_runn[0][q0], n1 = s_runn[1][q1], n2 = s_runn[2][q2];
-        const size_t rn = (size_t)tgt * CELLS + s_ri[0][a0] + ((size_t)s_ri[1][a1] + (size_t)s_ri[2][a2] * T) * T;
+        const uint32_t n0x = s_ri[0][a0], n0y = s_ri[1][a1], n0z = s_ri[2][a2];
+        const size_t rn = (size_t)tgt * CELLS +
+                          (L == 0 ? tile_cell<S, P, 0>(n0x, n0y, n0z)
+                                  : (by_cell ? 0u : tile_interior<S, P, L>(n0x - P, n0y - P, n0z - P)));
+        const Cells<S, P, L> cn{(size_t)tgt * CELLS, n0x, n0y, n0z};
         const float *dold = b.old_[v0 + v], *dnew = b.new_[v0 + v], *dolder = b.older[v0 + v];
         const bool three = dolder && quad;
         float wx[2][4], wy[2][4], wz[2][4];
@@ -406,8 +431,16 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
             for (int k = 0; k < 4; k++)
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const float *r = d + rn + (ptrdiff_t)(k - 1) * T2 + (ptrdiff_t)(j - 1) * T - 1;
-                    const float r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+                    float r0, r1, r2, r3;
+                    if (by_cell) {
+                        r0 = d[cn.at(-1, j - 1, k - 1)];
+                        r1 = d[cn.at(0, j - 1, k - 1)];
+                        r2 = d[cn.at(1, j - 1, k - 1)];
+                        r3 = d[cn.at(2, j - 1, k - 1)];
+                    } else {
+                        const float *r = d + rn + (ptrdiff_t)(k - 1) * PZ + (ptrdiff_t)(j - 1) * PY - 1;
+                        r0 = r[0]; r1 = r[1]; r2 = r[2]; r3 = r[3];
+                    }
 #pragma unroll
                     for (int cx = 0; cx < 2; cx++)
                         ax[cx][k][j] = r0 * wx[cx][0] + r1 * wx[cx][1] + r2 * wx[cx][2] + r3 * wx[cx][3];
@@ -438,14 +471,41 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
                     if (cx >= n0 || cy >= n1 || cz >= n2) continue;
                     const float o = res[1][cx][cy][cz], nw = res[2][cx][cy][cz];
                     const float val = three ? res[0][cx][cy][cz] * c0 + o * c1 + nw * c2 : o * (1.f - ty) + nw * ty;
-                    const size_t wn = (size_t)node * CELLS + (uint32_t)(wlo[0] + a0 + cx) +
-                                      ((size_t)(uint32_t)(wlo[1] + a1 + cy) + (size_t)(uint32_t)(wlo[2] + a2 + cz) * T) * T;
+                    const size_t wn = (size_t)node * CELLS + tile_cell<S, P, L>((uint32_t)(wlo[0] + a0 + cx),
+                                                                                 (uint32_t)(wlo[1] + a1 + cy),
+                                                                                 (uint32_t)(wlo[2] + a2 + cz));
                     b.new_[v0 + v][wn] = val;
                 }
     }
 }
 
 constexpr int GHOST_VPG = 6;  // fields per workgroup in the multi-field fill
+
+template <int S, int P, int L>
+static int launch_ghost_multi_l(const FieldBatchDev &d, int n_vars, const am_mesh *mesh, int interp, float frac,
+                                int equal_mode, int skip_learned, void *stream)
+{
+    dim3 grid(mesh->n_tiles * 6, (n_vars + GHOST_VPG - 1) / GHOST_VPG);
+    hipStream_t st = (hipStream_t)stream;
+    if (interp == AM_INTERP_LAGRANGE)
+        hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_LAGRANGE, GHOST_VPG, L>), grid, dim3(256), 0, st, d,
+                           n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, equal_mode,
+                           skip_learned);
+    else if (interp == AM_INTERP_TRILINEAR)
+        hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRILINEAR, GHOST_VPG, L>), grid, dim3(256), 0, st, d,
+                           n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, equal_mode,
+                           skip_learned);
+    else if (interp == AM_INTERP_TRICUBIC && !skip_learned)
+        hipLaunchKernelGGL((k_ghost_tricubic<S, P, GHOST_VPG, L>), grid, dim3(256), 0, st, d, n_vars,
+                           mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, equal_mode);
+    else if (interp == AM_INTERP_TRICUBIC)
+        hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRICUBIC, GHOST_VPG, L>), grid, dim3(256), 0, st, d,
+                           n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, equal_mode,
+                           skip_learned);
+    else
+        return set_error(AM_ERR_INVALID, "am_ghost_fill: unknown interpolation mode %d", interp);
+    return check_launch("k_ghost_fill_multi");
+}
 
 template <int S, int P>
 static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, int interp, float frac, int equal_mode,
@@ -458,32 +518,16 @@ static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, int 
         d.new_[v] = b->new_[v];
         d.older[v] = b->older[v];
     }
-    dim3 grid(mesh->n_tiles * 6, (b->n_vars + GHOST_VPG - 1) / GHOST_VPG);
-    hipStream_t st = (hipStream_t)stream;
-    if (interp == AM_INTERP_LAGRANGE)
-        hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_LAGRANGE, GHOST_VPG>), grid, dim3(256), 0, st, d,
-                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac,
-                           equal_mode, skip_learned);
-    else if (interp == AM_INTERP_TRILINEAR)
-        hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRILINEAR, GHOST_VPG>), grid, dim3(256), 0, st, d,
-                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac,
-                           equal_mode, skip_learned);
-    else if (interp == AM_INTERP_TRICUBIC && !skip_learned)
-        hipLaunchKernelGGL((k_ghost_tricubic<S, P, GHOST_VPG>), grid, dim3(256), 0, st, d, b->n_vars,
-                           mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, equal_mode);
-    else if (interp == AM_INTERP_TRICUBIC)
-        hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRICUBIC, GHOST_VPG>), grid, dim3(256), 0, st, d,
-                           b->n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac,
-                           equal_mode, skip_learned);
-    else
-        return set_error(AM_ERR_INVALID, "am_ghost_fill: unknown interpolation mode %d", interp);
-    return check_launch("k_ghost_fill_multi");
+    AM_DISPATCH_LAYOUT(b->tile_layout, launch_ghost_multi_l, d, b->n_vars, mesh, interp, frac, equal_mode, skip_learned,
+                       stream);
 }
 
 template <int S, int P>
 static int launch_ghost(const am_field_batch *b, const am_mesh *mesh, int copy_only_new, int interp, float frac,
                         void *stream)
 {
+    if (b->tile_layout != AM_LAYOUT_PADDED)
+        return set_error(AM_ERR_INVALID, "am_ghost_fill_batched: the reference fill takes padded tiles only");
     FieldBatchDev d;
     for (int v = 0; v < b->n_vars; v++) {
         if (!b->old_[v] || !b->new_[v]) return set_error(AM_ERR_INVALID, "am_ghost_fill: null field %d", v);
@@ -643,6 +687,91 @@ __global__ void __launch_bounds__(256) k_restricted_to_new(FieldBatchDev b, cons
     }
 }
 
+// k_copy_new_to_old and k_restricted_to_new on interior-first tiles: the
+// S^3 interior block comes first in each slot, so a tile that keeps its
+// ghost cells copies its whole slot and one that does not copies only that
+// block, both as 16-byte vectors; interior cells covered by a child take the
+// mean of its 2x2x2 interior cells, summed in the reference's order.
+template <int S, int P>
+__global__ void __launch_bounds__(256) k_copy_back_if(FieldBatchDev b, const uint32_t *__restrict__ positions,
+                                                      const int32_t *__restrict__ children,
+                                                      const int32_t *__restrict__ tiles, uint32_t substep,
+                                                      uint32_t maxd, const uint8_t *__restrict__ keep_ghosts)
+{
+    constexpr uint32_t TS = tile_stride<S, P, 1>(), S3 = S * S * S, H = S / 2;
+    static_assert(S3 % 4 == 0 && TS % 4 == 0, "vector copies");
+    const int node = tiles[blockIdx.x];
+    const int depth = pos_depth(positions[node]);
+    if (depth == -1 || !active_depth(substep, (uint32_t)depth, maxd)) return;
+    float *__restrict__ dold = b.old_[blockIdx.y];
+    const float *__restrict__ dnew = b.new_[blockIdx.y];
+    float *__restrict__ dolder = const_cast<float *>(b.older[blockIdx.y]);
+    __shared__ int ch[8];
+    if (threadIdx.x < 8) ch[threadIdx.x] = children[(size_t)node * 8 + threadIdx.x];
+    __syncthreads();
+    bool any_child = false;
+#pragma unroll
+    for (int i = 0; i < 8; i++) any_child |= ch[i] != -1;
+    const size_t base = (size_t)node * TS;
+    const bool whole = !keep_ghosts || keep_ghosts[node];
+    auto copy4 = [&](uint32_t lo, uint32_t hi) {
+        const float4 *s4 = reinterpret_cast<const float4 *>(dnew + base);
+        float4 *d4 = reinterpret_cast<float4 *>(dold + base);
+        float4 *o4 = dolder ? reinterpret_cast<float4 *>(dolder + base) : nullptr;
+        for (uint32_t i = lo / 4 + threadIdx.x; i < hi / 4; i += 256) {
+            if (o4) o4[i] = d4[i];
+            d4[i] = s4[i];
+        }
+    };
+    if (!any_child) {  // uniform per workgroup
+        copy4(0u, whole ? TS : S3);
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < S3; i += 256) {
+        const uint32_t zi = i / (S * S), yi = (i / S) % S, xi = i % S;
+        const size_t o = base + i;
+        if (dolder) dolder[o] = dold[o];
+        const int child = ch[xi / H + (yi / H) * 2u + (zi / H) * 4u];
+        if (child == -1) {
+            dold[o] = dnew[o];
+            continue;
+        }
+        const size_t c = (size_t)child * TS + (((zi * 2u) % S) * S + (yi * 2u) % S) * S + (xi * 2u) % S;
+        constexpr size_t SY = S, SZ = S * S;
+        dold[o] = (dnew[c] + dnew[c + 1] + dnew[c + SY] + dnew[c + SY + 1] + dnew[c + SZ] + dnew[c + SZ + 1] +
+                   dnew[c + SZ + SY] + dnew[c + SZ + SY + 1]) *
+                  .125f;
+    }
+    if (whole) copy4(S3, TS);
+}
+
+template <int S, int P>
+__global__ void __launch_bounds__(256) k_restricted_to_new_if(FieldBatchDev b, const uint32_t *__restrict__ positions,
+                                                              const int32_t *__restrict__ children,
+                                                              const int32_t *__restrict__ tiles, uint32_t substep,
+                                                              uint32_t maxd)
+{
+    constexpr uint32_t TS = tile_stride<S, P, 1>(), S3 = S * S * S, H = S / 2;
+    const int node = tiles[blockIdx.x];
+    const int depth = pos_depth(positions[node]);
+    if (depth == -1 || !active_depth(substep, (uint32_t)depth, maxd)) return;
+    __shared__ int ch[8];
+    if (threadIdx.x < 8) ch[threadIdx.x] = children[(size_t)node * 8 + threadIdx.x];
+    __syncthreads();
+    bool any_child = false;
+#pragma unroll
+    for (int i = 0; i < 8; i++) any_child |= ch[i] != -1;
+    if (!any_child) return;  // uniform per workgroup
+    const float *dold = b.old_[blockIdx.y];
+    float *dnew = b.new_[blockIdx.y];
+    const size_t base = (size_t)node * TS;
+    for (uint32_t i = threadIdx.x; i < S3; i += 256) {
+        const uint32_t zi = i / (S * S), yi = (i / S) % S, xi = i % S;
+        if (ch[xi / H + (yi / H) * 2u + (zi / H) * 4u] == -1) continue;
+        dnew[base + i] = dold[base + i];
+    }
+}
+
 template <int S, int P>
 static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, const uint8_t *keep_ghosts, void *stream)
 {
@@ -656,6 +785,16 @@ static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, const 
         d.older[v] = b->older[v];
     }
     dim3 grid(mesh->n_tiles, b->n_vars);
+    if (b->tile_layout == AM_LAYOUT_INTERIOR_FIRST) {
+        hipLaunchKernelGGL((k_copy_back_if<S, P>), grid, dim3(256), 0, (hipStream_t)stream, d, mesh->positions,
+                           mesh->children, mesh->tiles, mesh->substep, mesh->max_depth, keep_ghosts);
+        if (keep_ghosts)
+            hipLaunchKernelGGL((k_restricted_to_new_if<S, P>), grid, dim3(256), 0, (hipStream_t)stream, d,
+                               mesh->positions, mesh->children, mesh->tiles, mesh->substep, mesh->max_depth);
+        return check_launch("k_copy_back_if");
+    }
+    if (b->tile_layout != AM_LAYOUT_PADDED)
+        return set_error(AM_ERR_INVALID, "am_copy_new_to_old: tile layout %d", b->tile_layout);
     hipLaunchKernelGGL((k_copy_new_to_old<S, P>), grid, dim3(256), 0, (hipStream_t)stream, d, mesh->positions,
                        mesh->children, mesh->tiles, mesh->substep, mesh->max_depth, keep_ghosts);
     if (keep_ghosts)
@@ -664,12 +803,12 @@ static int launch_copy_back(const am_field_batch *b, const am_mesh *mesh, const 
     return check_launch("k_copy_new_to_old");
 }
 
-// __copy_to_child (AMR.h:1053-1073)
-template <int S, int P, int MODE>
+// __copy_to_child (AMR.h:1053-1073); L: tile layout of the buffers
+template <int S, int P, int MODE, int L>
 __global__ void __launch_bounds__(256) k_copy_to_child(ChildBufs bufs, uint32_t offset, int child, int parent)
 {
     constexpr int T = S + 2 * P, T2 = T * T;
-    constexpr size_t CELLS = (size_t)T * T2;
+    constexpr size_t CELLS = tile_stride<S, P, L>();
     float *buf = bufs.buf[blockIdx.y];
     const int z = blockIdx.x;
     const float invS = 1.f / (float)S;
@@ -686,29 +825,42 @@ __global__ void __launch_bounds__(256) k_copy_to_child(ChildBufs bufs, uint32_t 
             rp[k] = (rp[k] + .5f) * (float)(uint32_t)S + (float)(uint32_t)P - .5f;
             ri[k] = (MODE == AM_INTERP_TRILINEAR) ? (uint32_t)rp[k] : (uint32_t)(rp[k] + .5f);
         }
-        const size_t rn = (size_t)parent * CELLS + ri[0] + ((size_t)ri[1] + (size_t)ri[2] * T) * T;
 #pragma unroll
         for (int k = 0; k < 3; k++) rp[k] -= (float)ri[k];
-        const float v = (MODE == AM_INTERP_TRILINEAR) ? interp_trilinear<T>(buf, rp[0], rp[1], rp[2], rn)
-                                                      : interp_lagrange<T>(buf, rp[0], rp[1], rp[2], rn);
-        buf[(size_t)child * CELLS + x + ((size_t)y + (size_t)z * T) * T] = v;
+        float v;
+        if constexpr (L == 0) {
+            const size_t rn = (size_t)parent * CELLS + ri[0] + ((size_t)ri[1] + (size_t)ri[2] * T) * T;
+            v = (MODE == AM_INTERP_TRILINEAR) ? interp_trilinear<T>(buf, rp[0], rp[1], rp[2], rn)
+                                              : interp_lagrange<T>(buf, rp[0], rp[1], rp[2], rn);
+        } else {
+            const Cells<S, P, L> cn{(size_t)parent * CELLS, ri[0], ri[1], ri[2]};
+            v = (MODE == AM_INTERP_TRILINEAR) ? interp_trilinear_c<S, P, L>(buf, cn, rp[0], rp[1], rp[2])
+                                              : interp_lagrange_c<S, P, L>(buf, cn, rp[0], rp[1], rp[2]);
+        }
+        buf[(size_t)child * CELLS + tile_cell<S, P, L>((uint32_t)x, (uint32_t)y, (uint32_t)z)] = v;
     }
 }
 
 // one launch for n buffers (grid.y = n): each buffer is prolonged from its
 // own parent tile
-template <int S, int P>
-static int launch_to_child(const ChildBufs &bufs, int n, uint32_t offset, int child, int parent, int interp,
-                           void *stream)
+template <int S, int P, int L>
+static int launch_to_child_l(const ChildBufs &bufs, int n, uint32_t offset, int child, int parent, int interp,
+                             void *stream)
 {
     constexpr int T = S + 2 * P;
     if (interp == AM_INTERP_TRILINEAR)
-        hipLaunchKernelGGL((k_copy_to_child<S, P, AM_INTERP_TRILINEAR>), dim3(T, n), dim3(256), 0,
+        hipLaunchKernelGGL((k_copy_to_child<S, P, AM_INTERP_TRILINEAR, L>), dim3(T, n), dim3(256), 0,
                            (hipStream_t)stream, bufs, offset, child, parent);
     else
-        hipLaunchKernelGGL((k_copy_to_child<S, P, AM_INTERP_LAGRANGE>), dim3(T, n), dim3(256), 0,
+        hipLaunchKernelGGL((k_copy_to_child<S, P, AM_INTERP_LAGRANGE, L>), dim3(T, n), dim3(256), 0,
                            (hipStream_t)stream, bufs, offset, child, parent);
     return check_launch("k_copy_to_child");
+}
+template <int S, int P>
+static int launch_to_child(const ChildBufs &bufs, int n, uint32_t offset, int child, int parent, int interp,
+                           int layout, void *stream)
+{
+    AM_DISPATCH_LAYOUT(layout, launch_to_child_l, bufs, n, offset, child, parent, interp, stream);
 }
 
 static int mesh_checks(const am_mesh *mesh, const am_geometry *g, bool need_bnd, bool need_children)
@@ -814,7 +966,7 @@ extern "C" int am_copy_to_child(float *buf, uint32_t offset, int32_t child, int3
     if (int rc = to_child_checks(offset, child, parent, g, interp)) return rc;
     am::ChildBufs b{};
     b.buf[0] = buf;
-    AM_DISPATCH_TILE(*g, am::launch_to_child, b, 1, offset, child, parent, interp, stream);
+    AM_DISPATCH_TILE(*g, am::launch_to_child, b, 1, offset, child, parent, interp, AM_LAYOUT_PADDED, stream);
 }
 
 extern "C" int am_copy_to_child_batched(const am_field_batch *fb, uint32_t offset, int32_t child, int32_t parent,
@@ -832,5 +984,5 @@ extern "C" int am_copy_to_child_batched(const am_field_batch *fb, uint32_t offse
         if (fb->older[v]) b.buf[n++] = const_cast<float *>(fb->older[v]);
     }
     if (n == 0) return AM_OK;
-    AM_DISPATCH_TILE(*g, am::launch_to_child, b, n, offset, child, parent, interp, stream);
+    AM_DISPATCH_TILE(*g, am::launch_to_child, b, n, offset, child, parent, interp, fb->tile_layout, stream);
 }

@@ -44,11 +44,16 @@ __device__ __forceinline__ void rk4_combine(const float *const *state, const flo
     }
 }
 
-template <int NV, int S, int P>
+// L: tile layout (0 padded, 1 interior-first).  In the interior-first layout
+// the cell lands in the target's slab of the opposite face, layer = its
+// distance from the shared face, so the cells of one row of this tile go to
+// one contiguous slab row of the target.
+template <int NV, int S, int P, int L = 0>
 __device__ __forceinline__ void rk4_push(float *const *new_, const float (&y)[NV], const uint64_t *boundaries, int node,
                                          int x, int y_, int z, int layers, int axes)
 {
     constexpr int T = S + 2 * P;
+    constexpr size_t TS = tile_stride<S, P, L>();
     const int c3[3] = {x, y_, z};
 #pragma unroll
     for (int f = 0; f < 6; f++) {
@@ -58,9 +63,16 @@ __device__ __forceinline__ void rk4_push(float *const *new_, const float (&y)[NV
         const uint32_t nd = (uint32_t)boundaries[(size_t)node * 6 + f];
         const int tgt = bnd_target(nd);
         if (tgt < 0 || bnd_depth(nd) <= 0 || bnd_ddiff(nd) != 0) continue;
-        int p3[3] = {x + P, y_ + P, z + P};
-        p3[ax] += f < 3 ? -S : S;
-        const size_t dst = (size_t)tgt * (T * T * T) + ((size_t)p3[2] * T + p3[1]) * T + p3[0];
+        size_t dst;
+        if constexpr (L == 0) {
+            int p3[3] = {x + P, y_ + P, z + P};
+            p3[ax] += f < 3 ? -S : S;
+            dst = (size_t)tgt * TS + ((size_t)p3[2] * T + p3[1]) * T + p3[0];
+        } else {
+            const uint32_t l = f < 3 ? (uint32_t)(S - 1 - c3[ax]) : (uint32_t)c3[ax];
+            const uint32_t a = (uint32_t)c3[ax == 0 ? 1 : 0], bq = (uint32_t)c3[ax == 2 ? 1 : 2];
+            dst = (size_t)tgt * TS + slab_cell<S, P>((uint32_t)((f + 3) % 6), l, a, bq);
+        }
 #pragma unroll
         for (int v = 0; v < NV; v++) new_[v][dst] = y[v];
     }

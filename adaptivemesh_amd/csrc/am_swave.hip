@@ -18,6 +18,7 @@ struct SwaveArgsDev {
     const uint32_t *d2;
     int push_layers;
     int push_axes;  // bits 1 x, 2 y, 4 z
+    int layout;     // am_tile_layout of state, old_, new_
 };
 
 struct SwaveTiles {
@@ -28,10 +29,10 @@ struct SwaveTiles {
     int32_t n_items;  // tiles x 256-cell blocks; workgroups loop over them
 };
 
-template <int S, int P, int STAGE, bool PUSH, bool PERSIST>
+template <int S, int P, int STAGE, bool PUSH, bool PERSIST, int L>
 __global__ void __launch_bounds__(256) k_swave_stage(SwaveArgsDev a, SwaveTiles m, DepthScales sc)
 {
-    constexpr int T = S + 2 * P, S3 = S * S * S, BPT = (S3 + 255) / 256;
+    constexpr int S3 = S * S * S, BPT = (S3 + 255) / 256;
     const int item_end = PERSIST ? m.n_items : (int)blockIdx.x + 1;
     for (int item = blockIdx.x; item < item_end; item += PERSIST ? gridDim.x : 1) {
         const int ti = item / BPT, cb = item - ti * BPT;
@@ -41,27 +42,27 @@ __global__ void __launch_bounds__(256) k_swave_stage(SwaveArgsDev a, SwaveTiles 
         const int cell = cb * 256 + (int)threadIdx.x;
         if (cell >= S3) continue;
         const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
-        const size_t r = (size_t)node * (T * T * T) + ((size_t)(z + P) * T + (y + P)) * T + (x + P);
+        const size_t r = (size_t)node * tile_stride<S, P, L>() + tile_interior<S, P, L>(x, y, z);
         const size_t w = (size_t)node * S3 + cell;
         const float3 h = cf3_unpack(a.d2[w]);
         float k[2] = {a.state[1][r], (h.x + h.y) + h.z};
         rk4_combine<2, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
-        if (PUSH) rk4_push<2, S, P>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes);
+        if (PUSH) rk4_push<2, S, P, L>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes);
     }
 }
 
 // Per-workgroup FP64 partials {energy, sum u^2 h^3, cells, max |u|}.
-template <int S, int P>
+template <int S, int P, int L>
 __global__ void __launch_bounds__(256) k_swave_energy(SwaveArgsDev a, SwaveTiles m, double outer_dx, double *partial)
 {
-    constexpr int T = S + 2 * P, S3 = S * S * S, BPT = (S3 + 255) / 256;
+    constexpr int S3 = S * S * S, BPT = (S3 + 255) / 256;
     const int ti = blockIdx.x / BPT, cb = blockIdx.x - ti * BPT;
     const int node = m.tiles[ti];
     const int cell = cb * 256 + (int)threadIdx.x;
     double e = 0.0, u2 = 0.0, cnt = 0.0, umax = 0.0;
     if (cell < S3) {
         const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
-        const size_t r = (size_t)node * (T * T * T) + ((size_t)(z + P) * T + (y + P)) * T + (x + P);
+        const size_t r = (size_t)node * tile_stride<S, P, L>() + tile_interior<S, P, L>(x, y, z);
         const int depth = pos_depth(m.positions[node]);
         const double hc = outer_dx / (double)(1u << depth), vol = hc * hc * hc;
         const float3 g = cf3_unpack(a.d1[(size_t)node * S3 + cell]);
@@ -107,20 +108,30 @@ static int swave_args(const am_swave_fields *f, SwaveArgsDev &a, bool need_stage
     }
     a.d1 = f->d1_u;
     a.d2 = f->d2diag_u;
+    a.layout = f->tile_layout;
+    if (a.layout != AM_LAYOUT_PADDED && a.layout != AM_LAYOUT_INTERIOR_FIRST)
+        return set_error(AM_ERR_INVALID, "am_swave: tile layout %d", a.layout);
     if (need_stage ? !a.d2 : !a.d1) return set_error(AM_ERR_INVALID, "am_swave: null derivative word");
     return AM_OK;
 }
 
+template <int S, int P, bool PUSH, bool PERSIST, int L>
+static void swave_launch_l(int stage, dim3 grid, hipStream_t st, const SwaveArgsDev &a, const SwaveTiles &m,
+                           const DepthScales &sc)
+{
+    switch (stage) {
+    case 0: hipLaunchKernelGGL((k_swave_stage<S, P, 0, PUSH, PERSIST, L>), grid, dim3(256), 0, st, a, m, sc); break;
+    case 1: hipLaunchKernelGGL((k_swave_stage<S, P, 1, PUSH, PERSIST, L>), grid, dim3(256), 0, st, a, m, sc); break;
+    case 2: hipLaunchKernelGGL((k_swave_stage<S, P, 2, PUSH, PERSIST, L>), grid, dim3(256), 0, st, a, m, sc); break;
+    default: hipLaunchKernelGGL((k_swave_stage<S, P, 3, PUSH, PERSIST, L>), grid, dim3(256), 0, st, a, m, sc); break;
+    }
+}
 template <int S, int P, bool PUSH, bool PERSIST>
 static void swave_launch(int stage, dim3 grid, hipStream_t st, const SwaveArgsDev &a, const SwaveTiles &m,
                          const DepthScales &sc)
 {
-    switch (stage) {
-    case 0: hipLaunchKernelGGL((k_swave_stage<S, P, 0, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, sc); break;
-    case 1: hipLaunchKernelGGL((k_swave_stage<S, P, 1, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, sc); break;
-    case 2: hipLaunchKernelGGL((k_swave_stage<S, P, 2, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, sc); break;
-    default: hipLaunchKernelGGL((k_swave_stage<S, P, 3, PUSH, PERSIST>), grid, dim3(256), 0, st, a, m, sc); break;
-    }
+    if (a.layout == AM_LAYOUT_INTERIOR_FIRST) return swave_launch_l<S, P, PUSH, PERSIST, 1>(stage, grid, st, a, m, sc);
+    swave_launch_l<S, P, PUSH, PERSIST, 0>(stage, grid, st, a, m, sc);
 }
 
 template <int S, int P>
@@ -164,8 +175,12 @@ static int swave_energy(const am_swave_fields *f, const am_mesh *mesh, const am_
     if (int rc = swave_args(f, a, false)) return rc;
     const SwaveTiles m{mesh->positions, mesh->tiles, mesh->boundaries, mesh->substep, mesh->max_depth,
                        mesh->n_tiles * BPT};
-    hipLaunchKernelGGL((k_swave_energy<S, P>), dim3(mesh->n_tiles * BPT), dim3(256), 0, (hipStream_t)stream, a, m,
-                       (double)g->outer_dx, partial);
+    if (a.layout == AM_LAYOUT_INTERIOR_FIRST)
+        hipLaunchKernelGGL((k_swave_energy<S, P, 1>), dim3(mesh->n_tiles * BPT), dim3(256), 0, (hipStream_t)stream, a,
+                           m, (double)g->outer_dx, partial);
+    else
+        hipLaunchKernelGGL((k_swave_energy<S, P, 0>), dim3(mesh->n_tiles * BPT), dim3(256), 0, (hipStream_t)stream, a,
+                           m, (double)g->outer_dx, partial);
     return check_launch("k_swave_energy");
 }
 

@@ -71,12 +71,12 @@ class PairSet:
 
     @staticmethod
     def from_sim(sim, tiles, which: int = 1):
-        """Pairs of a driver's fields (`which` 0 old, 1 new)."""
-        base = sim.device_ptr(which)
-        T = sim.geom.T
-        stride = sim.max_slots * T ** 3 * 4
-        sim.sync()
-        return PairSet.from_fields([base + v * stride for v in range(24)], tiles, sim.geom)
+        """Pairs of a driver's fields (`which` 0 old, 1 new), converted to
+        padded tiles first (the driver keeps them interior-first)."""
+        padded = sim.padded_fields(which)
+        out = PairSet.from_fields([padded[v].data_ptr() for v in range(24)], tiles, sim.geom)
+        _torch().cuda.synchronize()  # the pair kernel has read `padded` before it is freed
+        return out
 
     @staticmethod
     def cat(sets):

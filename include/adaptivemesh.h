@@ -19,6 +19,10 @@
  *   children   int32  8 per slot, -1 = none (AMR.h:123)
  *   boundaries uint64 6 per slot, node_data | rel_pos << 32 (AMR.h:56-100)
  *
+ * Field arrays may instead use the interior-first tile layout
+ * (AM_LAYOUT_INTERIOR_FIRST, below), selected per call by the tile_layout
+ * member of the field structs; the RK4 drivers keep their state in it.
+ *
  * Each function comment names the reference kernel and host wrapper it
  * replaces.  The reference launches over every used slot and lets inactive
  * depths exit early; here the caller passes a device list of slots
@@ -58,6 +62,23 @@ enum am_interp {
     AM_INTERP_LEARNED = 3    /* learned per-octant prolongation on one-level-coarser faces, tricubic elsewhere */
 };
 
+/* Tile layouts of fp32 field arrays (the tile_layout member of the field
+ * structs).  No counterpart in the reference, which uses the padded one.
+ *   AM_LAYOUT_PADDED: cell (x, y, z) of the padded tile (0 <= x, y, z < T)
+ *     of slot s at s * T^3 + (z * T + y) * T + x (AMR.h:752, 758).
+ *   AM_LAYOUT_INTERIOR_FIRST: slot s starts at s * am_tile_stride() floats
+ *     (T^3 rounded up to a multiple of 32); within it, first the S^3
+ *     interior cells, ((z - P) * S + (y - P)) * S + (x - P); then the six
+ *     face ghost slabs, face f (0 +x, 1 +y, 2 +z, 3 -x, 4 -y, 5 -z) at
+ *     S^3 + f * P * S^2, each P layers of S^2 cells, layer l counted outward
+ *     from the interior, cell (a, b) at (l * S + b) * S + a with (a, b) the
+ *     interior coordinates of the two other axes in x, y, z order; then the
+ *     edge and corner cells (am_tile_layout_convert gives the exact map).
+ * Interior rows and slab rows are contiguous runs of S floats, so the
+ * pointwise stage kernels stream whole tiles and the equal-depth ghost push
+ * writes whole rows. */
+enum am_tile_layout { AM_LAYOUT_PADDED = 0, AM_LAYOUT_INTERIOR_FIRST = 1 };
+
 const char *am_last_error_string(void);
 int am_version(void);
 
@@ -77,6 +98,23 @@ typedef struct am_geometry {
 int am_geometry_init(am_geometry *g, int32_t size_domain, int32_t padding_domain, int32_t fd_order,
                      float outer_size, float cfl);
 int am_geometry_supported(const am_geometry *g);
+
+/* Floats per slot of a field array in `layout` (enum am_tile_layout): T^3,
+ * or T^3 rounded up to a multiple of 32 for AM_LAYOUT_INTERIOR_FIRST; -1 for
+ * a bad argument.  No counterpart in the reference. */
+int64_t am_tile_stride(const am_geometry *g, int32_t layout);
+
+/* Host only: out[(z * T + y) * T + x] = offset within its slot of padded
+ * cell (x, y, z) in `layout`, for every cell of one tile (T^3 entries).  No
+ * counterpart in the reference. */
+int am_tile_cell_offsets(const am_geometry *g, int32_t layout, uint32_t *out);
+
+/* Copy n_slots tiles of one field from layout `from` to layout `to` (device
+ * arrays, no overlap), every cell of every tile.  No counterpart in the
+ * reference: it moves data between the padded layout of the reference's
+ * interfaces and the interior-first layout of the RK4 drivers. */
+int am_tile_layout_convert(const float *src, float *dst, int32_t n_slots, const am_geometry *g, int32_t from,
+                           int32_t to, void *stream);
 
 /* What a kernel sees of the octree: device copies of AMR's arrays (the
  * reference's AMR::copy_to_gpu, AMR.h:268-275), the slot list to visit, and
@@ -191,6 +229,9 @@ typedef struct am_fd_batch {
      * The words equal those of a field whose ghost layers hold the pushed
      * copies. */
     int32_t direct_faces;
+    /* enum am_tile_layout of every field (AM_LAYOUT_INTERIOR_FIRST takes
+     * direct_faces 0 or 4 only) */
+    int32_t tile_layout;
 } am_fd_batch;
 enum am_word_format { AM_WORD_CF3 = 0, AM_WORD_F32 = 1, AM_WORD_F16 = 2 };
 /* bytes per cell of one word array in `format` (4, 12, 6), -1 if unknown */
@@ -214,6 +255,7 @@ typedef struct am_field_batch {
     float *new_[AM_MAX_VARS];
     float *older[AM_MAX_VARS];
     int32_t n_vars;
+    int32_t tile_layout; /* enum am_tile_layout of every field */
 } am_field_batch;
 int am_ghost_fill_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g,
                           int32_t copy_only_new, int32_t interp, void *stream);
@@ -361,6 +403,7 @@ typedef struct am_bssn_fields {
     int32_t word_format; /* enum am_word_format of d1/d2diag/d2off (non-cf3: S=16, P=3 only) */
     int32_t push_layers; /* ghost layers the push writes, innermost first: 1..P, 0 = all P */
     int32_t push_axes;   /* faces the push writes, by axis: bits 1 x, 2 y, 4 z; 0 = all three */
+    int32_t tile_layout; /* enum am_tile_layout of state, old_ and new_ */
 } am_bssn_fields;
 
 /* One classical RK4 stage on the interior of every listed active tile:
@@ -408,6 +451,7 @@ typedef struct am_swave_fields {
     const uint32_t *d2diag_u; /* (u_xx, u_yy, u_zz) word of the stage input u */
     int32_t push_layers;      /* as am_bssn_fields.push_layers */
     int32_t push_axes;        /* as am_bssn_fields.push_axes */
+    int32_t tile_layout;      /* as am_bssn_fields.tile_layout */
 } am_swave_fields;
 
 int am_swave_rhs_stage(const am_swave_fields *f, int32_t stage, const am_mesh *mesh, const am_geometry *g,

@@ -133,6 +133,7 @@ struct bssn_system {
     static am_bssn_fields fields(const rk4_views &v)
     {
         am_bssn_fields f{};
+        f.tile_layout = v.tile_layout;
         f.word_format = v.word_format;
         f.push_layers = v.push_layers;
         f.push_axes = v.push_axes;

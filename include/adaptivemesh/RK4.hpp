@@ -96,12 +96,14 @@ struct rk4_options {
     int32_t fd_grid_limit = 0;            // persistent FD launch: workgroup cap (0 = one per item)
     int32_t rhs_grid_limit = 0;           // persistent RHS launch: workgroup cap (0 = one per item)
     int32_t word_format = AM_WORD_CF3;    // derivative-word storage (am_word_format)
-    int32_t fd_prefetch = 1;              // FD planes prefetched ahead (1-3, cf3 words)
+    int32_t fd_prefetch = 1;              // FD planes prefetched ahead (1: the interior-first layout's FD)
     int32_t push_layers = 0;              // equal-depth ghost layers pushed (0 = the FD half-width)
     // axes (bits 1 x, 2 y, 4 z) whose equal-depth faces stages 0-2 do not
     // push; the next stage's FD reads them from the neighbours' interiors.
     // Stage 3 pushes every face, so the ghost layers are current after
-    // every fine step whatever the mask.
+    // every fine step whatever the mask.  In the interior-first layout a z
+    // ghost plane is the neighbour's contiguous interior plane, while x and
+    // y ghost columns would be strided reads, so the mask is 0 or 4.
     int32_t direct_faces = 4;
 };
 
@@ -119,6 +121,7 @@ struct rk4_tune_report {
 
 // Device views of the driver's buffers handed to a System's kernels.
 struct rk4_views {
+    int32_t tile_layout;  // am_tile_layout of state, old_ and new_
     const float *state[AM_MAX_VARS];
     const float *old_[AM_MAX_VARS];
     float *new_[AM_MAX_VARS];
@@ -136,6 +139,12 @@ template <class System>
 struct rk4_AMR_data {
     static constexpr int NV = System::NV;
     using options_t = typename System::options_t;
+    // The driver keeps old, new and older in the interior-first tile layout
+    // (am_tile_layout): the stage kernels then stream whole interiors and the
+    // equal-depth push writes whole slab rows.  Everything it exchanges with
+    // its user (set/get_field, checkpoints, images) is in the reference's
+    // padded layout, converted on the device through `conv`.
+    static constexpr int32_t LAYOUT = AM_LAYOUT_INTERIOR_FIRST;
 
     fast_prng rng;
     AMR<rk4_AMR_data> &parent;
@@ -144,6 +153,7 @@ struct rk4_AMR_data {
     options_t sys;
     void *stream = nullptr;
     size_t cells = 0, inner = 0, slots = 0;
+    size_t padded = 0;  // T^3: floats per slot of a padded field
     size_t word_stride = 0;  // bytes of one field's words: slots * inner * am_word_bytes(word_format)
     int n_d2 = 0, n_off = 0;
 
@@ -151,6 +161,7 @@ struct rk4_AMR_data {
     // tiles) or v * slots * inner (acc: interior-only, so every accumulator
     // row is a whole aligned run)
     smart_gpu_buffer<float> old_, new_, acc, older;
+    smart_gpu_buffer<float> conv;  // one padded field, for conversions (allocated on first use)
     smart_gpu_buffer<uint32_t> d1, d2d, d2o, chr;
     smart_gpu_buffer<double> partial;
     uint64_t zone_cycles = 0;  // interior cells advanced by one full step, all depths
@@ -175,7 +186,10 @@ struct rk4_AMR_data {
         : rng(), parent(p), geom(g), opt(o), sys(so)
     {
         const size_t T = (size_t)(g.size_domain + 2 * g.padding_domain), S = (size_t)g.size_domain;
-        cells = T * T * T;
+        padded = T * T * T;
+        const int64_t stride = am_tile_stride(&g, LAYOUT);
+        if (stride <= 0) throw std::runtime_error("no interior-first layout for this geometry");
+        cells = (size_t)stride;
         inner = S * S * S;
         slots = node_slots;
         for (int v = 0; v < NV; v++) {
@@ -209,7 +223,7 @@ struct rk4_AMR_data {
     {
         drop_graph();
         if (stream) am_stream_sync(stream);
-        old_.destroy(); new_.destroy(); acc.destroy(); older.destroy();
+        old_.destroy(); new_.destroy(); acc.destroy(); older.destroy(); conv.destroy();
         d1.destroy(); d2d.destroy(); d2o.destroy(); chr.destroy(); partial.destroy(); seed_table.destroy();
         mlp_stats.destroy();
         timer.destroy();
@@ -243,10 +257,29 @@ struct rk4_AMR_data {
         return std::min(l, P);
     }
 
+    // the padded form of field `f` (device, in driver layout) in `conv`
+    const float *to_padded(const float *f)
+    {
+        if (!conv.created) conv = smart_gpu_buffer<float>(slots * padded);
+        check(am_tile_layout_convert(f, conv.gpu_buffer_ptr, (int32_t)slots, &geom, LAYOUT, AM_LAYOUT_PADDED, stream),
+              "layout conversion");
+        return conv.gpu_buffer_ptr;
+    }
+    // field `f` (device, in driver layout) from the padded host array `h`
+    // of n_slots slots
+    void from_padded_host(float *f, const float *h, size_t n_slots)
+    {
+        if (!conv.created) conv = smart_gpu_buffer<float>(slots * padded);
+        upload(conv.gpu_buffer_ptr, h, n_slots * padded * sizeof(float), stream);
+        check(am_tile_layout_convert(conv.gpu_buffer_ptr, f, (int32_t)n_slots, &geom, AM_LAYOUT_PADDED, LAYOUT, stream),
+              "layout conversion");
+    }
+
     am_field_batch batch(bool with_older)
     {
         am_field_batch b{};
         b.n_vars = NV;
+        b.tile_layout = LAYOUT;
         for (int v = 0; v < NV; v++) {
             b.old_[v] = field(old_, v);
             b.new_[v] = field(new_, v);
@@ -257,6 +290,7 @@ struct rk4_AMR_data {
     rk4_views views(bool stage_input_is_old)
     {
         rk4_views f{};
+        f.tile_layout = LAYOUT;
         f.word_format = opt.word_format;
         f.push_layers = push_layers();
         f.push_axes = 7;
@@ -290,26 +324,27 @@ struct rk4_AMR_data {
     uint32_t ckpt_vars() const { return (uint32_t)NV; }
     uint32_t ckpt_flags() const { return (uint32_t)opt.interp | ((uint32_t)opt.word_format << 4); }
     uint32_t ckpt_sections() const { return older.created ? 3u : 2u; }
+    // (the file holds padded tiles, whatever the driver's layout)
     void ckpt_save(checkpoint_out &o)
     {
-        const size_t fb = parent.curr_used_slots() * cells * sizeof(float);
+        const size_t fb = parent.curr_used_slots() * padded * sizeof(float);
         smart_gpu_buffer<float> *bufs[3] = {&old_, &new_, &older};
         const char *names[3] = {"old", "new", "older"};
         for (uint32_t k = 0; k < ckpt_sections(); k++) {
-            std::vector<std::pair<const void *, size_t>> parts;
-            for (int v = 0; v < NV; v++) parts.push_back({field(*bufs[k], v), fb});
-            o.device(names[k], parts, stream);
+            o.begin(names[k], (uint64_t)NV * fb);
+            for (int v = 0; v < NV; v++) o.device_part(to_padded(field(*bufs[k], v)), fb, stream);
         }
     }
     void ckpt_load(checkpoint_in &in, uint32_t used, bool apply)
     {
-        const size_t fb = used * cells * sizeof(float);
+        const size_t fb = used * padded * sizeof(float);
         smart_gpu_buffer<float> *bufs[3] = {&old_, &new_, &older};
         const char *names[3] = {"old", "new", "older"};
         for (uint32_t k = 0; k < ckpt_sections(); k++) {
             const char *src = in.take(names[k], NV * fb);
             if (apply)
-                for (int v = 0; v < NV; v++) upload(field(*bufs[k], v), src + v * fb, fb, stream);
+                for (int v = 0; v < NV; v++)
+                    from_padded_host(field(*bufs[k], v), reinterpret_cast<const float *>(src + v * fb), used);
         }
         if (apply) drop_graph();
     }
@@ -329,6 +364,7 @@ struct rk4_AMR_data {
     void derivatives(smart_gpu_buffer<float> &src, am_mesh m, int direct_axes = 0)
     {
         am_fd_batch b{};
+        b.tile_layout = LAYOUT;
         b.direct_faces = direct_axes;
         b.z_block = opt.z_block;
         b.word_format = opt.word_format;
@@ -499,8 +535,8 @@ struct rk4_AMR_data {
     // Time the launch shapes on the live run and keep the fastest: whole
     // coarse steps (timestep(), eager) from a snapshot of the state, one
     // parameter at a time -- FD planes per workgroup, the direct-face mask
-    // (every mask gives the same fields), FD prefetch depth, then the FD and
-    // RHS workgroup caps.  Each trial starts from the same snapshot; the
+    // (every mask gives the same fields), then the FD and RHS workgroup
+    // caps.  Each trial starts from the same snapshot; the
     // snapshot, the host seed, the counter and the zone-cycle count are
     // restored at the end, so the run continues as if nothing happened.
     void autotune(int reps, rk4_tune_report *rep)
@@ -566,8 +602,7 @@ struct rk4_AMR_data {
         opt.fd_prefetch = 1;
         trial();  // warm-up
         sweep(0, opt.z_block, {S, S / 2, S / 4});
-        sweep(4, opt.direct_faces, {4, 0, 7});
-        if (opt.word_format == AM_WORD_CF3) sweep(3, opt.fd_prefetch, {1, 2, 3});
+        sweep(4, opt.direct_faces, {4, 0});
         sweep(1, opt.fd_grid_limit, {0, 8 * cu});
         sweep(2, opt.rhs_grid_limit, {0, 2 * cu, 4 * cu});
         copy_all(false);

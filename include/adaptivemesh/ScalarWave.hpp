@@ -28,6 +28,7 @@ struct swave_system {
     static am_swave_fields fields(const rk4_views &v)
     {
         am_swave_fields f{};
+        f.tile_layout = v.tile_layout;
         f.push_layers = v.push_layers;
         f.push_axes = v.push_axes;
         for (int i = 0; i < NV; i++) {

@@ -75,6 +75,13 @@ class checkpoint_out {
             raw(staging.data(), p.second);
         }
     }
+    // part of a section opened with begin(), read back from the device
+    void device_part(const void *d, size_t bytes, void *stream)
+    {
+        staging.resize(bytes);
+        check(am_memcpy_d2h(staging.data(), d, bytes, stream), "checkpoint read-back");
+        raw(staging.data(), bytes);
+    }
     void commit()
     {
         const bool ok = fflush(f) == 0;

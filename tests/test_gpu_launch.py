@@ -108,8 +108,8 @@ def test_graph_replay_equals_eager_swave(am, orc, torch_cuda):
         assert sims[0].rng_state == sims[1].rng_state
 
 
-@pytest.mark.parametrize("launch", [(8, 0, 0, 1), (4, 64, 32, 2), (16, 1, 1, 3), (0, 512, 7, 2), (16, 0, 0, 3),
-                                    (4, 0, 0, 3)])
+@pytest.mark.parametrize("launch", [(8, 0, 0, 1), (4, 64, 32, 1), (16, 1, 1, 1), (0, 512, 7, 1), (16, 0, 0, 1),
+                                    (4, 0, 0, 1)])
 def test_launch_shapes_bit_identical(am, orc, torch_cuda, launch):
     print(f"launch shape {launch}", flush=True)
     (ref, alt), n = bssn_pair(am, orc, 0, True)
@@ -129,10 +129,10 @@ def test_autotune_restores_state(am, orc, torch_cuda):
     tuned.timestep()
     rep = tuned.autotune(reps=2)
     print(f"autotune: {rep}", flush=True)
-    assert rep["z_block"] in (4, 8, 16) and rep["fd_prefetch"] in (1, 2, 3) and len(rep["trials"]) == 14
+    assert rep["z_block"] in (4, 8, 16) and rep["fd_prefetch"] == 1 and len(rep["trials"]) == 10
     # the mask it keeps need not be the default: the steps below still equal the reference run's
-    assert rep["direct_faces"] in (0, 4, 7) and tuned.direct_faces == rep["direct_faces"]
-    assert [t[1] for t in rep["trials"] if t[0] == "direct_faces"] == [4, 0, 7]
+    assert rep["direct_faces"] in (0, 4) and tuned.direct_faces == rep["direct_faces"]
+    assert [t[1] for t in rep["trials"] if t[0] == "direct_faces"] == [4, 0]
     assert tuned.get_launch() == (rep["z_block"], rep["fd_grid_limit"], rep["rhs_grid_limit"], rep["fd_prefetch"])
     tuned.sync()
     assert same_state(ref, tuned, 24, (0, 1, 2, 3)) is None

@@ -58,13 +58,13 @@ def run_gpu(am, torch, s, threshold):
     sim, g = s["sim"], s["g"]
     for v in range(24):
         sim.set_field(1, v, s["bufs"][1][v])  # the same input every call
-    stride = sim.max_slots * T ** 3 * 4
     fb = am.FieldBatch()
     fb.n_vars = 24
+    fb.tile_layout = am.AM_LAYOUT_INTERIOR_FIRST  # the driver's state layout
     for v in range(24):
-        fb.old_[v] = sim.device_ptr(0) + v * stride
-        fb.new_[v] = sim.device_ptr(1) + v * stride
-        fb.older[v] = sim.device_ptr(3) + v * stride
+        fb.old_[v] = sim.field_ptr(0, v)
+        fb.new_[v] = sim.field_ptr(1, v)
+        fb.older[v] = sim.field_ptr(3, v)
     m = s["tree"].mesh(5)
     m.substep = 1
     prm = torch.tensor(s["params"], device="cuda")

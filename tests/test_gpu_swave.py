@@ -59,7 +59,7 @@ def words(am, sim, n):
     return out
 
 
-@pytest.mark.parametrize("direct", [4, 0, 7])
+@pytest.mark.parametrize("direct", [4, 0])
 def test_swave_stages_match_oracle(am, orc, torch_cuda, direct):
     print(f"swave stages uniform1 periodic direct={direct}", flush=True)
     sim, e, n, g = make_sim(am, orc)
