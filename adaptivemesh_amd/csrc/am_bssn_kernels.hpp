@@ -55,15 +55,57 @@ __device__ __forceinline__ float sym_dot(const float *a, const float *b)
     return s;
 }
 
-// RHS (and optionally the constraints) at one interior cell.
+// The derivative words of one cell.  compressed_float3 words (4 bytes) are
+// all loaded by load(), up front, so that a wave has its 46 word loads in
+// flight together instead of one memory round trip per word decoded; the
+// study formats (12 and 6 bytes per word) are read where they are used.
+template <int FMT>
+struct CellWords {
+    uint32_t d1[NV], dd[NH], doff[NH];
+    __device__ __forceinline__ void load(const BssnArgsDev &a, size_t w)
+    {
+        if constexpr (FMT == WORD_CF3) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) d1[v] = static_cast<const uint32_t *>(a.d1[v])[w];
+#pragma unroll
+            for (int h = 0; h < NH; h++) {
+                dd[h] = static_cast<const uint32_t *>(a.d2d[h])[w];
+                doff[h] = static_cast<const uint32_t *>(a.d2o[h])[w];
+            }
+        }
+    }
+    __device__ __forceinline__ float3 grad(const BssnArgsDev &a, size_t w, int v) const
+    {
+        if constexpr (FMT == WORD_CF3) return cf3_unpack(d1[v]);
+        else return word_get<FMT>(a.d1[v], w);
+    }
+    __device__ __forceinline__ float3 diag(const BssnArgsDev &a, size_t w, int h) const
+    {
+        if constexpr (FMT == WORD_CF3) return cf3_unpack(dd[h]);
+        else return word_get<FMT>(a.d2d[h], w);
+    }
+    __device__ __forceinline__ float3 off(const BssnArgsDev &a, size_t w, int h) const
+    {
+        if constexpr (FMT == WORD_CF3) return cf3_unpack(doff[h]);
+        else return word_get<FMT>(a.d2o[h], w);
+    }
+};
+
+// RHS (and optionally the constraints) at one interior cell: u holds the
+// 24 state values of the cell, cw its derivative words.
 template <int FMT, bool WANT_RHS, bool WANT_CONS>
-__device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, size_t r, size_t w, const BssnParDev &par, float *rhs,
-                                          float *Hc, float *Mc)
+__device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, const float (&u)[NV], const CellWords<FMT> &cw, size_t w,
+                                          const BssnParDev &par, float *rhs, float *Hc, float *Mc)
 {
 #pragma clang fp contract(fast)
-    float u[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) u[v] = a.state[v][r];
+    auto d1v = [&](int v, float *o) {
+        const float3 t = cw.grad(a, w, v);
+        o[0] = t.x; o[1] = t.y; o[2] = t.z;
+    };
+    auto hsv = [&](int hh, float *h) {
+        const float3 d = cw.diag(a, w, hh), o = cw.off(a, w, hh);
+        h[0] = d.x; h[1] = o.x; h[2] = o.y; h[3] = d.y; h[4] = o.z; h[5] = d.z;
+    };
     const float chi = u[0], ichi = 1.f / fmaxf(chi, par.chi_floor);
     const float K = u[7], alpha = u[17];
     const float *g = u + 1, *At = u + 8, *Gam = u + 14, *beta = u + 18, *B = u + 21;
@@ -81,13 +123,13 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, size_t r, size_t
         gu[5] = (g[0] * g[3] - g[1] * g[1]) * idet;
     }
     float dchi[3], dK[3], dal[3], db[3][3], dg[6][3];
-    dec3<FMT>(a.d1[0], w, dchi);
-    dec3<FMT>(a.d1[7], w, dK);
-    dec3<FMT>(a.d1[17], w, dal);
+    d1v(0, dchi);
+    d1v(7, dK);
+    d1v(17, dal);
 #pragma unroll
-    for (int i = 0; i < 3; i++) dec3<FMT>(a.d1[18 + i], w, db[i]);  // db[i][k] = d_k beta^i
+    for (int i = 0; i < 3; i++) d1v(18 + i, db[i]);  // db[i][k] = d_k beta^i
 #pragma unroll
-    for (int c = 0; c < 6; c++) dec3<FMT>(a.d1[1 + c], w, dg[c]);
+    for (int c = 0; c < 6; c++) d1v(1 + c, dg[c]);
     const float divb = db[0][0] + db[1][1] + db[2][2];
     auto adv = [&](const float *d) { return beta[0] * d[0] + beta[1] * d[1] + beta[2] * d[2]; };
 
@@ -135,10 +177,10 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, size_t r, size_t
     float DDchi[6], DDalt[6];
     {
         float h[6];
-        hess6<FMT>(a.d2d[0], a.d2o[0], w, h);
+        hsv(0, h);
 #pragma unroll
         for (int c = 0; c < 6; c++) DDchi[c] = h[c] - (G[0][c] * dchi[0] + G[1][c] * dchi[1] + G[2][c] * dchi[2]);
-        hess6<FMT>(a.d2d[7], a.d2o[7], w, h);
+        hsv(7, h);
 #pragma unroll
         for (int c = 0; c < 6; c++) DDalt[c] = h[c] - (G[0][c] * dal[0] + G[1][c] * dal[1] + G[2][c] * dal[2]);
     }
@@ -149,7 +191,7 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, size_t r, size_t
     {
         float dGam[3][3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) dec3<FMT>(a.d1[14 + i], w, dGam[i]);
+        for (int i = 0; i < 3; i++) d1v(14 + i, dGam[i]);
         // E[k][i][m] = gt^ml Gt^k_li, F[k][m][j] = gt^ml Gt_klj
         float E[3][3][3], F[3][3][3];
 #pragma unroll
@@ -169,7 +211,7 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, size_t r, size_t
             for (int j = i; j < 3; j++) {
                 const int c = SI[i][j];
                 float h[6];
-                hess6<FMT>(a.d2d[1 + c], a.d2o[1 + c], w, h);
+                hsv(1 + c, h);
                 float rr = -.5f * sym_dot(gu, h);
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
@@ -217,7 +259,7 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, size_t r, size_t
             for (int j = i; j < 3; j++) {
                 const int c = SI[i][j];
                 float dA[3];
-                dec3<FMT>(a.d1[8 + c], w, dA);
+                d1v(8 + c, dA);
                 float s = adv(dA) - (2.f / 3.f) * At[c] * divb + chi * (X[c] - g[c] * trX * (1.f / 3.f)) +
                           alpha * K * At[c];
 #pragma unroll
@@ -230,11 +272,11 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, size_t r, size_t
         // Gt^i and B^i
         float ddb[3][6];
 #pragma unroll
-        for (int i = 0; i < 3; i++) hess6<FMT>(a.d2d[8 + i], a.d2o[8 + i], w, ddb[i]);
+        for (int i = 0; i < 3; i++) hsv(8 + i, ddb[i]);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             float dGi[3], dBi[3];
-            dec3<FMT>(a.d1[14 + i], w, dGi);
+            d1v(14 + i, dGi);
             float s = adv(dGi) + (2.f / 3.f) * Gam[i] * divb + sym_dot(gu, ddb[i]);
 #pragma unroll
             for (int k = 0; k < 3; k++) s -= Gam[k] * db[i][k];
@@ -247,7 +289,7 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, size_t r, size_t
                 s -= 2.f * alpha * (1.5f * ichi * Atu[SI[i][j]] * dchi[j] + (2.f / 3.f) * gu[SI[i][j]] * dK[j]);
             }
             rhs[14 + i] = s;
-            dec3<FMT>(a.d1[21 + i], w, dBi);
+            d1v(21 + i, dBi);
             const float bdb[3] = {dBi[0], dBi[1], dBi[2]};
             rhs[21 + i] = (par.advect_shift ? adv(bdb) - adv(dGi) : 0.f) + s - par.eta * B[i];
             float dbi[3] = {db[i][0], db[i][1], db[i][2]};
@@ -259,7 +301,7 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, size_t r, size_t
         *Hc = chi * sym_dot(gu, R) + (2.f / 3.f) * K * K - AA;
         float dA[6][3];
 #pragma unroll
-        for (int c = 0; c < 6; c++) dec3<FMT>(a.d1[8 + c], w, dA[c]);
+        for (int c = 0; c < 6; c++) d1v(8 + c, dA[c]);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             float m = -(2.f / 3.f) * dK[i];
@@ -291,6 +333,7 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
 #pragma clang fp contract(fast)
     constexpr int S3 = S * S * S, BPT = (S3 + 255) / 256;
     constexpr size_t TS = tile_stride<S, P, L>();
+    __shared__ float xbuf[PUSH && L == 1 && S * S == 256 ? 2 * P * NV * S : 1];
     // persistent launches loop over work items (a separate instantiation:
     // the loop keeps every kernel-argument pointer live across iterations)
     const int item_end = PERSIST ? m.n_items : (int)blockIdx.x + 1;
@@ -304,10 +347,35 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
         const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
         const size_t r = (size_t)node * TS + tile_interior<S, P, L>(x, y, z);
         const size_t w = (size_t)node * S3 + cell;
+        // Loads in two batches, each issued whole before it is waited on:
+        // the state and the 46 words before the RHS, y_n and acc of all 24
+        // fields after it and before any store.  Left to itself the
+        // compiler waits on each word where it is decoded, and it cannot
+        // move a field's y_n / acc load above the previous field's stores
+        // (they might alias), so every field costs a memory round trip.
+        // Loading the second batch with the first instead would hold 48
+        // more registers through the RHS: one wave per SIMD, and slower.
+        float u[NV], yn[NV], ac[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) u[v] = a.state[v][r];
+        CellWords<FMT> cw;
+        cw.load(a, w);
         float k[NV];
-        bssn_cell<FMT, true, false>(a, r, w, par, k, nullptr, nullptr);
-        rk4_combine<NV, STAGE>(a.state, a.old_, a.acc, a.new_, k, r, w, sc.dt[depth]);
-        if (PUSH) rk4_push<NV, S, P, L>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes);
+        bssn_cell<FMT, true, false>(a, u, cw, w, par, k, nullptr, nullptr);
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            yn[v] = STAGE == 0 ? u[v] : a.old_[v][r];
+            ac[v] = STAGE == 0 ? 0.f : a.acc[v][w];
+        }
+        rk4_combine_in<NV, STAGE>(a.acc, a.new_, k, yn, ac, r, w, sc.dt[depth]);
+        if constexpr (PUSH && L == 1 && S * S == 256) {
+            // y and z faces are whole slab rows per wave already; the x
+            // faces' columns are transposed through LDS
+            rk4_push<NV, S, P, L>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes & 6);
+            rk4_push_x_rows<NV, S, P>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes, xbuf);
+        } else if (PUSH) {
+            rk4_push<NV, S, P, L>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes);
+        }
     }
 }
 
@@ -324,8 +392,12 @@ __global__ void __launch_bounds__(256) k_bssn_constraints(BssnArgsDev a, TileArg
         const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
         const size_t r = (size_t)node * tile_stride<S, P, L>() + tile_interior<S, P, L>(x, y, z);
         const size_t w = (size_t)node * S3 + cell;
-        float H, M[3];
-        bssn_cell<FMT, false, true>(a, r, w, par, nullptr, &H, M);
+        float H, M[3], u[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) u[v] = a.state[v][r];
+        CellWords<FMT> cw;
+        cw.load(a, w);
+        bssn_cell<FMT, false, true>(a, u, cw, w, par, nullptr, &H, M);
         h2 = (double)H * (double)H;
         m2 = (double)M[0] * M[0] + (double)M[1] * M[1] + (double)M[2] * M[2];
         cnt = 1.0;

@@ -48,6 +48,30 @@ __device__ __forceinline__ void rk4_combine(const float *const *state, const flo
 // the cell lands in the target's slab of the opposite face, layer = its
 // distance from the shared face, so the cells of one row of this tile go to
 // one contiguous slab row of the target.
+// The same combine with the stage's inputs already loaded: yn[v] = y_n and
+// ac[v] = acc (stages 1-3) of the cell.
+template <int NV, int STAGE>
+__device__ __forceinline__ void rk4_combine_in(float *const *acc, float *const *new_, float (&k)[NV],
+                                               const float (&yn)[NV], const float (&ac)[NV], size_t r, size_t w,
+                                               float dt)
+{
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        float nv;
+        if (STAGE == 0) {
+            acc[v][w] = k[v];
+            nv = yn[v] + (.5f * dt) * k[v];
+        } else if (STAGE == 1 || STAGE == 2) {
+            acc[v][w] = ac[v] + 2.f * k[v];
+            nv = yn[v] + (STAGE == 1 ? .5f * dt : dt) * k[v];
+        } else {
+            nv = yn[v] + (dt * (1.f / 6.f)) * (ac[v] + k[v]);
+        }
+        new_[v][r] = nv;
+        k[v] = nv;  // the stage output, for the push
+    }
+}
+
 template <int NV, int S, int P, int L = 0>
 __device__ __forceinline__ void rk4_push(float *const *new_, const float (&y)[NV], const uint64_t *boundaries, int node,
                                          int x, int y_, int z, int layers, int axes)
@@ -76,6 +100,52 @@ __device__ __forceinline__ void rk4_push(float *const *new_, const float (&y)[NV
 #pragma unroll
         for (int v = 0; v < NV; v++) new_[v][dst] = y[v];
     }
+}
+
+// The x faces of the push for a workgroup that holds one whole z-plane of an
+// interior-first tile (S * S == 256 threads, thread = x + S * y): the cells
+// of an x face are a column of the plane, one lane in 16, while the target's
+// slab row (layer l, row b = z) holds them as S consecutive floats.  They go
+// through LDS (xbuf: 2 * P * NV * S floats) and leave as whole rows, one
+// field per wave: 64 lanes = all the rows of one field at two layers.  Every
+// thread of the workgroup must call this (it holds two barriers).
+template <int NV, int S, int P>
+__device__ __forceinline__ void rk4_push_x_rows(float *const *new_, const float (&y)[NV], const uint64_t *boundaries,
+                                                int node, int x, int y_, int z, int layers, int axes, float *xbuf)
+{
+    static_assert(S * S == 256, "one z-plane per 256-thread workgroup");
+    constexpr size_t TS = tile_stride<S, P, 1>();
+    int tgt[2] = {-1, -1};  // side 0: the +x face, side 1: the -x face (uniform per workgroup)
+    if (axes & 1) {
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++) {
+            const uint32_t nd = (uint32_t)boundaries[(size_t)node * 6 + (sd ? 3 : 0)];
+            const int t = bnd_target(nd);
+            if (t >= 0 && bnd_depth(nd) > 0 && bnd_ddiff(nd) == 0) tgt[sd] = t;
+        }
+    }
+    if (tgt[0] < 0 && tgt[1] < 0) return;
+    const int sd = x >= S - layers ? 0 : (x < layers ? 1 : -1);
+    if (sd >= 0 && tgt[sd] >= 0) {
+        const int l = sd == 0 ? S - 1 - x : x;  // layer of the target's slab: distance from the shared face
+#pragma unroll
+        for (int v = 0; v < NV; v++) xbuf[((sd * P + l) * NV + v) * S + y_] = y[v];
+    }
+    __syncthreads();
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    const int n_row = 2 * layers * S;  // (side, layer, row) of one field
+    for (int v0 = 0; v0 < NV; v0 += 4) {
+        const int v = __builtin_amdgcn_readfirstlane(v0 + wave);
+        if (v >= NV) break;
+        float *dst = new_[v];
+        for (int i = lane; i < n_row; i += 64) {
+            const int s2 = i / (layers * S), l = (i / S) % layers, yy = i % S;
+            if (tgt[s2] < 0) continue;
+            dst[(size_t)tgt[s2] * TS + slab_cell<S, P>(s2 == 0 ? 3u : 0u, (uint32_t)l, (uint32_t)yy, (uint32_t)z)] =
+                xbuf[((s2 * P + l) * NV + v) * S + yy];
+        }
+    }
+    __syncthreads();  // xbuf is refilled by the workgroup's next item
 }
 
 }  // namespace am

@@ -41,37 +41,64 @@ NOISE = 1e-6
 # algorithmic bytes per interior cell (each distinct element read once and
 # written once per launch)
 WORD = 4
-PUSH_LAYERS = FD_ORDER // 2  # the driver pushes the ghost layers the FD stencils read
-PUSH = 6 * PUSH_LAYERS * S * S * 24 * 4 / S ** 3  # equal-depth face ghosts written by the stage kernel
-# HBM bytes per depth-3 k_bssn_stage launch (512 tiles), mean of the four
-# stages with the driver's default direct z faces, from rocprofv3 --pmc
-# FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0, both scaled by a 1 GiB copy):
-# profiles/bssn_rk4_512tiles_hbm_counters.md.  The algorithmic bytes of the
-# same launches average 1.355e9.
-RHS_TRAFFIC_PER_LAUNCH = 1.889e9
-RHS_ALGORITHMIC_PER_LAUNCH = 1.355e9
+Q = FD_ORDER // 2  # stencil half-width: the ghost layers the FD reads and the push writes
+# HBM bytes per depth-3 k_bssn_stage launch from rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE (profiles/bssn_rk4_512tiles_hbm_counters.md); None until read on
+# the current kernels
+RHS_TRAFFIC_PER_LAUNCH = None
+RHS_ALGORITHMIC_PER_LAUNCH = None
 TRAFFIC_DIRECT_FACES = 4  # the mask the counters were read with
 
 
-def bytes_model(direct_faces):
-    """Algorithmic bytes per interior cell and launch class.  direct_faces is
-    the driver's axis mask (bits 1 x, 2 y, 4 z) of the equal-depth faces that
-    stages 0-2 do not push (the next stage's FD reads them from the
-    neighbours: the same number of distinct elements as the pushed copies)."""
-    push_early = PUSH * (3 - bin(direct_faces & 7).count("1")) / 3
+def face_stats(export, S, depth_filter=None):
+    """Mean, over the advanced tiles weighted by how often each advances
+    (2^depth), of the faces per tile whose neighbour has the same depth
+    (pushed) and of those whose neighbour is coarser (interpolated)."""
+    pos, bnd = export["positions"], export["boundaries"]
+    eq = co = wsum = 0.0
+    for s, p_ in enumerate(pos):
+        p_ = int(p_)
+        if p_ == 0xFFFFFFFF:
+            continue
+        d = p_ >> 27
+        if depth_filter is not None and d != depth_filter:
+            continue
+        w = 2.0 ** d
+        wsum += w
+        for f in range(6):
+            nd = int(bnd[s * 6 + f]) & 0xFFFFFFFF
+            tgt = nd & 0xFFFFFF
+            if tgt & 0x800000 or nd >> 28 in (0, 15):
+                continue
+            if (nd >> 24) & 15:
+                co += w
+            else:
+                eq += w
+    return eq / max(wsum, 1), co / max(wsum, 1)
+
+
+def bytes_model(direct_faces, eq_faces, coarse_faces):
+    """Algorithmic bytes per interior cell and launch class.  eq_faces /
+    coarse_faces: mean faces per advanced tile with an equal-depth / a
+    coarser neighbour (face_stats).  direct_faces is the driver's axis mask
+    (0 or 4) of the equal-depth faces that stages 0-2 do not push (the next
+    stage's FD reads those ghost planes from the neighbours: the same
+    distinct elements as the pushed copies)."""
+    push = eq_faces * Q * S * S * 24 * 4 / S ** 3  # equal-depth ghost layers written by the stage kernel
+    push_early = push * (3 - bin(direct_faces & 7).count("1")) / 3
     return {
-        # per stage, 24 fields: padded reads (S^3 + 6 P S^2) * 4 per field; 24 gradient + 11 diag + 11 off-diag
-        # words
-        "fd": 24 * (S ** 3 + 6 * P * S * S) * 4 / S ** 3 + 46 * WORD,
+        # per stage, 24 fields: the interior and Q ghost layers per face, (S^3 + 6 Q S^2) * 4 per field; 24
+        # gradient + 11 diag + 11 off-diag words
+        "fd": 24 * (S ** 3 + 6 * Q * S * S) * 4 / S ** 3 + 46 * WORD,
         # per RK4 step, the four stages summed: state 96 and 46 words 184 read in every stage; y_n 96 and acc 96
         # read in stages 1-3; new 96 written in every stage, acc 96 written in stages 0-2; the pushed face ghosts
-        # (stage 3 pushes every face)
+        # (stage 3 pushes every equal-depth face)
         "rhs_stage": (96 + 184 + 192 + push_early) + (96 + 184 + 96 + 96 + 192 + push_early) * 2
-                     + (96 + 184 + 96 + 96 + 96 + PUSH),
-        # per stage: 6 P S^2 face cells per tile, one source read and one write, 24 fields (coarse faces only)
-        "ghost": 24 * 6 * P * S * S * 8 / S ** 3,
-        # per step: interior read of new + interior write of old per field (no tile of this mesh is the coarse
-        # source of a finer face, so the copy-back leaves ghost cells alone)
+                     + (96 + 184 + 96 + 96 + 96 + push),
+        # per stage: P S^2 ghost cells per coarse face, one source read and one write, 24 fields
+        "ghost": coarse_faces * 24 * P * S * S * 8 / S ** 3,
+        # per step: interior read of new + interior write of old per field (tiles that a finer face reads
+        # copy their ghost cells too; not counted)
         "copy_back": 24 * 2 * 4,
     }
 
@@ -204,8 +231,8 @@ def main():
     ap.add_argument("--z-block", type=int, default=0, help="fixed FD z-block (skips the autotuner)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="bssn512")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--direct-faces", type=int, default=None, choices=range(8),
-                    help="axis mask (1 x, 2 y, 4 z) of the equal-depth faces RK4 stages 0-2 leave to the next FD "
+    ap.add_argument("--direct-faces", type=int, default=None, choices=(0, 4),
+                    help="axis mask (4: z) of the equal-depth faces RK4 stages 0-2 leave to the next FD "
                          "(default: the autotuner's choice)")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a recorded step")
     ap.add_argument("--no-learned", action="store_true",
@@ -234,7 +261,7 @@ def main():
         g = am.geometry(S, P, FD_ORDER, OUTER, CFL)
         n_slots = 1 + 8 + 64 + 512
         sim = am.BssnSim(g, n_slots, am.bssn_options(periodic=True))
-        assert sim.push_layers == PUSH_LAYERS, "the byte model assumes the driver's default push"
+        assert sim.push_layers == Q, "the byte model assumes the driver's default push"
         assert build_mesh(sim) == n_slots
         fields = initial_data(n_slots, T)
     else:
@@ -275,8 +302,9 @@ def main():
     value = zones / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     classes = WORKLOAD_CLASSES if args.workload == "bssn512" else WORKLOAD_CLASSES + ("ghost",)
-    BYTES_ = bytes_model(sim.direct_faces)
-    bytes_per_zone = sum(BYTES_[k] * STAGES_PER_STEP[k] for k in WORKLOAD_CLASSES)
+    eq_faces, coarse_faces = face_stats(sim.export(), S)
+    BYTES_ = bytes_model(sim.direct_faces, eq_faces, coarse_faces)
+    bytes_per_zone = sum(BYTES_[k] * STAGES_PER_STEP[k] for k in classes)
     whole_gbs = zones * bytes_per_zone / elapsed / 1e9
 
     roofline = None
@@ -296,7 +324,7 @@ def main():
             if cnt == 0:
                 continue
             kernels[k] = {"ms_total": round(ms, 4), "launches": cnt, "avg_us": round(1e3 * ms / cnt, 2)}
-            if k in WORKLOAD_CLASSES:
+            if k in classes:
                 total_bytes = zp * BYTES_[k] * STAGES_PER_STEP[k]
                 kernels[k].update(GBps=round(total_bytes / (ms * 1e-3) / 1e9, 1),
                                   bytes_per_zone_cycle=round(BYTES_[k] * STAGES_PER_STEP[k], 1))
@@ -306,12 +334,16 @@ def main():
                     "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
                     "traffic": RHS_TRAFFIC_PER_LAUNCH if dom == "rhs_stage" and args.workload == "bssn512"
                     and sim.direct_faces == TRAFFIC_DIRECT_FACES else None}
+        if args.workload != "bssn512":
+            roofline["note"] = (f"byte model from this mesh's faces: {eq_faces:.2f} equal-depth and "
+                                f"{coarse_faces:.2f} coarser neighbours per advanced tile")
         if roofline["traffic"]:
             roofline["traffic_note"] = ("HBM bytes per depth-3 launch from rocprofv3 FETCH_SIZE/WRITE_SIZE; "
                                         f"{RHS_TRAFFIC_PER_LAUNCH / RHS_ALGORITHMIC_PER_LAUNCH:.2f}x the algorithmic "
                                         "bytes of the same launches")
 
     cons = sim.constraints()
+    finite = all(np.isfinite(v) for v in cons.values())
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
@@ -331,6 +363,7 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "constraints_after": {k: float(f"{v:.6g}") for k, v in cons.items()},
+        "valid": finite,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline and args.workload == "bssn512":
@@ -364,6 +397,10 @@ def main():
                                  "unseen_tricubic": rep["unseen_mse_tricubic"]},
         }
     print(json.dumps(result), flush=True)
+    if not finite:
+        print("bench.py: the constraint norms after the timed steps are not finite; the run is invalid",
+              file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":
