@@ -68,7 +68,8 @@ class FdBatch(C.Structure):
                 ("seed_d", C.c_int32 * AM_MAX_VARS), ("seed_m", C.c_int32 * AM_MAX_VARS),
                 ("n_vars", C.c_int32), ("z_block", C.c_int32), ("seed_table", C.c_void_p),
                 ("word_format", C.c_int32), ("prefetch_planes", C.c_int32),
-                ("direct_faces", C.c_int32), ("tile_layout", C.c_int32)]
+                ("direct_faces", C.c_int32), ("tile_layout", C.c_int32), ("adv", C.c_void_p * AM_MAX_VARS),
+                ("shift", C.c_void_p * 3), ("advect_mask", C.c_uint32), ("ko_sigma", C.c_float)]
 
 
 class FieldBatch(C.Structure):
@@ -85,7 +86,7 @@ class BssnFields(C.Structure):
     _fields_ = [("state", C.c_void_p * 24), ("old_", C.c_void_p * 24), ("new_", C.c_void_p * 24),
                 ("acc", C.c_void_p * 24), ("d1", C.c_void_p * 24), ("d2diag", C.c_void_p * 11),
                 ("d2off", C.c_void_p * 11), ("word_format", C.c_int32), ("push_layers", C.c_int32),
-                ("push_axes", C.c_int32), ("tile_layout", C.c_int32)]
+                ("push_axes", C.c_int32), ("tile_layout", C.c_int32), ("adv", C.c_void_p * 24)]
 
 
 class BssnSimOptions(C.Structure):
@@ -219,6 +220,8 @@ _SIGS = {
                                       C.POINTER(Geometry), _P, _P]),
     "am_bssn_constraint_blocks": (C.c_int64, [C.POINTER(Geometry), C.c_int32]),
     "am_tile_stride": (C.c_int64, [C.POINTER(Geometry), C.c_int32]),
+    "am_bssn_sim_set_dissipation": (C.c_int, [_P, C.c_float, C.c_int32]),
+    "am_bssn_sim_get_dissipation": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
     "am_tile_cell_offsets": (C.c_int, [C.POINTER(Geometry), C.c_int32, _P]),
     "am_tile_layout_convert": (C.c_int, [_P, _P, C.c_int32, C.POINTER(Geometry), C.c_int32, C.c_int32, _P]),
     "am_bssn_constraints_noalloc": (C.c_int, [C.POINTER(BssnFields), C.POINTER(BssnParams), C.POINTER(Mesh),
@@ -601,7 +604,9 @@ class _Rk4Sim:
     def device_ptr(self, which: int) -> int:
         """Raw device pointer of a driver buffer: 0 old, 1 new, 2 acc, 3
         older (fields in the interior-first layout, field v at v *
-        max_slots * tile_stride floats; acc interior-only), 4-7 the words."""
+        max_slots * tile_stride floats; acc interior-only), 4-7 the words, 8
+        the upwinded advection + dissipation (interior-only, after
+        set_dissipation)."""
         return self._f("device_ptr")(self.h, which)
 
     @property
@@ -745,6 +750,17 @@ class BssnSim(_Rk4Sim):
         out = np.zeros(4)
         check(lib().am_bssn_sim_constraints(self.h, out.ctypes.data), "constraints")
         return dict(H_l2=out[0], M_l2=out[1], H_max=out[2], cells=out[3])
+
+    def set_dissipation(self, ko_sigma: float, upwind: bool = True):
+        """Upwinded advection with Kreiss-Oliger dissipation of strength
+        ko_sigma (upwind=False, ko_sigma=0: the centred scheme)."""
+        check(lib().am_bssn_sim_set_dissipation(self.h, float(ko_sigma), 1 if upwind else 0), "set_dissipation")
+
+    @property
+    def dissipation(self):
+        sg, up = C.c_float(), C.c_int32()
+        check(lib().am_bssn_sim_get_dissipation(self.h, C.byref(sg), C.byref(up)), "get_dissipation")
+        return sg.value, bool(up.value)
 
 
 def swave_options(interp=AM_INTERP_LAGRANGE, z_block=0, periodic=True, seed=12894712) -> SwaveSimOptions:

@@ -23,6 +23,7 @@ struct BssnArgsDev {
     const void *d2o[NH];
     int push_layers;
     int push_axes;  // bits 1 x, 2 y, 4 z
+    const float *adv[NV];  // upwinded advection + dissipation per field (UPW kernels)
 };
 
 struct BssnParDev {
@@ -93,9 +94,15 @@ struct CellWords {
 
 // RHS (and optionally the constraints) at one interior cell: u holds the
 // 24 state values of the cell, cw its derivative words.
-template <int FMT, bool WANT_RHS, bool WANT_CONS>
+// UPW: aw holds each field's upwinded advection plus dissipation (the FD's
+// adv output), which takes the place of the centred beta^k d_k terms; the B
+// equation's d_0 Gt then drops aw of Gt (advection and dissipation) with
+// advect_shift, and beta and B take aw whole (their dissipation, and their
+// advection with advect_shift).
+template <int FMT, bool WANT_RHS, bool WANT_CONS, bool UPW = false>
 __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, const float (&u)[NV], const CellWords<FMT> &cw, size_t w,
-                                          const BssnParDev &par, float *rhs, float *Hc, float *Mc)
+                                          const BssnParDev &par, float *rhs, float *Hc, float *Mc,
+                                          const float *aw = nullptr)
 {
 #pragma clang fp contract(fast)
     auto d1v = [&](int v, float *o) {
@@ -132,6 +139,10 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, const float (&u)
     for (int c = 0; c < 6; c++) d1v(1 + c, dg[c]);
     const float divb = db[0][0] + db[1][1] + db[2][2];
     auto adv = [&](const float *d) { return beta[0] * d[0] + beta[1] * d[1] + beta[2] * d[2]; };
+    auto A = [&](int v, const float *d) {
+        if constexpr (UPW) return aw[v];
+        else return adv(d);
+    };
 
     // Gl[i][jk] = Gt_ijk = (d_k g_ij + d_j g_ik - d_i g_jk) / 2, G[i][jk] = Gt^i_jk
     float Gl[3][6], G[3][6];
@@ -229,19 +240,19 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, const float (&u)
     }
 
     if (WANT_RHS) {
-        rhs[0] = adv(dchi) + (2.f / 3.f) * chi * (alpha * K - divb);
+        rhs[0] = A(0, dchi) + (2.f / 3.f) * chi * (alpha * K - divb);
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = i; j < 3; j++) {
                 const int c = SI[i][j];
-                float s = adv(dg[c]) - (2.f / 3.f) * g[c] * divb - 2.f * alpha * At[c];
+                float s = A(1 + c, dg[c]) - (2.f / 3.f) * g[c] * divb - 2.f * alpha * At[c];
 #pragma unroll
                 for (int k = 0; k < 3; k++) s += g[SI[i][k]] * db[k][j] + g[SI[j][k]] * db[k][i];
                 rhs[1 + c] = s;
             }
         const float lap_alpha = chi * sym_dot(gu, DDalt) - .5f * dchi_dal;
-        rhs[7] = adv(dK) - lap_alpha + alpha * (AA + K * K * (1.f / 3.f));
+        rhs[7] = A(7, dK) - lap_alpha + alpha * (AA + K * K * (1.f / 3.f));
         // At_ij
         float X[6];
 #pragma unroll
@@ -260,7 +271,7 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, const float (&u)
                 const int c = SI[i][j];
                 float dA[3];
                 d1v(8 + c, dA);
-                float s = adv(dA) - (2.f / 3.f) * At[c] * divb + chi * (X[c] - g[c] * trX * (1.f / 3.f)) +
+                float s = A(8 + c, dA) - (2.f / 3.f) * At[c] * divb + chi * (X[c] - g[c] * trX * (1.f / 3.f)) +
                           alpha * K * At[c];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
@@ -277,7 +288,7 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, const float (&u)
         for (int i = 0; i < 3; i++) {
             float dGi[3], dBi[3];
             d1v(14 + i, dGi);
-            float s = adv(dGi) + (2.f / 3.f) * Gam[i] * divb + sym_dot(gu, ddb[i]);
+            float s = A(14 + i, dGi) + (2.f / 3.f) * Gam[i] * divb + sym_dot(gu, ddb[i]);
 #pragma unroll
             for (int k = 0; k < 3; k++) s -= Gam[k] * db[i][k];
             s += 2.f * alpha * sym_dot(Atu, G[i]);
@@ -291,11 +302,17 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, const float (&u)
             rhs[14 + i] = s;
             d1v(21 + i, dBi);
             const float bdb[3] = {dBi[0], dBi[1], dBi[2]};
-            rhs[21 + i] = (par.advect_shift ? adv(bdb) - adv(dGi) : 0.f) + s - par.eta * B[i];
+            if constexpr (UPW)
+                rhs[21 + i] = aw[21 + i] + s - (par.advect_shift ? aw[14 + i] : 0.f) - par.eta * B[i];
+            else
+                rhs[21 + i] = (par.advect_shift ? adv(bdb) - adv(dGi) : 0.f) + s - par.eta * B[i];
             float dbi[3] = {db[i][0], db[i][1], db[i][2]};
-            rhs[18 + i] = (par.advect_shift ? adv(dbi) : 0.f) + par.shift_coeff * B[i];
+            if constexpr (UPW)
+                rhs[18 + i] = aw[18 + i] + par.shift_coeff * B[i];
+            else
+                rhs[18 + i] = (par.advect_shift ? adv(dbi) : 0.f) + par.shift_coeff * B[i];
         }
-        rhs[17] = adv(dal) - (par.slicing == 1 ? alpha * alpha * K : 2.f * alpha * K);
+        rhs[17] = A(17, dal) - (par.slicing == 1 ? alpha * alpha * K : 2.f * alpha * K);
     }
     if (WANT_CONS) {
         *Hc = chi * sym_dot(gu, R) + (2.f / 3.f) * K * K - AA;
@@ -327,7 +344,7 @@ struct TileArgs {
     int32_t n_items;  // tiles x 256-cell blocks; workgroups loop over them
 };
 
-template <int S, int P, int STAGE, bool PUSH, bool PERSIST, int FMT, int L>
+template <int S, int P, int STAGE, bool PUSH, bool PERSIST, int FMT, int L, bool UPW = false>
 __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, BssnParDev par, DepthScales sc)
 {
 #pragma clang fp contract(fast)
@@ -360,8 +377,13 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
         for (int v = 0; v < NV; v++) u[v] = a.state[v][r];
         CellWords<FMT> cw;
         cw.load(a, w);
+        float aw[UPW ? NV : 1];
+        if constexpr (UPW) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) aw[v] = a.adv[v][w];
+        }
         float k[NV];
-        bssn_cell<FMT, true, false>(a, u, cw, w, par, k, nullptr, nullptr);
+        bssn_cell<FMT, true, false, UPW>(a, u, cw, w, par, k, nullptr, nullptr, aw);
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             yn[v] = STAGE == 0 ? u[v] : a.old_[v][r];
@@ -469,10 +491,28 @@ static void launch_stage_push(int stage, dim3 grid, hipStream_t st, const BssnAr
     default: hipLaunchKernelGGL((k_bssn_stage<S, P, 3, PUSH, PERSIST, FMT, L>), grid, dim3(256), 0, st, a, m, d, sc); break;
     }
 }
+template <int S, int P, bool PUSH, bool PERSIST, int FMT, int L>
+static void launch_stage_upw(int stage, dim3 grid, hipStream_t st, const BssnArgsDev &a, const TileArgs &m,
+                             const BssnParDev &d, const DepthScales &sc)
+{
+    switch (stage) {
+    case 0: hipLaunchKernelGGL((k_bssn_stage<S, P, 0, PUSH, PERSIST, FMT, L, true>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    case 1: hipLaunchKernelGGL((k_bssn_stage<S, P, 1, PUSH, PERSIST, FMT, L, true>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    case 2: hipLaunchKernelGGL((k_bssn_stage<S, P, 2, PUSH, PERSIST, FMT, L, true>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    default: hipLaunchKernelGGL((k_bssn_stage<S, P, 3, PUSH, PERSIST, FMT, L, true>), grid, dim3(256), 0, st, a, m, d, sc); break;
+    }
+}
 template <int S, int P, bool PUSH, bool PERSIST, int FMT>
 static void launch_stage_layout(int layout, int stage, dim3 grid, hipStream_t st, const BssnArgsDev &a,
-                                const TileArgs &m, const BssnParDev &d, const DepthScales &sc)
+                                const TileArgs &m, const BssnParDev &d, const DepthScales &sc, bool upw)
 {
+    if constexpr (FMT == WORD_CF3 && S == 16 && P == 3) {  // the upwinded form: cf3 words, S=16, P=3
+        if (upw) {
+            if (layout == AM_LAYOUT_INTERIOR_FIRST)
+                return launch_stage_upw<S, P, PUSH, PERSIST, FMT, 1>(stage, grid, st, a, m, d, sc);
+            return launch_stage_upw<S, P, PUSH, PERSIST, FMT, 0>(stage, grid, st, a, m, d, sc);
+        }
+    }
     if (layout == AM_LAYOUT_INTERIOR_FIRST)
         return launch_stage_push<S, P, PUSH, PERSIST, FMT, 1>(stage, grid, st, a, m, d, sc);
     launch_stage_push<S, P, PUSH, PERSIST, FMT, 0>(stage, grid, st, a, m, d, sc);
@@ -504,14 +544,23 @@ int launch_stage_fmt(const am_bssn_fields *f, int stage, const am_bssn_params *p
     const int lay = f->tile_layout;
     if (lay != AM_LAYOUT_PADDED && lay != AM_LAYOUT_INTERIOR_FIRST)
         return set_error(AM_ERR_INVALID, "am_bssn_rhs_stage: tile layout %d", lay);
+    const bool upw = f->adv[0] != nullptr;
+    if (upw) {
+        if (!(FMT == WORD_CF3 && S == 16 && P == 3))
+            return set_error(AM_ERR_INVALID, "am_bssn_rhs_stage: the upwinded form needs S=16, P=3 and cf3 words");
+        for (int v = 0; v < NV; v++) {
+            if (!f->adv[v]) return set_error(AM_ERR_INVALID, "am_bssn_rhs_stage: null adv field %d", v);
+            a.adv[v] = f->adv[v];
+        }
+    }
     if (push && persist)
-        launch_stage_layout<S, P, true, true, FMT>(lay, stage, grid, st, a, m, d, sc);
+        launch_stage_layout<S, P, true, true, FMT>(lay, stage, grid, st, a, m, d, sc, upw);
     else if (push)
-        launch_stage_layout<S, P, true, false, FMT>(lay, stage, grid, st, a, m, d, sc);
+        launch_stage_layout<S, P, true, false, FMT>(lay, stage, grid, st, a, m, d, sc, upw);
     else if (persist)
-        launch_stage_layout<S, P, false, true, FMT>(lay, stage, grid, st, a, m, d, sc);
+        launch_stage_layout<S, P, false, true, FMT>(lay, stage, grid, st, a, m, d, sc, upw);
     else
-        launch_stage_layout<S, P, false, false, FMT>(lay, stage, grid, st, a, m, d, sc);
+        launch_stage_layout<S, P, false, false, FMT>(lay, stage, grid, st, a, m, d, sc, upw);
     return check_launch("k_bssn_stage");
 }
 

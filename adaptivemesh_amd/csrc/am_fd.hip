@@ -28,7 +28,18 @@ struct FdBatchDev {
     int32_t seed_d[AM_MAX_VARS];
     int32_t seed_m[AM_MAX_VARS];
     const int32_t *seed_table;  // non-null: seed_d/seed_m index it
+    float *adv[AM_MAX_VARS];    // upwinded advection + dissipation (ADV kernels)
+    const float *shift[3];
+    uint32_t advect_mask;
+    float ko_sigma;
 };
+
+// Weights of the fourth-order lopsided first derivative, offsets -1..3
+// (shift > 0) and -3..1 (shift <= 0), and of the sixth difference, as
+// floats: the oracle's orc_fd_adv uses the same values in the same order.
+__device__ constexpr float LOP_PLUS[5] = {-3.f / 12, -10.f / 12, 18.f / 12, -6.f / 12, 1.f / 12};
+__device__ constexpr float LOP_MINUS[5] = {-1.f / 12, 6.f / 12, -18.f / 12, 10.f / 12, 3.f / 12};
+__device__ constexpr float KO6[7] = {1.f, -6.f, 15.f, -20.f, 15.f, -6.f, 1.f};
 
 struct MeshDev {
     const uint32_t *positions;
@@ -93,10 +104,14 @@ __device__ __forceinline__ float mixed_bd(float c, float pp, float mm, float m0,
 // of each x and y face (S contiguous floats each), a z ghost plane from the
 // slab layer (or the z neighbour's interior plane), and ghost layers beyond
 // the stencil's half-width Q are not loaded at all.
-template <int S, int P, int Q, int ZB, bool PERSIST, int FMT, int PF, int DIR, int L>
+// ADV: also the upwinded advection + Kreiss-Oliger output (b.adv), which
+// reads three ghost layers and the shift at each output cell.
+template <int S, int P, int Q, int ZB, bool PERSIST, int FMT, int PF, int DIR, int L, bool ADV = false>
 __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScales sc)
 {
     static_assert(L == 0 || DIR != 2, "the interior-first layout takes direct z faces only");
+    static_assert(!ADV || P >= 3, "the lopsided and sixth-difference stencils reach 3 cells");
+    constexpr int QL = ADV ? 3 : Q;  // ghost layers the stencils read
     constexpr int T = S + 2 * P, T2 = T * T, R = 2 * P + 2, NZB = S / ZB;
     constexpr size_t TS = tile_stride<S, P, L>();
     constexpr FdCoef C = fd_coef();
@@ -163,11 +178,11 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     // interior-first layout: this thread's element of the x/y face slab rows
     // of an interior plane (4 faces x Q layers x S cells), if any: its offset
     // from the slab region (plus z * S per plane) and its place in the plane
-    constexpr int NF = 4 * Q * S;
+    constexpr int NF = 4 * QL * S;
     int f_goff = -1, f_loff = 0;
     if constexpr (L == 1) {
         if (tid < NF) {
-            const int k = tid / (Q * S), l = (tid / S) % Q, a = tid % S;
+            const int k = tid / (QL * S), l = (tid / S) % QL, a = tid % S;
             const int f = k == 0 ? 3 : (k == 1 ? 0 : (k == 2 ? 4 : 1));  // -x +x -y +y
             f_goff = (f * P + l) * S * S + a;
             const int g = (k & 1) ? P + S + l : P - 1 - l;  // the ghost coordinate of the face axis
@@ -183,7 +198,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
             } else {
                 const bool low = zp < P;
                 const int l = low ? P - 1 - zp : zp - P - S;
-                if (l < Q) {
+                if (l < QL) {
                     const float *pl_ = src + (size_t)(S * S * S) + (size_t)(((low ? 5 : 2) * P + l) * S * S);
                     if constexpr (DIR == 1) {
                         if (low && nb[5]) pl_ = nb[5] + (size_t)(S - 1 - l) * (S * S);
@@ -233,7 +248,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
             if (zp >= P && zp < P + S) {
                 d[(tid / S + P) * T + tid % S + P] = from[0].x;
                 if (f_goff >= 0) d[f_loff] = from[0].y;
-            } else if ((zp < P ? P - 1 - zp : zp - P - S) < Q) {
+            } else if ((zp < P ? P - 1 - zp : zp - P - S) < QL) {
                 d[(tid / S + P) * T + tid % S + P] = from[0].x;
             }
             return;
@@ -260,6 +275,24 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     // dither seeds: the (x, y) part once, the z mix per plane
     const int32_t sxy_d = thread_seed_xy(gs_d, (uint32_t)x, (uint32_t)y);
     const int32_t sxy_m = thread_seed_xy(gs_m, (uint32_t)x, (uint32_t)y);
+
+    // ADV: the shift at the output cell, loaded one plane ahead
+    float bsh[3] = {0.f, 0.f, 0.f};
+    float *o_adv = nullptr;
+    float kos = 0.f;
+    auto shift_at = [&](int zz) {
+        const size_t i = (size_t)slot * TS + tile_interior<S, P, L>((uint32_t)x, (uint32_t)y, (uint32_t)zz);
+#pragma unroll
+        for (int k = 0; k < 3; k++) bsh[k] = b.shift[k][i];
+    };
+    if constexpr (ADV) {
+        o_adv = b.adv[var];
+        if (o_adv) {  // uniform per workgroup
+            shift_at(z0);
+            kos = (b.ko_sigma * (1.f / 64.f)) * dx1;
+        }
+    }
+    const bool advect = ADV && ((b.advect_mask >> var) & 1u);
 
     for (int z = z0; z < z0 + ZB; z++) {
         // padded plane of output z is z+P; plane z+P+i sits in slot (z+P+i) % R
@@ -318,6 +351,31 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
             }
             int32_t rng = thread_seed_z(sxy_m, gz);
             word_put<FMT>(o_off, o, vxy * dx2, vxz * dx2, vyz * dx2, rng);
+        }
+        if constexpr (ADV) {
+            if (o_adv) {
+                // sum_k shift^k L_k f and the sixth differences, axes x, y, z,
+                // offsets in increasing order (the oracle's orc_fd_adv)
+                float adv = 0.f, ko = 0.f;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    auto fm = [&](int mm) { return k == 0 ? ctr[c + mm] : (k == 1 ? ctr[c + mm * T] : pl[P + mm][c]); };
+                    float l = 0.f, kk = 0.f;
+                    if (bsh[k] > 0.f) {
+#pragma unroll
+                        for (int mm = -1; mm <= 3; mm++) l = l + fm(mm) * LOP_PLUS[mm + 1];
+                    } else {
+#pragma unroll
+                        for (int mm = -3; mm <= 1; mm++) l = l + fm(mm) * LOP_MINUS[mm + 3];
+                    }
+#pragma unroll
+                    for (int mm = -3; mm <= 3; mm++) kk = kk + fm(mm) * KO6[mm + 3];
+                    adv = adv + bsh[k] * (l * dx1);
+                    ko = ko + kk;
+                }
+                o_adv[o] = (advect ? adv : 0.f) + ko * kos;
+                if (z + 1 < z0 + ZB) shift_at(z + 1);
+            }
         }
         if (z + 1 < z0 + ZB) {
             lstore(pre[0], z + 2 * P + 1);
@@ -379,10 +437,35 @@ static void fd_launch(int fmt, int pf, bool persist, dim3 grid, hipStream_t st, 
     if (m.direct_axes == 4u) return fd_launch_dir<S, P, Q, ZB, 1, L>(fmt, pf, persist, grid, st, d, m, sc);
     if constexpr (L == 0) fd_launch_dir<S, P, Q, ZB, 2, L>(fmt, pf, persist, grid, st, d, m, sc);
 }
+// the upwind / dissipation kernels: S = 16, P = 3, order 4, cf3 words,
+// one-plane prefetch, direct faces 0 or z
+template <int S, int P, int Q, int ZB, int L>
+static void fd_launch_adv_l(bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
+                            const DepthScales &sc)
+{
+    if constexpr (S == 16 && P == 3 && Q == 2) {
+        if (m.direct_axes == 4u) {
+            if (persist)
+                hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, WORD_CF3, 1, 1, L, true>), grid, dim3(S * S), 0, st, d, m, sc);
+            else
+                hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false, WORD_CF3, 1, 1, L, true>), grid, dim3(S * S), 0, st, d, m, sc);
+        } else {
+            if (persist)
+                hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, WORD_CF3, 1, 0, L, true>), grid, dim3(S * S), 0, st, d, m, sc);
+            else
+                hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false, WORD_CF3, 1, 0, L, true>), grid, dim3(S * S), 0, st, d, m, sc);
+        }
+    }
+}
+
 template <int S, int P, int Q, int ZB>
 static void fd_launch_layout(int layout, int fmt, int pf, bool persist, dim3 grid, hipStream_t st,
-                             const FdBatchDev &d, const MeshDev &m, const DepthScales &sc)
+                             const FdBatchDev &d, const MeshDev &m, const DepthScales &sc, bool adv)
 {
+    if (adv) {
+        if (layout == AM_LAYOUT_INTERIOR_FIRST) return fd_launch_adv_l<S, P, Q, ZB, 1>(persist, grid, st, d, m, sc);
+        return fd_launch_adv_l<S, P, Q, ZB, 0>(persist, grid, st, d, m, sc);
+    }
     if (layout == AM_LAYOUT_INTERIOR_FIRST) return fd_launch<S, P, Q, ZB, 1>(fmt, pf, persist, grid, st, d, m, sc);
     fd_launch<S, P, Q, ZB, 0>(fmt, pf, persist, grid, st, d, m, sc);
 }
@@ -390,7 +473,7 @@ static void fd_launch_layout(int layout, int fmt, int pf, bool persist, dim3 gri
 template <int S, int P, int Q>
 static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream)
 {
-    FdBatchDev d;
+    FdBatchDev d{};
     for (int v = 0; v < b->n_vars; v++) {
         d.field[v] = b->field[v];
         d.d1[v] = b->d1[v];
@@ -436,16 +519,34 @@ static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometr
     const int lay = b->tile_layout;
     if (lay != AM_LAYOUT_PADDED && lay != AM_LAYOUT_INTERIOR_FIRST)
         return set_error(AM_ERR_INVALID, "am_fd: tile layout %d", lay);
+    bool adv = false;
+    for (int v = 0; v < b->n_vars; v++) {
+        d.adv[v] = b->adv[v];
+        adv |= b->adv[v] != nullptr;
+    }
+    if (adv) {
+        if (!(S == 16 && P == 3 && Q == 2) || fmt != WORD_CF3 || pf > 1 ||
+            (b->direct_faces != 0 && b->direct_faces != 4))
+            return set_error(AM_ERR_INVALID,
+                             "am_fd: the upwind / dissipation output needs S=16, P=3, fd_order 4, cf3 words, "
+                             "prefetch 1 and direct_faces 0 or 4");
+        if (!b->shift[0] || !b->shift[1] || !b->shift[2])
+            return set_error(AM_ERR_INVALID, "am_fd: the upwind output needs the three shift fields");
+        if (!(b->ko_sigma >= 0.f)) return set_error(AM_ERR_INVALID, "am_fd: ko_sigma %g", (double)b->ko_sigma);
+        for (int k = 0; k < 3; k++) d.shift[k] = b->shift[k];
+        d.advect_mask = b->advect_mask;
+        d.ko_sigma = b->ko_sigma;
+    }
     if (lay == AM_LAYOUT_INTERIOR_FIRST && (pf > 1 || (b->direct_faces != 0 && b->direct_faces != 4)))
         return set_error(AM_ERR_INVALID,
                          "am_fd: the interior-first layout takes prefetch_planes 0-1 and direct_faces 0 or 4 "
                          "(%d, %d)", b->prefetch_planes, b->direct_faces);
     if (zb == S) {
-        fd_launch_layout<S, P, Q, S>(lay, fmt, pf, persist, dim3(gx(1), b->n_vars), st, d, items(1), sc);
+        fd_launch_layout<S, P, Q, S>(lay, fmt, pf, persist, dim3(gx(1), b->n_vars), st, d, items(1), sc, adv);
     } else if (zb == S / 2) {
-        fd_launch_layout<S, P, Q, S / 2>(lay, fmt, pf, persist, dim3(gx(2), b->n_vars), st, d, items(2), sc);
+        fd_launch_layout<S, P, Q, S / 2>(lay, fmt, pf, persist, dim3(gx(2), b->n_vars), st, d, items(2), sc, adv);
     } else if (zb == S / 4 && (S % 4) == 0) {
-        fd_launch_layout<S, P, Q, S / 4>(lay, fmt, pf, persist, dim3(gx(4), b->n_vars), st, d, items(4), sc);
+        fd_launch_layout<S, P, Q, S / 4>(lay, fmt, pf, persist, dim3(gx(4), b->n_vars), st, d, items(4), sc, adv);
     } else {
         return set_error(AM_ERR_INVALID, "am_fd: z_block %d must be S, S/2 or S/4", zb);
     }

@@ -211,6 +211,7 @@ void *rk4_device_ptr(H *s, int32_t which)
     case 5: return d.d2d.gpu_buffer_ptr;
     case 6: return d.d2o.gpu_buffer_ptr;
     case 7: return d.chr.gpu_buffer_ptr;
+    case 8: return d.adv.gpu_buffer_ptr;
     default: return nullptr;
     }
 }
@@ -597,6 +598,17 @@ int am_bssn_sim_constraints(am_bssn_sim *s, double *out4)
 {
     if (!out4) return am::set_error(AM_ERR_INVALID, "null output");
     return rk4_call(s, [&](auto &d) { d.diagnostics(out4); });
+}
+int am_bssn_sim_set_dissipation(am_bssn_sim *s, float ko_sigma, int32_t upwind)
+{
+    return rk4_call(s, [&](auto &d) { d.set_dissipation(ko_sigma, upwind != 0 ? 1 : 0); });
+}
+int am_bssn_sim_get_dissipation(const am_bssn_sim *s, float *ko_sigma, int32_t *upwind)
+{
+    if (!s) return am::set_error(AM_ERR_INVALID, "null sim");
+    if (ko_sigma) *ko_sigma = s->amr->sim_data.opt.ko_sigma;
+    if (upwind) *upwind = s->amr->sim_data.opt.upwind;
+    return AM_OK;
 }
 
 am_swave_sim *am_swave_sim_create(const am_geometry *g, uint32_t max_slots, const am_swave_sim_options *o)

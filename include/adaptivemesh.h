@@ -232,6 +232,20 @@ typedef struct am_fd_batch {
     /* enum am_tile_layout of every field (AM_LAYOUT_INTERIOR_FIRST takes
      * direct_faces 0 or 4 only) */
     int32_t tile_layout;
+    /* Upwinded advection with Kreiss-Oliger dissipation (no counterpart in
+     * the reference; S=16, P=3, fd_order 4, cf3 words, prefetch 1 only):
+     * adv[v] != NULL receives one float per interior cell (the words'
+     * layout),
+     *   [bit v of advect_mask] sum_k beta^k L_k f + (ko_sigma / 64) (1/h) sum_k K_k f
+     * with beta^k the fields shift[0..2] at the cell, L_k the fourth-order
+     * lopsided first derivative on the side beta^k points to (offsets -1..3
+     * for beta^k > 0, else -3..1) and K_k the sixth difference (1, -6, 15,
+     * -20, 15, -6, 1) along axis k, in float in a fixed order (the oracle's
+     * orc_fd_adv).  Reads three ghost layers. */
+    float *adv[AM_MAX_VARS];
+    const float *shift[3];
+    uint32_t advect_mask;
+    float ko_sigma;
 } am_fd_batch;
 enum am_word_format { AM_WORD_CF3 = 0, AM_WORD_F32 = 1, AM_WORD_F16 = 2 };
 /* bytes per cell of one word array in `format` (4, 12, 6), -1 if unknown */
@@ -404,6 +418,13 @@ typedef struct am_bssn_fields {
     int32_t push_layers; /* ghost layers the push writes, innermost first: 1..P, 0 = all P */
     int32_t push_axes;   /* faces the push writes, by axis: bits 1 x, 2 y, 4 z; 0 = all three */
     int32_t tile_layout; /* enum am_tile_layout of state, old_ and new_ */
+    /* NULL: the advection terms beta^k d_k use the gradient words.
+     * Otherwise 24 float arrays (words' layout) holding each field's
+     * upwinded advection plus Kreiss-Oliger dissipation (am_fd_batch.adv,
+     * advect_mask = fields 0-17, and 18-23 with params.advect_shift), which
+     * replace them: the B^i equation's d_0 Gt^i then drops adv[14+i] with
+     * advect_shift, and beta^i, B^i take adv whole.  S=16, P=3, cf3 words. */
+    const float *adv[AM_BSSN_NVARS];
 } am_bssn_fields;
 
 /* One classical RK4 stage on the interior of every listed active tile:

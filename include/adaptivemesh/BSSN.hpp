@@ -128,12 +128,18 @@ struct bssn_system {
     static constexpr bool CHRISTOFFEL = false;  // raised in registers by the RHS kernel
     static constexpr bool WORD_FORMATS = true;  // fp32 / fp16 words for S=16, P=3 (precision study)
     static constexpr uint32_t CKPT_KIND = AM_CKPT_BSSN;
+    // upwinded advection: beta^i is fields 18-20; fields 0-17 are always
+    // advected, beta and B with advect_shift
+    static constexpr int SHIFT_FIELD = 18;
     using options_t = am_bssn_params;
+    static uint32_t advect_mask(const options_t &p) { return p.advect_shift ? 0xFFFFFFu : 0x3FFFFu; }
 
     static am_bssn_fields fields(const rk4_views &v)
     {
         am_bssn_fields f{};
         f.tile_layout = v.tile_layout;
+        if (v.adv[0])
+            for (int i = 0; i < NV; i++) f.adv[i] = v.adv[i];
         f.word_format = v.word_format;
         f.push_layers = v.push_layers;
         f.push_axes = v.push_axes;

@@ -105,6 +105,12 @@ struct rk4_options {
     // ghost plane is the neighbour's contiguous interior plane, while x and
     // y ghost columns would be strided reads, so the mask is 0 or 4.
     int32_t direct_faces = 4;
+    // upwinded (lopsided fourth-order) advection with Kreiss-Oliger
+    // dissipation of strength ko_sigma (sixth order), computed by the FD
+    // kernel into `adv` and used by the System's RHS (BSSN only; needs P = 3
+    // and pushes all P ghost layers)
+    int32_t upwind = 0;
+    float ko_sigma = 0.f;
 };
 
 // What autotune() measured: each trial's launch shape and mean ms per launch.
@@ -130,6 +136,7 @@ struct rk4_views {
     const uint32_t *d2diag[AM_MAX_VARS];  // by Hessian index (fields with FD_KIND >= 2, in order)
     const uint32_t *d2off[AM_MAX_VARS];   // by full-Hessian index (FD_KIND == 3, in order)
     const uint32_t *chr[6];
+    const float *adv[AM_MAX_VARS];  // upwinded advection + dissipation per field, or null
     int32_t word_format;  // am_word_format of d1/d2diag/d2off
     int32_t push_layers;  // equal-depth ghost layers the stage kernel pushes
     int32_t push_axes;    // and the axes whose faces it pushes (bits 1 x, 2 y, 4 z)
@@ -163,6 +170,7 @@ struct rk4_AMR_data {
     smart_gpu_buffer<float> old_, new_, acc, older;
     smart_gpu_buffer<float> conv;  // one padded field, for conversions (allocated on first use)
     smart_gpu_buffer<uint32_t> d1, d2d, d2o, chr;
+    smart_gpu_buffer<float> adv;  // NV interior-only arrays (allocated with set_dissipation)
     smart_gpu_buffer<double> partial;
     uint64_t zone_cycles = 0;  // interior cells advanced by one full step, all depths
     kernel_timer timer;
@@ -223,7 +231,7 @@ struct rk4_AMR_data {
     {
         drop_graph();
         if (stream) am_stream_sync(stream);
-        old_.destroy(); new_.destroy(); acc.destroy(); older.destroy(); conv.destroy();
+        old_.destroy(); new_.destroy(); acc.destroy(); older.destroy(); conv.destroy(); adv.destroy();
         d1.destroy(); d2d.destroy(); d2o.destroy(); chr.destroy(); partial.destroy(); seed_table.destroy();
         mlp_stats.destroy();
         timer.destroy();
@@ -238,7 +246,22 @@ struct rk4_AMR_data {
     }
     float *field(smart_gpu_buffer<float> &b, int v)
     {
-        return b.gpu_buffer_ptr + (size_t)v * slots * (&b == &acc ? inner : cells);
+        return b.gpu_buffer_ptr + (size_t)v * slots * (&b == &acc || &b == &adv ? inner : cells);
+    }
+    // upwinded advection with Kreiss-Oliger dissipation (upwind = 1) or the
+    // centred advection of the words (upwind = 0, sigma = 0)
+    void set_dissipation(float sigma, int32_t upwind)
+    {
+        if (upwind && System::SHIFT_FIELD < 0) throw std::runtime_error("this system has no shift to upwind along");
+        if (!(sigma >= 0.f) || (sigma > 0.f && !upwind))
+            throw std::runtime_error("dissipation needs upwind = 1 and sigma >= 0");
+        if (upwind && (geom.padding_domain != 3 || geom.size_domain != 16 || geom.fd_order != 4 ||
+                       opt.word_format != AM_WORD_CF3))
+            throw std::runtime_error("upwinding needs S = 16, P = 3, fd_order 4 and compressed_float3 words");
+        opt.upwind = upwind;
+        opt.ko_sigma = sigma;
+        if (upwind && !adv.created) adv = smart_gpu_buffer<float>((size_t)NV * slots * inner);
+        drop_graph();
     }
     uint32_t *word(smart_gpu_buffer<uint32_t> &b, int v)
     {
@@ -253,6 +276,7 @@ struct rk4_AMR_data {
     int32_t push_layers() const
     {
         const int32_t P = geom.padding_domain;
+        if (opt.upwind) return P;  // the lopsided and sixth-difference stencils read every layer
         const int32_t l = opt.push_layers > 0 ? opt.push_layers : std::max(1, geom.fd_order / 2);
         return std::min(l, P);
     }
@@ -305,6 +329,8 @@ struct rk4_AMR_data {
         for (int h = 0; h < n_off; h++) f.d2off[h] = word(d2o, h);
         if (System::CHRISTOFFEL)
             for (int c = 0; c < 6; c++) f.chr[c] = word(chr, c);
+        if (opt.upwind)
+            for (int v = 0; v < NV; v++) f.adv[v] = field(adv, v);
         return f;
     }
 
@@ -322,7 +348,10 @@ struct rk4_AMR_data {
     // slots, variable-major
     static constexpr uint32_t CKPT_KIND = System::CKPT_KIND;
     uint32_t ckpt_vars() const { return (uint32_t)NV; }
-    uint32_t ckpt_flags() const { return (uint32_t)opt.interp | ((uint32_t)opt.word_format << 4); }
+    uint32_t ckpt_flags() const
+    {
+        return (uint32_t)opt.interp | ((uint32_t)opt.word_format << 4) | ((uint32_t)(opt.upwind != 0) << 8);
+    }
     uint32_t ckpt_sections() const { return older.created ? 3u : 2u; }
     // (the file holds padded tiles, whatever the driver's layout)
     void ckpt_save(checkpoint_out &o)
@@ -372,12 +401,21 @@ struct rk4_AMR_data {
         b.seed_table = recording ? seed_table.gpu_buffer_ptr : nullptr;
         m.grid_limit = opt.fd_grid_limit;
         int h = 0, o = 0;
+        const uint32_t amask = opt.upwind ? System::advect_mask(sys) : 0u;
+        if (opt.upwind) {
+            for (int k = 0; k < 3; k++) b.shift[k] = field(src, System::SHIFT_FIELD + k);
+            b.ko_sigma = opt.ko_sigma;
+        }
         for (int v = 0; v < NV; v++) {
             const int kind = System::FD_KIND[v];
             if (kind == 0) continue;
             const int i = b.n_vars++;
             b.field[i] = field(src, v);
             b.d1[i] = word(d1, v);
+            if (opt.upwind) {
+                b.adv[i] = field(adv, v);
+                b.advect_mask |= ((amask >> v) & 1u) << i;
+            }
             b.seed_d[i] = draw();
             if (kind >= 2) b.d2diag[i] = word(d2d, h++);
             if (kind == 3) {

@@ -23,7 +23,9 @@ struct swave_system {
     static constexpr bool CHRISTOFFEL = false;
     static constexpr bool WORD_FORMATS = false;  // compressed_float3 words only
     static constexpr uint32_t CKPT_KIND = AM_CKPT_SWAVE;
+    static constexpr int SHIFT_FIELD = -1;  // no shift: no upwinded advection
     using options_t = swave_options;
+    static uint32_t advect_mask(const options_t &) { return 0u; }
 
     static am_swave_fields fields(const rk4_views &v)
     {

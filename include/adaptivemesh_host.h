@@ -171,10 +171,13 @@ int am_bssn_sim_export(const am_bssn_sim *s, int32_t *parent, int32_t *children,
  * the host (acc is interior-only on the device: its ghost cells read as 0) */
 int am_bssn_sim_set_field(am_bssn_sim *s, int32_t which, int32_t var, const float *host);
 int am_bssn_sim_get_field(am_bssn_sim *s, int32_t which, int32_t var, float *host);
-/* device addresses: which 0, 1, 3 as above (var stride max_slots*T^3
+/* device addresses: which 0, 1, 3 as above, in the interior-first tile
+ * layout (var stride max_slots * am_tile_stride(g, AM_LAYOUT_INTERIOR_FIRST)
  * floats), 2 acc (var stride max_slots*S^3 floats), 4 d1, 5 d2diag, 6 d2off
  * words (var stride max_slots*S^3); 7 Christoffel words where a driver keeps
- * them (none: the BSSN RHS raises the symbols in registers) */
+ * them (none: the BSSN RHS raises the symbols in registers); 8 the upwinded
+ * advection + dissipation (var stride max_slots*S^3 floats; after
+ * am_bssn_sim_set_dissipation) */
 void *am_bssn_sim_device_ptr(am_bssn_sim *s, int32_t which);
 int am_bssn_sim_fill_ghosts(am_bssn_sim *s);
 int am_bssn_sim_copy_back(am_bssn_sim *s);
@@ -202,6 +205,17 @@ int am_bssn_sim_set_launch(am_bssn_sim *s, int32_t z_block, int32_t fd_grid_limi
 int am_bssn_sim_get_launch(const am_bssn_sim *s, int32_t *out4);
 /* FD planes prefetched ahead of the differentiated plane (1-3) */
 int am_bssn_sim_set_fd_prefetch(am_bssn_sim *s, int32_t planes);
+/* Upwinded advection with Kreiss-Oliger dissipation (no counterpart in the
+ * reference; for moving-puncture runs): upwind = 1 replaces the centred
+ * advection beta^k d_k of every field by the fourth-order lopsided stencil
+ * on the side the shift points to and adds sixth-order dissipation of
+ * strength ko_sigma (>= 0) to every field (am_fd_batch.adv); the stage
+ * kernels then push all P ghost layers.  upwind = 0, ko_sigma = 0 restores
+ * the centred scheme.  Needs S=16, P=3, fd_order 4, compressed_float3
+ * words; refused (AM_ERR_INVALID) otherwise.  A checkpoint records upwind
+ * in its flags. */
+int am_bssn_sim_set_dissipation(am_bssn_sim *s, float ko_sigma, int32_t upwind);
+int am_bssn_sim_get_dissipation(const am_bssn_sim *s, float *ko_sigma, int32_t *upwind);
 /* Equal-depth ghost layers the stage kernels refresh, innermost first:
  * 0 (the default) = fd_order / 2, the layers the FD stencils read; P =
  * every layer, as the reference's copy_boundaries writes them.  Layers

@@ -541,6 +541,55 @@ EXPORT void orc_mixed_partials(const float *field, uint32_t *off, const uint32_t
 {
     mixed_partials(field, off, pos, tiles, n_tiles, substep, maxd, gs, S, P, outer_dx, 0);
 }
+/* The FD's advection + dissipation output (an extension for the upwinded,
+ * dissipative BSSN runs; no reference counterpart), in float with no
+ * contraction, as the product computes it:
+ *   L_k = (sum over the five offsets m of the lopsided stencil of f_m * c_m,
+ *          in increasing m) * inv_dx, on the side beta^k points to
+ *          (c = (-3, -10, 18, -6, 1) / 12 at m = -1..3 for beta^k > 0, else
+ *          (-1, 6, -18, 10, 3) / 12 at m = -3..1, each rounded to float)
+ *   K_k = sum over m = -3..3 of f_m * (1, -6, 15, -20, 15, -6, 1)_m
+ *   out = (advect ? (beta^x L_x + beta^y L_y) + beta^z L_z : 0)
+ *         + ((K_x + K_y) + K_z) * ((sigma * (1/64)) * inv_dx)
+ * with beta^k the shift fields at the cell and inv_dx = 2^depth /
+ * outer_dx as float (AMR.h:1214).  out: interior-only layout, as the words. */
+static const float LOPF_P[5] = {-3.f / 12, -10.f / 12, 18.f / 12, -6.f / 12, 1.f / 12};
+static const float LOPF_M[5] = {-1.f / 12, 6.f / 12, -18.f / 12, 10.f / 12, 3.f / 12};
+static const float KO6F[7] = {1.f, -6.f, 15.f, -20.f, 15.f, -6.f, 1.f};
+EXPORT void orc_fd_adv(const float *field, const float *bx, const float *by, const float *bz, float *out,
+                       const uint32_t *pos, const int32_t *tiles, int n_tiles, uint32_t substep, uint32_t maxd,
+                       int S, int P, float outer_dx, float sigma, int advect)
+{
+    const int T = S + 2 * P;
+    const int64_t st[3] = {1, T, (int64_t)T * T};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int ti = 0; ti < n_tiles; ti++)
+        for (int z = 0; z < S; z++) {
+            const int slot = tiles[ti];
+            const int depth = pos_depth(pos[slot]);
+            if (depth == -1 || !active_depth(substep, (uint32_t)depth, maxd)) continue;
+            const float dx1 = (float)(1u << depth) / outer_dx;
+            const float kos = (sigma * (1.f / 64.f)) * dx1;
+            for (int y = 0; y < S; y++)
+                for (int x = 0; x < S; x++) {
+                    const int64_t rw = (((int64_t)slot * T + z) * T + y) * T + x + P * (1 + T + T * T);
+                    const float beta[3] = {bx[rw], by[rw], bz[rw]};
+                    float adv = 0.f, ko = 0.f;
+                    for (int k = 0; k < 3; k++) {
+                        float l = 0.f, kk = 0.f;
+                        if (beta[k] > 0.f)
+                            for (int m = -1; m <= 3; m++) l = l + field[rw + m * st[k]] * LOPF_P[m + 1];
+                        else
+                            for (int m = -3; m <= 1; m++) l = l + field[rw + m * st[k]] * LOPF_M[m + 3];
+                        for (int m = -3; m <= 3; m++) kk = kk + field[rw + m * st[k]] * KO6F[m + 3];
+                        adv = adv + beta[k] * (l * dx1);
+                        ko = ko + kk;
+                    }
+                    out[(((int64_t)slot * S + z) * S + y) * S + x] = (advect ? adv : 0.f) + ko * kos;
+                }
+        }
+}
+
 EXPORT void orc_mixed_partials_f32(const float *field, float *off, const uint32_t *pos, const int32_t *tiles,
                                    int n_tiles, uint32_t substep, uint32_t maxd, int S, int P, float outer_dx)
 {

@@ -80,7 +80,7 @@ static void inv3(const double g[3][3], double r[3][3])
  * from the metric derivatives.  Writes rhs[24] and, if H != NULL, the
  * constraints H and M[3]. */
 static void bssn_point(const double *u, const double (*d1)[3], const double (*dd)[6], const double (*Gin)[3],
-                       const orc_bssn_par *par, double *rhs, double *Hc, double *Mc)
+                       const orc_bssn_par *par, const double *advin, double *rhs, double *Hc, double *Mc)
 {
     const double chi = u[0], chid = fmax(u[0], par->chi_floor);
     double gt[3][3], At[3][3], gtu[3][3], Gam[3], beta[3], B[3];
@@ -158,8 +158,12 @@ static void bssn_point(const double *u, const double (*d1)[3], const double (*dd
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) AA += At[i][j] * Atu[i][j];
 
+    /* advection beta^k d_k of each field: centred, from the gradients, or
+     * (advin != NULL) given per field -- the upwinded advection plus the
+     * Kreiss-Oliger dissipation that the FD computes (see fd64_adv) */
     double adv[BSSN_NV];
-    for (int v = 0; v < BSSN_NV; v++) adv[v] = beta[0] * d1[v][0] + beta[1] * d1[v][1] + beta[2] * d1[v][2];
+    for (int v = 0; v < BSSN_NV; v++)
+        adv[v] = advin ? advin[v] : beta[0] * d1[v][0] + beta[1] * d1[v][1] + beta[2] * d1[v][2];
 
     /* covariant second derivatives of chi and alpha */
     double DDchi[3][3], DDalt[3][3], DDal[3][3];
@@ -252,8 +256,16 @@ static void bssn_point(const double *u, const double (*d1)[3], const double (*dd
     }
     rhs[17] = adv[17] - (par->slicing == 1 ? alpha * alpha * K : 2 * alpha * K);
     for (int i = 0; i < 3; i++) {
-        rhs[18 + i] = (par->advect_shift ? adv[18 + i] : 0.0) + par->shift_coeff * B[i];
-        rhs[21 + i] = (par->advect_shift ? adv[21 + i] - adv[14 + i] : 0.0) + rhs[14 + i] - par->eta * B[i];
+        if (advin) {
+            /* adv[18..23] hold the shift's advection (with advect_shift)
+             * plus its dissipation; d_0 Gt in the B equation drops Gt's
+             * advection term, dissipation included, with advect_shift */
+            rhs[18 + i] = adv[18 + i] + par->shift_coeff * B[i];
+            rhs[21 + i] = adv[21 + i] + rhs[14 + i] - (par->advect_shift ? adv[14 + i] : 0.0) - par->eta * B[i];
+        } else {
+            rhs[18 + i] = (par->advect_shift ? adv[18 + i] : 0.0) + par->shift_coeff * B[i];
+            rhs[21 + i] = (par->advect_shift ? adv[21 + i] - adv[14 + i] : 0.0) + rhs[14 + i] - par->eta * B[i];
+        }
     }
 
     if (Hc) {
@@ -292,8 +304,16 @@ EXPORT void orc_bssn_point(const double *u, const double *d1, const double *dd, 
                            double *rhs, double *HM)
 {
     const orc_bssn_par p = par_from(params);
-    bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd, (const double(*)[3])G, &p, rhs, HM,
+    bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd, (const double(*)[3])G, &p, NULL, rhs, HM,
                HM ? HM + 1 : NULL);
+}
+/* The same with the advection of every field given (advin[24]: upwinded
+ * advection plus dissipation, as the FD's adv output). */
+EXPORT void orc_bssn_point_adv(const double *u, const double *d1, const double *dd, const double *params,
+                               const double *advin, double *rhs)
+{
+    const orc_bssn_par p = par_from(params);
+    bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd, NULL, &p, advin, rhs, NULL, NULL);
 }
 
 /* --------------------------------------------------------- tile helpers */
@@ -361,8 +381,9 @@ static inline void rk4_combine(int stage, double dt, double yn, double k, double
 static void stage_from_words(const float *const *state, const float *const *old_, const double *const *acc_in,
                              double *const *acc_out, double *const *new_out, const void *const *d1w,
                              const void *const *d2dw, const void *const *d2ow, const uint32_t *const *chw,
-                             const uint32_t *pos, const int32_t *tiles, int n_tiles, uint32_t substep, uint32_t maxd,
-                             int stage, const double *dtd, const double *params, int S, int P, int fmt)
+                             const float *const *advw, const uint32_t *pos, const int32_t *tiles, int n_tiles,
+                             uint32_t substep, uint32_t maxd, int stage, const double *dtd, const double *params,
+                             int S, int P, int fmt)
 {
     const orc_bssn_par par = par_from(params);
     const int T = S + 2 * P;
@@ -392,8 +413,11 @@ static void stage_from_words(const float *const *state, const float *const *old_
                     }
                     if (chw)
                         for (int c = 0; c < 6; c++) cf3_dec(chw[c][w], G[c]);
+                    double advin[BSSN_NV];
+                    if (advw)
+                        for (int v = 0; v < BSSN_NV; v++) advin[v] = advw[v][w];
                     bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd,
-                               chw ? (const double(*)[3])G : NULL, &par, k, NULL, NULL);
+                               chw ? (const double(*)[3])G : NULL, &par, advw ? advin : NULL, k, NULL, NULL);
                     for (int v = 0; v < BSSN_NV; v++) {
                         const double ai = stage == 0 ? 0.0 : acc_in[v][r];
                         rk4_combine(stage, dt, (double)old_[v][r], k[v], ai, &acc_out[v][r], &new_out[v][r]);
@@ -410,7 +434,23 @@ EXPORT void orc_bssn_stage_from_words(const float *const *state, const float *co
                                       const double *params, int S, int P)
 {
     stage_from_words(state, old_, acc_in, acc_out, new_out, (const void *const *)d1w, (const void *const *)d2dw,
-                     (const void *const *)d2ow, chw, pos, tiles, n_tiles, substep, maxd, stage, dtd, params, S, P, 0);
+                     (const void *const *)d2ow, chw, NULL, pos, tiles, n_tiles, substep, maxd, stage, dtd, params, S,
+                     P, 0);
+}
+/* the same with the FD's advection + dissipation output of every field
+ * (advw: 24 fp32 arrays, interior-only layout) in place of the centred
+ * advection terms */
+EXPORT void orc_bssn_stage_from_words_adv(const float *const *state, const float *const *old_,
+                                          const double *const *acc_in, double *const *acc_out,
+                                          double *const *new_out, const uint32_t *const *d1w,
+                                          const uint32_t *const *d2dw, const uint32_t *const *d2ow,
+                                          const float *const *advw, const uint32_t *pos, const int32_t *tiles,
+                                          int n_tiles, uint32_t substep, uint32_t maxd, int stage, const double *dtd,
+                                          const double *params, int S, int P)
+{
+    stage_from_words(state, old_, acc_in, acc_out, new_out, (const void *const *)d1w, (const void *const *)d2dw,
+                     (const void *const *)d2ow, NULL, advw, pos, tiles, n_tiles, substep, maxd, stage, dtd, params,
+                     S, P, 0);
 }
 /* the same with the words in format fmt (0 cf3, 1 fp32, 2 fp16 triples) */
 EXPORT void orc_bssn_stage_from_words_fmt(const float *const *state, const float *const *old_,
@@ -420,8 +460,8 @@ EXPORT void orc_bssn_stage_from_words_fmt(const float *const *state, const float
                                           int n_tiles, uint32_t substep, uint32_t maxd, int stage, const double *dtd,
                                           const double *params, int S, int P, int fmt)
 {
-    stage_from_words(state, old_, acc_in, acc_out, new_out, d1w, d2dw, d2ow, NULL, pos, tiles, n_tiles, substep,
-                     maxd, stage, dtd, params, S, P, fmt);
+    stage_from_words(state, old_, acc_in, acc_out, new_out, d1w, d2dw, d2ow, NULL, NULL, pos, tiles, n_tiles,
+                     substep, maxd, stage, dtd, params, S, P, fmt);
 }
 
 /* Constraint sums on the product's inputs: per listed tile,
@@ -457,7 +497,7 @@ static void constraints_from_words(const float *const *state, const void *const 
                     if (chw)
                         for (int c = 0; c < 6; c++) cf3_dec(chw[c][w], G[c]);
                     bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd,
-                               chw ? (const double(*)[3])G : NULL, &par, k, &H, M);
+                               chw ? (const double(*)[3])G : NULL, &par, NULL, k, &H, M);
                     h2 += H * H;
                     m2 += M[0] * M[0] + M[1] * M[1] + M[2] * M[2];
                     hmax = fmax(hmax, fabs(H));
@@ -546,6 +586,48 @@ static void fd64_cell(const double *f, int64_t rw, int S, int T, int q, int x, i
     h[3] = yy * i2; h[4] = m[2] * i2; h[5] = zz * i2;
 }
 
+/* Upwinded advection and Kreiss-Oliger dissipation of the FP64 scheme (an
+ * extension for the moving-puncture runs; no reference counterpart):
+ *   adv = [advect] sum_k beta^k (D^lop_k f)  +  sigma/(64 h) sum_k (Delta^6_k f)
+ * D^lop_k: fourth-order one-sided-by-one first derivative on the side
+ * beta^k points to, (-3 f_-1 - 10 f_0 + 18 f_1 - 6 f_2 + f_3) / (12 h) for
+ * beta^k > 0 and its mirror (-f_-3 + 6 f_-2 - 18 f_-1 + 10 f_0 + 3 f_1) /
+ * (12 h) otherwise (exact on polynomials of degree <= 4); Delta^6_k: the
+ * sixth difference f_-3 - 6 f_-2 + 15 f_-1 - 20 f_0 + 15 f_1 - 6 f_2 + f_3
+ * (zero on polynomials of degree <= 5; -64 f on the mode (-1)^k), so the
+ * term damps the highest resolvable mode at rate sigma/h per axis.  The
+ * literature's form of both, e.g. Brugmann et al., Phys. Rev. D 77, 024027
+ * (2008).  Off until orc_bssn64_set_dissipation(sigma, 1); the dissipation
+ * comes only with the upwinded advection. */
+static double g_ko_sigma = 0.0;
+static int g_upwind = 0;
+EXPORT void orc_bssn64_set_dissipation(double sigma, int upwind)
+{
+    g_ko_sigma = sigma;
+    g_upwind = upwind;
+}
+static const double LOP_PLUS[5] = {-3. / 12, -10. / 12, 18. / 12, -6. / 12, 1. / 12};   /* offsets -1..3 */
+static const double LOP_MINUS[5] = {-1. / 12, 6. / 12, -18. / 12, 10. / 12, 3. / 12};  /* offsets -3..1 */
+static const double KO6[7] = {1., -6., 15., -20., 15., -6., 1.};
+static double fd64_adv(const double *f, int64_t rw, int T, const double *beta, double inv_dx, double sigma,
+                       int advect)
+{
+    const int64_t st[3] = {1, T, (int64_t)T * T};
+    double adv = 0.0, ko = 0.0;
+    for (int k = 0; k < 3; k++) {
+        const int64_t d = st[k];
+        double l = 0.0, kk = 0.0;
+        if (beta[k] > 0)
+            for (int m = -1; m <= 3; m++) l += LOP_PLUS[m + 1] * f[rw + m * d];
+        else
+            for (int m = -3; m <= 1; m++) l += LOP_MINUS[m + 3] * f[rw + m * d];
+        for (int m = -3; m <= 3; m++) kk += KO6[m + 3] * f[rw + m * d];
+        adv += beta[k] * l * inv_dx;
+        ko += kk;
+    }
+    return (advect ? adv : 0.0) + sigma / 64.0 * inv_dx * ko;
+}
+
 /* FP64 RK4 stage with exact (unquantised) derivatives and Christoffels.
  * state/old/acc/new: 24 pointers to double arrays over the padded layout.
  * inv_dx[d] and dtd[d] per depth. */
@@ -575,7 +657,15 @@ EXPORT void orc_bssn64_stage(const double *const *state, const double *const *ol
                         fd64_cell(state[v], r, S, T, q, x, y, z, inv_dx[depth], d1[v], has_h ? dd[h] : NULL);
                         if (has_h) h++;
                     }
-                    bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd, NULL, &par, k, NULL, NULL);
+                    double advin[BSSN_NV];
+                    if (g_upwind) {
+                        const double beta[3] = {u[18], u[19], u[20]};
+                        for (int v = 0; v < BSSN_NV; v++)
+                            advin[v] = fd64_adv(state[v], r, T, beta, inv_dx[depth], g_ko_sigma,
+                                                v < 18 || par.advect_shift);
+                    }
+                    bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd, NULL, &par, g_upwind ? advin : NULL,
+                               k, NULL, NULL);
                     for (int v = 0; v < BSSN_NV; v++) {
                         double a_out, o;
                         rk4_combine(stage, dtd[depth], old_[v][r], k[v], stage == 0 ? 0.0 : acc[v][r], &a_out, &o);
@@ -612,7 +702,7 @@ EXPORT void orc_bssn64_constraints(const double *const *state, const uint32_t *p
                         fd64_cell(state[v], r, S, T, q, x, y, z, inv_dx[depth], d1[v], has_h ? dd[h] : NULL);
                         if (has_h) h++;
                     }
-                    bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd, NULL, &par, k, &H, M);
+                    bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd, NULL, &par, NULL, k, &H, M);
                     h2 += H * H;
                     m2 += M[0] * M[0] + M[1] * M[1] + M[2] * M[2];
                     hmax = fmax(hmax, fabs(H));

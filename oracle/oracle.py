@@ -309,6 +309,11 @@ _SIGS_BSSN = {
                                 C.c_int]),
     "orc_bssn64_constraints": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P]),
     "orc_fd64_float_coefficients": (None, [C.c_int]),
+    "orc_bssn_point_adv": (None, [_P, _P, _P, _P, _P, _P]),
+    "orc_bssn_stage_from_words_adv": (None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32,
+                                             C.c_int, _P, _P, C.c_int, C.c_int]),
+    "orc_bssn64_set_dissipation": (None, [C.c_double, C.c_int]),
+    "orc_fd_adv": (None, [_P, _P, _P, _P, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int, _f32, _f32, C.c_int]),
     "orc_ghost64": (None, [_P, _P, _P, C.c_int, _P, _P, C.c_int, _u32, _u32, _f32, C.c_int, C.c_int, C.c_int,
                            C.c_int]),
     "orc_copy_back64": (None, [_P, _P, _P, C.c_int, _P, _P, _P, C.c_int, _u32, _u32, C.c_int, C.c_int]),
@@ -335,6 +340,34 @@ def bssn_point(u, d1, dd, G=None, params=BSSN_DEFAULT):
     hm = np.zeros(4)
     lib().orc_bssn_point(_p(u), _p(d1), _p(dd), _p(G), _p(p), _p(rhs), _p(hm))
     return rhs, hm
+
+
+def bssn_point_adv(u, d1, dd, advin, params=BSSN_DEFAULT):
+    """RHS (24) at one point with the advection of every field given
+    (advin[24]: upwinded advection plus dissipation, as the FD writes it)."""
+    u, d1, dd, a = (np.ascontiguousarray(x, np.float64) for x in (u, d1, dd, advin))
+    p = _params(params)
+    rhs = np.zeros(24)
+    lib().orc_bssn_point_adv(_p(u), _p(d1), _p(dd), _p(p), _p(a), _p(rhs))
+    return rhs
+
+
+def fd_adv(field, shift3, positions, tiles, substep, max_depth, S, P, outer_dx, sigma, advect, n_slots):
+    """The FD's advection + dissipation output of one padded float32 field
+    (interior-only float32 array), shift3: the three padded shift fields."""
+    out = np.zeros(n_slots * S ** 3, np.float32)
+    t = _i32arr(tiles)
+    f = np.ascontiguousarray(field, np.float32)
+    b = [np.ascontiguousarray(x, np.float32) for x in shift3]
+    lib().orc_fd_adv(_p(f), _p(b[0]), _p(b[1]), _p(b[2]), _p(out), _p(positions), _p(t), t.size, substep, max_depth,
+                     S, P, outer_dx, sigma, 1 if advect else 0)
+    return out
+
+
+def set_dissipation(sigma, upwind=True):
+    """Upwinded advection with Kreiss-Oliger dissipation of strength sigma in
+    the FP64 tile scheme (bssn64_stage, Bssn64Sim); upwind=False: centred."""
+    lib().orc_bssn64_set_dissipation(float(sigma), 1 if upwind else 0)
 
 
 def ptrs(arrs):
