@@ -191,9 +191,13 @@ class Prolongation:
 
 
 # ---------------------------------------------------------------- workflow
-def _sim_pairs(am, sim, min_depth: int = 1):
+def _sim_pairs(am, sim, min_depth: int = 1, max_tiles: int = 0):
+    """Pairs of every tile of depth >= min_depth, or of max_tiles of them
+    evenly spaced through the slot order when there are more."""
     e = sim.export()
     tiles = [s for s, p in enumerate(e["positions"]) if int(p) != 0xFFFFFFFF and (int(p) >> 27) >= min_depth]
+    if max_tiles and len(tiles) > max_tiles:
+        tiles = [tiles[i] for i in np.linspace(0, len(tiles) - 1, max_tiles).round().astype(int)]
     return PairSet.from_sim(sim, tiles)
 
 
@@ -202,14 +206,14 @@ def harvest(am, outer: float = 32.0, masses=(0.5, 1.0, 2.0), snapshots: int = 3,
     """Training pairs from the plain solver at twice the resolution of the
     puncture run on [-32, 32]^3 (outer = 32 by default): for each puncture
     mass, the tricubic octree run's fields after 0..snapshots-1 coarse steps,
-    every tile of depth >= 2; plus analytic gauge waves (amplitude, time,
+    264 of the tiles of depth >= 2; plus analytic gauge waves (amplitude, time,
     axis) on a uniform depth-2 mesh of one wavelength, every tile of depth
     >= 1."""
     from . import workloads as WL
     sets = []
     g = am.geometry(S, P, 4, outer, 0.25)
     for m in masses:
-        ops, n = WL.puncture_ops(outer, radius=outer / 4)
+        ops, n = WL.puncture_ops(outer)  # the run's mesh at half the size: twice the resolution
         sim = am.BssnSim(g, n, am.bssn_options(periodic=False, interp=am.AM_INTERP_TRICUBIC))
         for p_, c in ops:
             sim.add_node(p_, c)
@@ -221,7 +225,7 @@ def harvest(am, outer: float = 32.0, masses=(0.5, 1.0, 2.0), snapshots: int = 3,
         for k in range(snapshots):
             if k:
                 sim.timestep()
-            sets.append(_sim_pairs(am, sim, 2))
+            sets.append(_sim_pairs(am, sim, 2, max_tiles=264))
         del sim
     for (a, t, axis) in gauge:
         ops, n = WL.uniform_ops(2)
@@ -326,4 +330,4 @@ def _unseen(am, S: int = 16, P: int = 3):
         sim.set_field(1, v, f[v])
     sim.fill_ghosts()
     sim.timestep()
-    return _sim_pairs(am, sim, 2)
+    return _sim_pairs(am, sim, 2, max_tiles=264)

@@ -2,8 +2,8 @@
 
 * single-level mesh: every octant refined to one depth (the 512-tile BSSN
   workload at depth 3);
-* puncture mesh: a uniform base depth, then each tile whose centre lies
-  within radius / 2^k of the origin refined, k levels above the base;
+* puncture mesh: a uniform base depth, then each tile whose box lies within
+  radii[k] of the puncture refined, k levels above the base;
 * data: flat space with seeded noise; a single puncture (Brill-Lindquist
   conformal factor psi = 1 + M / (2 r), r regularised at r_min, chi =
   psi^-4, gt = delta, pre-collapsed lapse alpha = psi^-2, everything else 0).
@@ -33,27 +33,47 @@ def uniform_ops(depth: int):
     return ops, nxt
 
 
-def puncture_ops(outer: float = 64.0, base_depth: int = 2, levels: int = 3, radius: float = 16.0):
-    """add_node calls of the puncture mesh and the slot count.  A tile at
-    base_depth + k (k < levels) is refined when |centre| < radius / 2^k."""
+# The puncture mesh: the puncture off every tile corner, at (2, 2, 2) (in
+# units of M = 1), and a tile at depth 2 + k refined while the distance from
+# the puncture to its box is below PUNCTURE_RADII[k].  On [-32, 32]^3 this
+# gives 1/8/64/160/312/648 tiles at depths 0-5 (1,193 slots): uneven counts
+# per level near the 64/160/320/640 of the octree workload.  A puncture at
+# the origin would give only the symmetric shells (64, 256 or 448 tiles at
+# depth 3).
+PUNCTURE_CENTRE = (2.0, 2.0, 2.0)
+PUNCTURE_RADII = (16.0, 10.5, 8.0)
+
+
+def puncture_ops(outer: float = 64.0, base_depth: int = 2, radii=PUNCTURE_RADII, centre=PUNCTURE_CENTRE):
+    """add_node calls of the puncture mesh and the slot count: a uniform base
+    at base_depth, then a tile at base_depth + k (k < len(radii)) is refined
+    when the distance from `centre` to its box is below radii[k] (lengths in
+    the units of outer, which the radii and centre scale with: they are
+    given for outer = 64)."""
+    scale = outer / 64.0
+    c0 = np.asarray(centre, float) * scale
     ops, nxt = [], 1
-    centre = {0: np.zeros(3)}
-    depth = {0: 0}
+    lo = {0: np.full(3, -outer / 2)}
     frontier = [0]
     d = 0
     while frontier:
         nf = []
+        w = outer / 2 ** d
         for p in frontier:
             k = d - base_depth
-            refine = d < base_depth or (k < levels and np.linalg.norm(centre[p]) < radius / 2 ** k)
+            if d < base_depth:
+                refine = True
+            elif k < len(radii):
+                gap = np.maximum(0.0, np.maximum(lo[p] - c0, c0 - (lo[p] + w)))
+                refine = np.linalg.norm(gap) < radii[k] * scale
+            else:
+                refine = False
             if not refine:
                 continue
-            w = outer / 2 ** d
             for c in range(8):
-                off = np.array([1 if c & 1 else -1, 1 if c & 2 else -1, 1 if c & 4 else -1]) * (w / 4)
+                off = np.array([c & 1, (c >> 1) & 1, (c >> 2) & 1], float) * (w / 2)
                 ops.append((p, c))
-                centre[nxt] = centre[p] + off
-                depth[nxt] = d + 1
+                lo[nxt] = lo[p] + off
                 nf.append(nxt)
                 nxt += 1
         frontier = nf
@@ -77,9 +97,12 @@ def cell_coords(position: int, S: int, P: int, outer: float):
     return x, y, z
 
 
-def puncture_fields(positions, S: int, P: int, outer: float, mass: float = 1.0, r_min: float = 1e-3):
+def puncture_fields(positions, S: int, P: int, outer: float, mass: float = 1.0, r_min: float = 1e-3,
+                    centre=PUNCTURE_CENTRE):
     """24 float32 arrays (slots * T^3) of single-puncture data on every used
-    slot (unused slots hold flat space)."""
+    slot (unused slots hold flat space), the puncture at `centre` (given for
+    outer = 64 and scaled with outer, as in puncture_ops)."""
+    c0 = np.asarray(centre, float) * (outer / 64.0)
     n = len(positions)
     T = S + 2 * P
     out = [np.full((n, T, T, T), FLAT[v], np.float32) for v in range(24)]
@@ -88,6 +111,7 @@ def puncture_fields(positions, S: int, P: int, outer: float, mass: float = 1.0, 
         if pos == 0xFFFFFFFF:
             continue
         x, y, z = cell_coords(pos, S, P, outer)
+        x, y, z = x - c0[0], y - c0[1], z - c0[2]
         r = np.maximum(np.sqrt(x * x + y * y + z * z), r_min)
         psi = 1.0 + mass / (2.0 * r)
         out[0][s] = psi ** -4

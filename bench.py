@@ -37,6 +37,7 @@ METRIC = "BSSN zone-cycles/s + %HBM peak; learned vs plain ghost fill at equal â
 HBM_PEAK_GBS = 8000.0
 S, P, FD_ORDER, OUTER, CFL, DEPTH = 16, 3, 4, 64.0, 0.25, 3
 NOISE = 1e-6
+PUNCTURE_KO_SIGMA = 0.3
 
 # algorithmic bytes per interior cell (each distinct element read once and
 # written once per launch)
@@ -77,24 +78,28 @@ def face_stats(export, S, depth_filter=None):
     return eq / max(wsum, 1), co / max(wsum, 1)
 
 
-def bytes_model(direct_faces, eq_faces, coarse_faces):
+def bytes_model(direct_faces, eq_faces, coarse_faces, upwind=False):
     """Algorithmic bytes per interior cell and launch class.  eq_faces /
     coarse_faces: mean faces per advanced tile with an equal-depth / a
     coarser neighbour (face_stats).  direct_faces is the driver's axis mask
     (0 or 4) of the equal-depth faces that stages 0-2 do not push (the next
     stage's FD reads those ghost planes from the neighbours: the same
-    distinct elements as the pushed copies)."""
-    push = eq_faces * Q * S * S * 24 * 4 / S ** 3  # equal-depth ghost layers written by the stage kernel
+    distinct elements as the pushed copies).  upwind: the FD also writes the
+    upwinded advection + Kreiss-Oliger term per field (read back by the
+    stage kernel), and both the FD's stencils and the push reach P layers."""
+    q = P if upwind else Q
+    adv = 24 * 4 if upwind else 0
+    push = eq_faces * q * S * S * 24 * 4 / S ** 3  # equal-depth ghost layers written by the stage kernel
     push_early = push * (3 - bin(direct_faces & 7).count("1")) / 3
     return {
-        # per stage, 24 fields: the interior and Q ghost layers per face, (S^3 + 6 Q S^2) * 4 per field; 24
-        # gradient + 11 diag + 11 off-diag words
-        "fd": 24 * (S ** 3 + 6 * Q * S * S) * 4 / S ** 3 + 46 * WORD,
-        # per RK4 step, the four stages summed: state 96 and 46 words 184 read in every stage; y_n 96 and acc 96
-        # read in stages 1-3; new 96 written in every stage, acc 96 written in stages 0-2; the pushed face ghosts
-        # (stage 3 pushes every equal-depth face)
+        # per stage, 24 fields: the interior and q ghost layers per face, (S^3 + 6 q S^2) * 4 per field; 24
+        # gradient + 11 diag + 11 off-diag words (and the advection term)
+        "fd": 24 * (S ** 3 + 6 * q * S * S) * 4 / S ** 3 + 46 * WORD + adv,
+        # per RK4 step, the four stages summed: state 96 and 46 words 184 (and the advection term) read in every
+        # stage; y_n 96 and acc 96 read in stages 1-3; new 96 written in every stage, acc 96 written in stages
+        # 0-2; the pushed face ghosts (stage 3 pushes every equal-depth face)
         "rhs_stage": (96 + 184 + 192 + push_early) + (96 + 184 + 96 + 96 + 192 + push_early) * 2
-                     + (96 + 184 + 96 + 96 + 96 + push),
+                     + (96 + 184 + 96 + 96 + 96 + push) + 4 * adv,
         # per stage: P S^2 ghost cells per coarse face, one source read and one write, 24 fields
         "ghost": coarse_faces * 24 * P * S * S * 8 / S ** 3,
         # per step: interior read of new + interior write of old per field (tiles that a finer face reads
@@ -105,11 +110,14 @@ def bytes_model(direct_faces, eq_faces, coarse_faces):
 
 WORKLOADS = {
     "bssn512": "BSSN single level: 512 tiles of 16^3 (128^3) + 73 ancestors, P=3, 24 fp32 fields, 1+log + "
-               "Gamma-driver, flat + 1e-6 noise, cf3 derivative words (Christoffel symbols raised in registers), RK4, "
+               "Gamma-driver, flat + 1e-6 noise, cf3 derivative words (Christoffel symbols raised in registers, not "
+               "stored), RK4, "
                "FD order 4, periodic, [-32,32]^3, CFL 0.25",
-    "puncture": "BSSN octree: single puncture (M=1) on [-32,32]^3, depth-2 base of 64 tiles refined where "
-                "|centre| < 16/2^k to depth 5 (1/8/64/64/64/64 tiles), P=3, 24 fp32 fields, RK4 sub-cycled 2x per "
-                "level, tricubic + quadratic-time coarse-face ghosts, FD order 4, CFL 0.25",
+    "puncture": "BSSN octree: single puncture (M=1) at (2,2,2) on [-32,32]^3, depth-2 base of 64 tiles; a tile "
+                "at depth 2/3/4 is refined while the box distance from the puncture to its cube is below "
+                "16/10.5/8 (1/8/64/160/312/648 tiles to depth 5), P=3, 24 fp32 fields, upwinded advection + "
+                "Kreiss-Oliger sigma=%g, cf3 derivative words (Christoffel symbols raised in registers), RK4 "
+                "sub-cycled 2x per level, tricubic + quadratic-time coarse-face ghosts, FD order 4, CFL 0.25",
 }
 STAGES_PER_STEP = {"fd": 4, "rhs_stage": 1, "ghost": 4, "copy_back": 1}
 # kernel classes launched by this workload (every face of the uniform periodic mesh has an equal-depth target,
@@ -223,14 +231,15 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed coarse steps (default 12; 4 for the puncture, whose run has no Kreiss-Oliger "
-                         "dissipation and stays finite for about 8 coarse steps)")
+                    help="timed coarse steps (default 12; 64 for the puncture, 64 M of evolution)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing pass")
     ap.add_argument("--z-block", type=int, default=0, help="fixed FD z-block (skips the autotuner)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="bssn512")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--ko-sigma", type=float, default=PUNCTURE_KO_SIGMA,
+                    help="Kreiss-Oliger strength of the puncture run (it always upwinds the advection terms)")
     ap.add_argument("--direct-faces", type=int, default=None, choices=(0, 4),
                     help="axis mask (4: z) of the equal-depth faces RK4 stages 0-2 leave to the next FD "
                          "(default: the autotuner's choice)")
@@ -247,7 +256,7 @@ def main():
         sys.exit(2)
 
     if args.steps is None:
-        args.steps = 4 if args.workload == "puncture" else 12
+        args.steps = 64 if args.workload == "puncture" else 12
     if args.copy_probe:
         copy_probe()
         return
@@ -270,6 +279,7 @@ def main():
         sim = am.BssnSim(g, n_slots, am.bssn_options(periodic=False, interp=am.AM_INTERP_TRICUBIC))
         for p_, c in ops:
             sim.add_node(p_, c)
+        sim.set_dissipation(args.ko_sigma, True)
         fields = WL.puncture_fields(sim.export()["positions"], S, P, OUTER)
     for v in range(24):
         sim.set_field(0, v, fields[v])
@@ -303,7 +313,7 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     classes = WORKLOAD_CLASSES if args.workload == "bssn512" else WORKLOAD_CLASSES + ("ghost",)
     eq_faces, coarse_faces = face_stats(sim.export(), S)
-    BYTES_ = bytes_model(sim.direct_faces, eq_faces, coarse_faces)
+    BYTES_ = bytes_model(sim.direct_faces, eq_faces, coarse_faces, upwind=sim.dissipation[1])
     bytes_per_zone = sum(BYTES_[k] * STAGES_PER_STEP[k] for k in classes)
     whole_gbs = zones * bytes_per_zone / elapsed / 1e9
 
@@ -348,7 +358,8 @@ def main():
         "metric": METRIC, "value": round(value, 1), "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": WORKLOADS[args.workload],
+        "config": {"workload": WORKLOADS[args.workload] % ((args.ko_sigma,) if "%g" in WORKLOADS[args.workload]
+                                                           else ()),
                    "tiles": n_slots, "zone_cycles_per_step": zones // max(args.steps, 1),
                    "fine_rk4_steps_per_step": 2 ** sim.counter()[1],
                    "launch": {"z_block": sim.get_launch()[0], "fd_grid_limit": sim.get_launch()[1],

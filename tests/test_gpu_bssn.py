@@ -138,10 +138,11 @@ def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, 
         acc_o = [np.zeros(n * T ** 3) for _ in range(24)]
         new_o = [np.zeros(n * T ** 3) for _ in range(24)]
         sin32 = [np.ascontiguousarray(x, np.float32) for x in state_in]
+        tl = np.array(tiles, np.int32)  # alive through the call
         orc.lib().orc_bssn_stage_from_words(orc.ptrs(sin32), orc.ptrs(old), orc.ptrs(acc_in), orc.ptrs(acc_o),
                                             orc.ptrs(new_o), orc.ptrs(d1), orc.ptrs(d2d), orc.ptrs(d2o),
                                             None, e["positions"].ctypes.data,
-                                            np.array(tiles, np.int32).ctypes.data, len(tiles), 0, e["max_depth"],
+                                            tl.ctypes.data, len(tiles), 0, e["max_depth"],
                                             stage, dtd.ctypes.data, params.ctypes.data, S, P)
         new_g = [sim.get_field(1, v) for v in range(24)]
         acc_g = [sim.get_field(2, v) for v in range(24)]
@@ -322,9 +323,10 @@ def test_bssn_word_formats_match_oracle(am, orc, torch_cuda, fmt):
         acc_o = [np.zeros(n * T ** 3) for _ in range(24)]
         new_o = [np.zeros(n * T ** 3) for _ in range(24)]
         sin32 = [np.ascontiguousarray(x, np.float32) for x in state_in]
+        tl = np.array(tiles, np.int32)  # alive through the call
         orc.lib().orc_bssn_stage_from_words_fmt(orc.ptrs(sin32), orc.ptrs(old), orc.ptrs(acc_in), orc.ptrs(acc_o),
                                                 orc.ptrs(new_o), orc.ptrs(d1), orc.ptrs(d2d), orc.ptrs(d2o),
-                                                e["positions"].ctypes.data, np.array(tiles, np.int32).ctypes.data,
+                                                e["positions"].ctypes.data, tl.ctypes.data,
                                                 len(tiles), 0, e["max_depth"], stage, dtd.ctypes.data,
                                                 params.ctypes.data, S, P, fmt)
         new_g = [sim.get_field(1, v) for v in range(24)]
@@ -345,8 +347,9 @@ def test_bssn_word_formats_match_oracle(am, orc, torch_cuda, fmt):
     st = [sim.get_field(1, v) for v in range(24)]
     d1, d2d, d2o = sim_word_bytes(am, sim, n, fmt)
     out = np.zeros(4 * len(tiles))
+    tl = np.array(tiles, np.int32)  # alive through the call
     orc.lib().orc_bssn_constraints_from_words_fmt(orc.ptrs(st), orc.ptrs(d1), orc.ptrs(d2d), orc.ptrs(d2o),
-                                                  np.array(tiles, np.int32).ctypes.data, len(tiles),
+                                                  tl.ctypes.data, len(tiles),
                                                   params.ctypes.data, S, P, out.ctypes.data, fmt)
     q = out.reshape(-1, 4)
     h_l2, m_l2 = np.sqrt(q[:, 0].sum() / q[:, 2].sum()), np.sqrt(q[:, 1].sum() / q[:, 2].sum())

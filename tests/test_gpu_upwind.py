@@ -160,10 +160,11 @@ def test_bssn_upwind_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, 
         acc_o = [np.zeros(n * T ** 3) for _ in range(24)]
         new_o = [np.zeros(n * T ** 3) for _ in range(24)]
         sin32 = [np.ascontiguousarray(x, np.float32) for x in state_in]
+        tl = np.array(tiles, np.int32)  # alive through the call
         orc.lib().orc_bssn_stage_from_words_adv(orc.ptrs(sin32), orc.ptrs(old), orc.ptrs(acc_in), orc.ptrs(acc_o),
                                                 orc.ptrs(new_o), orc.ptrs(d1), orc.ptrs(d2d), orc.ptrs(d2o),
                                                 orc.ptrs(advg), e["positions"].ctypes.data,
-                                                np.array(tiles, np.int32).ctypes.data, len(tiles), 0,
+                                                tl.ctypes.data, len(tiles), 0,
                                                 e["max_depth"], stage, dtd.ctypes.data, params.ctypes.data, S, P)
         new_g = [sim.get_field(1, v) for v in range(24)]
         acc_g = [sim.get_field(2, v) for v in range(24)]
