@@ -214,6 +214,8 @@ _SIGS = {
     "am_wave_sim_load": (C.c_int, [_P, C.c_char_p]),
     "am_ghost_fill_stage_batched": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), C.POINTER(Geometry), C.c_float,
                                               C.c_int32, C.c_int32, _P]),
+    "am_ghost_fill_stage_faces": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), _P, C.c_int32,
+                                            C.POINTER(Geometry), C.c_float, C.c_int32, C.c_int32, _P]),
     "am_bssn_rhs_stage": (C.c_int, [C.POINTER(BssnFields), C.c_int32, C.POINTER(BssnParams), C.POINTER(Mesh),
                                     C.POINTER(Geometry), C.c_int32, _P]),
     "am_bssn_constraints": (C.c_int, [C.POINTER(BssnFields), C.POINTER(BssnParams), C.POINTER(Mesh),

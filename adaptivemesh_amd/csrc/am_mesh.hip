@@ -117,6 +117,23 @@ __global__ void __launch_bounds__(S *S) k_ghost_fill(FieldBatchDev b, const uint
     }
 }
 
+// The face a workgroup of the multi-field fills handles: blockIdx.x is a
+// (tile list entry, face) pair, or, with a face list, an entry slot * 6 +
+// face of it.
+__device__ __forceinline__ void face_of_block(const int32_t *__restrict__ tiles, const int32_t *__restrict__ faces,
+                                              int &node, int &dir)
+{
+    if (faces) {
+        const int code = faces[blockIdx.x];
+        node = code / 6;
+        dir = code - node * 6;
+    } else {
+        const int ti = blockIdx.x / 6;
+        dir = blockIdx.x - ti * 6;
+        node = tiles[ti];
+    }
+}
+
 // Ghost fill for many fields at once (the RK4 driver's stage fills).  One
 // workgroup per (tile, face, group of VPG fields); each thread owns ghost
 // cells of the face slab, laid out so consecutive lanes follow the slab's
@@ -128,14 +145,15 @@ __global__ void __launch_bounds__(S *S) k_ghost_fill(FieldBatchDev b, const uint
 // the full formula of k_ghost_fill.
 template <int S, int P, int MODE, int VPG, int L>
 __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n_vars, const uint64_t *__restrict__ bnd,
-                                                          const int32_t *__restrict__ tiles, uint32_t substep,
+                                                          const int32_t *__restrict__ tiles,
+                                                          const int32_t *__restrict__ faces, uint32_t substep,
                                                           uint32_t maxd, float frac, int equal_mode, int skip_learned)
 {
     constexpr int T = S + 2 * P, NC = S * S * P;
     constexpr size_t CELLS = tile_stride<S, P, L>();
-    const int ti = blockIdx.x / 6, dir = blockIdx.x - ti * 6;
+    int node, dir;
+    face_of_block(tiles, faces, node, dir);
     const int v0 = blockIdx.y * VPG;
-    const int node = tiles[ti];
     const uint64_t bb = bnd[(size_t)node * 6 + dir];
     const uint32_t nd = (uint32_t)bb;
     const int tgt = bnd_target(nd), cd = bnd_depth(nd);
@@ -282,31 +300,97 @@ __global__ void __launch_bounds__(256) k_ghost_fill_multi(FieldBatchDev b, int n
     }
 }
 
+// One job of the tricubic coarse-face fill: the 4^3 target nodes of each
+// time level (row(lv, j, k, r) gives the 4 nodes along x of row j, plane k of
+// level lv: 0 older, 1 old, 2 new), contracted along x, then y, then z for
+// up to 2 x 2 x 2 children with the weights of their fine indices (a, nn:
+// run start and length per axis), then combined in time: quadratic through
+// older/old/new when `three`, else linear between old and new.
+template <int S, class Row>
+__device__ __forceinline__ void tricubic_children(const Row &row, bool three, const float (&s_w)[3][S][4],
+                                                  const int (&a)[3], const int (&nn)[3], float ty, float c0, float c1,
+                                                  float c2, float (&val)[2][2][2])
+{
+    float wx[2][4], wy[2][4], wz[2][4];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            wx[c][m] = s_w[0][a[0] + min(c, nn[0] - 1)][m];
+            wy[c][m] = s_w[1][a[1] + min(c, nn[1] - 1)][m];
+            wz[c][m] = s_w[2][a[2] + min(c, nn[2] - 1)][m];
+        }
+    // time combination accumulated level by level, in the order of
+    // older * c0 + old * c1 + new * c2 (or old * (1 - ty) + new * ty)
+    float acc[2][2][2];
+    const float coef[3] = {c0, three ? c1 : 1.f - ty, three ? c2 : ty};
+#pragma unroll 1
+    for (int lv = three ? 0 : 1; lv < 3; lv++) {
+        float ax[2][4][4];  // child x, k, j
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float r[4];
+                row(lv, j, k, r);
+#pragma unroll
+                for (int cx = 0; cx < 2; cx++)
+                    ax[cx][k][j] = r[0] * wx[cx][0] + r[1] * wx[cx][1] + r[2] * wx[cx][2] + r[3] * wx[cx][3];
+            }
+        const float cl = coef[lv];
+        const bool first = lv == (three ? 0 : 1);
+#pragma unroll
+        for (int cx = 0; cx < 2; cx++)
+#pragma unroll
+            for (int cy = 0; cy < 2; cy++) {
+                float accz[2] = {0.f, 0.f};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    float ay = 0.f;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) ay = ay + ax[cx][k][j] * wy[cy][j];
+#pragma unroll
+                    for (int cz = 0; cz < 2; cz++) accz[cz] = accz[cz] + ay * wz[cz][k];
+                }
+#pragma unroll
+                for (int cz = 0; cz < 2; cz++)
+                    acc[cx][cy][cz] = first ? accz[cz] * cl : acc[cx][cy][cz] + accz[cz] * cl;
+            }
+    }
+#pragma unroll
+    for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+        for (int cy = 0; cy < 2; cy++)
+#pragma unroll
+            for (int cx = 0; cx < 2; cx++) val[cx][cy][cz] = acc[cx][cy][cz];
+}
+
 // Tricubic coarse-face fill, stencils shared between fine cells.  Along
 // each axis the fine ghost cells of a face map onto the target's clamped
 // stencil bases ri (AMR.h:736-745 positions); fine cells with the same base
 // on every axis -- usually 2 x 2 x 2 children of one coarse cell -- read the
-// same 4^3 nodes.  One thread gathers those nodes once per time level and
+// same 4^3 nodes.  One thread reads those nodes once per time level and
 // contracts them along x, then y, then z for each child's weights, which is
 // the same sequence of operations interp_tricubic performs for each child
-// alone, so every value is bit-identical to the per-cell fill while the
-// gathers drop about five-fold.  Where clamping at tile edges gives more
-// than 2 fine cells along an axis the same base, they are split into groups
-// of at most 2, so every thread's work stays 2 x 2 x 2 children at most.
-// L: tile layout.  In the interior-first layout the 4 x 4 x 4 nodes of a
-// non-root target are interior cells, rows of S with planes of S^2; the
-// root (whose ghost cells a fill may read) is addressed cell by cell.
+// alone, so every value is bit-identical to the per-cell fill.  Where
+// clamping at tile edges gives more than 2 fine cells along an axis the
+// same base, they are split into groups of at most 2, so every thread's
+// work stays 2 x 2 x 2 children at most.  The target nodes a face reads
+// are staged in LDS for a group of fields and every time level, each node
+// read from HBM once with whole-row reads; the jobs then read LDS.  L: tile
+// layout (tile_cell addresses the staged block in either).
 template <int S, int P, int VPG, int L>
 __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_vars, const uint64_t *__restrict__ bnd,
-                                                        const int32_t *__restrict__ tiles, uint32_t substep,
+                                                        const int32_t *__restrict__ tiles,
+                                                        const int32_t *__restrict__ faces, uint32_t substep,
                                                         uint32_t maxd, float frac, int equal_mode)
 {
     constexpr int T = S + 2 * P, NC = S * S * P;
     constexpr size_t CELLS = tile_stride<S, P, L>();
     constexpr uint32_t uP = P, uS = S;
-    const int ti = blockIdx.x / 6, dir = blockIdx.x - ti * 6;
+    int node, dir;
+    face_of_block(tiles, faces, node, dir);
     const int v0 = blockIdx.y * VPG;
-    const int node = tiles[ti];
     const uint64_t bb = bnd[(size_t)node * 6 + dir];
     const uint32_t nd = (uint32_t)bb;
     const int tgt = bnd_target(nd), cd = bnd_depth(nd);
@@ -396,120 +480,142 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
     const int items = nr0 * nr1 * nr2;
     const bool quad = ddiff > 0;  // with older: quadratic in time
     const float c0 = .5f * ty * (ty - 1.f), c1 = (1.f - ty) * (1.f + ty), c2 = .5f * ty * (ty + 1.f);
-    // row and plane pitch of the stencil rows: the padded tile, or the
-    // interior block of the interior-first layout (non-root targets)
-    constexpr int PY = L == 0 ? T : S, PZ = L == 0 ? T * T : S * S;
-    const bool by_cell = L == 1 && tgt == 0;  // uniform per workgroup
-    for (int job = threadIdx.x; job < items * nv; job += 256) {
-        const int v = job / items, it = job - v * items;
+    // one job: the run starts and lengths of its children along each axis
+    auto decode = [&](int it, int (&a)[3], int (&nn)[3]) {
         const int q0 = it % nr0, q1 = (it / nr0) % nr1, q2 = it / (nr0 * nr1);
-        const int a0 = s_run0[0][q0], a1 = s_run0[1][q1], a2 = s_run0[2][q2];
-        const int n0 = s_runn[0][q0], n1 = s_runn[1][q1], n2 = s_runn[2][q2];
-        const uint32_t n0x = s_ri[0][a0], n0y = s_ri[1][a1], n0z = s_ri[2][a2];
-        const size_t rn = (size_t)tgt * CELLS +
-                          (L == 0 ? tile_cell<S, P, 0>(n0x, n0y, n0z)
-                                  : (by_cell ? 0u : tile_interior<S, P, L>(n0x - P, n0y - P, n0z - P)));
-        const Cells<S, P, L> cn{(size_t)tgt * CELLS, n0x, n0y, n0z};
-        const float *dold = b.old_[v0 + v], *dnew = b.new_[v0 + v], *dolder = b.older[v0 + v];
-        const bool three = dolder && quad;
-        float wx[2][4], wy[2][4], wz[2][4];
-#pragma unroll
-        for (int c = 0; c < 2; c++)
-#pragma unroll
-            for (int m = 0; m < 4; m++) {
-                wx[c][m] = s_w[0][a0 + min(c, n0 - 1)][m];
-                wy[c][m] = s_w[1][a1 + min(c, n1 - 1)][m];
-                wz[c][m] = s_w[2][a2 + min(c, n2 - 1)][m];
-            }
-        float res[3][2][2][2];  // time level (older, old, new) x child
-#pragma unroll
-        for (int lv = 0; lv < 3; lv++) {
-            if (lv == 0 && !three) continue;
-            const float *d = lv == 0 ? dolder : (lv == 1 ? dold : dnew);
-            float ax[2][4][4];  // child x, k, j
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    float r0, r1, r2, r3;
-                    if (by_cell) {
-                        r0 = d[cn.at(-1, j - 1, k - 1)];
-                        r1 = d[cn.at(0, j - 1, k - 1)];
-                        r2 = d[cn.at(1, j - 1, k - 1)];
-                        r3 = d[cn.at(2, j - 1, k - 1)];
-                    } else {
-                        const float *r = d + rn + (ptrdiff_t)(k - 1) * PZ + (ptrdiff_t)(j - 1) * PY - 1;
-                        r0 = r[0]; r1 = r[1]; r2 = r[2]; r3 = r[3];
-                    }
-#pragma unroll
-                    for (int cx = 0; cx < 2; cx++)
-                        ax[cx][k][j] = r0 * wx[cx][0] + r1 * wx[cx][1] + r2 * wx[cx][2] + r3 * wx[cx][3];
-                }
-#pragma unroll
-            for (int cx = 0; cx < 2; cx++)
-#pragma unroll
-                for (int cy = 0; cy < 2; cy++) {
-                    float accz[2] = {0.f, 0.f};
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        float ay = 0.f;
-#pragma unroll
-                        for (int j = 0; j < 4; j++) ay = ay + ax[cx][k][j] * wy[cy][j];
-#pragma unroll
-                        for (int cz = 0; cz < 2; cz++) accz[cz] = accz[cz] + ay * wz[cz][k];
-                    }
-                    res[lv][cx][cy][0] = accz[0];
-                    res[lv][cx][cy][1] = accz[1];
-                }
-        }
+        a[0] = s_run0[0][q0], a[1] = s_run0[1][q1], a[2] = s_run0[2][q2];
+        nn[0] = s_runn[0][q0], nn[1] = s_runn[1][q1], nn[2] = s_runn[2][q2];
+    };
+    auto store = [&](int v, const int (&a)[3], const int (&nn)[3], const float (&val)[2][2][2]) {
 #pragma unroll
         for (int cz = 0; cz < 2; cz++)
 #pragma unroll
             for (int cy = 0; cy < 2; cy++)
 #pragma unroll
                 for (int cx = 0; cx < 2; cx++) {
-                    if (cx >= n0 || cy >= n1 || cz >= n2) continue;
-                    const float o = res[1][cx][cy][cz], nw = res[2][cx][cy][cz];
-                    const float val = three ? res[0][cx][cy][cz] * c0 + o * c1 + nw * c2 : o * (1.f - ty) + nw * ty;
-                    const size_t wn = (size_t)node * CELLS + tile_cell<S, P, L>((uint32_t)(wlo[0] + a0 + cx),
-                                                                                 (uint32_t)(wlo[1] + a1 + cy),
-                                                                                 (uint32_t)(wlo[2] + a2 + cz));
-                    b.new_[v0 + v][wn] = val;
+                    if (cx >= nn[0] || cy >= nn[1] || cz >= nn[2]) continue;
+                    const size_t wn = (size_t)node * CELLS + tile_cell<S, P, L>((uint32_t)(wlo[0] + a[0] + cx),
+                                                                                 (uint32_t)(wlo[1] + a[1] + cy),
+                                                                                 (uint32_t)(wlo[2] + a[2] + cz));
+                    b.new_[v][wn] = val[cx][cy][cz];
                 }
+    };
+
+    // The target nodes the face's stencils read span, per axis, the first
+    // base - 1 to the last base + 2 (the bases rise with the fine index).
+    // Each group of fields stages that block for every time level in LDS,
+    // each node read once with whole-row reads, and the jobs read LDS.  For
+    // n fine cells along an axis the bases spread over at most
+    // ceil((n - 1) / 2^ddiff) + 1 values, and over at most S - 3 (T - 3 for
+    // the root) after clamping, so a block holds at most 12 x 12 x 6 nodes
+    // one level coarser and (S + 4)^2 (P + 4) at equal depth (formula rule)
+    // for S = 16, P = 3; the LDS holds the larger of 2 fields x 3 levels of
+    // the first and 1 field x 2 levels of the second.
+    constexpr int ET = (S + 1) / 2 + 4, EN = P / 2 + 5;
+    constexpr int CAP = 2 * 3 * ET * ET * EN > 2 * (S + 4) * (S + 4) * (P + 4) ? 2 * 3 * ET * ET * EN
+                                                                                : 2 * (S + 4) * (S + 4) * (P + 4);
+    __shared__ float s_reg[CAP];
+    int lo[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) lo[k] = (int)s_ri[k][0] - 1;
+    const int E0 = (int)s_ri[0][cnt[0] - 1] + 3 - lo[0], E1 = (int)s_ri[1][cnt[1] - 1] + 3 - lo[1],
+              E2 = (int)s_ri[2][cnt[2] - 1] + 3 - lo[2];
+    const int EV = E0 * E1 * E2;
+    const int lv0 = quad ? 0 : 1, nlev = 3 - lv0;  // levels staged: (older,) old, new
+    const int per = min(nv, CAP / (nlev * EV));    // fields per group (>= 1 by the bounds above)
+    // staging in batches of 8 elements per thread, all 8 loads issued before
+    // their LDS writes; element i -> (field, level, z, y, x) by divisions
+    // through float reciprocals, (i + 1/2) / d rounded down, exact for
+    // i + d < 2^22
+    constexpr int BATCH = 8;
+    const int per_f = nlev * EV;
+    const float inv_pf = 1.f / (float)per_f, inv_ev = 1.f / (float)EV, inv_e0 = 1.f / (float)E0,
+                inv_e1 = 1.f / (float)E1;
+    auto qdiv = [](int i, float inv) { return (int)(((float)i + .5f) * inv); };
+    for (int vg = 0; vg < nv; vg += per) {
+        const int nf = min(per, nv - vg), total = nf * per_f;
+#pragma unroll 1
+        for (int i0 = 0; i0 < total; i0 += BATCH * 256) {
+            float tmp[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; u++) {
+                const int i = i0 + (int)threadIdx.x + u * 256;
+                tmp[u] = 0.f;
+                if (i < total) {
+                    const int f = qdiv(i, inv_pf), r = i - f * per_f;
+                    const int l = qdiv(r, inv_ev), c = r - l * EV;
+                    const int yz = qdiv(c, inv_e0), x = c - yz * E0;
+                    const int z = qdiv(yz, inv_e1), y = yz - z * E1;
+                    const int v = v0 + vg + f, lv = lv0 + l;
+                    const float *d = lv == 0 ? b.older[v] : (lv == 1 ? b.old_[v] : b.new_[v]);
+                    if (d)  // no older level: linear in time, never read
+                        tmp[u] = d[(size_t)tgt * CELLS + tile_cell<S, P, L>((uint32_t)(lo[0] + x),
+                                                                             (uint32_t)(lo[1] + y),
+                                                                             (uint32_t)(lo[2] + z))];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < BATCH; u++) {
+                const int i = i0 + (int)threadIdx.x + u * 256;
+                if (i < total) s_reg[i] = tmp[u];
+            }
+        }
+        __syncthreads();
+        for (int job = threadIdx.x; job < items * nf; job += 256) {
+            const int f = job / items;
+            int a[3], nn[3];
+            decode(job - f * items, a, nn);
+            const int v = v0 + vg + f;
+            const int off = (((int)s_ri[2][a[2]] - 1 - lo[2]) * E1 + ((int)s_ri[1][a[1]] - 1 - lo[1])) * E0 +
+                            ((int)s_ri[0][a[0]] - 1 - lo[0]);
+            const float *sr = s_reg + f * nlev * EV + off - lv0 * EV;
+            float val[2][2][2];
+            tricubic_children(
+                [&](int lv, int j, int k, float (&r)[4]) {
+                    const float *q = sr + lv * EV + (k * E1 + j) * E0;
+                    r[0] = q[0]; r[1] = q[1]; r[2] = q[2]; r[3] = q[3];
+                },
+                quad && b.older[v] != nullptr, s_w, a, nn, ty, c0, c1, c2, val);
+            store(v, a, nn, val);
+        }
+        __syncthreads();
     }
 }
 
 constexpr int GHOST_VPG = 6;  // fields per workgroup in the multi-field fill
+constexpr int TRICUBIC_VPG = 2;  // fields per workgroup in the tricubic fill
 
 template <int S, int P, int L>
-static int launch_ghost_multi_l(const FieldBatchDev &d, int n_vars, const am_mesh *mesh, int interp, float frac,
-                                int equal_mode, int skip_learned, void *stream)
+static int launch_ghost_multi_l(const FieldBatchDev &d, int n_vars, const am_mesh *mesh, const int32_t *faces,
+                                int n_faces, int interp, float frac, int equal_mode, int skip_learned, void *stream)
 {
-    dim3 grid(mesh->n_tiles * 6, (n_vars + GHOST_VPG - 1) / GHOST_VPG);
+    const unsigned items = faces ? (unsigned)n_faces : (unsigned)mesh->n_tiles * 6u;
+    const dim3 grid(items, (n_vars + GHOST_VPG - 1) / GHOST_VPG);
     hipStream_t st = (hipStream_t)stream;
     if (interp == AM_INTERP_LAGRANGE)
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_LAGRANGE, GHOST_VPG, L>), grid, dim3(256), 0, st, d,
-                           n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, equal_mode,
-                           skip_learned);
+                           n_vars, mesh->boundaries, mesh->tiles, faces, mesh->substep, mesh->max_depth, frac,
+                           equal_mode, skip_learned);
     else if (interp == AM_INTERP_TRILINEAR)
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRILINEAR, GHOST_VPG, L>), grid, dim3(256), 0, st, d,
-                           n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, equal_mode,
-                           skip_learned);
+                           n_vars, mesh->boundaries, mesh->tiles, faces, mesh->substep, mesh->max_depth, frac,
+                           equal_mode, skip_learned);
     else if (interp == AM_INTERP_TRICUBIC && !skip_learned)
-        hipLaunchKernelGGL((k_ghost_tricubic<S, P, GHOST_VPG, L>), grid, dim3(256), 0, st, d, n_vars,
-                           mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, equal_mode);
+        hipLaunchKernelGGL((k_ghost_tricubic<S, P, TRICUBIC_VPG, L>),
+                           dim3(items, (n_vars + TRICUBIC_VPG - 1) / TRICUBIC_VPG), dim3(256), 0, st, d, n_vars,
+                           mesh->boundaries, mesh->tiles, faces, mesh->substep, mesh->max_depth, frac, equal_mode);
     else if (interp == AM_INTERP_TRICUBIC)
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRICUBIC, GHOST_VPG, L>), grid, dim3(256), 0, st, d,
-                           n_vars, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, equal_mode,
-                           skip_learned);
+                           n_vars, mesh->boundaries, mesh->tiles, faces, mesh->substep, mesh->max_depth, frac,
+                           equal_mode, skip_learned);
     else
         return set_error(AM_ERR_INVALID, "am_ghost_fill: unknown interpolation mode %d", interp);
     return check_launch("k_ghost_fill_multi");
 }
 
 template <int S, int P>
-static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, int interp, float frac, int equal_mode,
-                              int skip_learned, void *stream)
+static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, const int32_t *faces, int n_faces,
+                              int interp, float frac, int equal_mode, int skip_learned, void *stream)
 {
     FieldBatchDev d;
     for (int v = 0; v < b->n_vars; v++) {
@@ -518,8 +624,8 @@ static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, int 
         d.new_[v] = b->new_[v];
         d.older[v] = b->older[v];
     }
-    AM_DISPATCH_LAYOUT(b->tile_layout, launch_ghost_multi_l, d, b->n_vars, mesh, interp, frac, equal_mode, skip_learned,
-                       stream);
+    AM_DISPATCH_LAYOUT(b->tile_layout, launch_ghost_multi_l, d, b->n_vars, mesh, faces, n_faces, interp, frac,
+                       equal_mode, skip_learned, stream);
 }
 
 template <int S, int P>
@@ -898,13 +1004,32 @@ extern "C" int am_ghost_fill_stage_batched(const am_field_batch *b, const am_mes
         return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_batched: stage_frac %g", (double)stage_frac);
     if (int rc = am::mesh_checks(mesh, g, true, false)) return rc;
     if (b->n_vars == 0 || mesh->n_tiles == 0) return AM_OK;
-    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, interp, stage_frac, equal_mode, 0, stream);
+    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, nullptr, 0, interp, stage_frac, equal_mode, 0, stream);
+}
+
+extern "C" int am_ghost_fill_stage_faces(const am_field_batch *b, const am_mesh *mesh, const int32_t *faces,
+                                         int32_t n_faces, const am_geometry *g, float stage_frac, int32_t interp,
+                                         int32_t equal_mode, void *stream)
+{
+    if (equal_mode < AM_EQUAL_FORMULA || equal_mode > AM_EQUAL_SKIP)
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_faces: equal_mode %d", equal_mode);
+    if (!b) return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_faces: null batch");
+    if (b->n_vars < 0 || b->n_vars > AM_MAX_VARS)
+        return am::set_error(AM_ERR_TOO_MANY_VARS, "am_ghost_fill_stage_faces: n_vars %d", b->n_vars);
+    if (!(stage_frac >= 0.f && stage_frac <= 1.f))
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_faces: stage_frac %g", (double)stage_frac);
+    if (n_faces < 0 || (n_faces > 0 && !faces))
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_stage_faces: %d faces", n_faces);
+    if (int rc = am::mesh_checks(mesh, g, true, false)) return rc;
+    if (b->n_vars == 0 || n_faces == 0) return AM_OK;
+    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, faces, n_faces, interp, stage_frac, equal_mode, 0, stream);
 }
 
 int am::ghost_stage_fill(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, float stage_frac,
                          int equal_mode, void *stream)
 {
-    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, AM_INTERP_TRICUBIC, stage_frac, equal_mode, 0, stream);
+    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, nullptr, 0, AM_INTERP_TRICUBIC, stage_frac, equal_mode, 0,
+                     stream);
 }
 
 extern "C" int am_ghost_fill(float *old_, float *new_, const float *older, const am_mesh *mesh, const am_geometry *g,

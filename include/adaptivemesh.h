@@ -386,6 +386,17 @@ int am_christoffel(const float *const *metric6, const uint32_t *const *dmetric6,
 enum am_equal_mode { AM_EQUAL_FORMULA = 0, AM_EQUAL_COPY = 1, AM_EQUAL_SKIP = 2 };
 int am_ghost_fill_stage_batched(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, float stage_frac,
                                 int32_t interp, int32_t equal_mode, void *stream);
+/* The same fill over a list of faces instead of every face of mesh->tiles:
+ * faces[i] = slot * 6 + face (face 0-5: +x, +y, +z, -x, -y, -z); each
+ * listed face is filled exactly as am_ghost_fill_stage_batched fills it, so
+ * a list holding every face whose target is coarser equals that call with
+ * AM_EQUAL_SKIP.  The slots must be used slots of the mesh's octree.  No
+ * counterpart in the reference, which launches one thread block per tile
+ * face and leaves the faces with nothing to fill at once (AMR.h:702-762);
+ * the list spares those launches. */
+int am_ghost_fill_stage_faces(const am_field_batch *b, const am_mesh *mesh, const int32_t *faces, int32_t n_faces,
+                              const am_geometry *g, float stage_frac, int32_t interp, int32_t equal_mode,
+                              void *stream);
 
 /* ----------------------------------------------------------- BSSN scheme
  * No counterpart in the reference (BSSN.h has no right-hand side or time

@@ -246,6 +246,9 @@ struct octree {
     int32_t list_offset[depth_limit + 3] = {};
     int32_t list_count[depth_limit + 3] = {};
     int32_t coarse_faces[depth_limit + 1] = {};  // faces per depth whose target is coarser
+    // device lists of those faces (slot * 6 + face), depth by depth
+    smart_gpu_cpu_buffer<int32_t> face_lists;
+    int32_t face_offset[depth_limit + 1] = {};
     // per slot: 1 when some finer tile's face reads this slot (its ghost cells
     // feed the interpolation), else 0
     smart_gpu_cpu_buffer<uint8_t> coarse_source;
@@ -261,7 +264,7 @@ struct octree {
     explicit octree(uint32_t node_slots, bool periodic_ = false)
         : parent_idx_b(node_slots), children_idx_b(node_slots * 8u), positions(node_slots),
           boundaries(node_slots * 6u), lifetime(node_slots), max_slots(node_slots), periodic(periodic_),
-          tile_lists(node_slots * 3u), coarse_source(node_slots)
+          tile_lists(node_slots * 3u), face_lists(node_slots * 6u), coarse_source(node_slots)
     {
         memset(parent_idx_b.cpu_buffer_ptr, -1, parent_idx_b.total_size());
         memset(children_idx_b.cpu_buffer_ptr, -1, children_idx_b.total_size());
@@ -279,6 +282,7 @@ struct octree {
         lifetime.destroy();
         tile_lists.destroy();
         coarse_source.destroy();
+        face_lists.destroy();
     }
 
     uint32_t read_first_free_slot() const { return first_free_slot; }
@@ -502,18 +506,22 @@ struct octree {
             if (leaf) L[n++] = (int32_t)i;
         }
         list_count[depth_limit + 2] = n - list_offset[depth_limit + 2];
-        for (uint32_t d = 0; d <= depth_limit; d++) coarse_faces[d] = 0;
         std::fill(coarse_source.cpu_buffer_ptr, coarse_source.cpu_buffer_ptr + max_slots, (uint8_t)0);
-        for (uint32_t i = 0; i <= final_used_slot; i++) {
-            const int d = positions.cpu_buffer_ptr[i].depth();
-            if (d <= 0) continue;
-            for (uint32_t f = 0; f < 6; f++) {
-                const domain_boundary &b = boundaries.cpu_buffer_ptr[i * 6u + f];
-                if (b.target_idx() >= 0 && b.depth_diff() > 0) {
-                    coarse_faces[d]++;
-                    coarse_source.cpu_buffer_ptr[b.target_idx()] = 1;
+        int32_t nf = 0;
+        for (uint32_t d = 0; d <= depth_limit; d++) {
+            face_offset[d] = nf;
+            for (int32_t k = 0; k < list_count[d + 1]; k++) {
+                const int32_t i = L[list_offset[d + 1] + k];
+                if (d == 0) continue;
+                for (uint32_t f = 0; f < 6; f++) {
+                    const domain_boundary &b = boundaries.cpu_buffer_ptr[(uint32_t)i * 6u + f];
+                    if (b.target_idx() >= 0 && b.depth_diff() > 0) {
+                        face_lists.cpu_buffer_ptr[nf++] = i * 6 + (int32_t)f;
+                        coarse_source.cpu_buffer_ptr[b.target_idx()] = 1;
+                    }
                 }
             }
+            coarse_faces[d] = nf - face_offset[d];
         }
         parent_idx_b.copy_to_gpu(stream);
         children_idx_b.copy_to_gpu(stream);
@@ -521,10 +529,17 @@ struct octree {
         boundaries.copy_to_gpu(stream);
         tile_lists.copy_to_gpu(stream);
         coarse_source.copy_to_gpu(stream);
+        face_lists.copy_to_gpu(stream);
         dirty = false;
     }
     int tiles_at_depth(int d) const { return list_count[d + 1]; }
     int coarse_faces_at_depth(int d) const { return coarse_faces[d]; }
+    // the device list of those faces (slot * 6 + face)
+    const int32_t *coarse_face_list(int d)
+    {
+        copy_to_gpu();
+        return face_lists.gpu_buffer_ptr + face_offset[d];
+    }
     int leaf_count() const { return list_count[depth_limit + 2]; }
 
     // Kernel view over every used slot (the reference's launch domain) ...

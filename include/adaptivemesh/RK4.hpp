@@ -482,7 +482,12 @@ struct rk4_AMR_data {
         if (d > 0 && parent.coarse_faces_at_depth(d) > 0) {
             am_field_batch b = batch(true);
             ev = timer.begin(kernel_timer::GHOST);
-            ghost_fill(b, m, frac[s], AM_EQUAL_SKIP);
+            if (opt.interp == AM_INTERP_LEARNED)
+                ghost_fill(b, m, frac[s], AM_EQUAL_SKIP);
+            else  // only the faces whose target is coarser have cells to fill
+                check(am_ghost_fill_stage_faces(&b, &m, parent.coarse_face_list(d), parent.coarse_faces_at_depth(d),
+                                                &geom, frac[s], opt.interp, AM_EQUAL_SKIP, stream),
+                      "ghost");
             timer.end(ev);
         }
     }

@@ -283,6 +283,73 @@ def test_ghost_fill_stage_equal_modes(am, orc, torch_cuda, S, P, mode, equal, re
             assert np.array_equal(got, new.view(np.uint32)), (substep, frac)
 
 
+@pytest.mark.parametrize("S,P", [(28, 2), (16, 3)])
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("layout", [0, 1])
+def test_ghost_fill_stage_faces_equals_full(am, orc, torch_cuda, S, P, mode, layout):
+    """am_ghost_fill_stage_faces over the list of a depth's faces whose
+    target is coarser (the RK4 driver's call), with equal-depth and outer
+    faces mixed in, writes what am_ghost_fill_stage_batched writes with
+    AM_EQUAL_SKIP over all the depth's tiles, bit for bit, on the graded
+    mesh in both tile layouts; the list argument is checked."""
+    torch = torch_cuda
+    print(f"face-list fill S={S} P={P} mode={mode} layout={layout}", flush=True)
+    g = am.geometry(S, P, 2 * P, 100.0, 0.4)
+    tree, e, n = setup_mesh(am, orc, "graded", False)
+    T = S + 2 * P
+    W = n * int(am.lib().am_tile_stride(C.byref(g), layout))
+    nv = 5
+    rng = np.random.default_rng(S + mode + 10 * layout)
+    depths = {}
+    for s in fields.used_slots(e):
+        depths.setdefault(fields.depth_of(e, s), []).append(s)
+    lists = {}
+    for d, ts in depths.items():
+        coarse, other = [], []
+        for t in ts:
+            for f in range(6):
+                nd = int(e["boundaries"][t * 6 + f]) & 0xFFFFFFFF
+                is_coarse = not (nd & 0x800000) and nd >> 28 not in (0, 15) and (nd >> 24) & 15
+                (coarse if is_coarse else other).append(t * 6 + f)
+        lists[d] = (coarse, other)
+    assert sum(len(c) for c, _ in lists.values()) > 0
+    for substep in range(4):
+        src = [rng.standard_normal((nv, W)).astype(np.float32) for _ in range(3)]
+        dev_a = [dev(torch, x) for x in src]
+        dev_b = [dev(torch, x) for x in src]
+        fbs = []
+        for dv in (dev_a, dev_b):
+            fb = am.FieldBatch()
+            fb.n_vars = nv
+            fb.tile_layout = layout
+            for v in range(nv):
+                fb.old_[v], fb.new_[v] = dv[0][v].data_ptr(), dv[1][v].data_ptr()
+                fb.older[v] = dv[2][v].data_ptr() if mode == 2 else None
+            fbs.append(fb)
+        for d in sorted(depths):
+            if d == 0:
+                continue
+            m = mesh_for(tree, d, substep)
+            am.check(am.lib().am_ghost_fill_stage_batched(C.byref(fbs[0]), C.byref(m), C.byref(g), C.c_float(0.5),
+                                                          mode, am.AM_EQUAL_SKIP, am.stream_handle()), "full")
+            coarse, other = lists[d]
+            fl = np.array(coarse + other[::3], np.int32)
+            rng.shuffle(fl)
+            dfl = torch.from_numpy(fl).cuda()
+            am.check(am.lib().am_ghost_fill_stage_faces(C.byref(fbs[1]), C.byref(m), dfl.data_ptr(), len(fl),
+                                                        C.byref(g), C.c_float(0.5), mode, am.AM_EQUAL_SKIP,
+                                                        am.stream_handle()), "faces")
+            sync(torch)
+        a, b = host(dev_a[1], np.float32), host(dev_b[1], np.float32)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), substep
+        assert not np.array_equal(a.view(np.uint32), src[1].view(np.uint32))
+    m = mesh_for(tree, 1, 0)
+    assert am.lib().am_ghost_fill_stage_faces(C.byref(fbs[1]), C.byref(m), None, 3, C.byref(g), C.c_float(0.5),
+                                              mode, am.AM_EQUAL_SKIP, am.stream_handle()) == -1
+    assert am.lib().am_ghost_fill_stage_faces(C.byref(fbs[1]), C.byref(m), None, 0, C.byref(g), C.c_float(0.5),
+                                              mode, am.AM_EQUAL_SKIP, am.stream_handle()) == 0
+
+
 @pytest.mark.parametrize("S,P", [(16, 3), (16, 2), (28, 2)])
 def test_copy_new_to_old_bit_exact(am, orc, torch_cuda, S, P):
     torch = torch_cuda
