@@ -150,58 +150,144 @@ def initial_data(n_slots, T):
     return out
 
 
-def cpu_baseline(n_slots, fields, positions, tiles_all, budget_s=12.0):
-    """The oracle (a port of the scheme: bit-exact FP32 stencil words, FP64
-    RHS with the Christoffel symbols raised from the metric gradient words)
-    running one RK4 step on a sample of depth-3 tiles on this host's cores."""
+def cpu_baseline(n_slots, fields, export, reps=5, target_s=2.0):
+    """The FP32 host implementation of the scheme (oracle/am_cpu_fp32.c,
+    OpenMP on this host's cores): one RK4 step -- per stage, derivatives of
+    order 4 straight from the fp32 state, the BSSN right-hand side and the
+    RK4 update in one pass per cell, then the equal-depth face ghosts of the
+    stage output refreshed from the neighbours -- over a sample of the 512
+    depth-3 tiles sized so a step takes about target_s; one warm-up step,
+    then the median of `reps` timed steps.  Only that scheme work is in the
+    timed region (no allocation, no Python array work)."""
+    from oracle import cpu_fp32 as CPU
     from oracle import oracle as orc
-    T = S + 2 * P
+    pos, bnd = export["positions"], export["boundaries"]
+    tiles_all = np.array([t for t in range(n_slots) if (int(pos[t]) >> 27) == DEPTH and int(pos[t]) != 0xFFFFFFFF],
+                         np.int32)
     outer_dx = np.float32(OUTER) / np.float32(S)
     inv_dx = float(np.float32(2 ** DEPTH) / outer_dx)
-    dtd = np.array([float(np.float32(outer_dx * np.float32(CFL)) / 2 ** d) for d in range(9)])
+    dt = float(np.float32(outer_dx * np.float32(CFL)) / 2 ** DEPTH)
     params = np.array(orc.BSSN_DEFAULT, np.float64)
-    tiles = np.array([t for t in tiles_all if (int(positions[t]) >> 27) == DEPTH], np.int32)
-    threads = int(orc.lib().orc_num_threads())
-    seed = 12345
+    yn = [np.ascontiguousarray(f, np.float32) for f in fields]
+    a = [x.copy() for x in yn]
+    b = [x.copy() for x in yn]
+    acc = [np.zeros_like(x) for x in yn]
+    threads = int(CPU.lib().cpu32_max_threads())
 
-    def one_step(sample):
-        state = list(fields)
-        acc = [np.zeros(n_slots * T ** 3) for _ in range(24)]
-        new = [np.zeros(n_slots * T ** 3) for _ in range(24)]
-        for stage in range(4):
-            d1, d2d, d2o = [], [], []
-            for v in range(24):
-                a, b = orc.differentiate(state[v], positions, sample, 0, DEPTH, seed + v, S, P, FD_ORDER // 2,
-                                         float(outer_dx), n_slots, with_d2=v in orc.HESS_VARS)
-                d1.append(a)
-                if v in orc.HESS_VARS:
-                    d2d.append(b)
-                    d2o.append(orc.mixed_partials(state[v], positions, sample, 0, DEPTH, seed - v, S, P,
-                                                  float(outer_dx), n_slots))
-            orc.lib().orc_bssn_stage_from_words(orc.ptrs(state), orc.ptrs(fields), orc.ptrs(acc), orc.ptrs(acc),
-                                                orc.ptrs(new), orc.ptrs(d1), orc.ptrs(d2d), orc.ptrs(d2o),
-                                                None, positions.ctypes.data, sample.ctypes.data,
-                                                len(sample), 0, DEPTH, stage, dtd.ctypes.data, params.ctypes.data,
-                                                S, P)
-            state = [x.astype(np.float32) for x in new]
-        return len(sample) * S ** 3
+    def step(tiles):
+        t0 = time.perf_counter()
+        CPU.rk4_step(yn, a, b, acc, bnd, tiles, S, P, Q, inv_dx, dt, params)
+        return time.perf_counter() - t0
 
-    n = 8
-    t0 = time.perf_counter()
-    cells = one_step(tiles[:n])
-    dt = time.perf_counter() - t0
-    # scale the sample so the timed run is about budget_s: more tiles, then
-    # repeated steps over all 512
-    want = n * budget_s / max(dt, 1e-3)
-    n = int(min(len(tiles), max(8, want)))
-    reps = max(1, int(round(want / n)))
-    t0 = time.perf_counter()
-    cells = sum(one_step(tiles[:n]) for _ in range(reps))
-    dt = time.perf_counter() - t0
-    return {"value": cells / dt, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} x one RK4 step (4 stages: FD words, FP64 RHS + update) of {n} of the "
-                      f"512 depth-3 tiles ({cells} zone-cycles) in {dt:.2f} s with {threads} OpenMP threads; "
-                      "ghost fill not included"}
+    t32 = step(tiles_all[:32])
+    n = int(min(len(tiles_all), max(8, 32 * target_s / max(t32, 1e-4))))
+    sample = np.ascontiguousarray(tiles_all[:n])
+    step(sample)  # warm-up
+    times = sorted(step(sample) for _ in range(reps))
+    med = times[len(times) // 2]
+    cells = n * S ** 3
+    return {"value": cells / med, "unit": "zone-cycles/s", "cores": threads, "kind": "port", "dtype": "fp32",
+            "sample": f"FP32 host implementation of the scheme (oracle/am_cpu_fp32.c, gcc -O3 x86-64-v3, OpenMP, "
+                      f"{threads} threads): one RK4 step of {n} of the 512 depth-3 tiles ({cells} zone-cycles); "
+                      f"timed region: 4 x (order-4 derivatives + BSSN RHS + RK4 update per cell, equal-depth ghost "
+                      f"refresh of the sample); median of {reps} steps after one warm-up: {med:.3f} s "
+                      f"(min {times[0]:.3f}, max {times[-1]:.3f})"}
+
+
+def cpu_kernel_table(n_slots, fields, export, sub=64):
+    """The reference's kernel bodies as the oracle restates them (gcc -O2,
+    no contraction, bit-exact with the reference's float arithmetic) and the
+    FP32 implementation's passes, timed on this host with 1 thread (the
+    first `sub` depth-3 tiles) and with every thread (all 512), per RK4
+    stage of the 24 fields: markdown rows of ms per call and ns per
+    interior cell."""
+    from oracle import cpu_fp32 as CPU
+    from oracle import oracle as orc
+    L, L32 = orc.lib(), CPU.lib()
+    pos, bnd, ch = export["positions"], export["boundaries"], export["children"]
+    tiles_all = np.array([t for t in range(n_slots) if (int(pos[t]) >> 27) == DEPTH and int(pos[t]) != 0xFFFFFFFF],
+                         np.int32)
+    used = np.array([t for t in range(n_slots) if int(pos[t]) != 0xFFFFFFFF], np.int32)
+    outer_dx = float(np.float32(OUTER) / np.float32(S))
+    inv_dx = float(np.float32(2 ** DEPTH) / np.float32(outer_dx))
+    dt = float(np.float32(np.float32(outer_dx) * np.float32(CFL)) / 2 ** DEPTH)
+    params = np.array(orc.BSSN_DEFAULT, np.float64)
+    st = [np.ascontiguousarray(f, np.float32) for f in fields]
+    W = n_slots * S ** 3
+    w1 = [np.zeros(W, np.uint32) for _ in range(24)]
+    w2 = [np.zeros(W, np.uint32) for _ in range(11)]
+    w3 = [np.zeros(W, np.uint32) for _ in range(11)]
+    old = [x.copy() for x in st]
+    new = [x.copy() for x in st]
+    out = [x.copy() for x in st]
+    acc = [np.zeros_like(x) for x in st]
+    d1 = [np.zeros(3 * len(tiles_all) * S ** 3, np.float32) for _ in range(24)]
+    d2 = [np.zeros(6 * len(tiles_all) * S ** 3, np.float32) for _ in range(11)]
+    hess = list(orc.HESS_VARS)
+
+    def differentiate(t):
+        h = 0
+        for v in range(24):
+            dd = w2[h].ctypes.data if v in hess else None
+            L.orc_differentiate(st[v].ctypes.data, w1[v].ctypes.data, dd, pos.ctypes.data, t.ctypes.data, len(t), 0,
+                                DEPTH, 11 + v, S, P, Q, outer_dx)
+            h += v in hess
+
+    def mixed(t):
+        for h, v in enumerate(hess):
+            L.orc_mixed_partials(st[v].ctypes.data, w3[h].ctypes.data, pos.ctypes.data, t.ctypes.data, len(t), 0,
+                                 DEPTH, 7 + v, S, P, outer_dx)
+
+    def copy_boundaries(t):
+        for v in range(24):
+            L.orc_copy_boundaries(old[v].ctypes.data, new[v].ctypes.data, None, bnd.ctypes.data, t.ctypes.data,
+                                  len(t), 0, DEPTH, 1, 0, S, P, 1.0, 0)
+
+    def copy_new_to_old(t):
+        for v in range(24):
+            L.orc_copy_new_to_old(old[v].ctypes.data, new[v].ctypes.data, pos.ctypes.data, ch.ctypes.data,
+                                  t.ctypes.data, len(t), 0, DEPTH, S, P)
+
+    def fp32_derivatives(t):
+        L32.cpu32_derivatives(CPU.ptrs(st), CPU.ptrs(d1), CPU.ptrs(d2), t.ctypes.data, len(t), S, P, Q, inv_dx)
+
+    def fp32_stage(t):
+        L32.cpu32_bssn_stage(CPU.ptrs(st), CPU.ptrs(st), CPU.ptrs(acc), CPU.ptrs(out), t.ctypes.data, len(t), S, P, Q,
+                             inv_dx, dt, 1, params.ctypes.data, 0.0, 0)
+
+    rows = [("`__differentiate` (AMR.h:1152-1219): 24 gradients + 11 Hessian diagonals, dithered words",
+             differentiate, "tiles"),
+            ("`__mixed_partials` (AMR.h:1232-1266): 11 fields, dithered words", mixed, "tiles"),
+            ("`__copy_boundaries` (AMR.h:702-762): 24 fields, quadratic formula, depth-3 faces", copy_boundaries,
+             "tiles"),
+            ("`__copy_new_to_old` (AMR.h:961-996): 24 fields, every slot", copy_new_to_old, "used"),
+            ("FP32 implementation: derivatives of the 24 fields (gradients + 11 Hessians, fp32 out)",
+             fp32_derivatives, "tiles"),
+            ("FP32 implementation: one RK4 stage (derivatives + RHS + update)", fp32_stage, "tiles")]
+    n_all = int(L32.cpu32_max_threads())
+    lines = [f"| kernel body | 1 thread: ms ({sub} tiles) | 1 thread: ns/cell | {n_all} threads: ms (512 tiles) | "
+             f"{n_all} threads: ns/cell |", "|---|---|---|---|---|"]
+    for name, fn, dom in rows:
+        res = []
+        for threads, t in ((1, tiles_all[:sub]), (n_all, tiles_all)):
+            if dom == "used":
+                t = used if threads == n_all else used[np.isin(used, tiles_all[:sub]) | (used < 73)]
+            t = np.ascontiguousarray(t, np.int32)
+            L.orc_set_threads(threads)
+            L32.cpu32_set_threads(threads)
+            fn(t)  # warm-up
+            times = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                fn(t)
+                times.append(time.perf_counter() - t0)
+            ms = 1e3 * sorted(times)[1]
+            cells = len([x for x in t if (int(pos[x]) >> 27) == DEPTH]) * S ** 3
+            res += [f"{ms:.1f}", f"{1e6 * ms / cells:.2f}"]
+        lines.append(f"| {name} | " + " | ".join(res) + " |")
+    L.orc_set_threads(n_all)
+    L32.cpu32_set_threads(n_all)
+    return "\n".join(lines)
 
 
 def copy_probe(mib=1024, reps=10):
@@ -247,6 +333,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-versus-plain ghost fill comparison (training + two puncture runs)")
     ap.add_argument("--learned-steps", type=int, default=3000, help="Adam steps of the learned prolongation")
+    ap.add_argument("--cpu-kernels", action="store_true",
+                    help="print only a markdown table of the reference's kernel bodies (the oracle's restatement) and "
+                         "the FP32 implementation's passes timed on this host, 1 thread and all threads")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time only a plain 1 GiB device copy (the calibration for HBM byte counters)")
     args = ap.parse_args()
@@ -259,6 +348,13 @@ def main():
         args.steps = 64 if args.workload == "puncture" else 12
     if args.copy_probe:
         copy_probe()
+        return
+    if args.cpu_kernels:  # host cores only: the mesh from the oracle's octree
+        from oracle import oracle as orc
+        n_slots = 1 + 8 + 64 + 512
+        ot = orc.OracleTree(n_slots, True)
+        assert build_mesh(ot) == n_slots
+        print(cpu_kernel_table(n_slots, initial_data(n_slots, S + 2 * P), ot.export()), flush=True)
         return
 
     import adaptivemesh_amd as am
@@ -378,8 +474,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline and args.workload == "bssn512":
-        e = sim.export()
-        result["cpu_baseline"] = cpu_baseline(n_slots, fields, e["positions"], list(range(n_slots)))
+        result["cpu_baseline"] = cpu_baseline(n_slots, fields, sim.export())
         result["cpu_baseline"]["value"] = round(result["cpu_baseline"]["value"], 1)
     if not args.no_learned and args.workload == "bssn512":
         # the second half of the metric: learned versus plain (tricubic)

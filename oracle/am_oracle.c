@@ -54,6 +54,7 @@
 #define EXPORT __attribute__((visibility("default")))
 
 EXPORT int orc_num_threads(void) { return omp_get_max_threads(); }
+EXPORT void orc_set_threads(int n) { omp_set_num_threads(n); }
 
 /* ------------------------------------------------------------------ PRNG */
 

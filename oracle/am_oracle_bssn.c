@@ -58,234 +58,11 @@
 static const int SYMI[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
 static const int HESS_VAR[BSSN_NH] = {0, 1, 2, 3, 4, 5, 6, 17, 18, 19, 20};
 
-static void inv3(const double g[3][3], double r[3][3])
-{
-    const double det = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[2][1]) -
-                       g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]) +
-                       g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
-    r[0][0] = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / det;
-    r[0][1] = (g[0][2] * g[2][1] - g[0][1] * g[2][2]) / det;
-    r[0][2] = (g[0][1] * g[1][2] - g[0][2] * g[1][1]) / det;
-    r[1][0] = (g[1][2] * g[2][0] - g[1][0] * g[2][2]) / det;
-    r[1][1] = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / det;
-    r[1][2] = (g[0][2] * g[1][0] - g[0][0] * g[1][2]) / det;
-    r[2][0] = (g[1][0] * g[2][1] - g[1][1] * g[2][0]) / det;
-    r[2][1] = (g[0][1] * g[2][0] - g[0][0] * g[2][1]) / det;
-    r[2][2] = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / det;
-}
-
-/* u[24]; d1[24][3] (d1[v][k] = d_k u_v); dd[11][6] symmetric Hessians in
- * xx,xy,xz,yy,yz,zz order for the HESS_VAR fields; G[6][3] with
- * G[jk][i] = Gt^i_jk (jk in xx..zz order).  G == NULL computes Gt^i_jk
- * from the metric derivatives.  Writes rhs[24] and, if H != NULL, the
- * constraints H and M[3]. */
-static void bssn_point(const double *u, const double (*d1)[3], const double (*dd)[6], const double (*Gin)[3],
-                       const orc_bssn_par *par, const double *advin, double *rhs, double *Hc, double *Mc)
-{
-    const double chi = u[0], chid = fmax(u[0], par->chi_floor);
-    double gt[3][3], At[3][3], gtu[3][3], Gam[3], beta[3], B[3];
-    double dchi[3], dgt[3][3][3], dK[3], dAt[3][3][3], dGam[3][3], dal[3], db[3][3];
-    double ddchi[3][3], ddgt[3][3][3][3], ddal[3][3], ddb[3][3][3];
-    const double K = u[7], alpha = u[17];
-    for (int i = 0; i < 3; i++) {
-        for (int j = 0; j < 3; j++) {
-            gt[i][j] = u[1 + SYMI[i][j]];
-            At[i][j] = u[8 + SYMI[i][j]];
-        }
-        Gam[i] = u[14 + i];
-        beta[i] = u[18 + i];
-        B[i] = u[21 + i];
-    }
-    for (int k = 0; k < 3; k++) {
-        dchi[k] = d1[0][k];
-        dK[k] = d1[7][k];
-        dal[k] = d1[17][k];
-        for (int i = 0; i < 3; i++) {
-            dGam[i][k] = d1[14 + i][k];
-            db[i][k] = d1[18 + i][k];
-            for (int j = 0; j < 3; j++) {
-                dgt[i][j][k] = d1[1 + SYMI[i][j]][k];
-                dAt[i][j][k] = d1[8 + SYMI[i][j]][k];
-            }
-        }
-    }
-    for (int k = 0; k < 3; k++)
-        for (int l = 0; l < 3; l++) {
-            ddchi[k][l] = dd[0][SYMI[k][l]];
-            ddal[k][l] = dd[7][SYMI[k][l]];
-            for (int i = 0; i < 3; i++) {
-                ddb[i][k][l] = dd[8 + i][SYMI[k][l]];
-                for (int j = 0; j < 3; j++) ddgt[i][j][k][l] = dd[1 + SYMI[i][j]][SYMI[k][l]];
-            }
-        }
-    inv3(gt, gtu);
-
-    /* Christoffels: lowered from the metric derivatives, raised as given */
-    double Gl[3][3][3], G[3][3][3], Gd[3];
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++) Gl[i][j][k] = 0.5 * (dgt[i][j][k] + dgt[i][k][j] - dgt[j][k][i]);
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++) {
-                if (Gin) {
-                    G[i][j][k] = Gin[SYMI[j][k]][i];
-                } else {
-                    double s = 0;
-                    for (int l = 0; l < 3; l++) s += gtu[i][l] * Gl[l][j][k];
-                    G[i][j][k] = s;
-                }
-            }
-    for (int i = 0; i < 3; i++) {
-        double s = 0;
-        for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++) s += gtu[j][k] * G[i][j][k];
-        Gd[i] = s;
-    }
-
-    double divb = db[0][0] + db[1][1] + db[2][2];
-    double Atu[3][3], Atm[3][3], AA = 0;
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            double s = 0, m = 0;
-            for (int k = 0; k < 3; k++) {
-                m += gtu[i][k] * At[k][j];
-                for (int l = 0; l < 3; l++) s += gtu[i][k] * gtu[j][l] * At[k][l];
-            }
-            Atu[i][j] = s;
-            Atm[i][j] = m;
-        }
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) AA += At[i][j] * Atu[i][j];
-
-    /* advection beta^k d_k of each field: centred, from the gradients, or
-     * (advin != NULL) given per field -- the upwinded advection plus the
-     * Kreiss-Oliger dissipation that the FD computes (see fd64_adv) */
-    double adv[BSSN_NV];
-    for (int v = 0; v < BSSN_NV; v++)
-        adv[v] = advin ? advin[v] : beta[0] * d1[v][0] + beta[1] * d1[v][1] + beta[2] * d1[v][2];
-
-    /* covariant second derivatives of chi and alpha */
-    double DDchi[3][3], DDalt[3][3], DDal[3][3];
-    double dchi_dal = 0, dchi2 = 0;
-    for (int k = 0; k < 3; k++)
-        for (int l = 0; l < 3; l++) {
-            dchi_dal += gtu[k][l] * dchi[k] * dal[l];
-            dchi2 += gtu[k][l] * dchi[k] * dchi[l];
-        }
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            double sc = 0, sa = 0;
-            for (int k = 0; k < 3; k++) {
-                sc += G[k][i][j] * dchi[k];
-                sa += G[k][i][j] * dal[k];
-            }
-            DDchi[i][j] = ddchi[i][j] - sc;
-            DDalt[i][j] = ddal[i][j] - sa;
-            DDal[i][j] = DDalt[i][j] + (dchi[i] * dal[j] + dchi[j] * dal[i] - gt[i][j] * dchi_dal) / (2 * chid);
-        }
-    double trDDchi = 0, trDDalt = 0;
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            trDDchi += gtu[i][j] * DDchi[i][j];
-            trDDalt += gtu[i][j] * DDalt[i][j];
-        }
-    const double lap_alpha = chi * trDDalt - 0.5 * dchi_dal;  /* D^i D_i alpha */
-
-    /* Ricci tensor */
-    double R[3][3];
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            double r = 0;
-            for (int l = 0; l < 3; l++)
-                for (int m = 0; m < 3; m++) r -= 0.5 * gtu[l][m] * ddgt[i][j][l][m];
-            for (int k = 0; k < 3; k++) {
-                r += 0.5 * (gt[k][i] * dGam[k][j] + gt[k][j] * dGam[k][i]);
-                r += 0.5 * Gd[k] * (Gl[i][j][k] + Gl[j][i][k]);
-            }
-            for (int l = 0; l < 3; l++)
-                for (int m = 0; m < 3; m++)
-                    for (int k = 0; k < 3; k++)
-                        r += gtu[l][m] * (G[k][l][i] * Gl[j][k][m] + G[k][l][j] * Gl[i][k][m] + G[k][i][m] * Gl[k][l][j]);
-            r += (DDchi[i][j] + gt[i][j] * trDDchi) / (2 * chid) -
-                 (dchi[i] * dchi[j] + 3 * gt[i][j] * dchi2) / (4 * chid * chid);
-            R[i][j] = r;
-        }
-
-    /* right-hand sides */
-    rhs[0] = adv[0] + (2.0 / 3.0) * chi * (alpha * K - divb);
-    for (int i = 0; i < 3; i++)
-        for (int j = i; j < 3; j++) {
-            double s = adv[1 + SYMI[i][j]] - (2.0 / 3.0) * gt[i][j] * divb - 2 * alpha * At[i][j];
-            for (int k = 0; k < 3; k++) s += gt[i][k] * db[k][j] + gt[j][k] * db[k][i];
-            rhs[1 + SYMI[i][j]] = s;
-        }
-    rhs[7] = adv[7] - lap_alpha + alpha * (AA + K * K / 3.0);
-    double X[3][3], trX = 0;
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) X[i][j] = -DDal[i][j] + alpha * R[i][j];
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) trX += gtu[i][j] * X[i][j];
-    for (int i = 0; i < 3; i++)
-        for (int j = i; j < 3; j++) {
-            double s = adv[8 + SYMI[i][j]] - (2.0 / 3.0) * At[i][j] * divb +
-                       chi * (X[i][j] - gt[i][j] * trX / 3.0) + alpha * K * At[i][j];
-            for (int k = 0; k < 3; k++) {
-                s += At[i][k] * db[k][j] + At[j][k] * db[k][i];
-                s -= 2 * alpha * At[i][k] * Atm[k][j];
-            }
-            rhs[8 + SYMI[i][j]] = s;
-        }
-    for (int i = 0; i < 3; i++) {
-        double s = adv[14 + i] + (2.0 / 3.0) * Gam[i] * divb;
-        double ddiv = 0;
-        for (int k = 0; k < 3; k++) s -= Gam[k] * db[i][k];
-        for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++) {
-                s += gtu[j][k] * ddb[i][j][k];
-                s += 2 * alpha * G[i][j][k] * Atu[j][k];
-            }
-        for (int j = 0; j < 3; j++) {
-            ddiv = 0;
-            for (int k = 0; k < 3; k++) ddiv += ddb[k][j][k];
-            s += gtu[i][j] * ddiv / 3.0;
-            s -= 2 * Atu[i][j] * dal[j];
-            s -= 2 * alpha * (1.5 * Atu[i][j] * dchi[j] / chid + (2.0 / 3.0) * gtu[i][j] * dK[j]);
-        }
-        rhs[14 + i] = s;
-    }
-    rhs[17] = adv[17] - (par->slicing == 1 ? alpha * alpha * K : 2 * alpha * K);
-    for (int i = 0; i < 3; i++) {
-        if (advin) {
-            /* adv[18..23] hold the shift's advection (with advect_shift)
-             * plus its dissipation; d_0 Gt in the B equation drops Gt's
-             * advection term, dissipation included, with advect_shift */
-            rhs[18 + i] = adv[18 + i] + par->shift_coeff * B[i];
-            rhs[21 + i] = adv[21 + i] + rhs[14 + i] - (par->advect_shift ? adv[14 + i] : 0.0) - par->eta * B[i];
-        } else {
-            rhs[18 + i] = (par->advect_shift ? adv[18 + i] : 0.0) + par->shift_coeff * B[i];
-            rhs[21 + i] = (par->advect_shift ? adv[21 + i] - adv[14 + i] : 0.0) + rhs[14 + i] - par->eta * B[i];
-        }
-    }
-
-    if (Hc) {
-        double Rs = 0;
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) Rs += gtu[i][j] * R[i][j];
-        *Hc = chi * Rs + (2.0 / 3.0) * K * K - AA;
-        for (int i = 0; i < 3; i++) {
-            double m = -(2.0 / 3.0) * dK[i];
-            for (int j = 0; j < 3; j++)
-                for (int k = 0; k < 3; k++) {
-                    double D = dAt[k][i][j];
-                    for (int l = 0; l < 3; l++) D -= G[l][j][k] * At[l][i] + G[l][j][i] * At[k][l];
-                    m += gtu[j][k] * D;
-                    m -= 1.5 / chid * gtu[j][k] * At[i][j] * dchi[k];
-                }
-            Mc[i] = m;
-        }
-    }
-}
+/* the right-hand side, FD cell stencils and RK4 combination in double */
+#define REAL double
+#define RMAX fmax
+#define SCHEME(n) n
+#include "bssn_scheme.inc"
 
 static orc_bssn_par par_from(const double *p)
 {
@@ -353,22 +130,6 @@ static inline void word_dec(const void *base, int64_t w, int fmt, double *o)
     } else {
         const uint16_t *p = (const uint16_t *)base + 3 * w;
         o[0] = half_to_double(p[0]); o[1] = half_to_double(p[1]); o[2] = half_to_double(p[2]);
-    }
-}
-
-/* RK4 stage combination shared by every stage kernel here:
- *   stage 0: acc = k;      new = y_n + dt/2 k
- *   stage 1: acc += 2k;    new = y_n + dt/2 k
- *   stage 2: acc += 2k;    new = y_n + dt k
- *   stage 3:               new = y_n + dt/6 (acc + k)      */
-static inline void rk4_combine(int stage, double dt, double yn, double k, double acc_in, double *acc_out,
-                               double *out)
-{
-    switch (stage) {
-    case 0: *acc_out = k; *out = yn + 0.5 * dt * k; break;
-    case 1: *acc_out = acc_in + 2 * k; *out = yn + 0.5 * dt * k; break;
-    case 2: *acc_out = acc_in + 2 * k; *out = yn + dt * k; break;
-    default: *acc_out = acc_in; *out = yn + dt / 6.0 * (acc_in + k); break;
     }
 }
 
@@ -546,58 +307,8 @@ EXPORT void orc_fd64_float_coefficients(int on)
     cd_ready = 1;
 }
 
-/* Order-2q gradient and diagonal Hessian plus the mixed partials with the
- * reference's edge formula (AMR.h:1152-1266), in double, unquantised, at
- * one interior cell.  inv_dx = 2^depth / outer_dx. */
-static void fd64_cell(const double *f, int64_t rw, int S, int T, int q, int x, int y, int z, double inv_dx,
-                      double *g, double *h)
-{
-    const int base = q * q + q - 1;
-    const int64_t T2 = (int64_t)T * T;
-    double vx = 0, vy = 0, vz = 0, xx = 0, yy = 0, zz = 0;
-    for (int i = -q; i <= q; i++) {
-        vx += f[rw + i] * CD1[base + i];
-        xx += f[rw + i] * CD2[base + i];
-        vy += f[rw + (int64_t)i * T] * CD1[base + i];
-        yy += f[rw + (int64_t)i * T] * CD2[base + i];
-        vz += f[rw + (int64_t)i * T2] * CD1[base + i];
-        zz += f[rw + (int64_t)i * T2] * CD2[base + i];
-    }
-    g[0] = vx * inv_dx; g[1] = vy * inv_dx; g[2] = vz * inv_dx;
-    if (!h) return;
-    const double i2 = inv_dx * inv_dx;
-    const int ex = x - (x < 1 ? 1 : (x > S - 2 ? S - 2 : x));
-    const int ey = y - (y < 1 ? 1 : (y > S - 2 ? S - 2 : y));
-    const int ez = z - (z < 1 ? 1 : (z > S - 2 ? S - 2 : z));
-    const int c[3] = {ex * ey, ex * ez, ey * ez};
-    const int64_t dA[3] = {1, 1, T}, dB[3] = {T, T2, T2};
-    double m[3];
-    for (int k = 0; k < 3; k++) {
-        const int64_t a = dA[k], b = dB[k];
-        if (c[k] == 0) {
-            m[k] = .25 * (f[rw + a + b] + f[rw - a - b] - f[rw + a - b] - f[rw - a + b]);
-        } else {
-            const int64_t da = -c[k] * a;
-            m[k] = -c[k] * (f[rw] + .5 * (f[rw + da + b] + f[rw - da - b] - f[rw - da] - f[rw + da] - f[rw - b] -
-                                          f[rw + b]));
-        }
-    }
-    h[0] = xx * i2; h[1] = m[0] * i2; h[2] = m[1] * i2;
-    h[3] = yy * i2; h[4] = m[2] * i2; h[5] = zz * i2;
-}
-
-/* Upwinded advection and Kreiss-Oliger dissipation of the FP64 scheme (an
- * extension for the moving-puncture runs; no reference counterpart):
- *   adv = [advect] sum_k beta^k (D^lop_k f)  +  sigma/(64 h) sum_k (Delta^6_k f)
- * D^lop_k: fourth-order one-sided-by-one first derivative on the side
- * beta^k points to, (-3 f_-1 - 10 f_0 + 18 f_1 - 6 f_2 + f_3) / (12 h) for
- * beta^k > 0 and its mirror (-f_-3 + 6 f_-2 - 18 f_-1 + 10 f_0 + 3 f_1) /
- * (12 h) otherwise (exact on polynomials of degree <= 4); Delta^6_k: the
- * sixth difference f_-3 - 6 f_-2 + 15 f_-1 - 20 f_0 + 15 f_1 - 6 f_2 + f_3
- * (zero on polynomials of degree <= 5; -64 f on the mode (-1)^k), so the
- * term damps the highest resolvable mode at rate sigma/h per axis.  The
- * literature's form of both, e.g. Brugmann et al., Phys. Rev. D 77, 024027
- * (2008).  Off until orc_bssn64_set_dissipation(sigma, 1); the dissipation
+/* Upwinded advection and Kreiss-Oliger dissipation of the FP64 scheme
+ * (fd_adv): off until orc_bssn64_set_dissipation(sigma, 1); the dissipation
  * comes only with the upwinded advection. */
 static double g_ko_sigma = 0.0;
 static int g_upwind = 0;
@@ -605,27 +316,6 @@ EXPORT void orc_bssn64_set_dissipation(double sigma, int upwind)
 {
     g_ko_sigma = sigma;
     g_upwind = upwind;
-}
-static const double LOP_PLUS[5] = {-3. / 12, -10. / 12, 18. / 12, -6. / 12, 1. / 12};   /* offsets -1..3 */
-static const double LOP_MINUS[5] = {-1. / 12, 6. / 12, -18. / 12, 10. / 12, 3. / 12};  /* offsets -3..1 */
-static const double KO6[7] = {1., -6., 15., -20., 15., -6., 1.};
-static double fd64_adv(const double *f, int64_t rw, int T, const double *beta, double inv_dx, double sigma,
-                       int advect)
-{
-    const int64_t st[3] = {1, T, (int64_t)T * T};
-    double adv = 0.0, ko = 0.0;
-    for (int k = 0; k < 3; k++) {
-        const int64_t d = st[k];
-        double l = 0.0, kk = 0.0;
-        if (beta[k] > 0)
-            for (int m = -1; m <= 3; m++) l += LOP_PLUS[m + 1] * f[rw + m * d];
-        else
-            for (int m = -3; m <= 1; m++) l += LOP_MINUS[m + 3] * f[rw + m * d];
-        for (int m = -3; m <= 3; m++) kk += KO6[m + 3] * f[rw + m * d];
-        adv += beta[k] * l * inv_dx;
-        ko += kk;
-    }
-    return (advect ? adv : 0.0) + sigma / 64.0 * inv_dx * ko;
 }
 
 /* FP64 RK4 stage with exact (unquantised) derivatives and Christoffels.
@@ -654,14 +344,14 @@ EXPORT void orc_bssn64_stage(const double *const *state, const double *const *ol
                     for (int v = 0; v < BSSN_NV; v++) {
                         u[v] = state[v][r];
                         const int has_h = (h < BSSN_NH && HESS_VAR[h] == v);
-                        fd64_cell(state[v], r, S, T, q, x, y, z, inv_dx[depth], d1[v], has_h ? dd[h] : NULL);
+                        fd_cell(state[v], r, S, T, q, x, y, z, inv_dx[depth], CD1, CD2, d1[v], has_h ? dd[h] : NULL);
                         if (has_h) h++;
                     }
                     double advin[BSSN_NV];
                     if (g_upwind) {
                         const double beta[3] = {u[18], u[19], u[20]};
                         for (int v = 0; v < BSSN_NV; v++)
-                            advin[v] = fd64_adv(state[v], r, T, beta, inv_dx[depth], g_ko_sigma,
+                            advin[v] = fd_adv(state[v], r, T, beta, inv_dx[depth], g_ko_sigma,
                                                 v < 18 || par.advect_shift);
                     }
                     bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd, NULL, &par, g_upwind ? advin : NULL,
@@ -699,7 +389,7 @@ EXPORT void orc_bssn64_constraints(const double *const *state, const uint32_t *p
                     for (int v = 0; v < BSSN_NV; v++) {
                         u[v] = state[v][r];
                         const int has_h = (h < BSSN_NH && HESS_VAR[h] == v);
-                        fd64_cell(state[v], r, S, T, q, x, y, z, inv_dx[depth], d1[v], has_h ? dd[h] : NULL);
+                        fd_cell(state[v], r, S, T, q, x, y, z, inv_dx[depth], CD1, CD2, d1[v], has_h ? dd[h] : NULL);
                         if (has_h) h++;
                     }
                     bssn_point(u, (const double(*)[3])d1, (const double(*)[6])dd, NULL, &par, NULL, k, &H, M);

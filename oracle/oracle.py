@@ -20,6 +20,7 @@ _i32, _u32, _f32 = C.c_int32, C.c_uint32, C.c_float
 
 _SIGS = {
     "orc_num_threads": (C.c_int, []),
+    "orc_set_threads": (None, [C.c_int]),
     "orc_prng_int": (_i32, [C.POINTER(_i32)]),
     "orc_prng_float": (_f32, [C.POINTER(_i32)]),
     "orc_thread_seed": (_i32, [_i32, _u32, _u32, _u32]),
