@@ -216,6 +216,9 @@ _SIGS = {
                                               C.c_int32, C.c_int32, _P]),
     "am_ghost_fill_stage_faces": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), _P, C.c_int32,
                                             C.POINTER(Geometry), C.c_float, C.c_int32, C.c_int32, _P]),
+    "am_ghost_fill_learned_faces": (C.c_int, [C.POINTER(FieldBatch), C.POINTER(Mesh), _P, C.c_int32,
+                                              C.POINTER(Geometry), C.c_float, _P, C.c_float, _P, _P, C.c_int32,
+                                              _P]),
     "am_bssn_rhs_stage": (C.c_int, [C.POINTER(BssnFields), C.c_int32, C.POINTER(BssnParams), C.POINTER(Mesh),
                                     C.POINTER(Geometry), C.c_int32, _P]),
     "am_bssn_constraints": (C.c_int, [C.POINTER(BssnFields), C.POINTER(BssnParams), C.POINTER(Mesh),
@@ -730,7 +733,7 @@ class _Rk4Sim:
     def prolongation_stats(self) -> dict:
         out = np.zeros(4, np.uint64)
         check(self._f("prolongation_stats")(self.h, out.ctypes.data), "prolongation_stats")
-        return dict(learned_blocks=int(out[0]), fallback_blocks=int(out[1]), tricubic_cells=int(out[2]),
+        return dict(learned_faces=int(out[0]), fallback_faces=int(out[1]), tricubic_cells=int(out[2]),
                     cells=int(out[3]))
 
     def kernel_times(self) -> dict:

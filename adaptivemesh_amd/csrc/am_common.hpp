@@ -111,9 +111,11 @@ int set_error(int code, const char *fmt, ...);
 int check_launch(const char *what);
 DepthScales depth_scales(const am_geometry &g);
 bool aligned16(const void *p);
-// the tricubic stage fill of every face (what the learned fill starts from)
-int ghost_stage_fill(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, float stage_frac,
-                     int equal_mode, void *stream);
+// the tricubic stage fill of every face of mesh->tiles, or of the listed
+// faces (what the learned fill starts from; gate: see LearnedGate)
+struct LearnedGate;
+int ghost_stage_fill(const am_field_batch *b, const am_mesh *mesh, const int32_t *faces, int n_faces,
+                     const am_geometry *g, float stage_frac, int equal_mode, void *stream, LearnedGate gate);
 
 }  // namespace am
 

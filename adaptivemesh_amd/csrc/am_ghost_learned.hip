@@ -6,7 +6,7 @@
 // overwrites the cells it predicts.  Faces whose target is one level coarser
 // and not the root are covered by a box of coarse cells of the target (2 x 8
 // x 8 for S = 16, P = 3); each coarse cell is one sample, and one workgroup
-// (4 waves) takes a block of 64 samples:
+// (8 waves, 8 threads per sample) takes a block of 64 samples:
 //   1. per field: gather the cell's 4^3 stencil (offsets -1..2) of the
 //      target at the stage time (quadratic in time through older/old/new,
 //      as the tricubic fill), normalise by the centre value mu and the
@@ -67,14 +67,16 @@ __device__ __forceinline__ float read_pos(uint32_t w, float rel, float rel_scale
 }
 
 template <int S, int P, int L>
-__global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const uint64_t *__restrict__ bnd,
-                                                       const int32_t *__restrict__ tiles, uint32_t substep,
+__global__ void __launch_bounds__(512) k_ghost_learned(FieldBatchDev b, const uint64_t *__restrict__ bnd,
+                                                       const int32_t *__restrict__ tiles,
+                                                       const int32_t *__restrict__ faces, uint32_t substep,
                                                        uint32_t maxd, float frac, const float *__restrict__ prm,
                                                        float threshold, unsigned long long *__restrict__ stats,
-                                                       int n_blocks)
+                                                       int n_blocks, float *__restrict__ tile_resid)
 {
-    constexpr int T = S + 2 * P, NC = S * S * P;
+    constexpr int NC = S * S * P;
     constexpr size_t CELLS = tile_stride<S, P, L>();
+    constexpr int T = S + 2 * P;
     __shared__ float Xs[2][64][65];
     __shared__ float Hs[64][129];
     __shared__ float Bn[64][193];
@@ -84,8 +86,16 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
     __shared__ int n_invalid;
 
     const int face = blockIdx.x / n_blocks, blk = blockIdx.x - face * n_blocks;
-    const int ti = face / 6, dir = face - ti * 6;
-    const int node = tiles[ti];
+    int node, dir;
+    if (faces) {
+        const int code = faces[face];
+        node = code / 6;
+        dir = code - node * 6;
+    } else {
+        const int ti = face / 6;
+        dir = face - ti * 6;
+        node = tiles[ti];
+    }
     const uint64_t bb = bnd[(size_t)node * 6 + dir];
     const uint32_t nd = (uint32_t)bb;
     const int tgt = bnd_target(nd), cd = bnd_depth(nd);
@@ -124,212 +134,201 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
         n_invalid = 0;
     }
     const size_t tbase = (size_t)tgt * CELLS;
-    {
-        const int ns = min(64, NS - chunk);
-        if (tid < 64) valid[tid] = 1;
+    const int ns = min(64, NS - chunk);
+    if (tid < 64) valid[tid] = 1;
+    __syncthreads();
+    floatx4 acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // sample s and its eighth q of the 4^3 stencil: plane k = q / 2, rows
+    // j = 2 (q % 2) + {0, 1}, 4 points each
+    const int s = tid >> 3, q = tid & 7;
+    int cc[3] = {0, 0, 0};
+    if (s < ns) {
+        const int sa = chunk + s;
+        cc[0] = clo[0] + sa % ncl[0];
+        cc[1] = clo[1] + (sa / ncl[0]) % ncl[1];
+        cc[2] = clo[2] + sa / (ncl[0] * ncl[1]);
+    }
+    // 1. per field: the stencil at the stage time (8 points per thread),
+    // normalised in registers -- mu is stencil point (1, 1, 1), held by
+    // lane 2 of the sample's 8, sigma the largest deviation over the 8 --
+    // then into the double-buffered LDS tile.  The next field's points and
+    // this wave's layer-1 weights of the next field are loaded while this
+    // field's baseline and layer-1 products run.
+    float raw[8];
+    bool bad = false;
+    auto gather = [&](int v) {
+        const float *dold = b.old_[v], *dnew = b.new_[v], *dolder = b.older[v];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int px = cc[0] - 1 + (i & 3), py = cc[1] - 1 + 2 * (q & 1) + (i >> 2), pz = cc[2] - 1 + (q >> 1);
+            const int out = (px < P || px >= P + S) + (py < P || py >= P + S) + (pz < P || pz >= P + S);
+            const bool inside = s < ns && px >= 0 && px < T && py >= 0 && py < T && pz >= 0 && pz < T && out <= 1;
+            float val = 0.f;
+            if (inside) {
+                const size_t r = tbase + tile_cell<S, P, L>((uint32_t)px, (uint32_t)py, (uint32_t)pz);
+                val = dold[r] * tc1 + dnew[r] * tc2;
+                if (quad) val = dolder[r] * tc0 + val;
+            } else if (s < ns) {
+                bad = true;
+            }
+            raw[i] = val;
+        }
+    };
+    // layer-1 weights of field v for this lane: W1[v * 64 + 4 kk + lane / 16][16 wave + lane % 16], kk = 0..15
+    float wcur[16], wnext[16];
+    auto load_w1 = [&](int v, float (&w)[16]) {
+        const float *wrow = prm + L_W1 + (int64_t)(v * 64 + (lane >> 4)) * L_HID + wave * 16 + (lane & 15);
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) w[kk] = wrow[(int64_t)kk * 4 * L_HID];
+    };
+    gather(0);
+    load_w1(0, wnext);
+    for (int v = 0; v < 24; v++) {
+        const int buf = v & 1;
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) wcur[kk] = wnext[kk];
+        const float mu = __shfl(raw[5], (lane & ~7) | 2, 64);
+        float dev = 0.f;
+#pragma unroll
+        for (int i = 0; i < 8; i++) dev = fmaxf(dev, fabsf(raw[i] - mu));
+        dev = fmaxf(dev, __shfl_xor(dev, 1, 64));
+        dev = fmaxf(dev, __shfl_xor(dev, 2, 64));
+        dev = fmaxf(dev, __shfl_xor(dev, 4, 64));
+        const float sg = dev + 1e-3f * fabsf(mu) + 1e-7f;
+        const float inv = 1.f / sg;
+#pragma unroll
+        for (int i = 0; i < 8; i++) Xs[buf][s][q * 8 + i] = s < ns ? (raw[i] - mu) * inv : 0.f;
+        if (s < ns && q == 0) {
+            musg[0][v][s] = mu;
+            musg[1][v][s] = sg;
+        }
+        if (bad) valid[s] = 0;
+        // Xs[buf] is complete; every wave has finished reading Xs[buf ^ 1]
         __syncthreads();
-        floatx4 acc[4][2];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-        const int s = tid >> 2, q = tid & 3;  // sample and stencil quarter (z = q) of this thread
-        int cc[3] = {0, 0, 0};
+        if (v + 1 < 24) {
+            gather(v + 1);
+            load_w1(v + 1, wnext);
+        }
+        // baseline of the 8 children (normalised units): child q of sample s
         if (s < ns) {
-            const int sa = chunk + s;
-            cc[0] = clo[0] + sa % ncl[0];
-            cc[1] = clo[1] + (sa / ncl[0]) % ncl[1];
-            cc[2] = clo[2] + sa / (ncl[0] * ncl[1]);
+            const int jx = q & 1, jy = (q >> 1) & 1, jz = q >> 2;
+            float bsum = 0.f;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        bsum += LQ[jz][k] * LQ[jy][j] * LQ[jx][i] * Xs[buf][s][k * 16 + j * 4 + i];
+            Bn[s][v * 8 + q] = bsum;
         }
-        // 1. per field: the stencil (16 points of quarter z = q per thread),
-        // normalised in registers -- mu is stencil point 21 (quarter 1,
-        // point 5), sigma the largest deviation over the sample's 4 lanes,
-        // which are adjacent in one wave -- then into the double-buffered
-        // LDS tile; the next field's points are loaded while this field's
-        // baseline and layer-1 products run
-        float raw[16];
-        bool bad = false;
-        auto gather = [&](int v) {
-            const float *dold = b.old_[v], *dnew = b.new_[v], *dolder = b.older[v];
+        // layer 1 contribution of this field: Z1[:, 16 wave ..] += Xs (64 x 64) W1[v*64.., 16 wave ..]
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int px = cc[0] - 1 + (i & 3), py = cc[1] - 1 + (i >> 2), pz = cc[2] - 1 + q;
-                const int out = (px < P || px >= P + S) + (py < P || py >= P + S) + (pz < P || pz >= P + S);
-                const bool inside = s < ns && px >= 0 && px < T && py >= 0 && py < T && pz >= 0 && pz < T && out <= 1;
-                float val = 0.f;
-                if (inside) {
-                    const size_t r = tbase + tile_cell<S, P, L>((uint32_t)px, (uint32_t)py, (uint32_t)pz);
-                    val = dold[r] * tc1 + dnew[r] * tc2;
-                    if (quad) val = dolder[r] * tc0 + val;
-                } else if (s < ns) {
-                    bad = true;
-                }
-                raw[i] = val;
-            }
-        };
-        gather(0);
-        for (int v = 0; v < 24; v++) {
-            const int buf = v & 1;
-            const float mu = __shfl(raw[5], (lane & ~3) | 1, 64);
-            float dev = 0.f;
-#pragma unroll
-            for (int i = 0; i < 16; i++) dev = fmaxf(dev, fabsf(raw[i] - mu));
-            dev = fmaxf(dev, __shfl_xor(dev, 1, 64));
-            dev = fmaxf(dev, __shfl_xor(dev, 2, 64));
-            const float sg = dev + 1e-3f * fabsf(mu) + 1e-7f;
-            const float inv = 1.f / sg;
-#pragma unroll
-            for (int i = 0; i < 16; i++) Xs[buf][s][q * 16 + i] = s < ns ? (raw[i] - mu) * inv : 0.f;
-            if (s < ns && q == 0) {
-                musg[0][v][s] = mu;
-                musg[1][v][s] = sg;
-            }
-            if (bad) valid[s] = 0;
-            // Xs[buf] is complete; every wave has finished reading Xs[buf ^ 1]
-            __syncthreads();
-            if (v + 1 < 24) gather(v + 1);
-            // baseline of the 8 children (normalised units): 2 children per thread
-            if (s < ns) {
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int ch = q * 2 + h;
-                    const int jx = ch & 1, jy = (ch >> 1) & 1, jz = ch >> 2;
-                    float bsum = 0.f;
-#pragma unroll
-                    for (int k = 0; k < 4; k++)
-#pragma unroll
-                        for (int j = 0; j < 4; j++)
-#pragma unroll
-                            for (int i = 0; i < 4; i++)
-                                bsum += LQ[jz][k] * LQ[jy][j] * LQ[jx][i] * Xs[buf][s][k * 16 + j * 4 + i];
-                    Bn[s][v * 8 + ch] = bsum;
-                }
-            }
-            // layer 1 contribution of this field: Z1 += Xs (64 x 64) W1[v*64.., :]
-#pragma unroll 4
-            for (int kk = 0; kk < 16; kk++) {
-                const int ka = kk * 4 + (lane >> 4);
-                const float *wrow = prm + L_W1 + (int64_t)(v * 64 + ka) * L_HID + wave * 32 + (lane & 15);
-                const float b0 = wrow[0], b1 = wrow[16];
-#pragma unroll
-                for (int rb = 0; rb < 4; rb++) {
-                    const float a = Xs[buf][rb * 16 + (lane & 15)][ka];
-                    acc[rb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[rb][0], 0, 0, 0);
-                    acc[rb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[rb][1], 0, 0, 0);
-                }
-            }
-        }
-        __syncthreads();
-        // layer 1 epilogue
-#pragma unroll
-        for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-            for (int cb = 0; cb < 2; cb++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int row = rb * 16 + 4 * (lane >> 4) + r, col = wave * 32 + cb * 16 + (lane & 15);
-                    Hs[row][col] = gelu_l(acc[rb][cb][r] + prm[L_B1 + col]);
-                }
-        __syncthreads();
-        // layer 2
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-        for (int kk = 0; kk < 32; kk++) {
+        for (int kk = 0; kk < 16; kk++) {
             const int ka = kk * 4 + (lane >> 4);
-            const float *wrow = prm + L_W2 + (int64_t)ka * L_HID + wave * 32 + (lane & 15);
-            const float b0 = wrow[0], b1 = wrow[16];
 #pragma unroll
             for (int rb = 0; rb < 4; rb++) {
-                const float a = Hs[rb * 16 + (lane & 15)][ka];
-                acc[rb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[rb][0], 0, 0, 0);
-                acc[rb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[rb][1], 0, 0, 0);
+                const float a = Xs[buf][rb * 16 + (lane & 15)][ka];
+                acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wcur[kk], acc[rb], 0, 0, 0);
             }
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    // layer 1 epilogue
+    const int col = wave * 16 + (lane & 15);
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) Hs[rb * 16 + 4 * (lane >> 4) + r][col] = gelu_l(acc[rb][r] + prm[L_B1 + col]);
+    __syncthreads();
+    // layer 2
+#pragma unroll
+    for (int i = 0; i < 4; i++) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+    {
+        const float *wrow = prm + L_W2 + (int64_t)(lane >> 4) * L_HID + col;
+#pragma unroll 8
+        for (int kk = 0; kk < 32; kk++) {
+            const int ka = kk * 4 + (lane >> 4);
+            const float b0 = wrow[(int64_t)kk * 4 * L_HID];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++)
+                acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(Hs[rb * 16 + (lane & 15)][ka], b0, acc[rb], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) Hs[rb * 16 + 4 * (lane >> 4) + r][col] = gelu_l(acc[rb][r] + prm[L_B2 + col]);
+    __syncthreads();
+    // layer 3: 12 output tiles of 16 columns, tiles wave and wave + 8, added onto the baseline
+    for (int t = wave; t < 12; t += 8) {
+        const int oc = t * 16 + (lane & 15);
+        floatx4 acc3[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc3[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+        const float *wrow = prm + L_W3 + (int64_t)(lane >> 4) * L_OUT + oc;
+#pragma unroll 8
+        for (int kk = 0; kk < 32; kk++) {
+            const int ka = kk * 4 + (lane >> 4);
+            const float b0 = wrow[(int64_t)kk * 4 * L_OUT];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++)
+                acc3[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(Hs[rb * 16 + (lane & 15)][ka], b0, acc3[rb], 0, 0, 0);
+        }
 #pragma unroll
         for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-            for (int cb = 0; cb < 2; cb++)
+            for (int r = 0; r < 4; r++) Bn[rb * 16 + 4 * (lane >> 4) + r][oc] += acc3[rb][r] + prm[L_B3 + oc];
+    }
+    __syncthreads();
+    // residual of the restriction (field units) and the shift that removes it
+    if (s < ns) {
+        float rmax = 0.f;
+        for (int v = q * 3; v < q * 3 + 3; v++) {
+            float m = 0.f;
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int row = rb * 16 + 4 * (lane >> 4) + r, col = wave * 32 + cb * 16 + (lane & 15);
-                    Hs[row][col] = gelu_l(acc[rb][cb][r] + prm[L_B2 + col]);
-                }
-        __syncthreads();
-        // layer 3: 48 output columns per wave, added onto the baseline
-        {
-            floatx4 acc3[4][3];
+            for (int ch = 0; ch < 8; ch++) m += Bn[s][v * 8 + ch];
+            m *= .125f;
+            rmax = fmaxf(rmax, fabsf(m * musg[1][v][s]));
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) acc3[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-            for (int kk = 0; kk < 32; kk++) {
-                const int ka = kk * 4 + (lane >> 4);
-                const float *wrow = prm + L_W3 + (int64_t)ka * L_OUT + wave * 48 + (lane & 15);
-                const float b0 = wrow[0], b1 = wrow[16], b2 = wrow[32];
-#pragma unroll
-                for (int rb = 0; rb < 4; rb++) {
-                    const float a = Hs[rb * 16 + (lane & 15)][ka];
-                    acc3[rb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc3[rb][0], 0, 0, 0);
-                    acc3[rb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc3[rb][1], 0, 0, 0);
-                    acc3[rb][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b2, acc3[rb][2], 0, 0, 0);
-                }
-            }
-#pragma unroll
-            for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-                for (int cb = 0; cb < 3; cb++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int row = rb * 16 + 4 * (lane >> 4) + r, col = wave * 48 + cb * 16 + (lane & 15);
-                        Bn[row][col] += acc3[rb][cb][r] + prm[L_B3 + col];
-                    }
+            for (int ch = 0; ch < 8; ch++) Bn[s][v * 8 + ch] -= m;
         }
-        __syncthreads();
-        // residual of the restriction (field units) and the shift that removes it
-        if (s < ns) {
-            float rmax = 0.f;
-            for (int v = q * 6; v < q * 6 + 6; v++) {
-                float m = 0.f;
+        if (valid[s]) atomicMax(&face_resid_bits, __float_as_int(rmax));  // non-negative: int order = float order
+        if (q == 0 && !valid[s]) atomicAdd(&n_invalid, 1);
+    }
+    __syncthreads();
+    // 4. the block's samples' slab cells: with the per-block gate when the
+    // block is within the threshold; with the per-tile gate always, the
+    // block's residual going to the tile's (a later pass refills the tiles
+    // beyond the threshold)
+    const bool pass = tile_resid || __int_as_float(face_resid_bits) <= threshold;
+    if (pass) {
+        for (int c = tid; c < NC; c += 512) {
+            uint32_t w[3];
+            ghost_cell_of<S, P>(dir, c, w[0], w[1], w[2]);
+            int cl[3], child = 0;
 #pragma unroll
-                for (int ch = 0; ch < 8; ch++) m += Bn[s][v * 8 + ch];
-                m *= .125f;
-                rmax = fmaxf(rmax, fabsf(m * musg[1][v][s]));
-#pragma unroll
-                for (int ch = 0; ch < 8; ch++) Bn[s][v * 8 + ch] -= m;
+            for (int k = 0; k < 3; k++) {
+                const float rp = read_pos<S, P>(w[k], rl[k], rel_scale);
+                cl[k] = (int)floorf(rp + .5f);
+                child |= (rp > (float)cl[k] ? 1 : 0) << k;
             }
-            if (valid[s]) atomicMax(&face_resid_bits, __float_as_int(rmax));  // non-negative: int order = float order
-            if (q == 0 && !valid[s]) atomicAdd(&n_invalid, 1);
+            const int sa = (cl[0] - clo[0]) + ncl[0] * ((cl[1] - clo[1]) + ncl[1] * (cl[2] - clo[2]));
+            if (sa < chunk || sa >= chunk + ns) continue;
+            const int sl = sa - chunk;
+            if (!valid[sl]) continue;
+            const size_t wn = (size_t)node * CELLS + tile_cell<S, P, L>(w[0], w[1], w[2]);
+            for (int v = 0; v < 24; v++)
+                b.new_[v][wn] = musg[0][v][sl] + musg[1][v][sl] * Bn[sl][v * 8 + child];
         }
-        __syncthreads();
-        // 4. a block within the threshold writes its samples' slab cells
-        const bool pass = __int_as_float(face_resid_bits) <= threshold;
-        if (pass) {
-            for (int c = tid; c < NC; c += 256) {
-                uint32_t w[3];
-                ghost_cell_of<S, P>(dir, c, w[0], w[1], w[2]);
-                int cl[3], child = 0;
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const float rp = read_pos<S, P>(w[k], rl[k], rel_scale);
-                    cl[k] = (int)floorf(rp + .5f);
-                    child |= (rp > (float)cl[k] ? 1 : 0) << k;
-                }
-                const int sa = (cl[0] - clo[0]) + ncl[0] * ((cl[1] - clo[1]) + ncl[1] * (cl[2] - clo[2]));
-                if (sa < chunk || sa >= chunk + ns) continue;
-                const int sl = sa - chunk;
-                if (!valid[sl]) continue;
-                const size_t wn = (size_t)node * CELLS + tile_cell<S, P, L>(w[0], w[1], w[2]);
-                for (int v = 0; v < 24; v++)
-                    b.new_[v][wn] = musg[0][v][sl] + musg[1][v][sl] * Bn[sl][v * 8 + child];
-            }
-        }
-        if (tid == 0 && stats) {
-            atomicAdd(&stats[pass ? 0 : 1], 1ull);
+    }
+    if (tid == 0) {
+        if (tile_resid) atomicMax(reinterpret_cast<int *>(tile_resid) + node, face_resid_bits);
+        if (stats) {
+            if (!tile_resid) atomicAdd(&stats[pass ? 0 : 1], 1ull);
             atomicAdd(&stats[2], (unsigned long long)n_invalid);
             atomicAdd(&stats[3], (unsigned long long)ns);
         }
@@ -337,9 +336,9 @@ __global__ void __launch_bounds__(256) k_ghost_learned(FieldBatchDev b, const ui
 }
 
 template <int S, int P>
-static int launch_learned(const am_field_batch *fb, const am_mesh *mesh, const am_geometry *g, float frac,
-                          const float *params, float threshold, unsigned long long *stats, int equal_mode,
-                          void *stream)
+static int launch_learned(const am_field_batch *fb, const am_mesh *mesh, const int32_t *faces, int n_faces,
+                          const am_geometry *g, float frac, const float *params, float threshold,
+                          unsigned long long *stats, float *tile_resid, int equal_mode, void *stream)
 {
     if (fb->n_vars != 24) return set_error(AM_ERR_INVALID, "am_ghost_fill_learned: needs the 24 BSSN fields");
     if (fb->tile_layout != AM_LAYOUT_PADDED && fb->tile_layout != AM_LAYOUT_INTERIOR_FIRST)
@@ -353,21 +352,37 @@ static int launch_learned(const am_field_batch *fb, const am_mesh *mesh, const a
         if ((fb->older[v] == nullptr) != (fb->older[0] == nullptr))
             return set_error(AM_ERR_INVALID, "am_ghost_fill_learned: older must be given for all fields or none");
     }
-    // the tricubic stage fill everywhere, then the learned cells on top
-    if (int rc = ghost_stage_fill(fb, mesh, g, frac, equal_mode, stream)) return rc;
+    // the tricubic stage fill everywhere (with the per-tile gate: except on
+    // the network's faces, clearing the tiles' residuals), then the learned
+    // cells on top
+    LearnedGate gate;
+    if (tile_resid) {
+        gate.resid = tile_resid;
+        gate.mode = 1;
+    }
+    if (int rc = ghost_stage_fill(fb, mesh, faces, n_faces, g, frac, equal_mode, stream, gate)) return rc;
     // samples per face: at most S/2 + 1 coarse cells per tangential axis and
     // P/2 + 1 along the normal (the slab's P fine layers)
     constexpr int NS_MAX = (S / 2 + 1) * (S / 2 + 1) * (P / 2 + 1);
     constexpr int NB = (NS_MAX + 63) / 64;
+    const unsigned items = faces ? (unsigned)n_faces : (unsigned)mesh->n_tiles * 6u;
     if (fb->tile_layout == AM_LAYOUT_INTERIOR_FIRST)
-        hipLaunchKernelGGL((k_ghost_learned<S, P, 1>), dim3(mesh->n_tiles * 6 * NB), dim3(256), 0, (hipStream_t)stream,
-                           d, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, params, threshold,
-                           stats, NB);
+        hipLaunchKernelGGL((k_ghost_learned<S, P, 1>), dim3(items * NB), dim3(512), 0, (hipStream_t)stream, d,
+                           mesh->boundaries, mesh->tiles, faces, mesh->substep, mesh->max_depth, frac, params,
+                           threshold, stats, NB, tile_resid);
     else
-        hipLaunchKernelGGL((k_ghost_learned<S, P, 0>), dim3(mesh->n_tiles * 6 * NB), dim3(256), 0, (hipStream_t)stream,
-                           d, mesh->boundaries, mesh->tiles, mesh->substep, mesh->max_depth, frac, params, threshold,
-                           stats, NB);
-    return check_launch("k_ghost_learned");
+        hipLaunchKernelGGL((k_ghost_learned<S, P, 0>), dim3(items * NB), dim3(512), 0, (hipStream_t)stream, d,
+                           mesh->boundaries, mesh->tiles, faces, mesh->substep, mesh->max_depth, frac, params,
+                           threshold, stats, NB, tile_resid);
+    if (int rc = check_launch("k_ghost_learned")) return rc;
+    if (!tile_resid) return AM_OK;
+    // the per-tile gate: the network's faces take the tricubic fill -- all
+    // their cells for tiles beyond the threshold, else the cells the network
+    // leaves
+    gate.mode = 2;
+    gate.threshold = threshold;
+    gate.stats = stats;
+    return ghost_stage_fill(fb, mesh, faces, n_faces, g, frac, equal_mode, stream, gate);
 }
 
 }  // namespace am
@@ -383,6 +398,26 @@ extern "C" int am_ghost_fill_learned(const am_field_batch *b, const am_mesh *mes
     if (!(stage_frac >= 0.f && stage_frac <= 1.f))
         return am::set_error(AM_ERR_INVALID, "am_ghost_fill_learned: stage_frac %g", (double)stage_frac);
     if (mesh->n_tiles <= 0) return AM_OK;
-    AM_DISPATCH_TILE(*g, am::launch_learned, b, mesh, g, stage_frac, params, threshold,
-                     reinterpret_cast<unsigned long long *>(stats), equal_mode, stream);
+    AM_DISPATCH_TILE(*g, am::launch_learned, b, mesh, nullptr, 0, g, stage_frac, params, threshold,
+                     reinterpret_cast<unsigned long long *>(stats), nullptr, equal_mode, stream);
+}
+
+extern "C" int am_ghost_fill_learned_faces(const am_field_batch *b, const am_mesh *mesh, const int32_t *faces,
+                                           int32_t n_faces, const am_geometry *g, float stage_frac,
+                                           const float *params, float threshold, float *tile_resid, uint64_t *stats,
+                                           int32_t equal_mode, void *stream)
+{
+    if (!b || !mesh || !g || !params || !tile_resid)
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_learned_faces: null argument");
+    if (!mesh->boundaries || !mesh->tiles)
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_learned_faces: no tiles");
+    if (n_faces < 0 || (n_faces > 0 && !faces))
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_learned_faces: %d faces", n_faces);
+    if (equal_mode < AM_EQUAL_FORMULA || equal_mode > AM_EQUAL_SKIP)
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_learned_faces: equal_mode %d", equal_mode);
+    if (!(stage_frac >= 0.f && stage_frac <= 1.f))
+        return am::set_error(AM_ERR_INVALID, "am_ghost_fill_learned_faces: stage_frac %g", (double)stage_frac);
+    if (n_faces == 0) return AM_OK;
+    AM_DISPATCH_TILE(*g, am::launch_learned, b, mesh, faces, n_faces, g, stage_frac, params, threshold,
+                     reinterpret_cast<unsigned long long *>(stats), tile_resid, equal_mode, stream);
 }

@@ -16,6 +16,21 @@ struct FieldBatchDev {
     const float *older[AM_MAX_VARS];
 };
 
+// The learned fill's per-tile gate as the tricubic coarse-face fill sees
+// it.  mode 0: plain fill.  mode 1 (the fill before the network): every
+// listed face except those the network writes (target one level coarser,
+// not the root), clearing resid[slot] of each tile with a listed face, the
+// largest residual the network's blocks then record.  mode 2 (after the
+// network): only the network's faces -- all their cells for a tile whose
+// resid exceeds threshold, only the cells the network leaves for a tile
+// within it; stats[0] / stats[1] count those faces.
+struct LearnedGate {
+    float *resid = nullptr;
+    float threshold = 0.f;
+    unsigned long long *stats = nullptr;
+    int mode = 0;
+};
+
 // padded-tile buffers prolonged by one k_copy_to_child launch
 struct ChildBufs {
     float *buf[3 * AM_MAX_VARS];

@@ -383,7 +383,7 @@ template <int S, int P, int VPG, int L>
 __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_vars, const uint64_t *__restrict__ bnd,
                                                         const int32_t *__restrict__ tiles,
                                                         const int32_t *__restrict__ faces, uint32_t substep,
-                                                        uint32_t maxd, float frac, int equal_mode)
+                                                        uint32_t maxd, float frac, int equal_mode, LearnedGate gate)
 {
     constexpr int T = S + 2 * P, NC = S * S * P;
     constexpr size_t CELLS = tile_stride<S, P, L>();
@@ -400,6 +400,21 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
     const float rl[3] = {relp.x, relp.y, relp.z};
     const uint32_t ddiff = bnd_ddiff(nd);
     if (ddiff == 0 && equal_mode == AM_EQUAL_SKIP) return;
+    // the learned fill's faces (see LearnedGate): skipped before the
+    // network; after it, refilled for a tile beyond the threshold, and for
+    // a tile within it only the cells the network leaves (those whose
+    // coarse cell's 4^3 stencil would read an edge or corner ghost)
+    const bool learned_face = ddiff == 1u && tgt > 0;
+    bool network_cells_kept = false;
+    if (gate.mode == 1) {
+        if (threadIdx.x == 0) gate.resid[node] = 0.f;
+        if (learned_face) return;
+    } else if (gate.mode == 2) {  // uniform per workgroup
+        if (!learned_face) return;
+        network_cells_kept = gate.resid[node] <= gate.threshold;
+        if (threadIdx.x == 0 && blockIdx.y == 0 && gate.stats)
+            atomicAdd(&gate.stats[network_cells_kept ? 0 : 1], 1ull);
+    }
     const float rel_scale = 1.f / (float)(1u << ddiff);
     const float tmp = (float)((substep >> (maxd - (uint32_t)cd)) & ((1u << ddiff) - 1u));
     const float ty = ddiff == 0 ? 1.f : (tmp + frac) / (float)(1u << ddiff);
@@ -486,6 +501,23 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
         a[0] = s_run0[0][q0], a[1] = s_run0[1][q1], a[2] = s_run0[2][q2];
         nn[0] = s_runn[0][q0], nn[1] = s_runn[1][q1], nn[2] = s_runn[2][q2];
     };
+    // a fine ghost cell the network wrote: its coarse cell (nearest target
+    // cell) has a 4^3 stencil (offsets -1..2) inside the target that leaves
+    // the interior along one axis at most
+    auto network_cell = [&](const int (&w)[3]) {
+        int outside = 0;
+        bool in_tile = true;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float r = ((float)w[k] - (float)uP + .5f) * invS - .5f;
+            r = r * rel_scale + rl[k];
+            r = (r + .5f) * (float)uS + (float)uP - 0.5f;
+            const int c = (int)floorf(r + .5f);
+            in_tile = in_tile && c - 1 >= 0 && c + 2 < T;
+            outside += (c - 1 < P || c + 2 >= P + S) ? 1 : 0;
+        }
+        return in_tile && outside <= 1;
+    };
     auto store = [&](int v, const int (&a)[3], const int (&nn)[3], const float (&val)[2][2][2]) {
 #pragma unroll
         for (int cz = 0; cz < 2; cz++)
@@ -494,9 +526,10 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
 #pragma unroll
                 for (int cx = 0; cx < 2; cx++) {
                     if (cx >= nn[0] || cy >= nn[1] || cz >= nn[2]) continue;
-                    const size_t wn = (size_t)node * CELLS + tile_cell<S, P, L>((uint32_t)(wlo[0] + a[0] + cx),
-                                                                                 (uint32_t)(wlo[1] + a[1] + cy),
-                                                                                 (uint32_t)(wlo[2] + a[2] + cz));
+                    const int w[3] = {wlo[0] + a[0] + cx, wlo[1] + a[1] + cy, wlo[2] + a[2] + cz};
+                    if (network_cells_kept && network_cell(w)) continue;
+                    const size_t wn = (size_t)node * CELLS + tile_cell<S, P, L>((uint32_t)w[0], (uint32_t)w[1],
+                                                                                 (uint32_t)w[2]);
                     b.new_[v][wn] = val[cx][cy][cz];
                 }
     };
@@ -587,7 +620,8 @@ constexpr int TRICUBIC_VPG = 2;  // fields per workgroup in the tricubic fill
 
 template <int S, int P, int L>
 static int launch_ghost_multi_l(const FieldBatchDev &d, int n_vars, const am_mesh *mesh, const int32_t *faces,
-                                int n_faces, int interp, float frac, int equal_mode, int skip_learned, void *stream)
+                                int n_faces, int interp, float frac, int equal_mode, int skip_learned,
+                                LearnedGate gate, void *stream)
 {
     const unsigned items = faces ? (unsigned)n_faces : (unsigned)mesh->n_tiles * 6u;
     const dim3 grid(items, (n_vars + GHOST_VPG - 1) / GHOST_VPG);
@@ -603,7 +637,8 @@ static int launch_ghost_multi_l(const FieldBatchDev &d, int n_vars, const am_mes
     else if (interp == AM_INTERP_TRICUBIC && !skip_learned)
         hipLaunchKernelGGL((k_ghost_tricubic<S, P, TRICUBIC_VPG, L>),
                            dim3(items, (n_vars + TRICUBIC_VPG - 1) / TRICUBIC_VPG), dim3(256), 0, st, d, n_vars,
-                           mesh->boundaries, mesh->tiles, faces, mesh->substep, mesh->max_depth, frac, equal_mode);
+                           mesh->boundaries, mesh->tiles, faces, mesh->substep, mesh->max_depth, frac, equal_mode,
+                           gate);
     else if (interp == AM_INTERP_TRICUBIC)
         hipLaunchKernelGGL((k_ghost_fill_multi<S, P, AM_INTERP_TRICUBIC, GHOST_VPG, L>), grid, dim3(256), 0, st, d,
                            n_vars, mesh->boundaries, mesh->tiles, faces, mesh->substep, mesh->max_depth, frac,
@@ -615,7 +650,8 @@ static int launch_ghost_multi_l(const FieldBatchDev &d, int n_vars, const am_mes
 
 template <int S, int P>
 static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, const int32_t *faces, int n_faces,
-                              int interp, float frac, int equal_mode, int skip_learned, void *stream)
+                              int interp, float frac, int equal_mode, int skip_learned, void *stream,
+                              LearnedGate gate = LearnedGate{})
 {
     FieldBatchDev d;
     for (int v = 0; v < b->n_vars; v++) {
@@ -625,7 +661,7 @@ static int launch_ghost_multi(const am_field_batch *b, const am_mesh *mesh, cons
         d.older[v] = b->older[v];
     }
     AM_DISPATCH_LAYOUT(b->tile_layout, launch_ghost_multi_l, d, b->n_vars, mesh, faces, n_faces, interp, frac,
-                       equal_mode, skip_learned, stream);
+                       equal_mode, skip_learned, gate, stream);
 }
 
 template <int S, int P>
@@ -1025,11 +1061,11 @@ extern "C" int am_ghost_fill_stage_faces(const am_field_batch *b, const am_mesh 
     AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, faces, n_faces, interp, stage_frac, equal_mode, 0, stream);
 }
 
-int am::ghost_stage_fill(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, float stage_frac,
-                         int equal_mode, void *stream)
+int am::ghost_stage_fill(const am_field_batch *b, const am_mesh *mesh, const int32_t *faces, int n_faces,
+                         const am_geometry *g, float stage_frac, int equal_mode, void *stream, LearnedGate gate)
 {
-    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, nullptr, 0, AM_INTERP_TRICUBIC, stage_frac, equal_mode, 0,
-                     stream);
+    AM_DISPATCH_TILE(*g, am::launch_ghost_multi, b, mesh, faces, n_faces, AM_INTERP_TRICUBIC, stage_frac, equal_mode,
+                     0, stream, gate);
 }
 
 extern "C" int am_ghost_fill(float *old_, float *new_, const float *older, const am_mesh *mesh, const am_geometry *g,

@@ -484,16 +484,20 @@ def main():
         pl, le, la = rep["plain"], rep["learned"], rep["learned_all"]
         g6 = lambda x: float(f"{x:.6g}")
         result["learned_vs_plain"] = {
-            "workload": WORKLOADS["puncture"] + "; 2 coarse steps",
+            "workload": WORKLOADS["puncture"] % PUNCTURE_KO_SIGMA + "; 2 coarse steps; learned fill gated per "
+                        "tile (a tile whose largest restriction residual exceeds the threshold keeps tricubic)",
             "H_l2_plain": g6(pl["H_l2"]), "H_l2_learned": g6(le["H_l2"]),
             "H_l2_ratio": round(le["H_l2"] / pl["H_l2"], 5),
             "M_l2_plain": g6(pl["M_l2"]), "M_l2_learned": g6(le["M_l2"]),
             "zone_cycles_per_s_plain": round(pl["zone_cycles_per_s"], 1),
             "zone_cycles_per_s_learned": round(le["zone_cycles_per_s"], 1),
+            "zone_cycles_per_s_ratio": round(le["zone_cycles_per_s"] / pl["zone_cycles_per_s"], 4),
             "residual_threshold": rep["threshold"],
             "learned_fill": le["learned"],
             "without_threshold": {"H_l2": g6(la["H_l2"]), "H_l2_ratio": round(la["H_l2"] / pl["H_l2"], 5),
                                   "M_l2": g6(la["M_l2"]), "zone_cycles_per_s": round(la["zone_cycles_per_s"], 1),
+                                  "zone_cycles_per_s_ratio": round(la["zone_cycles_per_s"] / pl["zone_cycles_per_s"],
+                                                                   4),
                                   "learned_fill": la["learned"]},
             "training": {k: rep[k] for k in ("pairs", "train_steps", "batch", "loss_start", "loss_end", "harvest_s",
                                              "train_s")},

@@ -558,6 +558,19 @@ int am_mlp_pairs(const float *const *fine, const int32_t *tiles, int32_t n_tiles
  * cells evaluated. */
 int am_ghost_fill_learned(const am_field_batch *b, const am_mesh *mesh, const am_geometry *g, float stage_frac,
                           const float *params, float threshold, uint64_t *stats, int32_t equal_mode, void *stream);
+/* The same over a list of faces (slot * 6 + face, as am_ghost_fill_stage_
+ * faces) with the gate per tile: the largest restriction residual over
+ * every block of a tile's learned faces is gathered in tile_resid[slot]
+ * (device, one float per slot of the octree, a workspace the call clears
+ * for the tiles it touches); a tile beyond `threshold` keeps the tricubic
+ * fill on all its learned faces, a tile within it takes the network's
+ * values on all of them.  stats (device, 4 uint64, accumulated): faces kept
+ * learned, faces refilled tricubic, cells kept tricubic for their stencil,
+ * cells evaluated.  No counterpart in the reference, whose prolongation is
+ * the fixed interpolation of AMR.h:1053-1115. */
+int am_ghost_fill_learned_faces(const am_field_batch *b, const am_mesh *mesh, const int32_t *faces, int32_t n_faces,
+                                const am_geometry *g, float stage_frac, const float *params, float threshold,
+                                float *tile_resid, uint64_t *stats, int32_t equal_mode, void *stream);
 
 #ifdef __cplusplus
 }

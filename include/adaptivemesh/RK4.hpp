@@ -188,6 +188,7 @@ struct rk4_AMR_data {
     const float *mlp_params = nullptr;
     float mlp_threshold = 1e-5f;
     smart_gpu_buffer<uint64_t> mlp_stats;
+    smart_gpu_buffer<float> mlp_tile_resid;  // per slot: the per-tile gate's residual
 
     rk4_AMR_data(uint32_t node_slots, AMR<rk4_AMR_data> &p, const am_geometry &g, const rk4_options &o,
                  const options_t &so)
@@ -217,6 +218,7 @@ struct rk4_AMR_data {
             older = smart_gpu_buffer<float>(NV * slots * cells);
         mlp_stats = smart_gpu_buffer<uint64_t>(4);
         mlp_stats.zero();
+        if (opt.interp == AM_INTERP_LEARNED) mlp_tile_resid = smart_gpu_buffer<float>(slots);
         d1 = smart_gpu_buffer<uint32_t>(NV * word_stride / 4);
         if (n_d2) d2d = smart_gpu_buffer<uint32_t>(n_d2 * word_stride / 4);
         if (n_off) d2o = smart_gpu_buffer<uint32_t>(n_off * word_stride / 4);
@@ -234,6 +236,7 @@ struct rk4_AMR_data {
         old_.destroy(); new_.destroy(); acc.destroy(); older.destroy(); conv.destroy(); adv.destroy();
         d1.destroy(); d2d.destroy(); d2o.destroy(); chr.destroy(); partial.destroy(); seed_table.destroy();
         mlp_stats.destroy();
+        mlp_tile_resid.destroy();
         timer.destroy();
         am_stream_destroy(stream);
         stream = nullptr;
@@ -482,12 +485,20 @@ struct rk4_AMR_data {
         if (d > 0 && parent.coarse_faces_at_depth(d) > 0) {
             am_field_batch b = batch(true);
             ev = timer.begin(kernel_timer::GHOST);
-            if (opt.interp == AM_INTERP_LEARNED)
-                ghost_fill(b, m, frac[s], AM_EQUAL_SKIP);
-            else  // only the faces whose target is coarser have cells to fill
-                check(am_ghost_fill_stage_faces(&b, &m, parent.coarse_face_list(d), parent.coarse_faces_at_depth(d),
-                                                &geom, frac[s], opt.interp, AM_EQUAL_SKIP, stream),
+            // only the faces whose target is coarser have cells to fill
+            const int32_t *faces = parent.coarse_face_list(d);
+            const int32_t n_faces = parent.coarse_faces_at_depth(d);
+            if (opt.interp == AM_INTERP_LEARNED) {
+                if (!mlp_params) throw std::runtime_error("learned prolongation selected but no parameters set");
+                check(am_ghost_fill_learned_faces(&b, &m, faces, n_faces, &geom, frac[s], mlp_params, mlp_threshold,
+                                                  mlp_tile_resid.gpu_buffer_ptr, mlp_stats.gpu_buffer_ptr,
+                                                  AM_EQUAL_SKIP, stream),
+                      "learned ghost fill");
+            } else {
+                check(am_ghost_fill_stage_faces(&b, &m, faces, n_faces, &geom, frac[s], opt.interp, AM_EQUAL_SKIP,
+                                                stream),
                       "ghost");
+            }
             timer.end(ev);
         }
     }

@@ -129,8 +129,9 @@ def test_learned_fill_falls_back_to_tricubic(am, orc, torch_cuda, setup):
 
 def test_driver_learned_fallback_equals_tricubic(am, orc, torch_cuda, setup):
     """The RK4 driver with the learned prolongation and a threshold every
-    face exceeds runs bit for bit like the tricubic driver over a coarse step
-    (its statistics count every learned face as a fallback)."""
+    tile exceeds runs bit for bit like the tricubic driver over a coarse step
+    (the per-tile gate refills every learned face; its statistics count
+    them all as fallbacks)."""
     torch = torch_cuda
     ops, n = WL.puncture_ops(OUTER)
     g = am.geometry(S, P, ORDER, OUTER, CFL)
@@ -154,4 +155,81 @@ def test_driver_learned_fallback_equals_tricubic(am, orc, torch_cuda, setup):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (w, v)
     stats = sims[1].prolongation_stats()
     print(f"  driver fallback statistics {stats}", flush=True)
-    assert stats["learned_blocks"] == 0 and stats["fallback_blocks"] > 0
+    assert stats["learned_faces"] == 0 and stats["fallback_faces"] > 0
+
+
+def test_learned_faces_gate_per_tile(am, orc, torch_cuda, setup):
+    """am_ghost_fill_learned_faces (the driver's call: a face list and the
+    gate per tile) on the depth-5 faces whose target is coarser: with an
+    unreachable threshold it writes what am_ghost_fill_learned writes with
+    its per-block gate passing every block, with a negative one what it
+    writes with none passing (the tricubic fill), bit for bit; with the
+    median of the tiles' residuals, each tile's slot equals the first
+    result when its residual is within the threshold and the second
+    otherwise, and the statistics count its learned faces accordingly."""
+    torch = torch_cuda
+    s = setup
+    sim, g, e = s["sim"], s["g"], s["e"]
+    faces, learned_faces = [], {}
+    for t in s["tiles"]:
+        for f in range(6):
+            nd = int(e["boundaries"][t * 6 + f]) & 0xFFFFFFFF
+            tgt = nd & 0xFFFFFF
+            if tgt & 0x800000 or nd >> 28 in (0, 15) or not (nd >> 24) & 15:
+                continue
+            faces.append(t * 6 + f)
+            if (nd >> 24) & 15 == 1 and tgt > 0:
+                learned_faces[t] = learned_faces.get(t, 0) + 1
+    dfaces = torch.tensor(np.array(faces, np.int32), device="cuda")
+    prm = torch.tensor(s["params"], device="cuda")
+    resid = torch.full((s["n"],), float("nan"), dtype=torch.float32, device="cuda")
+
+    def run(threshold, per_tile):
+        for v in range(24):
+            sim.set_field(1, v, s["bufs"][1][v])
+        fb = am.FieldBatch()
+        fb.n_vars = 24
+        fb.tile_layout = am.AM_LAYOUT_INTERIOR_FIRST
+        for v in range(24):
+            fb.old_[v], fb.new_[v], fb.older[v] = sim.field_ptr(0, v), sim.field_ptr(1, v), sim.field_ptr(3, v)
+        m = s["tree"].mesh(5)
+        m.substep = 1
+        stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+        if per_tile:
+            am.check(am.lib().am_ghost_fill_learned_faces(C.byref(fb), C.byref(m), dfaces.data_ptr(), len(faces),
+                                                          C.byref(g), 0.5, prm.data_ptr(), threshold,
+                                                          resid.data_ptr(), stats.data_ptr(), am.AM_EQUAL_SKIP,
+                                                          am.stream_handle()), "learned faces")
+        else:
+            am.check(am.lib().am_ghost_fill_learned(C.byref(fb), C.byref(m), C.byref(g), 0.5, prm.data_ptr(),
+                                                    threshold, stats.data_ptr(), am.AM_EQUAL_SKIP,
+                                                    am.stream_handle()), "learned fill")
+        torch.cuda.synchronize()
+        return [sim.get_field(1, v) for v in range(24)], [int(x) for x in stats.cpu().numpy()]
+
+    every, _ = run(1e30, False)
+    none, _ = run(-1.0, False)
+    got, st = run(1e30, True)
+    r = resid.cpu().numpy()
+    n_learned = sum(learned_faces.values())
+    assert st[0] == n_learned > 0 and st[1] == 0
+    for v in range(24):
+        assert np.array_equal(got[v].view(np.uint32), every[v].view(np.uint32)), v
+    got, st = run(-1.0, True)
+    assert st[0] == 0 and st[1] == n_learned
+    for v in range(24):
+        assert np.array_equal(got[v].view(np.uint32), none[v].view(np.uint32)), v
+    tr = np.array([r[t] for t in learned_faces])
+    assert np.all(np.isfinite(tr)) and np.all(tr >= 0)
+    thr = float(np.median(tr))
+    got, st = run(thr, True)
+    keep = [t for t in learned_faces if r[t] <= thr]
+    assert 0 < len(keep) < len(learned_faces)
+    assert st[0] == sum(learned_faces[t] for t in keep) and st[0] + st[1] == n_learned
+    T3 = T ** 3
+    for v in range(24):
+        a, w_every, w_none = (x.reshape(s["n"], T3) for x in (got[v], every[v], none[v]))
+        for t in learned_faces:
+            want = w_every[t] if t in keep else w_none[t]
+            assert np.array_equal(a[t].view(np.uint32), want.view(np.uint32)), (v, t)
+    print(f"  per-tile gate at {thr:.3e}: {len(keep)} of {len(learned_faces)} tiles keep the network", flush=True)
