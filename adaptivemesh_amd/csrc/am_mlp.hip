@@ -93,14 +93,34 @@ __global__ void __launch_bounds__(256) k_gemm(const float *__restrict__ A, int64
             }
 }
 
-// bias gradient: out[n] = sum_m dz[m][n]
-__global__ void __launch_bounds__(256) k_colsum(const float *__restrict__ dz, int M, int N, float *__restrict__ out)
+// bias gradient: out[n] = sum_m dz[m][n].  One workgroup per 64 columns,
+// 16 waves: wave w sums rows w, w + 16, ... (lanes on consecutive columns,
+// 16 rows in flight), then the 16 partial sums are added in wave order, so
+// the result does not depend on scheduling.
+__global__ void __launch_bounds__(1024) k_colsum(const float *__restrict__ dz, int M, int N, float *__restrict__ out)
 {
-    const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n >= N) return;
+    __shared__ float part[16][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + lane;
     float s = 0.f;
-    for (int m = 0; m < M; m++) s += dz[(int64_t)m * N + n];
-    out[n] = s;
+    if (n < N) {
+        int m = wave;
+        for (; m + 15 * 16 < M; m += 16 * 16) {
+            float v[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) v[u] = dz[(int64_t)(m + 16 * u) * N + n];
+#pragma unroll
+            for (int u = 0; u < 16; u++) s += v[u];
+        }
+        for (; m < M; m += 16) s += dz[(int64_t)m * N + n];
+    }
+    part[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && n < N) {
+        float t = part[0][lane];
+        for (int w = 1; w < 16; w++) t += part[w][lane];
+        out[n] = t;
+    }
 }
 
 // dy = 2 (y - t) / count; per-workgroup FP64 sum of (y - t)^2
@@ -282,7 +302,8 @@ extern "C" int am_mlp_backward(const float *params, const float *x, int32_t batc
     if (int rc = gemm(w->h2, 1, MLP_HID, w->dy, MLP_OUT, 1, grads + MLP_W3, MLP_OUT, MLP_HID, MLP_OUT, batch, nullptr,
                       EPI_PLAIN, nullptr, nullptr, st))
         return rc;
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, st, w->dy, batch, MLP_OUT, grads + MLP_B3);
+    hipLaunchKernelGGL(k_colsum, dim3((MLP_OUT + 63) / 64), dim3(1024), 0, st, w->dy, batch, MLP_OUT,
+                       grads + MLP_B3);
     if (int rc = gemm(w->dy, MLP_OUT, 1, params + MLP_W3, 1, MLP_OUT, w->dz2, MLP_HID, batch, MLP_HID, MLP_OUT,
                       nullptr, EPI_DGELU, w->z2, nullptr, st))
         return rc;
@@ -290,7 +311,8 @@ extern "C" int am_mlp_backward(const float *params, const float *x, int32_t batc
     if (int rc = gemm(w->h1, 1, MLP_HID, w->dz2, MLP_HID, 1, grads + MLP_W2, MLP_HID, MLP_HID, MLP_HID, batch,
                       nullptr, EPI_PLAIN, nullptr, nullptr, st))
         return rc;
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, st, w->dz2, batch, MLP_HID, grads + MLP_B2);
+    hipLaunchKernelGGL(k_colsum, dim3((MLP_HID + 63) / 64), dim3(1024), 0, st, w->dz2, batch, MLP_HID,
+                       grads + MLP_B2);
     if (int rc = gemm(w->dz2, MLP_HID, 1, params + MLP_W2, 1, MLP_HID, w->dz1, MLP_HID, batch, MLP_HID, MLP_HID,
                       nullptr, EPI_DGELU, w->z1, nullptr, st))
         return rc;
@@ -298,7 +320,8 @@ extern "C" int am_mlp_backward(const float *params, const float *x, int32_t batc
     if (int rc = gemm(x, 1, MLP_IN, w->dz1, MLP_HID, 1, grads + MLP_W1, MLP_HID, MLP_IN, MLP_HID, batch, nullptr,
                       EPI_PLAIN, nullptr, nullptr, st))
         return rc;
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, st, w->dz1, batch, MLP_HID, grads + MLP_B1);
+    hipLaunchKernelGGL(k_colsum, dim3((MLP_HID + 63) / 64), dim3(1024), 0, st, w->dz1, batch, MLP_HID,
+                       grads + MLP_B1);
     return check_launch("k_colsum");
 }
 

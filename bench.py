@@ -43,11 +43,13 @@ PUNCTURE_KO_SIGMA = 0.3
 # written once per launch)
 WORD = 4
 Q = FD_ORDER // 2  # stencil half-width: the ghost layers the FD reads and the push writes
-# HBM bytes per depth-3 k_bssn_stage launch from rocprofv3 --pmc FETCH_SIZE /
-# WRITE_SIZE (profiles/bssn_rk4_512tiles_hbm_counters.md); None until read on
-# the current kernels
-RHS_TRAFFIC_PER_LAUNCH = None
-RHS_ALGORITHMIC_PER_LAUNCH = None
+# HBM bytes per depth-3 k_bssn_stage launch (2,097,152 cells), averaged over
+# the four stages, from rocprofv3 --pmc FETCH_SIZE (x 2.000, the copy and
+# read probes' calibration) and WRITE_SIZE (x 1.000)
+# (profiles/bssn_rk4_512tiles_hbm_counters.md), and the algorithmic bytes of
+# the same launches
+RHS_TRAFFIC_PER_LAUNCH = 1355523224
+RHS_ALGORITHMIC_PER_LAUNCH = 1354760192
 TRAFFIC_DIRECT_FACES = 4  # the mask the counters were read with
 
 
@@ -291,26 +293,29 @@ def cpu_kernel_table(n_slots, fields, export, sub=64):
 
 
 def copy_probe(mib=1024, reps=10):
-    """A plain device-to-device copy of `mib` MiB: the reference point for
-    the HBM byte counters (it reads and writes exactly that many bytes) and
-    the achievable copy bandwidth."""
+    """A plain device-to-device copy of `mib` MiB, and a sum over the same
+    buffer: the reference points for the HBM byte counters (the copy reads
+    and writes exactly that many bytes, the sum's main kernel only reads
+    them) and the achievable copy and read bandwidths."""
     import torch
     n = mib * 2 ** 20 // 4
     a = torch.ones(n, device="cuda")
     b = torch.empty_like(a)
-    for _ in range(2):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    print(json.dumps({"copy_probe": {"bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
-                                     "avg_us": round(1e3 * ms, 2),
-                                     "GBps": round(8 * n / (ms * 1e-3) / 1e9, 1)}}), flush=True)
+    out = {}
+    for name, fn, moved in (("copy_probe", lambda: b.copy_(a), 8 * n), ("read_probe", lambda: a.sum(), 4 * n)):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n if moved == 8 * n else 0,
+                     "avg_us": round(1e3 * ms, 2), "GBps": round(moved / (ms * 1e-3) / 1e9, 1)}
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -337,7 +342,8 @@ def main():
                     help="print only a markdown table of the reference's kernel bodies (the oracle's restatement) and "
                          "the FP32 implementation's passes timed on this host, 1 thread and all threads")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="time only a plain 1 GiB device copy (the calibration for HBM byte counters)")
+                    help="time only a plain 1 GiB device copy and a 1 GiB sum (the calibrations for HBM byte "
+                         "counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: sharding this path over several GPUs is a design only; "
@@ -444,8 +450,9 @@ def main():
             roofline["note"] = (f"byte model from this mesh's faces: {eq_faces:.2f} equal-depth and "
                                 f"{coarse_faces:.2f} coarser neighbours per advanced tile")
         if roofline["traffic"]:
-            roofline["traffic_note"] = ("HBM bytes per depth-3 launch from rocprofv3 FETCH_SIZE/WRITE_SIZE; "
-                                        f"{RHS_TRAFFIC_PER_LAUNCH / RHS_ALGORITHMIC_PER_LAUNCH:.2f}x the algorithmic "
+            roofline["traffic_note"] = ("HBM bytes per depth-3 launch (mean of the four stages) from rocprofv3 "
+                                        "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000; "
+                                        f"{RHS_TRAFFIC_PER_LAUNCH / RHS_ALGORITHMIC_PER_LAUNCH:.4f}x the algorithmic "
                                         "bytes of the same launches")
 
     cons = sim.constraints()
