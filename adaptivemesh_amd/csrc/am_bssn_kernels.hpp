@@ -50,9 +50,9 @@ __device__ __forceinline__ void hess6(const void *bd, const void *bo, size_t w, 
 // sum_ij a^ij b_ij for symmetric a, b
 __device__ __forceinline__ float sym_dot(const float *a, const float *b)
 {
-    float s = 0.f;
+    float s = a[0] * b[0];
 #pragma unroll
-    for (int c = 0; c < 6; c++) s += MULT[c] * a[c] * b[c];
+    for (int c = 1; c < 6; c++) s += MULT[c] * a[c] * b[c];
     return s;
 }
 
@@ -60,18 +60,38 @@ __device__ __forceinline__ float sym_dot(const float *a, const float *b)
 // all loaded by load(), up front, so that a wave has its 46 word loads in
 // flight together instead of one memory round trip per word decoded; the
 // study formats (12 and 6 bytes per word) are read where they are used.
+// The cell's loads and stores address each array as its base pointer (a
+// kernel argument, in scalar registers) plus a 32-bit byte offset per lane
+// that every array of the same kind shares, so no lane does 64-bit address
+// arithmetic per array.  A byte offset fits 32 bits because one field array
+// of a stage call stays below 4 GiB: the 24 fields' state, y_n, new and
+// accumulator plus the 46 word arrays of n slots take more than 72 times one
+// field array, beyond the HBM of one GPU long before a field reaches 4 GiB
+// (the drivers check it when they allocate).
+template <class T>
+__device__ __forceinline__ T ld_at(const T *base, uint32_t byte_off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+template <class T>
+__device__ __forceinline__ void st_at(T *base, uint32_t byte_off, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off) = v;
+}
+
 template <int FMT>
 struct CellWords {
     uint32_t d1[NV], dd[NH], doff[NH];
-    __device__ __forceinline__ void load(const BssnArgsDev &a, size_t w)
+    // wo: the cell's byte offset in every word array
+    __device__ __forceinline__ void load(const BssnArgsDev &a, uint32_t wo)
     {
         if constexpr (FMT == WORD_CF3) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) d1[v] = static_cast<const uint32_t *>(a.d1[v])[w];
+            for (int v = 0; v < NV; v++) d1[v] = ld_at(static_cast<const uint32_t *>(a.d1[v]), wo);
 #pragma unroll
             for (int h = 0; h < NH; h++) {
-                dd[h] = static_cast<const uint32_t *>(a.d2d[h])[w];
-                doff[h] = static_cast<const uint32_t *>(a.d2o[h])[w];
+                dd[h] = ld_at(static_cast<const uint32_t *>(a.d2d[h]), wo);
+                doff[h] = ld_at(static_cast<const uint32_t *>(a.d2o[h]), wo);
             }
         }
     }
@@ -113,7 +133,10 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, const float (&u)
         const float3 d = cw.diag(a, w, hh), o = cw.off(a, w, hh);
         h[0] = d.x; h[1] = o.x; h[2] = o.y; h[3] = d.y; h[4] = o.z; h[5] = d.z;
     };
-    const float chi = u[0], ichi = 1.f / fmaxf(chi, par.chi_floor);
+    // the two reciprocals by the hardware's (1 ulp; the RHS is not
+    // bit-exact with anything, and correctly rounded division costs ten
+    // instructions each)
+    const float chi = u[0], ichi = __builtin_amdgcn_rcpf(fmaxf(chi, par.chi_floor));
     const float K = u[7], alpha = u[17];
     const float *g = u + 1, *At = u + 8, *Gam = u + 14, *beta = u + 18, *B = u + 21;
 
@@ -121,7 +144,7 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, const float (&u)
     float gu[6];
     {
         const float c0 = g[3] * g[5] - g[4] * g[4], c1 = g[2] * g[4] - g[1] * g[5], c2 = g[1] * g[4] - g[2] * g[3];
-        const float idet = 1.f / (g[0] * c0 + g[1] * c1 + g[2] * c2);
+        const float idet = __builtin_amdgcn_rcpf(g[0] * c0 + g[1] * c1 + g[2] * c2);
         gu[0] = c0 * idet;
         gu[1] = c1 * idet;
         gu[2] = c2 * idet;
@@ -175,14 +198,12 @@ __device__ __forceinline__ void bssn_cell(const BssnArgsDev &a, const float (&u)
             Atu[SI[i][j]] = Atm[i][0] * gu[SI[0][j]] + Atm[i][1] * gu[SI[1][j]] + Atm[i][2] * gu[SI[2][j]];
     const float AA = sym_dot(At, Atu);
 
-    float dchi_dal = 0.f, dchi2 = 0.f;
+    // gt^kl d_l chi, then its contractions with d chi and d alpha
+    float uchi[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-        for (int l = 0; l < 3; l++) {
-            dchi_dal += gu[SI[k][l]] * dchi[k] * dal[l];
-            dchi2 += gu[SI[k][l]] * dchi[k] * dchi[l];
-        }
+    for (int k = 0; k < 3; k++) uchi[k] = gu[SI[k][0]] * dchi[0] + gu[SI[k][1]] * dchi[1] + gu[SI[k][2]] * dchi[2];
+    const float dchi_dal = uchi[0] * dal[0] + uchi[1] * dal[1] + uchi[2] * dal[2];
+    const float dchi2 = uchi[0] * dchi[0] + uchi[1] * dchi[1] + uchi[2] * dchi[2];
 
     // conformal covariant Hessians of chi and alpha
     float DDchi[6], DDalt[6];
@@ -362,7 +383,9 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
         const int cell = cb * 256 + (int)threadIdx.x;
         if (cell >= S3) continue;
         const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
-        const size_t r = (size_t)node * TS + tile_interior<S, P, L>(x, y, z);
+        // the cell's byte offsets in the field arrays and in the interior-only arrays
+        const uint32_t ro = 4u * ((uint32_t)node * (uint32_t)TS + tile_interior<S, P, L>(x, y, z));
+        const uint32_t wo = 4u * ((uint32_t)node * (uint32_t)S3 + (uint32_t)cell);
         const size_t w = (size_t)node * S3 + cell;
         // Loads in two batches, each issued whole before it is waited on:
         // the state and the 46 words before the RHS, y_n and acc of all 24
@@ -374,22 +397,27 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
         // more registers through the RHS: one wave per SIMD, and slower.
         float u[NV], yn[NV], ac[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) u[v] = a.state[v][r];
+        for (int v = 0; v < NV; v++) u[v] = ld_at(a.state[v], ro);
         CellWords<FMT> cw;
-        cw.load(a, w);
+        cw.load(a, wo);
         float aw[UPW ? NV : 1];
         if constexpr (UPW) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) aw[v] = a.adv[v][w];
+            for (int v = 0; v < NV; v++) aw[v] = ld_at(a.adv[v], wo);
         }
+        // Nothing moves across this point: all 70 loads of the first batch
+        // are issued before the RHS starts.  Left free, the scheduler
+        // defers about 30 of them into the RHS, and each deferred group
+        // costs the wave another memory round trip (1.7% slower).
+        __builtin_amdgcn_sched_barrier(0);
         float k[NV];
         bssn_cell<FMT, true, false, UPW>(a, u, cw, w, par, k, nullptr, nullptr, aw);
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            yn[v] = STAGE == 0 ? u[v] : a.old_[v][r];
-            ac[v] = STAGE == 0 ? 0.f : a.acc[v][w];
+            yn[v] = STAGE == 0 ? u[v] : ld_at(a.old_[v], ro);
+            ac[v] = STAGE == 0 ? 0.f : ld_at(a.acc[v], wo);
         }
-        rk4_combine_in<NV, STAGE>(a.acc, a.new_, k, yn, ac, r, w, sc.dt[depth]);
+        rk4_combine_at<NV, STAGE>(a.acc, a.new_, k, yn, ac, ro, wo, sc.dt[depth]);
         if constexpr (PUSH && L == 1 && S * S == 256) {
             // y and z faces are whole slab rows per wave already; the x
             // faces' columns are transposed through LDS
@@ -412,13 +440,14 @@ __global__ void __launch_bounds__(256) k_bssn_constraints(BssnArgsDev a, TileArg
     double h2 = 0.0, m2 = 0.0, cnt = 0.0, hmax = 0.0;
     if (cell < S3) {
         const int z = cell / (S * S), y = (cell / S) % S, x = cell % S;
-        const size_t r = (size_t)node * tile_stride<S, P, L>() + tile_interior<S, P, L>(x, y, z);
+        const uint32_t ro = 4u * ((uint32_t)node * tile_stride<S, P, L>() + tile_interior<S, P, L>(x, y, z));
+        const uint32_t wo = 4u * ((uint32_t)node * (uint32_t)S3 + (uint32_t)cell);
         const size_t w = (size_t)node * S3 + cell;
         float H, M[3], u[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) u[v] = a.state[v][r];
+        for (int v = 0; v < NV; v++) u[v] = ld_at(a.state[v], ro);
         CellWords<FMT> cw;
-        cw.load(a, w);
+        cw.load(a, wo);
         bssn_cell<FMT, false, true>(a, u, cw, w, par, nullptr, &H, M);
         h2 = (double)H * (double)H;
         m2 = (double)M[0] * M[0] + (double)M[1] * M[1] + (double)M[2] * M[2];

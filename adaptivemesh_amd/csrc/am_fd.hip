@@ -301,12 +301,24 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
         for (int i = 0; i <= 2 * P; i++) pl[i] = reinterpret_cast<const float *>(ring4[(z + i) % R]);
         const float *ctr = pl[P];
         const uint32_t gz = (uint32_t)(slot * S + z);
-        const size_t o = ((size_t)slot * S + z) * (S * S) + (size_t)tid;
+        // the word index (the word arrays stay far below 2^32 cells)
+        const uint32_t o = (gz * (uint32_t)S + (uint32_t)y) * (uint32_t)S + (uint32_t)x;
 
+        // The reference sums the 2Q+1 products from 0 in offset order.  For
+        // cf3 words the sums start from the first product instead: that
+        // changes only the sign of a sum whose products are all zero, and
+        // +0 and -0 pack to the same word (exponent field 0, and the dither
+        // added to a zero of either sign gives the same value).  The study
+        // formats store the value itself, so they keep the 0 + start.
+        constexpr bool FROM0 = FMT != WORD_CF3;
         if (want_d) {
-            float vx = 0.f, vy = 0.f, vz = 0.f;
+            float vx = ctr[c - Q] * C.d1[BASE - Q], vy = ctr[c - Q * T] * C.d1[BASE - Q],
+                  vz = pl[P - Q][c] * C.d1[BASE - Q];
+            if constexpr (FROM0) {
+                vx = 0.f + vx; vy = 0.f + vy; vz = 0.f + vz;
+            }
 #pragma unroll
-            for (int i = -Q; i <= Q; i++) {
+            for (int i = -Q + 1; i <= Q; i++) {
                 vx = vx + ctr[c + i] * C.d1[BASE + i];
                 vy = vy + ctr[c + i * T] * C.d1[BASE + i];
                 vz = vz + pl[P + i][c] * C.d1[BASE + i];
@@ -314,9 +326,13 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
             vx = vx * dx1; vy = vy * dx1; vz = vz * dx1;
             int32_t rng = thread_seed_z(sxy_d, gz);
             if (o_d2) {  // uniform: the second-derivative sums only where they are stored
-                float xx = 0.f, yy = 0.f, zz = 0.f;
+                float xx = ctr[c - Q] * C.d2[BASE - Q], yy = ctr[c - Q * T] * C.d2[BASE - Q],
+                      zz = pl[P - Q][c] * C.d2[BASE - Q];
+                if constexpr (FROM0) {
+                    xx = 0.f + xx; yy = 0.f + yy; zz = 0.f + zz;
+                }
 #pragma unroll
-                for (int i = -Q; i <= Q; i++) {
+                for (int i = -Q + 1; i <= Q; i++) {
                     xx = xx + ctr[c + i] * C.d2[BASE + i];
                     yy = yy + ctr[c + i * T] * C.d2[BASE + i];
                     zz = zz + pl[P + i][c] * C.d2[BASE + i];

@@ -44,34 +44,36 @@ __device__ __forceinline__ void rk4_combine(const float *const *state, const flo
     }
 }
 
-// L: tile layout (0 padded, 1 interior-first).  In the interior-first layout
-// the cell lands in the target's slab of the opposite face, layer = its
-// distance from the shared face, so the cells of one row of this tile go to
-// one contiguous slab row of the target.
-// The same combine with the stage's inputs already loaded: yn[v] = y_n and
-// ac[v] = acc (stages 1-3) of the cell.
+// The same combine with the stage's inputs already loaded (yn[v] = y_n and
+// ac[v] = acc of the cell, stages 1-3), the cell addressed by its byte
+// offsets ro in the field arrays and wo in the accumulator arrays.
 template <int NV, int STAGE>
-__device__ __forceinline__ void rk4_combine_in(float *const *acc, float *const *new_, float (&k)[NV],
-                                               const float (&yn)[NV], const float (&ac)[NV], size_t r, size_t w,
-                                               float dt)
+__device__ __forceinline__ void rk4_combine_at(float *const *acc, float *const *new_, float (&k)[NV],
+                                               const float (&yn)[NV], const float (&ac)[NV], uint32_t ro,
+                                               uint32_t wo, float dt)
 {
 #pragma unroll
     for (int v = 0; v < NV; v++) {
         float nv;
+        float *ap = reinterpret_cast<float *>(reinterpret_cast<char *>(acc[v]) + wo);
         if (STAGE == 0) {
-            acc[v][w] = k[v];
+            *ap = k[v];
             nv = yn[v] + (.5f * dt) * k[v];
         } else if (STAGE == 1 || STAGE == 2) {
-            acc[v][w] = ac[v] + 2.f * k[v];
+            *ap = ac[v] + 2.f * k[v];
             nv = yn[v] + (STAGE == 1 ? .5f * dt : dt) * k[v];
         } else {
             nv = yn[v] + (dt * (1.f / 6.f)) * (ac[v] + k[v]);
         }
-        new_[v][r] = nv;
+        *reinterpret_cast<float *>(reinterpret_cast<char *>(new_[v]) + ro) = nv;
         k[v] = nv;  // the stage output, for the push
     }
 }
 
+// L: tile layout (0 padded, 1 interior-first).  In the interior-first layout
+// the cell lands in the target's slab of the opposite face, layer = its
+// distance from the shared face, so the cells of one row of this tile go to
+// one contiguous slab row of the target.
 template <int NV, int S, int P, int L = 0>
 __device__ __forceinline__ void rk4_push(float *const *new_, const float (&y)[NV], const uint64_t *boundaries, int node,
                                          int x, int y_, int z, int layers, int axes)

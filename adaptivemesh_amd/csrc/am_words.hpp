@@ -13,13 +13,15 @@ enum { WORD_CF3 = 0, WORD_F32 = 1, WORD_F16 = 2 };
 // bytes per cell of one word array
 __host__ __device__ constexpr int word_bytes(int fmt) { return fmt == WORD_F32 ? 12 : fmt == WORD_F16 ? 6 : 4; }
 
+// o: the cell's index in the word array (below 2^30: the cf3 store
+// addresses it as a 32-bit byte offset from the array's base)
 template <int FMT>
-__device__ __forceinline__ void word_put(void *base, size_t o, float x, float y, float z, int32_t &rng)
+__device__ __forceinline__ void word_put(void *base, uint32_t o, float x, float y, float z, int32_t &rng)
 {
     if (FMT == WORD_CF3) {
-        static_cast<uint32_t *>(base)[o] = cf3_pack(x, y, z, rng);
+        *reinterpret_cast<uint32_t *>(static_cast<char *>(base) + 4u * o) = cf3_pack(x, y, z, rng);
     } else if (FMT == WORD_F32) {
-        float *p = static_cast<float *>(base) + 3 * o;
+        float *p = static_cast<float *>(base) + 3 * (size_t)o;
         p[0] = x; p[1] = y; p[2] = z;
     } else {
         // The fp32 values are materialised before the conversion: left to
@@ -28,7 +30,7 @@ __device__ __forceinline__ void word_put(void *base, size_t o, float x, float y,
         // exact product once instead of the fp32 product, and turns a -0
         // product into +0.  The empty asm only pins the value in a register.
         asm volatile("" : "+v"(x), "+v"(y), "+v"(z));
-        _Float16 *p = static_cast<_Float16 *>(base) + 3 * o;
+        _Float16 *p = static_cast<_Float16 *>(base) + 3 * (size_t)o;
         p[0] = (_Float16)x; p[1] = (_Float16)y; p[2] = (_Float16)z;
     }
 }

@@ -250,6 +250,8 @@ typedef struct am_fd_batch {
 enum am_word_format { AM_WORD_CF3 = 0, AM_WORD_F32 = 1, AM_WORD_F16 = 2 };
 /* bytes per cell of one word array in `format` (4, 12, 6), -1 if unknown */
 int32_t am_word_bytes(int32_t format);
+/* cf3 word arrays are addressed with 32-bit byte offsets: every listed slot
+ * must satisfy (slot + 1) * S^3 <= 2^30. */
 int am_fd_batched(const am_fd_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream);
 
 /* ------------------------------------------------- ghost cells and tree */
@@ -444,7 +446,11 @@ typedef struct am_bssn_fields {
  * new = old + dt/6 (acc + k), with dt = outer_dt / 2^depth.  With
  * push_ghosts != 0 (mesh->boundaries required) every new value within P
  * cells of a face whose neighbour has the same depth is also written into
- * that neighbour's face ghost cell: the AM_EQUAL_COPY fill, fused. */
+ * that neighbour's face ghost cell: the AM_EQUAL_COPY fill, fused.  Each
+ * array is addressed with 32-bit byte offsets: every slot a tile list names
+ * must lie within the first 4 GiB of each array (slot < 2^32 / (4 x the
+ * tile stride)), which the 118 arrays of one call reach only beyond the HBM
+ * of one GPU. */
 int am_bssn_rhs_stage(const am_bssn_fields *f, int32_t stage, const am_bssn_params *p, const am_mesh *mesh,
                       const am_geometry *g, int32_t push_ghosts, void *stream);
 

@@ -211,6 +211,9 @@ struct rk4_AMR_data {
             throw std::runtime_error("word format " + std::to_string(opt.word_format) +
                                      " needs the BSSN system on S=16, P=3, fd_order 4 tiles");
         word_stride = slots * inner * (size_t)am_word_bytes(opt.word_format);
+        // the stage and FD kernels address one field or word array with 32-bit byte offsets
+        if (slots * cells * sizeof(float) > 0xFFFFFFFFull || word_stride > 0xFFFFFFFFull)
+            throw std::runtime_error(std::to_string(slots) + " slots: one field array would exceed 4 GiB");
         old_ = smart_gpu_buffer<float>(NV * slots * cells);
         new_ = smart_gpu_buffer<float>(NV * slots * cells);
         acc = smart_gpu_buffer<float>(NV * slots * inner);
