@@ -405,11 +405,14 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
 #pragma unroll
             for (int v = 0; v < NV; v++) aw[v] = ld_at(a.adv[v], wo);
         }
-        // Nothing moves across this point: all 70 loads of the first batch
-        // are issued before the RHS starts.  Left free, the scheduler
-        // defers about 30 of them into the RHS, and each deferred group
-        // costs the wave another memory round trip (1.7% slower).
-        __builtin_amdgcn_sched_barrier(0);
+        // Stages 0 and 3: nothing moves across this point, so all 70 loads
+        // of the first batch are issued before the RHS starts.  Left free,
+        // the scheduler defers about 30 of them into the RHS, each deferred
+        // group costing the wave another memory round trip: depth-3 stage 0
+        // 270 -> 238 us, stage 3 319 -> 303 us.  Stages 1 and 2, which load
+        // a second batch after the RHS, run 1-3% slower with it (322 -> 325
+        // and 334 us), so they are left to the scheduler.
+        if constexpr (STAGE == 0 || STAGE == 3) __builtin_amdgcn_sched_barrier(0);
         float k[NV];
         bssn_cell<FMT, true, false, UPW>(a, u, cw, w, par, k, nullptr, nullptr, aw);
 #pragma unroll

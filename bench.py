@@ -80,16 +80,18 @@ def face_stats(export, S, depth_filter=None):
     return eq / max(wsum, 1), co / max(wsum, 1)
 
 
-def bytes_model(direct_faces, eq_faces, coarse_faces, upwind=False):
+def bytes_model(direct_faces, eq_faces, coarse_faces, upwind=False, q=Q):
     """Algorithmic bytes per interior cell and launch class.  eq_faces /
     coarse_faces: mean faces per advanced tile with an equal-depth / a
     coarser neighbour (face_stats).  direct_faces is the driver's axis mask
     (0 or 4) of the equal-depth faces that stages 0-2 do not push (the next
     stage's FD reads those ghost planes from the neighbours: the same
-    distinct elements as the pushed copies).  upwind: the FD also writes the
-    upwinded advection + Kreiss-Oliger term per field (read back by the
-    stage kernel), and both the FD's stencils and the push reach P layers."""
-    q = P if upwind else Q
+    distinct elements as the pushed copies).  q: the stencils' half-width,
+    the ghost layers the FD reads and the push writes (fd_order / 2).
+    upwind: the FD also writes the upwinded advection + Kreiss-Oliger term
+    per field (read back by the stage kernel), and both the FD's stencils
+    and the push reach P layers."""
+    q = P if upwind else q
     adv = 24 * 4 if upwind else 0
     push = eq_faces * q * S * S * 24 * 4 / S ** 3  # equal-depth ghost layers written by the stage kernel
     push_early = push * (3 - bin(direct_faces & 7).count("1")) / 3
@@ -318,66 +320,30 @@ def copy_probe(mib=1024, reps=10):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None,
-                    help="timed coarse steps (default 12; 64 for the puncture, 64 M of evolution)")
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing pass")
-    ap.add_argument("--z-block", type=int, default=0, help="fixed FD z-block (skips the autotuner)")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="bssn512")
-    ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--ko-sigma", type=float, default=PUNCTURE_KO_SIGMA,
-                    help="Kreiss-Oliger strength of the puncture run (it always upwinds the advection terms)")
-    ap.add_argument("--direct-faces", type=int, default=None, choices=(0, 4),
-                    help="axis mask (4: z) of the equal-depth faces RK4 stages 0-2 leave to the next FD "
-                         "(default: the autotuner's choice)")
-    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a recorded step")
-    ap.add_argument("--no-learned", action="store_true",
-                    help="skip the learned-versus-plain ghost fill comparison (training + two puncture runs)")
-    ap.add_argument("--learned-steps", type=int, default=3000, help="Adam steps of the learned prolongation")
-    ap.add_argument("--cpu-kernels", action="store_true",
-                    help="print only a markdown table of the reference's kernel bodies (the oracle's restatement) and "
-                         "the FP32 implementation's passes timed on this host, 1 thread and all threads")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="time only a plain 1 GiB device copy and a 1 GiB sum (the calibrations for HBM byte "
-                         "counters)")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU: sharding this path over several GPUs is a design only; "
-              "run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
-    if args.steps is None:
-        args.steps = 64 if args.workload == "puncture" else 12
-    if args.copy_probe:
-        copy_probe()
-        return
-    if args.cpu_kernels:  # host cores only: the mesh from the oracle's octree
-        from oracle import oracle as orc
-        n_slots = 1 + 8 + 64 + 512
-        ot = orc.OracleTree(n_slots, True)
-        assert build_mesh(ot) == n_slots
-        print(cpu_kernel_table(n_slots, initial_data(n_slots, S + 2 * P), ot.export()), flush=True)
-        return
-
+def timed_run(args, fd_order, steps, autotune=True):
+    """Builds the workload's driver at stencil order fd_order, autotunes it
+    (unless fixed by the flags), records the coarse step as a graph, runs
+    args.warmup untimed and `steps` timed coarse steps, then times every
+    kernel class with HIP events over one more eager step.  Returns the
+    driver, the zone-cycles and seconds of the timed steps, the kernel table,
+    the roofline of the dominant kernel, the whole-step HBM figures and the
+    autotuner's report."""
     import adaptivemesh_amd as am
     import torch
     from adaptivemesh_amd import workloads as WL
 
+    q = fd_order // 2
     T = S + 2 * P
     if args.workload == "bssn512":
-        g = am.geometry(S, P, FD_ORDER, OUTER, CFL)
+        g = am.geometry(S, P, fd_order, OUTER, CFL)
         n_slots = 1 + 8 + 64 + 512
         sim = am.BssnSim(g, n_slots, am.bssn_options(periodic=True))
-        assert sim.push_layers == Q, "the byte model assumes the driver's default push"
+        assert sim.push_layers == q, "the byte model assumes the driver's default push"
         assert build_mesh(sim) == n_slots
         fields = initial_data(n_slots, T)
     else:
         ops, n_slots = WL.puncture_ops(OUTER)
-        g = am.geometry(S, P, FD_ORDER, OUTER, CFL)
+        g = am.geometry(S, P, fd_order, OUTER, CFL)
         sim = am.BssnSim(g, n_slots, am.bssn_options(periodic=False, interp=am.AM_INTERP_TRICUBIC))
         for p_, c in ops:
             sim.add_node(p_, c)
@@ -389,7 +355,7 @@ def main():
     sim.fill_ghosts()
     sim.sync()
     tune = None
-    if args.z_block or args.no_autotune:
+    if args.z_block or args.no_autotune or not autotune:
         sim.set_launch(args.z_block, 0, 0)
     else:
         tune = sim.autotune(reps=3)
@@ -404,18 +370,16 @@ def main():
     torch.cuda.synchronize()
     z0 = sim.zone_cycles
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         sim.timestep()
     sim.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     zones = sim.zone_cycles - z0
     elapsed = t1 - t0
-    value = zones / elapsed
-    ms_per_step = 1e3 * elapsed / args.steps
     classes = WORKLOAD_CLASSES if args.workload == "bssn512" else WORKLOAD_CLASSES + ("ghost",)
     eq_faces, coarse_faces = face_stats(sim.export(), S)
-    BYTES_ = bytes_model(sim.direct_faces, eq_faces, coarse_faces, upwind=sim.dissipation[1])
+    BYTES_ = bytes_model(sim.direct_faces, eq_faces, coarse_faces, upwind=sim.dissipation[1], q=q)
     bytes_per_zone = sum(BYTES_[k] * STAGES_PER_STEP[k] for k in classes)
     whole_gbs = zones * bytes_per_zone / elapsed / 1e9
 
@@ -442,10 +406,11 @@ def main():
                                   bytes_per_zone_cycle=round(BYTES_[k] * STAGES_PER_STEP[k], 1))
         dom = max((k for k in kernels if "GBps" in kernels[k]), key=lambda k: kernels[k]["ms_total"])
         d = kernels[dom]
+        measured = (dom == "rhs_stage" and args.workload == "bssn512" and fd_order == FD_ORDER
+                    and sim.direct_faces == TRAFFIC_DIRECT_FACES)
         roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
-                    "traffic": RHS_TRAFFIC_PER_LAUNCH if dom == "rhs_stage" and args.workload == "bssn512"
-                    and sim.direct_faces == TRAFFIC_DIRECT_FACES else None}
+                    "traffic": RHS_TRAFFIC_PER_LAUNCH if measured else None}
         if args.workload != "bssn512":
             roofline["note"] = (f"byte model from this mesh's faces: {eq_faces:.2f} equal-depth and "
                                 f"{coarse_faces:.2f} coarser neighbours per advanced tile")
@@ -454,7 +419,66 @@ def main():
                                         "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000; "
                                         f"{RHS_TRAFFIC_PER_LAUNCH / RHS_ALGORITHMIC_PER_LAUNCH:.4f}x the algorithmic "
                                         "bytes of the same launches")
+    hbm = {"achieved": round(whole_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(whole_gbs / HBM_PEAK_GBS, 4), "bytes_per_zone_cycle": round(bytes_per_zone)}
+    return {"sim": sim, "fields": fields, "n_slots": n_slots, "zones": zones, "elapsed": elapsed, "tune": tune,
+            "kernels": kernels, "roofline": roofline, "hbm": hbm}
 
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed coarse steps (default 12; 64 for the puncture, 64 M of evolution)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing pass")
+    ap.add_argument("--z-block", type=int, default=0, help="fixed FD z-block (skips the autotuner)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="bssn512")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--ko-sigma", type=float, default=PUNCTURE_KO_SIGMA,
+                    help="Kreiss-Oliger strength of the puncture run (it always upwinds the advection terms)")
+    ap.add_argument("--direct-faces", type=int, default=None, choices=(0, 4),
+                    help="axis mask (4: z) of the equal-depth faces RK4 stages 0-2 leave to the next FD "
+                         "(default: the autotuner's choice)")
+    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a recorded step")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-versus-plain ghost fill comparison (training + two puncture runs)")
+    ap.add_argument("--learned-steps", type=int, default=3000, help="Adam steps of the learned prolongation")
+    ap.add_argument("--no-order6", action="store_true",
+                    help="skip the second measurement with the reference's order-6 stencil (3 ghost layers)")
+    ap.add_argument("--cpu-kernels", action="store_true",
+                    help="print only a markdown table of the reference's kernel bodies (the oracle's restatement) and "
+                         "the FP32 implementation's passes timed on this host, 1 thread and all threads")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="time only a plain 1 GiB device copy and a 1 GiB sum (the calibrations for HBM byte "
+                         "counters)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU: sharding this path over several GPUs is a design only; "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    if args.steps is None:
+        args.steps = 64 if args.workload == "puncture" else 12
+    if args.copy_probe:
+        copy_probe()
+        return
+    if args.cpu_kernels:  # host cores only: the mesh from the oracle's octree
+        from oracle import oracle as orc
+        n_slots = 1 + 8 + 64 + 512
+        ot = orc.OracleTree(n_slots, True)
+        assert build_mesh(ot) == n_slots
+        print(cpu_kernel_table(n_slots, initial_data(n_slots, S + 2 * P), ot.export()), flush=True)
+        return
+
+    import adaptivemesh_amd as am  # fails loudly without the HIP library
+
+    R = timed_run(args, FD_ORDER, args.steps)
+    sim, fields, n_slots, tune = R["sim"], R["fields"], R["n_slots"], R["tune"]
+    zones, elapsed = R["zones"], R["elapsed"]
+    value = zones / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
     cons = sim.constraints()
     finite = all(np.isfinite(v) for v in cons.values())
     result = {
@@ -472,10 +496,9 @@ def main():
                               "autotuned": tune is not None,
                               "autotune_trials": tune["trials"] if tune else None,
                               "graph_per_coarse_step": not args.no_graph}},
-        "hbm_whole_step": {"achieved": round(whole_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(whole_gbs / HBM_PEAK_GBS, 4), "bytes_per_zone_cycle": round(bytes_per_zone)},
-        "roofline": roofline,
-        "kernels": kernels,
+        "hbm_whole_step": R["hbm"],
+        "roofline": R["roofline"],
+        "kernels": R["kernels"],
         "constraints_after": {k: float(f"{v:.6g}") for k, v in cons.items()},
         "valid": finite,
         "cpu_baseline": None,
@@ -483,6 +506,19 @@ def main():
     if not args.no_cpu_baseline and args.workload == "bssn512":
         result["cpu_baseline"] = cpu_baseline(n_slots, fields, sim.export())
         result["cpu_baseline"]["value"] = round(result["cpu_baseline"]["value"], 1)
+    if not args.no_order6 and args.workload == "bssn512" and not args.cpu_kernels:
+        # the reference's own stencil at P = 3: order 2P = 6 (AMR.h:1144-1150,
+        # 1171), reading and pushing all three ghost layers
+        R6 = timed_run(args, 6, max(2, args.steps // 2))
+        d6 = R6["roofline"]
+        result["fd_order_6"] = {
+            "workload": "as config.workload with FD order 6 (the reference's order at P=3: three ghost layers read "
+                        "by the stencils and pushed)",
+            "value": round(R6["zones"] / R6["elapsed"], 1), "unit": "zone-cycles/s",
+            "steps": max(2, args.steps // 2), "ms_per_step": round(1e3 * R6["elapsed"] / max(2, args.steps // 2), 3),
+            "push_layers": R6["sim"].push_layers, "direct_faces": R6["sim"].direct_faces,
+            "hbm_whole_step": R6["hbm"], "roofline": d6, "kernels": R6["kernels"]}
+        del R6
     if not args.no_learned and args.workload == "bssn512":
         # the second half of the metric: learned versus plain (tricubic)
         # coarse-face ghost fill on the octree puncture run, same resolution

@@ -32,7 +32,7 @@ def gaussian(x, y, z):
     return np.exp(-(x * x + y * y + z * z) / 0.01)
 
 
-def make_sim(am, orc, periodic=True):
+def make_sim(am, orc, periodic=True, S=S, P=P):
     ops, n = meshes.uniform(1)
     g = am.geometry(S, P, ORDER, OUTER, CFL)
     sim = am.SwaveSim(g, n, am.swave_options(periodic=periodic))
@@ -42,14 +42,15 @@ def make_sim(am, orc, periodic=True):
     return sim, ot.export(), n, g
 
 
-def interior_mask(n, tiles):
+def interior_mask(n, tiles, S=S, P=P):
+    T = S + 2 * P
     m = np.zeros((n, T, T, T), bool)
     for t in tiles:
         m[t, P:P + S, P:P + S, P:P + S] = True
     return m.reshape(-1)
 
 
-def words(am, sim, n):
+def words(am, sim, n, S=S):
     W_ = n * S ** 3
     out = []
     for which in (4, 5):
@@ -59,10 +60,15 @@ def words(am, sim, n):
     return out
 
 
+@pytest.mark.parametrize("S,P", [(16, 3), (28, 2)])
 @pytest.mark.parametrize("direct", [4, 0])
-def test_swave_stages_match_oracle(am, orc, torch_cuda, direct):
-    print(f"swave stages uniform1 periodic direct={direct}", flush=True)
-    sim, e, n, g = make_sim(am, orc)
+def test_swave_stages_match_oracle(am, orc, torch_cuda, direct, S, P):
+    """Also at the reference's default tile (S = 28, P = 2): there 1/S is
+    inexact in float, so the equal-depth copies, the pushed layers (P = 2 =
+    fd_order / 2) and the direct z reads run on the rounded source index."""
+    print(f"swave stages uniform1 periodic direct={direct} S={S} P={P}", flush=True)
+    T = S + 2 * P
+    sim, e, n, g = make_sim(am, orc, S=S, P=P)
     sim.set_direct_faces(direct)
     u0 = F.sample(e["positions"], n, S, P, OUTER, gaussian)
     v0 = 0.3 * F.sample(e["positions"], n, S, P, OUTER, lambda x, y, z: np.sin(2 * np.pi * x) * np.cos(2 * np.pi * z))
@@ -72,7 +78,7 @@ def test_swave_stages_match_oracle(am, orc, torch_cuda, direct):
         sim.set_field(1, v, state0[v])
     depth = 1
     tiles = [s for s in F.used_slots(e) if F.depth_of(e, s) == depth]
-    imask = interior_mask(n, tiles)
+    imask = interior_mask(n, tiles, S, P)
     wmask = np.zeros((n, S ** 3), bool)
     wmask[tiles] = True
     wmask = wmask.reshape(-1)
@@ -86,7 +92,7 @@ def test_swave_stages_match_oracle(am, orc, torch_cuda, direct):
         seed = sim.rng_state
         sim.stage_depth(depth, stage)
         sim.sync()
-        d1, d2 = words(am, sim, n)
+        d1, d2 = words(am, sim, n, S)
         sd = orc.prng_ints(seed, 1)[0]
         w1, w2 = orc.differentiate(state_fd[0], e["positions"], tiles, 0, e["max_depth"], sd, S, P, ORDER // 2,
                                    outer_dx, n, with_d2=True)
