@@ -180,6 +180,8 @@ _SIGS = {
     "am_deriv_trace": (C.c_int, [_P, _P, C.POINTER(C.c_void_p), _P, C.POINTER(Mesh), C.POINTER(Geometry), _P]),
     "am_christoffel": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32,
                                  C.POINTER(Mesh), C.POINTER(Geometry), C.c_int32, _P]),
+    "am_tile_lists_workspace": (C.c_int32, [C.c_int32, C.c_int32]),
+    "am_tile_lists": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P]),
     # host runtime
     "am_tree_create": (_P, [C.c_uint32, C.c_int32]),
     "am_tree_destroy": (None, [_P]),

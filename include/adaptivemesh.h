@@ -308,6 +308,28 @@ int am_copy_to_child(float *buf, uint32_t offset, int32_t child, int32_t parent,
 int am_copy_to_child_batched(const am_field_batch *b, uint32_t offset, int32_t child, int32_t parent,
                              const am_geometry *g, int32_t interp, void *stream);
 
+/* Launch lists of the octree, built on the device by stream compaction
+ * (the reference's GPU scan, parallel.h:10-80, applied to the launch
+ * domain that its wrappers cover with every slot, AMR.h:640-644 and
+ * 880-1310).  From the device copies of positions, children (8 per slot)
+ * and boundaries (6 per slot) of slots 0 .. n_slots-1 it writes, in
+ * ascending slot order:
+ *   tiles: [every slot < n_slots][depth 0]..[depth depth_limit][the leaves]
+ *          (at most 3 * n_slots entries);
+ *   faces: [depth 0]..[depth depth_limit]: slot * 6 + face of every face of
+ *          a tile of depth >= 1 whose target is coarser (at most 6 * n_slots);
+ *   coarse_source[s] = 1 for every slot such a face reads, 0 for the other
+ *          slots < max_slots;
+ *   counts (device, 4 * depth_limit + 8 int32): tile list offsets and
+ *          lengths (depth_limit + 3 each), then face list offsets and
+ *          lengths (depth_limit + 1 each).
+ * workspace: am_tile_lists_workspace(max_slots, depth_limit) int32 of device
+ * memory.  Three launches on `stream`; nothing is allocated. */
+int32_t am_tile_lists_workspace(int32_t max_slots, int32_t depth_limit);
+int am_tile_lists(const uint32_t *positions, const int32_t *children, const uint64_t *boundaries, int32_t n_slots,
+                  int32_t max_slots, int32_t depth_limit, int32_t *tiles, int32_t *faces, uint8_t *coarse_source,
+                  int32_t *workspace, int32_t *counts, void *stream);
+
 /* ------------------------------------------------------ SimpleWave */
 
 /* __init_temp (kernel.cu:9-43), the reference's wave-test initial data:

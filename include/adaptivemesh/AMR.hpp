@@ -240,18 +240,20 @@ struct octree {
     bool periodic = false;
     void *stream = nullptr;  // stream used for octree uploads
 
-    // device slot lists: [0] every used slot, [1 + d] the slots at depth d,
-    // [depth_limit + 2] the leaves (no child)
-    smart_gpu_cpu_buffer<int32_t> tile_lists;
+    // device slot lists, built on the device (am_tile_lists) after every
+    // structural change: [0] every slot up to the last used one, [1 + d] the
+    // slots at depth d, [depth_limit + 2] the leaves (no child)
+    lazy_gpu_buffer<int32_t> tile_lists;
     int32_t list_offset[depth_limit + 3] = {};
     int32_t list_count[depth_limit + 3] = {};
     int32_t coarse_faces[depth_limit + 1] = {};  // faces per depth whose target is coarser
     // device lists of those faces (slot * 6 + face), depth by depth
-    smart_gpu_cpu_buffer<int32_t> face_lists;
+    lazy_gpu_buffer<int32_t> face_lists;
     int32_t face_offset[depth_limit + 1] = {};
     // per slot: 1 when some finer tile's face reads this slot (its ghost cells
     // feed the interpolation), else 0
-    smart_gpu_cpu_buffer<uint8_t> coarse_source;
+    lazy_gpu_buffer<uint8_t> coarse_source;
+    lazy_gpu_buffer<int32_t> list_workspace, list_counts;  // the compaction's scratch and its counts
 
    protected:
     bool dirty = true;
@@ -263,8 +265,7 @@ struct octree {
    public:
     explicit octree(uint32_t node_slots, bool periodic_ = false)
         : parent_idx_b(node_slots), children_idx_b(node_slots * 8u), positions(node_slots),
-          boundaries(node_slots * 6u), lifetime(node_slots), max_slots(node_slots), periodic(periodic_),
-          tile_lists(node_slots * 3u), face_lists(node_slots * 6u), coarse_source(node_slots)
+          boundaries(node_slots * 6u), lifetime(node_slots), max_slots(node_slots), periodic(periodic_)
     {
         memset(parent_idx_b.cpu_buffer_ptr, -1, parent_idx_b.total_size());
         memset(children_idx_b.cpu_buffer_ptr, -1, children_idx_b.total_size());
@@ -283,6 +284,8 @@ struct octree {
         tile_lists.destroy();
         coarse_source.destroy();
         face_lists.destroy();
+        list_workspace.destroy();
+        list_counts.destroy();
     }
 
     uint32_t read_first_free_slot() const { return first_free_slot; }
@@ -483,53 +486,37 @@ struct octree {
     }
 
    public:
-    // AMR::copy_to_gpu (AMR.h:268-275), plus the slot lists
+    // AMR::copy_to_gpu (AMR.h:268-275), then the slot and face lists built
+    // from the uploaded tree by the device compaction; only their lengths
+    // come back to the host (launch sizes)
     void copy_to_gpu()
     {
         if (!dirty) return;
-        int32_t *L = tile_lists.cpu_buffer_ptr;
-        int32_t n = 0;
-        list_offset[0] = 0;
-        for (uint32_t i = 0; i <= final_used_slot; i++) L[n++] = (int32_t)i;
-        list_count[0] = n;
-        for (uint32_t d = 0; d <= depth_limit; d++) {
-            list_offset[d + 1] = n;
-            for (uint32_t i = 0; i <= final_used_slot; i++)
-                if (positions.cpu_buffer_ptr[i].depth() == (int)d) L[n++] = (int32_t)i;
-            list_count[d + 1] = n - list_offset[d + 1];
-        }
-        list_offset[depth_limit + 2] = n;
-        for (uint32_t i = 0; i <= final_used_slot; i++) {
-            if (positions.cpu_buffer_ptr[i].depth() == -1) continue;
-            bool leaf = true;
-            for (int c = 0; c < 8; c++) leaf &= children_idx_b.cpu_buffer_ptr[i * 8u + c] == -1;
-            if (leaf) L[n++] = (int32_t)i;
-        }
-        list_count[depth_limit + 2] = n - list_offset[depth_limit + 2];
-        std::fill(coarse_source.cpu_buffer_ptr, coarse_source.cpu_buffer_ptr + max_slots, (uint8_t)0);
-        int32_t nf = 0;
-        for (uint32_t d = 0; d <= depth_limit; d++) {
-            face_offset[d] = nf;
-            for (int32_t k = 0; k < list_count[d + 1]; k++) {
-                const int32_t i = L[list_offset[d + 1] + k];
-                if (d == 0) continue;
-                for (uint32_t f = 0; f < 6; f++) {
-                    const domain_boundary &b = boundaries.cpu_buffer_ptr[(uint32_t)i * 6u + f];
-                    if (b.target_idx() >= 0 && b.depth_diff() > 0) {
-                        face_lists.cpu_buffer_ptr[nf++] = i * 6 + (int32_t)f;
-                        coarse_source.cpu_buffer_ptr[b.target_idx()] = 1;
-                    }
-                }
-            }
-            coarse_faces[d] = nf - face_offset[d];
-        }
         parent_idx_b.copy_to_gpu(stream);
         children_idx_b.copy_to_gpu(stream);
         positions.copy_to_gpu(stream);
         boundaries.copy_to_gpu(stream);
-        tile_lists.copy_to_gpu(stream);
-        coarse_source.copy_to_gpu(stream);
-        face_lists.copy_to_gpu(stream);
+        constexpr int NT = depth_limit + 3, NF = depth_limit + 1;
+        int32_t counts[2 * NT + 2 * NF];
+        const int32_t n = (int32_t)final_used_slot + 1;
+        check(am_tile_lists(reinterpret_cast<const uint32_t *>(positions.gpu_buffer_ptr),
+                            children_idx_b.gpu_buffer_ptr,
+                            reinterpret_cast<const uint64_t *>(boundaries.gpu_buffer_ptr), n, (int32_t)max_slots,
+                            (int32_t)depth_limit, tile_lists.ensure(3u * max_slots), face_lists.ensure(6u * max_slots),
+                            coarse_source.ensure(max_slots),
+                            list_workspace.ensure((size_t)am_tile_lists_workspace((int32_t)max_slots, depth_limit)),
+                            list_counts.ensure(2 * NT + 2 * NF), stream),
+              "am_tile_lists");
+        check(am_memcpy_d2h(counts, list_counts.gpu_buffer_ptr, sizeof counts, stream), "tile list counts");
+        check(am_stream_sync(stream), "tile list counts");
+        for (int k = 0; k < NT; k++) {
+            list_offset[k] = counts[k];
+            list_count[k] = counts[NT + k];
+        }
+        for (int d = 0; d < NF; d++) {
+            face_offset[d] = counts[2 * NT + d];
+            coarse_faces[d] = counts[2 * NT + NF + d];
+        }
         dirty = false;
     }
     int tiles_at_depth(int d) const { return list_count[d + 1]; }

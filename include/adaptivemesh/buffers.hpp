@@ -106,6 +106,31 @@ struct smart_gpu_cpu_buffer {
     }
 };
 
+// Device-only array allocated on first use (so the octree logic that owns
+// one still runs without a GPU), grown when a larger length is asked for.
+template <class T>
+struct lazy_gpu_buffer {
+    size_t dedicated_len = 0;
+    T *gpu_buffer_ptr = nullptr;
+    T *ensure(size_t n)
+    {
+        if (gpu_buffer_ptr && n <= dedicated_len) return gpu_buffer_ptr;
+        if (gpu_buffer_ptr) am_device_free(gpu_buffer_ptr);
+        gpu_buffer_ptr = nullptr;
+        void *p = nullptr;
+        check(am_device_alloc(&p, n * sizeof(T) + 16), "lazy_gpu_buffer");
+        gpu_buffer_ptr = static_cast<T *>(p);
+        dedicated_len = n;
+        return gpu_buffer_ptr;
+    }
+    void destroy()
+    {
+        if (gpu_buffer_ptr) am_device_free(gpu_buffer_ptr);
+        gpu_buffer_ptr = nullptr;
+        dedicated_len = 0;
+    }
+};
+
 // cuda_sync (CUDA_memory.h:213-220): wait for every launch on the device
 inline int cuda_sync() { return am_device_sync(); }
 
