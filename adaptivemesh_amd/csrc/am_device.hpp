@@ -25,10 +25,6 @@ __device__ __forceinline__ int32_t prng_next(int32_t &s)
     s = v;
     return v;
 }
-__device__ __forceinline__ float prng_float(int32_t &s)
-{
-    return __fmul_rn((float)prng_next(s), 2.3283064E-10f);
-}
 // seed = gs + x*2178614; seed ^= next()*y; seed ^= next()*z  (AMR.h:1160-1162)
 __device__ __forceinline__ int32_t thread_seed(int32_t gs, uint32_t x, uint32_t y, uint32_t z)
 {
@@ -81,15 +77,26 @@ __device__ __forceinline__ uint32_t cf3_assemble(uint32_t e, float x, float y, f
 // result.x *= ((1<<15)/(1/128)) / float(1<<e) (tensors.h:361-363).
 __device__ __forceinline__ float cf3_scale(uint32_t e) { return __uint_as_float((127u + 22u - e) << 23); }
 
+// One dither term of the pack, generate_float() * 0.95 (tensors.h:297,
+// 365-367), with one rounding instead of two: generate_float() is float(v)
+// times 2.3283064E-10f, which is exactly 2^-32, so that product is exact and
+// the reference's result is float(v) * 0.95f correctly rounded, then scaled
+// by 2^-32 -- the same value as float(v) times the float 0.95f * 2^-32
+// correctly rounded (no product here is subnormal: |float(v)| >= 1 or 0).
+__device__ __forceinline__ float cf3_dither(int32_t &rng)
+{
+    constexpr float K = 0.95f * 2.3283064E-10f;  // exact: a power-of-two scaling of 0.95f
+    return __fmul_rn((float)prng_next(rng), K);
+}
 // dithered pack (tensors.h:350-373): three draws, x then y then z
 __device__ __forceinline__ uint32_t cf3_pack(float x, float y, float z, int32_t &rng)
 {
     const uint32_t e = cf3_exponent(x, y, z);
     const float sc = cf3_scale(e);
     x = __fmul_rn(x, sc); y = __fmul_rn(y, sc); z = __fmul_rn(z, sc);
-    x = __fadd_rn(x, __fmul_rn(prng_float(rng), 0.95f));
-    y = __fadd_rn(y, __fmul_rn(prng_float(rng), 0.95f));
-    z = __fadd_rn(z, __fmul_rn(prng_float(rng), 0.95f));
+    x = __fadd_rn(x, cf3_dither(rng));
+    y = __fadd_rn(y, cf3_dither(rng));
+    z = __fadd_rn(z, cf3_dither(rng));
     return cf3_assemble(e, x, y, z);
 }
 // undithered pack (tensors.h:374-393)
