@@ -29,7 +29,7 @@ HESS = (0, 1, 2, 3, 4, 5, 6, 17, 18, 19, 20)
 STAGE_FRAC = (0.5, 0.5, 1.0, 1.0)
 
 
-def smooth_state(e, n, seed=0, eps=0.04):
+def smooth_state(e, n, seed=0, eps=0.04, S=S, P=P):
     """Flat space plus smooth periodic perturbations on every field."""
     rng = np.random.default_rng(seed)
     flat = np.zeros(24)
@@ -54,7 +54,7 @@ def download(am, dev_ptr, count, dtype, offset_elems=0):
     return a
 
 
-def sim_words(am, sim, n):
+def sim_words(am, sim, n, S=S):
     W = n * S ** 3
     d1 = [download(am, sim.device_ptr(4), W, np.uint32, v * W) for v in range(24)]
     d2d = [download(am, sim.device_ptr(5), W, np.uint32, h * W) for h in range(11)]
@@ -62,29 +62,36 @@ def sim_words(am, sim, n):
     return d1, d2d, d2o
 
 
-def tile_mask(n, tiles, inner=True):
+def tile_mask(n, tiles, inner=True, S=S, P=P):
+    T = S + 2 * P
     m = np.zeros((n, T, T, T) if not inner else (n, S, S, S), bool)
     m[tiles] = True
     return m.reshape(-1)
 
 
-def interior_mask(n, tiles):
+def interior_mask(n, tiles, S=S, P=P):
+    T = S + 2 * P
     m = np.zeros((n, T, T, T), bool)
     for t in tiles:
         m[t, P:P + S, P:P + S, P:P + S] = True
     return m.reshape(-1)
 
 
-def fd_input(orc, e, n, tiles, state_in, old, stage, direct):
+def fd_input(orc, e, n, tiles, state_in, old, stage, direct, S=S, P=P):
     return F.fd_input(orc, e, n, tiles, state_in, old, stage, direct, S, P, STAGE_FRAC[stage])
 
 
-@pytest.mark.parametrize("recipe,periodic,depth,layers,direct", [
-    ("uniform1", True, 1, 0, 4), ("graded", False, 2, 0, 4), ("graded", False, 3, 0, 4),
-    ("uniform1", True, 1, P, 4), ("graded", False, 2, 0, 0), ("uniform1", True, 1, P, 0),
-    ("graded", False, 3, P, 0), ("graded", False, 3, P, 4)])
-def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, layers, direct):
-    print(f"bssn stages {recipe} periodic={periodic} depth={depth} push layers={layers} direct={direct}", flush=True)
+@pytest.mark.parametrize("recipe,periodic,depth,layers,direct,S,P", [
+    ("uniform1", True, 1, 0, 4, S, P), ("graded", False, 2, 0, 4, S, P), ("graded", False, 3, 0, 4, S, P),
+    ("uniform1", True, 1, P, 4, S, P), ("graded", False, 2, 0, 0, S, P), ("uniform1", True, 1, P, 0, S, P),
+    ("graded", False, 3, P, 0, S, P), ("graded", False, 3, P, 4, S, P),
+    ("uniform1", True, 1, 0, 4, 28, 2), ("graded", False, 2, 0, 0, 28, 2), ("graded", False, 3, 0, 4, 28, 2)])
+def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, layers, direct, S, P):
+    """Also at the reference's default tile, S = 28, P = 2 (1/S inexact in
+    float; every ghost layer pushed, the x faces by the general push)."""
+    print(f"bssn stages {recipe} periodic={periodic} depth={depth} push layers={layers} direct={direct} S={S} "
+          f"P={P}", flush=True)
+    T = S + 2 * P
     ops, n = {"uniform1": meshes.uniform(1), "graded": meshes.graded()}[recipe]
     g = am.geometry(S, P, ORDER, OUTER, CFL)
     sim = am.BssnSim(g, n, am.bssn_options(periodic=periodic))
@@ -96,7 +103,7 @@ def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, 
     for p_, c in ops:
         assert sim.add_node(p_, c) == ot.add_node(p_, c)
     e = ot.export()
-    state0 = smooth_state(e, n, seed=depth)
+    state0 = smooth_state(e, n, seed=depth, S=S, P=P)
     for v in range(24):
         sim.set_field(0, v, state0[v])
         sim.set_field(1, v, state0[v])
@@ -105,17 +112,17 @@ def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, 
     outer_dx = float(np.float32(OUTER) / np.float32(S))
     dtd = np.array([float(np.float32(np.float32(outer_dx) * np.float32(CFL)) / np.float32(2 ** d)) for d in range(9)])
     params = np.array(orc.BSSN_DEFAULT, np.float64)
-    imask = interior_mask(n, tiles)
-    wmask = tile_mask(n, tiles)
+    imask = interior_mask(n, tiles, S, P)
+    wmask = tile_mask(n, tiles, True, S, P)
     old = [x.copy() for x in state0]
     for stage in range(4):
         state_in = old if stage == 0 else [sim.get_field(1, v) for v in range(24)]
-        state_fd = fd_input(orc, e, n, tiles, state_in, old, stage, direct)
+        state_fd = fd_input(orc, e, n, tiles, state_in, old, stage, direct, S, P)
         acc_in = [sim.get_field(2, v).astype(np.float64) for v in range(24)]
         seed = sim.rng_state
         sim.stage_depth(depth, stage)
         sim.sync()
-        d1, d2d, d2o = sim_words(am, sim, n)
+        d1, d2d, d2o = sim_words(am, sim, n, S)
         # words: bit-exact with the oracle on the same input and seeds
         s_ = seed
         h = 0
@@ -295,8 +302,8 @@ def test_bssn_word_formats_match_oracle(am, orc, torch_cuda, fmt):
     outer_dx = float(np.float32(OUTER) / np.float32(S))
     dtd = np.array([float(np.float32(np.float32(outer_dx) * np.float32(CFL)) / np.float32(2 ** d)) for d in range(9)])
     params = np.array(orc.BSSN_DEFAULT, np.float64)
-    imask = interior_mask(n, tiles)
-    wmask = tile_mask(n, tiles)
+    imask = interior_mask(n, tiles, S, P)
+    wmask = tile_mask(n, tiles, True, S, P)
     old = [x.copy() for x in state0]
     as_words = (lambda a: a.view(np.float32).reshape(-1, 3)) if fmt == 1 else \
         (lambda a: a.view(np.float16).reshape(-1, 3))
