@@ -96,7 +96,9 @@ __device__ __forceinline__ float mixed_bd(float c, float pp, float mm, float m0,
 }
 
 // DIR: 0 the tile's own ghost layers only; 1 direct reads across z faces
-// (a base-pointer swap per ghost plane); 2 direct reads on any axis mask.
+// (a base-pointer swap per ghost plane); 2 direct reads on any axis mask
+// (padded layout); 3 direct reads across y and/or z faces (interior-first
+// layout: a y ghost row is a whole interior row of the neighbour).
 // Separate instantiations, so the plain kernel carries none of the code.
 // L: tile layout of the fields (0 padded, 1 interior-first).  The LDS ring
 // holds padded planes either way; with L = 1 a plane is assembled from the
@@ -109,7 +111,8 @@ __device__ __forceinline__ float mixed_bd(float c, float pp, float mm, float m0,
 template <int S, int P, int Q, int ZB, bool PERSIST, int FMT, int PF, int DIR, int L, bool ADV = false>
 __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScales sc)
 {
-    static_assert(L == 0 || DIR != 2, "the interior-first layout takes direct z faces only");
+    static_assert(L == 0 || DIR != 2, "the interior-first layout takes direct y and z faces only");
+    static_assert(DIR != 3 || L == 1, "direct y faces are built for the interior-first layout");
     static_assert(!ADV || P >= 3, "the lopsided and sixth-difference stencils reach 3 cells");
     constexpr int QL = ADV ? 3 : Q;  // ghost layers the stencils read
     constexpr int T = S + 2 * P, T2 = T * T, R = 2 * P + 2, NZB = S / ZB;
@@ -180,27 +183,40 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     // from the slab region (plus z * S per plane) and its place in the plane
     constexpr int NF = 4 * QL * S;
     int f_goff = -1, f_loff = 0;
+    int f_yface = -1, f_yrow = 0;  // DIR 3: a y face's element = the neighbour's interior row f_yrow, cell a
     if constexpr (L == 1) {
         if (tid < NF) {
             const int k = tid / (QL * S), l = (tid / S) % QL, a = tid % S;
             const int f = k == 0 ? 3 : (k == 1 ? 0 : (k == 2 ? 4 : 1));  // -x +x -y +y
             f_goff = (f * P + l) * S * S + a;
+            if (k >= 2) {
+                f_yface = f;
+                f_yrow = (f == 4 ? S - 1 - l : l) * S + a;
+            }
             const int g = (k & 1) ? P + S + l : P - 1 - l;  // the ghost coordinate of the face axis
             f_loff = k < 2 ? (a + P) * T + g : g * T + a + P;
         }
     }
+    // DIR 3: this thread's y-face element comes from the neighbour's interior
+    const float *ynb = nullptr;
+    if constexpr (DIR == 3) ynb = f_yface == 4 ? nb[4] : (f_yface == 1 ? nb[1] : nullptr);
     auto gload = [&](float4 *dst, int zp) {
         if constexpr (L == 1) {
             if (zp >= P && zp < P + S) {
                 const float *pz = src + (size_t)(zp - P) * (S * S);
                 dst[0].x = pz[tid];
-                if (f_goff >= 0) dst[0].y = src[(size_t)(S * S * S) + f_goff + (zp - P) * S];
+                if (f_goff >= 0) {
+                    if (DIR == 3 && ynb)
+                        dst[0].y = ynb[(size_t)(zp - P) * (S * S) + f_yrow];
+                    else
+                        dst[0].y = src[(size_t)(S * S * S) + f_goff + (zp - P) * S];
+                }
             } else {
                 const bool low = zp < P;
                 const int l = low ? P - 1 - zp : zp - P - S;
                 if (l < QL) {
                     const float *pl_ = src + (size_t)(S * S * S) + (size_t)(((low ? 5 : 2) * P + l) * S * S);
-                    if constexpr (DIR == 1) {
+                    if constexpr (DIR == 1 || DIR == 3) {
                         if (low && nb[5]) pl_ = nb[5] + (size_t)(S - 1 - l) * (S * S);
                         if (!low && nb[2]) pl_ = nb[2] + (size_t)l * (S * S);
                     }
@@ -452,6 +468,7 @@ static void fd_launch(int fmt, int pf, bool persist, dim3 grid, hipStream_t st, 
     if (m.direct_axes == 0u) return fd_launch_dir<S, P, Q, ZB, 0, L>(fmt, pf, persist, grid, st, d, m, sc);
     if (m.direct_axes == 4u) return fd_launch_dir<S, P, Q, ZB, 1, L>(fmt, pf, persist, grid, st, d, m, sc);
     if constexpr (L == 0) fd_launch_dir<S, P, Q, ZB, 2, L>(fmt, pf, persist, grid, st, d, m, sc);
+    else fd_launch_dir<S, P, Q, ZB, 3, L>(fmt, pf, persist, grid, st, d, m, sc);  // y, or y and z
 }
 // the upwind / dissipation kernels: S = 16, P = 3, order 4, cf3 words,
 // one-plane prefetch, direct faces 0 or z
@@ -553,9 +570,9 @@ static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometr
         d.advect_mask = b->advect_mask;
         d.ko_sigma = b->ko_sigma;
     }
-    if (lay == AM_LAYOUT_INTERIOR_FIRST && (pf > 1 || (b->direct_faces != 0 && b->direct_faces != 4)))
+    if (lay == AM_LAYOUT_INTERIOR_FIRST && (pf > 1 || (b->direct_faces & 1)))
         return set_error(AM_ERR_INVALID,
-                         "am_fd: the interior-first layout takes prefetch_planes 0-1 and direct_faces 0 or 4 "
+                         "am_fd: the interior-first layout takes prefetch_planes 0-1 and direct_faces 0, 2, 4 or 6 "
                          "(%d, %d)", b->prefetch_planes, b->direct_faces);
     if (zb == S) {
         fd_launch_layout<S, P, Q, S>(lay, fmt, pf, persist, dim3(gx(1), b->n_vars), st, d, items(1), sc, adv);

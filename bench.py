@@ -84,8 +84,8 @@ def bytes_model(direct_faces, eq_faces, coarse_faces, upwind=False, q=Q):
     """Algorithmic bytes per interior cell and launch class.  eq_faces /
     coarse_faces: mean faces per advanced tile with an equal-depth / a
     coarser neighbour (face_stats).  direct_faces is the driver's axis mask
-    (0 or 4) of the equal-depth faces that stages 0-2 do not push (the next
-    stage's FD reads those ghost planes from the neighbours: the same
+    (0, 2, 4 or 6) of the equal-depth faces that stages 0-2 do not push (the
+    next stage's FD reads those ghost layers from the neighbours: the same
     distinct elements as the pushed copies).  q: the stencils' half-width,
     the ghost layers the FD reads and the push writes (fd_order / 2).
     upwind: the FD also writes the upwinded advection + Kreiss-Oliger term
@@ -438,7 +438,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--ko-sigma", type=float, default=PUNCTURE_KO_SIGMA,
                     help="Kreiss-Oliger strength of the puncture run (it always upwinds the advection terms)")
-    ap.add_argument("--direct-faces", type=int, default=None, choices=(0, 4),
+    ap.add_argument("--direct-faces", type=int, default=None, choices=(0, 2, 4, 6),
                     help="axis mask (4: z) of the equal-depth faces RK4 stages 0-2 leave to the next FD "
                          "(default: the autotuner's choice)")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a recorded step")

@@ -230,7 +230,7 @@ typedef struct am_fd_batch {
      * copies. */
     int32_t direct_faces;
     /* enum am_tile_layout of every field (AM_LAYOUT_INTERIOR_FIRST takes
-     * direct_faces 0 or 4 only) */
+     * direct_faces 0, 2, 4 or 6: no x) */
     int32_t tile_layout;
     /* Upwinded advection with Kreiss-Oliger dissipation (no counterpart in
      * the reference; S=16, P=3, fd_order 4, cf3 words, prefetch 1 only):

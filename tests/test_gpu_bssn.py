@@ -84,8 +84,10 @@ def fd_input(orc, e, n, tiles, state_in, old, stage, direct, S=S, P=P):
 @pytest.mark.parametrize("recipe,periodic,depth,layers,direct,S,P", [
     ("uniform1", True, 1, 0, 4, S, P), ("graded", False, 2, 0, 4, S, P), ("graded", False, 3, 0, 4, S, P),
     ("uniform1", True, 1, P, 4, S, P), ("graded", False, 2, 0, 0, S, P), ("uniform1", True, 1, P, 0, S, P),
-    ("graded", False, 3, P, 0, S, P), ("graded", False, 3, P, 4, S, P),
-    ("uniform1", True, 1, 0, 4, 28, 2), ("graded", False, 2, 0, 0, 28, 2), ("graded", False, 3, 0, 4, 28, 2)])
+    ("graded", False, 3, P, 0, S, P), ("graded", False, 3, P, 4, S, P), ("uniform1", True, 1, 0, 6, S, P),
+    ("graded", False, 3, 0, 6, S, P), ("graded", False, 2, P, 2, S, P),
+    ("uniform1", True, 1, 0, 4, 28, 2), ("graded", False, 2, 0, 0, 28, 2), ("graded", False, 3, 0, 4, 28, 2),
+    ("graded", False, 3, 0, 6, 28, 2)])
 def test_bssn_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, layers, direct, S, P):
     """Also at the reference's default tile, S = 28, P = 2 (1/S inexact in
     float; every ghost layer pushed, the x faces by the general push)."""
@@ -234,19 +236,19 @@ def test_bssn_flat_space_stays_flat(am, torch_cuda):
 
 @pytest.mark.parametrize("recipe,periodic", [("graded", False), ("uniform1", True)])
 def test_direct_faces_leave_every_step_unchanged(am, torch_cuda, recipe, periodic):
-    """Whole coarse steps (graph replays) with the direct-face masks 0 and
-    4: old, new and the accumulator, ghost cells included, and the seed
-    state are the same bit for bit after every step.  Masks with x or y
-    faces are refused (the driver's interior-first tiles give those ghost
-    cells only as strided columns and rows)."""
+    """Whole coarse steps (graph replays) with the direct-face masks 0, 2, 4
+    and 6: old, new and the accumulator, ghost cells included, and the seed
+    state are the same bit for bit after every step.  Masks with x faces are
+    refused (the driver's interior-first tiles give those ghost cells only as
+    strided columns), and so is y with upwinding."""
     print(f"direct faces over whole steps: {recipe} periodic={periodic}", flush=True)
     ops, n = {"uniform1": meshes.uniform(1), "graded": meshes.graded()}[recipe]
     g = am.geometry(S, P, ORDER, OUTER, CFL)
     results = {}
-    for mask in (0, 4):
+    for mask in (0, 2, 4, 6):
         sim = am.BssnSim(g, n, am.bssn_options(periodic=periodic))
         meshes.apply(sim, ops)
-        for bad in (1, 2, 3, 5, 6, 7, 8):
+        for bad in (1, 3, 5, 7, 8, -2):
             with pytest.raises(am.AMError):
                 sim.set_direct_faces(bad)
         sim.set_direct_faces(mask)
@@ -264,7 +266,7 @@ def test_direct_faces_leave_every_step_unchanged(am, torch_cuda, recipe, periodi
             snaps.append(([sim.get_field(w, v).view(np.uint32) for w in (0, 1, 2) for v in range(24)],
                           sim.rng_state))
         results[mask] = snaps
-    for mask in (4,):
+    for mask in (2, 4, 6):
         for step, ((fa, sa), (fb, sb)) in enumerate(zip(results[0], results[mask])):
             assert sa == sb, (mask, step)
             for i, (a, b) in enumerate(zip(fa, fb)):

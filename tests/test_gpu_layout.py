@@ -1,7 +1,7 @@
 """Kernels on interior-first tiles (am_tile_layout 1, the RK4 drivers'
 storage) against the oracle on the same data in the reference's padded
 layout, bit for bit: the layout conversion itself, the FD stencils (every
-geometry and z-block, own ghost layers or direct z faces), the stage ghost
+geometry and z-block, own ghost layers or direct y / z faces), the stage ghost
 fill (three interpolations, copy and formula rules, S = 16 and 28), the
 driver's copy-back with restriction, and prolongation into a child.  Arrays
 move between the layouts on the host with adaptivemesh_amd's
@@ -58,10 +58,10 @@ def test_convert_round_trip(am, torch_cuda, S, P):
 @pytest.mark.parametrize("recipe,periodic", [("graded", False), ("uniform2", True)])
 def test_fd_interior_first(am, orc, torch_cuda, S, P, order, recipe, periodic):
     """FD words of an interior-first field equal the oracle's on the padded
-    field, for every z-block and two counter values; with direct z faces
-    (mask 4) the equal-depth z ghost layers are poisoned (NaN) and the words
-    equal the oracle's on the field whose ghost layers hold the neighbours'
-    copies."""
+    field, for every z-block and two counter values; with direct faces
+    (masks 2 y, 4 z, 6 both) the equal-depth ghost layers on those axes are
+    poisoned (NaN) and the words equal the oracle's on the field whose ghost
+    layers hold the neighbours' copies."""
     torch = torch_cuda
     print(f"fd interior-first S={S} P={P} order={order} {recipe}", flush=True)
     g = am.geometry(S, P, order, 100.0, 0.4)
@@ -73,16 +73,17 @@ def test_fd_interior_first(am, orc, torch_cuda, S, P, order, recipe, periodic):
     filled = field.copy()
     orc.copy_boundaries(field.copy(), filled, None, e["boundaries"], tiles, 0, e["max_depth"], True, 0, S, P,
                         equal_copy=True)
-    eq_z = fields.outer_push_layers(e, n, tiles, S, P, 0, 4)
+    eq_yz = fields.outer_push_layers(e, n, tiles, S, P, 0, 6)
     good = field.copy()
-    good[eq_z] = filled[eq_z]
+    good[eq_yz] = filled[eq_yz]
     L, st = am.lib(), am.stream_handle()
     W = n * S ** 3
-    for direct in (0, 4):
+    for direct in (0, 2, 4, 6):
         src = good.copy()
         if direct:
-            assert eq_z.any()
-            src[eq_z] = np.nan
+            poison = fields.outer_push_layers(e, n, tiles, S, P, 0, direct)
+            assert poison.any()
+            src[poison] = np.nan
         f = dev(torch, to_if(am, src, S, P, n))
         for substep in (0, 1):
             seed_d, seed_m = int(rng.integers(-2 ** 31, 2 ** 31)), int(rng.integers(-2 ** 31, 2 ** 31))
@@ -106,8 +107,8 @@ def test_fd_interior_first(am, orc, torch_cuda, S, P, order, recipe, periodic):
                 assert np.array_equal(host(d1, np.uint32), r1), (direct, substep, zb)
                 assert np.array_equal(host(d2, np.uint32), r2), (direct, substep, zb)
                 assert np.array_equal(host(off, np.uint32), rm), (direct, substep, zb)
-    # the interior-first FD takes direct masks 0 and 4 and the one-plane prefetch only
-    for bad_direct, bad_pf in ((1, 1), (7, 1), (0, 2)):
+    # the interior-first FD takes direct masks 0, 2, 4 and 6 and the one-plane prefetch only
+    for bad_direct, bad_pf in ((1, 1), (7, 1), (3, 1), (5, 1), (0, 2), (6, 2)):
         b.direct_faces, b.prefetch_planes = bad_direct, bad_pf
         assert L.am_fd_batched(C.byref(b), C.byref(m), C.byref(g), st) == -1
 

@@ -61,7 +61,7 @@ def words(am, sim, n, S=S):
 
 
 @pytest.mark.parametrize("S,P", [(16, 3), (28, 2)])
-@pytest.mark.parametrize("direct", [4, 0])
+@pytest.mark.parametrize("direct", [4, 0, 6])
 def test_swave_stages_match_oracle(am, orc, torch_cuda, direct, S, P):
     """Also at the reference's default tile (S = 28, P = 2): there 1/S is
     inexact in float, so the equal-depth copies, the pushed layers (P = 2 =

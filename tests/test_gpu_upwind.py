@@ -203,3 +203,12 @@ def test_dissipation_setting_refusals(am, torch_cuda):
         sim2.set_dissipation(0.2, True)  # P = 2: the stencils reach 3 cells
     sim.set_dissipation(0.0, False)
     assert sim.dissipation == (0.0, False) and sim.push_layers == ORDER // 2
+    # the upwind FD reads direct z faces only: upwinding clears the y bit and refuses it afterwards
+    sim.set_direct_faces(6)
+    sim.set_dissipation(0.2, True)
+    assert sim.direct_faces == 4
+    with pytest.raises(am.AMError):
+        sim.set_direct_faces(2)
+    sim.set_dissipation(0.0, False)
+    sim.set_direct_faces(2)
+    assert sim.direct_faces == 2
