@@ -88,16 +88,42 @@ __device__ __forceinline__ float cf3_dither(int32_t &rng)
     constexpr float K = 0.95f * 2.3283064E-10f;  // exact: a power-of-two scaling of 0.95f
     return __fmul_rn((float)prng_next(rng), K);
 }
+// The dithered pack's exponent as a field, e << 23: the largest exponent
+// field of the three components, less 112 << 23, clamped to [0, 31 << 23]
+// (cf3_exponent with the shifts folded away; NaN and inf have the largest
+// field, so they give 31 as in the reference).
+__device__ __forceinline__ uint32_t cf3_exponent_field(float x, float y, float z)
+{
+    constexpr uint32_t EXP = 0x7f800000u, LOW = 112u << 23;
+    const uint32_t m = max(max(__float_as_uint(x) & EXP, __float_as_uint(y) & EXP), __float_as_uint(z) & EXP);
+    return min(max(m, LOW) - LOW, 31u << 23);
+}
+// One component of a dithered word, the 8-bit magnitude with the sign at bit
+// 8 inverted (cf3_pack flips the three sign bits at the end).  c = -clamp(x)
+// comes from one median: v_med3_f32(-x, -255, 255) equals
+// -fmaxf(-255, fminf(x, 255)) for every finite and infinite x and every
+// quiet NaN (x here is a sum, never a signalling NaN).  The reference tests
+// x < 0 after the clamp (tensors.h:368-371); the sign bit of clamp(x) is that
+// test because a dithered value is never -0 (the dither is +0 or nonzero,
+// and an exact zero sum rounds to +0) and NaN clamps to +255.
+__device__ __forceinline__ uint32_t cf3_dithered_component(float x)
+{
+    const float c = __builtin_amdgcn_fmed3f(-x, -255.f, 255.f);
+    const uint32_t m = (uint32_t)__fadd_rn(fabsf(c), .5f);
+    return m | ((__float_as_uint(c) >> 31) << 8);
+}
 // dithered pack (tensors.h:350-373): three draws, x then y then z
 __device__ __forceinline__ uint32_t cf3_pack(float x, float y, float z, int32_t &rng)
 {
-    const uint32_t e = cf3_exponent(x, y, z);
-    const float sc = cf3_scale(e);
+    const uint32_t e23 = cf3_exponent_field(x, y, z);
+    const float sc = __uint_as_float((127u + 22u) * (1u << 23) - e23);  // 2^(22-e), exact
     x = __fmul_rn(x, sc); y = __fmul_rn(y, sc); z = __fmul_rn(z, sc);
     x = __fadd_rn(x, cf3_dither(rng));
     y = __fadd_rn(y, cf3_dither(rng));
     z = __fadd_rn(z, cf3_dither(rng));
-    return cf3_assemble(e, x, y, z);
+    const uint32_t w = (e23 << 4) | cf3_dithered_component(x) | (cf3_dithered_component(y) << 9) |
+                       (cf3_dithered_component(z) << 18);
+    return w ^ ((1u << 8) | (1u << 17) | (1u << 26));
 }
 // undithered pack (tensors.h:374-393)
 __device__ __forceinline__ uint32_t cf3_pack_plain(float x, float y, float z)

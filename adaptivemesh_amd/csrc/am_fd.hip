@@ -285,9 +285,10 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
         if (j + 1 < ZB) gload(pre[j], z0 + 2 * P + 1 + j);
     __syncthreads();
 
+    // boundary offsets and their products are -1, 0 or 1: 24-bit multiplies
     const int ex = x - min(max(x, 1), S - 2);
     const int ey = y - min(max(y, 1), S - 2);
-    const int cxy = ex * ey;
+    const int cxy = __mul24(ex, ey);
     // dither seeds: the (x, y) part once, the z mix per plane
     const int32_t sxy_d = thread_seed_xy(gs_d, (uint32_t)x, (uint32_t)y);
     const int32_t sxy_m = thread_seed_xy(gs_m, (uint32_t)x, (uint32_t)y);
@@ -359,7 +360,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
         }
         if (want_m) {
             const int ez = z - min(max(z, 1), S - 2);
-            const int cxz = ex * ez, cyz = ey * ez;
+            const int cxz = __mul24(ex, ez), cyz = __mul24(ey, ez);
             const float *zm = pl[P - 1], *zp = pl[P + 1];
             float vxy, vxz, vyz;
             if (cxy == 0) {
@@ -378,7 +379,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
             if (cyz == 0) {
                 vyz = mixed_c(zp[c + T], zm[c - T], zm[c + T], zp[c - T]);
             } else {
-                const int dy = -cyz * T;  // +-1 along y, dz = one plane up
+                const int dy = __mul24(-cyz, T);  // +-1 along y, dz = one plane up
                 vyz = (float)(-cyz) * mixed_bd(ctr[c], zp[c + dy], zm[c - dy], ctr[c - dy], ctr[c + dy], zm[c], zp[c]);
             }
             int32_t rng = thread_seed_z(sxy_m, gz);

@@ -100,6 +100,62 @@ def test_fd_bit_exact(am, orc, torch_cuda, S, P, order, recipe, substep):
     assert np.array_equal(host(off, np.uint32), rm)
 
 
+def extreme_field(rng, n_slots, T):
+    """Per tile, normal values scaled by 1, 1e-6 (words whose exponent clamps
+    at 0), 1e-39 (subnormal values and products) or 0, with NaN, +-inf,
+    values near the float maximum and signed zeros sprinkled in: stencil
+    sums then cover NaN (from inf - inf or a NaN tap), inf, words whose
+    exponent clamps at 31 next to tiny components, exponent 0 and exact
+    zeros of either sign."""
+    f = rng.standard_normal((n_slots, T ** 3)).astype(np.float32)
+    f *= np.array([1.0, 1e-6, 1e-39, 0.0], np.float32)[rng.integers(0, 4, n_slots)][:, None]
+    f = f.reshape(-1)
+    pick = rng.random(f.size)
+    specials = np.array([np.nan, np.inf, -np.inf, 3e38, -3e38, 1e30, -2e-40, 3e-41, 0.0, -0.0, 1e-30, 7e-39],
+                        np.float32)
+    sel = pick < 0.02
+    f[sel] = specials[rng.integers(0, len(specials), sel.sum())]
+    return f
+
+
+@pytest.mark.parametrize("S,P,order", [(16, 3, 4), (28, 2, 4), (16, 3, 6)])
+def test_fd_words_on_extreme_values(am, orc, torch_cuda, S, P, order):
+    """The dithered pack over the whole float range: FD words of a field
+    holding NaN, inf, huge, subnormal and signed-zero values equal the
+    oracle's bit for bit (the saturating clamp, the exponent rule at both
+    ends, the sign of every component)."""
+    torch = torch_cuda
+    print(f"fd extreme values S={S} P={P} order={order}", flush=True)
+    g = am.geometry(S, P, order, 100.0, 0.4)
+    tree, e, n = setup_mesh(am, orc, "graded")
+    T = S + 2 * P
+    rng = np.random.default_rng(S * 31 + order)
+    field = extreme_field(rng, n, T)
+    seed_d, seed_m = int(rng.integers(-2 ** 31, 2 ** 31)), int(rng.integers(-2 ** 31, 2 ** 31))
+    tiles = fields.used_slots(e)
+    with np.errstate(all="ignore"):
+        r1, r2 = orc.differentiate(field, e["positions"], tiles, 0, e["max_depth"], seed_d, S, P, order // 2,
+                                   g.outer_dx, n)
+        rm = orc.mixed_partials(field, e["positions"], tiles, 0, e["max_depth"], seed_m, S, P, g.outer_dx, n)
+    # the sample does reach both exponent ends and the NaN / inf lanes
+    ex = r1 >> 27
+    assert (ex == 31).any() and (ex == 0).any() and ((r1 & 255) == 255).any()
+    f = dev(torch, field)
+    m = mesh_for(tree, -1, 0)
+    W = n * S ** 3
+    d1, d2, off = (torch.zeros(W, dtype=torch.int32, device="cuda") for _ in range(3))
+    b = am.FdBatch()
+    b.n_vars = 1
+    b.field[0], b.d1[0], b.d2diag[0], b.d2off[0] = f.data_ptr(), d1.data_ptr(), d2.data_ptr(), off.data_ptr()
+    b.seed_d[0], b.seed_m[0] = seed_d, seed_m
+    am.check(am.lib().am_fd_batched(C.byref(b), C.byref(m), C.byref(g), am.stream_handle()), "am_fd_batched")
+    sync(torch)
+    for got, want, what in ((d1, r1, "d1"), (d2, r2, "d2diag"), (off, rm, "d2off")):
+        g_ = host(got, np.uint32)
+        bad = np.flatnonzero(g_ != want)
+        assert bad.size == 0, (what, bad[:5], g_[bad[:5]], want[bad[:5]])
+
+
 @pytest.mark.parametrize("S,P,order", GEOMS)
 @pytest.mark.parametrize("recipe,periodic", [("graded", False), ("uniform2", True), ("nested3", False)])
 def test_fd_direct_faces(am, orc, torch_cuda, S, P, order, recipe, periodic):
