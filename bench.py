@@ -48,9 +48,9 @@ Q = FD_ORDER // 2  # stencil half-width: the ghost layers the FD reads and the p
 # read probes' calibration) and WRITE_SIZE (x 1.000)
 # (profiles/bssn_rk4_512tiles_hbm_counters.md), and the algorithmic bytes of
 # the same launches
-RHS_TRAFFIC_PER_LAUNCH = 1355523224
-RHS_ALGORITHMIC_PER_LAUNCH = 1354760192
-TRAFFIC_DIRECT_FACES = 4  # the mask the counters were read with
+RHS_TRAFFIC_PER_LAUNCH = 1317664164
+RHS_ALGORITHMIC_PER_LAUNCH = 1317011456
+TRAFFIC_DIRECT_FACES = 6  # the mask the counters were read with
 
 
 def face_stats(export, S, depth_filter=None):
