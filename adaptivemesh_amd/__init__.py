@@ -101,7 +101,7 @@ class _Trial(C.Structure):
 class TuneReport(C.Structure):
     _fields_ = [("z_block", C.c_int32), ("fd_grid_limit", C.c_int32), ("rhs_grid_limit", C.c_int32),
                 ("n_trials", C.c_int32), ("trial", _Trial * 32), ("fd_prefetch", C.c_int32),
-                ("direct_faces", C.c_int32)]
+                ("direct_faces", C.c_int32), ("overlap_depths", C.c_int32)]
 
 
 class MlpWork(C.Structure):
@@ -235,6 +235,8 @@ _SIGS = {
                                               C.POINTER(Geometry), _P, _P]),
     "am_event_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "am_event_record": (C.c_int, [_P, _P]),
+    "am_event_create_ordering": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "am_stream_wait_event": (C.c_int, [_P, _P]),
     "am_event_elapsed_ms": (C.c_int, [_P, _P, C.POINTER(C.c_float)]),
     "am_event_destroy": (C.c_int, [_P]),
     "am_bssn_sim_create": (_P, [C.POINTER(Geometry), C.c_uint32, C.POINTER(BssnSimOptions)]),
@@ -274,6 +276,8 @@ _RK4_SIM_SIGS = {
     "get_push_layers": (C.c_int, [_P]),
     "set_direct_faces": (C.c_int, [_P, C.c_int32]),
     "get_direct_faces": (C.c_int, [_P]),
+    "set_overlap_depths": (C.c_int, [_P, C.c_int32]),
+    "get_overlap_depths": (C.c_int, [_P]),
     "use_graph": (C.c_int, [_P, C.c_int32]),
     "autotune": (C.c_int, [_P, C.c_int32, _P]),
     "set_prolongation": (C.c_int, [_P, _P, C.c_float]),
@@ -706,6 +710,16 @@ class _Rk4Sim:
     def direct_faces(self) -> int:
         return self._f("get_direct_faces")(self.h)
 
+    def set_overlap_depths(self, on: bool):
+        """Run the coarser active depths of a sub-step on a second stream
+        beside the finest (when the finest has no coarser face); the
+        results do not change."""
+        check(self._f("set_overlap_depths")(self.h, 1 if on else 0), "set_overlap_depths")
+
+    @property
+    def overlap_depths(self) -> bool:
+        return bool(self._f("get_overlap_depths")(self.h))
+
     def get_launch(self) -> tuple:
         """(z_block, fd_grid_limit, rhs_grid_limit, fd_prefetch)"""
         out = np.zeros(4, np.int32)
@@ -718,9 +732,10 @@ class _Rk4Sim:
     def autotune(self, reps: int = 5) -> dict:
         r = TuneReport()
         check(self._f("autotune")(self.h, reps, C.byref(r)), "autotune")
-        kinds = ("fd_z_block", "fd_grid_limit", "rhs_grid_limit", "fd_prefetch", "direct_faces")
+        kinds = ("fd_z_block", "fd_grid_limit", "rhs_grid_limit", "fd_prefetch", "direct_faces", "overlap_depths")
         return {"z_block": r.z_block, "fd_grid_limit": r.fd_grid_limit, "rhs_grid_limit": r.rhs_grid_limit,
                 "fd_prefetch": r.fd_prefetch, "direct_faces": r.direct_faces,
+                "overlap_depths": bool(r.overlap_depths),
                 "trials": [(kinds[r.trial[i].kind], r.trial[i].value, round(r.trial[i].ms, 4))
                            for i in range(r.n_trials)]}
 

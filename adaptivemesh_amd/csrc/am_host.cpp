@@ -532,6 +532,16 @@ int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
         });                                                                                                        \
     }                                                                                                              \
     int P##_get_direct_faces(const H *s) { return s ? s->amr->sim_data.opt.direct_faces : -1; }                   \
+    int P##_set_overlap_depths(H *s, int32_t enable)                                                               \
+    {                                                                                                              \
+        if (enable != 0 && enable != 1)                                                                            \
+            return am::set_error(AM_ERR_INVALID, "overlap depths %d (0 or 1)", enable);                            \
+        return rk4_call(s, [&](auto &d) {                                                                          \
+            d.opt.overlap_depths = enable;                                                                         \
+            d.drop_graph();                                                                                        \
+        });                                                                                                        \
+    }                                                                                                              \
+    int P##_get_overlap_depths(const H *s) { return s ? s->amr->sim_data.opt.overlap_depths : -1; }               \
     int P##_set_fd_prefetch(H *s, int32_t planes)                                                                  \
     {                                                                                                              \
         if (planes != 1)                                                                                           \

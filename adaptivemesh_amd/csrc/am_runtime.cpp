@@ -197,5 +197,16 @@ int am_event_elapsed_ms(void *start, void *stop, float *ms)
     return am::hip_status(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop), "hipEventElapsedTime");
 }
 int am_event_destroy(void *ev) { return am::hip_status(hipEventDestroy((hipEvent_t)ev), "hipEventDestroy"); }
+int am_event_create_ordering(void **ev)
+{
+    if (!ev) return am::set_error(AM_ERR_INVALID, "am_event_create_ordering: null");
+    return am::hip_status(hipEventCreateWithFlags((hipEvent_t *)ev, hipEventDisableTiming),
+                          "hipEventCreateWithFlags");
+}
+int am_stream_wait_event(void *stream, void *ev)
+{
+    if (!ev) return am::set_error(AM_ERR_INVALID, "am_stream_wait_event: null event");
+    return am::hip_status(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)ev, 0), "hipStreamWaitEvent");
+}
 
 }  // extern "C"

@@ -492,7 +492,7 @@ def main():
                    "launch": {"z_block": sim.get_launch()[0], "fd_grid_limit": sim.get_launch()[1],
                               "rhs_grid_limit": sim.get_launch()[2], "fd_prefetch_planes": sim.get_launch()[3],
                               "push_layers": sim.push_layers,
-                              "direct_faces": sim.direct_faces,
+                              "direct_faces": sim.direct_faces, "overlap_depths": sim.overlap_depths,
                               "autotuned": tune is not None,
                               "autotune_trials": tune["trials"] if tune else None,
                               "graph_per_coarse_step": not args.no_graph}},

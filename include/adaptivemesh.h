@@ -153,6 +153,12 @@ int am_event_create(void **ev);
 int am_event_record(void *ev, void *stream);
 int am_event_elapsed_ms(void *start, void *stop, float *ms); /* synchronises on stop */
 int am_event_destroy(void *ev);
+/* Ordering between streams (no counterpart in the reference, which runs on
+ * one stream): an event without timing, and a wait that makes later work on
+ * `stream` start after the work recorded into `ev`.  Both work under graph
+ * capture, where they become edges between the two streams' nodes. */
+int am_event_create_ordering(void **ev);
+int am_stream_wait_event(void *stream, void *ev);
 
 /* ------------------------------------------------------------- graphs
  * No counterpart in the reference, which launches eagerly between device

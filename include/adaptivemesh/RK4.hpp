@@ -112,18 +112,28 @@ struct rk4_options {
     // and pushes all P ghost layers)
     int32_t upwind = 0;
     float ko_sigma = 0.f;
+    // When the finest depth active in a sub-step has no face whose target
+    // is coarser, its step reads nothing the coarser depths write in that
+    // sub-step (their kernels touch only their own tiles and equal-depth
+    // neighbours), so the coarser depths run on a second stream beside it;
+    // both join before the sub-step ends.  The launches, their order on
+    // each stream and the dither draws are those of the serial schedule, so
+    // the results are the same bit for bit.  0: one stream.
+    int32_t overlap_depths = 1;
 };
 
 // What autotune() measured: each trial's launch shape and mean ms per launch.
 struct rk4_tune_report {
     int32_t z_block, fd_grid_limit, rhs_grid_limit, n_trials;
     struct {
-        int32_t kind;  // 0 FD z_block, 1 FD grid limit, 2 RHS grid limit, 3 FD prefetch planes, 4 direct faces
+        int32_t kind;  // 0 FD z_block, 1 FD grid limit, 2 RHS grid limit, 3 FD prefetch planes, 4 direct faces,
+                       // 5 overlap depths
         int32_t value;
         float ms;      // best of the repetitions, one whole coarse step
     } trial[32];
     int32_t fd_prefetch;
     int32_t direct_faces;
+    int32_t overlap_depths;
 };
 
 // Device views of the driver's buffers handed to a System's kernels.
@@ -160,6 +170,7 @@ struct rk4_AMR_data {
     rk4_options opt;
     options_t sys;
     void *stream = nullptr;
+    void *aux_stream = nullptr, *fork_ev = nullptr, *join_ev = nullptr;  // the overlap of coarser depths
     size_t cells = 0, inner = 0, slots = 0;
     size_t padded = 0;  // T^3: floats per slot of a padded field
     size_t word_stride = 0;  // bytes of one field's words: slots * inner * am_word_bytes(word_format)
@@ -232,6 +243,9 @@ struct rk4_AMR_data {
         old_.zero(); new_.zero(); acc.zero();
         if (older.created) older.zero();
         check(am_stream_create(&stream), "am_stream_create");
+        check(am_stream_create(&aux_stream), "am_stream_create");
+        check(am_event_create_ordering(&fork_ev), "am_event_create_ordering");
+        check(am_event_create_ordering(&join_ev), "am_event_create_ordering");
     }
     void destroy()
     {
@@ -242,8 +256,12 @@ struct rk4_AMR_data {
         mlp_stats.destroy();
         mlp_tile_resid.destroy();
         timer.destroy();
+        if (aux_stream) am_stream_sync(aux_stream);
+        if (fork_ev) am_event_destroy(fork_ev);
+        if (join_ev) am_event_destroy(join_ev);
+        am_stream_destroy(aux_stream);
         am_stream_destroy(stream);
-        stream = nullptr;
+        stream = aux_stream = fork_ev = join_ev = nullptr;
     }
     void drop_graph()
     {
@@ -534,8 +552,33 @@ struct rk4_AMR_data {
     void substep()
     {
         copy_back();
+        int finest = -1, active = 0;
         for (int d = 0; d <= parent.read_max_depth(); d++)
-            if (parent.depth_active(d)) step_depth(d);
+            if (parent.depth_active(d)) { finest = d; active++; }
+        // per-launch event timing stays on one stream
+        const bool overlap = opt.overlap_depths && !timer.on && active > 1 && finest > 0 &&
+                             parent.coarse_faces_at_depth(finest) == 0;
+        if (!overlap) {
+            for (int d = 0; d <= finest; d++)
+                if (parent.depth_active(d)) step_depth(d);
+        } else {
+            // the coarser depths on the second stream, after the copy-back
+            check(am_event_record(fork_ev, stream), "fork");
+            check(am_stream_wait_event(aux_stream, fork_ev), "fork");
+            void *const main_stream = stream;
+            stream = aux_stream;
+            try {
+                for (int d = 0; d < finest; d++)
+                    if (parent.depth_active(d)) step_depth(d);
+            } catch (...) {
+                stream = main_stream;
+                throw;
+            }
+            stream = main_stream;
+            step_depth(finest);
+            check(am_event_record(join_ev, aux_stream), "join");
+            check(am_stream_wait_event(stream, join_ev), "join");
+        }
         parent.increment_timer();
     }
     // the reference's timestep(): sub-steps until the counter wraps; with
@@ -667,6 +710,7 @@ struct rk4_AMR_data {
             sweep(4, opt.direct_faces, {4, 6, 0});
         sweep(1, opt.fd_grid_limit, {0, 8 * cu});
         sweep(2, opt.rhs_grid_limit, {0, 2 * cu, 4 * cu});
+        sweep(5, opt.overlap_depths, {1, 0});
         copy_all(false);
         check(am_stream_sync(stream), "autotune");
         s_old.destroy(); s_new.destroy(); s_acc.destroy(); s_older.destroy();
@@ -683,6 +727,7 @@ struct rk4_AMR_data {
         r.rhs_grid_limit = opt.rhs_grid_limit;
         r.fd_prefetch = opt.fd_prefetch;
         r.direct_faces = opt.direct_faces;
+        r.overlap_depths = opt.overlap_depths;
         if (rep) *rep = r;
     }
     // fill every face ghost of `new` from the current data (initial data):

@@ -136,12 +136,13 @@ typedef struct am_tune_report {
     int32_t z_block, fd_grid_limit, rhs_grid_limit, n_trials;
     struct {
         int32_t kind; /* 0 FD z_block, 1 FD workgroup cap, 2 RHS workgroup cap, 3 FD prefetch planes,
-                       * 4 direct-face axis mask */
+                       * 4 direct-face axis mask, 5 overlap of coarser depths */
         int32_t value;
         float ms; /* best of the repetitions, one coarse step */
     } trial[32];
     int32_t fd_prefetch;
     int32_t direct_faces;
+    int32_t overlap_depths;
 } am_tune_report;
 
 /* --------------------------------------------- BSSN RK4 evolution driver
@@ -234,6 +235,14 @@ int am_bssn_sim_get_push_layers(const am_bssn_sim *s);
  * _get_direct_faces gives the mask. */
 int am_bssn_sim_set_direct_faces(am_bssn_sim *s, int32_t axes);
 int am_bssn_sim_get_direct_faces(const am_bssn_sim *s);
+/* Overlap of depths (default 1): when the finest depth active in a sub-step
+ * has no face whose target is coarser, the coarser active depths run on the
+ * driver's second stream beside it (their kernels touch only their own
+ * tiles and equal-depth neighbours); the two streams join before the
+ * sub-step ends and the results are those of one stream bit for bit.  With
+ * per-kernel timing on, or 0 here, every depth runs on one stream. */
+int am_bssn_sim_set_overlap_depths(am_bssn_sim *s, int32_t enable);
+int am_bssn_sim_get_overlap_depths(const am_bssn_sim *s);
 int am_bssn_sim_use_graph(am_bssn_sim *s, int32_t enable);
 int am_bssn_sim_autotune(am_bssn_sim *s, int32_t reps, am_tune_report *out);
 /* AM_INTERP_LEARNED: the MLP parameters (device, am_mlp_param_count()
@@ -297,6 +306,8 @@ int am_swave_sim_set_push_layers(am_swave_sim *s, int32_t layers);
 int am_swave_sim_get_push_layers(const am_swave_sim *s);
 int am_swave_sim_set_direct_faces(am_swave_sim *s, int32_t axes);
 int am_swave_sim_get_direct_faces(const am_swave_sim *s);
+int am_swave_sim_set_overlap_depths(am_swave_sim *s, int32_t enable);
+int am_swave_sim_get_overlap_depths(const am_swave_sim *s);
 int am_swave_sim_use_graph(am_swave_sim *s, int32_t enable);
 int am_swave_sim_autotune(am_swave_sim *s, int32_t reps, am_tune_report *out);
 int am_swave_sim_save(am_swave_sim *s, const char *path);

@@ -10,6 +10,10 @@
 * autotune() (whole coarse steps from a snapshot) leaves the state as it
   found it: the next timestep equals an untuned simulation's, counter,
   seeds and tile lifetimes included.
+* The overlap of coarser depths on a second stream (on by default) gives
+  the one-stream results bit for bit, eager and replayed, on meshes where it
+  forks (uniform levels with their ancestors) and where it must not (a
+  graded mesh whose finest depth reads coarser faces).
 """
 import numpy as np
 import pytest
@@ -40,7 +44,7 @@ def smooth(e, n, outer, seed):
 
 
 def bssn_pair(am, orc, interp, periodic, recipe="graded", outer=16.0):
-    ops, n = {"graded": meshes.graded(), "uniform1": meshes.uniform(1)}[recipe]
+    ops, n = {"graded": meshes.graded(), "uniform1": meshes.uniform(1), "uniform2": meshes.uniform(2)}[recipe]
     g = am.geometry(S, P, ORDER, outer, 0.25)
     sims = [am.BssnSim(g, n, am.bssn_options(periodic=periodic, interp=interp)) for _ in range(2)]
     ot = orc.OracleTree(n, periodic)
@@ -129,10 +133,12 @@ def test_autotune_restores_state(am, orc, torch_cuda):
     tuned.timestep()
     rep = tuned.autotune(reps=2)
     print(f"autotune: {rep}", flush=True)
-    assert rep["z_block"] in (4, 8, 16) and rep["fd_prefetch"] == 1 and len(rep["trials"]) == 11
+    assert rep["z_block"] in (4, 8, 16) and rep["fd_prefetch"] == 1 and len(rep["trials"]) == 13
     # the mask it keeps need not be the default: the steps below still equal the reference run's
     assert rep["direct_faces"] in (0, 4, 6) and tuned.direct_faces == rep["direct_faces"]
     assert [t[1] for t in rep["trials"] if t[0] == "direct_faces"] == [4, 6, 0]
+    assert [t[1] for t in rep["trials"] if t[0] == "overlap_depths"] == [1, 0]
+    assert tuned.overlap_depths == rep["overlap_depths"]
     assert tuned.get_launch() == (rep["z_block"], rep["fd_grid_limit"], rep["rhs_grid_limit"], rep["fd_prefetch"])
     tuned.sync()
     assert same_state(ref, tuned, 24, (0, 1, 2, 3)) is None
@@ -143,3 +149,25 @@ def test_autotune_restores_state(am, orc, torch_cuda):
     ref.sync()
     tuned.sync()
     assert same_state(ref, tuned, 24, (0, 1, 2, 3)) is None
+
+
+@pytest.mark.parametrize("recipe,periodic,interp", [("uniform1", True, 0), ("uniform2", True, 0),
+                                                    ("graded", False, 2)])
+def test_overlap_depths_bit_identical(am, orc, torch_cuda, recipe, periodic, interp):
+    print(f"overlap of depths: {recipe} periodic={periodic} interp={interp}", flush=True)
+    (one, two), n = bssn_pair(am, orc, interp, periodic, recipe)
+    assert two.overlap_depths  # the default
+    one.set_overlap_depths(False)
+    assert not one.overlap_depths
+    for graph in (False, True):
+        one.use_graph(graph)
+        two.use_graph(graph)
+        for step in range(2):
+            one.timestep()
+            two.timestep()
+            one.sync()
+            two.sync()
+            assert same_state(one, two, 24) is None, (graph, step)
+            assert one.rng_state == two.rng_state and one.counter() == two.counter()
+    with pytest.raises(am.AMError):  # the switch is 0 or 1
+        am.check(one._f("set_overlap_depths")(one.h, 2), "set_overlap_depths")
