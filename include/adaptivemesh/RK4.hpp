@@ -171,6 +171,8 @@ struct rk4_AMR_data {
     options_t sys;
     void *stream = nullptr;
     void *aux_stream = nullptr, *fork_ev = nullptr, *join_ev = nullptr;  // the overlap of coarser depths
+    void *copied_ev = nullptr;      // the copy-back done, on the second stream
+    void *stage0_wait = nullptr;    // the next stage-0 RHS waits for this event (then cleared)
     size_t cells = 0, inner = 0, slots = 0;
     size_t padded = 0;  // T^3: floats per slot of a padded field
     size_t word_stride = 0;  // bytes of one field's words: slots * inner * am_word_bytes(word_format)
@@ -246,6 +248,7 @@ struct rk4_AMR_data {
         check(am_stream_create(&aux_stream), "am_stream_create");
         check(am_event_create_ordering(&fork_ev), "am_event_create_ordering");
         check(am_event_create_ordering(&join_ev), "am_event_create_ordering");
+        check(am_event_create_ordering(&copied_ev), "am_event_create_ordering");
     }
     void destroy()
     {
@@ -259,9 +262,10 @@ struct rk4_AMR_data {
         if (aux_stream) am_stream_sync(aux_stream);
         if (fork_ev) am_event_destroy(fork_ev);
         if (join_ev) am_event_destroy(join_ev);
+        if (copied_ev) am_event_destroy(copied_ev);
         am_stream_destroy(aux_stream);
         am_stream_destroy(stream);
-        stream = aux_stream = fork_ev = join_ev = nullptr;
+        stream = aux_stream = fork_ev = join_ev = copied_ev = nullptr;
     }
     void drop_graph()
     {
@@ -502,6 +506,10 @@ struct rk4_AMR_data {
         if (s < 3) f.push_axes = 7 & ~direct;
         am_mesh mr = m;
         mr.grid_limit = opt.rhs_grid_limit;
+        if (stage0_wait && s == 0) {  // the stage-0 RHS writes new, which the copy-back reads
+            check(am_stream_wait_event(stream, stage0_wait), "copy-back wait");
+            stage0_wait = nullptr;
+        }
         int ev = timer.begin(kernel_timer::RHS);
         check(System::rhs_stage(f, s, sys, mr, geom, f.push_axes ? 1 : 0, stream), "rhs");
         timer.end(ev);
@@ -551,23 +559,33 @@ struct rk4_AMR_data {
     }
     void substep()
     {
-        copy_back();
         int finest = -1, active = 0;
         for (int d = 0; d <= parent.read_max_depth(); d++)
             if (parent.depth_active(d)) { finest = d; active++; }
-        // per-launch event timing stays on one stream
-        const bool overlap = opt.overlap_depths && !timer.on && active > 1 && finest > 0 &&
+        // Per-launch event timing stays on one stream.  When the finest
+        // active depth reads no coarser face, the coarser depths go to the
+        // second stream; when its tiles are also leaves (the tree's deepest
+        // level), the copy-back goes there too: the finest depth's stage-0
+        // derivatives only read new, and its stage-0 RHS, which writes new,
+        // waits for the copy-back.
+        const bool overlap = opt.overlap_depths && !timer.on && finest > 0 &&
                              parent.coarse_faces_at_depth(finest) == 0;
-        if (!overlap) {
+        const bool copy_aside = overlap && finest == parent.read_max_depth();
+        if (!overlap || (active == 1 && !copy_aside)) {
+            copy_back();
             for (int d = 0; d <= finest; d++)
                 if (parent.depth_active(d)) step_depth(d);
         } else {
-            // the coarser depths on the second stream, after the copy-back
+            if (!copy_aside) copy_back();
             check(am_event_record(fork_ev, stream), "fork");
             check(am_stream_wait_event(aux_stream, fork_ev), "fork");
             void *const main_stream = stream;
             stream = aux_stream;
             try {
+                if (copy_aside) {
+                    copy_back();
+                    check(am_event_record(copied_ev, stream), "copy-back done");
+                }
                 for (int d = 0; d < finest; d++)
                     if (parent.depth_active(d)) step_depth(d);
             } catch (...) {
@@ -575,7 +593,14 @@ struct rk4_AMR_data {
                 throw;
             }
             stream = main_stream;
-            step_depth(finest);
+            stage0_wait = copy_aside ? copied_ev : nullptr;
+            try {
+                step_depth(finest);
+            } catch (...) {
+                stage0_wait = nullptr;
+                throw;
+            }
+            stage0_wait = nullptr;
             check(am_event_record(join_ev, aux_stream), "join");
             check(am_stream_wait_event(stream, join_ev), "join");
         }
