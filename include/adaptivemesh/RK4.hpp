@@ -112,13 +112,13 @@ struct rk4_options {
     // and pushes all P ghost layers)
     int32_t upwind = 0;
     float ko_sigma = 0.f;
-    // When the finest depth active in a sub-step has no face whose target
-    // is coarser, its step reads nothing the coarser depths write in that
-    // sub-step (their kernels touch only their own tiles and equal-depth
-    // neighbours), so the coarser depths run on a second stream beside it;
-    // both join before the sub-step ends.  The launches, their order on
-    // each stream and the dither draws are those of the serial schedule, so
-    // the results are the same bit for bit.  0: one stream.
+    // The copy-back and the coarser depths of a sub-step run on a second
+    // stream beside the finest depth, which reads nothing they write until
+    // its first coarse-face fill (their kernels touch only their own tiles
+    // and equal-depth neighbours); that fill waits for them, and the stage-0
+    // RHS waits for the copy-back (substep()).  The launches, their order
+    // on each stream and the dither draws are those of the one-stream
+    // schedule, so the results are the same bit for bit.  0: one stream.
     int32_t overlap_depths = 1;
 };
 
@@ -173,6 +173,7 @@ struct rk4_AMR_data {
     void *aux_stream = nullptr, *fork_ev = nullptr, *join_ev = nullptr;  // the overlap of coarser depths
     void *copied_ev = nullptr;      // the copy-back done, on the second stream
     void *stage0_wait = nullptr;    // the next stage-0 RHS waits for this event (then cleared)
+    void *fill_wait = nullptr;      // the next coarse-face fill waits for this event (then cleared)
     size_t cells = 0, inner = 0, slots = 0;
     size_t padded = 0;  // T^3: floats per slot of a padded field
     size_t word_stride = 0;  // bytes of one field's words: slots * inner * am_word_bytes(word_format)
@@ -514,6 +515,10 @@ struct rk4_AMR_data {
         check(System::rhs_stage(f, s, sys, mr, geom, f.push_axes ? 1 : 0, stream), "rhs");
         timer.end(ev);
         if (d > 0 && parent.coarse_faces_at_depth(d) > 0) {
+            if (fill_wait) {  // the coarser depths of this sub-step have finished their step
+                check(am_stream_wait_event(stream, fill_wait), "coarser depths wait");
+                fill_wait = nullptr;
+            }
             am_field_batch b = batch(true);
             ev = timer.begin(kernel_timer::GHOST);
             // only the faces whose target is coarser have cells to fill
@@ -562,46 +567,45 @@ struct rk4_AMR_data {
         int finest = -1, active = 0;
         for (int d = 0; d <= parent.read_max_depth(); d++)
             if (parent.depth_active(d)) { finest = d; active++; }
-        // Per-launch event timing stays on one stream.  When the finest
-        // active depth reads no coarser face, the coarser depths go to the
-        // second stream; when its tiles are also leaves (the tree's deepest
-        // level), the copy-back goes there too: the finest depth's stage-0
-        // derivatives only read new, and its stage-0 RHS, which writes new,
-        // waits for the copy-back.
-        const bool overlap = opt.overlap_depths && !timer.on && finest > 0 &&
-                             parent.coarse_faces_at_depth(finest) == 0;
-        const bool copy_aside = overlap && finest == parent.read_max_depth();
-        if (!overlap || (active == 1 && !copy_aside)) {
+        // Two streams (per-launch event timing stays on one).  The finest
+        // active depth is the tree's deepest level, so its tiles are leaves.
+        // The copy-back and the coarser depths go to the second stream.  The
+        // finest depth's kernels read only its own tiles and equal-depth
+        // neighbours, except its coarse-face fills, which read the coarser
+        // depths' step: its stage-0 derivatives only read new, its stage-0
+        // RHS (the first write of new) waits for the copy-back, and its
+        // first coarse-face fill waits for the coarser depths.  Without
+        // coarse faces the streams join at the end of the sub-step.
+        const bool overlap = opt.overlap_depths && !timer.on && finest > 0 && finest == parent.read_max_depth();
+        if (!overlap) {
             copy_back();
             for (int d = 0; d <= finest; d++)
                 if (parent.depth_active(d)) step_depth(d);
         } else {
-            if (!copy_aside) copy_back();
             check(am_event_record(fork_ev, stream), "fork");
             check(am_stream_wait_event(aux_stream, fork_ev), "fork");
             void *const main_stream = stream;
             stream = aux_stream;
             try {
-                if (copy_aside) {
-                    copy_back();
-                    check(am_event_record(copied_ev, stream), "copy-back done");
-                }
+                copy_back();
+                check(am_event_record(copied_ev, stream), "copy-back done");
                 for (int d = 0; d < finest; d++)
                     if (parent.depth_active(d)) step_depth(d);
+                check(am_event_record(join_ev, stream), "join");
             } catch (...) {
                 stream = main_stream;
                 throw;
             }
             stream = main_stream;
-            stage0_wait = copy_aside ? copied_ev : nullptr;
+            stage0_wait = copied_ev;
+            fill_wait = join_ev;
             try {
                 step_depth(finest);
             } catch (...) {
-                stage0_wait = nullptr;
+                stage0_wait = fill_wait = nullptr;
                 throw;
             }
-            stage0_wait = nullptr;
-            check(am_event_record(join_ev, aux_stream), "join");
+            stage0_wait = fill_wait = nullptr;
             check(am_stream_wait_event(stream, join_ev), "join");
         }
         parent.increment_timer();

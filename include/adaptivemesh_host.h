@@ -235,12 +235,13 @@ int am_bssn_sim_get_push_layers(const am_bssn_sim *s);
  * _get_direct_faces gives the mask. */
 int am_bssn_sim_set_direct_faces(am_bssn_sim *s, int32_t axes);
 int am_bssn_sim_get_direct_faces(const am_bssn_sim *s);
-/* Overlap of depths (default 1): when the finest depth active in a sub-step
- * has no face whose target is coarser, the coarser active depths run on the
- * driver's second stream beside it (their kernels touch only their own
- * tiles and equal-depth neighbours); the two streams join before the
- * sub-step ends and the results are those of one stream bit for bit.  With
- * per-kernel timing on, or 0 here, every depth runs on one stream. */
+/* Overlap of depths (default 1): the copy-back and the coarser active depths
+ * of a sub-step run on the driver's second stream beside the finest depth
+ * (their kernels touch only their own tiles and equal-depth neighbours); the
+ * finest depth's first write of new waits for the copy-back, its first
+ * coarse-face fill for the coarser depths, and the two streams join before
+ * the sub-step ends.  The results are those of one stream bit for bit.  With
+ * per-kernel timing on, or 0 here, every launch runs on one stream. */
 int am_bssn_sim_set_overlap_depths(am_bssn_sim *s, int32_t enable);
 int am_bssn_sim_get_overlap_depths(const am_bssn_sim *s);
 int am_bssn_sim_use_graph(am_bssn_sim *s, int32_t enable);

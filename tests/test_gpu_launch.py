@@ -11,9 +11,10 @@
   found it: the next timestep equals an untuned simulation's, counter,
   seeds and tile lifetimes included.
 * The overlap of coarser depths on a second stream (on by default) gives
-  the one-stream results bit for bit, eager and replayed, on meshes where it
-  forks (uniform levels with their ancestors) and where it must not (a
-  graded mesh whose finest depth reads coarser faces).
+  the one-stream results bit for bit, eager and replayed, on uniform levels
+  with their ancestors and on a graded mesh whose finest depth reads
+  coarser faces (its first coarse-face fill waits for the coarser depths),
+  with tricubic three-level fills.
 """
 import numpy as np
 import pytest
