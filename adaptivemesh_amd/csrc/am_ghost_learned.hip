@@ -6,12 +6,12 @@
 // overwrites the cells it predicts.  Faces whose target is one level coarser
 // and not the root are covered by a box of coarse cells of the target (2 x 8
 // x 8 for S = 16, P = 3); each coarse cell is one sample, and one workgroup
-// (8 waves, 8 threads per sample) takes a block of 64 samples:
+// (NW waves, 8 threads per sample) takes a block of 8 NW samples:
 //   1. per field: gather the cell's 4^3 stencil (offsets -1..2) of the
 //      target at the stage time (quadratic in time through older/old/new,
 //      as the tricubic fill), normalise by the centre value mu and the
 //      spread sigma, add the tricubic baseline of the 8 children, and
-//      accumulate layer 1 (64 samples x 64 inputs x 128) on MFMA;
+//      accumulate layer 1 (8 NW samples x 64 inputs x 128) on MFMA;
 //   2. layers 2 and 3 on MFMA from LDS; output + baseline = normalised
 //      children;
 //   3. the restriction residual |mean(children) - mu| (field units) is the
@@ -66,8 +66,14 @@ __device__ __forceinline__ float read_pos(uint32_t w, float rel, float rel_scale
     return (r + .5f) * (float)S + (float)P - 0.5f;
 }
 
-template <int S, int P, int L>
-__global__ void __launch_bounds__(512) k_ghost_learned(FieldBatchDev b, const uint64_t *__restrict__ bnd,
+// NW = 8 (64-sample blocks, 128 KB of LDS, one workgroup per CU) serves the
+// per-block gate, whose blocks are 64 samples.  NW = 4 (32 samples, 64 KB)
+// serves the per-tile gate, where the block size does not show in the
+// result: two workgroups share a CU, so one computes while the other waits
+// on its field barrier or its gathers.  Every output element goes through
+// the same operations in the same order for either NW.
+template <int S, int P, int L, int NW>
+__global__ void __launch_bounds__(64 * NW) k_ghost_learned(FieldBatchDev b, const uint64_t *__restrict__ bnd,
                                                        const int32_t *__restrict__ tiles,
                                                        const int32_t *__restrict__ faces, uint32_t substep,
                                                        uint32_t maxd, float frac, const float *__restrict__ prm,
@@ -77,11 +83,14 @@ __global__ void __launch_bounds__(512) k_ghost_learned(FieldBatchDev b, const ui
     constexpr int NC = S * S * P;
     constexpr size_t CELLS = tile_stride<S, P, L>();
     constexpr int T = S + 2 * P;
-    __shared__ float Xs[2][64][65];
-    __shared__ float Hs[64][129];
-    __shared__ float Bn[64][193];
-    __shared__ float musg[2][24][64];
-    __shared__ int valid[64];
+    constexpr int NT = 64 * NW, NSMP = 8 * NW;  // threads, samples per block
+    constexpr int RB = NSMP / 16, CT = L_HID / 16 / NW;  // row blocks of samples, hidden column tiles per wave
+    static_assert(NW == 4 || NW == 8, "8 threads per sample, 16-row MFMA blocks, 128 hidden columns");
+    __shared__ float Xs[2][NSMP][65];
+    __shared__ float Hs[NSMP][129];
+    __shared__ float Bn[NSMP][193];
+    __shared__ float musg[2][24][NSMP];
+    __shared__ int valid[NSMP];
     __shared__ int face_resid_bits;
     __shared__ int n_invalid;
 
@@ -127,19 +136,21 @@ __global__ void __launch_bounds__(512) k_ghost_learned(FieldBatchDev b, const ui
         }
     }
     const int NS = ncl[0] * ncl[1] * ncl[2];
-    const int chunk = blk * 64;
+    const int chunk = blk * NSMP;
     if (chunk >= NS) return;  // uniform per workgroup
     if (tid == 0) {
         face_resid_bits = 0;
         n_invalid = 0;
     }
     const size_t tbase = (size_t)tgt * CELLS;
-    const int ns = min(64, NS - chunk);
-    if (tid < 64) valid[tid] = 1;
+    const int ns = min(NSMP, NS - chunk);
+    if (tid < NSMP) valid[tid] = 1;
     __syncthreads();
-    floatx4 acc[4];
+    floatx4 acc[RB][CT];
 #pragma unroll
-    for (int i = 0; i < 4; i++) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int c = 0; c < CT; c++) acc[i][c] = floatx4{0.f, 0.f, 0.f, 0.f};
     // sample s and its eighth q of the 4^3 stencil: plane k = q / 2, rows
     // j = 2 (q % 2) + {0, 1}, 4 points each
     const int s = tid >> 3, q = tid & 7;
@@ -176,19 +187,34 @@ __global__ void __launch_bounds__(512) k_ghost_learned(FieldBatchDev b, const ui
             raw[i] = val;
         }
     };
-    // layer-1 weights of field v for this lane: W1[v * 64 + 4 kk + lane / 16][16 wave + lane % 16], kk = 0..15
-    float wcur[16], wnext[16];
-    auto load_w1 = [&](int v, float (&w)[16]) {
-        const float *wrow = prm + L_W1 + (int64_t)(v * 64 + (lane >> 4)) * L_HID + wave * 16 + (lane & 15);
+    // layer-1 weights of field v for this lane and column tile ct:
+    // W1[v * 64 + 4 kk + lane / 16][16 (CT wave + ct) + lane % 16], kk = 0..15
+    float wcur[CT][16], wnext[NW == 8 ? CT : 1][16];
+    auto load_w1 = [&](int v, float (&w)[CT][16]) {
 #pragma unroll
-        for (int kk = 0; kk < 16; kk++) w[kk] = wrow[(int64_t)kk * 4 * L_HID];
+        for (int ct = 0; ct < CT; ct++) {
+            const float *wrow =
+                prm + L_W1 + (int64_t)(v * 64 + (lane >> 4)) * L_HID + (CT * wave + ct) * 16 + (lane & 15);
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) w[ct][kk] = wrow[(int64_t)kk * 4 * L_HID];
+        }
     };
+    // NW = 8 loads the next field's weights a field ahead; NW = 4, which
+    // holds twice the weights per wave, loads each field's at its start
+    // (their latency then overlaps the field's normalisation, barrier and
+    // baseline) to stay within two waves' registers per SIMD.
     gather(0);
-    load_w1(0, wnext);
+    if constexpr (NW == 8) load_w1(0, wnext);
     for (int v = 0; v < 24; v++) {
         const int buf = v & 1;
+        if constexpr (NW == 8) {
 #pragma unroll
-        for (int kk = 0; kk < 16; kk++) wcur[kk] = wnext[kk];
+            for (int ct = 0; ct < CT; ct++)
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) wcur[ct][kk] = wnext[ct][kk];
+        } else {
+            load_w1(v, wcur);
+        }
         const float mu = __shfl(raw[5], (lane & ~7) | 2, 64);
         float dev = 0.f;
 #pragma unroll
@@ -209,7 +235,7 @@ __global__ void __launch_bounds__(512) k_ghost_learned(FieldBatchDev b, const ui
         __syncthreads();
         if (v + 1 < 24) {
             gather(v + 1);
-            load_w1(v + 1, wnext);
+            if constexpr (NW == 8) load_w1(v + 1, wnext);
         }
         // baseline of the 8 children (normalised units): child q of sample s
         if (s < ns) {
@@ -224,62 +250,78 @@ __global__ void __launch_bounds__(512) k_ghost_learned(FieldBatchDev b, const ui
                         bsum += LQ[jz][k] * LQ[jy][j] * LQ[jx][i] * Xs[buf][s][k * 16 + j * 4 + i];
             Bn[s][v * 8 + q] = bsum;
         }
-        // layer 1 contribution of this field: Z1[:, 16 wave ..] += Xs (64 x 64) W1[v*64.., 16 wave ..]
+        // layer 1 contribution of this field: Z1[:, this wave's columns] += Xs (NSMP x 64) W1[v*64.., columns]
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) {
             const int ka = kk * 4 + (lane >> 4);
 #pragma unroll
-            for (int rb = 0; rb < 4; rb++) {
+            for (int rb = 0; rb < RB; rb++) {
                 const float a = Xs[buf][rb * 16 + (lane & 15)][ka];
-                acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wcur[kk], acc[rb], 0, 0, 0);
+#pragma unroll
+                for (int ct = 0; ct < CT; ct++)
+                    acc[rb][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wcur[ct][kk], acc[rb][ct], 0, 0, 0);
             }
         }
     }
     __syncthreads();
     // layer 1 epilogue
-    const int col = wave * 16 + (lane & 15);
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++)
+    for (int ct = 0; ct < CT; ct++) {
+        const int col = (CT * wave + ct) * 16 + (lane & 15);
 #pragma unroll
-        for (int r = 0; r < 4; r++) Hs[rb * 16 + 4 * (lane >> 4) + r][col] = gelu_l(acc[rb][r] + prm[L_B1 + col]);
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                Hs[rb * 16 + 4 * (lane >> 4) + r][col] = gelu_l(acc[rb][ct][r] + prm[L_B1 + col]);
+    }
     __syncthreads();
     // layer 2
 #pragma unroll
-    for (int i = 0; i < 4; i++) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
-    {
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int c = 0; c < CT; c++) acc[i][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+        const int col = (CT * wave + ct) * 16 + (lane & 15);
         const float *wrow = prm + L_W2 + (int64_t)(lane >> 4) * L_HID + col;
 #pragma unroll 8
         for (int kk = 0; kk < 32; kk++) {
             const int ka = kk * 4 + (lane >> 4);
             const float b0 = wrow[(int64_t)kk * 4 * L_HID];
 #pragma unroll
-            for (int rb = 0; rb < 4; rb++)
-                acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(Hs[rb * 16 + (lane & 15)][ka], b0, acc[rb], 0, 0, 0);
+            for (int rb = 0; rb < RB; rb++)
+                acc[rb][ct] =
+                    __builtin_amdgcn_mfma_f32_16x16x4f32(Hs[rb * 16 + (lane & 15)][ka], b0, acc[rb][ct], 0, 0, 0);
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++)
+    for (int ct = 0; ct < CT; ct++) {
+        const int col = (CT * wave + ct) * 16 + (lane & 15);
 #pragma unroll
-        for (int r = 0; r < 4; r++) Hs[rb * 16 + 4 * (lane >> 4) + r][col] = gelu_l(acc[rb][r] + prm[L_B2 + col]);
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                Hs[rb * 16 + 4 * (lane >> 4) + r][col] = gelu_l(acc[rb][ct][r] + prm[L_B2 + col]);
+    }
     __syncthreads();
-    // layer 3: 12 output tiles of 16 columns, tiles wave and wave + 8, added onto the baseline
-    for (int t = wave; t < 12; t += 8) {
+    // layer 3: 12 output tiles of 16 columns, tiles wave, wave + NW, ..., added onto the baseline
+    for (int t = wave; t < 12; t += NW) {
         const int oc = t * 16 + (lane & 15);
-        floatx4 acc3[4];
+        floatx4 acc3[RB];
 #pragma unroll
-        for (int i = 0; i < 4; i++) acc3[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int i = 0; i < RB; i++) acc3[i] = floatx4{0.f, 0.f, 0.f, 0.f};
         const float *wrow = prm + L_W3 + (int64_t)(lane >> 4) * L_OUT + oc;
 #pragma unroll 8
         for (int kk = 0; kk < 32; kk++) {
             const int ka = kk * 4 + (lane >> 4);
             const float b0 = wrow[(int64_t)kk * 4 * L_OUT];
 #pragma unroll
-            for (int rb = 0; rb < 4; rb++)
+            for (int rb = 0; rb < RB; rb++)
                 acc3[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(Hs[rb * 16 + (lane & 15)][ka], b0, acc3[rb], 0, 0, 0);
         }
 #pragma unroll
-        for (int rb = 0; rb < 4; rb++)
+        for (int rb = 0; rb < RB; rb++)
 #pragma unroll
             for (int r = 0; r < 4; r++) Bn[rb * 16 + 4 * (lane >> 4) + r][oc] += acc3[rb][r] + prm[L_B3 + oc];
     }
@@ -306,7 +348,7 @@ __global__ void __launch_bounds__(512) k_ghost_learned(FieldBatchDev b, const ui
     // beyond the threshold)
     const bool pass = tile_resid || __int_as_float(face_resid_bits) <= threshold;
     if (pass) {
-        for (int c = tid; c < NC; c += 512) {
+        for (int c = tid; c < NC; c += NT) {
             uint32_t w[3];
             ghost_cell_of<S, P>(dir, c, w[0], w[1], w[2]);
             int cl[3], child = 0;
@@ -364,16 +406,21 @@ static int launch_learned(const am_field_batch *fb, const am_mesh *mesh, const i
     // samples per face: at most S/2 + 1 coarse cells per tangential axis and
     // P/2 + 1 along the normal (the slab's P fine layers)
     constexpr int NS_MAX = (S / 2 + 1) * (S / 2 + 1) * (P / 2 + 1);
-    constexpr int NB = (NS_MAX + 63) / 64;
     const unsigned items = faces ? (unsigned)n_faces : (unsigned)mesh->n_tiles * 6u;
-    if (fb->tile_layout == AM_LAYOUT_INTERIOR_FIRST)
-        hipLaunchKernelGGL((k_ghost_learned<S, P, 1>), dim3(items * NB), dim3(512), 0, (hipStream_t)stream, d,
+    auto go = [&](auto kern, int nw) {
+        const int nb = (NS_MAX + 8 * nw - 1) / (8 * nw);
+        hipLaunchKernelGGL(kern, dim3(items * (unsigned)nb), dim3(64 * nw), 0, (hipStream_t)stream, d,
                            mesh->boundaries, mesh->tiles, faces, mesh->substep, mesh->max_depth, frac, params,
-                           threshold, stats, NB, tile_resid);
-    else
-        hipLaunchKernelGGL((k_ghost_learned<S, P, 0>), dim3(items * NB), dim3(512), 0, (hipStream_t)stream, d,
-                           mesh->boundaries, mesh->tiles, faces, mesh->substep, mesh->max_depth, frac, params,
-                           threshold, stats, NB, tile_resid);
+                           threshold, stats, nb, tile_resid);
+    };
+    // the per-tile gate does not see the block size: half-size blocks, two workgroups per CU
+    if (fb->tile_layout == AM_LAYOUT_INTERIOR_FIRST) {
+        if (tile_resid) go(k_ghost_learned<S, P, 1, 4>, 4);
+        else go(k_ghost_learned<S, P, 1, 8>, 8);
+    } else {
+        if (tile_resid) go(k_ghost_learned<S, P, 0, 4>, 4);
+        else go(k_ghost_learned<S, P, 0, 8>, 8);
+    }
     if (int rc = check_launch("k_ghost_learned")) return rc;
     if (!tile_resid) return AM_OK;
     // the per-tile gate: the network's faces take the tricubic fill -- all

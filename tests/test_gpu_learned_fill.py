@@ -160,9 +160,10 @@ def test_driver_learned_fallback_equals_tricubic(am, orc, torch_cuda, setup):
 
 def test_learned_faces_gate_per_tile(am, orc, torch_cuda, setup):
     """am_ghost_fill_learned_faces (the driver's call: a face list and the
-    gate per tile) on the depth-5 faces whose target is coarser: with an
-    unreachable threshold it writes what am_ghost_fill_learned writes with
-    its per-block gate passing every block, with a negative one what it
+    gate per tile, 32-sample workgroups) on the depth-5 faces whose target is
+    coarser: with an unreachable threshold it writes what
+    am_ghost_fill_learned (64-sample workgroups) writes with its per-block
+    gate passing every block, with a negative one what it
     writes with none passing (the tricubic fill), bit for bit; with the
     median of the tiles' residuals, each tile's slot equals the first
     result when its residual is within the threshold and the second
