@@ -169,22 +169,41 @@ __global__ void __launch_bounds__(64 * NW) k_ghost_learned(FieldBatchDev b, cons
     // field's baseline and layer-1 products run.
     float raw[8];
     bool bad = false;
+    // The 8 points' offsets first (the interior-first map branches), then
+    // all their loads back to back, so they are in flight together: a point
+    // outside the stored cells (beyond the tile, or an edge or corner ghost)
+    // loads the tile's first interior cell instead and takes 0.
+    uint32_t pofs[8];
+    uint32_t pin = 0u;  // bit i: point i is a stored cell
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int px = cc[0] - 1 + (i & 3), py = cc[1] - 1 + 2 * (q & 1) + (i >> 2), pz = cc[2] - 1 + (q >> 1);
+        const int out = (px < P || px >= P + S) + (py < P || py >= P + S) + (pz < P || pz >= P + S);
+        const bool inside = s < ns && px >= 0 && px < T && py >= 0 && py < T && pz >= 0 && pz < T && out <= 1;
+        pofs[i] = tile_cell<S, P, L>(inside ? (uint32_t)px : (uint32_t)P, inside ? (uint32_t)py : (uint32_t)P,
+                                     inside ? (uint32_t)pz : (uint32_t)P);
+        pin |= (uint32_t)inside << i;
+        bad |= s < ns && !inside;
+    }
+    // No branch anywhere from these loads to the field's MFMAs (the third
+    // time level is read from old when there is none and not used), so the
+    // scheduler can leave the combination of the next field's points until
+    // after this field's work.
     auto gather = [&](int v) {
-        const float *dold = b.old_[v], *dnew = b.new_[v], *dolder = b.older[v];
+        const float *dold = b.old_[v] + tbase, *dnew = b.new_[v] + tbase;
+        const float *dolder = quad ? b.older[v] + tbase : dold;
+        float f_old[8], f_new[8], f_older[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const int px = cc[0] - 1 + (i & 3), py = cc[1] - 1 + 2 * (q & 1) + (i >> 2), pz = cc[2] - 1 + (q >> 1);
-            const int out = (px < P || px >= P + S) + (py < P || py >= P + S) + (pz < P || pz >= P + S);
-            const bool inside = s < ns && px >= 0 && px < T && py >= 0 && py < T && pz >= 0 && pz < T && out <= 1;
-            float val = 0.f;
-            if (inside) {
-                const size_t r = tbase + tile_cell<S, P, L>((uint32_t)px, (uint32_t)py, (uint32_t)pz);
-                val = dold[r] * tc1 + dnew[r] * tc2;
-                if (quad) val = dolder[r] * tc0 + val;
-            } else if (s < ns) {
-                bad = true;
-            }
-            raw[i] = val;
+            f_old[i] = dold[pofs[i]];
+            f_new[i] = dnew[pofs[i]];
+            f_older[i] = dolder[pofs[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float val = f_old[i] * tc1 + f_new[i] * tc2;
+            const float val3 = f_older[i] * tc0 + val;
+            raw[i] = (pin >> i) & 1u ? (quad ? val3 : val) : 0.f;
         }
     };
     // layer-1 weights of field v for this lane and column tile ct:
@@ -233,12 +252,12 @@ __global__ void __launch_bounds__(64 * NW) k_ghost_learned(FieldBatchDev b, cons
         if (bad) valid[s] = 0;
         // Xs[buf] is complete; every wave has finished reading Xs[buf ^ 1]
         __syncthreads();
-        if (v + 1 < 24) {
-            gather(v + 1);
-            if constexpr (NW == 8) load_w1(v + 1, wnext);
-        }
+        // the last field gathers field 23 again, unused
+        gather(v + 1 < 24 ? v + 1 : v);
+        if constexpr (NW == 8) load_w1(v + 1 < 24 ? v + 1 : v, wnext);
         // baseline of the 8 children (normalised units): child q of sample s
-        if (s < ns) {
+        // (the rows of absent samples hold zeros and are never read)
+        {
             const int jx = q & 1, jy = (q >> 1) & 1, jz = q >> 2;
             float bsum = 0.f;
 #pragma unroll
