@@ -518,7 +518,7 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
         }
         return in_tile && outside <= 1;
     };
-    auto store = [&](int v, const int (&a)[3], const int (&nn)[3], const float (&val)[2][2][2]) {
+    auto store = [&](float *dst, const int (&a)[3], const int (&nn)[3], const float (&val)[2][2][2]) {
 #pragma unroll
         for (int cz = 0; cz < 2; cz++)
 #pragma unroll
@@ -530,7 +530,7 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
                     if (network_cells_kept && network_cell(w)) continue;
                     const size_t wn = (size_t)node * CELLS + tile_cell<S, P, L>((uint32_t)w[0], (uint32_t)w[1],
                                                                                  (uint32_t)w[2]);
-                    b.new_[v][wn] = val[cx][cy][cz];
+                    dst[wn] = val[cx][cy][cz];
                 }
     };
 
@@ -556,41 +556,61 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
     const int EV = E0 * E1 * E2;
     const int lv0 = quad ? 0 : 1, nlev = 3 - lv0;  // levels staged: (older,) old, new
     const int per = min(nv, CAP / (nlev * EV));    // fields per group (>= 1 by the bounds above)
-    // staging in batches of 8 elements per thread, all 8 loads issued before
-    // their LDS writes; element i -> (field, level, z, y, x) by divisions
-    // through float reciprocals, (i + 1/2) / d rounded down, exact for
-    // i + d < 2^22
-    constexpr int BATCH = 8;
-    const int per_f = nlev * EV;
-    const float inv_pf = 1.f / (float)per_f, inv_ev = 1.f / (float)EV, inv_e0 = 1.f / (float)E0,
-                inv_e1 = 1.f / (float)E1;
+    // Staging: per field (uniform, so its level pointers stay in scalar
+    // registers), batches of 4 elements per thread; the 4 elements' offsets
+    // first (the interior-first map branches), then all their loads for
+    // every level back to back, then the LDS writes.  Element c of a level
+    // is (z, y, x) of the block, by divisions through float reciprocals,
+    // (c + 1/2) / d rounded down, exact for c + d < 2^22.
+    constexpr int BATCH = 4;
+    const float inv_e0 = 1.f / (float)E0, inv_e1 = 1.f / (float)E1;
     auto qdiv = [](int i, float inv) { return (int)(((float)i + .5f) * inv); };
-    for (int vg = 0; vg < nv; vg += per) {
-        const int nf = min(per, nv - vg), total = nf * per_f;
-#pragma unroll 1
-        for (int i0 = 0; i0 < total; i0 += BATCH * 256) {
-            float tmp[BATCH];
+    const size_t tb = (size_t)tgt * CELLS;
+    float *newp[VPG];
+    bool older_in[VPG];
 #pragma unroll
-            for (int u = 0; u < BATCH; u++) {
-                const int i = i0 + (int)threadIdx.x + u * 256;
-                tmp[u] = 0.f;
-                if (i < total) {
-                    const int f = qdiv(i, inv_pf), r = i - f * per_f;
-                    const int l = qdiv(r, inv_ev), c = r - l * EV;
+    for (int i = 0; i < VPG; i++) {
+        const int vi = min(v0 + i, n_vars - 1);
+        newp[i] = b.new_[vi];
+        older_in[i] = b.older[vi] != nullptr;
+    }
+    for (int vg = 0; vg < nv; vg += per) {
+        const int nf = min(per, nv - vg);
+#pragma unroll 1
+        for (int f = 0; f < nf; f++) {
+            const int v = v0 + vg + f;
+            // without an older level (linear in time; the jobs never read it)
+            // the older slot loads old and stores zeros
+            const bool has_older = quad && b.older[v] != nullptr;
+            const float *lvp[3] = {has_older ? b.older[v] + tb : b.old_[v] + tb, b.old_[v] + tb, b.new_[v] + tb};
+#pragma unroll 1
+            for (int c0 = 0; c0 < EV; c0 += BATCH * 256) {
+                uint32_t ofs[BATCH];
+#pragma unroll
+                for (int u = 0; u < BATCH; u++) {
+                    int c = c0 + (int)threadIdx.x + u * 256;
+                    c = c < EV ? c : 0;  // past the block: element 0, loaded and not stored
                     const int yz = qdiv(c, inv_e0), x = c - yz * E0;
                     const int z = qdiv(yz, inv_e1), y = yz - z * E1;
-                    const int v = v0 + vg + f, lv = lv0 + l;
-                    const float *d = lv == 0 ? b.older[v] : (lv == 1 ? b.old_[v] : b.new_[v]);
-                    if (d)  // no older level: linear in time, never read
-                        tmp[u] = d[(size_t)tgt * CELLS + tile_cell<S, P, L>((uint32_t)(lo[0] + x),
-                                                                             (uint32_t)(lo[1] + y),
-                                                                             (uint32_t)(lo[2] + z))];
+                    ofs[u] = tile_cell<S, P, L>((uint32_t)(lo[0] + x), (uint32_t)(lo[1] + y), (uint32_t)(lo[2] + z));
                 }
-            }
+                float val[3][BATCH];
 #pragma unroll
-            for (int u = 0; u < BATCH; u++) {
-                const int i = i0 + (int)threadIdx.x + u * 256;
-                if (i < total) s_reg[i] = tmp[u];
+                for (int l = 0; l < 3; l++)
+#pragma unroll
+                    for (int u = 0; u < BATCH; u++) {
+                        const float x = l >= lv0 ? lvp[l][ofs[u]] : 0.f;
+                        val[l][u] = l == 0 && !has_older ? 0.f : x;
+                    }
+#pragma unroll
+                for (int l = 0; l < 3; l++) {
+                    if (l < lv0) continue;
+#pragma unroll
+                    for (int u = 0; u < BATCH; u++) {
+                        const int c = c0 + (int)threadIdx.x + u * 256;
+                        if (c < EV) s_reg[(f * nlev + (l - lv0)) * EV + c] = val[l][u];
+                    }
+                }
             }
         }
         __syncthreads();
@@ -598,7 +618,16 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
             const int f = job / items;
             int a[3], nn[3];
             decode(job - f * items, a, nn);
-            const int v = v0 + vg + f;
+            // the job's field: its pointers picked from registers (an indexed
+            // read of the argument table would be a memory round trip per job)
+            float *dst = newp[0];
+            bool older_ok = older_in[0];
+#pragma unroll
+            for (int i = 1; i < VPG; i++)
+                if (vg + f == i) {
+                    dst = newp[i];
+                    older_ok = older_in[i];
+                }
             const int off = (((int)s_ri[2][a[2]] - 1 - lo[2]) * E1 + ((int)s_ri[1][a[1]] - 1 - lo[1])) * E0 +
                             ((int)s_ri[0][a[0]] - 1 - lo[0]);
             const float *sr = s_reg + f * nlev * EV + off - lv0 * EV;
@@ -608,8 +637,8 @@ __global__ void __launch_bounds__(256) k_ghost_tricubic(FieldBatchDev b, int n_v
                     const float *q = sr + lv * EV + (k * E1 + j) * E0;
                     r[0] = q[0]; r[1] = q[1]; r[2] = q[2]; r[3] = q[3];
                 },
-                quad && b.older[v] != nullptr, s_w, a, nn, ty, c0, c1, c2, val);
-            store(v, a, nn, val);
+                quad && older_ok, s_w, a, nn, ty, c0, c1, c2, val);
+            store(dst, a, nn, val);
         }
         __syncthreads();
     }
