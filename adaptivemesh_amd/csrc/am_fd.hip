@@ -472,7 +472,7 @@ static void fd_launch(int fmt, int pf, bool persist, dim3 grid, hipStream_t st, 
     else fd_launch_dir<S, P, Q, ZB, 3, L>(fmt, pf, persist, grid, st, d, m, sc);  // y, or y and z
 }
 // the upwind / dissipation kernels: S = 16, P = 3, order 4, cf3 words,
-// one-plane prefetch, direct faces 0 or z
+// one-plane prefetch, direct faces 0, z, or (interior-first) y and/or z
 template <int S, int P, int Q, int ZB, int L>
 static void fd_launch_adv_l(bool persist, dim3 grid, hipStream_t st, const FdBatchDev &d, const MeshDev &m,
                             const DepthScales &sc)
@@ -483,6 +483,11 @@ static void fd_launch_adv_l(bool persist, dim3 grid, hipStream_t st, const FdBat
                 hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, WORD_CF3, 1, 1, L, true>), grid, dim3(S * S), 0, st, d, m, sc);
             else
                 hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false, WORD_CF3, 1, 1, L, true>), grid, dim3(S * S), 0, st, d, m, sc);
+        } else if (L == 1 && m.direct_axes != 0u) {
+            if (persist)
+                hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, WORD_CF3, 1, 3, 1, true>), grid, dim3(S * S), 0, st, d, m, sc);
+            else
+                hipLaunchKernelGGL((k_fd<S, P, Q, ZB, false, WORD_CF3, 1, 3, 1, true>), grid, dim3(S * S), 0, st, d, m, sc);
         } else {
             if (persist)
                 hipLaunchKernelGGL((k_fd<S, P, Q, ZB, true, WORD_CF3, 1, 0, L, true>), grid, dim3(S * S), 0, st, d, m, sc);
@@ -560,10 +565,11 @@ static int launch_fd(const am_fd_batch *b, const am_mesh *mesh, const am_geometr
     }
     if (adv) {
         if (!(S == 16 && P == 3 && Q == 2) || fmt != WORD_CF3 || pf > 1 ||
-            (b->direct_faces != 0 && b->direct_faces != 4))
+            (b->direct_faces != 0 && b->direct_faces != 4 &&
+             (b->tile_layout != AM_LAYOUT_INTERIOR_FIRST || (b->direct_faces & 1))))
             return set_error(AM_ERR_INVALID,
                              "am_fd: the upwind / dissipation output needs S=16, P=3, fd_order 4, cf3 words, "
-                             "prefetch 1 and direct_faces 0 or 4");
+                             "prefetch 1 and direct_faces 0 or 4 (or 2 and 6 in the interior-first layout)");
         if (!b->shift[0] || !b->shift[1] || !b->shift[2])
             return set_error(AM_ERR_INVALID, "am_fd: the upwind output needs the three shift fields");
         if (!(b->ko_sigma >= 0.f)) return set_error(AM_ERR_INVALID, "am_fd: ko_sigma %g", (double)b->ko_sigma);

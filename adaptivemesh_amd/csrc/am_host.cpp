@@ -524,9 +524,8 @@ int am_wave_sim_pressure(const am_wave_sim *s, float *out2)
     int P##_set_direct_faces(H *s, int32_t enable)                                                                 \
     {                                                                                                              \
         return rk4_call(s, [&](auto &d) {                                                                          \
-            if (enable < 0 || enable > 7 || (enable & 1) || (d.opt.upwind && (enable & 2)))                       \
-                throw std::runtime_error("direct faces axis mask " + std::to_string(enable) +                      \
-                                         " (0, 2, 4 or 6; 0 or 4 with upwinding)");                               \
+            if (enable < 0 || enable > 7 || (enable & 1))                                                          \
+                throw std::runtime_error("direct faces axis mask " + std::to_string(enable) + " (0, 2, 4 or 6)"); \
             d.opt.direct_faces = enable;                                                                               \
             d.drop_graph();                                                                                        \
         });                                                                                                        \

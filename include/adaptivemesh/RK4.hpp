@@ -104,7 +104,7 @@ struct rk4_options {
     // every fine step whatever the mask.  In the interior-first layout a z
     // ghost plane is the neighbour's contiguous interior plane and a y ghost
     // row one of its interior rows, while x ghost columns would be strided
-    // reads, so the mask is 0, 2, 4 or 6 (0 or 4 with upwinding).
+    // reads, so the mask is 0, 2, 4 or 6.
     int32_t direct_faces = 4;
     // upwinded (lopsided fourth-order) advection with Kreiss-Oliger
     // dissipation of strength ko_sigma (sixth order), computed by the FD
@@ -290,7 +290,6 @@ struct rk4_AMR_data {
             throw std::runtime_error("upwinding needs S = 16, P = 3, fd_order 4 and compressed_float3 words");
         opt.upwind = upwind;
         opt.ko_sigma = sigma;
-        if (upwind) opt.direct_faces &= 4;  // the upwind FD reads direct z faces only
         if (upwind && !adv.created) adv = smart_gpu_buffer<float>((size_t)NV * slots * inner);
         drop_graph();
     }
@@ -733,10 +732,7 @@ struct rk4_AMR_data {
         opt.fd_prefetch = 1;
         trial();  // warm-up
         sweep(0, opt.z_block, {S, S / 2, S / 4});
-        if (opt.upwind)
-            sweep(4, opt.direct_faces, {4, 0});
-        else
-            sweep(4, opt.direct_faces, {4, 6, 0});
+        sweep(4, opt.direct_faces, {4, 6, 0});
         sweep(1, opt.fd_grid_limit, {0, 8 * cu});
         sweep(2, opt.rhs_grid_limit, {0, 2 * cu, 4 * cu});
         sweep(5, opt.overlap_depths, {1, 0});

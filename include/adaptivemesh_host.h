@@ -230,8 +230,7 @@ int am_bssn_sim_get_push_layers(const am_bssn_sim *s);
  * from the neighbours' interiors.  Stage 3 pushes every face.  The fields
  * after each fine step are the same for every mask; between stages 0-2 the
  * equal-depth ghost layers on the masked axes are not current.  The mask is
- * 0, 2, 4 or 6 (x ghost columns would be strided reads); with upwinding 0 or
- * 4, and set_dissipation with upwind = 1 clears the y bit.
+ * 0, 2, 4 or 6 (x ghost columns would be strided reads).
  * _get_direct_faces gives the mask. */
 int am_bssn_sim_set_direct_faces(am_bssn_sim *s, int32_t axes);
 int am_bssn_sim_get_direct_faces(const am_bssn_sim *s);

@@ -240,7 +240,7 @@ def test_direct_faces_leave_every_step_unchanged(am, torch_cuda, recipe, periodi
     and 6: old, new and the accumulator, ghost cells included, and the seed
     state are the same bit for bit after every step.  Masks with x faces are
     refused (the driver's interior-first tiles give those ghost cells only as
-    strided columns), and so is y with upwinding."""
+    strided columns)."""
     print(f"direct faces over whole steps: {recipe} periodic={periodic}", flush=True)
     ops, n = {"uniform1": meshes.uniform(1), "graded": meshes.graded()}[recipe]
     g = am.geometry(S, P, ORDER, OUTER, CFL)

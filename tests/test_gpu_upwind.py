@@ -3,7 +3,7 @@ oracle (tests/test_oracle_dissipation.py pins the oracle):
 
 * the FD kernel's advection + dissipation output equals the oracle's float
   restatement (orc_fd_adv) bit for bit, on padded and interior-first tiles,
-  with direct z faces (their ghost layers poisoned), every z-block, two
+  with direct y and z faces (their ghost layers poisoned), every z-block, two
   counter values, with and without the advection term; its words stay the
   oracle's;
 * the BSSN driver with set_dissipation: per RK4 stage the words and the
@@ -27,7 +27,7 @@ from tests.test_gpu_parity import dev, host, mesh_for, setup_mesh, sync
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("layout,direct", [(0, 0), (0, 4), (1, 0), (1, 4)])
+@pytest.mark.parametrize("layout,direct", [(0, 0), (0, 4), (1, 0), (1, 4), (1, 2), (1, 6)])
 @pytest.mark.parametrize("recipe,periodic", [("graded", False), ("uniform2", True)])
 def test_fd_adv_bit_exact(am, orc, torch_cuda, layout, direct, recipe, periodic):
     torch = torch_cuda
@@ -39,20 +39,21 @@ def test_fd_adv_bit_exact(am, orc, torch_cuda, layout, direct, recipe, periodic)
     flds = [rng.standard_normal(n * T ** 3).astype(np.float32) for _ in range(nv)]
     shift = [(0.5 * rng.standard_normal(n * T ** 3)).astype(np.float32) for _ in range(3)]
     tiles = [s for s in F.used_slots(e) if F.depth_of(e, s) > 0]
-    eq_z = F.outer_push_layers(e, n, tiles, S, P, 0, 4)
+    eq_yz = F.outer_push_layers(e, n, tiles, S, P, 0, 6)
     good = []
     for f in flds:
         filled = f.copy()
         orc.copy_boundaries(f.copy(), filled, None, e["boundaries"], tiles, 0, e["max_depth"], True, 0, S, P,
                             equal_copy=True)
         x = f.copy()
-        x[eq_z] = filled[eq_z]
+        x[eq_yz] = filled[eq_yz]
         good.append(x)
     src = [x.copy() for x in good]
-    if direct:
-        assert eq_z.any()
+    if direct:  # the equal-depth layers on the direct axes poisoned: they must come from the neighbours
+        poison = F.outer_push_layers(e, n, tiles, S, P, 0, direct)
+        assert poison.any()
         for x in src:
-            x[eq_z] = np.nan
+            x[poison] = np.nan
     conv = (lambda a: to_if(am, a, S, P, n)) if layout else (lambda a: a)
     dfl = [dev(torch, conv(x)) for x in src]
     dsh = [dev(torch, conv(x)) for x in shift]
@@ -96,12 +97,18 @@ def test_fd_adv_bit_exact(am, orc, torch_cuda, layout, direct, recipe, periodic)
     b.prefetch_planes = 2
     assert L.am_fd_batched(C.byref(b), C.byref(m), C.byref(g), st) == -1
     b.prefetch_planes = 1
+    for bad_direct in (1, 3, 7):  # x faces are never direct
+        b.direct_faces = bad_direct
+        assert L.am_fd_batched(C.byref(b), C.byref(m), C.byref(g), st) == -1
+    b.direct_faces = direct
+    b.prefetch_planes = 1
     b.shift[2] = None
     assert L.am_fd_batched(C.byref(b), C.byref(m), C.byref(g), st) == -1
 
 
 @pytest.mark.parametrize("recipe,periodic,depth,direct,advect", [
-    ("uniform1", True, 1, 4, 1), ("graded", False, 2, 4, 1), ("graded", False, 3, 0, 1), ("uniform1", True, 1, 0, 0)])
+    ("uniform1", True, 1, 4, 1), ("graded", False, 2, 4, 1), ("graded", False, 3, 0, 1), ("uniform1", True, 1, 0, 0),
+    ("uniform1", True, 1, 6, 1), ("graded", False, 3, 6, 1)])
 def test_bssn_upwind_stages_match_oracle(am, orc, torch_cuda, recipe, periodic, depth, direct, advect):
     print(f"bssn upwind stages {recipe} depth={depth} direct={direct} advect={advect}", flush=True)
     sigma = 0.3
@@ -203,12 +210,11 @@ def test_dissipation_setting_refusals(am, torch_cuda):
         sim2.set_dissipation(0.2, True)  # P = 2: the stencils reach 3 cells
     sim.set_dissipation(0.0, False)
     assert sim.dissipation == (0.0, False) and sim.push_layers == ORDER // 2
-    # the upwind FD reads direct z faces only: upwinding clears the y bit and refuses it afterwards
+    # the upwind FD reads direct y and z faces as the plain one does: upwinding keeps the mask
     sim.set_direct_faces(6)
     sim.set_dissipation(0.2, True)
-    assert sim.direct_faces == 4
-    with pytest.raises(am.AMError):
-        sim.set_direct_faces(2)
-    sim.set_dissipation(0.0, False)
+    assert sim.direct_faces == 6
     sim.set_direct_faces(2)
     assert sim.direct_faces == 2
+    with pytest.raises(am.AMError):
+        sim.set_direct_faces(3)
