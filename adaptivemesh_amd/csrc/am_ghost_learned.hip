@@ -73,7 +73,7 @@ __device__ __forceinline__ float read_pos(uint32_t w, float rel, float rel_scale
 // on its field barrier or its gathers.  Every output element goes through
 // the same operations in the same order for either NW.
 template <int S, int P, int L, int NW>
-__global__ void __launch_bounds__(64 * NW) k_ghost_learned(FieldBatchDev b, const uint64_t *__restrict__ bnd,
+__global__ void __launch_bounds__(64 * NW, NW == 4 ? 3 : 1) k_ghost_learned(FieldBatchDev b, const uint64_t *__restrict__ bnd,
                                                        const int32_t *__restrict__ tiles,
                                                        const int32_t *__restrict__ faces, uint32_t substep,
                                                        uint32_t maxd, float frac, const float *__restrict__ prm,
@@ -86,8 +86,13 @@ __global__ void __launch_bounds__(64 * NW) k_ghost_learned(FieldBatchDev b, cons
     constexpr int NT = 64 * NW, NSMP = 8 * NW;  // threads, samples per block
     constexpr int RB = NSMP / 16, CT = L_HID / 16 / NW;  // row blocks of samples, hidden column tiles per wave
     static_assert(NW == 4 || NW == 8, "8 threads per sample, 16-row MFMA blocks, 128 hidden columns");
-    __shared__ float Xs[2][NSMP][65];
-    __shared__ float Hs[NSMP][129];
+    // the input tiles (double-buffered per field) and the hidden layer share
+    // their LDS: the hidden layer is written after the barrier that ends the
+    // last field's reads of the inputs
+    constexpr int XH = 2 * NSMP * 65 > NSMP * 129 ? 2 * NSMP * 65 : NSMP * 129;
+    __shared__ float xh[XH];
+    float(*const Xs)[NSMP][65] = reinterpret_cast<float(*)[NSMP][65]>(xh);
+    float(*const Hs)[129] = reinterpret_cast<float(*)[129]>(xh);
     __shared__ float Bn[NSMP][193];
     __shared__ float musg[2][24][NSMP];
     __shared__ int valid[NSMP];
@@ -257,17 +262,30 @@ __global__ void __launch_bounds__(64 * NW) k_ghost_learned(FieldBatchDev b, cons
         if constexpr (NW == 8) load_w1(v + 1 < 24 ? v + 1 : v, wnext);
         // baseline of the 8 children (normalised units): child q of sample s
         // (the rows of absent samples hold zeros and are never read)
+        // (separable: along x, then y, then z, 84 fused multiply-adds)
         {
             const int jx = q & 1, jy = (q >> 1) & 1, jz = q >> 2;
-            float bsum = 0.f;
+            float wx[4], wy[4], wz[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++)
+            for (int i = 0; i < 4; i++) {
+                wx[i] = LQ[jx][i];
+                wy[i] = LQ[jy][i];
+                wz[i] = LQ[jz][i];
+            }
+            float bz = 0.f;
 #pragma unroll
-                for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 4; k++) {
+                float by = 0.f;
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        bsum += LQ[jz][k] * LQ[jy][j] * LQ[jx][i] * Xs[buf][s][k * 16 + j * 4 + i];
-            Bn[s][v * 8 + q] = bsum;
+                for (int j = 0; j < 4; j++) {
+                    float bx = 0.f;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) bx = fmaf(wx[i], Xs[buf][s][k * 16 + j * 4 + i], bx);
+                    by = fmaf(wy[j], bx, by);
+                }
+                bz = fmaf(wz[k], by, bz);
+            }
+            Bn[s][v * 8 + q] = bz;
         }
         // layer 1 contribution of this field: Z1[:, this wave's columns] += Xs (NSMP x 64) W1[v*64.., columns]
 #pragma unroll
