@@ -415,19 +415,26 @@ __global__ void __launch_bounds__(256) k_bssn_stage(BssnArgsDev a, TileArgs m, B
         if constexpr (STAGE == 0 || STAGE == 3) __builtin_amdgcn_sched_barrier(0);
         float k[NV];
         bssn_cell<FMT, true, false, UPW>(a, u, cw, w, par, k, nullptr, nullptr, aw);
+        // The offsets again, as values defined here: the compiler otherwise
+        // carries their 64-bit extensions from the first batch's block and,
+        // seeing no 32-bit offset in this one, forms a 64-bit address per
+        // load and store of the second batch (48 extra 64-bit adds per cell)
+        // instead of using the scalar base + 32-bit offset form.
+        uint32_t ro2 = ro, wo2 = wo;
+        asm("" : "+v"(ro2), "+v"(wo2));
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            yn[v] = STAGE == 0 ? u[v] : ld_at(a.old_[v], ro);
-            ac[v] = STAGE == 0 ? 0.f : ld_at(a.acc[v], wo);
+            yn[v] = STAGE == 0 ? u[v] : ld_at(a.old_[v], ro2);
+            ac[v] = STAGE == 0 ? 0.f : ld_at(a.acc[v], wo2);
         }
-        rk4_combine_at<NV, STAGE>(a.acc, a.new_, k, yn, ac, ro, wo, sc.dt[depth]);
+        rk4_combine_at<NV, STAGE>(a.acc, a.new_, k, yn, ac, ro2, wo2, sc.dt[depth]);
         if constexpr (PUSH && L == 1 && S * S == 256) {
             // y and z faces are whole slab rows per wave already; the x
             // faces' columns are transposed through LDS
-            rk4_push<NV, S, P, L>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes & 6);
+            rk4_push<NV, S, P, L, true>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes & 6);
             rk4_push_x_rows<NV, S, P>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes, xbuf);
         } else if (PUSH) {
-            rk4_push<NV, S, P, L>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes);
+            rk4_push<NV, S, P, L, true>(a.new_, k, m.boundaries, node, x, y, z, a.push_layers, a.push_axes);
         }
     }
 }

@@ -74,7 +74,10 @@ __device__ __forceinline__ void rk4_combine_at(float *const *acc, float *const *
 // the cell lands in the target's slab of the opposite face, layer = its
 // distance from the shared face, so the cells of one row of this tile go to
 // one contiguous slab row of the target.
-template <int NV, int S, int P, int L = 0>
+// B32: address the fields with 32-bit byte offsets on their scalar bases, so
+// no lane does 64-bit arithmetic per field (the caller's field arrays stay
+// below 4 GiB)
+template <int NV, int S, int P, int L = 0, bool B32 = false>
 __device__ __forceinline__ void rk4_push(float *const *new_, const float (&y)[NV], const uint64_t *boundaries, int node,
                                          int x, int y_, int z, int layers, int axes)
 {
@@ -99,8 +102,14 @@ __device__ __forceinline__ void rk4_push(float *const *new_, const float (&y)[NV
             const uint32_t a = (uint32_t)c3[ax == 0 ? 1 : 0], bq = (uint32_t)c3[ax == 2 ? 1 : 2];
             dst = (size_t)tgt * TS + slab_cell<S, P>((uint32_t)((f + 3) % 6), l, a, bq);
         }
+        if constexpr (B32) {
+            const uint32_t db = 4u * (uint32_t)dst;
 #pragma unroll
-        for (int v = 0; v < NV; v++) new_[v][dst] = y[v];
+            for (int v = 0; v < NV; v++) *reinterpret_cast<float *>(reinterpret_cast<char *>(new_[v]) + db) = y[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; v++) new_[v][dst] = y[v];
+        }
     }
 }
 
@@ -110,7 +119,8 @@ __device__ __forceinline__ void rk4_push(float *const *new_, const float (&y)[NV
 // slab row (layer l, row b = z) holds them as S consecutive floats.  They go
 // through LDS (xbuf: 2 * P * NV * S floats) and leave as whole rows, one
 // field per wave: 64 lanes = all the rows of one field at two layers.  Every
-// thread of the workgroup must call this (it holds two barriers).
+// thread of the workgroup must call this (it holds two barriers).  The
+// fields are addressed with 32-bit byte offsets, as rk4_push<..., true>.
 template <int NV, int S, int P>
 __device__ __forceinline__ void rk4_push_x_rows(float *const *new_, const float (&y)[NV], const uint64_t *boundaries,
                                                 int node, int x, int y_, int z, int layers, int axes, float *xbuf)
@@ -143,8 +153,9 @@ __device__ __forceinline__ void rk4_push_x_rows(float *const *new_, const float 
         for (int i = lane; i < n_row; i += 64) {
             const int s2 = i / (layers * S), l = (i / S) % layers, yy = i % S;
             if (tgt[s2] < 0) continue;
-            dst[(size_t)tgt[s2] * TS + slab_cell<S, P>(s2 == 0 ? 3u : 0u, (uint32_t)l, (uint32_t)yy, (uint32_t)z)] =
-                xbuf[((s2 * P + l) * NV + v) * S + yy];
+            const uint32_t db =
+                4u * (uint32_t)((size_t)tgt[s2] * TS + slab_cell<S, P>(s2 == 0 ? 3u : 0u, (uint32_t)l, (uint32_t)yy, (uint32_t)z));
+            *reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + db) = xbuf[((s2 * P + l) * NV + v) * S + yy];
         }
     }
     __syncthreads();  // xbuf is refilled by the workgroup's next item

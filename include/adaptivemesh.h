@@ -475,10 +475,10 @@ typedef struct am_bssn_fields {
  * push_ghosts != 0 (mesh->boundaries required) every new value within P
  * cells of a face whose neighbour has the same depth is also written into
  * that neighbour's face ghost cell: the AM_EQUAL_COPY fill, fused.  Each
- * array is addressed with 32-bit byte offsets: every slot a tile list names
- * must lie within the first 4 GiB of each array (slot < 2^32 / (4 x the
- * tile stride)), which the 118 arrays of one call reach only beyond the HBM
- * of one GPU. */
+ * array is addressed with 32-bit byte offsets: every slot a tile list names,
+ * and every neighbour the push writes, must lie within the first 4 GiB of
+ * each array (slot < 2^32 / (4 x the tile stride)), which the 118 arrays of
+ * one call reach only beyond the HBM of one GPU. */
 int am_bssn_rhs_stage(const am_bssn_fields *f, int32_t stage, const am_bssn_params *p, const am_mesh *mesh,
                       const am_geometry *g, int32_t push_ghosts, void *stream);
 
