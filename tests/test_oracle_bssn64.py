@@ -1,4 +1,4 @@
-"""Pins the FP64 BSSN scheme on tiles (the oracle the octree runs are judged
+"""Pins the FP64 BSSN scheme on tiles (the oracle the octree runs are checked
 against) by functions already pinned elsewhere:
 
 * orc_bssn64_stage: on fields that are quadratic polynomials the order-4
