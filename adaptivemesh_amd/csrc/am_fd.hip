@@ -298,9 +298,9 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
     float *o_adv = nullptr;
     float kos = 0.f;
     auto shift_at = [&](int zz) {
-        const size_t i = (size_t)slot * TS + tile_interior<S, P, L>((uint32_t)x, (uint32_t)y, (uint32_t)zz);
+        const uint32_t ib = 4u * (uint32_t)((size_t)slot * TS + tile_interior<S, P, L>((uint32_t)x, (uint32_t)y, (uint32_t)zz));
 #pragma unroll
-        for (int k = 0; k < 3; k++) bsh[k] = b.shift[k][i];
+        for (int k = 0; k < 3; k++) bsh[k] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(b.shift[k]) + ib);
     };
     if constexpr (ADV) {
         o_adv = b.adv[var];
@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(S *S) k_fd(FdBatchDev b, MeshDev m, DepthScale
                     adv = adv + bsh[k] * (l * dx1);
                     ko = ko + kk;
                 }
-                o_adv[o] = (advect ? adv : 0.f) + ko * kos;
+                *reinterpret_cast<float *>(reinterpret_cast<char *>(o_adv) + 4u * o) = (advect ? adv : 0.f) + ko * kos;
                 if (z + 1 < z0 + ZB) shift_at(z + 1);
             }
         }

@@ -256,8 +256,9 @@ typedef struct am_fd_batch {
 enum am_word_format { AM_WORD_CF3 = 0, AM_WORD_F32 = 1, AM_WORD_F16 = 2 };
 /* bytes per cell of one word array in `format` (4, 12, 6), -1 if unknown */
 int32_t am_word_bytes(int32_t format);
-/* cf3 word arrays are addressed with 32-bit byte offsets: every listed slot
- * must satisfy (slot + 1) * S^3 <= 2^30. */
+/* cf3 word arrays (and adv) are addressed with 32-bit byte offsets: every
+ * listed slot must satisfy (slot + 1) * S^3 <= 2^30; with upwinding the
+ * shift fields are too, so (slot + 1) * (the tile stride) <= 2^30. */
 int am_fd_batched(const am_fd_batch *b, const am_mesh *mesh, const am_geometry *g, void *stream);
 
 /* ------------------------------------------------- ghost cells and tree */
